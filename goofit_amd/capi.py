@@ -1,0 +1,254 @@
+"""ctypes binding of the C ABI in include/goofit_hip.h (libgoofit_hip.so).
+
+This is the thinnest possible host path onto the gfx950 engine: every method
+is one C call.  Loading fails loudly when the library has not been built; there
+is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libgoofit_hip.so")
+
+# include/goofit_hip.h
+GF_OK, GF_ERR_INVALID, GF_ERR_HIP, GF_ERR_NOMEM, GF_ERR_STATE, GF_ERR_NODEVICE = range(6)
+FN = dict(NONE=0, GAUSSIAN=1, EXP=2, EXP_OFFSET=3, EXP_POLY=4, EXP_POLY_OFFSET=5, EXPGAUS=6,
+          POLYNOMIAL=7, OFFSET_POLYNOMIAL=8, MULTI_POLYNOMIAL=9, ADD=10, ADD_EXT=11, PROD=12,
+          DALITZ=13, RBW=14, RBW_SYM=15, GS=16, NONRES=17)
+METRIC = dict(EVAL=0, NLL=1, PROB=2, BINAVG=3, BINWITHERROR=4, CHISQ=5)
+
+# Every symbol the header declares (checked by the tests against the built library).
+EXPORTS = [
+    "gf_device_count", "gf_ctx_create", "gf_ctx_destroy", "gf_last_error",
+    "gf_events_upload", "gf_events_upload_aos", "gf_events_alloc", "gf_events_column",
+    "gf_events_set_identity_column", "gf_program_set", "gf_params_update", "gf_norms_update",
+    "gf_decay3_set", "gf_reduce_metric", "gf_eval_metric", "gf_eval_metric_device", "gf_reduce_bins",
+    "gf_dalitz_set_data_size", "gf_dalitz_cache", "gf_dalitz_gram", "gf_dalitz_gram_points",
+    "gf_dalitz_gram_points_device", "gf_dalitz_get_cache", "gf_dalitz_cache_gram",
+    "gf_time_reduce_metric", "gf_time_dalitz_cache", "gf_time_dalitz_gram",
+]
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int32)
+_lib = None
+
+
+class GooFitError(RuntimeError):
+    """A failing C-ABI call (status code + gf_last_error text)."""
+
+    def __init__(self, status, msg):
+        super().__init__(f"[{status}] {msg}")
+        self.status = status
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m goofit_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        c_p = ctypes.c_void_p
+        L.gf_device_count.restype = ctypes.c_int
+        L.gf_last_error.restype = ctypes.c_char_p
+        L.gf_ctx_create.argtypes = [ctypes.POINTER(c_p), ctypes.c_int]
+        L.gf_ctx_destroy.argtypes = [c_p]
+        L.gf_ctx_destroy.restype = None
+        L.gf_events_upload.argtypes = [c_p, ctypes.c_int, ctypes.c_int64, ctypes.POINTER(_D)]
+        L.gf_events_upload_aos.argtypes = [c_p, ctypes.c_int, ctypes.c_int64, _D]
+        L.gf_events_alloc.argtypes = [c_p, ctypes.c_int, ctypes.c_int64]
+        L.gf_events_column.argtypes = [c_p, ctypes.c_int, ctypes.POINTER(c_p)]
+        L.gf_events_set_identity_column.argtypes = [c_p, ctypes.c_int]
+        L.gf_program_set.argtypes = [c_p, _I, ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int,
+                                     _D, ctypes.c_int]
+        L.gf_params_update.argtypes = [c_p, _D, ctypes.c_int]
+        L.gf_norms_update.argtypes = [c_p, _D, ctypes.c_int]
+        L.gf_decay3_set.argtypes = [c_p, _D]
+        L.gf_reduce_metric.argtypes = [c_p, ctypes.c_int, _D, _D]
+        L.gf_eval_metric.argtypes = [c_p, ctypes.c_int, _D, ctypes.c_int64]
+        L.gf_eval_metric_device.argtypes = [c_p, ctypes.c_int, c_p, ctypes.c_int64]
+        L.gf_reduce_bins.argtypes = [c_p, ctypes.c_int, _D, ctypes.c_int, _D]
+        L.gf_dalitz_set_data_size.argtypes = [c_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64]
+        L.gf_dalitz_cache.argtypes = [c_p, ctypes.c_int, ctypes.c_uint64]
+        L.gf_dalitz_gram.argtypes = [c_p, ctypes.c_int, _D, ctypes.c_int, _D]
+        L.gf_dalitz_gram_points.argtypes = [c_p, ctypes.c_int, _D, _D, _D, ctypes.c_int64, ctypes.c_int, _D]
+        L.gf_dalitz_gram_points_device.argtypes = [c_p, ctypes.c_int, c_p, c_p, c_p, ctypes.c_int64, ctypes.c_int,
+                                                   _D]
+        L.gf_dalitz_get_cache.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int64]
+        L.gf_dalitz_cache_gram.argtypes = [c_p, ctypes.c_int, _D]
+        L.gf_time_reduce_metric.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D]
+        L.gf_time_dalitz_cache.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D]
+        L.gf_time_dalitz_gram.argtypes = [c_p, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int, _D]
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != GF_OK:
+        raise GooFitError(status, lib().gf_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def device_count() -> int:
+    return lib().gf_device_count()
+
+
+class Context:
+    """One gf_ctx: a HIP stream and the device state of one flattened PDF tree."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().gf_ctx_create(ctypes.byref(h), device))
+        self.h = h
+        self.nevents = 0
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            lib().gf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- events -----------------------------------------------------------
+    def upload_columns(self, cols):
+        cols = [_f64(c) for c in cols]
+        n = cols[0].size if cols else 0
+        arr = (_D * len(cols))(*[_dp(c) for c in cols])
+        _check(lib().gf_events_upload(self.h, len(cols), n, arr))
+        self.nevents = n
+
+    def upload_aos(self, aos):
+        aos = _f64(aos)
+        _check(lib().gf_events_upload_aos(self.h, aos.shape[1], aos.shape[0], _dp(aos)))
+        self.nevents = aos.shape[0]
+
+    def alloc_events(self, ncols, n):
+        _check(lib().gf_events_alloc(self.h, ncols, n))
+        self.nevents = n
+
+    def column_ptr(self, k) -> int:
+        p = ctypes.c_void_p()
+        _check(lib().gf_events_column(self.h, k, ctypes.byref(p)))
+        return p.value
+
+    def set_identity_column(self, k):
+        _check(lib().gf_events_set_identity_column(self.h, k))
+
+    # -- program ----------------------------------------------------------
+    def set_program(self, funcs, params, consts, obs, norms):
+        f = np.ascontiguousarray(funcs, dtype=np.int32)
+        p, c, o, n = _f64(params), _f64(consts), _f64(obs), _f64(norms)
+        _check(lib().gf_program_set(self.h, f.ctypes.data_as(_I), f.size, _dp(p), p.size, _dp(c), c.size, _dp(o),
+                                    o.size, _dp(n), n.size))
+
+    def update_params(self, params):
+        p = _f64(params)
+        _check(lib().gf_params_update(self.h, _dp(p), p.size))
+
+    def update_norms(self, norms):
+        n = _f64(norms)
+        _check(lib().gf_norms_update(self.h, _dp(n), n.size))
+
+    def set_decay(self, decay6):
+        d = _f64(decay6)
+        _check(lib().gf_decay3_set(self.h, _dp(d)))
+
+    # -- reductions -------------------------------------------------------
+    def reduce_metric(self, metric="NLL"):
+        s, c = ctypes.c_double(), ctypes.c_double()
+        m = METRIC[metric] if isinstance(metric, str) else metric
+        _check(lib().gf_reduce_metric(self.h, m, ctypes.byref(s), ctypes.byref(c)))
+        return s.value, c.value
+
+    def eval_metric(self, metric="EVAL"):
+        out = np.empty(self.nevents)
+        m = METRIC[metric] if isinstance(metric, str) else metric
+        _check(lib().gf_eval_metric(self.h, m, _dp(out), out.size))
+        return out
+
+    def eval_metric_device(self, metric, dev_ptr, n):
+        m = METRIC[metric] if isinstance(metric, str) else metric
+        _check(lib().gf_eval_metric_device(self.h, m, ctypes.c_void_p(dev_ptr), n))
+
+    def reduce_bins(self, func_idx, ranges):
+        r = _f64(ranges)
+        s = ctypes.c_double()
+        _check(lib().gf_reduce_bins(self.h, func_idx, _dp(r), r.size // 3, ctypes.byref(s)))
+        return s.value
+
+    # -- Dalitz -----------------------------------------------------------
+    def dalitz_set_data_size(self, dalitz_func, n, offset=0):
+        _check(lib().gf_dalitz_set_data_size(self.h, dalitz_func, n, offset))
+
+    def dalitz_cache(self, dalitz_func, mask=(1 << 64) - 1):
+        _check(lib().gf_dalitz_cache(self.h, dalitz_func, mask))
+
+    def dalitz_gram(self, dalitz_func, grid6, eff_func, nres):
+        g = _f64(grid6)
+        out = np.zeros(2 * nres * nres)
+        _check(lib().gf_dalitz_gram(self.h, dalitz_func, _dp(g), eff_func, _dp(out)))
+        return out.view(np.complex128).reshape(nres, nres)
+
+    def dalitz_gram_points(self, dalitz_func, m12, m13, w, eff_func, nres):
+        a, b = _f64(m12), _f64(m13)
+        wk = _f64(w) if w is not None else None
+        out = np.zeros(2 * nres * nres)
+        _check(lib().gf_dalitz_gram_points(self.h, dalitz_func, _dp(a), _dp(b), _dp(wk) if wk is not None else None,
+                                           a.size, eff_func, _dp(out)))
+        return out.view(np.complex128).reshape(nres, nres)
+
+    def dalitz_gram_points_device(self, dalitz_func, m12_ptr, m13_ptr, w_ptr, npts, eff_func, nres):
+        out = np.zeros(2 * nres * nres)
+        _check(lib().gf_dalitz_gram_points_device(self.h, dalitz_func, ctypes.c_void_p(m12_ptr),
+                                                  ctypes.c_void_p(m13_ptr), ctypes.c_void_p(w_ptr or 0), npts,
+                                                  eff_func, _dp(out)))
+        return out.view(np.complex128).reshape(nres, nres)
+
+    def dalitz_get_cache(self, dalitz_func, res, n):
+        out = np.zeros(2 * n)
+        _check(lib().gf_dalitz_get_cache(self.h, dalitz_func, res, _dp(out), n))
+        return out.view(np.complex128)
+
+    def dalitz_cache_gram(self, dalitz_func, nres):
+        out = np.zeros(2 * nres * nres)
+        _check(lib().gf_dalitz_cache_gram(self.h, dalitz_func, _dp(out)))
+        return out.view(np.complex128).reshape(nres, nres)
+
+    # -- timing -----------------------------------------------------------
+    def time_reduce_metric(self, metric, reps):
+        ms = ctypes.c_double()
+        m = METRIC[metric] if isinstance(metric, str) else metric
+        _check(lib().gf_time_reduce_metric(self.h, m, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def time_dalitz_cache(self, dalitz_func, reps):
+        ms = ctypes.c_double()
+        _check(lib().gf_time_dalitz_cache(self.h, dalitz_func, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def time_dalitz_gram(self, dalitz_func, grid6, eff_func, reps):
+        ms = ctypes.c_double()
+        g = _f64(grid6)
+        _check(lib().gf_time_dalitz_gram(self.h, dalitz_func, _dp(g), eff_func, reps, ctypes.byref(ms)))
+        return ms.value

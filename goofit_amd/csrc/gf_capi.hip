@@ -1,0 +1,1445 @@
+// gf_capi.hip -- the C ABI of include/goofit_hip.h: contexts, event columns,
+// flattened-tree compilation, and the launches of gf_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/goofit_hip.h"
+#include "gf_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+gf_status fail(gf_status code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define GF_HIP(call)                                                                                                   \
+    do {                                                                                                               \
+        hipError_t e_ = (call);                                                                                        \
+        if(e_ != hipSuccess)                                                                                           \
+            return fail(GF_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));                                    \
+    } while(0)
+
+// Wave-cache storage of one Dalitz node, keyed by its cacheToUse constant as the
+// reference keys cResonances[16 * cacheToUse + i] (Amp3Body.cu:54, 222-233).
+struct WaveCache {
+    double2 *wave = nullptr;  // GF_MAX_RES x n complex
+    int64_t n = 0, offset = 0;
+};
+
+// A Dalitz node of the installed program.
+struct DalitzNode {
+    int func = -1;        // function index
+    int cache_key = 0;    // cacheToUse constant
+    int nres = 0;
+    std::vector<int> res_funcs;
+    std::vector<GfResDesc> res;
+    int eff_func = -1;    // efficiencyFunction (last function of the subtree, Amp3Body.cu:198-202)
+    int eff_root = -1;    // first function of the efficiency subtree
+    int col_m12 = 0, col_m13 = 1, col_num = 2;
+};
+
+}  // namespace
+
+struct gf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 256;
+
+    // events
+    int ncols        = 0;
+    int64_t n        = 0;
+    double *col_base = nullptr;  // ncols * n doubles, column k at col_base + k*n
+    double **d_cols  = nullptr;  // device array [GF_MAX_OBS]
+    int identity_col = -1;
+    bool identity_ok = false;
+
+    // program (host copies)
+    std::vector<int32_t> funcs;
+    std::vector<double> params, consts, obs, norms;
+    std::vector<GfIns> ins;           // root program followed by compiled subprograms
+    std::map<int, int> sub_pc;        // function index -> first instruction
+    int max_col = -1;                 // largest observable column any instruction reads
+    std::vector<DalitzNode> dalitz;   // Dalitz nodes of the program, slot order
+
+    // device copies
+    double *d_params = nullptr, *d_consts = nullptr, *d_norms = nullptr;
+    int32_t *d_obsid = nullptr;
+    GfIns *d_ins = nullptr;
+    size_t cap_params = 0, cap_consts = 0, cap_norms = 0, cap_obsid = 0, cap_ins = 0;
+    GfDalitzSlot *d_dz = nullptr;
+    GfResDesc *d_res = nullptr;  // GF_MAX_DALITZ * GF_MAX_RES
+    GfDecay decay{};
+
+    std::map<int, WaveCache> caches;
+
+    // reduction scratch
+    gfk::Partial *d_partials = nullptr;
+    unsigned *d_counter      = nullptr;
+    double *d_result         = nullptr;  // [0] sum, [1] residual, [2] error word (as int at byte 16)
+    double *h_result         = nullptr;  // pinned
+    int32_t *d_err           = nullptr;
+    double *d_gram_partials  = nullptr;
+    size_t cap_gram_partials = 0;
+    double *d_gram_tiles     = nullptr;  // 768
+    double *h_gram_tiles     = nullptr;  // pinned 768
+    int mfma_layout          = -1;       // -1 unknown, else a layout_ij() code
+
+    // pending uploads of params/norms through pinned staging
+    double *h_stage       = nullptr;
+    size_t cap_stage      = 0;
+    bool params_dirty     = false;
+    bool norms_dirty      = false;
+    bool stage_in_flight  = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+bool valid_ctx(gf_ctx *ctx) {
+    if(!ctx) {
+        fail(GF_ERR_INVALID, "null context");
+        return false;
+    }
+    return hipSetDevice(ctx->device) == hipSuccess;
+}
+
+template <class T>
+gf_status ensure_dev(T *&ptr, size_t &cap, size_t count, hipStream_t s) {
+    if(count <= cap && ptr)
+        return GF_OK;
+    if(ptr) {
+        GF_HIP(hipStreamSynchronize(s));
+        GF_HIP(hipFree(ptr));
+        ptr = nullptr;
+    }
+    size_t c = count < 16 ? 16 : count;
+    if(hipMalloc((void **)&ptr, c * sizeof(T)) != hipSuccess) {
+        ptr = nullptr;
+        cap = 0;
+        return fail(GF_ERR_NOMEM, "device allocation of %zu bytes failed", c * sizeof(T));
+    }
+    cap = c;
+    return GF_OK;
+}
+
+// ------------------------------------------------------- tree compilation
+
+// ParameterContainer cursor (include/goofit/PDFs/ParameterContainer.h:17-21)
+struct Cur {
+    int p = 0, c = 0, o = 0, n = 0, f = 0;
+};
+
+struct Compiler {
+    gf_ctx *ctx;
+    std::vector<GfIns> out;
+    std::string err;
+    int max_col = -1;
+
+    bool in_range(const Cur &q) const {
+        return q.p >= 0 && q.p < (int)ctx->params.size() && q.c >= 0 && q.c < (int)ctx->consts.size() && q.o >= 0
+               && q.o < (int)ctx->obs.size() && q.n >= 0 && q.n < (int)ctx->norms.size() && q.f >= 0
+               && q.f < (int)ctx->funcs.size();
+    }
+    int np(const Cur &q) const { return (int)ctx->params[q.p]; }
+    int nc(const Cur &q) const { return (int)ctx->consts[q.c]; }
+    int no(const Cur &q) const { return (int)ctx->obs[q.o]; }
+    int nn(const Cur &q) const { return (int)ctx->norms[q.n]; }
+
+    // slow incrementIndex(), src/PDFs/ParameterContainer.cu:48-60
+    bool inc(Cur &q) {
+        if(!in_range(q)) {
+            err = "flattened arrays end before function " + std::to_string(q.f);
+            return false;
+        }
+        int a = np(q), b = nc(q), c = no(q), d = nn(q);
+        if(a < 0 || b < 0 || c < 0 || d < 0) {
+            err = "negative count in the block of function " + std::to_string(q.f);
+            return false;
+        }
+        q.f++;
+        q.p += a + 1;
+        q.c += b + 1;
+        q.o += c + 1;
+        q.n += d + 1;
+        return true;
+    }
+
+    bool expect(const Cur &q, int p, int c, int o, const char *name) {
+        if(!in_range(q)) {
+            err = std::string(name) + ": arrays end at function " + std::to_string(q.f);
+            return false;
+        }
+        if((p >= 0 && np(q) != p) || (c >= 0 && nc(q) != c) || (o >= 0 && no(q) != o) || nn(q) != 1) {
+            char b[256];
+            snprintf(b, sizeof(b), "%s at function %d: block counts (%d,%d,%d,%d) differ from (%d,%d,%d,1)", name,
+                     q.f, np(q), nc(q), no(q), nn(q), p, c, o);
+            err = b;
+            return false;
+        }
+        // every value of the blocks must exist
+        if(np(q) > 0 && q.p + np(q) > (int)ctx->params.size() - 1) {
+            err = std::string(name) + ": parameter block runs past the array";
+            return false;
+        }
+        if(nc(q) > 0 && q.c + nc(q) > (int)ctx->consts.size() - 1) {
+            err = std::string(name) + ": constant block runs past the array";
+            return false;
+        }
+        if(no(q) > 0 && q.o + no(q) > (int)ctx->obs.size() - 1) {
+            err = std::string(name) + ": observable block runs past the array";
+            return false;
+        }
+        if(q.n + 1 > (int)ctx->norms.size() - 1) {
+            err = std::string(name) + ": normalisation block runs past the array";
+            return false;
+        }
+        return true;
+    }
+
+    bool col_of(const Cur &q, int i, int &col) {
+        double v = ctx->obs[q.o + 1 + i];
+        col      = (int)v;
+        if(col < 0 || col >= GF_MAX_OBS || (double)col != v) {
+            err = "observable index " + std::to_string(v) + " not in [0," + std::to_string(GF_MAX_OBS) + ")";
+            return false;
+        }
+        if(col > max_col)
+            max_col = col;
+        return true;
+    }
+
+    void push(int op, int lvl, int a, int p, int c, int n, int k, int x) {
+        GfIns g;
+        g.op  = op;
+        g.lvl = lvl;
+        g.a   = a;
+        g.p   = p;
+        g.c   = c;
+        g.n   = n;
+        g.k   = k;
+        g.x   = x;
+        out.push_back(g);
+    }
+
+    // Emit the subtree rooted at q.f; on return q points past it.  lvl is the first
+    // free accumulator level.
+    bool emit(Cur &q, int lvl) {
+        if(!in_range(q)) {
+            err = "function index " + std::to_string(q.f) + " outside the flattened arrays";
+            return false;
+        }
+        const int f = ctx->funcs[q.f];
+        int col     = 0;
+        switch(f) {
+        case GF_FN_GAUSSIAN:
+            if(!expect(q, 2, 0, 1, "GaussianPdf") || !col_of(q, 0, col))
+                return false;
+            push(GI_GAUSS, lvl, col, q.p + 1, q.c + 1, q.n + 1, 2, 0);
+            return inc(q);
+        case GF_FN_EXP:
+            if(!expect(q, 1, 0, 1, "ExpPdf") || !col_of(q, 0, col))
+                return false;
+            push(GI_EXP, lvl, col, q.p + 1, q.c + 1, q.n + 1, 1, 0);
+            return inc(q);
+        case GF_FN_EXP_OFFSET:
+            if(!expect(q, 2, 0, 1, "ExpPdf(offset)") || !col_of(q, 0, col))
+                return false;
+            push(GI_EXP_OFF, lvl, col, q.p + 1, q.c + 1, q.n + 1, 2, 0);
+            return inc(q);
+        case GF_FN_EXP_POLY:
+        case GF_FN_EXP_POLY_OFFSET:
+            if(!expect(q, -1, 0, 1, "ExpPdf(poly)") || !col_of(q, 0, col))
+                return false;
+            push(f == GF_FN_EXP_POLY ? GI_EXP_POLY : GI_EXP_POLY_OFF, lvl, col, q.p + 1, q.c + 1, q.n + 1, np(q), 0);
+            return inc(q);
+        case GF_FN_EXPGAUS:
+            if(!expect(q, 3, 0, 1, "ExpGausPdf") || !col_of(q, 0, col))
+                return false;
+            push(GI_EXPGAUS, lvl, col, q.p + 1, q.c + 1, q.n + 1, 3, 0);
+            return inc(q);
+        case GF_FN_POLYNOMIAL:
+        case GF_FN_OFFSET_POLYNOMIAL:
+            if(!expect(q, -1, 1, 1, "PolynomialPdf") || !col_of(q, 0, col))
+                return false;
+            push(f == GF_FN_POLYNOMIAL ? GI_POLY : GI_OFF_POLY, lvl, col, q.p + 1, q.c + 1, q.n + 1, np(q), 0);
+            return inc(q);
+        case GF_FN_MULTI_POLYNOMIAL: {
+            if(!expect(q, -1, -1, -1, "PolynomialPdf(multi)"))
+                return false;
+            const int nobs = no(q);
+            if(nobs < 1 || nobs > GF_MAX_OBS || nc(q) < 1) {
+                err = "PolynomialPdf(multi): bad observable/constant counts";
+                return false;
+            }
+            for(int i = 0; i < nobs; i++)
+                if(!col_of(q, i, col))
+                    return false;
+            // coefficient count the device loop will read (PolynomialPdf.cu:61-111)
+            int maxDegree = (int)ctx->consts[q.c + 1] + 1, boxes = 1, used = 1;
+            for(int i = 0; i < nobs; i++)
+                boxes *= maxDegree;
+            for(int b = 1; b < boxes; b++) {
+                int ci = b, sum = 0;
+                for(int o = 0; o < nobs; o++) {
+                    sum += ci % maxDegree;
+                    ci /= maxDegree;
+                }
+                if(sum < maxDegree)
+                    used++;
+            }
+            if(np(q) < nobs + used) {
+                err = "PolynomialPdf(multi): fewer coefficients than terms";
+                return false;
+            }
+            push(GI_MULTI_POLY, lvl, nobs, q.p + 1, q.c + 1, q.n + 1, np(q), q.o + 1);
+            return inc(q);
+        }
+        case GF_FN_ADD:
+        case GF_FN_ADD_EXT: {
+            if(!expect(q, -1, -1, -1, "AddPdf"))
+                return false;
+            if(lvl >= GF_MAX_LVL) {
+                err = "PDF tree nests deeper than " + std::to_string(GF_MAX_LVL) + " combining levels";
+                return false;
+            }
+            const int nw   = np(q);
+            const int wbase = q.p + 1;
+            const bool ext = f == GF_FN_ADD_EXT;
+            if(!inc(q))
+                return false;
+            push(GF_LVL_OP(GK_PUSH0, lvl), lvl, 0, 0, 0, 0, 0, 0);
+            for(int i = 0; i < nw; i++) {
+                const int normslot = q.n + 1;
+                if(!emit(q, lvl + 1))
+                    return false;
+                push(GF_LVL_OP(GK_ADD_ACC, lvl), lvl, 0, wbase + i, 0, normslot, 0, 0);
+            }
+            if(ext) {
+                push(GF_LVL_OP(GK_ADDEXT_END, lvl), lvl, 0, wbase, 0, 0, nw, 0);
+            } else {
+                const int normslot = q.n + 1;
+                if(!emit(q, lvl + 1))
+                    return false;
+                push(GF_LVL_OP(GK_ADD_LAST, lvl), lvl, 0, wbase, 0, normslot, nw, 0);
+                push(GF_LVL_OP(GK_POP, lvl), lvl, 0, 0, 0, 0, 0, 0);
+            }
+            return true;
+        }
+        case GF_FN_PROD: {
+            // device_ProdPdfs walks incrementIndex(1, 0, numCons, numObs, 1) (ProdPdf.cu:15)
+            if(!expect(q, 0, -1, -1, "ProdPdf"))
+                return false;
+            if(nc(q) < 1) {
+                err = "ProdPdf without component count";
+                return false;
+            }
+            if(lvl >= GF_MAX_LVL) {
+                err = "PDF tree nests deeper than " + std::to_string(GF_MAX_LVL) + " combining levels";
+                return false;
+            }
+            const int ncomp = (int)ctx->consts[q.c + 1];
+            if(!inc(q))
+                return false;
+            push(GF_LVL_OP(GK_PUSH1, lvl), lvl, 0, 0, 0, 0, 0, 0);
+            for(int i = 0; i < ncomp; i++) {
+                const int normslot = q.n + 1;
+                if(!emit(q, lvl + 1))
+                    return false;
+                push(GF_LVL_OP(GK_PROD_ACC, lvl), lvl, 0, 0, 0, normslot, 0, 0);
+            }
+            push(GF_LVL_OP(GK_POP, lvl), lvl, 0, 0, 0, 0, 0, 0);
+            return true;
+        }
+        case GF_FN_DALITZ: {
+            if(!expect(q, -1, 2, -1, "Amp3Body"))
+                return false;
+            const int nres = (int)ctx->consts[q.c + 1];
+            if(nres < 0 || nres > GF_MAX_RES || np(q) != 2 * nres || no(q) < 3) {
+                err = "Amp3Body: bad resonance/parameter/observable counts";
+                return false;
+            }
+            if(lvl >= GF_MAX_LVL) {
+                err = "PDF tree nests deeper than " + std::to_string(GF_MAX_LVL) + " combining levels";
+                return false;
+            }
+            int slot = -1;
+            for(size_t s = 0; s < ctx->dalitz.size(); s++)
+                if(ctx->dalitz[s].func == q.f)
+                    slot = (int)s;
+            if(slot < 0) {
+                err = "Amp3Body node not registered";
+                return false;
+            }
+            const int pbase = q.p + 1;
+            if(!inc(q))
+                return false;
+            for(int i = 0; i < nres; i++)
+                if(!inc(q))
+                    return false;
+            push(GF_LVL_OP(GK_DALITZ, lvl), lvl, nres, pbase, 0, 0, 0, slot);
+            if(!emit(q, lvl + 1))
+                return false;
+            push(GF_LVL_OP(GK_DALITZ_END, lvl), lvl, 0, 0, 0, 0, 0, 0);
+            return true;
+        }
+        case GF_FN_RBW:
+        case GF_FN_RBW_SYM:
+        case GF_FN_GS:
+        case GF_FN_NONRES:
+            err = "a resonance is only valid as a component of Amp3Body";
+            return false;
+        default: err = "unknown function code " + std::to_string(f); return false;
+        }
+    }
+};
+
+// Skip the subtree at q (no code emitted).
+bool skip_subtree(Compiler &C, Cur &q) {
+    if(!C.in_range(q)) {
+        C.err = "subtree runs past the arrays";
+        return false;
+    }
+    const int f  = C.ctx->funcs[q.f];
+    int nchild   = 0;
+    if(f == GF_FN_ADD)
+        nchild = C.np(q) + 1;
+    else if(f == GF_FN_ADD_EXT)
+        nchild = C.np(q);
+    else if(f == GF_FN_PROD)
+        nchild = C.nc(q) >= 1 ? (int)C.ctx->consts[q.c + 1] : 0;
+    else if(f == GF_FN_DALITZ)
+        nchild = C.nc(q) >= 1 ? (int)C.ctx->consts[q.c + 1] + 1 : 0;
+    if(!C.inc(q))
+        return false;
+    for(int i = 0; i < nchild; i++)
+        if(!skip_subtree(C, q))
+            return false;
+    return true;
+}
+
+bool cursor_at(Compiler &C, int func, Cur &q) {
+    q = Cur();
+    while(q.f < func)
+        if(!C.inc(q))
+            return false;
+    return true;
+}
+
+// Discover the Dalitz nodes (resonance layout, efficiency index, observable columns).
+gf_status scan_dalitz(gf_ctx *ctx) {
+    ctx->dalitz.clear();
+    Compiler C{ctx, {}, {}, -1};
+    for(int f = 0; f < (int)ctx->funcs.size(); f++) {
+        if(ctx->funcs[f] != GF_FN_DALITZ)
+            continue;
+        if((int)ctx->dalitz.size() >= GF_MAX_DALITZ)
+            return fail(GF_ERR_INVALID, "more than %d Amp3Body nodes in one tree", GF_MAX_DALITZ);
+        Cur q;
+        if(!cursor_at(C, f, q))
+            return fail(GF_ERR_INVALID, "%s", C.err.c_str());
+        DalitzNode d;
+        d.func = f;
+        if(C.nc(q) != 2 || C.no(q) < 3)
+            return fail(GF_ERR_INVALID, "Amp3Body at %d: expected 2 constants and >= 3 observables", f);
+        d.nres      = (int)ctx->consts[q.c + 1];
+        d.cache_key = (int)ctx->consts[q.c + 2];
+        if(d.nres < 0 || d.nres > GF_MAX_RES)
+            return fail(GF_ERR_INVALID, "Amp3Body at %d: %d resonances (at most %d)", f, d.nres, GF_MAX_RES);
+        int c12, c13, cnum;
+        if(!C.col_of(q, 0, c12) || !C.col_of(q, 1, c13) || !C.col_of(q, 2, cnum))
+            return fail(GF_ERR_INVALID, "%s", C.err.c_str());
+        d.col_m12 = c12;
+        d.col_m13 = c13;
+        d.col_num = cnum;
+        if(!C.inc(q))
+            return fail(GF_ERR_INVALID, "%s", C.err.c_str());
+        for(int i = 0; i < d.nres; i++) {
+            if(!C.in_range(q))
+                return fail(GF_ERR_INVALID, "Amp3Body at %d: resonance %d outside the arrays", f, i);
+            int rf = ctx->funcs[q.f];
+            GfResDesc r{};
+            if(rf == GF_FN_RBW || rf == GF_FN_RBW_SYM || rf == GF_FN_GS) {
+                if(C.np(q) != 2 || C.nc(q) != 3 || C.no(q) != 0)
+                    return fail(GF_ERR_INVALID, "resonance at %d: expected 2 parameters and 3 constants", q.f);
+                r.kind = rf == GF_FN_RBW ? GR_RBW : (rf == GF_FN_RBW_SYM ? GR_RBW_SYM : GR_GS);
+            } else if(rf == GF_FN_NONRES) {
+                r.kind = GR_NONRES;
+            } else {
+                return fail(GF_ERR_INVALID, "Amp3Body at %d: component %d is not a resonance", f, i);
+            }
+            r.p = q.p + 1;
+            r.c = q.c + 1;
+            d.res_funcs.push_back(q.f);
+            d.res.push_back(r);
+            if(!C.inc(q))
+                return fail(GF_ERR_INVALID, "%s", C.err.c_str());
+        }
+        d.eff_root = q.f;
+        if(!skip_subtree(C, q))
+            return fail(GF_ERR_INVALID, "%s", C.err.c_str());
+        d.eff_func = q.f - 1;
+        ctx->dalitz.push_back(d);
+    }
+    return GF_OK;
+}
+
+gf_status upload_dalitz_table(gf_ctx *ctx) {
+    GfDalitzSlot slots[GF_MAX_DALITZ];
+    GfResDesc res[GF_MAX_DALITZ * GF_MAX_RES];
+    memset(slots, 0, sizeof(slots));
+    memset(res, 0, sizeof(res));
+    for(size_t s = 0; s < ctx->dalitz.size(); s++) {
+        const DalitzNode &d = ctx->dalitz[s];
+        auto it             = ctx->caches.find(d.cache_key);
+        slots[s].wave       = it != ctx->caches.end() ? it->second.wave : nullptr;
+        slots[s].ncache     = it != ctx->caches.end() ? it->second.n : 0;
+        slots[s].col_m12    = d.col_m12;
+        slots[s].col_m13    = d.col_m13;
+        slots[s].col_num    = d.col_num;
+        slots[s].nres       = d.nres;
+        for(int i = 0; i < d.nres; i++)
+            res[s * GF_MAX_RES + i] = d.res[i];
+    }
+    GF_HIP(hipMemcpyAsync(ctx->d_dz, slots, sizeof(slots), hipMemcpyHostToDevice, ctx->stream));
+    GF_HIP(hipMemcpyAsync(ctx->d_res, res, sizeof(res), hipMemcpyHostToDevice, ctx->stream));
+    GF_HIP(hipStreamSynchronize(ctx->stream));
+    return GF_OK;
+}
+
+gf_status upload_ins(gf_ctx *ctx) {
+    gf_status st = ensure_dev(ctx->d_ins, ctx->cap_ins, ctx->ins.size(), ctx->stream);
+    if(st)
+        return st;
+    GF_HIP(hipMemcpyAsync(ctx->d_ins, ctx->ins.data(), ctx->ins.size() * sizeof(GfIns), hipMemcpyHostToDevice,
+                          ctx->stream));
+    GF_HIP(hipStreamSynchronize(ctx->stream));
+    return GF_OK;
+}
+
+// Instruction index of the program that evaluates the subtree at `func` (compiled on first use).
+gf_status subprogram(gf_ctx *ctx, int func, int &pc) {
+    auto it = ctx->sub_pc.find(func);
+    if(it != ctx->sub_pc.end()) {
+        pc = it->second;
+        return GF_OK;
+    }
+    Compiler C{ctx, {}, {}, -1};
+    Cur q;
+    if(func < 0 || func >= (int)ctx->funcs.size() || !cursor_at(C, func, q))
+        return fail(GF_ERR_INVALID, "function index %d outside the tree", func);
+    if(!C.emit(q, 0))
+        return fail(GF_ERR_INVALID, "%s", C.err.c_str());
+    C.push(GI_END, 0, 0, 0, 0, 0, 0, 0);
+    pc = (int)ctx->ins.size();
+    ctx->ins.insert(ctx->ins.end(), C.out.begin(), C.out.end());
+    if(C.max_col > ctx->max_col)
+        ctx->max_col = C.max_col;
+    ctx->sub_pc[func] = pc;
+    return upload_ins(ctx);
+}
+
+// Flush pending parameter/normalisation updates through the pinned staging buffer.
+gf_status flush_uploads(gf_ctx *ctx) {
+    if(!ctx->params_dirty && !ctx->norms_dirty)
+        return GF_OK;
+    if(ctx->stage_in_flight) {
+        GF_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->stage_in_flight = false;
+    }
+    const size_t np = ctx->params.size(), nn = ctx->norms.size();
+    if(np + nn > ctx->cap_stage) {
+        if(ctx->h_stage)
+            GF_HIP(hipHostFree(ctx->h_stage));
+        ctx->h_stage   = nullptr;
+        ctx->cap_stage = 0;
+        size_t c       = (np + nn) * 2 + 64;
+        GF_HIP(hipHostMalloc((void **)&ctx->h_stage, c * sizeof(double), hipHostMallocDefault));
+        ctx->cap_stage = c;
+    }
+    if(ctx->params_dirty && np) {
+        memcpy(ctx->h_stage, ctx->params.data(), np * sizeof(double));
+        GF_HIP(hipMemcpyAsync(ctx->d_params, ctx->h_stage, np * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
+    if(ctx->norms_dirty && nn) {
+        memcpy(ctx->h_stage + np, ctx->norms.data(), nn * sizeof(double));
+        GF_HIP(hipMemcpyAsync(ctx->d_norms, ctx->h_stage + np, nn * sizeof(double), hipMemcpyHostToDevice,
+                              ctx->stream));
+    }
+    ctx->params_dirty = ctx->norms_dirty = false;
+    ctx->stage_in_flight                 = true;
+    return GF_OK;
+}
+
+GfProgArgs prog_args(gf_ctx *ctx) {
+    GfProgArgs a;
+    a.ins          = ctx->d_ins;
+    a.params       = ctx->d_params;
+    a.consts       = ctx->d_consts;
+    a.norms        = ctx->d_norms;
+    a.obsid        = ctx->d_obsid;
+    a.cols         = ctx->d_cols;
+    a.dz           = ctx->d_dz;
+    a.nevents      = ctx->n;
+    a.identity_num = ctx->identity_ok ? 1 : 0;
+    a.nobs         = ctx->ncols;
+    a.total_events = (double)ctx->n;
+    a.decay        = ctx->decay;
+    a.err          = ctx->d_err;
+    return a;
+}
+
+int event_blocks(gf_ctx *ctx, int64_t n) {
+    int64_t per = (int64_t)GF_BLOCK * 2;
+    int64_t nb  = (n + per - 1) / per;
+    int64_t cap = (int64_t)ctx->cu_count * 8;
+    if(cap > GF_MAX_PARTIALS)
+        cap = GF_MAX_PARTIALS;
+    if(nb > cap)
+        nb = cap;
+    if(nb < 1)
+        nb = 1;
+    return (int)nb;
+}
+
+gf_status check_events_for_program(gf_ctx *ctx, int metric) {
+    if(ctx->ins.empty())
+        return fail(GF_ERR_STATE, "no program installed (gf_program_set)");
+    if(!ctx->col_base || ctx->n <= 0)
+        return fail(GF_ERR_STATE, "no events uploaded");
+    if(ctx->max_col >= ctx->ncols)
+        return fail(GF_ERR_STATE, "program reads observable column %d but only %d columns are uploaded", ctx->max_col,
+                    ctx->ncols);
+    if(metric < 0 || metric > GF_METRIC_CHISQ)
+        return fail(GF_ERR_INVALID, "unknown metric %d", metric);
+    if(metric >= GF_METRIC_BINAVG && ctx->ncols < 2)
+        return fail(GF_ERR_STATE, "binned metric needs (bin entry, bin volume) columns");
+    for(const DalitzNode &d : ctx->dalitz) {
+        auto it = ctx->caches.find(d.cache_key);
+        if(it == ctx->caches.end() || !it->second.wave)
+            return fail(GF_ERR_STATE, "Amp3Body at %d has no wave cache (gf_dalitz_set_data_size)", d.func);
+    }
+    return GF_OK;
+}
+
+// Read back the reduction result and the error word; clears the error word.
+gf_status read_result(gf_ctx *ctx, double *sum, double *comp) {
+    GF_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    GF_HIP(hipMemcpyAsync(ctx->h_result + 2, ctx->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    GF_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->stage_in_flight = false;
+    int32_t err;
+    memcpy(&err, ctx->h_result + 2, sizeof(int32_t));
+    if(err) {
+        GF_HIP(hipMemsetAsync(ctx->d_err, 0, sizeof(int32_t), ctx->stream));
+        GF_HIP(hipStreamSynchronize(ctx->stream));
+        return fail(GF_ERR_INVALID, "Dalitz event number outside the wave cache (error word %d)", err);
+    }
+    if(sum)
+        *sum = ctx->h_result[0];
+    if(comp)
+        *comp = ctx->h_result[1];
+    return GF_OK;
+}
+
+DalitzNode *find_dalitz(gf_ctx *ctx, int func, int *slot) {
+    for(size_t s = 0; s < ctx->dalitz.size(); s++)
+        if(ctx->dalitz[s].func == func) {
+            if(slot)
+                *slot = (int)s;
+            return &ctx->dalitz[s];
+        }
+    return nullptr;
+}
+
+// Candidate result layouts of v_mfma_f64_16x16x4f64: which D[i][j] register r of
+// lane l holds.  The probe in ensure_layout picks the one the hardware uses
+// (on gfx950: layout 2, D[4r + l/16][l%16]).
+void layout_ij(int layout, int l, int r, int &i, int &j) {
+    switch(layout) {
+    case 0: i = 4 * (l / 16) + r; j = l % 16; break;
+    case 1: j = 4 * (l / 16) + r; i = l % 16; break;
+    case 2: i = 4 * r + l / 16; j = l % 16; break;
+    default: j = 4 * r + l / 16; i = l % 16; break;
+    }
+}
+
+// Run one MFMA on host operands a[64], b[64]; out[256] = D registers by lane.
+gf_status mfma_run(gf_ctx *ctx, const double *a, const double *b, double *out) {
+    double *d = ctx->d_gram_tiles;  // 768 doubles: a at 0, b at 64, result at 256
+    GF_HIP(hipMemcpyAsync(d, a, 64 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    GF_HIP(hipMemcpyAsync(d + 64, b, 64 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    int rc = gfk::mfma_probe(d, d + 64, d + 256, ctx->stream);
+    if(rc)
+        return fail(GF_ERR_HIP, "MFMA probe launch failed (%d)", rc);
+    GF_HIP(hipMemcpyAsync(out, d + 256, 256 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    GF_HIP(hipStreamSynchronize(ctx->stream));
+    return GF_OK;
+}
+
+gf_status ensure_layout(gf_ctx *ctx) {
+    if(ctx->mfma_layout >= 0)
+        return GF_OK;
+    // A[i][0] = i+1, B[0][j] = 1, A[i][1] = 1, B[1][j] = 100*(j+1) under the operand
+    // layout k_gram uses (A[i][k] in lane i + 16k, B[k][j] in lane j + 16k), so
+    // D[i][j] = (i+1) + 100*(j+1); accept either result register layout.
+    double a[64], b[64], out[256];
+    for(int l = 0; l < 64; l++) {
+        const int k = l >> 4, i = l & 15;
+        a[l]        = (k == 0) ? (double)(i + 1) : (k == 1 ? 1.0 : 0.0);
+        b[l]        = (k == 0) ? 1.0 : (k == 1 ? 100.0 * (double)(i + 1) : 0.0);
+    }
+    gf_status st = mfma_run(ctx, a, b, out);
+    if(st)
+        return st;
+    bool ok[4] = {true, true, true, true};
+    for(int l = 0; l < 64; l++)
+        for(int r = 0; r < 4; r++)
+            for(int c = 0; c < 4; c++) {
+                int i, j;
+                layout_ij(c, l, r, i, j);
+                if(out[4 * l + r] != (double)(i + 1) + 100.0 * (double)(j + 1))
+                    ok[c] = false;
+            }
+    for(int c = 0; c < 4; c++)
+        if(ok[c]) {
+            ctx->mfma_layout = c;
+            return GF_OK;
+        }
+    return fail(GF_ERR_HIP, "unexpected v_mfma_f64_16x16x4f64 layout: lane0 %g %g %g %g lane1 %g lane16 %g lane17 %g",
+                out[0], out[1], out[2], out[3], out[4], out[64], out[68]);
+}
+
+// Map the three MFMA tiles (RR, II, RI) into I_ij = sum e B_i conj(B_j), nres x nres.
+void tiles_to_matrix(gf_ctx *ctx, const double *tiles, int nres, double *out) {
+    double RR[16][16], II[16][16], RI[16][16];
+    for(int t = 0; t < 3; t++)
+        for(int r = 0; r < 4; r++)
+            for(int l = 0; l < 64; l++) {
+                int i, j;
+                layout_ij(ctx->mfma_layout, l, r, i, j);
+                double v = tiles[(t * 4 + r) * 64 + l];
+                (t == 0 ? RR : (t == 1 ? II : RI))[i][j] = v;
+            }
+    for(int i = 0; i < nres; i++)
+        for(int j = 0; j < nres; j++) {
+            out[2 * (i * nres + j)]     = RR[i][j] + II[i][j];
+            out[2 * (i * nres + j) + 1] = RI[j][i] - RI[i][j];
+        }
+}
+
+gf_status run_gram(gf_ctx *ctx, const DalitzNode &d, int slot, int mode, const double *grid6, const double *m12,
+                   const double *m13, const double *w, int64_t np, int eff_func, double *out) {
+    gf_status st = ensure_layout(ctx);
+    if(st)
+        return st;
+    int eff_pc = -1;
+    if(mode != 2 && eff_func >= 0) {
+        st = subprogram(ctx, eff_func, eff_pc);
+        if(st)
+            return st;
+    }
+    st = flush_uploads(ctx);
+    if(st)
+        return st;
+    int64_t nb = (np + 127) / 128;
+    int64_t cap = (int64_t)ctx->cu_count * 4;
+    if(nb > cap)
+        nb = cap;
+    if(nb < 1)
+        nb = 1;
+    st = ensure_dev(ctx->d_gram_partials, ctx->cap_gram_partials, (size_t)nb * 768, ctx->stream);
+    if(st)
+        return st;
+    GfProgArgs a = prog_args(ctx);
+    int rc = gfk::launch_gram(a, mode, grid6, m12, m13, w, np, slot, d.col_m12, d.col_m13, eff_pc,
+                              ctx->d_res + slot * GF_MAX_RES, d.nres, ctx->d_gram_partials, ctx->d_gram_tiles,
+                              (int)nb, ctx->stream);
+    if(rc)
+        return fail(GF_ERR_HIP, "Gram launch failed (%d)", rc);
+    GF_HIP(hipMemcpyAsync(ctx->h_gram_tiles, ctx->d_gram_tiles, 768 * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    GF_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->stage_in_flight = false;
+    tiles_to_matrix(ctx, ctx->h_gram_tiles, d.nres, out);
+    return GF_OK;
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+
+extern "C" {
+
+int gf_device_count(void) {
+    int n = 0;
+    if(hipGetDeviceCount(&n) != hipSuccess)
+        return 0;
+    return n;
+}
+
+const char *gf_last_error(void) { return g_last_error.c_str(); }
+
+gf_status gf_ctx_create(gf_ctx **out, int device) {
+    if(!out)
+        return fail(GF_ERR_INVALID, "null output pointer");
+    *out  = nullptr;
+    int n = gf_device_count();
+    if(n <= 0)
+        return fail(GF_ERR_NODEVICE, "no GPU visible to HIP");
+    if(device < 0 || device >= n)
+        return fail(GF_ERR_INVALID, "device %d out of range (%d visible)", device, n);
+    GF_HIP(hipSetDevice(device));
+    gf_ctx *c = new gf_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if(hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->cu_count = prop.multiProcessorCount;
+    gf_status st = GF_OK;
+#define TRY(x)                                                                                                         \
+    do {                                                                                                               \
+        hipError_t e_ = (x);                                                                                           \
+        if(e_ != hipSuccess) {                                                                                         \
+            st = fail(GF_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));                                         \
+            gf_ctx_destroy(c);                                                                                         \
+            return st;                                                                                                 \
+        }                                                                                                              \
+    } while(0)
+    TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    TRY(hipMalloc((void **)&c->d_cols, GF_MAX_OBS * sizeof(double *)));
+    TRY(hipMalloc((void **)&c->d_dz, GF_MAX_DALITZ * sizeof(GfDalitzSlot)));
+    TRY(hipMalloc((void **)&c->d_res, GF_MAX_DALITZ * GF_MAX_RES * sizeof(GfResDesc)));
+    TRY(hipMalloc((void **)&c->d_partials, GF_MAX_PARTIALS * sizeof(gfk::Partial)));
+    TRY(hipMalloc((void **)&c->d_counter, 64));
+    TRY(hipMalloc((void **)&c->d_result, 64));
+    TRY(hipMalloc((void **)&c->d_err, 64));
+    TRY(hipMalloc((void **)&c->d_gram_tiles, 768 * sizeof(double)));
+    TRY(hipHostMalloc((void **)&c->h_result, 64, hipHostMallocDefault));
+    TRY(hipHostMalloc((void **)&c->h_gram_tiles, 768 * sizeof(double), hipHostMallocDefault));
+    TRY(hipMemset(c->d_counter, 0, 64));
+    TRY(hipMemset(c->d_err, 0, 64));
+    TRY(hipMemset(c->d_cols, 0, GF_MAX_OBS * sizeof(double *)));
+    TRY(hipMemset(c->d_dz, 0, GF_MAX_DALITZ * sizeof(GfDalitzSlot)));
+    TRY(hipEventCreate(&c->ev0));
+    TRY(hipEventCreate(&c->ev1));
+#undef TRY
+    *out = c;
+    return GF_OK;
+}
+
+void gf_ctx_destroy(gf_ctx *c) {
+    if(!c)
+        return;
+    (void)hipSetDevice(c->device);
+    if(c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    for(auto &kv : c->caches)
+        if(kv.second.wave)
+            (void)hipFree(kv.second.wave);
+    void *dev[] = {c->col_base,  c->d_cols,     c->d_params,  c->d_consts,       c->d_norms,
+                   c->d_obsid,   c->d_ins,      c->d_dz,      c->d_res,          c->d_partials,
+                   c->d_counter, c->d_result,   c->d_err,     c->d_gram_partials, c->d_gram_tiles};
+    for(void *p : dev)
+        if(p)
+            (void)hipFree(p);
+    if(c->h_result)
+        (void)hipHostFree(c->h_result);
+    if(c->h_gram_tiles)
+        (void)hipHostFree(c->h_gram_tiles);
+    if(c->h_stage)
+        (void)hipHostFree(c->h_stage);
+    if(c->ev0)
+        (void)hipEventDestroy(c->ev0);
+    if(c->ev1)
+        (void)hipEventDestroy(c->ev1);
+    if(c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// ---------------------------------------------------------------- events
+
+static gf_status alloc_columns(gf_ctx *ctx, int ncols, int64_t n) {
+    if(ncols < 1 || ncols > GF_MAX_OBS)
+        return fail(GF_ERR_INVALID, "column count %d not in [1,%d]", ncols, GF_MAX_OBS);
+    if(n < 0)
+        return fail(GF_ERR_INVALID, "negative event count");
+    GF_HIP(hipStreamSynchronize(ctx->stream));
+    if(ctx->col_base) {
+        GF_HIP(hipFree(ctx->col_base));
+        ctx->col_base = nullptr;
+    }
+    ctx->ncols       = ncols;
+    ctx->n           = n;
+    ctx->identity_ok = false;
+    ctx->identity_col = -1;
+    size_t bytes     = (size_t)ncols * (size_t)(n > 0 ? n : 1) * sizeof(double);
+    if(hipMalloc((void **)&ctx->col_base, bytes) != hipSuccess) {
+        ctx->col_base = nullptr;
+        ctx->n        = 0;
+        return fail(GF_ERR_NOMEM, "cannot allocate %zu bytes of event columns", bytes);
+    }
+    double *ptrs[GF_MAX_OBS];
+    for(int k = 0; k < GF_MAX_OBS; k++)
+        ptrs[k] = k < ncols ? ctx->col_base + (size_t)k * (size_t)n : ctx->col_base;
+    GF_HIP(hipMemcpy(ctx->d_cols, ptrs, sizeof(ptrs), hipMemcpyHostToDevice));
+    return GF_OK;
+}
+
+gf_status gf_events_alloc(gf_ctx *ctx, int nobs, int64_t n) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    return alloc_columns(ctx, nobs, n);
+}
+
+gf_status gf_events_upload(gf_ctx *ctx, int nobs, int64_t n, const double *const *cols) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(!cols)
+        return fail(GF_ERR_INVALID, "null column array");
+    for(int k = 0; k < nobs; k++)
+        if(!cols[k] && n > 0)
+            return fail(GF_ERR_INVALID, "column %d is null", k);
+    gf_status st = alloc_columns(ctx, nobs, n);
+    if(st)
+        return st;
+    for(int k = 0; k < nobs && n > 0; k++)
+        GF_HIP(hipMemcpy(ctx->col_base + (size_t)k * n, cols[k], (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    return GF_OK;
+}
+
+gf_status gf_events_upload_aos(gf_ctx *ctx, int nobs, int64_t n, const double *aos) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(!aos && n > 0)
+        return fail(GF_ERR_INVALID, "null event array");
+    if(nobs < 1 || nobs > GF_MAX_OBS)
+        return fail(GF_ERR_INVALID, "column count %d not in [1,%d]", nobs, GF_MAX_OBS);
+    std::vector<std::vector<double>> cols(nobs, std::vector<double>((size_t)n));
+    for(int64_t i = 0; i < n; i++)
+        for(int k = 0; k < nobs; k++)
+            cols[k][i] = aos[i * nobs + k];
+    std::vector<const double *> ptrs(nobs);
+    for(int k = 0; k < nobs; k++)
+        ptrs[k] = cols[k].data();
+    return gf_events_upload(ctx, nobs, n, ptrs.data());
+}
+
+gf_status gf_events_column(gf_ctx *ctx, int k, double **dev_ptr) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(!dev_ptr || k < 0 || k >= ctx->ncols || !ctx->col_base)
+        return fail(GF_ERR_INVALID, "column %d not allocated", k);
+    *dev_ptr = ctx->col_base + (size_t)k * ctx->n;
+    return GF_OK;
+}
+
+gf_status gf_events_set_identity_column(gf_ctx *ctx, int k) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(k < 0 || k >= ctx->ncols || !ctx->col_base)
+        return fail(GF_ERR_INVALID, "column %d not allocated", k);
+    int32_t one = 1;
+    GF_HIP(hipMemcpyAsync(ctx->d_err + 4, &one, sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    int rc = gfk::launch_identity_check(ctx->col_base + (size_t)k * ctx->n, ctx->n, ctx->d_err + 4, ctx->stream);
+    if(rc)
+        return fail(GF_ERR_HIP, "identity check launch failed (%d)", rc);
+    int32_t flag = 0;
+    GF_HIP(hipMemcpyAsync(&flag, ctx->d_err + 4, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    GF_HIP(hipStreamSynchronize(ctx->stream));
+    if(!flag) {
+        ctx->identity_ok  = false;
+        ctx->identity_col = -1;
+        return fail(GF_ERR_INVALID, "column %d is not the row index 0..n-1", k);
+    }
+    ctx->identity_ok  = true;
+    ctx->identity_col = k;
+    return GF_OK;
+}
+
+// ---------------------------------------------------------------- program
+
+gf_status gf_program_set(gf_ctx *ctx, const int32_t *funcs, int nfunc, const double *params, int np,
+                         const double *consts, int nc, const double *obs, int no, const double *norms, int nn) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(nfunc < 1 || !funcs || np < 1 || !params || nc < 1 || !consts || no < 1 || !obs || nn < 2 || !norms)
+        return fail(GF_ERR_INVALID, "empty flattened arrays");
+    ctx->funcs.assign(funcs, funcs + nfunc);
+    ctx->params.assign(params, params + np);
+    ctx->consts.assign(consts, consts + nc);
+    ctx->obs.assign(obs, obs + no);
+    ctx->norms.assign(norms, norms + nn);
+    ctx->ins.clear();
+    ctx->sub_pc.clear();
+    ctx->max_col = -1;
+    gf_status st = scan_dalitz(ctx);
+    if(st) {
+        ctx->funcs.clear();
+        return st;
+    }
+    Compiler C{ctx, {}, {}, -1};
+    Cur q;
+    if(!C.emit(q, 0)) {
+        ctx->funcs.clear();
+        return fail(GF_ERR_INVALID, "%s", C.err.c_str());
+    }
+    if(q.f != nfunc) {
+        std::string e = "the tree at function 0 covers " + std::to_string(q.f) + " of " + std::to_string(nfunc)
+                        + " functions";
+        ctx->funcs.clear();
+        return fail(GF_ERR_INVALID, "%s", e.c_str());
+    }
+    C.push(GI_END, 0, 0, 0, 0, 0, 0, 0);
+    ctx->ins     = C.out;
+    ctx->max_col = C.max_col;
+    ctx->sub_pc[0] = 0;
+    for(const DalitzNode &d : ctx->dalitz) {
+        int cols[3] = {d.col_m12, d.col_m13, d.col_num};
+        for(int c : cols)
+            if(c > ctx->max_col)
+                ctx->max_col = c;
+    }
+    std::vector<int32_t> obsid(no);
+    for(int i = 0; i < no; i++)
+        obsid[i] = (int32_t)obs[i];
+    if((st = ensure_dev(ctx->d_params, ctx->cap_params, (size_t)np, ctx->stream))
+       || (st = ensure_dev(ctx->d_consts, ctx->cap_consts, (size_t)nc, ctx->stream))
+       || (st = ensure_dev(ctx->d_norms, ctx->cap_norms, (size_t)nn, ctx->stream))
+       || (st = ensure_dev(ctx->d_obsid, ctx->cap_obsid, (size_t)no, ctx->stream)))
+        return st;
+    GF_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->stage_in_flight = false;
+    GF_HIP(hipMemcpy(ctx->d_params, params, np * sizeof(double), hipMemcpyHostToDevice));
+    GF_HIP(hipMemcpy(ctx->d_consts, consts, nc * sizeof(double), hipMemcpyHostToDevice));
+    GF_HIP(hipMemcpy(ctx->d_norms, norms, nn * sizeof(double), hipMemcpyHostToDevice));
+    GF_HIP(hipMemcpy(ctx->d_obsid, obsid.data(), no * sizeof(int32_t), hipMemcpyHostToDevice));
+    ctx->params_dirty = ctx->norms_dirty = false;
+    if((st = upload_ins(ctx)))
+        return st;
+    return upload_dalitz_table(ctx);
+}
+
+gf_status gf_params_update(gf_ctx *ctx, const double *params, int np) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(ctx->funcs.empty())
+        return fail(GF_ERR_STATE, "no program installed");
+    if(np != (int)ctx->params.size() || !params)
+        return fail(GF_ERR_INVALID, "parameter array has %d entries, the program %zu", np, ctx->params.size());
+    for(int i = 0; i < np; i++)
+        if(ctx->params[i] != params[i]) {
+            // counts are structure: a changed count would change the tree walk
+            ctx->params.assign(params, params + np);
+            ctx->params_dirty = true;
+            break;
+        }
+    return GF_OK;
+}
+
+gf_status gf_norms_update(gf_ctx *ctx, const double *norms, int nn) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(ctx->funcs.empty())
+        return fail(GF_ERR_STATE, "no program installed");
+    if(nn != (int)ctx->norms.size() || !norms)
+        return fail(GF_ERR_INVALID, "normalisation array has %d entries, the program %zu", nn, ctx->norms.size());
+    for(int i = 0; i < nn; i++)
+        if(ctx->norms[i] != norms[i]) {
+            ctx->norms.assign(norms, norms + nn);
+            ctx->norms_dirty = true;
+            break;
+        }
+    return GF_OK;
+}
+
+gf_status gf_decay3_set(gf_ctx *ctx, const double decay[6]) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(!decay)
+        return fail(GF_ERR_INVALID, "null decay array");
+    ctx->decay.motherMass          = decay[0];
+    ctx->decay.daug1Mass           = decay[1];
+    ctx->decay.daug2Mass           = decay[2];
+    ctx->decay.daug3Mass           = decay[3];
+    ctx->decay.meson_radius        = decay[4];
+    ctx->decay.mother_meson_radius = decay[5];
+    return GF_OK;
+}
+
+// ------------------------------------------------------------- reductions
+
+gf_status gf_reduce_metric(gf_ctx *ctx, int metric, double *sum, double *comp) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    gf_status st = check_events_for_program(ctx, metric);
+    if(st)
+        return st;
+    if((st = flush_uploads(ctx)))
+        return st;
+    GfProgArgs a = prog_args(ctx);
+    int rc = gfk::launch_reduce_metric(a, metric, ctx->d_partials, ctx->d_counter, ctx->d_result,
+                                       event_blocks(ctx, ctx->n), ctx->stream);
+    if(rc)
+        return fail(GF_ERR_HIP, "reduce launch failed (%d)", rc);
+    return read_result(ctx, sum, comp);
+}
+
+gf_status gf_eval_metric_device(gf_ctx *ctx, int metric, double *dev_out, int64_t n) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    gf_status st = check_events_for_program(ctx, metric);
+    if(st)
+        return st;
+    if(!dev_out || n != ctx->n)
+        return fail(GF_ERR_INVALID, "output has %lld entries, %lld events uploaded", (long long)n, (long long)ctx->n);
+    if((st = flush_uploads(ctx)))
+        return st;
+    GfProgArgs a = prog_args(ctx);
+    int rc       = gfk::launch_eval_metric(a, metric, dev_out, event_blocks(ctx, ctx->n), ctx->stream);
+    if(rc)
+        return fail(GF_ERR_HIP, "eval launch failed (%d)", rc);
+    return GF_OK;
+}
+
+gf_status gf_eval_metric(gf_ctx *ctx, int metric, double *out, int64_t n) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(!out || n != ctx->n)
+        return fail(GF_ERR_INVALID, "output has %lld entries, %lld events uploaded", (long long)n, (long long)ctx->n);
+    double *d_out = nullptr;
+    if(hipMalloc((void **)&d_out, (size_t)(n > 0 ? n : 1) * sizeof(double)) != hipSuccess)
+        return fail(GF_ERR_NOMEM, "cannot allocate the output of %lld events", (long long)n);
+    gf_status st = gf_eval_metric_device(ctx, metric, d_out, n);
+    if(!st) {
+        if(hipMemcpyAsync(out, d_out, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+            st = fail(GF_ERR_HIP, "copy of the evaluated events failed");
+        else
+            st = read_result(ctx, nullptr, nullptr);
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_out);
+    return st;
+}
+
+gf_status gf_reduce_bins(gf_ctx *ctx, int func_idx, const double *ranges, int k, double *sum) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(ctx->funcs.empty())
+        return fail(GF_ERR_STATE, "no program installed");
+    if(!ranges || k < 1 || k > GF_MAX_OBS || !sum)
+        return fail(GF_ERR_INVALID, "grid needs 1..%d observables", GF_MAX_OBS);
+    int64_t total = 1;
+    for(int i = 0; i < k; i++) {
+        double nb = floor(ranges[3 * i + 2] + 0.5);
+        if(!(nb >= 1) || nb > 2147483647.0)
+            return fail(GF_ERR_INVALID, "bin count %g of grid dimension %d", ranges[3 * i + 2], i);
+        total *= (int64_t)nb;
+        if(total > ((int64_t)1 << 40))
+            return fail(GF_ERR_INVALID, "grid of more than 2^40 bins");
+    }
+    int pc       = 0;
+    gf_status st = subprogram(ctx, func_idx, pc);
+    if(st)
+        return st;
+    // observable slot of each grid dimension: the function's own observables in order
+    Compiler C{ctx, {}, {}, -1};
+    Cur q;
+    if(!cursor_at(C, func_idx, q))
+        return fail(GF_ERR_INVALID, "%s", C.err.c_str());
+    int slots[GF_MAX_OBS] = {0};
+    int nobs_f             = C.no(q);
+    for(int i = 0; i < k; i++) {
+        int idx = (k == 1) ? 0 : i;
+        if(idx >= nobs_f)
+            return fail(GF_ERR_INVALID, "function %d has %d observables, grid has %d", func_idx, nobs_f, k);
+        if(!C.col_of(q, idx, slots[i]))
+            return fail(GF_ERR_INVALID, "%s", C.err.c_str());
+    }
+    for(const DalitzNode &d : ctx->dalitz)
+        if(d.func >= func_idx) {
+            auto it = ctx->caches.find(d.cache_key);
+            if(it == ctx->caches.end() || !it->second.wave) {
+                Cur r;
+                // only an error if the Dalitz node sits inside this subtree
+                Compiler C2{ctx, {}, {}, -1};
+                if(cursor_at(C2, func_idx, r) && skip_subtree(C2, r) && d.func < r.f)
+                    return fail(GF_ERR_STATE, "Amp3Body at %d has no wave cache", d.func);
+            }
+        }
+    if((st = flush_uploads(ctx)))
+        return st;
+    GfProgArgs a = prog_args(ctx);
+    int rc = gfk::launch_reduce_bins(a, pc, ranges, slots, k, total, ctx->d_partials, ctx->d_counter, ctx->d_result,
+                                     event_blocks(ctx, total), ctx->stream);
+    if(rc)
+        return fail(GF_ERR_HIP, "grid reduce launch failed (%d)", rc);
+    return read_result(ctx, sum, nullptr);
+}
+
+// ---------------------------------------------------------------- Dalitz
+
+gf_status gf_dalitz_set_data_size(gf_ctx *ctx, int dalitz_func, int64_t n, int64_t offset) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    DalitzNode *d = find_dalitz(ctx, dalitz_func, nullptr);
+    if(!d)
+        return fail(GF_ERR_INVALID, "function %d is not an Amp3Body node", dalitz_func);
+    if(n < 1 || offset < 0)
+        return fail(GF_ERR_INVALID, "bad cache size %lld / offset %lld", (long long)n, (long long)offset);
+    WaveCache &wc = ctx->caches[d->cache_key];
+    GF_HIP(hipStreamSynchronize(ctx->stream));
+    if(wc.wave) {
+        GF_HIP(hipFree(wc.wave));
+        wc.wave = nullptr;
+    }
+    size_t bytes = (size_t)GF_MAX_RES * (size_t)n * sizeof(double2);
+    if(hipMalloc((void **)&wc.wave, bytes) != hipSuccess) {
+        wc.wave = nullptr;
+        wc.n    = 0;
+        return fail(GF_ERR_NOMEM, "cannot allocate a wave cache of %zu bytes", bytes);
+    }
+    GF_HIP(hipMemset(wc.wave, 0, bytes));
+    wc.n      = n;
+    wc.offset = offset;
+    return upload_dalitz_table(ctx);
+}
+
+gf_status gf_dalitz_cache(gf_ctx *ctx, int dalitz_func, uint64_t res_mask) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    int slot      = 0;
+    DalitzNode *d = find_dalitz(ctx, dalitz_func, &slot);
+    if(!d)
+        return fail(GF_ERR_INVALID, "function %d is not an Amp3Body node", dalitz_func);
+    auto it = ctx->caches.find(d->cache_key);
+    if(it == ctx->caches.end() || !it->second.wave)
+        return fail(GF_ERR_STATE, "no wave cache (gf_dalitz_set_data_size)");
+    const WaveCache &wc = it->second;
+    if(!ctx->col_base || wc.offset + wc.n > ctx->n)
+        return fail(GF_ERR_STATE, "wave cache rows %lld..%lld exceed the %lld uploaded events", (long long)wc.offset,
+                    (long long)(wc.offset + wc.n), (long long)ctx->n);
+    if(d->col_m12 >= ctx->ncols || d->col_m13 >= ctx->ncols)
+        return fail(GF_ERR_STATE, "m12/m13 columns not uploaded");
+    uint32_t mask = (uint32_t)(res_mask & ((d->nres >= 32) ? 0xffffffffu : ((1u << d->nres) - 1u)));
+    if(!mask)
+        return GF_OK;
+    gf_status st = flush_uploads(ctx);
+    if(st)
+        return st;
+    int64_t nb = (wc.n + 255) / 256;
+    if(nb > (int64_t)ctx->cu_count * 8)
+        nb = (int64_t)ctx->cu_count * 8;
+    GfProgArgs a = prog_args(ctx);
+    int rc = gfk::launch_dalitz_cache(a, slot, wc.offset, wc.n, mask, ctx->d_res + slot * GF_MAX_RES, d->nres,
+                                      (int)(nb < 1 ? 1 : nb), ctx->stream);
+    if(rc)
+        return fail(GF_ERR_HIP, "cache launch failed (%d)", rc);
+    GF_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->stage_in_flight = false;
+    return GF_OK;
+}
+
+gf_status gf_dalitz_gram(gf_ctx *ctx, int dalitz_func, const double grid6[6], int eff_func, double *out_re_im) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    int slot      = 0;
+    DalitzNode *d = find_dalitz(ctx, dalitz_func, &slot);
+    if(!d)
+        return fail(GF_ERR_INVALID, "function %d is not an Amp3Body node", dalitz_func);
+    if(!grid6 || !out_re_im)
+        return fail(GF_ERR_INVALID, "null grid or output");
+    double n12 = floor(grid6[2] + 0.5), n13 = floor(grid6[5] + 0.5);
+    if(!(n12 >= 1) || !(n13 >= 1) || n12 * n13 > 4.0e9)
+        return fail(GF_ERR_INVALID, "bad grid bin counts %g x %g", grid6[2], grid6[5]);
+    if(eff_func >= (int)ctx->funcs.size())
+        return fail(GF_ERR_INVALID, "efficiency function %d outside the tree", eff_func);
+    return run_gram(ctx, *d, slot, 0, grid6, nullptr, nullptr, nullptr, (int64_t)n12 * (int64_t)n13, eff_func,
+                    out_re_im);
+}
+
+gf_status gf_dalitz_gram_points_device(gf_ctx *ctx, int dalitz_func, const double *dev_m12, const double *dev_m13,
+                                       const double *dev_w, int64_t np, int eff_func, double *out_re_im) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    int slot      = 0;
+    DalitzNode *d = find_dalitz(ctx, dalitz_func, &slot);
+    if(!d)
+        return fail(GF_ERR_INVALID, "function %d is not an Amp3Body node", dalitz_func);
+    if(!dev_m12 || !dev_m13 || !out_re_im || np < 0)
+        return fail(GF_ERR_INVALID, "null point arrays or output");
+    if(eff_func >= (int)ctx->funcs.size())
+        return fail(GF_ERR_INVALID, "efficiency function %d outside the tree", eff_func);
+    return run_gram(ctx, *d, slot, 1, nullptr, dev_m12, dev_m13, dev_w, np, eff_func, out_re_im);
+}
+
+gf_status gf_dalitz_gram_points(gf_ctx *ctx, int dalitz_func, const double *m12, const double *m13, const double *w,
+                                int64_t np, int eff_func, double *out_re_im) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(!m12 || !m13 || np < 1)
+        return fail(GF_ERR_INVALID, "null or empty point arrays");
+    double *buf  = nullptr;
+    size_t bytes = (size_t)np * sizeof(double) * (w ? 3 : 2);
+    if(hipMalloc((void **)&buf, bytes) != hipSuccess)
+        return fail(GF_ERR_NOMEM, "cannot allocate %zu bytes of points", bytes);
+    gf_status st = GF_OK;
+    if(hipMemcpy(buf, m12, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(buf + np, m13, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
+       || (w && hipMemcpy(buf + 2 * np, w, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
+        st = fail(GF_ERR_HIP, "copy of the points failed");
+    if(!st)
+        st = gf_dalitz_gram_points_device(ctx, dalitz_func, buf, buf + np, w ? buf + 2 * np : nullptr, np, eff_func,
+                                          out_re_im);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(buf);
+    return st;
+}
+
+gf_status gf_dalitz_get_cache(gf_ctx *ctx, int dalitz_func, int res, double *out_re_im, int64_t n) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    DalitzNode *d = find_dalitz(ctx, dalitz_func, nullptr);
+    if(!d)
+        return fail(GF_ERR_INVALID, "function %d is not an Amp3Body node", dalitz_func);
+    auto it = ctx->caches.find(d->cache_key);
+    if(it == ctx->caches.end() || !it->second.wave)
+        return fail(GF_ERR_STATE, "no wave cache");
+    if(res < 0 || res >= d->nres || !out_re_im || n != it->second.n)
+        return fail(GF_ERR_INVALID, "resonance %d / size %lld do not match the cache (%d, %lld)", res, (long long)n,
+                    d->nres, (long long)it->second.n);
+    GF_HIP(hipMemcpyAsync(out_re_im, it->second.wave + (size_t)res * it->second.n, (size_t)n * sizeof(double2),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    GF_HIP(hipStreamSynchronize(ctx->stream));
+    return GF_OK;
+}
+
+gf_status gf_dalitz_cache_gram(gf_ctx *ctx, int dalitz_func, double *out_re_im) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    int slot      = 0;
+    DalitzNode *d = find_dalitz(ctx, dalitz_func, &slot);
+    if(!d)
+        return fail(GF_ERR_INVALID, "function %d is not an Amp3Body node", dalitz_func);
+    auto it = ctx->caches.find(d->cache_key);
+    if(it == ctx->caches.end() || !it->second.wave)
+        return fail(GF_ERR_STATE, "no wave cache");
+    if(!out_re_im)
+        return fail(GF_ERR_INVALID, "null output");
+    return run_gram(ctx, *d, slot, 2, nullptr, nullptr, nullptr, nullptr, it->second.n, -1, out_re_im);
+}
+
+// Internal hook (not in the public header): one FP64 MFMA on host operands.
+gf_status gf_internal_mfma_f64_probe(gf_ctx *ctx, const double *a64, const double *b64, double *out256) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(!a64 || !b64 || !out256)
+        return fail(GF_ERR_INVALID, "null operand");
+    return mfma_run(ctx, a64, b64, out256);
+}
+
+// ---------------------------------------------------------------- timing
+
+gf_status gf_time_reduce_metric(gf_ctx *ctx, int metric, int reps, double *ms_total) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    gf_status st = check_events_for_program(ctx, metric);
+    if(st)
+        return st;
+    if(reps < 1 || !ms_total)
+        return fail(GF_ERR_INVALID, "reps must be >= 1");
+    if((st = flush_uploads(ctx)))
+        return st;
+    GfProgArgs a = prog_args(ctx);
+    int nb       = event_blocks(ctx, ctx->n);
+    GF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    for(int r = 0; r < reps; r++) {
+        int rc = gfk::launch_reduce_metric(a, metric, ctx->d_partials, ctx->d_counter, ctx->d_result, nb, ctx->stream);
+        if(rc)
+            return fail(GF_ERR_HIP, "reduce launch failed (%d)", rc);
+    }
+    GF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    GF_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    GF_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *ms_total = ms;
+    return read_result(ctx, nullptr, nullptr);
+}
+
+gf_status gf_time_dalitz_cache(gf_ctx *ctx, int dalitz_func, int reps, double *ms_total) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(reps < 1 || !ms_total)
+        return fail(GF_ERR_INVALID, "reps must be >= 1");
+    int slot      = 0;
+    DalitzNode *d = find_dalitz(ctx, dalitz_func, &slot);
+    if(!d)
+        return fail(GF_ERR_INVALID, "function %d is not an Amp3Body node", dalitz_func);
+    auto it = ctx->caches.find(d->cache_key);
+    if(it == ctx->caches.end() || !it->second.wave || it->second.offset + it->second.n > ctx->n)
+        return fail(GF_ERR_STATE, "no wave cache over the uploaded events");
+    gf_status st = flush_uploads(ctx);
+    if(st)
+        return st;
+    const WaveCache &wc = it->second;
+    int64_t nb          = (wc.n + 255) / 256;
+    if(nb > (int64_t)ctx->cu_count * 8)
+        nb = (int64_t)ctx->cu_count * 8;
+    GfProgArgs a  = prog_args(ctx);
+    uint32_t mask = d->nres >= 32 ? 0xffffffffu : ((1u << d->nres) - 1u);
+    GF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    for(int r = 0; r < reps; r++) {
+        int rc = gfk::launch_dalitz_cache(a, slot, wc.offset, wc.n, mask, ctx->d_res + slot * GF_MAX_RES, d->nres,
+                                          (int)(nb < 1 ? 1 : nb), ctx->stream);
+        if(rc)
+            return fail(GF_ERR_HIP, "cache launch failed (%d)", rc);
+    }
+    GF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    GF_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    GF_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *ms_total            = ms;
+    ctx->stage_in_flight = false;
+    return GF_OK;
+}
+
+gf_status gf_time_dalitz_gram(gf_ctx *ctx, int dalitz_func, const double grid6[6], int eff_func, int reps,
+                              double *ms_total) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(reps < 1 || !ms_total || !grid6)
+        return fail(GF_ERR_INVALID, "reps must be >= 1");
+    std::vector<double> tmp(2 * GF_MAX_RES * GF_MAX_RES);
+    // first call compiles the efficiency program and probes the MFMA layout
+    gf_status st = gf_dalitz_gram(ctx, dalitz_func, grid6, eff_func, tmp.data());
+    if(st)
+        return st;
+    GF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    for(int r = 0; r < reps; r++) {
+        st = gf_dalitz_gram(ctx, dalitz_func, grid6, eff_func, tmp.data());
+        if(st)
+            return st;
+    }
+    GF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    GF_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    GF_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *ms_total = ms;
+    return GF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,324 @@
+// gf_device.h -- per-event FP64 arithmetic of the likelihood path, for gfx950.
+//
+// Each function states, in the same operation order, what the reference's
+// device function computes (file:line given per function), so that results
+// round the same way on both sides; the library is built with
+// -ffp-contract=off so no multiply-add is fused behind our back.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gf_internal.h"
+
+namespace gfd {
+
+// ---------------------------------------------------------------- leaf PDFs
+
+// device_Gaussian, src/PDFs/basic/GaussianPdf.cu:7-17
+__device__ __forceinline__ double gaussian(double x, double mean, double sigma) {
+    return exp(-0.5 * (x - mean) * (x - mean) / (sigma * sigma));
+}
+
+// device_ExpGaus, src/PDFs/basic/ExpGausPdf.cu:6-24 (sqrt 2 truncated to 1.4142135623 as there)
+__device__ __forceinline__ double expgaus(double x, double mean, double sigma, double alpha) {
+    double ret    = 0.5 * alpha;
+    double exparg = ret * (2 * mean + alpha * sigma * sigma - 2 * x);
+    double erfarg = (mean + alpha * sigma * sigma - x) / (sigma * 1.4142135623);
+    ret *= exp(exparg);
+    ret *= erfc(erfarg);
+    return ret;
+}
+
+// ------------------------------------------------------------ compensated sums
+
+// Neumaier's variant of Kahan summation: s carries the rounded sum, c the
+// accumulated rounding errors.
+__device__ __forceinline__ void kb_add(double &s, double &c, double v) {
+    double t = s + v;
+    double d = (fabs(s) >= fabs(v)) ? ((s - t) + v) : ((v - t) + s);
+    c += d;
+    s = t;
+}
+__device__ __forceinline__ void kb_merge(double &s, double &c, double s2, double c2) {
+    kb_add(s, c, s2);
+    c += c2;
+}
+
+// ------------------------------------------------------------------- metrics
+
+// src/PDFs/MetricPointer.cu:8-58.  evt0/evt1 are the (bin entry, bin volume or
+// error) of a binned event, unused for unbinned metrics.
+__device__ __forceinline__ double metric(int m, double rawPdf, double evt0, double evt1, double norm,
+                                         double totalEvents) {
+    switch(m) {
+    case 0: return rawPdf;
+    case 1: rawPdf *= norm; return rawPdf > 0.0 ? -log(rawPdf) : 0.0;
+    case 2: return rawPdf * norm;
+    case 3: {
+        rawPdf *= norm;
+        rawPdf *= evt1;
+        if(rawPdf > 0) {
+            double expEvents = totalEvents * rawPdf;
+            return (expEvents - evt0 * log(expEvents));
+        }
+        return 0;
+    }
+    case 4:
+        rawPdf -= evt0;
+        rawPdf /= evt1;
+        rawPdf *= rawPdf;
+        return rawPdf;
+    case 5: {
+        rawPdf *= norm;
+        rawPdf *= evt1;
+        double d = rawPdf * totalEvents - evt0;
+        return d * d / (evt0 > 1 ? evt0 : 1);
+    }
+    default: return __builtin_nan("");
+    }
+}
+
+// --------------------------------------------------------- Dalitz kinematics
+
+// inDalitz, src/PDFs/physics/DalitzPlotHelpers.cu:14-59
+__device__ __forceinline__ bool in_dalitz(double m12, double m13, double bigM, double dm1, double dm2, double dm3) {
+    double dm1pdm2  = dm1 + dm2;
+    double bigMmdm3 = bigM - dm3;
+    double bigMmdm1 = bigM - dm1;
+    double m23      = bigM * bigM + dm1 * dm1 + dm2 * dm2 + dm3 * dm3 - m12 - m13;
+    if(m23 < 0. || m23 > bigMmdm1 * bigMmdm1)
+        return false;
+    bool m12less     = (m12 < dm1pdm2 * dm1pdm2) ? false : true;
+    bool m12grea     = (m12 > bigMmdm3 * bigMmdm3) ? false : true;
+    double sqrtM12   = sqrt(m12);
+    double dm11      = dm1 * dm1;
+    double dm22      = dm2 * dm2;
+    double dm33      = dm3 * dm3;
+    double e1star    = 0.5 * (m12 - dm22 + dm11) / sqrtM12;
+    double e3star    = 0.5 * (bigM * bigM - m12 - dm33) / sqrtM12;
+    double rte1mdm11 = sqrt(e1star * e1star - dm11);
+    double rte3mdm33 = sqrt(e3star * e3star - dm33);
+    double minimum   = (e1star + e3star) * (e1star + e3star) - (rte1mdm11 + rte3mdm33) * (rte1mdm11 + rte3mdm33);
+    bool m13less     = (m13 < minimum) ? false : true;
+    double maximum   = (e1star + e3star) * (e1star + e3star) - (rte1mdm11 - rte3mdm33) * (rte1mdm11 - rte3mdm33);
+    bool m13grea     = (m13 > maximum) ? false : true;
+    return m12less && m12grea && m13less && m13grea;
+}
+
+// twoBodyCMmom, src/PDFs/physics/resonances/Resonance.cu:48-60
+__device__ __forceinline__ double two_body_cm_mom(double rMassSq, double d1m, double d2m) {
+    double kin1 = 1 - (d1m + d2m) * (d1m + d2m) / rMassSq;
+    kin1        = kin1 >= 0 ? sqrt(kin1) : 1;
+    double kin2 = 1 - (d1m - d2m) * (d1m - d2m) / rMassSq;
+    kin2        = kin2 >= 0 ? sqrt(kin2) : 1;
+    return 0.5 * sqrt(rMassSq) * kin1 * kin2;
+}
+
+// twoBodyCMMothermom, Resonance.cu:62-75
+__device__ __forceinline__ double two_body_cm_mother_mom(double rMassSq, double dm, double d3m) {
+    double kin1 = 1 - (dm + d3m) * (dm + d3m) / rMassSq;
+    kin1        = kin1 >= 0 ? sqrt(kin1) : 1;
+    double kin2 = 1 - (dm - d3m) * (dm - d3m) / rMassSq;
+    kin2        = kin2 >= 0 ? sqrt(kin2) : 1;
+    return 0.5 * rMassSq * kin1 * kin2 / dm;
+}
+
+// dampingFactorSquare, Resonance.cu:77-86 (spin-2 form as written there)
+__device__ __forceinline__ double damping_factor_square(double cmmom, int spin, double mRadius) {
+    double square  = mRadius * mRadius * cmmom * cmmom;
+    double dfsq    = 2 * square;
+    double dfsqres = 13 * square / pow(square - 3, 2.0) + 9 * square * square;
+    return (spin == 2) ? dfsqres : dfsq;
+}
+
+// dampingFactorSquareNorm, Resonance.cu:88-97
+__device__ __forceinline__ double damping_factor_square_norm(double cmmom, int spin, double mRadius) {
+    double square  = mRadius * mRadius * cmmom * cmmom;
+    double dfsq    = 1 + square;
+    double dfsqres = dfsq + 8 + 2 * square + square * square;
+    return (spin == 2) ? dfsqres : dfsq;
+}
+
+// spinFactor, Resonance.cu:99-155 (PAIR_12 = 0, PAIR_13 = 1, PAIR_23 = 2)
+__device__ __forceinline__ double spin_factor(unsigned spin, const GfDecay &d, double m12, double m13, double m23,
+                                              unsigned cyc) {
+    if(0 == spin)
+        return 1;
+    double _mA  = (0 == cyc ? d.daug1Mass : (1 == cyc ? d.daug3Mass : d.daug2Mass));
+    double _mB  = (0 == cyc ? d.daug2Mass : (1 == cyc ? d.daug1Mass : d.daug3Mass));
+    double _mC  = (0 == cyc ? d.daug3Mass : (1 == cyc ? d.daug2Mass : d.daug1Mass));
+    double _mAC = (0 == cyc ? m13 : (1 == cyc ? m23 : m12));
+    double _mBC = (0 == cyc ? m23 : (1 == cyc ? m12 : m13));
+    double _mAB = (0 == cyc ? m12 : (1 == cyc ? m13 : m23));
+    double massFactor = 1.0 / (_mAB);
+    double sFactor    = -1;
+    sFactor *= ((_mBC - _mAC) + (massFactor * (d.motherMass * d.motherMass - _mC * _mC) * (_mA * _mA - _mB * _mB)));
+    if(2 == spin) {
+        sFactor *= sFactor;
+        double t1        = d.motherMass * d.motherMass - _mC * _mC;
+        double extraterm = ((_mAB - (2 * d.motherMass * d.motherMass) - (2 * _mC * _mC)) + massFactor * (t1 * t1));
+        double t2        = _mA * _mA - _mB * _mB;
+        extraterm *= ((_mAB - (2 * _mA * _mA) - (2 * _mB * _mB)) + massFactor * (t2 * t2));
+        extraterm /= 3;
+        sFactor -= extraterm;
+    }
+    return sFactor;
+}
+
+// plainBW<I>, src/PDFs/physics/resonances/RBW.cu:9-86
+template <int I>
+__device__ __forceinline__ double2 rbw(const GfDecay &d, double resmass, double reswidth, unsigned spin,
+                                       unsigned cyclic_index, bool norm, double m12, double m13, double m23) {
+    double2 result  = make_double2(0.0, 0.0);
+    double resmass2 = resmass * resmass;
+#pragma unroll
+    for(int i = 0; i < I; i++) {
+        double rMassSq      = (0 == cyclic_index ? m12 : (1 == cyclic_index ? m13 : m23));
+        double mass_daug1   = 2 == cyclic_index ? d.daug2Mass : d.daug1Mass;
+        double mass_daug2   = 0 == cyclic_index ? d.daug2Mass : d.daug3Mass;
+        double bachelorMass = (0 == cyclic_index ? d.daug3Mass : (1 == cyclic_index ? d.daug2Mass : d.daug1Mass));
+        double frFactor       = 1;
+        double frFactorMother = 1;
+        double measureDaughterMoms = two_body_cm_mom(rMassSq, mass_daug1, mass_daug2);
+        double nominalDaughterMoms = two_body_cm_mom(resmass2, mass_daug1, mass_daug2);
+        double measureDaughterMomsMother;
+        double nominalDaughterMomsMother = 0;
+        if(norm) {
+            measureDaughterMomsMother = two_body_cm_mother_mom(rMassSq, d.motherMass, bachelorMass);
+            nominalDaughterMomsMother = two_body_cm_mother_mom(resmass2, d.motherMass, bachelorMass);
+        } else {
+            measureDaughterMomsMother = two_body_cm_mom(d.motherMass * d.motherMass, sqrt(rMassSq), bachelorMass);
+        }
+        if(0 != spin) {
+            if(norm) {
+                frFactor = damping_factor_square_norm(nominalDaughterMoms, spin, d.meson_radius)
+                           / damping_factor_square_norm(measureDaughterMoms, spin, d.meson_radius);
+                frFactorMother = damping_factor_square_norm(nominalDaughterMomsMother, spin, d.mother_meson_radius)
+                                 / damping_factor_square_norm(measureDaughterMomsMother, spin, d.mother_meson_radius);
+            } else {
+                frFactor       = damping_factor_square(measureDaughterMoms, spin, d.meson_radius);
+                frFactorMother = damping_factor_square(measureDaughterMomsMother, spin, d.mother_meson_radius);
+            }
+        }
+        double A  = (resmass2 - rMassSq);
+        double B  = resmass2 * reswidth * pow(measureDaughterMoms / nominalDaughterMoms, 2.0 * spin + 1) * frFactor
+                   / sqrt(rMassSq);
+        double C  = 1.0 / (A * A + B * B);
+        double re = A * C, im = B * C;
+        double s1 = sqrt(frFactor);
+        re *= s1;
+        im *= s1;
+        double s2 = sqrt(frFactorMother);
+        re *= s2;
+        im *= s2;
+        double sf = spin_factor(spin, d, m12, m13, m23, cyclic_index);
+        re *= sf;
+        im *= sf;
+        result.x += re;
+        result.y += im;
+        if(I > 1)
+            cyclic_index = (cyclic_index + 1) % 3;
+    }
+    return result;
+}
+
+// hFun, dh_dsFun, fsFun, dFun: Resonance.cu:5-46 (pi truncated to 3.14159265359 as there)
+__device__ __forceinline__ double h_fun(double s, double daug2Mass, double daug3Mass) {
+    const double _pi = 3.14159265359;
+    double sm        = daug2Mass + daug3Mass;
+    double sqrt_s    = sqrt(s);
+    double k_s       = two_body_cm_mom(s, daug2Mass, daug3Mass);
+    return ((2 / _pi) * (k_s / sqrt_s) * log((sqrt_s + 2 * k_s) / (sm)));
+}
+__device__ __forceinline__ double dh_ds_fun(double s, double daug2Mass, double daug3Mass) {
+    const double _pi = 3.14159265359;
+    double k_s       = two_body_cm_mom(s, daug2Mass, daug3Mass);
+    return h_fun(s, daug2Mass, daug3Mass) * (1.0 / (8.0 * (k_s * k_s)) - 1.0 / (2.0 * s)) + 1.0 / (2.0 * _pi * s);
+}
+__device__ __forceinline__ double fs_fun(double s, double m2, double gam, double daug2Mass, double daug3Mass) {
+    double k_s   = two_body_cm_mom(s, daug2Mass, daug3Mass);
+    double k_Am2 = two_body_cm_mom(m2, daug2Mass, daug3Mass);
+    double f     = gam * m2 / (k_Am2 * k_Am2 * k_Am2);
+    f *= ((k_s * k_s) * (h_fun(s, daug2Mass, daug3Mass) - h_fun(m2, daug2Mass, daug3Mass))
+          + (m2 - s) * (k_Am2 * k_Am2) * dh_ds_fun(m2, daug2Mass, daug3Mass));
+    return f;
+}
+__device__ __forceinline__ double d_fun(double s, double daug2Mass, double daug3Mass) {
+    const double _pi = 3.14159265359;
+    double sm        = daug2Mass + daug3Mass;
+    double sm24      = sm * sm / 4.0;
+    double m         = sqrt(s);
+    double k_m2      = two_body_cm_mom(s, daug2Mass, daug3Mass);
+    return 3.0 / _pi * sm24 / (k_m2 * k_m2) * log((m + 2 * k_m2) / sm) + m / (2 * _pi * k_m2)
+           - sm24 * m / (_pi * (k_m2 * k_m2 * k_m2));
+}
+
+// gouSak, src/PDFs/physics/resonances/GS.cu:9-75 (dFun/fsFun take daug2/daug3 masses in every channel, as there)
+__device__ __forceinline__ double2 gounaris_sakurai(const GfDecay &d, double resmass, double reswidth, unsigned spin,
+                                                   unsigned cyclic_index, bool norm, double m12, double m13,
+                                                   double m23) {
+    double rMassSq      = (0 == cyclic_index ? m12 : (1 == cyclic_index ? m13 : m23));
+    double bachelorMass = (0 == cyclic_index ? d.daug3Mass : (1 == cyclic_index ? d.daug2Mass : d.daug1Mass));
+    double frFactor     = 1;
+    double frFactorD    = 1;
+    resmass *= resmass;
+    double md1 = (2 == cyclic_index ? d.daug2Mass : d.daug1Mass);
+    double md2 = (0 == cyclic_index ? d.daug2Mass : d.daug3Mass);
+    double measureDaughterMoms = two_body_cm_mom(rMassSq, md1, md2);
+    double nominalDaughterMoms = two_body_cm_mom(resmass, md1, md2);
+    double measureDaughterMomsMother;
+    double nominalDaughterMomsMother = 0;
+    if(norm) {
+        measureDaughterMomsMother = two_body_cm_mother_mom(rMassSq, d.motherMass, bachelorMass);
+        nominalDaughterMomsMother = two_body_cm_mother_mom(resmass, d.motherMass, bachelorMass);
+    } else {
+        measureDaughterMomsMother = two_body_cm_mom(d.motherMass * d.motherMass, sqrt(rMassSq), bachelorMass);
+    }
+    if(0 != spin) {
+        if(norm) {
+            frFactor = damping_factor_square_norm(nominalDaughterMoms, spin, d.meson_radius);
+            frFactor /= damping_factor_square_norm(measureDaughterMoms, spin, d.meson_radius);
+            frFactorD = damping_factor_square_norm(nominalDaughterMomsMother, spin, d.mother_meson_radius);
+            frFactorD /= damping_factor_square_norm(measureDaughterMomsMother, spin, d.mother_meson_radius);
+        } else {
+            frFactor  = damping_factor_square(measureDaughterMoms, spin, d.meson_radius);
+            frFactorD = damping_factor_square(measureDaughterMomsMother, spin, d.mother_meson_radius);
+        }
+    }
+    double D = (1.0 + d_fun(resmass, d.daug2Mass, d.daug3Mass) * reswidth / sqrt(resmass));
+    double E = resmass - rMassSq + fs_fun(rMassSq, resmass, reswidth, d.daug2Mass, d.daug3Mass);
+    double F = sqrt(resmass) * reswidth * pow(measureDaughterMoms / nominalDaughterMoms, 2.0 * spin + 1) * frFactor;
+    D /= (E * E + F * F);
+    double re = D * E, im = D * F;
+    double s1 = sqrt(frFactor);
+    re *= s1;
+    im *= s1;
+    double s2 = sqrt(frFactorD);
+    re *= s2;
+    im *= s2;
+    double sf = spin_factor(spin, d, m12, m13, m23, cyclic_index);
+    re *= sf;
+    im *= sf;
+    return make_double2(re, im);
+}
+
+// One resonance of a Dalitz node: getResonanceAmplitude (Amp3Body_TD.cu:36-39)
+// dispatched on the wave-uniform kind instead of a function pointer.
+__device__ __forceinline__ double2 lineshape(const GfResDesc &r, const double *__restrict__ P,
+                                             const double *__restrict__ Cn, const GfDecay &d, double m12,
+                                             double m13, double m23) {
+    if(r.kind == GR_NONRES)
+        return make_double2(1.0, 0.0);  // nonres, NonRes.cu:8-11
+    double mass   = P[r.p];
+    double width  = P[r.p + 1];
+    unsigned spin = (unsigned)Cn[r.c];
+    unsigned cyc  = (unsigned)Cn[r.c + 1];
+    bool norm     = Cn[r.c + 2] != 0.0;
+    switch(r.kind) {
+    case GR_RBW: return rbw<1>(d, mass, width, spin, cyc, norm, m12, m13, m23);
+    case GR_RBW_SYM: return rbw<2>(d, mass, width, spin, cyc, norm, m12, m13, m23);
+    default: return gounaris_sakurai(d, mass, width, spin, cyc, norm, m12, m13, m23);
+    }
+}
+
+}  // namespace gfd

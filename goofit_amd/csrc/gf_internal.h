@@ -1,0 +1,126 @@
+// gf_internal.h -- structures shared by the C-ABI host code and the gfx950 kernels.
+//
+// A flattened PDF tree (the reference's four arrays + function table,
+// src/PDFs/PdfBase.cu:268-311) is compiled on the host into a linear program
+// of GfIns instructions.  The kernels interpret it with wave-uniform control
+// flow: every lane of a wavefront runs the same instruction on its own events,
+// so the instruction fetch and dispatch live in scalar registers and only the
+// arithmetic is per lane.  Tree recursion becomes a small stack of per-level
+// accumulators (acc[level]); a leaf leaves its value in T.
+#pragma once
+
+#include <stdint.h>
+
+#define GF_MAX_OBS 8      // observable columns per event
+#define GF_MAX_LVL 4      // nesting depth of combining nodes
+#define GF_MAX_RES 16     // resonances per Dalitz node (Amp3Body.h:88 caps it at 16 as well)
+#define GF_MAX_DALITZ 4   // Dalitz nodes per program
+#define GF_BLOCK 256      // threads per block of the event kernels
+#define GF_MAX_PARTIALS 4096
+
+// Instruction codes of the compiled program.  Leaf codes compute T; the
+// combining steps act on the accumulator of one nesting level and are encoded
+// as GI_LVL + 4*kind + level, so each (kind, level) pair is its own case with
+// the accumulator fixed at compile time (no register array indexed at run time).
+enum GfOp : int32_t {
+    GI_END = 0,
+    GI_GAUSS,          // T = exp(-0.5*(x-m)*(x-m)/(s*s))
+    GI_EXP,            // T = exp(a*x)
+    GI_EXP_OFF,        // T = exp(a*(x-x0))
+    GI_EXP_POLY,       // T = exp(sum_i x^i p_i)
+    GI_EXP_POLY_OFF,   // T = exp(sum_{i>=1} (x-x0)^i p_i)
+    GI_EXPGAUS,        // exponentially modified Gaussian
+    GI_POLY,           // sum_i p_i x^(d+i)
+    GI_OFF_POLY,       // sum_{i>=2} p_i x^(d+i-2)
+    GI_MULTI_POLY,     // multivariate polynomial
+    GI_LVL = 16
+};
+enum GfLvlKind : int32_t {
+    GK_PUSH0 = 0,      // acc = 0
+    GK_PUSH1,          // acc = 1
+    GK_ADD_ACC,        // acc += w * T * norm
+    GK_ADD_LAST,       // acc += (1 - sum w) * T * norm
+    GK_ADDEXT_END,     // T = acc / sum w
+    GK_PROD_ACC,       // acc *= T * norm
+    GK_POP,            // T = acc
+    GK_DALITZ,         // acc = inside ? |sum_i c_i W_i|^2 : -1
+    GK_DALITZ_END,     // T = (acc == -1) ? 0 : acc * T
+    GK_COUNT
+};
+#define GF_LVL_OP(kind, lvl) (GI_LVL + 4 * (kind) + (lvl))
+
+struct GfIns {
+    int32_t op;   // GfOp
+    int32_t lvl;  // accumulator level
+    int32_t a;    // observable column of a leaf; resonance count of GI_DALITZ
+    int32_t p;    // index in params[] of the first parameter value
+    int32_t c;    // index in consts[] of the first constant value
+    int32_t n;    // index in norms[] of the normalisation value used by a combine step
+    int32_t k;    // count (number of parameters / weights / observables)
+    int32_t x;    // GI_MULTI_POLY: index in obsid[] of its first column id; GI_DALITZ: node slot
+};
+
+// Lineshape kinds of the resonances of a Dalitz node.
+enum GfRes : int32_t { GR_RBW = 0, GR_RBW_SYM = 1, GR_GS = 2, GR_NONRES = 3 };
+
+struct GfResDesc {
+    int32_t kind;  // GfRes
+    int32_t p;     // index in params[] of mass (width at p+1)
+    int32_t c;     // index in consts[] of spin (cyclic index at c+1, norm flag at c+2)
+    int32_t pad;
+};
+
+struct GfDecay {
+    double motherMass, daug1Mass, daug2Mass, daug3Mass, meson_radius, mother_meson_radius;
+};
+
+// Per-Dalitz-node data the coherent sum needs.
+struct GfDalitzSlot {
+    const double2 *wave;  // nres x ncache complex, resonance-major (W[i*ncache + e])
+    int64_t ncache;
+    int32_t col_m12, col_m13, col_num;
+    int32_t nres;
+};
+
+// Everything an event-evaluating kernel needs, passed by value as the kernel
+// argument.  Arrays indexed by a program operand (column pointers, Dalitz
+// slots) live in device memory so that a wave-uniform index becomes a scalar
+// load instead of a dynamically indexed copy of the argument block.
+struct GfProgArgs {
+    const GfIns *ins;            // program (device)
+    const double *params;        // device copies of the flattened arrays
+    const double *consts;
+    const double *norms;
+    const int32_t *obsid;        // observable ids of the obs[] array as ints (device)
+    const double *const *cols;   // [GF_MAX_OBS] device column pointers
+    const GfDalitzSlot *dz;      // [GF_MAX_DALITZ] Dalitz node data
+    int64_t nevents;
+    int32_t identity_num;        // 1: event-number column equals the row index
+    int32_t nobs;
+    double total_events;         // c_totalEvents
+    GfDecay decay;
+    int32_t *err;                // device error word (bit 0: cache index out of range)
+};
+
+#define GF_METRIC_BINAVG_CODE 3  // metrics from here on read (bin entry, bin volume/error) columns
+
+// Host-side launchers (gf_kernels.hip).
+namespace gfk {
+struct Partial {
+    double s, c;
+};
+int launch_reduce_metric(const GfProgArgs &a, int metric, Partial *partials, unsigned *counter, double *result,
+                         int nblocks, void *stream);
+int launch_eval_metric(const GfProgArgs &a, int metric, double *out, int nblocks, void *stream);
+int launch_reduce_bins(const GfProgArgs &a, int pc0, const double *ranges, const int *slots, int k, int64_t total,
+                       Partial *partials, unsigned *counter, double *result, int nblocks, void *stream);
+int launch_dalitz_cache(const GfProgArgs &a, int slot, int64_t offset, int64_t n, uint32_t mask,
+                        const GfResDesc *res, int nres, int nblocks, void *stream);
+// mode 0: grid cells (grid6 host array); 1: points (m12, m13, w device arrays); 2: data cache of `slot`.
+// Writes 768 doubles (tiles RR, II, RI in MFMA register layout) to tiles_dev.
+int launch_gram(const GfProgArgs &a, int mode, const double *grid6, const double *m12, const double *m13,
+                const double *w, int64_t npoints, int slot, int id_m12, int id_m13, int eff_pc,
+                const GfResDesc *res, int nres, double *partials, double *tiles_dev, int nblocks, void *stream);
+int launch_identity_check(const double *col, int64_t n, int32_t *flag, void *stream);
+int mfma_probe(const double *dev_a, const double *dev_b, double *dev_out, void *stream);
+}  // namespace gfk

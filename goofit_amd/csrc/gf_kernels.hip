@@ -1,0 +1,801 @@
+// gf_kernels.hip -- gfx950 kernels of the likelihood path.
+//
+//  * k_reduce_metric / k_eval_metric: the fused per-event evaluation of a
+//    compiled PDF tree over SoA event columns, the metric (NLL, Prob, ...),
+//    and a compensated FP64 sum: Neumaier per thread, wave shuffles, LDS
+//    across waves, one partial per block and a last-block pass that adds the
+//    partials in a fixed order (deterministic for a given grid).  Replaces the
+//    thrust::transform_reduce / transform over MetricTaker
+//    (src/PDFs/GooPdf.cu:45-78, 112-173; src/PDFs/MetricTaker.cu:17-44).
+//  * k_reduce_bins: the midpoint-grid integral of one function
+//    (src/PDFs/GooPdf.cu:83-108, src/PDFs/MetricTaker.cu:48-96).
+//  * k_dalitz_cache: the resonance lineshape cache W_i[e]
+//    (src/PDFs/physics/Amp3Body.cu:292-310, detail/SpecialResonanceCalculator.cu:12-46).
+//  * k_gram: the per-amplitude normalisation integrals as one FP64-MFMA Gram
+//    A^H diag(eff*w) A over grid cells, phase-space points or the data cache
+//    (src/PDFs/physics/Amp3Body.cu:312-330 and 376-428,
+//    detail/SpecialResonanceIntegrator.cu:7-121).
+#include <hip/hip_runtime.h>
+
+#include "gf_device.h"
+#include "gf_internal.h"
+
+namespace {
+
+using gfk::Partial;
+
+// ------------------------------------------------------------ event sources
+
+// Events read from the SoA columns in HBM.
+template <int EPT>
+struct ColSrc {
+    const double *const *cols;
+    int64_t idx[EPT];
+    __device__ __forceinline__ double get(int a, int j) const { return cols[a][idx[j]]; }
+};
+
+// Synthetic events held in registers (grid points); slot chosen by a
+// wave-uniform switch so no register array is indexed dynamically.
+template <int EPT>
+struct RegSrc {
+    double v[EPT][GF_MAX_OBS];
+    __device__ __forceinline__ double get(int a, int j) const {
+        switch(a) {
+        case 0: return v[j][0];
+        case 1: return v[j][1];
+        case 2: return v[j][2];
+        case 3: return v[j][3];
+        case 4: return v[j][4];
+        case 5: return v[j][5];
+        case 6: return v[j][6];
+        default: return v[j][7];
+        }
+    }
+    __device__ __forceinline__ void set(int a, int j, double x) {
+        switch(a) {
+        case 0: v[j][0] = x; break;
+        case 1: v[j][1] = x; break;
+        case 2: v[j][2] = x; break;
+        case 3: v[j][3] = x; break;
+        case 4: v[j][4] = x; break;
+        case 5: v[j][5] = x; break;
+        case 6: v[j][6] = x; break;
+        default: v[j][7] = x; break;
+        }
+    }
+};
+
+// Per-level accumulators of the combining nodes.
+template <int EPT>
+struct Acc {
+    double a0[EPT], a1[EPT], a2[EPT], a3[EPT];
+};
+template <int L, int EPT>
+__device__ __forceinline__ double (&level(Acc<EPT> &a))[EPT] {
+    if constexpr(L == 0)
+        return a.a0;
+    else if constexpr(L == 1)
+        return a.a1;
+    else if constexpr(L == 2)
+        return a.a2;
+    else
+        return a.a3;
+}
+
+// ------------------------------------------------------------ interpreter
+
+// One combining step on the accumulator of level L.
+template <int L, int EPT, class Src>
+__device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs &A, const Src &src,
+                                     const int64_t (&row)[EPT], const bool (&valid)[EPT], Acc<EPT> &acc,
+                                     double (&T)[EPT]) {
+    const double *__restrict__ P  = A.params;
+    const double *__restrict__ Nm = A.norms;
+    double(&a)[EPT]               = level<L>(acc);
+    switch(kind) {
+    case GK_PUSH0:
+#pragma unroll
+        for(int j = 0; j < EPT; ++j)
+            a[j] = 0.0;
+        break;
+    case GK_PUSH1:
+#pragma unroll
+        for(int j = 0; j < EPT; ++j)
+            a[j] = 1.0;
+        break;
+    case GK_ADD_ACC: {
+        // device_AddPdfs / device_AddPdfsExt: ret += weight * curr * norm
+        const double w = P[in.p], nm = Nm[in.n];
+#pragma unroll
+        for(int j = 0; j < EPT; ++j)
+            a[j] += w * T[j] * nm;
+    } break;
+    case GK_ADD_LAST: {
+        // AddPdf.cu:43-48: ret += (1 - totalWeight) * last * normFactor
+        double tw = 0;
+        for(int i = 0; i < in.k; ++i)
+            tw += P[in.p + i];
+        const double nm = Nm[in.n];
+#pragma unroll
+        for(int j = 0; j < EPT; ++j)
+            a[j] += (1 - tw) * T[j] * nm;
+    } break;
+    case GK_ADDEXT_END: {
+        // AddPdf.cu:76: ret /= totalWeight
+        double tw = 0;
+        for(int i = 0; i < in.k; ++i)
+            tw += P[in.p + i];
+#pragma unroll
+        for(int j = 0; j < EPT; ++j)
+            T[j] = a[j] / tw;
+    } break;
+    case GK_PROD_ACC: {
+        // ProdPdf.cu:19-23: curr *= norm; ret *= curr
+        const double nm = Nm[in.n];
+#pragma unroll
+        for(int j = 0; j < EPT; ++j) {
+            double cur = T[j] * nm;
+            a[j] *= cur;
+        }
+    } break;
+    case GK_POP:
+#pragma unroll
+        for(int j = 0; j < EPT; ++j)
+            T[j] = a[j];
+        break;
+    case GK_DALITZ: {
+        // device_DalitzPlot, src/PDFs/physics/Amp3Body.cu:60-122
+        const GfDalitzSlot dz = A.dz[in.x];
+        const GfDecay &D      = A.decay;
+        int64_t en[EPT];
+        bool inside[EPT];
+        double re[EPT], im[EPT];
+#pragma unroll
+        for(int j = 0; j < EPT; ++j) {
+            const double m12 = src.get(dz.col_m12, j), m13 = src.get(dz.col_m13, j);
+            inside[j] = gfd::in_dalitz(m12, m13, D.motherMass, D.daug1Mass, D.daug2Mass, D.daug3Mass);
+            int64_t e = A.identity_num ? row[j] : (int64_t)floor(0.5 + src.get(dz.col_num, j));
+            if(e < 0 || e >= dz.ncache) {
+                if(inside[j] && valid[j])
+                    atomicOr(A.err, 1);
+                e = 0;
+            }
+            en[j] = e;
+            re[j] = 0.0;
+            im[j] = 0.0;
+        }
+        for(int i = 0; i < in.a; ++i) {
+            const double ar = P[in.p + 2 * i], ai = P[in.p + 2 * i + 1];
+            const double2 *__restrict__ w = dz.wave + (int64_t)i * dz.ncache;
+#pragma unroll
+            for(int j = 0; j < EPT; ++j) {
+                const double2 me = w[en[j]];
+                re[j] += ar * me.x - ai * me.y;
+                im[j] += ar * me.y + ai * me.x;
+            }
+        }
+#pragma unroll
+        for(int j = 0; j < EPT; ++j)
+            a[j] = inside[j] ? re[j] * re[j] + im[j] * im[j] : -1.0;
+    } break;
+    case GK_DALITZ_END:
+        // Amp3Body.cu:118-121: ret *= eff (0 for events outside the Dalitz region)
+#pragma unroll
+        for(int j = 0; j < EPT; ++j)
+            T[j] = (a[j] == -1.0) ? 0.0 : a[j] * T[j];
+        break;
+    default: break;
+    }
+}
+
+
+// Run the compiled program from instruction pc0 for EPT events per lane; the
+// value of the tree ends up in T.  row[] are the event rows (used as wave-cache
+// index when the event-number column is the identity), valid[] masks tail lanes.
+template <int EPT, class Src>
+__device__ __forceinline__ void run(const GfProgArgs &A, int pc0, const Src &src, const int64_t (&row)[EPT],
+                                    const bool (&valid)[EPT], double (&T)[EPT]) {
+    const double *__restrict__ P  = A.params;
+    const double *__restrict__ Cn = A.consts;
+    Acc<EPT> acc;
+#pragma unroll
+    for(int j = 0; j < EPT; ++j) {
+        acc.a0[j] = acc.a1[j] = acc.a2[j] = acc.a3[j] = 0.0;
+        T[j]                                          = 0.0;
+    }
+    for(int pc = pc0;; ++pc) {
+        const GfIns in = A.ins[pc];
+        switch(in.op) {
+        case GI_END: return;
+        case GI_GAUSS: {
+            const double m = P[in.p], s = P[in.p + 1];
+#pragma unroll
+            for(int j = 0; j < EPT; ++j)
+                T[j] = gfd::gaussian(src.get(in.a, j), m, s);
+        } break;
+        case GI_EXP: {
+            const double al = P[in.p];
+#pragma unroll
+            for(int j = 0; j < EPT; ++j)
+                T[j] = exp(al * src.get(in.a, j));
+        } break;
+        case GI_EXP_OFF: {
+            const double x0 = P[in.p], al = P[in.p + 1];
+#pragma unroll
+            for(int j = 0; j < EPT; ++j) {
+                double x = src.get(in.a, j);
+                x -= x0;
+                T[j] = exp(al * x);
+            }
+        } break;
+        case GI_EXP_POLY: {
+#pragma unroll
+            for(int j = 0; j < EPT; ++j) {
+                double x = src.get(in.a, j), e = 0;
+                for(int i = 0; i < in.k; ++i)
+                    e += pow(x, (double)i) * P[in.p + i];
+                T[j] = exp(e);
+            }
+        } break;
+        case GI_EXP_POLY_OFF: {
+#pragma unroll
+            for(int j = 0; j < EPT; ++j) {
+                double x = src.get(in.a, j), e = 0;
+                x -= P[in.p];
+                for(int i = 1; i < in.k; ++i)
+                    e += pow(x, (double)i) * P[in.p + i];
+                T[j] = exp(e);
+            }
+        } break;
+        case GI_EXPGAUS: {
+            const double m = P[in.p], s = P[in.p + 1], al = P[in.p + 2];
+#pragma unroll
+            for(int j = 0; j < EPT; ++j)
+                T[j] = gfd::expgaus(src.get(in.a, j), m, s, al);
+        } break;
+        case GI_POLY: {
+            const int lowest = (int)Cn[in.c];
+#pragma unroll
+            for(int j = 0; j < EPT; ++j) {
+                double x = src.get(in.a, j), r = 0;
+                for(int i = 0; i < in.k; ++i)
+                    r += P[in.p + i] * pow(x, (double)(lowest + i));
+                T[j] = r;
+            }
+        } break;
+        case GI_OFF_POLY: {
+            const int lowest = (int)Cn[in.c];
+#pragma unroll
+            for(int j = 0; j < EPT; ++j) {
+                double x = src.get(in.a, j), r = 0;
+                for(int i = 2; i < in.k; ++i)
+                    r += P[in.p + i] * pow(x, (double)(lowest + i - 2));
+                T[j] = r;
+            }
+        } break;
+        case GI_MULTI_POLY: {
+            // device_MultiPolynomial, src/PDFs/basic/PolynomialPdf.cu:47-118
+            const int nobs      = in.a;
+            const int maxDegree = (int)Cn[in.c] + 1;
+            int numBoxes        = 1;
+            for(int i = 0; i < nobs; ++i)
+                numBoxes *= maxDegree;
+#pragma unroll
+            for(int j = 0; j < EPT; ++j) {
+                int coeff = nobs;
+                double r  = P[in.p + coeff];
+                coeff++;
+                for(int i = 1; i < numBoxes; ++i) {
+                    double term = 1;
+                    int ci = i, sum = 0;
+                    for(int o = 0; o < nobs; ++o) {
+                        double x = src.get(A.obsid[in.x + o], j);
+                        x -= P[in.p + o];
+                        int pw = ci % maxDegree;
+                        ci /= maxDegree;
+                        term *= pow(x, (double)pw);
+                        sum += pw;
+                    }
+                    if(sum >= maxDegree)
+                        continue;
+                    term *= P[in.p + coeff];
+                    coeff++;
+                    r += term;
+                }
+                T[j] = r;
+            }
+        } break;
+        default:
+            if(in.op >= GI_LVL) {
+                const int kind = (in.op - GI_LVL) >> 2;
+                switch((in.op - GI_LVL) & 3) {
+                case 0: step<0>(kind, in, A, src, row, valid, acc, T); break;
+                case 1: step<1>(kind, in, A, src, row, valid, acc, T); break;
+                case 2: step<2>(kind, in, A, src, row, valid, acc, T); break;
+                default: step<3>(kind, in, A, src, row, valid, acc, T); break;
+                }
+                break;
+            }
+            return;
+        }
+    }
+}
+
+// ------------------------------------------------------------- reductions
+
+// Reduce (s, c) over the block; the result is valid in thread 0.
+__device__ __forceinline__ void block_reduce(double &s, double &c) {
+    __shared__ double ls[GF_BLOCK / 64], lc[GF_BLOCK / 64];
+#pragma unroll
+    for(int off = 32; off > 0; off >>= 1) {
+        double s2 = __shfl_xor(s, off, 64);
+        double c2 = __shfl_xor(c, off, 64);
+        gfd::kb_merge(s, c, s2, c2);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if(lane == 0) {
+        ls[wave] = s;
+        lc[wave] = c;
+    }
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        s = ls[0];
+        c = lc[0];
+        for(int w = 1; w < (int)(blockDim.x >> 6); ++w)
+            gfd::kb_merge(s, c, ls[w], lc[w]);
+    }
+    __syncthreads();
+}
+
+// Store this block's partial; the last block to finish adds all partials in
+// block order and writes result[0] = sum, result[1] = residual correction.
+__device__ __forceinline__ void finish_reduction(double s, double c, Partial *partials, unsigned *counter,
+                                                 double *result) {
+    __shared__ bool am_last;
+    block_reduce(s, c);
+    if(threadIdx.x == 0) {
+        partials[blockIdx.x].s = s;
+        partials[blockIdx.x].c = c;
+        __threadfence();
+        unsigned prev = atomicAdd(counter, 1u);
+        am_last       = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if(!am_last)
+        return;
+    double ts = 0.0, tc = 0.0;
+    for(unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
+        double ps = __hip_atomic_load(&partials[i].s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        double pc = __hip_atomic_load(&partials[i].c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        gfd::kb_merge(ts, tc, ps, pc);
+    }
+    block_reduce(ts, tc);
+    if(threadIdx.x == 0) {
+        double tot = ts + tc;
+        result[0]  = tot;
+        result[1]  = tc;
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ------------------------------------------------------------- event kernels
+
+template <int EPT>
+__global__ __launch_bounds__(GF_BLOCK) void k_reduce_metric(GfProgArgs A, int metric, Partial *partials,
+                                                            unsigned *counter, double *result) {
+    const int64_t n    = A.nevents;
+    const int64_t tile = (int64_t)GF_BLOCK * EPT;
+    const double norm  = A.norms[1];
+    const bool binned  = metric >= GF_METRIC_BINAVG_CODE;
+    double s = 0.0, c = 0.0;
+    for(int64_t base = (int64_t)blockIdx.x * tile; base < n; base += (int64_t)gridDim.x * tile) {
+        ColSrc<EPT> src;
+        src.cols = A.cols;
+        int64_t row[EPT];
+        bool valid[EPT];
+#pragma unroll
+        for(int j = 0; j < EPT; ++j) {
+            int64_t e  = base + (int64_t)j * GF_BLOCK + threadIdx.x;
+            valid[j]   = e < n;
+            row[j]     = valid[j] ? e : n - 1;
+            src.idx[j] = row[j];
+        }
+        double T[EPT];
+        run<EPT>(A, 0, src, row, valid, T);
+#pragma unroll
+        for(int j = 0; j < EPT; ++j) {
+            double e0 = 0.0, e1 = 0.0;
+            if(binned) {
+                e0 = A.cols[A.nobs - 2][row[j]];
+                e1 = A.cols[A.nobs - 1][row[j]];
+            }
+            double v = gfd::metric(metric, T[j], e0, e1, norm, A.total_events);
+            if(valid[j])
+                gfd::kb_add(s, c, v);
+        }
+    }
+    finish_reduction(s, c, partials, counter, result);
+}
+
+template <int EPT>
+__global__ __launch_bounds__(GF_BLOCK) void k_eval_metric(GfProgArgs A, int metric, double *out) {
+    const int64_t n    = A.nevents;
+    const int64_t tile = (int64_t)GF_BLOCK * EPT;
+    const double norm  = A.norms[1];
+    const bool binned  = metric >= GF_METRIC_BINAVG_CODE;
+    for(int64_t base = (int64_t)blockIdx.x * tile; base < n; base += (int64_t)gridDim.x * tile) {
+        ColSrc<EPT> src;
+        src.cols = A.cols;
+        int64_t row[EPT];
+        bool valid[EPT];
+#pragma unroll
+        for(int j = 0; j < EPT; ++j) {
+            int64_t e  = base + (int64_t)j * GF_BLOCK + threadIdx.x;
+            valid[j]   = e < n;
+            row[j]     = valid[j] ? e : n - 1;
+            src.idx[j] = row[j];
+        }
+        double T[EPT];
+        run<EPT>(A, 0, src, row, valid, T);
+#pragma unroll
+        for(int j = 0; j < EPT; ++j) {
+            double e0 = 0.0, e1 = 0.0;
+            if(binned) {
+                e0 = A.cols[A.nobs - 2][row[j]];
+                e1 = A.cols[A.nobs - 1][row[j]];
+            }
+            double v = gfd::metric(metric, T[j], e0, e1, norm, A.total_events);
+            if(valid[j])
+                out[row[j]] = v;
+        }
+    }
+}
+
+}  // namespace
+
+// Grid description of k_reduce_bins (by value; indexed only by unrolled constants).
+struct GfGrid {
+    double lo[GF_MAX_OBS], hi[GF_MAX_OBS];
+    int nb[GF_MAX_OBS];
+    int slot[GF_MAX_OBS];
+    int k;
+};
+
+namespace {
+
+template <int EPT>
+__global__ __launch_bounds__(GF_BLOCK) void k_reduce_bins(GfProgArgs A, int pc0, GfGrid G, int64_t total,
+                                                          Partial *partials, unsigned *counter, double *result) {
+    const int64_t tile = (int64_t)GF_BLOCK * EPT;
+    double s = 0.0, c = 0.0;
+    for(int64_t base = (int64_t)blockIdx.x * tile; base < total; base += (int64_t)gridDim.x * tile) {
+        RegSrc<EPT> src;
+        int64_t row[EPT];
+        bool valid[EPT];
+#pragma unroll
+        for(int j = 0; j < EPT; ++j) {
+            int64_t b = base + (int64_t)j * GF_BLOCK + threadIdx.x;
+            valid[j]  = b < total;
+            row[j]    = valid[j] ? b : 0;
+#pragma unroll
+            for(int o = 0; o < GF_MAX_OBS; ++o)
+                src.v[j][o] = 0.0;
+            int64_t binNumber = row[j];
+            // MetricTaker.cu:76-86: first observable fastest, centre ((u-l)/n)*(b+0.5)+l
+#pragma unroll
+            for(int d = 0; d < GF_MAX_OBS; ++d) {
+                if(d < G.k) {
+                    int64_t localBin = binNumber % G.nb[d];
+                    double x         = G.hi[d] - G.lo[d];
+                    x /= G.nb[d];
+                    x *= ((double)localBin + 0.5);
+                    x += G.lo[d];
+                    src.set(G.slot[d], j, x);
+                    binNumber /= G.nb[d];
+                }
+            }
+        }
+        double T[EPT];
+        run<EPT>(A, pc0, src, row, valid, T);
+#pragma unroll
+        for(int j = 0; j < EPT; ++j)
+            if(valid[j])
+                gfd::kb_add(s, c, T[j]);
+    }
+    finish_reduction(s, c, partials, counter, result);
+}
+
+// ------------------------------------------------------------- Dalitz cache
+
+__global__ __launch_bounds__(256) void k_dalitz_cache(GfProgArgs A, int slot, int64_t offset, int64_t n,
+                                                      uint32_t mask, const GfResDesc *__restrict__ res, int nres) {
+    const GfDalitzSlot dz = A.dz[slot];
+    const GfDecay D       = A.decay;
+    double2 *wave         = const_cast<double2 *>(dz.wave);
+    const double *c12     = A.cols[dz.col_m12];
+    const double *c13     = A.cols[dz.col_m13];
+    for(int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const double m12  = c12[offset + e];
+        const double m13  = c13[offset + e];
+        const bool inside = gfd::in_dalitz(m12, m13, D.motherMass, D.daug1Mass, D.daug2Mass, D.daug3Mass);
+        // SpecialResonanceCalculator.cu:36-37
+        const double m23 = D.motherMass * D.motherMass + D.daug1Mass * D.daug1Mass + D.daug2Mass * D.daug2Mass
+                           + D.daug3Mass * D.daug3Mass - m12 - m13;
+        for(int i = 0; i < nres; ++i) {
+            if(!((mask >> i) & 1u))
+                continue;
+            double2 w = make_double2(0.0, 0.0);
+            if(inside)
+                w = gfd::lineshape(res[i], A.params, A.consts, D, m12, m13, m23);
+            wave[(int64_t)i * dz.ncache + e] = w;
+        }
+    }
+}
+
+// ------------------------------------------------------------- FP64-MFMA Gram
+
+typedef double gf_double4 __attribute__((ext_vector_type(4)));
+
+#define GG_BLOCK 128  // two waves
+#define GG_LD 65      // padded LDS row (doubles): 16 lanes reading one column hit distinct banks
+
+struct GramSrc {
+    int mode;  // 0 grid, 1 points, 2 data cache
+    double l12, u12, l13, u13;
+    int n12, n13;
+    const double *m12, *m13, *w;
+    int64_t npoints;
+    int slot;          // mode 2
+    int id_m12, id_m13;
+    int eff_pc;        // -1: efficiency 1
+};
+
+// One block: 128 points per pass; each lane computes the 16 complex amplitudes of
+// its point (wave-uniform control flow), stages them in LDS component-major,
+// and each wave multiplies its 64 points as 16 MFMA k-steps for each of three
+// 16x16 output tiles: RR = R^T E R, II = I^T E I, RI = R^T E I, with
+// E = diag(eff * weight).  Partials: 3 tiles x 64 lanes x 4 values per block.
+__global__ __launch_bounds__(GG_BLOCK) void k_gram(GfProgArgs A, GramSrc S, const GfResDesc *__restrict__ res,
+                                                   int nres, double *partials) {
+    __shared__ double Xr[2][16][GG_LD];
+    __shared__ double Xi[2][16][GG_LD];
+    __shared__ double Ew[2][64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const GfDecay D = A.decay;
+    gf_double4 RR = {0, 0, 0, 0}, II = {0, 0, 0, 0}, RI = {0, 0, 0, 0};
+    const int64_t npts = S.npoints;
+    for(int64_t base = (int64_t)blockIdx.x * GG_BLOCK; base < npts; base += (int64_t)gridDim.x * GG_BLOCK) {
+        const int64_t p  = base + tid;
+        const bool valid = p < npts;
+        double m12 = 0.0, m13 = 0.0, wt = 1.0;
+        if(S.mode == 0) {
+            // SpecialResonanceIntegrator.cu:61-76; the m13 bin count is read from the
+            // m12 entry there (numBinsM13 = numBinsM12), reproduced here.
+            int globalBinNumber = valid ? (int)p : 0;
+            int numBinsM12      = S.n12;
+            int binNumberM12    = globalBinNumber % numBinsM12;
+            double c12          = S.u12 - S.l12;
+            c12 /= numBinsM12;
+            c12 *= (binNumberM12 + 0.5);
+            c12 += S.l12;
+            globalBinNumber /= numBinsM12;
+            int numBinsM13 = S.n12;
+            double c13     = S.u13 - S.l13;
+            c13 /= numBinsM13;
+            c13 *= (globalBinNumber + 0.5);
+            c13 += S.l13;
+            m12 = c12;
+            m13 = c13;
+        } else if(S.mode == 1) {
+            m12 = valid ? S.m12[p] : 0.0;
+            m13 = valid ? S.m13[p] : 0.0;
+            wt  = (valid && S.w) ? S.w[p] : 1.0;
+        }
+        double ew = 0.0;
+        if(S.mode == 2) {
+            const GfDalitzSlot dz = A.dz[S.slot];
+            const int64_t q       = valid ? p : 0;
+            for(int i = 0; i < GF_MAX_RES; ++i) {
+                double2 b = (i < nres) ? dz.wave[(int64_t)i * dz.ncache + q] : make_double2(0.0, 0.0);
+                Xr[wave][i][lane] = b.x;
+                Xi[wave][i][lane] = b.y;
+            }
+            ew = valid ? 1.0 : 0.0;
+        } else {
+            const bool inside = valid && gfd::in_dalitz(m12, m13, D.motherMass, D.daug1Mass, D.daug2Mass, D.daug3Mass);
+            const double m23  = D.motherMass * D.motherMass + D.daug1Mass * D.daug1Mass + D.daug2Mass * D.daug2Mass
+                               + D.daug3Mass * D.daug3Mass - m12 - m13;
+            for(int i = 0; i < GF_MAX_RES; ++i) {
+                double2 b = make_double2(0.0, 0.0);
+                if(i < nres && inside)
+                    b = gfd::lineshape(res[i], A.params, A.consts, D, m12, m13, m23);
+                Xr[wave][i][lane] = b.x;
+                Xi[wave][i][lane] = b.y;
+            }
+            double eff = 1.0;
+            if(S.eff_pc >= 0) {
+                // The fake event of SpecialResonanceIntegrator.cu:97-101: slot 0 = 2,
+                // then m12 and m13 at their observable slots; other slots 0.
+                RegSrc<1> ev;
+#pragma unroll
+                for(int o = 0; o < GF_MAX_OBS; ++o)
+                    ev.v[0][o] = 0.0;
+                ev.v[0][0] = 2.0;
+                ev.set(S.id_m12, 0, m12);
+                ev.set(S.id_m13, 0, m13);
+                const int64_t r0[1] = {0};
+                const bool v0[1]    = {true};
+                double T[1];
+                run<1>(A, S.eff_pc, ev, r0, v0, T);
+                eff = T[0];
+            }
+            ew = valid ? eff * wt : 0.0;
+        }
+        Ew[wave][lane] = ew;
+        __syncthreads();
+        // A operand lane l: A[i = l%16][k = l/16]; B operand lane l: B[k = l/16][j = l%16].
+        const int comp = lane & 15, sub = lane >> 4;
+#pragma unroll
+        for(int kk = 0; kk < 16; ++kk) {
+            const int pt    = 4 * kk + sub;
+            const double ar = Xr[wave][comp][pt];
+            const double ai = Xi[wave][comp][pt];
+            const double e  = Ew[wave][pt];
+            const double br = e * ar, bi = e * ai;
+            RR = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, RR, 0, 0, 0);
+            II = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, II, 0, 0, 0);
+            RI = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, RI, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // add wave 1 into wave 0 through LDS, then write this block's partial
+    double *stage = &Xr[0][0][0];  // 2*16*65 doubles >= 64*12
+    if(wave == 1) {
+#pragma unroll
+        for(int r = 0; r < 4; ++r) {
+            stage[(0 * 4 + r) * 64 + lane] = RR[r];
+            stage[(1 * 4 + r) * 64 + lane] = II[r];
+            stage[(2 * 4 + r) * 64 + lane] = RI[r];
+        }
+    }
+    __syncthreads();
+    if(wave == 0) {
+        double *out = partials + (int64_t)blockIdx.x * 768;
+#pragma unroll
+        for(int r = 0; r < 4; ++r) {
+            out[(0 * 4 + r) * 64 + lane] = RR[r] + stage[(0 * 4 + r) * 64 + lane];
+            out[(1 * 4 + r) * 64 + lane] = II[r] + stage[(1 * 4 + r) * 64 + lane];
+            out[(2 * 4 + r) * 64 + lane] = RI[r] + stage[(2 * 4 + r) * 64 + lane];
+        }
+    }
+}
+
+// Sum the per-block Gram partials in block order (one thread per element).
+__global__ __launch_bounds__(256) void k_gram_final(const double *__restrict__ partials, int nblocks, double *out) {
+    for(int e = threadIdx.x; e < 768; e += blockDim.x) {
+        double s = 0.0, c = 0.0;
+        for(int b = 0; b < nblocks; ++b)
+            gfd::kb_add(s, c, partials[(int64_t)b * 768 + e]);
+        out[e] = s + c;
+    }
+}
+
+// One v_mfma_f64_16x16x4f64 on given per-lane operands: out[4*lane + r] is
+// D-register r of the lane.  Used to establish the operand/result lane layout.
+__global__ __launch_bounds__(64) void k_mfma_probe(const double *__restrict__ a, const double *__restrict__ b,
+                                                   double *out) {
+    const int l    = threadIdx.x;
+    gf_double4 acc = {0, 0, 0, 0};
+    acc            = __builtin_amdgcn_mfma_f64_16x16x4f64(a[l], b[l], acc, 0, 0, 0);
+    out[4 * l + 0] = acc[0];
+    out[4 * l + 1] = acc[1];
+    out[4 * l + 2] = acc[2];
+    out[4 * l + 3] = acc[3];
+}
+
+// flag stays 1 only if floor(0.5 + col[i]) == i for every row (Amp3Body.cu:82-84).
+__global__ __launch_bounds__(256) void k_identity_check(const double *__restrict__ col, int64_t n, int32_t *flag) {
+    bool ok = true;
+    for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        ok = ok && ((int64_t)floor(0.5 + col[i]) == i);
+    if(!ok)
+        atomicAnd(flag, 0);
+}
+
+inline int check(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+
+namespace gfk {
+
+static int ept_choice() { return 2; }
+
+int launch_reduce_metric(const GfProgArgs &a, int metric, Partial *partials, unsigned *counter, double *result,
+                         int nblocks, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    switch(ept_choice()) {
+    case 1: hipLaunchKernelGGL(k_reduce_metric<1>, dim3(nblocks), dim3(GF_BLOCK), 0, s, a, metric, partials, counter,
+                               result); break;
+    default: hipLaunchKernelGGL(k_reduce_metric<2>, dim3(nblocks), dim3(GF_BLOCK), 0, s, a, metric, partials,
+                                counter, result); break;
+    }
+    return check(hipGetLastError());
+}
+
+int launch_eval_metric(const GfProgArgs &a, int metric, double *out, int nblocks, void *stream) {
+    hipLaunchKernelGGL(k_eval_metric<2>, dim3(nblocks), dim3(GF_BLOCK), 0, (hipStream_t)stream, a, metric, out);
+    return check(hipGetLastError());
+}
+
+int launch_reduce_bins(const GfProgArgs &a, int pc0, const double *ranges, const int *slots, int k, int64_t total,
+                       Partial *partials, unsigned *counter, double *result, int nblocks, void *stream) {
+    GfGrid G;
+    for(int d = 0; d < GF_MAX_OBS; ++d) {
+        G.lo[d]   = d < k ? ranges[3 * d + 0] : 0.0;
+        G.hi[d]   = d < k ? ranges[3 * d + 1] : 1.0;
+        G.nb[d]   = d < k ? (int)floor(ranges[3 * d + 2] + 0.5) : 1;
+        G.slot[d] = d < k ? slots[d] : 0;
+    }
+    G.k = k;
+    hipLaunchKernelGGL(k_reduce_bins<2>, dim3(nblocks), dim3(GF_BLOCK), 0, (hipStream_t)stream, a, pc0, G, total,
+                       partials, counter, result);
+    return check(hipGetLastError());
+}
+
+int launch_dalitz_cache(const GfProgArgs &a, int slot, int64_t offset, int64_t n, uint32_t mask, const GfResDesc *res,
+                        int nres, int nblocks, void *stream) {
+    hipLaunchKernelGGL(k_dalitz_cache, dim3(nblocks), dim3(256), 0, (hipStream_t)stream, a, slot, offset, n, mask, res,
+                       nres);
+    return check(hipGetLastError());
+}
+
+int launch_gram(const GfProgArgs &a, int mode, const double *grid6, const double *m12, const double *m13,
+                const double *w, int64_t npoints, int slot, int id_m12, int id_m13, int eff_pc, const GfResDesc *res,
+                int nres, double *partials, double *tiles_dev, int nblocks, void *stream) {
+    GramSrc S;
+    S.mode = mode;
+    S.l12 = S.u12 = S.l13 = S.u13 = 0.0;
+    S.n12 = S.n13 = 1;
+    if(grid6) {
+        S.l12 = grid6[0];
+        S.u12 = grid6[1];
+        S.n12 = (int)floor(grid6[2] + 0.5);
+        S.l13 = grid6[3];
+        S.u13 = grid6[4];
+        S.n13 = (int)floor(grid6[5] + 0.5);
+    }
+    S.m12     = m12;
+    S.m13     = m13;
+    S.w       = w;
+    S.npoints = npoints;
+    S.slot    = slot;
+    S.id_m12  = id_m12;
+    S.id_m13  = id_m13;
+    S.eff_pc  = eff_pc;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_gram, dim3(nblocks), dim3(GG_BLOCK), 0, s, a, S, res, nres, partials);
+    int rc = check(hipGetLastError());
+    if(rc)
+        return rc;
+    hipLaunchKernelGGL(k_gram_final, dim3(1), dim3(256), 0, s, partials, nblocks, tiles_dev);
+    return check(hipGetLastError());
+}
+
+int launch_identity_check(const double *col, int64_t n, int32_t *flag, void *stream) {
+    int64_t nb = (n + 255) / 256;
+    if(nb > 2048)
+        nb = 2048;
+    if(nb < 1)
+        nb = 1;
+    hipLaunchKernelGGL(k_identity_check, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, col, n, flag);
+    return check(hipGetLastError());
+}
+
+int mfma_probe(const double *dev_a, const double *dev_b, double *dev_out, void *stream) {
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, dev_a, dev_b, dev_out);
+    return check(hipGetLastError());
+}
+
+}  // namespace gfk

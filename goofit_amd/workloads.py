@@ -1,0 +1,162 @@
+"""Synthetic workloads of the benchmark configurations (seeded numpy generators).
+
+* ``sample_add1d``   -- 1-D AddPdf fit: 0.3 Gaussian(0.3, 0.8) + 0.7 Exp(-0.4) on [-5, 5].
+* ``sample_prod3d``  -- 3-D ProdPdf: Gaussian(0, 1) on [-6, 6] x Exp(-1.2) on [0, 8] x
+  Gaussian-convolved exponential decay (tau 1.5, resolution 0.1) on [0, 10].
+* ``DALITZ_*``       -- the D0 -> pi+ pi- pi0 model with 16 resonances (the resonance
+  list of the reference's examples/dalitz/dalitz.cpp:124-296, with the rho0 as a
+  Gounaris-Sakurai lineshape as in examples/pipipi0DPFit/pipipi0DPFit.cpp:563-570),
+  couplings drawn from a seed with |c| in [0.1, 2] and a uniform phase.
+* ``dalitz_uniform`` -- points uniform over the Dalitz plot (flat phase space).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+# --------------------------------------------------------------------- 1-D / 3-D
+
+
+def _trunc_gauss(rng, n, mu, sigma, lo, hi):
+    out = np.empty(0)
+    while out.size < n:
+        x = rng.normal(mu, sigma, size=max(2 * (n - out.size), 1024))
+        out = np.concatenate([out, x[(x >= lo) & (x <= hi)]])
+    return out[:n]
+
+
+def _trunc_exp(rng, n, alpha, lo, hi):
+    """Inverse CDF of exp(alpha x) on [lo, hi]."""
+    u = rng.random(n)
+    if alpha == 0:
+        return lo + u * (hi - lo)
+    a, b = math.exp(alpha * lo), math.exp(alpha * hi)
+    return np.log(a + u * (b - a)) / alpha
+
+
+def sample_add1d(n=100_000, seed=1, frac=0.3, mu=0.3, sigma=0.8, alpha=-0.4, lo=-5.0, hi=5.0):
+    rng = np.random.default_rng(seed)
+    is_sig = rng.random(n) < frac
+    ns = int(is_sig.sum())
+    x = np.empty(n)
+    x[is_sig] = _trunc_gauss(rng, ns, mu, sigma, lo, hi)
+    x[~is_sig] = _trunc_exp(rng, n - ns, alpha, lo, hi)
+    return x
+
+
+def sample_emg(rng, n, mu, sigma, tau, lo, hi):
+    out = np.empty(0)
+    while out.size < n:
+        m = max(2 * (n - out.size), 1024)
+        z = rng.normal(mu, sigma, size=m) + rng.exponential(tau, size=m)
+        out = np.concatenate([out, z[(z >= lo) & (z <= hi)]])
+    return out[:n]
+
+
+def sample_prod3d(n=10_000_000, seed=2, mu=0.0, sigma=1.0, alpha=-1.2, tau=1.5, res=0.1):
+    rng = np.random.default_rng(seed)
+    x = _trunc_gauss(rng, n, mu, sigma, -6.0, 6.0)
+    y = _trunc_exp(rng, n, alpha, 0.0, 8.0)
+    z = sample_emg(rng, n, 0.0, res, tau, 0.0, 10.0)
+    return x, y, z
+
+
+# ------------------------------------------------------------------------ Dalitz
+
+DALITZ_MOTHER = 1.8648
+DALITZ_PI0 = 0.13498
+DALITZ_PIPM = 0.13957
+DALITZ_MESON_RADIUS = 1.5
+DALITZ_MOTHER_RADIUS = 5.0  # set explicitly: the reference leaves it uninitialised
+# (motherMass, daug1 = pi0, daug2 = pi+, daug3 = pi-, meson radius, mother meson radius)
+DALITZ_DECAY = (DALITZ_MOTHER, DALITZ_PI0, DALITZ_PIPM, DALITZ_PIPM, DALITZ_MESON_RADIUS, DALITZ_MOTHER_RADIUS)
+
+PAIR_12, PAIR_13, PAIR_23 = 0, 1, 2
+
+# name, kind, mass, width, spin, pair
+DALITZ_RESONANCES = [
+    ("nonr", "NONRES", 0.0, 0.0, 0, 0),
+    ("rhop", "RBW", 0.7758, 0.1503, 1, PAIR_12),
+    ("rho0", "GS", 0.7758, 0.1503, 1, PAIR_23),
+    ("rhom", "RBW", 0.7758, 0.1503, 1, PAIR_13),
+    ("rhop_1450", "RBW", 1.465, 0.400, 1, PAIR_12),
+    ("rho0_1450", "RBW", 1.465, 0.400, 1, PAIR_23),
+    ("rhom_1450", "RBW", 1.465, 0.400, 1, PAIR_13),
+    ("rhop_1700", "RBW", 1.720, 0.250, 1, PAIR_12),
+    ("rho0_1700", "RBW", 1.720, 0.250, 1, PAIR_23),
+    ("rhom_1700", "RBW", 1.720, 0.250, 1, PAIR_13),
+    ("f0_980", "RBW", 0.980, 0.044, 0, PAIR_23),
+    ("f0_1370", "RBW", 1.434, 0.173, 0, PAIR_23),
+    ("f0_1500", "RBW", 1.507, 0.109, 0, PAIR_23),
+    ("f0_1710", "RBW", 1.714, 0.140, 0, PAIR_23),
+    ("f2_1270", "RBW", 1.2754, 0.1851, 2, PAIR_23),
+    ("f0_600", "RBW", 0.500, 0.400, 0, PAIR_23),
+]
+DALITZ_FIXED = "rhop"  # coupling held at (1, 0): removes the overall complex scale
+
+
+def dalitz_couplings(seed=3, n=len(DALITZ_RESONANCES), fixed_index=1):
+    """|c| ~ U[0.1, 2], phase ~ U[0, 2 pi); the reference coupling is (1, 0)."""
+    rng = np.random.default_rng(seed)
+    mag = rng.uniform(0.1, 2.0, size=n)
+    ph = rng.uniform(0.0, 2 * math.pi, size=n)
+    c = mag * np.exp(1j * ph)
+    if fixed_index is not None:
+        c[fixed_index] = 1.0 + 0.0j
+    return c
+
+
+def dalitz_box(decay=DALITZ_DECAY):
+    M, m1, m2, m3 = decay[:4]
+    return ((m1 + m2) ** 2, (M - m3) ** 2, (m1 + m3) ** 2, (M - m2) ** 2)
+
+
+def in_dalitz_np(m12, m13, decay=DALITZ_DECAY):
+    """Vectorised kinematic boundary (same arithmetic as the engine's inDalitz)."""
+    M, d1, d2, d3 = decay[:4]
+    m23 = M * M + d1 * d1 + d2 * d2 + d3 * d3 - m12 - m13
+    ok = (m23 >= 0) & (m23 <= (M - d1) ** 2)
+    ok &= (m12 >= (d1 + d2) ** 2) & (m12 <= (M - d3) ** 2)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        s = np.sqrt(m12)
+        e1 = 0.5 * (m12 - d2 * d2 + d1 * d1) / s
+        e3 = 0.5 * (M * M - m12 - d3 * d3) / s
+        r1 = np.sqrt(e1 * e1 - d1 * d1)
+        r3 = np.sqrt(e3 * e3 - d3 * d3)
+        lo = (e1 + e3) ** 2 - (r1 + r3) ** 2
+        hi = (e1 + e3) ** 2 - (r1 - r3) ** 2
+        ok &= (m13 >= lo) & (m13 <= hi)
+    return ok
+
+
+def dalitz_uniform(n, seed=4, decay=DALITZ_DECAY):
+    """n points uniform over the Dalitz plot (rejection from the bounding box)."""
+    rng = np.random.default_rng(seed)
+    a, b, c, d = dalitz_box(decay)
+    m12 = np.empty(0)
+    m13 = np.empty(0)
+    while m12.size < n:
+        k = max(int(2.5 * (n - m12.size)), 4096)
+        x = rng.uniform(a, b, size=k)
+        y = rng.uniform(c, d, size=k)
+        ok = in_dalitz_np(x, y, decay)
+        m12 = np.concatenate([m12, x[ok]])
+        m13 = np.concatenate([m13, y[ok]])
+    return m12[:n], m13[:n]
+
+
+def dalitz_area(decay=DALITZ_DECAY, n=4000):
+    """Area of the Dalitz region in (m12, m13), by the m13 range integrated over m12."""
+    M, d1, d2, d3 = decay[:4]
+    lo, hi = (d1 + d2) ** 2, (M - d3) ** 2
+    # Gauss-Legendre over m12 of (m13max - m13min)
+    x, w = np.polynomial.legendre.leggauss(n)
+    m12 = 0.5 * (hi - lo) * x + 0.5 * (hi + lo)
+    s = np.sqrt(m12)
+    e1 = 0.5 * (m12 - d2 * d2 + d1 * d1) / s
+    e3 = 0.5 * (M * M - m12 - d3 * d3) / s
+    r1 = np.sqrt(np.maximum(e1 * e1 - d1 * d1, 0))
+    r3 = np.sqrt(np.maximum(e3 * e3 - d3 * d3, 0))
+    width = 4 * r1 * r3
+    return float(0.5 * (hi - lo) * np.sum(w * width))

@@ -1,0 +1,232 @@
+/*
+ * goofit_hip.h -- C ABI of the MI355X (gfx950) likelihood engine.
+ *
+ * This is the drop-in boundary that sits under the GooFit C++ API
+ * (GooPdf / PdfBase / Amp3Body / FitManager).  Every entry point takes plain
+ * pointers and sizes; no C++ exception and no torch type crosses it.  Each
+ * declaration names the reference interface it replaces (paths relative to
+ * the GooFit 2.4.0 source tree).
+ *
+ * Data model, identical to the reference's flattened PDF tree
+ * (src/PDFs/PdfBase.cu:268-311, include/goofit/PDFs/ParameterContainer.h:9-47):
+ *   params[], consts[], obs[], norms[]  -- per function a block [count, v0, v1, ...]
+ *                                          (norms blocks are [1, 1/integral])
+ *   funcs[]                             -- one gf_func code per function, in the
+ *                                          depth-first order of the tree walk
+ *                                          (replaces host_function_table's device
+ *                                          function pointers, src/PDFs/detail/Globals.cu:13)
+ * Events are uploaded as structure-of-arrays columns, column k holding the
+ * observable whose GooFit index is k (src/goofit/PdfBase.cpp:109-117).
+ *
+ * Errors: every call returns gf_status; gf_last_error() gives the message of the
+ * last failing call on the calling thread.  Threading: one gf_ctx per host
+ * thread; calls on one context are not re-entrant (the reference is not either,
+ * src/PDFs/detail/Globals.cu:9-34).
+ */
+#ifndef GOOFIT_HIP_H
+#define GOOFIT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t gf_status;
+#define GF_OK 0
+#define GF_ERR_INVALID 1  /* bad argument / program */
+#define GF_ERR_HIP 2      /* HIP runtime error */
+#define GF_ERR_NOMEM 3    /* device allocation failed */
+#define GF_ERR_STATE 4    /* call order violated (e.g. no events uploaded) */
+#define GF_ERR_NODEVICE 5 /* no GPU visible */
+
+/* Function codes.  Each replaces one registered device function pointer. */
+enum gf_func {
+    GF_FN_NONE              = 0,
+    GF_FN_GAUSSIAN          = 1,  /* device_Gaussian        src/PDFs/basic/GaussianPdf.cu:7-17 */
+    GF_FN_EXP               = 2,  /* device_Exp             src/PDFs/basic/ExpPdf.cu:7-17 */
+    GF_FN_EXP_OFFSET        = 3,  /* device_ExpOffset       src/PDFs/basic/ExpPdf.cu:19-30 */
+    GF_FN_EXP_POLY          = 4,  /* device_ExpPoly         src/PDFs/basic/ExpPdf.cu:32-48 */
+    GF_FN_EXP_POLY_OFFSET   = 5,  /* device_ExpPolyOffset   src/PDFs/basic/ExpPdf.cu:50-67 */
+    GF_FN_EXPGAUS           = 6,  /* device_ExpGaus         src/PDFs/basic/ExpGausPdf.cu:6-24 */
+    GF_FN_POLYNOMIAL        = 7,  /* device_Polynomial      src/PDFs/basic/PolynomialPdf.cu:8-25 */
+    GF_FN_OFFSET_POLYNOMIAL = 8,  /* device_OffsetPolynomial src/PDFs/basic/PolynomialPdf.cu:28-45 */
+    GF_FN_MULTI_POLYNOMIAL  = 9,  /* device_MultiPolynomial src/PDFs/basic/PolynomialPdf.cu:47-118 */
+    GF_FN_ADD               = 10, /* device_AddPdfs         src/PDFs/combine/AddPdf.cu:16-51 */
+    GF_FN_ADD_EXT           = 11, /* device_AddPdfsExt      src/PDFs/combine/AddPdf.cu:53-79 */
+    GF_FN_PROD              = 12, /* device_ProdPdfs        src/PDFs/combine/ProdPdf.cu:9-26 */
+    GF_FN_DALITZ            = 13, /* device_DalitzPlot      src/PDFs/physics/Amp3Body.cu:60-122 */
+    GF_FN_RBW               = 14, /* plainBW<1>             src/PDFs/physics/resonances/RBW.cu:9-86 */
+    GF_FN_RBW_SYM           = 15, /* plainBW<2>             src/PDFs/physics/resonances/RBW.cu:89 */
+    GF_FN_GS                = 16, /* gouSak                 src/PDFs/physics/resonances/GS.cu:9-75 */
+    GF_FN_NONRES            = 17, /* nonres                 src/PDFs/physics/resonances/NonRes.cu:8-11 */
+    GF_FN_COUNT             = 18
+};
+
+/* Metric codes: same numbering as the reference's EvalFunc
+ * (include/goofit/FitControl.h:10); functions in src/PDFs/MetricPointer.cu:8-58. */
+enum gf_metric {
+    GF_METRIC_EVAL         = 0,
+    GF_METRIC_NLL          = 1,
+    GF_METRIC_PROB         = 2,
+    GF_METRIC_BINAVG       = 3,
+    GF_METRIC_BINWITHERROR = 4,
+    GF_METRIC_CHISQ        = 5
+};
+
+typedef struct gf_ctx gf_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Number of visible GPUs (0 when none).  No reference counterpart. */
+int gf_device_count(void);
+
+/* Create a context on `device`: one HIP stream plus the device buffers of the
+ * flattened tree.  Replaces the process-global SmartVector mirrors
+ * (include/goofit/detail/SmartVectorGPU.h, src/PDFs/detail/Globals.cu:9-13). */
+gf_status gf_ctx_create(gf_ctx **out, int device);
+
+/* Free every device buffer of the context.  Replaces GooFit::cleanup()
+ * (src/PDFs/detail/Globals.cu:37-43) and gooFree (src/PDFs/PdfBase.cu:573-580). */
+void gf_ctx_destroy(gf_ctx *ctx);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *gf_last_error(void);
+
+/* ---- events ------------------------------------------------------------- */
+
+/* Upload n events as nobs host columns (cols[k] = observable with index k).
+ * Replaces the AoS pack + gooMalloc + MEMCPY of PdfBase::setData
+ * (src/PDFs/PdfBase.cu:329-437) and sets c_totalEvents (PdfBase.cu:438). */
+gf_status gf_events_upload(gf_ctx *ctx, int nobs, int64_t n, const double *const *cols);
+
+/* Same, from the reference's AoS host array host_array[i*nobs + k]
+ * (src/PDFs/PdfBase.cu:398-403). */
+gf_status gf_events_upload_aos(gf_ctx *ctx, int nobs, int64_t n, const double *aos);
+
+/* Allocate nobs device columns of n events without host data (for events
+ * generated on the GPU); column pointers via gf_events_column.
+ * No reference counterpart (the reference generates on the host). */
+gf_status gf_events_alloc(gf_ctx *ctx, int nobs, int64_t n);
+
+/* Device address of column k (valid until the next upload/alloc). */
+gf_status gf_events_column(gf_ctx *ctx, int k, double **dev_ptr);
+
+/* Declare that column k holds the event number of each row and that it equals
+ * the row index (0..n-1), as UnbinnedDataSet + EventNumber produce it.  The
+ * Dalitz coherent sum then indexes its wave cache by row instead of reading the
+ * column (src/PDFs/physics/Amp3Body.cu:82-84).  Checked on the host data when
+ * the events came through gf_events_upload*. */
+gf_status gf_events_set_identity_column(gf_ctx *ctx, int k);
+
+/* ---- flattened PDF tree ------------------------------------------------- */
+
+/* Install a flattened tree.  Replaces PdfBase::pre_run (src/PDFs/PdfBase.cu:24-31),
+ * i.e. the sync of host_function_table/parameters/constants/observables/
+ * normalizations to the device.  The tree is validated (counts walk exactly
+ * as ParameterContainer::incrementIndex, src/PDFs/ParameterContainer.cu:48-60)
+ * and compiled into a linear device program. */
+gf_status gf_program_set(gf_ctx *ctx,
+                         const int32_t *funcs, int nfunc,
+                         const double *params, int np,
+                         const double *consts, int nc,
+                         const double *obs, int no,
+                         const double *norms, int nn);
+
+/* Replace the parameter values (same layout and size as at gf_program_set).
+ * Replaces PdfBase::pre_call -> host_parameters.smart_sync
+ * (src/PDFs/PdfBase.cu:33-36). */
+gf_status gf_params_update(gf_ctx *ctx, const double *params, int np);
+
+/* Replace the normalisation values.  Replaces host_normalizations.sync
+ * (src/PDFs/GooPdf.cu:264). */
+gf_status gf_norms_update(gf_ctx *ctx, const double *norms, int nn);
+
+/* Three-body decay constants {motherMass, daug1Mass, daug2Mass, daug3Mass,
+ * meson_radius, mother_meson_radius}.  Replaces the MEMCPY_TO_SYMBOL of
+ * c_motherMass .. c_mother_meson_radius in the Amp3Body constructor
+ * (src/PDFs/physics/Amp3Body.cu:146-151). */
+gf_status gf_decay3_set(gf_ctx *ctx, const double decay[6]);
+
+/* ---- reductions over events and grids ----------------------------------- */
+
+/* Sum over all events of metric(pdf(evt), norm of the top node).  Replaces
+ * GooPdf::reduce_with_metric (src/PDFs/GooPdf.cu:45-78) with the MetricTaker
+ * unbinned operator (src/PDFs/MetricTaker.cu:17-44).  The sum is accumulated
+ * with compensated FP64 arithmetic; *sum is the rounded total, *comp (may be
+ * NULL) the residual correction term that was folded in. */
+gf_status gf_reduce_metric(gf_ctx *ctx, int metric, double *sum, double *comp);
+
+/* Per-event metric values into host array out[n] (n = number of events).
+ * Replaces GooPdf::evaluate_with_metric (src/PDFs/GooPdf.cu:112-173). */
+gf_status gf_eval_metric(gf_ctx *ctx, int metric, double *out, int64_t n);
+
+/* Same, into a device buffer (no host copy). */
+gf_status gf_eval_metric_device(gf_ctx *ctx, int metric, double *dev_out, int64_t n);
+
+/* Sum of the raw value of function func_idx over the midpoint grid given by
+ * ranges[3*i+0..2] = {lower, upper, nbins} of the function's k observables
+ * (first observable fastest).  Replaces GooPdf::reduce_with_bins
+ * (src/PDFs/GooPdf.cu:83-108) with the MetricTaker binned operator
+ * (src/PDFs/MetricTaker.cu:48-96). */
+gf_status gf_reduce_bins(gf_ctx *ctx, int func_idx, const double *ranges, int k, double *sum);
+
+/* ---- three-body Dalitz amplitude ---------------------------------------- */
+
+/* Allocate the wave cache of Dalitz node dalitz_func for n entries, read from
+ * event rows offset..offset+n-1.  Replaces Amp3Body::setDataSize
+ * (src/PDFs/physics/Amp3Body.cu:204-236). */
+gf_status gf_dalitz_set_data_size(gf_ctx *ctx, int dalitz_func, int64_t n, int64_t offset);
+
+/* Recompute the cached lineshape W_i[e] of every resonance i with bit i set in
+ * res_mask (0 outside the Dalitz region).  Replaces the per-resonance
+ * thrust::transform with SpecialResonanceCalculator
+ * (src/PDFs/physics/Amp3Body.cu:292-310, detail/SpecialResonanceCalculator.cu:12-46). */
+gf_status gf_dalitz_cache(gf_ctx *ctx, int dalitz_func, uint64_t res_mask);
+
+/* Normalisation integrals I_ij = sum_cells eff * B_i * conj(B_j) over the
+ * m12 x m13 midpoint grid grid6 = {l12, u12, n12, l13, u13, n13}, as one
+ * fused lineshape + FP64-MFMA Gram kernel; out[2*(i*n+j)+{0,1}] = {Re, Im}.
+ * eff_func is the function index of the efficiency.  Replaces the n*n
+ * transform_reduce calls with SpecialResonanceIntegrator
+ * (src/PDFs/physics/Amp3Body.cu:312-330, detail/SpecialResonanceIntegrator.cu:7-121). */
+gf_status gf_dalitz_gram(gf_ctx *ctx, int dalitz_func, const double grid6[6], int eff_func,
+                         double *out_re_im);
+
+/* Same integrals over a list of np points (m12[p], m13[p]) with weights w[p]
+ * (w may be NULL for 1): I_ij = sum_p w_p eff_p B_i(p) conj(B_j(p)).  Host
+ * arrays.  Normalisation by phase-space Monte Carlo; no reference counterpart. */
+gf_status gf_dalitz_gram_points(gf_ctx *ctx, int dalitz_func, const double *m12, const double *m13,
+                                const double *w, int64_t np, int eff_func, double *out_re_im);
+
+/* Same, with the point arrays already in device memory. */
+gf_status gf_dalitz_gram_points_device(gf_ctx *ctx, int dalitz_func, const double *dev_m12,
+                                       const double *dev_m13, const double *dev_w, int64_t np,
+                                       int eff_func, double *out_re_im);
+
+/* Copy cached wave `res` (n complex values, re/im interleaved) to the host.
+ * Replaces Amp3Body::getCachedWave (src/PDFs/physics/Amp3Body.cu:370-375). */
+gf_status gf_dalitz_get_cache(gf_ctx *ctx, int dalitz_func, int res, double *out_re_im, int64_t n);
+
+/* Data-event Gram G_ij = sum_e W_i[e] conj(W_j[e]) over the wave cache, used for
+ * fit fractions.  Replaces the n*n transform_reduce calls of
+ * Amp3Body::fit_fractions (src/PDFs/physics/Amp3Body.cu:376-428). */
+gf_status gf_dalitz_cache_gram(gf_ctx *ctx, int dalitz_func, double *out_re_im);
+
+/* ---- timing ------------------------------------------------------------- */
+
+/* Run gf_reduce_metric `reps` times back to back on the context stream and
+ * return the elapsed device time between HIP events recorded on that stream
+ * before the first and after the last launch.  No reference counterpart. */
+gf_status gf_time_reduce_metric(gf_ctx *ctx, int metric, int reps, double *ms_total);
+
+/* Same for gf_dalitz_cache (all resonances) and gf_dalitz_gram. */
+gf_status gf_time_dalitz_cache(gf_ctx *ctx, int dalitz_func, int reps, double *ms_total);
+gf_status gf_time_dalitz_gram(gf_ctx *ctx, int dalitz_func, const double grid6[6], int eff_func, int reps,
+                              double *ms_total);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GOOFIT_HIP_H */

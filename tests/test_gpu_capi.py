@@ -1,0 +1,284 @@
+"""Parity of the gfx950 engine with the oracle, through the C ABI.
+
+Tolerances: the NLL sum within 1e-11 relative; single
+event values within 1e-13 relative (libm exp/log/erfc/pow differ by <= a few
+ulp between glibc and the device math library); grid indices bit-exact.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from goofit_amd import capi
+from goofit_amd import workloads as W
+
+from _models import add1d, arrays, dalitz, dalitz_events, prod3d
+
+pytestmark = pytest.mark.gpu
+
+NLL_RTOL = 1e-11
+EVT_RTOL = 1e-13
+
+
+def _install(ctx, top, decay=None):
+    prog, (f, p, c, o, n) = arrays(top)
+    ctx.set_program(f, p, c, o, n)
+    if decay is not None:
+        ctx.set_decay(decay)
+    return prog
+
+
+def _sync(ctx, model: O.Model):
+    """Push the oracle model's current parameters and norms to the engine."""
+    model.op.sync_params()
+    model.op.sync_norms()
+    ctx.update_params(model.op.p)
+    ctx.update_norms(model.op.n)
+
+
+@pytest.mark.parametrize("n", [1, 7, 513, 100_000])
+def test_add1d_nll(ctx, n):
+    top, x = add1d()
+    data = W.sample_add1d(n, seed=11)
+    aos = data.reshape(-1, 1)
+    M = O.Model(top, aos)
+    ref = M.calculate_nll()
+    _install(ctx, top)
+    ctx.upload_columns([data])
+    _sync(ctx, M)
+    got, comp = ctx.reduce_metric("NLL")
+    assert math.isclose(got, ref, rel_tol=NLL_RTOL), (got, ref)
+    exact = O.exact_sum(M.nll_terms())
+    assert math.isclose(got, exact, rel_tol=1e-13), (got, exact)
+
+
+def test_add1d_per_event_prob(ctx):
+    top, x = add1d()
+    data = W.sample_add1d(4099, seed=12)
+    M = O.Model(top, data.reshape(-1, 1))
+    M.normalize()
+    M.op.sync_params()
+    M.op.sync_norms()
+    ref = M.op.eval_metric(data.reshape(-1, 1), "PROB")
+    _install(ctx, top)
+    ctx.upload_columns([data])
+    _sync(ctx, M)
+    got = ctx.eval_metric("PROB")
+    np.testing.assert_allclose(got, ref, rtol=EVT_RTOL, atol=0)
+
+
+def test_add1d_extended(ctx):
+    top, x = add1d(extended=True, frac=0.25)
+    data = W.sample_add1d(20_000, seed=13)
+    M = O.Model(top, data.reshape(-1, 1))
+    M.normalize()
+    _install(ctx, top)
+    ctx.upload_columns([data])
+    _sync(ctx, M)
+    got = ctx.reduce_metric("NLL")[0]
+    ref = M.op.reduce_metric(data.reshape(-1, 1), "NLL")
+    assert math.isclose(got, ref, rel_tol=NLL_RTOL)
+
+
+def test_gaussian_grid_kat(ctx):
+    """tests/simple/NormalizeTest.cpp:57-100 -- Gaussian(1.5, 2.5) Prob on the 100-bin grid of [-5, 10]."""
+    x = O.Obs("x", -5, 10, 100)
+    g = O.Gaussian("g", x, O.Var("m", 1.5), O.Var("s", 2.5))
+    grid = -5 + (np.arange(100) + 0.5) * 0.15
+    M = O.Model(g, grid.reshape(-1, 1))
+    M.normalize()
+    _install(ctx, g)
+    ctx.upload_columns([grid])
+    _sync(ctx, M)
+    vv = ctx.eval_metric("PROB")
+    kat = {0: 0.00587129964482, 20: 0.0624318782242, 40: 0.157287605852, 60: 0.0938855055588}
+    for i, v in kat.items():
+        assert math.isclose(vv[i], v, rel_tol=1e-11), (i, vv[i], v)
+
+
+def test_prod3d_nll_and_grid(ctx):
+    top, (x, y, z) = prod3d()
+    xs, ys, zs = W.sample_prod3d(200_000, seed=21)
+    aos = np.stack([xs, ys, zs], axis=1)
+    M = O.Model(top, aos)
+    ref = M.calculate_nll()
+    _install(ctx, top)
+    ctx.upload_columns([xs, ys, zs])
+    # the ExpGaus grid integral (GooPdf::normalize numeric branch)
+    eg = top.components[2]
+    ranges = [z.lo, z.hi, z.nbins]
+    M.op.sync_params()
+    ctx.update_params(M.op.p)
+    got_bins = ctx.reduce_bins(eg.function_idx, ranges)
+    ref_bins = M.op.reduce_bins(eg.function_idx, ranges)
+    assert math.isclose(got_bins, ref_bins, rel_tol=1e-12), (got_bins, ref_bins)
+    _sync(ctx, M)
+    got = ctx.reduce_metric("NLL")[0]
+    assert math.isclose(got, ref, rel_tol=NLL_RTOL), (got, ref)
+
+
+def test_grid_nd_midpoint(ctx):
+    """2-D grid over a ProdPdf of two Gaussians: the midpoint rule, first axis fastest."""
+    x = O.Obs("x", -5, 10, 37)
+    y = O.Obs("y", -5, 10, 23)
+    g1 = O.Gaussian("g1", x, O.Var("m1", 1.5), O.Var("s1", 2.5))
+    g2 = O.Gaussian("g2", y, O.Var("m2", -1.0), O.Var("s2", 0.5))
+    top = O.Prod("p", [g1, g2])
+    M = O.Model(top)
+    M.normalize()
+    _install(ctx, top)
+    ctx.upload_columns([np.zeros(4), np.zeros(4)])
+    _sync(ctx, M)
+    r = [x.lo, x.hi, x.nbins, y.lo, y.hi, y.nbins]
+    got = ctx.reduce_bins(0, r)
+    ref = M.op.reduce_bins(0, r)
+    assert math.isclose(got, ref, rel_tol=1e-12)
+    # closed form of the same midpoint sum: product of the two 1-D sums
+    cx = x.lo + (np.arange(x.nbins) + 0.5) * x.bin_size()
+    cy = y.lo + (np.arange(y.nbins) + 0.5) * y.bin_size()
+    sx = np.exp(-0.5 * (cx - 1.5) ** 2 / 2.5 ** 2).sum() / (2.5 * math.sqrt(2 * math.pi))
+    sy = np.exp(-0.5 * (cy + 1.0) ** 2 / 0.5 ** 2).sum() / (0.5 * math.sqrt(2 * math.pi))
+    assert math.isclose(got, sx * sy, rel_tol=1e-12)
+
+
+def test_polynomials(ctx):
+    x = O.Obs("x", -2, 3, 50)
+    t = [O.Var("c0", 0.5), O.Var("c1", -0.25), O.Var("c2", 0.125)]
+    pol = O.Pdf("poly", [x], t, [1])
+    pol.func = O.FN["POLYNOMIAL"]
+    ep = O.Pdf("expoly", [x], [O.Var("e0", 0.1), O.Var("e1", -0.3), O.Var("e2", 0.02)])
+    ep.func = O.FN["EXP_POLY"]
+    top = O.Add("s", [O.Var("f", 0.4)], [pol, ep])
+    data = np.linspace(-1.9, 2.9, 301)
+    M = O.Model(top, data.reshape(-1, 1))
+    M.op.sync_params()
+    ref = M.op.eval_metric(data.reshape(-1, 1), "EVAL")
+    _install(ctx, top)
+    ctx.upload_columns([data])
+    _sync(ctx, M)
+    got = ctx.eval_metric("EVAL")
+    np.testing.assert_allclose(got, ref, rtol=EVT_RTOL, atol=1e-300)
+
+
+def test_multipolynomial_2d(ctx):
+    x = O.Obs("x", 0, 1, 10)
+    y = O.Obs("y", 0, 1, 10)
+    coeffs = [O.Var(f"k{i}", 0.3 + 0.1 * i) for i in range(6)]
+    mp = O.MultiPolynomial("mp", [x, y], coeffs, [O.Var("ox", 0.1), O.Var("oy", -0.2)], 2)
+    top = O.Prod("p", [mp])
+    rng = np.random.default_rng(3)
+    xs, ys = rng.random(1000), rng.random(1000)
+    aos = np.stack([xs, ys], 1)
+    M = O.Model(top, aos)
+    M.op.sync_params()
+    ref = M.op.eval_metric(aos, "EVAL")
+    _install(ctx, top)
+    ctx.upload_columns([xs, ys])
+    _sync(ctx, M)
+    got = ctx.eval_metric("EVAL")
+    np.testing.assert_allclose(got, ref, rtol=EVT_RTOL)
+
+
+# ------------------------------------------------------------------- Dalitz
+
+
+def _dalitz_setup(ctx, n, nbins=64, seed=5):
+    top, (m12, m13, num) = dalitz(nbins=nbins)
+    aos = dalitz_events(n, seed)
+    M = O.Model(top, aos, decay=W.DALITZ_DECAY)
+    _install(ctx, top, decay=W.DALITZ_DECAY)
+    ctx.upload_columns([aos[:, 0], aos[:, 1], aos[:, 2]])
+    ctx.set_identity_column(2)
+    ctx.dalitz_set_data_size(top.function_idx, n)
+    return top, M, aos
+
+
+def test_dalitz_wave_cache(ctx):
+    n = 5000
+    top, M, aos = _dalitz_setup(ctx, n)
+    M.op.sync_params()
+    ctx.update_params(M.op.p)
+    ctx.dalitz_cache(top.function_idx)
+    res_funcs, eff = M.op.dalitz_layout(top.function_idx)
+    for i, rf in enumerate(res_funcs):
+        ref = M.op.dalitz_wave(top.function_idx, rf, aos)
+        got = ctx.dalitz_get_cache(top.function_idx, i, n)
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-14 * np.abs(ref).max())
+    outside = ~W.in_dalitz_np(aos[:, 0], aos[:, 1])
+    assert outside.any()
+    assert np.all(ctx.dalitz_get_cache(top.function_idx, 1, n)[outside] == 0)
+
+
+def test_dalitz_grid_gram(ctx):
+    top, M, aos = _dalitz_setup(ctx, 100, nbins=97)
+    M.op.sync_params()
+    ctx.update_params(M.op.p)
+    res_funcs, eff = M.op.dalitz_layout(top.function_idx)
+    grid6 = [0.0, 3.0, 97, 0.0, 3.0, 97]
+    ref = M.op.dalitz_integrals(top.function_idx, grid6, eff)
+    got = ctx.dalitz_gram(top.function_idx, grid6, eff, len(res_funcs))
+    scale = np.abs(ref).max()
+    np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-13 * scale)
+    # Hermitian, real non-negative diagonal
+    np.testing.assert_allclose(got, got.conj().T, rtol=0, atol=1e-13 * scale)
+    assert np.all(np.diag(got).real >= 0)
+
+
+def test_dalitz_nll(ctx):
+    n = 20_000
+    top, M, aos = _dalitz_setup(ctx, n, nbins=128)
+    ref = M.calculate_nll()  # normalize (grid) + coherent-sum NLL
+    M.op.sync_params()
+    ctx.update_params(M.op.p)
+    ctx.dalitz_cache(top.function_idx)
+    _sync(ctx, M)
+    got = ctx.reduce_metric("NLL")[0]
+    assert math.isclose(got, ref, rel_tol=NLL_RTOL), (got, ref)
+
+
+def test_dalitz_points_and_cache_gram(ctx):
+    n = 3000
+    top, M, aos = _dalitz_setup(ctx, n)
+    M.op.sync_params()
+    ctx.update_params(M.op.p)
+    ctx.dalitz_cache(top.function_idx)
+    res_funcs, eff = M.op.dalitz_layout(top.function_idx)
+    waves = np.stack([M.op.dalitz_wave(top.function_idx, rf, aos) for rf in res_funcs])  # nres x n
+    ref = waves @ waves.conj().T
+    got = ctx.dalitz_cache_gram(top.function_idx, len(res_funcs))
+    np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+    w = np.random.default_rng(9).random(n)
+    inside = W.in_dalitz_np(aos[:, 0], aos[:, 1]).astype(float)
+    ref_p = (waves * (w * inside)) @ waves.conj().T
+    got_p = ctx.dalitz_gram_points(top.function_idx, aos[:, 0], aos[:, 1], w, eff, len(res_funcs))
+    np.testing.assert_allclose(got_p, ref_p, rtol=1e-11, atol=1e-12 * np.abs(ref_p).max())
+
+
+def test_dalitz_event_number_out_of_cache(ctx):
+    n = 256
+    top, M, aos = _dalitz_setup(ctx, n)
+    bad = aos.copy()
+    bad[:, 2] = np.arange(n) + 10 * n  # event numbers past the cache
+    ctx.upload_columns([bad[:, 0], bad[:, 1], bad[:, 2]])
+    ctx.dalitz_set_data_size(top.function_idx, n)
+    M.op.sync_params()
+    ctx.update_params(M.op.p)
+    ctx.dalitz_cache(top.function_idx)
+    with pytest.raises(capi.GooFitError):
+        ctx.reduce_metric("NLL")
+    with pytest.raises(capi.GooFitError):
+        ctx.set_identity_column(2)
+
+
+def test_bad_program_rejected(ctx):
+    top, x = add1d()
+    prog, (f, p, c, o, n) = arrays(top)
+    p2 = p.copy()
+    p2[prog.nodes[1].parameters_idx] = 3  # Gaussian claims 3 parameters
+    with pytest.raises(capi.GooFitError):
+        ctx.set_program(f, p2, c, o, n)
+    f2 = f.copy()
+    f2[1] = 99
+    with pytest.raises(capi.GooFitError):
+        ctx.set_program(f2, p, c, o, n)
