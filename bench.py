@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""Benchmark of the likelihood hot path on one MI355X.
+
+Workload: the 16-resonance D0 -> pi+ pi- pi0 Dalitz model (Amp3Body with relativistic
+Breit-Wigner and Gounaris-Sakurai lineshapes, spin 0/1/2, Blatt-Weisskopf radius 1.5,
+couplings drawn from a seed), 1e6 events generated on the GPU by accept-reject on
+|A|^2, normalisation on the 1414 x 1414 (m12, m13) midpoint grid, all FP64.
+
+A step is one calculateNLL() of the GooFit API with new coupling values -- what
+MIGRAD asks for at every function call: the normalisation (host quadratic form of
+the cached Gram matrix, couplings only) and the coherent-sum NLL over the 1e6
+events in HBM.  value = events * steps / wall time of the timed steps.
+
+Also reported: the wall time of a full MIGRAD fit of the 30 free coupling
+parameters, the lineshape-cache and Gram kernels, the 1-D AddPdf and 3-D ProdPdf
+NLL rates, the roofline of the dominant kernel (coherent-sum NLL, HBM bound) and
+a CPU baseline: the oracle (the C restatement of the reference's OMP algorithm)
+on a bounded sample on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--events 1000000]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+METRIC = "Dalitz fit wall-time & NLL events/s; norm-integral error at equal MC events"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def perturb(params, names, k, scale=1e-3):
+    """Deterministic small moves of the couplings, as a minimiser makes."""
+    for i, n in enumerate(names):
+        v = params[n]
+        v.value = v.value * (1.0 + scale * math.sin(0.37 * k + 1.3 * i))
+
+
+def time_steps(pdf, params, steps, warmup, sync):
+    names = sorted(params)
+    base = {n: params[n].value for n in names}
+    for k in range(warmup):
+        perturb(params, names, k)
+        pdf.copyParams()
+        pdf.calculateNLL()
+    sync()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        perturb(params, names, warmup + k)
+        pdf.copyParams()
+        pdf.calculateNLL()
+    sync()
+    t1 = time.perf_counter()
+    for n in names:
+        params[n].value = base[n]
+    pdf.copyParams()
+    return t1 - t0
+
+
+def cpu_baseline_dalitz(flat, decay, m12, m13, target_s=10.0):
+    """Oracle (C/OpenMP restatement of the reference algorithm) NLL over the same events."""
+    import oracle as O
+    f, p, c, o, nrm = flat
+    prog = O.RawProg(f, p, c, o, nrm, decay)
+    n = len(m12)
+    aos = np.stack([m12, m13, np.arange(n, dtype=np.float64)], axis=1)
+    prog.set_total_events(n)
+    res_funcs, _ = prog.dalitz_layout(0)
+    slot0 = 16 * int(c[2])  # cResonances[16 * cacheToUse + i]
+    t0 = time.perf_counter()
+    for i, rf in enumerate(res_funcs):
+        prog.set_cache(slot0 + i, prog.dalitz_wave(0, rf, aos))
+    t_cache = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    v = prog.reduce_metric(aos, "NLL")
+    t1 = time.perf_counter() - t0
+    reps = int(min(max(math.ceil(target_s / max(t1, 1e-6)), 2), 200))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        v = prog.reduce_metric(aos, "NLL")
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=n * reps / dt, unit="events/s", cores=threads, kind="port",
+                sample=f"{reps} NLL evaluations of the {n}-event 16-resonance Dalitz sample (cached lineshapes; "
+                       f"lineshape cache build {t_cache:.2f} s not included)",
+                nll=v, cache_build_s=t_cache)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--events", type=int, default=1_000_000)
+    ap.add_argument("--grid", type=int, default=1414)
+    ap.add_argument("--no-fit", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the path shards by events across GPUs "
+              f"but this build runs on a single MI355X.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import goofit_amd as G
+    from goofit_amd import workloads as W
+
+    G.require_api()
+    sync = torch.cuda.synchronize
+
+    # ---------------------------------------------------------------- Dalitz
+    log("building the 16-resonance Dalitz model")
+    model = W.build_dalitz(nbins=args.grid)
+    pdf = model.pdf
+    t0 = time.perf_counter()
+    m12, m13 = W.generate_dalitz(model, args.events, seed=6)
+    t_gen = time.perf_counter() - t0
+    log(f"generated {len(m12)} events on the GPU in {t_gen:.1f} s")
+    data = W.dalitz_dataset(model, m12, m13)
+    pdf.setData(data)
+    pdf.setDataSize(len(m12))
+    pdf.setForceIntegrals()
+    sync()
+    t0 = time.perf_counter()
+    nll0 = pdf.calculateNLL()  # lineshape cache + Gram on the grid + NLL
+    sync()
+    t_first = time.perf_counter() - t0
+    log(f"first NLL {nll0:.6f} ({t_first * 1e3:.1f} ms incl. cache and Gram)")
+
+    dt = time_steps(pdf, model.params, args.steps, args.warmup, sync)
+    ms_per_step = dt / args.steps * 1e3
+    value = args.events * args.steps / dt
+    log(f"{args.steps} NLL steps: {ms_per_step:.4f} ms/step, {value:.4e} events/s")
+
+    # dominant kernel: the fused coherent-sum NLL, timed with HIP events on its stream
+    reps = 200
+    ms_kernel = G.engine.time_reduce_metric(1, reps) / reps
+    bytes_per_event = 16 * 16 + 16  # 16 complex waves + (m12, m13); event number = row index
+    achieved = bytes_per_event * args.events / (ms_kernel * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="k_reduce_metric<2> (Dalitz coherent-sum NLL)", kernel_ms=round(ms_kernel, 5),
+                    bytes_per_event=bytes_per_event)
+    log(f"NLL kernel {ms_kernel * 1e3:.1f} us -> {achieved:.0f} GB/s")
+
+    extra = dict(events=len(m12), generation_s=round(t_gen, 2), first_nll_ms=round(t_first * 1e3, 2), nll=nll0)
+    grid6 = [0.0, 3.0, float(args.grid), 0.0, 3.0, float(args.grid)]
+    ms_gram = G.engine.time_dalitz_gram(pdf, grid6, 3) / 3
+    cells = args.grid * args.grid
+    extra["gram_ms"] = round(ms_gram, 3)
+    extra["gram_cells"] = cells
+    extra["gram_fp64_tflops_mfma"] = round(3 * 2 * 16 * 16 * 4 * (cells / 4) / (ms_gram * 1e-3) / 1e12, 3)
+    ms_cache = G.engine.time_dalitz_cache(pdf, 5) / 5
+    extra["cache_ms"] = round(ms_cache, 3)
+    extra["cache_lineshapes_per_s"] = round(16 * args.events / (ms_cache * 1e-3), 1)
+
+    nll_ref = pdf.calculateNLL()  # current parameters and normalisation for the CPU comparison
+    flat = G.engine.flattened()
+
+    fit = None
+    if not args.no_fit:
+        names = sorted(model.params)
+        truth = {n: model.params[n].value for n in names}
+        rng = np.random.default_rng(11)
+        for n in names:
+            model.params[n].value = truth[n] * (1.0 + 0.05 * rng.standard_normal())
+        fm = G.FitManager(pdf)
+        fm.setVerbosity(0)
+        sync()
+        t0 = time.perf_counter()
+        m = fm.fit()
+        sync()
+        t_fit = time.perf_counter() - t0
+        fit = dict(wall_s=round(t_fit, 3), nfcn=m.NFcn(), valid=bool(m.IsValid()), fval=m.Fval(), edm=m.Edm(),
+                   free_params=len(names))
+        log(f"MIGRAD fit: {t_fit:.2f} s, {m.NFcn()} calls, valid={m.IsValid()}")
+        for n in names:
+            model.params[n].value = truth[n]
+        pdf.copyParams()
+
+    workloads = {}
+    if not args.no_extra:
+        for name, builder, sampler, nevt, bpe in (
+                ("add1d_1e5", W.build_add1d, lambda n: [W.sample_add1d(n, seed=1)], 100_000, 8),
+                ("prod3d_1e7", W.build_prod3d, lambda n: list(W.sample_prod3d(n, seed=2)), 10_000_000, 24)):
+            mdl = builder()
+            cols = sampler(nevt)
+            d = G.UnbinnedDataSet(mdl.observables)
+            d.from_columns(cols)
+            mdl.pdf.setData(d)
+            mdl.pdf.calculateNLL()
+            dtw = time_steps(mdl.pdf, mdl.params, max(args.steps, 20), 3, sync)
+            k_ms = G.engine.time_reduce_metric(1, 100) / 100
+            workloads[name] = dict(events=nevt, nll_events_per_s=round(nevt * max(args.steps, 20) / dtw, 1),
+                                   ms_per_step=round(dtw / max(args.steps, 20) * 1e3, 4), kernel_ms=round(k_ms, 5),
+                                   hbm_gbs=round(bpe * nevt / (k_ms * 1e-3) / 1e9, 1))
+            log(name, workloads[name])
+            del d
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline (oracle, OpenMP) on the same events")
+        decay = list(W.DALITZ_DECAY)
+        cpu = cpu_baseline_dalitz(flat, decay, m12, m13)
+        rel = abs(cpu["nll"] - nll_ref) / abs(nll_ref)
+        cpu["nll_rel_diff_vs_gpu"] = rel
+        log(f"CPU baseline {cpu['value']:.3e} events/s on {cpu['cores']} threads; NLL rel diff {rel:.2e}")
+        cpu.pop("nll")
+
+    line = dict(metric=METRIC, value=round(value, 1), unit="events/s", n_gpus=1, steps=args.steps,
+                warmup=args.warmup, ms_per_step=round(ms_per_step, 5), higher_is_better=True, scaling="weak",
+                vs_baseline=None, dtype="f64", data="synthetic (GPU accept-reject on the model)",
+                config=dict(workload="dalitz16_d0_pipipi0", events=args.events, norm_grid=f"{args.grid}x{args.grid}",
+                            resonances=16, free_params=30, step="calculateNLL with new couplings"),
+                roofline=roofline, cpu_baseline=cpu, fit=fit, dalitz=extra, workloads=workloads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

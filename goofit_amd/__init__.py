@@ -1,0 +1,39 @@
+"""goofit_amd -- GooFit's likelihood hot path on AMD Instinct MI355X (gfx950).
+
+The GooFit C++ API (GooPdf, AddPdf, ProdPdf, Amp3Body, UnbinnedDataSet,
+FitManager, ...) is built as libgoofit.so on top of the C ABI of
+include/goofit_hip.h (libgoofit_hip.so, hand-written HIP kernels).  This package
+exposes that API to Python (the `_goofit` binding) and the raw C ABI (`capi`).
+
+Importing the API requires the native libraries to have been built in-tree
+(`python -m goofit_amd.build`); there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+try:
+    from . import _goofit  # noqa: F401
+    from ._goofit import (  # noqa: F401
+        AddPdf, Amp3Body, DataSet, DaughterPair, DecayInfo3, EvalFunc, EventNumber, ExpGausPdf, ExpPdf,
+        FitManager, FunctionMinimum, GaussianPdf, GeneralError, GooPdf, Observable, PAIR_12, PAIR_13, PAIR_23,
+        PdfBase, PolynomialPdf, ProdPdf, Resonances, UnbinnedDataSet, Variable, engine, inDalitz,
+    )
+    HAVE_API = True
+except ImportError as _e:  # the binding is not built yet
+    HAVE_API = False
+    _IMPORT_ERROR = _e
+
+
+def require_api():
+    """Raise if the compiled API is missing (the product never falls back to Python)."""
+    if not HAVE_API:
+        raise ImportError(f"goofit_amd native API not built ({_IMPORT_ERROR}); run `python -m goofit_amd.build`")
+
+
+def native_libraries():
+    """Paths of the in-tree native libraries this package loads."""
+    lib = os.path.join(_HERE, "lib")
+    return [os.path.join(lib, "libgoofit_hip.so"), os.path.join(lib, "libgoofit.so")]

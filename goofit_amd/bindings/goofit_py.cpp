@@ -1,0 +1,341 @@
+// Python binding of the GooFit C++ API (the reference ships the same classes as the
+// pybind11 module `goofit`, python/goofit.cpp and python/PDFs/**).  Everything here
+// forwards to libgoofit.so; there is no numerics in this file.
+#include <pybind11/complex.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <sstream>
+
+#include <Minuit2/Minuit2.h>
+#include <goofit/Engine.h>
+#include <goofit/FitManager.h>
+#include <goofit/PDFs/GooPdf.h>
+#include <goofit/PDFs/basic/ExpGausPdf.h>
+#include <goofit/PDFs/basic/ExpPdf.h>
+#include <goofit/PDFs/basic/GaussianPdf.h>
+#include <goofit/PDFs/basic/PolynomialPdf.h>
+#include <goofit/PDFs/combine/AddPdf.h>
+#include <goofit/PDFs/combine/ProdPdf.h>
+#include <goofit/PDFs/physics/Amp3Body.h>
+#include <goofit/PDFs/physics/ResonancePdf.h>
+#include <goofit/UnbinnedDataSet.h>
+#include <goofit/Variable.h>
+
+namespace py = pybind11;
+using namespace GooFit;
+namespace M2 = ROOT::Minuit2;
+
+namespace {
+
+template <class T>
+std::string repr_of(const T &v) {
+    std::ostringstream os;
+    os << v;
+    return os.str();
+}
+
+py::array_t<double> to_numpy(const std::vector<double> &v) {
+    py::array_t<double> a(v.size());
+    std::copy(v.begin(), v.end(), a.mutable_data());
+    return a;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_goofit, m) {
+    m.doc() = "GooFit API on the MI355X likelihood engine";
+
+    py::register_exception<GeneralError>(m, "GeneralError", PyExc_RuntimeError);
+
+    // ------------------------------------------------------------ variables
+    py::class_<Indexable>(m, "Indexable")
+        .def_property_readonly("name", &Indexable::getName)
+        .def_property("value", &Indexable::getValue, &Indexable::setValue)
+        .def_property("lowerlimit", &Indexable::getLowerLimit, &Indexable::setLowerLimit)
+        .def_property("upperlimit", &Indexable::getUpperLimit, &Indexable::setUpperLimit)
+        .def_property("index", &Indexable::getIndex, &Indexable::setIndex)
+        .def("getValue", &Indexable::getValue)
+        .def("setValue", &Indexable::setValue)
+        .def("getName", &Indexable::getName)
+        .def("__bool__", [](const Indexable &v) { return (bool)v; })
+        .def("__float__", [](const Indexable &v) { return v.getValue(); })
+        .def("__eq__", [](const Indexable &a, const Indexable &b) { return a == b; })
+        .def("__hash__", [](const Indexable &a) { return std::hash<std::string>()(a.getName()); });
+
+    py::class_<Observable, Indexable>(m, "Observable")
+        .def(py::init<std::string, fptype, fptype>(), py::arg("name"), py::arg("min"), py::arg("max"))
+        .def_property("numbins", &Observable::getNumBins, &Observable::setNumBins)
+        .def("getNumBins", &Observable::getNumBins)
+        .def("setNumBins", &Observable::setNumBins)
+        .def("getBinSize", &Observable::getBinSize)
+        .def("isEventNumber", &Observable::isEventNumber)
+        .def("__repr__", [](const Observable &o) { return repr_of(o); });
+
+    py::class_<EventNumber, Observable>(m, "EventNumber")
+        .def(py::init<std::string>(), py::arg("name"))
+        .def(py::init<std::string, fptype, fptype>(), py::arg("name"), py::arg("min"), py::arg("max"));
+
+    py::class_<Variable, Indexable>(m, "Variable")
+        .def(py::init<std::string, fptype>(), py::arg("name"), py::arg("value"))
+        .def(py::init<std::string, fptype, fptype>(), py::arg("name"), py::arg("value"), py::arg("error"))
+        .def(py::init<std::string, fptype, fptype, fptype>(), py::arg("name"), py::arg("value"), py::arg("min"),
+             py::arg("max"))
+        .def(py::init<std::string, fptype, fptype, fptype, fptype, bool>(), py::arg("name"), py::arg("value"),
+             py::arg("error"), py::arg("min"), py::arg("max"), py::arg("fixed") = false)
+        .def_property("error", &Variable::getError, &Variable::setError)
+        .def_property("fixed", &Variable::IsFixed, &Variable::setFixed)
+        .def_property("fitterindex", &Variable::getFitterIndex, &Variable::setFitterIndex)
+        .def("getError", &Variable::getError)
+        .def("setBlind", &Variable::setBlind)
+        .def("__repr__", [](const Variable &v) { return repr_of(v); });
+
+    // ------------------------------------------------------------ data
+    py::class_<DataSet>(m, "DataSet")
+        .def("getNumEvents", &DataSet::getNumEvents)
+        .def("getObservables", &DataSet::getObservables)
+        .def("addEvent", [](DataSet &d) { d.addEvent(); })
+        .def("getName", &DataSet::getName);
+
+    py::class_<UnbinnedDataSet, DataSet>(m, "UnbinnedDataSet")
+        .def(py::init<const Observable &, std::string>(), py::arg("var"), py::arg("name") = "")
+        .def(py::init<const std::vector<Observable> &, std::string>(), py::arg("vars"), py::arg("name") = "")
+        .def(py::init([](py::args args) {
+            std::vector<Observable> obs;
+            for(auto a : args)
+                obs.push_back(a.cast<Observable>());
+            return new UnbinnedDataSet(obs);
+        }))
+        .def("getValue", &UnbinnedDataSet::getValue)
+        .def("loadEvent", &UnbinnedDataSet::loadEvent)
+        .def("fillWithGrid", &UnbinnedDataSet::fillWithGrid)
+        .def("clear", &UnbinnedDataSet::clear)
+        .def("setValueForAllEvents", &UnbinnedDataSet::setValueForAllEvents)
+        .def("column", [](const UnbinnedDataSet &d, const Observable &o) { return to_numpy(d.getColumn(o)); })
+        .def("from_columns",
+             [](UnbinnedDataSet &d, std::vector<py::array_t<double, py::array::c_style | py::array::forcecast>> cols) {
+                 std::vector<const double *> ptrs;
+                 size_t n = cols.empty() ? 0 : (size_t)cols[0].size();
+                 for(auto &c : cols) {
+                     if((size_t)c.size() != n)
+                         throw GeneralError("from_columns: columns of different lengths");
+                     ptrs.push_back(c.data());
+                 }
+                 d.appendColumns(ptrs, n);
+             })
+        .def("__len__", &UnbinnedDataSet::getNumEvents);
+
+    // ------------------------------------------------------------ PDFs
+    py::enum_<EvalFunc>(m, "EvalFunc")
+        .value("Eval", EvalFunc::Eval)
+        .value("NLL", EvalFunc::NLL)
+        .value("Prob", EvalFunc::Prob)
+        .value("BinAvg", EvalFunc::BinAvg)
+        .value("BinWithError", EvalFunc::BinWithError)
+        .value("Chisq", EvalFunc::Chisq);
+
+    py::class_<PdfBase>(m, "PdfBase")
+        .def("getName", &PdfBase::getName)
+        .def("getParameters", &PdfBase::getParameters)
+        .def("getObservables", &PdfBase::getObservables)
+        .def("setData", &PdfBase::setData, py::keep_alive<1, 2>())
+        .def("getData", &PdfBase::getData, py::return_value_policy::reference)
+        .def("normalize", &PdfBase::normalize)
+        .def("calculateNLL", &PdfBase::calculateNLL)
+        .def("copyParams", &PdfBase::copyParams)
+        .def("setIntegrationFineness", &PdfBase::setIntegrationFineness)
+        .def("getFunctionIndex", &PdfBase::getFunctionIndex)
+        .def("getParameterIndex", &PdfBase::getParameterIndex)
+        .def("getNormalization", &PdfBase::getNormalization)
+        .def("setupObservables", &PdfBase::setupObservables)
+        .def("setIndices", &PdfBase::setIndices)
+        .def("SigGenSetIndices", &PdfBase::SigGenSetIndices)
+        .def("setCommonNorm", &PdfBase::setCommonNorm, py::arg("v") = true)
+        .def("setSeparateNorm", &PdfBase::setSeparateNorm, py::arg("v") = true)
+        .def("fillMCDataSimple", &PdfBase::fillMCDataSimple, py::arg("events"), py::arg("seed") = 0)
+        .def("fitTo", &PdfBase::fitTo, py::arg("data"), py::arg("verbosity") = 3, py::keep_alive<1, 2>())
+        .def("fit", &PdfBase::fit, py::arg("verbosity") = 3)
+        .def("__repr__", [](const PdfBase &p) { return repr_of(p); });
+
+    py::class_<GooPdf, PdfBase>(m, "GooPdf")
+        .def("integrate", &GooPdf::integrate)
+        .def("hasAnalyticIntegral", &GooPdf::hasAnalyticIntegral)
+        .def("getValue", &GooPdf::getValue, py::arg("evalfunc") = EvalFunc::Eval)
+        .def("evaluateAtPoints", &GooPdf::evaluateAtPoints)
+        .def("getCompProbsAtDataPoints", &GooPdf::getCompProbsAtDataPoints)
+        .def("evaluateAtData",
+             [](GooPdf &p, EvalFunc f) { return to_numpy(p.evaluateAtData(f)); }, py::arg("evalfunc") = EvalFunc::Prob)
+        .def("makeGrid", &GooPdf::makeGrid)
+        .def("reduce_with_bins", &GooPdf::reduce_with_bins)
+        .def("get_bin_grid_size", &GooPdf::get_bin_grid_size)
+        .def("get_bin_grid_volume", &GooPdf::get_bin_grid_volume)
+        .def("setParameterConstantness", &GooPdf::setParameterConstantness, py::arg("constant") = true)
+        .def("scan", [](GooPdf &p, Observable v) {
+            std::vector<fptype> vals;
+            p.scan(v, vals);
+            return vals;
+        });
+
+    py::class_<GaussianPdf, GooPdf>(m, "GaussianPdf")
+        .def(py::init<std::string, Observable, Variable, Variable>(), py::arg("name"), py::arg("x"), py::arg("mean"),
+             py::arg("sigma"));
+    py::class_<ExpPdf, GooPdf>(m, "ExpPdf")
+        .def(py::init<std::string, Observable, Variable>(), py::arg("name"), py::arg("x"), py::arg("alpha"))
+        .def(py::init<std::string, Observable, Variable, Variable>(), py::arg("name"), py::arg("x"), py::arg("alpha"),
+             py::arg("offset"))
+        .def(py::init([](std::string n, Observable x, std::vector<Variable> w) { return new ExpPdf(n, x, w); }),
+             py::arg("name"), py::arg("x"), py::arg("weights"))
+        .def(py::init([](std::string n, Observable x, std::vector<Variable> w, Variable off) {
+                 return new ExpPdf(n, x, w, off);
+             }),
+             py::arg("name"), py::arg("x"), py::arg("weights"), py::arg("offset"));
+    py::class_<ExpGausPdf, GooPdf>(m, "ExpGausPdf")
+        .def(py::init<std::string, Observable, Variable, Variable, Variable>(), py::arg("name"), py::arg("x"),
+             py::arg("mean"), py::arg("sigma"), py::arg("tau"));
+    py::class_<PolynomialPdf, GooPdf>(m, "PolynomialPdf")
+        .def(py::init<std::string, Observable, std::vector<Variable>, unsigned int>(), py::arg("name"), py::arg("x"),
+             py::arg("weights"), py::arg("lowestDegree") = 0)
+        .def(py::init<std::string, Observable, std::vector<Variable>, Variable, unsigned int>(), py::arg("name"),
+             py::arg("x"), py::arg("weights"), py::arg("x0"), py::arg("lowestDegree") = 0)
+        .def(py::init<std::string, std::vector<Observable>, std::vector<Variable>, std::vector<Variable>,
+                      unsigned int>(),
+             py::arg("name"), py::arg("obses"), py::arg("coeffs"), py::arg("offsets"), py::arg("maxDegree") = 0);
+
+    py::class_<CombinePdf, GooPdf>(m, "CombinePdf");
+    py::class_<AddPdf, CombinePdf>(m, "AddPdf")
+        .def(py::init<std::string, std::vector<Variable>, std::vector<PdfBase *>>(), py::arg("name"),
+             py::arg("weights"), py::arg("comps"), py::keep_alive<1, 4>())
+        .def(py::init<std::string, Variable, PdfBase *, PdfBase *>(), py::arg("name"), py::arg("frac"),
+             py::arg("func1"), py::arg("func2"), py::keep_alive<1, 4>(), py::keep_alive<1, 5>());
+    py::class_<ProdPdf, CombinePdf>(m, "ProdPdf")
+        .def(py::init<std::string, std::vector<PdfBase *>>(), py::arg("name"), py::arg("comps"),
+             py::keep_alive<1, 3>());
+
+    // ------------------------------------------------------------ Dalitz
+    py::enum_<DaughterPair>(m, "DaughterPair")
+        .value("PAIR_12", PAIR_12)
+        .value("PAIR_13", PAIR_13)
+        .value("PAIR_23", PAIR_23)
+        .export_values();
+
+    py::class_<AmpComponent, GooPdf>(m, "AmpComponent");
+    py::class_<ResonancePdf, AmpComponent>(m, "ResonancePdf")
+        .def("get_amp_real", &ResonancePdf::get_amp_real)
+        .def("get_amp_img", &ResonancePdf::get_amp_img);
+    auto mres = m.def_submodule("Resonances");
+    py::class_<Resonances::RBW, ResonancePdf>(mres, "RBW")
+        .def(py::init<std::string, Variable, Variable, Variable, Variable, unsigned int, unsigned int, bool, bool>(),
+             py::arg("name"), py::arg("ar"), py::arg("ai"), py::arg("mass"), py::arg("width"), py::arg("sp"),
+             py::arg("cyc"), py::arg("norm") = true, py::arg("sym") = false);
+    py::class_<Resonances::GS, ResonancePdf>(mres, "GS")
+        .def(py::init<std::string, Variable, Variable, Variable, Variable, unsigned int, unsigned int, bool, bool>(),
+             py::arg("name"), py::arg("ar"), py::arg("ai"), py::arg("mass"), py::arg("width"), py::arg("sp"),
+             py::arg("cyc"), py::arg("norm") = true, py::arg("sym") = false);
+    py::class_<Resonances::NonRes, ResonancePdf>(mres, "NonRes")
+        .def(py::init<std::string, Variable, Variable>(), py::arg("name"), py::arg("ar"), py::arg("ai"));
+
+    py::class_<DecayInfo3>(m, "DecayInfo3", py::dynamic_attr())
+        .def(py::init<>())
+        .def_readwrite("motherMass", &DecayInfo3::motherMass)
+        .def_readwrite("daug1Mass", &DecayInfo3::daug1Mass)
+        .def_readwrite("daug2Mass", &DecayInfo3::daug2Mass)
+        .def_readwrite("daug3Mass", &DecayInfo3::daug3Mass)
+        .def_readwrite("meson_radius", &DecayInfo3::meson_radius)
+        .def_readwrite("mother_meson_radius", &DecayInfo3::mother_meson_radius)
+        .def_property(
+            "resonances", [](DecayInfo3 &d) { return d.resonances; },
+            [](py::object self, py::list res) {
+                DecayInfo3 &d = self.cast<DecayInfo3 &>();
+                d.resonances.clear();
+                for(auto r : res)
+                    d.resonances.push_back(r.cast<ResonancePdf *>());
+                self.attr("_resonance_refs") = res;  // keep the Python objects alive
+            });
+
+    py::class_<Amp3Body, CombinePdf>(m, "Amp3Body")
+        .def(py::init<std::string, Observable, Observable, EventNumber, DecayInfo3, GooPdf *>(), py::arg("name"),
+             py::arg("m12"), py::arg("m13"), py::arg("eventNumber"), py::arg("decay"), py::arg("eff"),
+             py::keep_alive<1, 6>(), py::keep_alive<1, 7>())
+        .def("setDataSize", &Amp3Body::setDataSize, py::arg("dataSize"), py::arg("evtSize") = 3, py::arg("offset") = 0)
+        .def("setForceIntegrals", &Amp3Body::setForceIntegrals, py::arg("f") = true)
+        .def("getCachedWave", &Amp3Body::getCachedWave)
+        .def("sumCachedWave", &Amp3Body::sumCachedWave)
+        .def("fit_fractions", &Amp3Body::fit_fractions)
+        .def("getIntegrals", &Amp3Body::getIntegrals)
+        .def("getEfficiencyIndex", &Amp3Body::getEfficiencyIndex)
+        .def_static("resetCacheCounter", &Amp3Body::resetCacheCounter);
+
+    m.def("inDalitz", &inDalitz);
+
+    // ------------------------------------------------------------ fitting
+    py::class_<M2::FunctionMinimum>(m, "FunctionMinimum")
+        .def("IsValid", &M2::FunctionMinimum::IsValid)
+        .def("HasCovariance", &M2::FunctionMinimum::HasCovariance)
+        .def("HasAccurateCovar", &M2::FunctionMinimum::HasAccurateCovar)
+        .def("IsAboveMaxEdm", &M2::FunctionMinimum::IsAboveMaxEdm)
+        .def("HasReachedCallLimit", &M2::FunctionMinimum::HasReachedCallLimit)
+        .def("Fval", &M2::FunctionMinimum::Fval)
+        .def("Edm", &M2::FunctionMinimum::Edm)
+        .def("NFcn", &M2::FunctionMinimum::NFcn)
+        .def("values", [](const M2::FunctionMinimum &f) { return f.UserParameters().Params(); })
+        .def("errors", [](const M2::FunctionMinimum &f) { return f.UserParameters().Errors(); })
+        .def("names",
+             [](const M2::FunctionMinimum &f) {
+                 std::vector<std::string> n;
+                 for(auto &p : f.UserParameters().Parameters())
+                     n.push_back(p.name);
+                 return n;
+             })
+        .def("covariance", [](const M2::FunctionMinimum &f) { return f.UserState().cov_; })
+        .def("__repr__", [](const M2::FunctionMinimum &f) { return repr_of(f); });
+
+    py::class_<FitManagerMinuit2>(m, "FitManager")
+        .def(py::init<PdfBase *>(), py::arg("pdf"), py::keep_alive<1, 2>())
+        .def("fit", &FitManagerMinuit2::fit)
+        .def("setMaxCalls", &FitManagerMinuit2::setMaxCalls, py::arg("max_calls") = 0)
+        .def("setVerbosity", &FitManagerMinuit2::setVerbosity)
+        .def("getVerbosity", &FitManagerMinuit2::getVerbosity)
+        .def("setMinos", &FitManagerMinuit2::setMinos, py::arg("minos_flag") = true)
+        .def("getMinosErrors", &FitManagerMinuit2::getMinosErrors)
+        .def("setStrategy", &FitManagerMinuit2::setStrategy)
+        .def("setTolerance", &FitManagerMinuit2::setTolerance)
+        .def("getLastFitTime", &FitManagerMinuit2::getLastFitTime)
+        .def("__bool__", [](const FitManagerMinuit2 &f) { return (bool)f; })
+        .def("__int__", [](const FitManagerMinuit2 &f) { return (int)f; });
+
+    // ------------------------------------------------------------ engine state
+    auto me = m.def_submodule("engine");
+    me.def("set_device", &engine::set_device);
+    me.def("get_device", &engine::get_device);
+    me.def("release", &engine::release);
+    me.def("flush", &engine::flush);
+    me.def("callnumber", []() { return host_callnumber; });
+    me.def("flattened", []() {
+        return py::make_tuple(host_function_table, host_parameters, host_constants, host_observables,
+                              host_normalizations);
+    });
+    me.def("function_names", []() { return host_function_name; });
+    // device time of back-to-back launches on the engine's stream (HIP events), ms
+    me.def("time_reduce_metric", [](int metric, int reps) {
+        engine::flush();
+        double ms = 0;
+        engine::check(gf_time_reduce_metric(engine::ctx(), metric, reps, &ms), "gf_time_reduce_metric");
+        return ms;
+    });
+    me.def("time_dalitz_cache", [](const Amp3Body &p, int reps) {
+        engine::flush();
+        double ms = 0;
+        engine::check(gf_time_dalitz_cache(engine::ctx(), (int)p.getFunctionIndex(), reps, &ms),
+                      "gf_time_dalitz_cache");
+        return ms;
+    });
+    me.def("time_dalitz_gram", [](const Amp3Body &p, std::vector<double> grid6, int reps) {
+        engine::flush();
+        double ms = 0;
+        engine::check(gf_time_dalitz_gram(engine::ctx(), (int)p.getFunctionIndex(), grid6.data(),
+                                          (int)p.getEfficiencyIndex(), reps, &ms),
+                      "gf_time_dalitz_gram");
+        return ms;
+    });
+}

@@ -1,0 +1,131 @@
+// CombinePdfs.cpp -- AddPdf and ProdPdf (src/PDFs/combine/{AddPdf,ProdPdf}.cu of
+// the reference): construction and the host side of normalize()/calculateNLL().
+#include <goofit/Error.h>
+#include <goofit/PDFs/combine/AddPdf.h>
+#include <goofit/PDFs/combine/ProdPdf.h>
+
+#include <algorithm>
+#include <cmath>
+
+namespace GooFit {
+
+// ---------------------------------------------------------------- AddPdf
+
+AddPdf::AddPdf(std::string n, std::vector<Variable> weights, std::vector<PdfBase *> comps)
+    : CombinePdf("AddPdf", n)
+    , extended(true) {
+    if(weights.size() != comps.size() && (weights.size() + 1) != comps.size())
+        throw GeneralError("Size of weights {} (+1) != comps {}", weights.size(), comps.size());
+    for(PdfBase *p : comps) {
+        if(p == nullptr)
+            throw GeneralError("Invalid component");
+        components.push_back(p);
+    }
+    observablesList = getObservables();
+    for(auto &w : weights)
+        registerParameter(w);
+    if(weights.size() < components.size())
+        extended = false;
+    if(extended)
+        registerFunction("ptr_to_AddPdfsExt", GF_FN_ADD_EXT);
+    else
+        registerFunction("ptr_to_AddPdfs", GF_FN_ADD);
+    initialize();
+}
+
+AddPdf::AddPdf(std::string n, Variable frac1, PdfBase *func1, PdfBase *func2)
+    : CombinePdf("AddPdf", n, frac1)
+    , extended(false) {
+    if(!func1 || !func2)
+        throw GeneralError("Invalid component");
+    components.push_back(func1);
+    components.push_back(func2);
+    observablesList = getObservables();
+    registerFunction("ptr_to_AddPdfs", GF_FN_ADD);
+    initialize();
+}
+
+// AddPdf::normalize, src/PDFs/combine/AddPdf.cu:143-185: components normalise
+// separately; the sum itself keeps normalisation 1.
+fptype AddPdf::normalize() {
+    fptype ret         = 0;
+    fptype totalWeight = 0;
+    for(size_t i = 0; i + 1 < components.size(); ++i) {
+        fptype weight = parametersList[i].getValue();
+        totalWeight += weight;
+        fptype curr = components[i]->normalize();
+        ret += curr * weight;
+    }
+    fptype last = components.back()->normalize();
+    if(extended) {
+        fptype lastWeight = parametersList[components.size() - 1].getValue();
+        totalWeight += lastWeight;
+        ret += last * lastWeight;
+        ret /= totalWeight;
+    } else {
+        ret += (1 - totalWeight) * last;
+    }
+    setNormalization(1.0);
+    if(getCommonNorm())
+        for(auto *c : components)
+            c->setNormalization(1.0 / ret);
+    return ret;
+}
+
+// AddPdf::calculateNLL, AddPdf.cu:187-204: extended likelihood adds E - N log E.
+double AddPdf::calculateNLL() {
+    double ret = GooPdf::calculateNLL();
+    if(extended) {
+        fptype expEvents = 0;
+        for(size_t i = 0; i < components.size(); ++i)
+            expEvents += parametersList[i].getValue();
+        ret += (expEvents - numEvents * std::log(expEvents));
+    }
+    return ret;
+}
+
+// ---------------------------------------------------------------- ProdPdf
+
+ProdPdf::ProdPdf(std::string n, std::vector<PdfBase *> comps)
+    : CombinePdf("ProdPdf", n)
+    , varOverlaps(false) {
+    for(PdfBase *p : comps)
+        components.push_back(p);
+    observablesList = getObservables();
+    registerConstant((fptype)components.size());
+    // components sharing an observable cannot be normalised one by one
+    // (ProdPdf.cu:44-68: each component is compared with the one before it)
+    std::vector<Observable> observableCheck;
+    for(PdfBase *p : comps) {
+        if(varOverlaps)
+            continue;
+        std::vector<Observable> curr = p->getObservables();
+        for(Observable &o : curr) {
+            if(std::find(observableCheck.begin(), observableCheck.end(), o) == observableCheck.end())
+                continue;
+            varOverlaps = true;
+            break;
+        }
+        observableCheck = p->getObservables();
+    }
+    if(varOverlaps)
+        for(PdfBase *p : comps)
+            if(p->getSeparateNorm())
+                varOverlaps = false;
+    registerFunction("ptr_to_ProdPdfs", GF_FN_PROD);
+    initialize();
+}
+
+// ProdPdf::normalize, ProdPdf.cu:75-99
+fptype ProdPdf::normalize() {
+    if(varOverlaps) {
+        recursiveSetNormalization(1.0);
+        return GooPdf::normalize();
+    }
+    for(PdfBase *c : components)
+        c->normalize();
+    setNormalization(1.0);
+    return 1.0;
+}
+
+}  // namespace GooFit

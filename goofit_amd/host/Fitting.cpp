@@ -1,0 +1,156 @@
+// Fitting.cpp -- FCN, Params and FitManagerMinuit2 (src/goofit/{FCN,Params,
+// FitManagerMinuit2}.cpp of the reference) on the Minuit2-compatible minimiser.
+#include <goofit/Error.h>
+#include <goofit/PdfBase.h>
+#include <goofit/fitting/FCN.h>
+#include <goofit/fitting/FitManagerMinuit2.h>
+#include <goofit/fitting/Params.h>
+
+#include <Minuit2/FunctionMinimum.h>
+#include <Minuit2/MnMigrad.h>
+#include <Minuit2/MnMinos.h>
+#include <Minuit2/MnPrint.h>
+
+#include <chrono>
+#include <iostream>
+
+namespace GooFit {
+
+// ---------------------------------------------------------------- Params
+
+Params::Params(PdfBase &pdf)
+    : pdf_(&pdf) {
+    vars_ = pdf_->getParameters();
+    for(Variable &var : vars_) {
+        bool added;
+        if(var.IsFixed())
+            added = Add(var.getName(), var.getValue());
+        else if(var.getLowerLimit() == var.getUpperLimit())
+            added = Add(var.getName(), var.getValue(), var.getError());
+        else
+            added = Add(var.getName(), var.getValue(), var.getError(), var.getLowerLimit(), var.getUpperLimit());
+        if(!added)
+            throw std::runtime_error("The name " + var.getName() + " appears more than once!");
+        var.setFitterIndex((int)Index(var.getName()));
+    }
+    num_ = vars_.size();
+}
+
+void Params::SetGooFitParams(const Minuit2::MnUserParameterState &input) {
+    for(Variable &var : vars_) {
+        int k = var.getFitterIndex();
+        var.setValue(input.Value(k));
+        var.setError(input.Error(k));
+        SetValue(k, var.getValue());
+        SetError(k, var.getError());
+    }
+}
+
+std::vector<double> Params::make_minuit_vector() const {
+    std::vector<double> v(max_fitter_index(vars_) + 1);
+    for(const Variable &var : vars_)
+        v.at(var.getFitterIndex()) = var.getValue();
+    return v;
+}
+
+// Params::from_minuit_vector, src/goofit/Params.cpp:47-57
+void Params::from_minuit_vector(const std::vector<double> &values, bool force_changed) {
+    for(Variable &var : vars_) {
+        var.setChanged(force_changed ? true : var.getValue() != values.at(var.getFitterIndex()));
+        pdf_->updateVariable(var, values.at(var.getFitterIndex()) - var.getBlind(Variable::Key()));
+    }
+    if(do_record_)
+        recorded_.push_back(values);
+    pdf_->updateParameters();
+}
+
+// ---------------------------------------------------------------- FCN
+
+FCN::FCN(Params &params)
+    : params_(&params) {
+    host_callnumber = 0;
+    for(Variable &var : params_->vars_)
+        var.setChanged(true);
+}
+
+double FCN::operator()(const std::vector<double> &pars) const {
+    params_->from_minuit_vector(pars);
+    double nll = params_->pdf_->calculateNLL();
+    host_callnumber++;
+    return nll;
+}
+
+double FCN::operator()() const {
+    std::vector<double> pars = params_->make_minuit_vector();
+    params_->from_minuit_vector(pars, true);
+    double nll = params_->pdf_->calculateNLL();
+    host_callnumber++;
+    return nll;
+}
+
+Params *FCN::GetParams() { return params_; }
+
+// ---------------------------------------------------------------- FitManagerMinuit2
+
+FitManagerMinuit2::FitManagerMinuit2(PdfBase *dat)
+    : upar_(*dat)
+    , fcn_(upar_) {}
+
+// FitManagerMinuit2::fit, src/goofit/FitManagerMinuit2.cpp:31-119
+Minuit2::FunctionMinimum FitManagerMinuit2::fit() {
+    int old_level   = Minuit2::MnPrint::SetGlobalLevel(verbosity > 1 ? verbosity - 1 : 0);
+    host_callnumber = 0;
+    auto t0         = std::chrono::steady_clock::now();
+    Minuit2::MnStrategy strat((unsigned int)m_strategy);
+    Minuit2::MnMigrad migrad{fcn_, upar_, strat};
+    Minuit2::FunctionMinimum min = migrad(maxfcn_, tolerance_);
+    if(minos) {
+        Minuit2::MnMinos mns{fcn_, min, strat};
+        for(Variable &var : upar_.GetGooFitParams()) {
+            if(var.IsFixed())
+                continue;
+            minos_errors.push_back(mns((unsigned int)var.getFitterIndex()));
+        }
+        if(verbosity > 0) {
+            std::cout << "1-sigma minos errors: " << std::endl;
+            size_t k = 0;
+            for(Variable &var : upar_.GetGooFitParams()) {
+                if(var.IsFixed())
+                    continue;
+                std::cout << var.getName() << ": " << minos_errors[k].first << " " << minos_errors[k].second
+                          << std::endl;
+                k++;
+            }
+        }
+    }
+    last_time_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if(verbosity > 0) {
+        std::cout << min;
+        std::cout << "The minimization took: " << last_time_ * 1e3 << " ms\n";
+        if(min.NFcn() > 0)
+            std::cout << "Average time per call: " << last_time_ * 1e3 / min.NFcn() << " ms\n";
+    }
+    if(min.IsValid() && min.HasCovariance() && !min.IsAboveMaxEdm() && !min.HasReachedCallLimit()) {
+        retval_ = FitErrors::Valid;
+    } else {
+        if(verbosity > 0) {
+            std::cout << "HesseFailed: " << min.HesseFailed() << std::endl;
+            std::cout << "HasCovariance: " << min.HasCovariance() << std::endl;
+            std::cout << "HasValidCovariance: " << min.HasValidCovariance() << std::endl;
+            std::cout << "HasValidParameters: " << min.HasValidParameters() << std::endl;
+            std::cout << "IsAboveMaxEdm: " << min.IsAboveMaxEdm() << std::endl;
+            std::cout << "HasReachedCallLimit: " << min.HasReachedCallLimit() << std::endl;
+            std::cout << "HasAccurateCovar: " << min.HasAccurateCovar() << std::endl;
+            std::cout << "HasPosDefCovar : " << min.HasPosDefCovar() << std::endl;
+            std::cout << "HasMadePosDefCovar : " << min.HasMadePosDefCovar() << std::endl;
+        }
+        retval_ = FitErrors::InValid;
+    }
+    upar_.SetGooFitParams(min.UserState());
+    Minuit2::MnPrint::SetGlobalLevel(old_level);
+    return min;
+}
+
+Minuit2::MnScan FitManagerMinuit2::getMnScan() { return Minuit2::MnScan{fcn_, upar_}; }
+
+}  // namespace GooFit

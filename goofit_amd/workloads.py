@@ -160,3 +160,125 @@ def dalitz_area(decay=DALITZ_DECAY, n=4000):
     r3 = np.sqrt(np.maximum(e3 * e3 - d3 * d3, 0))
     width = 4 * r1 * r3
     return float(0.5 * (hi - lo) * np.sum(w * width))
+
+
+# --------------------------------------------------------- models (C++ API)
+
+
+class Model:
+    """A PDF built with the GooFit API plus the objects it references."""
+
+    def __init__(self, pdf, observables, params, keep=()):
+        self.pdf = pdf
+        self.observables = observables
+        self.params = params          # name -> Variable
+        self._keep = list(keep)      # component PDFs, decay info, datasets
+
+
+def build_add1d(start=1.0, nbins=1000):
+    """0.3 Gaussian(0.3, 0.8) + 0.7 Exp(-0.4) on [-5, 5]; free parameters start at start*truth."""
+    import goofit_amd as G
+    G.require_api()
+    x = G.Observable("x", -5.0, 5.0)
+    x.numbins = nbins
+    mu = G.Variable("mu", 0.3 * start, 0.01, -2.0, 2.0)
+    sigma = G.Variable("sigma", 0.8 * start, 0.01, 0.1, 3.0)
+    alpha = G.Variable("alpha", -0.4 * start, 0.01, -2.0, -0.01)
+    frac = G.Variable("frac", 0.3 * start, 0.01, 0.0, 1.0)
+    g = G.GaussianPdf("gauss", x, mu, sigma)
+    e = G.ExpPdf("exp", x, alpha)
+    s = G.AddPdf("sum", frac, g, e)
+    return Model(s, [x], dict(mu=mu, sigma=sigma, alpha=alpha, frac=frac), keep=[g, e])
+
+
+def build_prod3d(start=1.0, nbins=200):
+    """Gaussian(x) x Exp(y) x ExpGaus(z); free mu, sigma, alpha and the decay rate."""
+    import goofit_amd as G
+    G.require_api()
+    x = G.Observable("x", -6.0, 6.0)
+    y = G.Observable("y", 0.0, 8.0)
+    z = G.Observable("z", 0.0, 10.0)
+    for o in (x, y, z):
+        o.numbins = nbins
+    mu = G.Variable("mu", 0.0 + 0.05 * (start - 1.0), 0.01, -1.0, 1.0)
+    sigma = G.Variable("sigma", 1.0 * start, 0.01, 0.2, 3.0)
+    alpha = G.Variable("alpha", -1.2 * start, 0.01, -5.0, -0.05)
+    emu = G.Variable("emg_mu", 0.0)
+    esig = G.Variable("emg_sigma", 0.1)
+    rate = G.Variable("rate", (1.0 / 1.5) * start, 0.01, 0.05, 5.0)
+    gx = G.GaussianPdf("gx", x, mu, sigma)
+    ey = G.ExpPdf("ey", y, alpha)
+    ez = G.ExpGausPdf("ez", z, emu, esig, rate)
+    p = G.ProdPdf("prod", [gx, ey, ez])
+    return Model(p, [x, y, z], dict(mu=mu, sigma=sigma, alpha=alpha, rate=rate), keep=[gx, ey, ez])
+
+
+def build_dalitz(nbins=1414, coup_seed=3, resonances=None, float_couplings=True, coup_scale=1.0):
+    """D0 -> pi+ pi- pi0 with the 16 resonances of DALITZ_RESONANCES, flat efficiency,
+    m12 and m13 on [0, 3] with an nbins x nbins normalisation grid."""
+    import goofit_amd as G
+    G.require_api()
+    table = resonances if resonances is not None else DALITZ_RESONANCES
+    m12 = G.Observable("m12", 0.0, 3.0)
+    m13 = G.Observable("m13", 0.0, 3.0)
+    m12.numbins = nbins
+    m13.numbins = nbins
+    evt = G.EventNumber("eventNumber")
+    coups = dalitz_couplings(coup_seed, len(table))
+    params, res = {}, []
+    for (name, kind, mass, width, spin, pair), c in zip(table, coups):
+        c = c * (coup_scale if name != DALITZ_FIXED else 1.0)
+        if name == DALITZ_FIXED or not float_couplings:
+            ar, ai = G.Variable(name + "_re", c.real), G.Variable(name + "_im", c.imag)
+        else:
+            ar = G.Variable(name + "_re", c.real, 0.01, 0.0, 0.0)
+            ai = G.Variable(name + "_im", c.imag, 0.01, 0.0, 0.0)
+            params[name + "_re"], params[name + "_im"] = ar, ai
+        if kind == "NONRES":
+            r = G.Resonances.NonRes(name, ar, ai)
+        elif kind == "GS":
+            r = G.Resonances.GS(name, ar, ai, G.Variable(name + "_mass", mass), G.Variable(name + "_width", width),
+                                spin, pair)
+        else:
+            r = G.Resonances.RBW(name, ar, ai, G.Variable(name + "_mass", mass), G.Variable(name + "_width", width),
+                                 spin, pair)
+        res.append(r)
+    decay = G.DecayInfo3()
+    decay.motherMass, decay.daug1Mass, decay.daug2Mass, decay.daug3Mass = DALITZ_DECAY[:4]
+    decay.meson_radius, decay.mother_meson_radius = DALITZ_DECAY[4:]
+    decay.resonances = res
+    eff = G.PolynomialPdf("constantEff", [m12, m13], [G.Variable("constantOne", 1.0)],
+                          [G.Variable("constantZero0", 0.0), G.Variable("constantZero1", 0.0)], 0)
+    amp = G.Amp3Body("signalPDF", m12, m13, evt, decay, eff)
+    return Model(amp, [m12, m13, evt], params, keep=res + [decay, eff])
+
+
+def dalitz_dataset(model, m12, m13):
+    import goofit_amd as G
+    d = G.UnbinnedDataSet(model.observables)
+    d.from_columns([np.asarray(m12, float), np.asarray(m13, float), np.arange(len(m12), dtype=np.float64)])
+    return d
+
+
+def generate_dalitz(model, n, seed=6, batch=4_000_000, safety=1.05):
+    """n events of the model by accept-reject on the GPU-evaluated |A|^2 * eff over
+    points uniform in the Dalitz plot.  Returns (m12, m13)."""
+    rng = np.random.default_rng(seed)
+    pdf = model.pdf
+    acc12, acc13, fmax = [], [], None
+    have = 0
+    while have < n:
+        c12, c13 = dalitz_uniform(batch, int(rng.integers(1 << 31)))
+        data = dalitz_dataset(model, c12, c13)
+        pdf.setData(data)
+        pdf.setDataSize(batch)
+        pdf.setForceIntegrals()
+        from goofit_amd import EvalFunc
+        f = pdf.evaluateAtData(EvalFunc.Eval)
+        if fmax is None:
+            fmax = safety * float(f.max())
+        keep = rng.random(batch) * fmax < f
+        acc12.append(c12[keep])
+        acc13.append(c13[keep])
+        have += int(keep.sum())
+    return np.concatenate(acc12)[:n], np.concatenate(acc13)[:n]

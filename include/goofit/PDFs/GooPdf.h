@@ -1,0 +1,64 @@
+// GooPdf: a PDF evaluated by the GPU engine (include/goofit/PDFs/GooPdf.h of the
+// reference).  The reductions that the reference runs as thrust algorithms over
+// MetricTaker are single calls into the C ABI of include/goofit_hip.h.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include <goofit/FitControl.h>
+#include <goofit/PdfBase.h>
+#include <goofit/UnbinnedDataSet.h>
+
+namespace GooFit {
+
+class GooPdf : public PdfBase {
+  protected:
+    /// Number of event columns (negative for binned data: -(observables + 2)).
+    int get_event_size() const;
+    /// Sum over the events of metric(pdf(event), norm) (does not normalise).
+    double reduce_with_metric() const;
+    /// Metric of every event (does not normalise).
+    void evaluate_with_metric(std::vector<fptype> &results) const;
+    std::vector<fptype> evaluate_with_metric() const;
+    /// gf_metric code of the current fit control.
+    int metric_code() const;
+
+  public:
+    using PdfBase::PdfBase;
+    ~GooPdf() override = default;
+
+    /// Number of cells of the normalisation grid.
+    int get_bin_grid_size() const;
+    /// Volume of the normalisation box.
+    fptype get_bin_grid_volume() const;
+    /// Sum of the raw PDF over the normalisation grid (does not normalise).
+    double reduce_with_bins() const;
+
+    double calculateNLL() override;
+    std::vector<fptype> evaluateAtPoints(Observable var);
+    fptype normalize() override;
+    fptype normalise() { return normalize(); }
+
+    virtual fptype integrate(fptype lo, fptype hi) const;
+    bool hasAnalyticIntegral() const override;
+    fptype getValue(EvalFunc evalfunc = EvalFunc::Eval);
+
+    /// Metric of every event of the current data (normalised first), e.g. the
+    /// probability of each event for EvalFunc::Prob.  The reference keeps this
+    /// as the protected evaluate_with_metric(); exposed for event generation.
+    std::vector<fptype> evaluateAtData(EvalFunc evalfunc = EvalFunc::Prob);
+    /// Probabilities of the PDF and of each component at the data points.
+    std::vector<std::vector<fptype>> getCompProbsAtDataPoints();
+    /// Dataset of the bin centres of the PDF's observables.
+    UnbinnedDataSet makeGrid();
+
+    void initialize();
+    void scan(Observable var, std::vector<fptype> &values);
+    void setFitControl(std::shared_ptr<FitControl> fc) override;
+    virtual void setMetrics() {}
+    void setParameterConstantness(bool constant = true);
+    void setIndices() override;
+};
+
+}  // namespace GooFit

@@ -1,0 +1,20 @@
+// Weighted sum of PDFs, extended or not (include/goofit/PDFs/combine/AddPdf.h of the reference).
+#pragma once
+
+#include <goofit/PDFs/CombinePdf.h>
+
+namespace GooFit {
+
+class AddPdf : public CombinePdf {
+  public:
+    AddPdf(std::string n, std::vector<Variable> weights, std::vector<PdfBase *> comps);
+    AddPdf(std::string n, Variable frac1, PdfBase *func1, PdfBase *func2);
+    fptype normalize() override;
+    double calculateNLL() override;
+    bool hasAnalyticIntegral() const override { return false; }
+
+  private:
+    bool extended;
+};
+
+}  // namespace GooFit

@@ -1,0 +1,65 @@
+// Time-independent three-body Dalitz-plot PDF (include/goofit/PDFs/physics/Amp3Body.h
+// of the reference):
+//     P(m12^2, m13^2) = |sum_i c_i B_i(m12^2, m13^2)|^2 * eff(m12^2, m13^2)
+// with complex couplings c_i and resonance lineshapes B_i.  The lineshapes of the
+// data events are cached on the GPU and recomputed only when a resonance's mass
+// or width changes; the normalisation integrals I_ij = sum eff B_i conj(B_j)
+// over the m12 x m13 bin grid come from one FP64-MFMA Gram kernel.
+#pragma once
+
+#include <complex>
+#include <vector>
+
+#include <goofit/PDFs/physics/Amp3BodyBase.h>
+#include <goofit/PDFs/physics/DalitzPlotHelpers.h>
+#include <goofit/PDFs/physics/resonances/Resonance.h>
+
+namespace GooFit {
+
+class Amp3Body : public Amp3BodyBase {
+  public:
+    Amp3Body(std::string n, Observable m12, Observable m13, EventNumber eventNumber, DecayInfo3 decay,
+             GooPdf *eff = nullptr);
+
+    fptype normalize() override;
+    /// Allocate the wave cache for dataSize events starting at row offset
+    /// (call after setData).
+    void setDataSize(unsigned int dataSize, unsigned int evtSize = 3, unsigned int offset = 0);
+    void setForceIntegrals(bool f = true) { forceRedoIntegrals = f; }
+
+    void populateArrays() override;
+
+    /// Host copy of the cached lineshape of resonance i.
+    const std::vector<std::complex<fptype>> getCachedWave(size_t i) const;
+    /// Sum of the cached lineshape of resonance i over the events.
+    std::complex<fptype> sumCachedWave(size_t i) const;
+
+    DecayInfo3 &getDecayInfo() { return decayInfo; }
+    static void resetCacheCounter() { cacheCount = 0; }
+
+    /// Fit fractions from the data-event cache (cache must be filled).
+    std::vector<std::vector<fptype>> fit_fractions();
+
+    /// Normalisation integrals I_ij of the last normalize() (n x n).
+    const std::vector<std::complex<fptype>> &getIntegrals() const { return integrals; }
+    /// Function index of the efficiency (the last function of this subtree).
+    unsigned int getEfficiencyIndex() const { return efficiencyFunction; }
+
+  protected:
+    DecayInfo3 decayInfo;
+    Observable _m12;
+    Observable _m13;
+    EventNumber _eventNumber;
+
+    std::vector<std::complex<fptype>> integrals;
+    std::vector<bool> redoIntegral;
+    mutable bool forceRedoIntegrals{true};
+    int totalEventSize{3};
+    int eventOffset{0};
+    int cacheToUse{0};
+    static int cacheCount;
+    unsigned int efficiencyFunction{0};
+    bool cacheReady{false};
+};
+
+}  // namespace GooFit

@@ -1,0 +1,31 @@
+// Three-body kinematics shared by the Dalitz PDFs
+// (include/goofit/PDFs/physics/DalitzPlotHelpers.h of the reference).
+#pragma once
+
+#include <vector>
+
+#include <goofit/GlobalCudaDefines.h>
+#include <goofit/Variable.h>
+
+namespace GooFit {
+
+class ResonancePdf;
+
+enum DaughterPair { PAIR_12 = 0, PAIR_13, PAIR_23 };
+
+/// Kinematic boundary of the (m12^2, m13^2) plane.
+bool inDalitz(const fptype &m12, const fptype &m13, const fptype &bigM, const fptype &dm1, const fptype &dm2,
+              const fptype &dm3);
+
+struct DecayInfo3 {
+    fptype motherMass;
+    fptype daug1Mass;
+    fptype daug2Mass;
+    fptype daug3Mass;
+    fptype meson_radius;
+    fptype mother_meson_radius;
+
+    std::vector<ResonancePdf *> resonances;
+};
+
+}  // namespace GooFit

@@ -446,6 +446,27 @@ class OracleProg:
         return out.view(np.complex128).reshape(len(res), len(res))
 
 
+class RawProg(OracleProg):
+    """A gfo_prog built directly from flattened arrays (e.g. those of the engine's API)."""
+
+    def __init__(self, funcs, params, consts, obs, norms, decay=None):
+        self.f = np.ascontiguousarray(funcs, np.int32)
+        self.p, self.c, self.o, self.n = (np.ascontiguousarray(a, np.float64) for a in (params, consts, obs, norms))
+        self.decay = np.asarray(decay if decay is not None else [0.0] * 6, np.float64)
+        L = lib()
+        self.h = L.gfo_prog_new(self.f.ctypes.data_as(_I), self.f.size, _dp(self.p), self.p.size, _dp(self.c),
+                                self.c.size, _dp(self.o), self.o.size, _dp(self.n), self.n.size, _dp(self.decay))
+        self._caches = {}
+
+    def set_norms(self, norms):
+        self.n = np.ascontiguousarray(norms, np.float64)
+        lib().gfo_prog_set_norms(self.h, _dp(self.n), self.n.size)
+
+    def set_params(self, params):
+        self.p = np.ascontiguousarray(params, np.float64)
+        lib().gfo_prog_set_params(self.h, _dp(self.p), self.p.size)
+
+
 def dalitz_norm_from_integrals(coups, I, bs12, bs13):
     c = np.ascontiguousarray(np.asarray(coups, np.complex128).view(np.float64))
     Ia = np.ascontiguousarray(np.asarray(I, np.complex128).view(np.float64))

@@ -1,0 +1,165 @@
+"""The GooFit C++ API (through its Python binding) on the GPU: the reference's
+known-answer tests and parity of calculateNLL / normalize with the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+import goofit_amd as G
+import oracle as O
+from goofit_amd import workloads as W
+
+from _models import add1d, dalitz as oracle_dalitz, prod3d
+
+pytestmark = pytest.mark.gpu
+NLL_RTOL = 1e-11
+
+
+def test_grid_1d_kat():
+    """tests/simple/NormalizeTest.cpp:57-100."""
+    x = G.Observable("xvar", -5, 10)
+    g = G.GaussianPdf("gausspdf", x, G.Variable("mean", 1.5), G.Variable("sigma", 2.5))
+    data = g.makeGrid()
+    g.setData(data)
+    assert data.getValue(x, 0) == pytest.approx(-4.925)
+    assert data.getValue(x, 20) == pytest.approx(-1.925)
+    assert data.getValue(x, 40) == pytest.approx(1.075)
+    vv = g.getCompProbsAtDataPoints()
+    assert len(vv) == 1 and len(vv[0]) == 100
+    for i, v in {0: 0.00587129964482, 20: 0.0624318782242, 40: 0.157287605852, 60: 0.0938855055588}.items():
+        assert vv[0][i] == pytest.approx(v, rel=1e-11)
+
+
+def test_grid_2d_kat():
+    """NormalizeTest.cpp:105-174: ProdPdf of two Gaussians on a 100 x 100 grid."""
+    x = G.Observable("xvar", -5, 10)
+    y = G.Observable("yvar", -5, 10)
+    g1 = G.GaussianPdf("g1", x, G.Variable("mean1", 1.5), G.Variable("sigma1", 2.5))
+    g2 = G.GaussianPdf("g2", y, G.Variable("mean2", -1), G.Variable("sigma2", 0.5))
+    p = G.ProdPdf("product", [g1, g2])
+    data = p.makeGrid()
+    p.setData(data)
+    assert data.getValue(y, 100 * 20) == pytest.approx(-1.925)
+    vv = p.getCompProbsAtDataPoints()
+    assert len(vv) == 3 and all(len(v) == 10000 for v in vv)
+    for i, v in {0: 0.00587129964482, 20: 0.0624318782242, 40: 0.157287605852, 60: 0.0938855055588}.items():
+        assert vv[1][i] == pytest.approx(v, rel=1e-11)
+    np.testing.assert_allclose(np.asarray(vv[0]), np.asarray(vv[1]) * np.asarray(vv[2]), rtol=1e-13)
+
+
+def test_eval_1d_kat():
+    """NormalizeTest.cpp:176-219."""
+    x = G.Observable("xvar", -5, 10)
+    g = G.GaussianPdf("gausspdf", x, G.Variable("mean", 1.5), G.Variable("sigma", 2.5))
+    v = g.evaluateAtPoints(x)
+    for i, ref in {0: 0.00587129964482, 20: 0.0624318782242, 40: 0.157287605852, 60: 0.0938855055588}.items():
+        assert v[i] == pytest.approx(ref, rel=1e-11)
+    assert g.normalize() == pytest.approx(6.26657, rel=1e-6)
+    x.value = 1.2
+    assert g.getValue(G.EvalFunc.Prob) == pytest.approx(0.15843208471746245, rel=1e-13)
+    assert g.getValue(G.EvalFunc.Eval) == pytest.approx(0.99282585857903813, rel=1e-13)
+    x.value = 7.2
+    assert g.getValue(G.EvalFunc.Prob) == pytest.approx(0.0118618339389365, rel=1e-13)
+    assert g.getValue(G.EvalFunc.Eval) == pytest.approx(0.0743330208507896, rel=1e-13)
+
+
+def test_add_norm_kat():
+    """tests/simple/AddNormTest.cpp:16-44."""
+    x = G.Observable("xvar", -5, 10)
+    mean = G.Variable("mean", 1.5)
+    a = G.GaussianPdf("gaussA", x, mean, G.Variable("sigmaA", 2.5))
+    b = G.GaussianPdf("gaussB", x, mean, G.Variable("sigmaB", 0.8))
+    s = G.AddPdf("sum", G.Variable("frac", 0.6), a, b)
+    data = s.makeGrid()
+    s.setData(data)
+    vv = [np.asarray(v) for v in s.getCompProbsAtDataPoints()]
+    assert len(vv) == 3
+    np.testing.assert_allclose(vv[0][::7], 0.6 * vv[1][::7] + 0.4 * vv[2][::7], rtol=1e-13)
+
+
+def test_add1d_nll_parity():
+    m = W.build_add1d()
+    data = W.sample_add1d(100_000, seed=31)
+    d = G.UnbinnedDataSet(m.observables)
+    d.from_columns([data])
+    m.pdf.setData(d)
+    got = m.pdf.calculateNLL()
+    top, _ = add1d(mu=m.params["mu"].value, sigma=m.params["sigma"].value, alpha=m.params["alpha"].value,
+                   frac=m.params["frac"].value)
+    ref = O.Model(top, data.reshape(-1, 1)).calculate_nll()
+    assert math.isclose(got, ref, rel_tol=NLL_RTOL), (got, ref)
+
+
+def test_prod3d_nll_parity():
+    m = W.build_prod3d()
+    xs, ys, zs = W.sample_prod3d(300_000, seed=32)
+    d = G.UnbinnedDataSet(m.observables)
+    d.from_columns([xs, ys, zs])
+    m.pdf.setData(d)
+    got = m.pdf.calculateNLL()
+    top, _ = prod3d(mu=m.params["mu"].value, sigma=m.params["sigma"].value, alpha=m.params["alpha"].value,
+                    rate=m.params["rate"].value)
+    ref = O.Model(top, np.stack([xs, ys, zs], 1)).calculate_nll()
+    assert math.isclose(got, ref, rel_tol=NLL_RTOL), (got, ref)
+    # flattened arrays are those of the oracle's restatement of populateArrays
+    f, p, c, o, n = G.engine.flattened()
+    rf, rp, rc, ro, rn = O.flatten(top).arrays()
+    assert list(f) == list(rf) and np.array_equal(p, rp) and np.array_equal(c, rc) and np.array_equal(o, ro)
+
+
+def test_dalitz_nll_and_fit_fractions():
+    m = W.build_dalitz(nbins=200)
+    m12, m13 = W.dalitz_uniform(50_000, seed=33)
+    d = W.dalitz_dataset(m, m12, m13)
+    m.pdf.setData(d)
+    m.pdf.setDataSize(len(m12))
+    got = m.pdf.calculateNLL()
+    top, _ = oracle_dalitz(nbins=200)
+    aos = np.stack([m12, m13, np.arange(len(m12), dtype=float)], 1)
+    ref = O.Model(top, aos, decay=W.DALITZ_DECAY).calculate_nll()
+    assert math.isclose(got, ref, rel_tol=NLL_RTOL), (got, ref)
+    ff = np.asarray(m.pdf.fit_fractions())
+    assert ff.shape == (16, 16)
+    assert ff.sum() == pytest.approx(1.0, abs=1e-12)
+    w = np.asarray(m.pdf.getCachedWave(2))
+    assert w.shape == (len(m12),)
+
+
+def _scipy_fit(model: O.Model, names, x0):
+    from scipy.optimize import minimize
+
+    vars_ = {}
+    for node in model.op.prog.nodes:
+        for p in node.params:
+            vars_[p.name] = p
+
+    def f(x):
+        for n, v in zip(names, x):
+            vars_[n].value = v
+        return model.calculate_nll()
+
+    r = minimize(f, x0, method="BFGS", options=dict(gtol=1e-7, maxiter=500))
+    r2 = minimize(f, r.x, method="Nelder-Mead", options=dict(xatol=1e-10, fatol=1e-11, maxiter=4000))
+    return r2.x, r2.fun
+
+
+def test_add1d_fit_parity():
+    """MIGRAD on the GPU NLL vs scipy on the oracle NLL: parameters within 0.01 sigma."""
+    truth = W.sample_add1d(20_000, seed=41)
+    m = W.build_add1d(start=1.2)
+    d = G.UnbinnedDataSet(m.observables)
+    d.from_columns([truth])
+    m.pdf.setData(d)
+    fm = G.FitManager(m.pdf)
+    fm.setVerbosity(0)
+    fm.setTolerance(0.001)
+    res = fm.fit()
+    assert res.IsValid()
+    names = ["mu", "sigma", "alpha", "frac"]
+    gpu = np.array([m.params[n].value for n in names])
+    err = np.array([m.params[n].error for n in names])
+    top, _ = add1d(mu=0.36, sigma=0.96, alpha=-0.48, frac=0.36)
+    om = O.Model(top, truth.reshape(-1, 1))
+    xs, fs = _scipy_fit(om, names, gpu + 0.05 * err)
+    assert np.all(np.abs(gpu - xs) < 0.01 * err), (gpu, xs, err)
+    assert res.Fval() <= fs + 1e-6
