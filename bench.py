@@ -41,31 +41,42 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def perturb(params, names, k, scale=1e-3):
-    """Deterministic small moves of the couplings, as a minimiser makes."""
-    for i, n in enumerate(names):
-        v = params[n]
-        v.value = v.value * (1.0 + scale * math.sin(0.37 * k + 1.3 * i))
+def measured_traffic(name="dalitz16_1e6_hbm_counters.md"):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the same counter on a 256 MiB copy)."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
 
 
 def time_steps(pdf, params, steps, warmup, sync):
-    names = sorted(params)
-    base = {n: params[n].value for n in names}
+    """Time `steps` FCN calls (Params::from_minuit_vector + calculateNLL), each with
+    slightly moved free parameters: exactly what MIGRAD requests per call."""
+    import goofit_amd as G
+    fm = G.FitManager(pdf)
+    x0 = np.asarray(fm.minuit_vector(), float)
+    free = np.asarray(fm.free_mask(), bool)
+    idx = np.nonzero(free)[0]
+
+    def vec(k):
+        x = x0.copy()
+        x[idx] = x0[idx] * (1.0 + 1e-3 * np.sin(0.37 * k + 1.3 * np.arange(idx.size)))
+        return x.tolist()
+
     for k in range(warmup):
-        perturb(params, names, k)
-        pdf.copyParams()
-        pdf.calculateNLL()
+        fm.evaluate(vec(k))
     sync()
     t0 = time.perf_counter()
     for k in range(steps):
-        perturb(params, names, warmup + k)
-        pdf.copyParams()
-        pdf.calculateNLL()
+        fm.evaluate(vec(warmup + k))
     sync()
     t1 = time.perf_counter()
-    for n in names:
-        params[n].value = base[n]
-    pdf.copyParams()
+    fm.evaluate(x0.tolist())
     return t1 - t0
 
 
@@ -108,6 +119,9 @@ def main():
     ap.add_argument("--no-fit", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="set up the Dalitz workload, run 20 NLL reductions and a 256 MiB device copy "
+                         "(HBM counter calibration), print nothing else; for rocprofv3 runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the path shards by events across GPUs "
@@ -139,22 +153,37 @@ def main():
     sync()
     t_first = time.perf_counter() - t0
     log(f"first NLL {nll0:.6f} ({t_first * 1e3:.1f} ms incl. cache and Gram)")
+    if args.profile_only:
+        for _ in range(20):
+            pdf.calculateNLL()
+        src = torch.ones(32 << 20, dtype=torch.float64, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        sync()
+        log("profile-only run done: 20 NLL reductions, 3 copies of 256 MiB")
+        return
 
     dt = time_steps(pdf, model.params, args.steps, args.warmup, sync)
     ms_per_step = dt / args.steps * 1e3
     value = args.events * args.steps / dt
     log(f"{args.steps} NLL steps: {ms_per_step:.4f} ms/step, {value:.4e} events/s")
 
-    # dominant kernel: the fused coherent-sum NLL, timed with HIP events on its stream
+    # dominant kernel: the coherent-sum NLL sweep over the events, timed with HIP
+    # events on the engine's stream around each sweep launch (the one-workgroup
+    # kernel that adds the per-workgroup partials is timed separately)
     reps = 200
-    ms_kernel = G.engine.time_reduce_metric(1, reps) / reps
+    ms_sweep, ms_span = G.engine.time_reduce_sweep(1, reps)
+    ms_kernel = ms_sweep / reps
     bytes_per_event = 16 * 16 + 16  # 16 complex waves + (m12, m13); event number = row index
     achieved = bytes_per_event * args.events / (ms_kernel * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic() if args.events == 1_000_000 else (None, None)
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_reduce_metric<2> (Dalitz coherent-sum NLL)", kernel_ms=round(ms_kernel, 5),
-                    bytes_per_event=bytes_per_event)
-    log(f"NLL kernel {ms_kernel * 1e3:.1f} us -> {achieved:.0f} GB/s")
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=traffic_src,
+                    kernel="k_dalitz_fast (coherent-sum NLL sweep)", kernel_ms=round(ms_kernel, 5),
+                    reduction_ms=round(ms_span / reps, 5), bytes_per_event=bytes_per_event,
+                    bytes_per_launch=bytes_per_event * args.events)
+    log(f"NLL sweep {ms_kernel * 1e3:.1f} us -> {achieved:.0f} GB/s; whole reduction {ms_span / reps * 1e3:.1f} us")
 
     extra = dict(events=len(m12), generation_s=round(t_gen, 2), first_nll_ms=round(t_first * 1e3, 2), nll=nll0)
     grid6 = [0.0, 3.0, float(args.grid), 0.0, 3.0, float(args.grid)]

@@ -290,6 +290,21 @@ PYBIND11_MODULE(_goofit, m) {
         .def("covariance", [](const M2::FunctionMinimum &f) { return f.UserState().cov_; })
         .def("__repr__", [](const M2::FunctionMinimum &f) { return repr_of(f); });
 
+    // python/goofit/FitManager.cpp:19-30 of the reference
+    py::class_<Params>(m, "Params")
+        .def(py::init<PdfBase &>(), py::arg("pdf"), py::keep_alive<1, 2>())
+        .def("SetGooFitParams", &Params::SetGooFitParams, py::arg("input"))
+        .def("size", &Params::size)
+        .def("make_minuit_vector", &Params::make_minuit_vector)
+        .def("from_minuit_vector", &Params::from_minuit_vector, py::arg("values"), py::arg("force_changed") = false)
+        .def_property("record", &Params::get_recorded,
+                      [](Params &p, bool r) { p.set_record(r); })
+        .def_property_readonly("recorded", &Params::get_recorded);
+    py::class_<FCN>(m, "FCN")
+        .def(py::init<Params &>(), py::arg("params"), py::keep_alive<1, 2>())
+        .def("__call__", [](const FCN &f, const std::vector<double> &x) { return f(x); })
+        .def("Up", &FCN::Up);
+
     py::class_<FitManagerMinuit2>(m, "FitManager")
         .def(py::init<PdfBase *>(), py::arg("pdf"), py::keep_alive<1, 2>())
         .def("fit", &FitManagerMinuit2::fit)
@@ -300,7 +315,22 @@ PYBIND11_MODULE(_goofit, m) {
         .def("getMinosErrors", &FitManagerMinuit2::getMinosErrors)
         .def("setStrategy", &FitManagerMinuit2::setStrategy)
         .def("setTolerance", &FitManagerMinuit2::setTolerance)
+        .def_property("verbosity", &FitManagerMinuit2::getVerbosity, &FitManagerMinuit2::setVerbosity)
+        .def_property("max_calls", nullptr, [](FitManagerMinuit2 &f, unsigned v) { f.setMaxCalls(v); })
+        .def_property_readonly("params", &FitManagerMinuit2::getParams, py::return_value_policy::reference_internal)
+        .def_property_readonly("fcn", &FitManagerMinuit2::getFCN, py::return_value_policy::reference_internal)
+        .def("__nonzero__", [](const FitManagerMinuit2 &f) { return (bool)f; })
         .def("getLastFitTime", &FitManagerMinuit2::getLastFitTime)
+        .def("minuit_vector", [](FitManagerMinuit2 &f) { return f.getParams()->make_minuit_vector(); })
+        .def("free_mask",
+             [](FitManagerMinuit2 &f) {
+                 std::vector<bool> m;
+                 for(auto &p : f.getParams()->Parameters())
+                     m.push_back(!p.fixed);
+                 return m;
+             })
+        .def("evaluate", [](FitManagerMinuit2 &f, const std::vector<double> &x) { return (*f.getFCN())(x); },
+             "One FCN call: push the Minuit vector into the PDF and return its NLL")
         .def("__bool__", [](const FitManagerMinuit2 &f) { return (bool)f; })
         .def("__int__", [](const FitManagerMinuit2 &f) { return (int)f; });
 
@@ -310,6 +340,20 @@ PYBIND11_MODULE(_goofit, m) {
     me.def("get_device", &engine::get_device);
     me.def("release", &engine::release);
     me.def("flush", &engine::flush);
+    // launch-shape knobs of this thread's context (gf_tune codes)
+    me.def("set_tuning", [](int key, int value) {
+        engine::check(gf_ctx_set_tuning(engine::ctx(), key, value), "gf_ctx_set_tuning");
+    });
+    me.def("get_tuning", [](int key) {
+        int v = 0;
+        engine::check(gf_ctx_get_tuning(engine::ctx(), key, &v), "gf_ctx_get_tuning");
+        return v;
+    });
+    me.attr("TUNE_INLINE_ARGS")         = (int)GF_TUNE_INLINE_ARGS;
+    me.attr("TUNE_FAST_DALITZ")         = (int)GF_TUNE_FAST_DALITZ;
+    me.attr("TUNE_BLOCKS_PER_CU")       = (int)GF_TUNE_BLOCKS_PER_CU;
+    me.attr("TUNE_FAST_BLOCKS_PER_CU")  = (int)GF_TUNE_FAST_BLOCKS_PER_CU;
+    me.attr("TUNE_SPIN_US")             = (int)GF_TUNE_SPIN_US;
     me.def("callnumber", []() { return host_callnumber; });
     me.def("flattened", []() {
         return py::make_tuple(host_function_table, host_parameters, host_constants, host_observables,
@@ -317,6 +361,12 @@ PYBIND11_MODULE(_goofit, m) {
     });
     me.def("function_names", []() { return host_function_name; });
     // device time of back-to-back launches on the engine's stream (HIP events), ms
+    me.def("time_reduce_sweep", [](int metric, int reps) {
+        engine::flush();
+        double sweep = 0, total = 0;
+        engine::check(gf_time_reduce_sweep(engine::ctx(), metric, reps, &sweep, &total), "gf_time_reduce_sweep");
+        return py::make_tuple(sweep, total);
+    });
     me.def("time_reduce_metric", [](int metric, int reps) {
         engine::flush();
         double ms = 0;

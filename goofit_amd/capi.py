@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libgoofit_hip.so")
 
 # include/goofit_hip.h
 GF_OK, GF_ERR_INVALID, GF_ERR_HIP, GF_ERR_NOMEM, GF_ERR_STATE, GF_ERR_NODEVICE = range(6)
+TUNE_INLINE_ARGS, TUNE_FAST_DALITZ, TUNE_BLOCKS_PER_CU, TUNE_FAST_BLOCKS_PER_CU, TUNE_SPIN_US = range(5)
 FN = dict(NONE=0, GAUSSIAN=1, EXP=2, EXP_OFFSET=3, EXP_POLY=4, EXP_POLY_OFFSET=5, EXPGAUS=6,
           POLYNOMIAL=7, OFFSET_POLYNOMIAL=8, MULTI_POLYNOMIAL=9, ADD=10, ADD_EXT=11, PROD=12,
           DALITZ=13, RBW=14, RBW_SYM=15, GS=16, NONRES=17)
@@ -23,13 +24,14 @@ METRIC = dict(EVAL=0, NLL=1, PROB=2, BINAVG=3, BINWITHERROR=4, CHISQ=5)
 
 # Every symbol the header declares (checked by the tests against the built library).
 EXPORTS = [
-    "gf_device_count", "gf_ctx_create", "gf_ctx_destroy", "gf_last_error",
+    "gf_device_count", "gf_ctx_create", "gf_ctx_destroy", "gf_last_error", "gf_ctx_set_tuning",
+    "gf_ctx_get_tuning",
     "gf_events_upload", "gf_events_upload_aos", "gf_events_alloc", "gf_events_column",
     "gf_events_set_identity_column", "gf_program_set", "gf_params_update", "gf_norms_update",
     "gf_decay3_set", "gf_reduce_metric", "gf_eval_metric", "gf_eval_metric_device", "gf_reduce_bins",
     "gf_dalitz_set_data_size", "gf_dalitz_cache", "gf_dalitz_gram", "gf_dalitz_gram_points",
     "gf_dalitz_gram_points_device", "gf_dalitz_get_cache", "gf_dalitz_cache_gram",
-    "gf_time_reduce_metric", "gf_time_dalitz_cache", "gf_time_dalitz_gram",
+    "gf_time_reduce_metric", "gf_time_reduce_sweep", "gf_time_dalitz_cache", "gf_time_dalitz_gram",
 ]
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -57,6 +59,8 @@ def lib():
         L.gf_ctx_create.argtypes = [ctypes.POINTER(c_p), ctypes.c_int]
         L.gf_ctx_destroy.argtypes = [c_p]
         L.gf_ctx_destroy.restype = None
+        L.gf_ctx_set_tuning.argtypes = [c_p, ctypes.c_int, ctypes.c_int]
+        L.gf_ctx_get_tuning.argtypes = [c_p, ctypes.c_int, _I]
         L.gf_events_upload.argtypes = [c_p, ctypes.c_int, ctypes.c_int64, ctypes.POINTER(_D)]
         L.gf_events_upload_aos.argtypes = [c_p, ctypes.c_int, ctypes.c_int64, _D]
         L.gf_events_alloc.argtypes = [c_p, ctypes.c_int, ctypes.c_int64]
@@ -80,6 +84,7 @@ def lib():
         L.gf_dalitz_get_cache.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int64]
         L.gf_dalitz_cache_gram.argtypes = [c_p, ctypes.c_int, _D]
         L.gf_time_reduce_metric.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D]
+        L.gf_time_reduce_sweep.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D, _D]
         L.gf_time_dalitz_cache.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D]
         L.gf_time_dalitz_gram.argtypes = [c_p, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int, _D]
         _lib = L
@@ -112,6 +117,14 @@ class Context:
         self.h = h
         self.nevents = 0
         self._keep = []
+
+    def set_tuning(self, key: int, value: int):
+        _check(lib().gf_ctx_set_tuning(self.h, key, value))
+
+    def get_tuning(self, key: int) -> int:
+        v = ctypes.c_int32()
+        _check(lib().gf_ctx_get_tuning(self.h, key, ctypes.byref(v)))
+        return v.value
 
     def close(self):
         if self.h:
@@ -241,6 +254,13 @@ class Context:
         m = METRIC[metric] if isinstance(metric, str) else metric
         _check(lib().gf_time_reduce_metric(self.h, m, reps, ctypes.byref(ms)))
         return ms.value
+
+    def time_reduce_sweep(self, metric, reps):
+        """(summed device ms of the event sweeps alone, whole span ms)."""
+        sw, tot = ctypes.c_double(), ctypes.c_double()
+        m = METRIC[metric] if isinstance(metric, str) else metric
+        _check(lib().gf_time_reduce_sweep(self.h, m, reps, ctypes.byref(sw), ctypes.byref(tot)))
+        return sw.value, tot.value
 
     def time_dalitz_cache(self, dalitz_func, reps):
         ms = ctypes.c_double()

@@ -2,10 +2,12 @@
 // flattened-tree compilation, and the launches of gf_kernels.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <chrono>
 #include <map>
 #include <string>
 #include <vector>
@@ -90,9 +92,12 @@ struct gf_ctx {
     // reduction scratch
     gfk::Partial *d_partials = nullptr;
     unsigned *d_counter      = nullptr;
-    double *d_result         = nullptr;  // [0] sum, [1] residual, [2] error word (as int at byte 16)
-    double *h_result         = nullptr;  // pinned
-    int32_t *d_err           = nullptr;
+    double *d_result         = nullptr;  // scratch of the grid/probe paths
+    GfResult *h_res          = nullptr;  // host-mapped result block written by the last block
+    GfResult *d_res_mapped   = nullptr;  // its device address
+    double *h_result         = nullptr;  // pinned staging for small copies
+    int32_t *d_err           = nullptr;  // device error word (bit 0: cache index out of range)
+    bool fast_dalitz         = false;    // program is [DALITZ, MULTI_POLY, DALITZ_END, END]
     double *d_gram_partials  = nullptr;
     size_t cap_gram_partials = 0;
     double *d_gram_tiles     = nullptr;  // 768
@@ -106,6 +111,12 @@ struct gf_ctx {
     bool norms_dirty      = false;
     bool stage_in_flight  = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> ev_sweep;  // gf_time_reduce_sweep
+
+    uint32_t res_seq = 0;  // sequence number of the last reduction launched
+
+    // launch-shape knobs (gf_tune)
+    int tune[GF_TUNE_COUNT] = {1, 1, 4, 4, 2000};
 };
 
 namespace {
@@ -600,13 +611,15 @@ GfProgArgs prog_args(gf_ctx *ctx) {
     a.total_events = (double)ctx->n;
     a.decay        = ctx->decay;
     a.err          = ctx->d_err;
+    a.seq          = ctx->res_seq;
+    a.mark         = nullptr;
     return a;
 }
 
 int event_blocks(gf_ctx *ctx, int64_t n) {
     int64_t per = (int64_t)GF_BLOCK * 2;
     int64_t nb  = (n + per - 1) / per;
-    int64_t cap = (int64_t)ctx->cu_count * 8;
+    int64_t cap = (int64_t)ctx->cu_count * ctx->tune[GF_TUNE_BLOCKS_PER_CU];
     if(cap > GF_MAX_PARTIALS)
         cap = GF_MAX_PARTIALS;
     if(nb > cap)
@@ -636,23 +649,98 @@ gf_status check_events_for_program(gf_ctx *ctx, int metric) {
     return GF_OK;
 }
 
-// Read back the reduction result and the error word; clears the error word.
+// Wait for the reduction launched last and read the block k_finish wrote to
+// host-mapped memory ({sum, residual, error word}; the kernel clears the error).
+// The host polls the block's sequence number for up to GF_TUNE_SPIN_US
+// microseconds (a finished reduction is seen within ~1 us instead of after a
+// stream synchronisation); after that, or when polling is off, it blocks on
+// the stream, which also reports a failed kernel.
 gf_status read_result(gf_ctx *ctx, double *sum, double *comp) {
-    GF_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    GF_HIP(hipMemcpyAsync(ctx->h_result + 2, ctx->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-    GF_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->stage_in_flight = false;
-    int32_t err;
-    memcpy(&err, ctx->h_result + 2, sizeof(int32_t));
-    if(err) {
-        GF_HIP(hipMemsetAsync(ctx->d_err, 0, sizeof(int32_t), ctx->stream));
-        GF_HIP(hipStreamSynchronize(ctx->stream));
-        return fail(GF_ERR_INVALID, "Dalitz event number outside the wave cache (error word %d)", err);
+    const volatile GfResult *h = ctx->h_res;  // written by the device
+    const uint32_t want        = ctx->res_seq;
+    bool seen                  = false;
+    if(ctx->tune[GF_TUNE_SPIN_US] > 0) {
+        const auto t0    = std::chrono::steady_clock::now();
+        const auto limit = std::chrono::microseconds(ctx->tune[GF_TUNE_SPIN_US]);
+        for(unsigned it = 0;; ++it) {
+            if(__atomic_load_n(&ctx->h_res->seq, __ATOMIC_ACQUIRE) == want) {
+                seen = true;
+                break;
+            }
+            if((it & 255u) == 255u && std::chrono::steady_clock::now() - t0 > limit)
+                break;
+            __builtin_ia32_pause();
+        }
     }
+    if(!seen) {
+        GF_HIP(hipStreamSynchronize(ctx->stream));
+        if(__atomic_load_n(&ctx->h_res->seq, __ATOMIC_ACQUIRE) != want)
+            return fail(GF_ERR_HIP, "reduction %u finished without writing its result", want);
+    }
+    // every command before k_finish on the stream is complete
+    ctx->stage_in_flight = false;
+    GfResult r;
+    r.sum  = h->sum;
+    r.comp = h->comp;
+    r.err  = h->err;
+    if(r.err)
+        return fail(GF_ERR_INVALID, "Dalitz event number outside the wave cache (error word %d)", r.err);
     if(sum)
-        *sum = ctx->h_result[0];
+        *sum = r.sum;
     if(comp)
-        *comp = ctx->h_result[1];
+        *comp = r.comp;
+    return GF_OK;
+}
+
+// Parameter and normalisation values for the kernel arguments (false if they do not fit).
+bool make_inline(const gf_ctx *ctx, GfInline &inl) {
+    if(!ctx->tune[GF_TUNE_INLINE_ARGS] || ctx->params.size() > GF_PIN || ctx->norms.size() > GF_NIN)
+        return false;
+    std::copy(ctx->params.begin(), ctx->params.end(), inl.p);
+    std::copy(ctx->norms.begin(), ctx->norms.end(), inl.n);
+    return true;
+}
+
+// The coherent-sum fast path applies to this launch.
+bool use_fast_dalitz(const gf_ctx *ctx, int metric) {
+    if(!ctx->tune[GF_TUNE_FAST_DALITZ] || !ctx->fast_dalitz || !ctx->identity_ok || metric >= GF_METRIC_BINAVG || ctx->dalitz.empty())
+        return false;
+    auto it = ctx->caches.find(ctx->dalitz[0].cache_key);
+    return it != ctx->caches.end() && it->second.wave && it->second.n >= ctx->n;
+}
+
+// The fast path's efficiency polynomial has maximum degree 0.
+bool const_efficiency(const gf_ctx *ctx) { return (int)ctx->consts[ctx->ins[1].c] == 0; }
+
+int fast_blocks(const gf_ctx *ctx, int64_t n) {
+    int64_t nb  = (n + 255) / 256;
+    int64_t cap = (int64_t)ctx->cu_count * ctx->tune[GF_TUNE_FAST_BLOCKS_PER_CU];
+    if(cap > GF_MAX_PARTIALS)
+        cap = GF_MAX_PARTIALS;
+    return (int)std::max<int64_t>(1, std::min(nb, cap));
+}
+
+// Launch the likelihood reduction (fast path or interpreter) without waiting;
+// mark (optional) is recorded between the event sweep and k_finish.
+gf_status launch_reduce(gf_ctx *ctx, int metric, hipEvent_t mark = nullptr) {
+    GfInline inl;
+    const bool have_inl = make_inline(ctx, inl);
+    gf_status st        = have_inl ? GF_OK : flush_uploads(ctx);
+    if(st)
+        return st;
+    ++ctx->res_seq;
+    GfProgArgs a = prog_args(ctx);
+    a.mark       = mark;
+    int rc;
+    if(use_fast_dalitz(ctx, metric))
+        rc = gfk::launch_dalitz_fast(a, have_inl ? &inl : nullptr, const_efficiency(ctx), metric, ctx->d_partials,
+                                     ctx->d_res_mapped, nullptr, fast_blocks(ctx, ctx->n),
+                                     ctx->stream);
+    else
+        rc = gfk::launch_reduce_metric(a, have_inl ? &inl : nullptr, metric, ctx->d_partials, ctx->d_res_mapped,
+                                       event_blocks(ctx, ctx->n), ctx->stream);
+    if(rc)
+        return fail(GF_ERR_HIP, "reduce launch failed (%d)", rc);
     return GF_OK;
 }
 
@@ -794,6 +882,44 @@ int gf_device_count(void) {
 
 const char *gf_last_error(void) { return g_last_error.c_str(); }
 
+gf_status gf_ctx_set_tuning(gf_ctx *ctx, int key, int value) {
+    if(!ctx)
+        return fail(GF_ERR_INVALID, "null context");
+    switch(key) {
+    case GF_TUNE_INLINE_ARGS:
+    case GF_TUNE_FAST_DALITZ:
+        if(value != 0 && value != 1)
+            return fail(GF_ERR_INVALID, "tuning key %d takes 0 or 1 (got %d)", key, value);
+        break;
+    case GF_TUNE_BLOCKS_PER_CU:
+    case GF_TUNE_FAST_BLOCKS_PER_CU:
+        if(value < 1 || value > 64)
+            return fail(GF_ERR_INVALID, "tuning key %d takes 1..64 workgroups per CU (got %d)", key, value);
+        break;
+    case GF_TUNE_SPIN_US:
+        if(value < 0 || value > 1000000)
+            return fail(GF_ERR_INVALID, "tuning key %d takes 0..1000000 microseconds (got %d)", key, value);
+        break;
+    default: return fail(GF_ERR_INVALID, "unknown tuning key %d", key);
+    }
+    if(key == GF_TUNE_INLINE_ARGS && value == 0 && ctx->tune[key] == 1) {
+        // the device buffers may lag the host copies while values travel inline
+        ctx->params_dirty = true;
+        ctx->norms_dirty  = true;
+    }
+    ctx->tune[key] = value;
+    return GF_OK;
+}
+
+gf_status gf_ctx_get_tuning(gf_ctx *ctx, int key, int *value) {
+    if(!ctx || !value)
+        return fail(GF_ERR_INVALID, "null argument");
+    if(key < 0 || key >= GF_TUNE_COUNT)
+        return fail(GF_ERR_INVALID, "unknown tuning key %d", key);
+    *value = ctx->tune[key];
+    return GF_OK;
+}
+
 gf_status gf_ctx_create(gf_ctx **out, int device) {
     if(!out)
         return fail(GF_ERR_INVALID, "null output pointer");
@@ -827,10 +953,13 @@ gf_status gf_ctx_create(gf_ctx **out, int device) {
     TRY(hipMalloc((void **)&c->d_counter, 64));
     TRY(hipMalloc((void **)&c->d_result, 64));
     TRY(hipMalloc((void **)&c->d_err, 64));
+    TRY(hipHostMalloc((void **)&c->h_res, sizeof(GfResult), hipHostMallocMapped | hipHostMallocCoherent));
+    TRY(hipHostGetDevicePointer((void **)&c->d_res_mapped, c->h_res, 0));
     TRY(hipMalloc((void **)&c->d_gram_tiles, 768 * sizeof(double)));
     TRY(hipHostMalloc((void **)&c->h_result, 64, hipHostMallocDefault));
     TRY(hipHostMalloc((void **)&c->h_gram_tiles, 768 * sizeof(double), hipHostMallocDefault));
     TRY(hipMemset(c->d_counter, 0, 64));
+    TRY(hipMemset(c->d_result, 0, 64));
     TRY(hipMemset(c->d_err, 0, 64));
     TRY(hipMemset(c->d_cols, 0, GF_MAX_OBS * sizeof(double *)));
     TRY(hipMemset(c->d_dz, 0, GF_MAX_DALITZ * sizeof(GfDalitzSlot)));
@@ -850,14 +979,16 @@ void gf_ctx_destroy(gf_ctx *c) {
     for(auto &kv : c->caches)
         if(kv.second.wave)
             (void)hipFree(kv.second.wave);
-    void *dev[] = {c->col_base,  c->d_cols,     c->d_params,  c->d_consts,       c->d_norms,
-                   c->d_obsid,   c->d_ins,      c->d_dz,      c->d_res,          c->d_partials,
-                   c->d_counter, c->d_result,   c->d_err,     c->d_gram_partials, c->d_gram_tiles};
+    void *dev[] = {c->col_base,  c->d_cols,   c->d_params,   c->d_consts,        c->d_norms,
+                   c->d_obsid,   c->d_ins,    c->d_dz,       c->d_res,           c->d_partials,
+                   c->d_counter, c->d_result, c->d_err, c->d_gram_partials, c->d_gram_tiles};
     for(void *p : dev)
         if(p)
             (void)hipFree(p);
     if(c->h_result)
         (void)hipHostFree(c->h_result);
+    if(c->h_res)
+        (void)hipHostFree(c->h_res);
     if(c->h_gram_tiles)
         (void)hipHostFree(c->h_gram_tiles);
     if(c->h_stage)
@@ -866,6 +997,8 @@ void gf_ctx_destroy(gf_ctx *c) {
         (void)hipEventDestroy(c->ev0);
     if(c->ev1)
         (void)hipEventDestroy(c->ev1);
+    for(hipEvent_t e : c->ev_sweep)
+        (void)hipEventDestroy(e);
     if(c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;
@@ -953,13 +1086,14 @@ gf_status gf_events_set_identity_column(gf_ctx *ctx, int k) {
         return GF_ERR_INVALID;
     if(k < 0 || k >= ctx->ncols || !ctx->col_base)
         return fail(GF_ERR_INVALID, "column %d not allocated", k);
-    int32_t one = 1;
-    GF_HIP(hipMemcpyAsync(ctx->d_err + 4, &one, sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-    int rc = gfk::launch_identity_check(ctx->col_base + (size_t)k * ctx->n, ctx->n, ctx->d_err + 4, ctx->stream);
+    int32_t one    = 1;
+    int32_t *dflag = reinterpret_cast<int32_t *>(ctx->d_counter) + 4;  // word 4 of the counter block
+    GF_HIP(hipMemcpyAsync(dflag, &one, sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    int rc = gfk::launch_identity_check(ctx->col_base + (size_t)k * ctx->n, ctx->n, dflag, ctx->stream);
     if(rc)
         return fail(GF_ERR_HIP, "identity check launch failed (%d)", rc);
     int32_t flag = 0;
-    GF_HIP(hipMemcpyAsync(&flag, ctx->d_err + 4, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    GF_HIP(hipMemcpyAsync(&flag, dflag, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     GF_HIP(hipStreamSynchronize(ctx->stream));
     if(!flag) {
         ctx->identity_ok  = false;
@@ -1008,6 +1142,9 @@ gf_status gf_program_set(gf_ctx *ctx, const int32_t *funcs, int nfunc, const dou
     ctx->ins     = C.out;
     ctx->max_col = C.max_col;
     ctx->sub_pc[0] = 0;
+    ctx->fast_dalitz = ctx->ins.size() >= 4 && ctx->ins[0].op == GF_LVL_OP(GK_DALITZ, 0)
+                       && ctx->ins[1].op == GI_MULTI_POLY && ctx->ins[2].op == GF_LVL_OP(GK_DALITZ_END, 0)
+                       && ctx->ins[3].op == GI_END;
     for(const DalitzNode &d : ctx->dalitz) {
         int cols[3] = {d.col_m12, d.col_m13, d.col_num};
         for(int c : cols)
@@ -1089,13 +1226,8 @@ gf_status gf_reduce_metric(gf_ctx *ctx, int metric, double *sum, double *comp) {
     gf_status st = check_events_for_program(ctx, metric);
     if(st)
         return st;
-    if((st = flush_uploads(ctx)))
+    if((st = launch_reduce(ctx, metric)))
         return st;
-    GfProgArgs a = prog_args(ctx);
-    int rc = gfk::launch_reduce_metric(a, metric, ctx->d_partials, ctx->d_counter, ctx->d_result,
-                                       event_blocks(ctx, ctx->n), ctx->stream);
-    if(rc)
-        return fail(GF_ERR_HIP, "reduce launch failed (%d)", rc);
     return read_result(ctx, sum, comp);
 }
 
@@ -1107,10 +1239,19 @@ gf_status gf_eval_metric_device(gf_ctx *ctx, int metric, double *dev_out, int64_
         return st;
     if(!dev_out || n != ctx->n)
         return fail(GF_ERR_INVALID, "output has %lld entries, %lld events uploaded", (long long)n, (long long)ctx->n);
-    if((st = flush_uploads(ctx)))
+    GfInline inl;
+    const bool have_inl = make_inline(ctx, inl);
+    if(!have_inl && (st = flush_uploads(ctx)))
         return st;
     GfProgArgs a = prog_args(ctx);
-    int rc       = gfk::launch_eval_metric(a, metric, dev_out, event_blocks(ctx, ctx->n), ctx->stream);
+    int rc;
+    if(use_fast_dalitz(ctx, metric))
+        rc = gfk::launch_dalitz_fast(a, have_inl ? &inl : nullptr, const_efficiency(ctx), metric, ctx->d_partials,
+                                     ctx->d_res_mapped, dev_out, fast_blocks(ctx, ctx->n),
+                                     ctx->stream);
+    else
+        rc = gfk::launch_eval_metric(a, have_inl ? &inl : nullptr, metric, dev_out, event_blocks(ctx, ctx->n),
+                                     ctx->stream);
     if(rc)
         return fail(GF_ERR_HIP, "eval launch failed (%d)", rc);
     return GF_OK;
@@ -1126,10 +1267,15 @@ gf_status gf_eval_metric(gf_ctx *ctx, int metric, double *out, int64_t n) {
         return fail(GF_ERR_NOMEM, "cannot allocate the output of %lld events", (long long)n);
     gf_status st = gf_eval_metric_device(ctx, metric, d_out, n);
     if(!st) {
-        if(hipMemcpyAsync(out, d_out, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+        int32_t err = 0;
+        if(hipMemcpyAsync(out, d_out, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess
+           || hipMemcpyAsync(&err, ctx->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess
+           || hipStreamSynchronize(ctx->stream) != hipSuccess)
             st = fail(GF_ERR_HIP, "copy of the evaluated events failed");
-        else
-            st = read_result(ctx, nullptr, nullptr);
+        else if(err) {
+            (void)hipMemset(ctx->d_err, 0, sizeof(int32_t));
+            st = fail(GF_ERR_INVALID, "Dalitz event number outside the wave cache (error word %d)", err);
+        }
     }
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(d_out);
@@ -1183,9 +1329,10 @@ gf_status gf_reduce_bins(gf_ctx *ctx, int func_idx, const double *ranges, int k,
         }
     if((st = flush_uploads(ctx)))
         return st;
+    ++ctx->res_seq;
     GfProgArgs a = prog_args(ctx);
-    int rc = gfk::launch_reduce_bins(a, pc, ranges, slots, k, total, ctx->d_partials, ctx->d_counter, ctx->d_result,
-                                     event_blocks(ctx, total), ctx->stream);
+    int rc = gfk::launch_reduce_bins(a, pc, ranges, slots, k, total, ctx->d_partials,
+                                     ctx->d_res_mapped, event_blocks(ctx, total), ctx->stream);
     if(rc)
         return fail(GF_ERR_HIP, "grid reduce launch failed (%d)", rc);
     return read_result(ctx, sum, nullptr);
@@ -1362,21 +1509,53 @@ gf_status gf_time_reduce_metric(gf_ctx *ctx, int metric, int reps, double *ms_to
         return st;
     if(reps < 1 || !ms_total)
         return fail(GF_ERR_INVALID, "reps must be >= 1");
-    if((st = flush_uploads(ctx)))
-        return st;
-    GfProgArgs a = prog_args(ctx);
-    int nb       = event_blocks(ctx, ctx->n);
     GF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    for(int r = 0; r < reps; r++) {
-        int rc = gfk::launch_reduce_metric(a, metric, ctx->d_partials, ctx->d_counter, ctx->d_result, nb, ctx->stream);
-        if(rc)
-            return fail(GF_ERR_HIP, "reduce launch failed (%d)", rc);
-    }
+    for(int r = 0; r < reps; r++)
+        if((st = launch_reduce(ctx, metric)))
+            return st;
     GF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     GF_HIP(hipEventSynchronize(ctx->ev1));
     float ms = 0;
     GF_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     *ms_total = ms;
+    return read_result(ctx, nullptr, nullptr);
+}
+
+gf_status gf_time_reduce_sweep(gf_ctx *ctx, int metric, int reps, double *ms_sweep, double *ms_total) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    gf_status st = check_events_for_program(ctx, metric);
+    if(st)
+        return st;
+    if(reps < 1 || reps > 4096 || !ms_sweep)
+        return fail(GF_ERR_INVALID, "reps must be in 1..4096");
+    while((int)ctx->ev_sweep.size() < 2 * reps + 1) {
+        hipEvent_t e;
+        GF_HIP(hipEventCreate(&e));
+        ctx->ev_sweep.push_back(e);
+    }
+    std::vector<hipEvent_t> &ev = ctx->ev_sweep;
+    GF_HIP(hipEventRecord(ev[0], ctx->stream));
+    for(int r = 0; r < reps; r++) {
+        if(r)
+            GF_HIP(hipEventRecord(ev[2 * r], ctx->stream));
+        if((st = launch_reduce(ctx, metric, ev[2 * r + 1])))
+            return st;
+    }
+    GF_HIP(hipEventRecord(ev[2 * reps], ctx->stream));
+    GF_HIP(hipEventSynchronize(ev[2 * reps]));
+    double sweep = 0.0;
+    for(int r = 0; r < reps; r++) {
+        float ms = 0;
+        GF_HIP(hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]));
+        sweep += ms;
+    }
+    *ms_sweep = sweep;
+    if(ms_total) {
+        float ms = 0;
+        GF_HIP(hipEventElapsedTime(&ms, ev[0], ev[2 * reps]));
+        *ms_total = ms;
+    }
     return read_result(ctx, nullptr, nullptr);
 }
 

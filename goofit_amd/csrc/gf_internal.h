@@ -100,20 +100,50 @@ struct GfProgArgs {
     double total_events;         // c_totalEvents
     GfDecay decay;
     int32_t *err;                // device error word (bit 0: cache index out of range)
+    uint32_t seq;                // sequence number k_finish stamps on its GfResult
+    void *mark;                  // host only: hipEvent_t recorded between a sweep and its k_finish
 };
 
 #define GF_METRIC_BINAVG_CODE 3  // metrics from here on read (bin entry, bin volume/error) columns
+
+// Parameter and normalisation values passed inside the kernel arguments of the
+// per-call kernels (no host-to-device copy per likelihood evaluation) when the
+// flattened arrays fit; larger trees read the device copies instead.
+#define GF_PIN 256
+#define GF_NIN 128
+struct GfInline {
+    double p[GF_PIN];
+    double n[GF_NIN];
+};
+
+// Result block of a reduction, written by k_finish to host-mapped memory:
+// the values first, then (after a system-scope fence) the launch's sequence
+// number, so a host that sees seq reads complete values.
+struct GfResult {
+    double sum;
+    double comp;
+    int32_t err;
+    uint32_t seq;
+    int32_t pad[2];
+};
 
 // Host-side launchers (gf_kernels.hip).
 namespace gfk {
 struct Partial {
     double s, c;
 };
-int launch_reduce_metric(const GfProgArgs &a, int metric, Partial *partials, unsigned *counter, double *result,
+// inl: parameter/normalisation values in the kernel arguments (null: device arrays).
+// Reductions: one partial per workgroup, then one workgroup adds them into *result.
+int launch_reduce_metric(const GfProgArgs &a, const GfInline *inl, int metric, Partial *partials, GfResult *result,
                          int nblocks, void *stream);
-int launch_eval_metric(const GfProgArgs &a, int metric, double *out, int nblocks, void *stream);
+int launch_eval_metric(const GfProgArgs &a, const GfInline *inl, int metric, double *out, int nblocks, void *stream);
+// Coherent-sum fast path: program [DALITZ, MULTI_POLY efficiency, DALITZ_END, END],
+// event number = row index, every row inside the wave cache.
+// const_eff: the efficiency polynomial has degree 0 (a constant coefficient).
+int launch_dalitz_fast(const GfProgArgs &a, const GfInline *inl, bool const_eff, int metric, Partial *partials,
+                       GfResult *result, double *out, int nblocks, void *stream);
 int launch_reduce_bins(const GfProgArgs &a, int pc0, const double *ranges, const int *slots, int k, int64_t total,
-                       Partial *partials, unsigned *counter, double *result, int nblocks, void *stream);
+                       Partial *partials, GfResult *result, int nblocks, void *stream);
 int launch_dalitz_cache(const GfProgArgs &a, int slot, int64_t offset, int64_t n, uint32_t mask,
                         const GfResDesc *res, int nres, int nblocks, void *stream);
 // mode 0: grid cells (grid6 host array); 1: points (m12, m13, w device arrays); 2: data cache of `slot`.

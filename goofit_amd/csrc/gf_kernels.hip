@@ -86,12 +86,10 @@ __device__ __forceinline__ double (&level(Acc<EPT> &a))[EPT] {
 
 // One combining step on the accumulator of level L.
 template <int L, int EPT, class Src>
-__device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs &A, const Src &src,
-                                     const int64_t (&row)[EPT], const bool (&valid)[EPT], Acc<EPT> &acc,
-                                     double (&T)[EPT]) {
-    const double *__restrict__ P  = A.params;
-    const double *__restrict__ Nm = A.norms;
-    double(&a)[EPT]               = level<L>(acc);
+__device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs &A, const double *__restrict__ P,
+                                     const double *__restrict__ Nm, const Src &src, const int64_t (&row)[EPT],
+                                     const bool (&valid)[EPT], Acc<EPT> &acc, double (&T)[EPT]) {
+    double(&a)[EPT] = level<L>(acc);
     switch(kind) {
     case GK_PUSH0:
 #pragma unroll
@@ -164,14 +162,31 @@ __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs
             re[j] = 0.0;
             im[j] = 0.0;
         }
-        for(int i = 0; i < in.a; ++i) {
-            const double ar = P[in.p + 2 * i], ai = P[in.p + 2 * i + 1];
-            const double2 *__restrict__ w = dz.wave + (int64_t)i * dz.ncache;
+        // Loads of GF_DZ_CHUNK resonances are issued back to back before any of them is
+        // consumed, so each wave keeps 8 * EPT 16-byte loads in flight; the sum still
+        // runs over i = 0, 1, ... in order, as the reference's loop does.
+        constexpr int GF_DZ_CHUNK = 8;
+        for(int i0 = 0; i0 < in.a; i0 += GF_DZ_CHUNK) {
+            double2 w[GF_DZ_CHUNK][EPT];
 #pragma unroll
-            for(int j = 0; j < EPT; ++j) {
-                const double2 me = w[en[j]];
-                re[j] += ar * me.x - ai * me.y;
-                im[j] += ar * me.y + ai * me.x;
+            for(int u = 0; u < GF_DZ_CHUNK; ++u) {
+                const bool use              = i0 + u < in.a;
+                const double2 *__restrict__ b = dz.wave + (int64_t)(use ? i0 + u : 0) * dz.ncache;
+#pragma unroll
+                for(int j = 0; j < EPT; ++j)
+                    w[u][j] = use ? b[en[j]] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for(int u = 0; u < GF_DZ_CHUNK; ++u) {
+                if(i0 + u < in.a) {
+                    const double ar = P[in.p + 2 * (i0 + u)], ai = P[in.p + 2 * (i0 + u) + 1];
+#pragma unroll
+                    for(int j = 0; j < EPT; ++j) {
+                        const double2 me = w[u][j];
+                        re[j] += ar * me.x - ai * me.y;
+                        im[j] += ar * me.y + ai * me.x;
+                    }
+                }
             }
         }
 #pragma unroll
@@ -193,9 +208,9 @@ __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs
 // value of the tree ends up in T.  row[] are the event rows (used as wave-cache
 // index when the event-number column is the identity), valid[] masks tail lanes.
 template <int EPT, class Src>
-__device__ __forceinline__ void run(const GfProgArgs &A, int pc0, const Src &src, const int64_t (&row)[EPT],
-                                    const bool (&valid)[EPT], double (&T)[EPT]) {
-    const double *__restrict__ P  = A.params;
+__device__ __forceinline__ void run(const GfProgArgs &A, const double *__restrict__ P, const double *__restrict__ Nm,
+                                    int pc0, const Src &src, const int64_t (&row)[EPT], const bool (&valid)[EPT],
+                                    double (&T)[EPT]) {
     const double *__restrict__ Cn = A.consts;
     Acc<EPT> acc;
 #pragma unroll
@@ -309,10 +324,10 @@ __device__ __forceinline__ void run(const GfProgArgs &A, int pc0, const Src &src
             if(in.op >= GI_LVL) {
                 const int kind = (in.op - GI_LVL) >> 2;
                 switch((in.op - GI_LVL) & 3) {
-                case 0: step<0>(kind, in, A, src, row, valid, acc, T); break;
-                case 1: step<1>(kind, in, A, src, row, valid, acc, T); break;
-                case 2: step<2>(kind, in, A, src, row, valid, acc, T); break;
-                default: step<3>(kind, in, A, src, row, valid, acc, T); break;
+                case 0: step<0>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
+                case 1: step<1>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
+                case 2: step<2>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
+                default: step<3>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
                 }
                 break;
             }
@@ -347,45 +362,50 @@ __device__ __forceinline__ void block_reduce(double &s, double &c) {
     __syncthreads();
 }
 
-// Store this block's partial; the last block to finish adds all partials in
-// block order and writes result[0] = sum, result[1] = residual correction.
-__device__ __forceinline__ void finish_reduction(double s, double c, Partial *partials, unsigned *counter,
-                                                 double *result) {
-    __shared__ bool am_last;
+// Reduce the block's (sum, residual) pair and store it as partial blockIdx.x.
+// No fence and no atomic: on a multi-XCD device an agent-scope release per
+// workgroup writes back the XCD's L2 each time, which costs more than the
+// whole sweep over the events once there are thousands of workgroups.  The
+// partials are added by k_finish, whose launch orders it after every store.
+__device__ __forceinline__ void store_partial(double s, double c, Partial *partials) {
     block_reduce(s, c);
     if(threadIdx.x == 0) {
         partials[blockIdx.x].s = s;
         partials[blockIdx.x].c = c;
-        __threadfence();
-        unsigned prev = atomicAdd(counter, 1u);
-        am_last       = (prev == gridDim.x - 1);
     }
-    __syncthreads();
-    if(!am_last)
-        return;
+}
+
+// One workgroup: add the nparts partials in a fixed order (thread t takes
+// t, t+256, ...; then the wave butterfly and the waves in order), write
+// {sum, residual correction, error word} to *result (host-mapped memory),
+// clear the device error word, then stamp seq.
+__global__ __launch_bounds__(GF_BLOCK) void k_finish(const Partial *partials, int nparts, GfResult *result,
+                                                     int32_t *err, uint32_t seq) {
     double ts = 0.0, tc = 0.0;
-    for(unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
-        double ps = __hip_atomic_load(&partials[i].s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        double pc = __hip_atomic_load(&partials[i].c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        gfd::kb_merge(ts, tc, ps, pc);
-    }
+    for(int i = threadIdx.x; i < nparts; i += GF_BLOCK)
+        gfd::kb_merge(ts, tc, partials[i].s, partials[i].c);
     block_reduce(ts, tc);
     if(threadIdx.x == 0) {
-        double tot = ts + tc;
-        result[0]  = tot;
-        result[1]  = tc;
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int32_t e = *err;
+        result->sum     = ts + tc;
+        result->comp    = tc;
+        result->err     = e;
+        if(e)
+            *err = 0;
+        __threadfence_system();
+        __hip_atomic_store(&result->seq, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
 // ------------------------------------------------------------- event kernels
 
-template <int EPT>
-__global__ __launch_bounds__(GF_BLOCK) void k_reduce_metric(GfProgArgs A, int metric, Partial *partials,
-                                                            unsigned *counter, double *result) {
+template <int EPT, bool INL>
+__global__ __launch_bounds__(GF_BLOCK) void k_reduce_metric(GfProgArgs A, GfInline I, int metric, Partial *partials) {
+    const double *__restrict__ P  = INL ? I.p : A.params;
+    const double *__restrict__ Nm = INL ? I.n : A.norms;
     const int64_t n    = A.nevents;
     const int64_t tile = (int64_t)GF_BLOCK * EPT;
-    const double norm  = A.norms[1];
+    const double norm  = Nm[1];
     const bool binned  = metric >= GF_METRIC_BINAVG_CODE;
     double s = 0.0, c = 0.0;
     for(int64_t base = (int64_t)blockIdx.x * tile; base < n; base += (int64_t)gridDim.x * tile) {
@@ -401,7 +421,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_reduce_metric(GfProgArgs A, int me
             src.idx[j] = row[j];
         }
         double T[EPT];
-        run<EPT>(A, 0, src, row, valid, T);
+        run<EPT>(A, P, Nm, 0, src, row, valid, T);
 #pragma unroll
         for(int j = 0; j < EPT; ++j) {
             double e0 = 0.0, e1 = 0.0;
@@ -414,14 +434,16 @@ __global__ __launch_bounds__(GF_BLOCK) void k_reduce_metric(GfProgArgs A, int me
                 gfd::kb_add(s, c, v);
         }
     }
-    finish_reduction(s, c, partials, counter, result);
+    store_partial(s, c, partials);
 }
 
-template <int EPT>
-__global__ __launch_bounds__(GF_BLOCK) void k_eval_metric(GfProgArgs A, int metric, double *out) {
+template <int EPT, bool INL>
+__global__ __launch_bounds__(GF_BLOCK) void k_eval_metric(GfProgArgs A, GfInline I, int metric, double *out) {
+    const double *__restrict__ P  = INL ? I.p : A.params;
+    const double *__restrict__ Nm = INL ? I.n : A.norms;
     const int64_t n    = A.nevents;
     const int64_t tile = (int64_t)GF_BLOCK * EPT;
-    const double norm  = A.norms[1];
+    const double norm  = Nm[1];
     const bool binned  = metric >= GF_METRIC_BINAVG_CODE;
     for(int64_t base = (int64_t)blockIdx.x * tile; base < n; base += (int64_t)gridDim.x * tile) {
         ColSrc<EPT> src;
@@ -436,7 +458,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_eval_metric(GfProgArgs A, int metr
             src.idx[j] = row[j];
         }
         double T[EPT];
-        run<EPT>(A, 0, src, row, valid, T);
+        run<EPT>(A, P, Nm, 0, src, row, valid, T);
 #pragma unroll
         for(int j = 0; j < EPT; ++j) {
             double e0 = 0.0, e1 = 0.0;
@@ -449,6 +471,93 @@ __global__ __launch_bounds__(GF_BLOCK) void k_eval_metric(GfProgArgs A, int metr
                 out[row[j]] = v;
         }
     }
+}
+
+// ------------------------------------------------------ coherent-sum fast path
+
+// Multivariate polynomial of one event (device_MultiPolynomial,
+// src/PDFs/basic/PolynomialPdf.cu:47-118), columns read at row e.
+__device__ __forceinline__ double multipoly_at(const GfIns &in, const GfProgArgs &A, const double *__restrict__ P,
+                                               int64_t e) {
+    const int nobs      = in.a;
+    const int maxDegree = (int)A.consts[in.c] + 1;
+    int numBoxes        = 1;
+    for(int i = 0; i < nobs; ++i)
+        numBoxes *= maxDegree;
+    int coeff = nobs;
+    double r  = P[in.p + coeff];
+    coeff++;
+    for(int i = 1; i < numBoxes; ++i) {
+        double term = 1;
+        int ci = i, sum = 0;
+        for(int o = 0; o < nobs; ++o) {
+            double x = A.cols[A.obsid[in.x + o]][e];
+            x -= P[in.p + o];
+            int pw = ci % maxDegree;
+            ci /= maxDegree;
+            term *= pow(x, (double)pw);
+            sum += pw;
+        }
+        if(sum >= maxDegree)
+            continue;
+        term *= P[in.p + coeff];
+        coeff++;
+        r += term;
+    }
+    return r;
+}
+
+// |sum_i c_i W_i[e]|^2 * eff for a Dalitz node at the root of the tree.  All
+// wave loads of an event are issued before any arithmetic (16 x 16 bytes per lane
+// in flight), the rest is device_DalitzPlot's arithmetic in its order.
+template <bool WRITE, bool INL, bool CONST_EFF>
+__global__ __launch_bounds__(256) void k_dalitz_fast(GfProgArgs A, GfInline I, int metric, Partial *partials,
+                                                     double *out) {
+    const double *__restrict__ P  = INL ? I.p : A.params;
+    const double *__restrict__ Nm = INL ? I.n : A.norms;
+    const GfIns dzi       = A.ins[0];
+    const GfIns effi      = A.ins[1];
+    const GfDalitzSlot dz = A.dz[dzi.x];
+    const GfDecay &D      = A.decay;
+    const int nres        = dzi.a;
+    const int64_t n       = A.nevents;
+    const double norm     = Nm[1];
+    const double *__restrict__ c12 = A.cols[dz.col_m12];
+    const double *__restrict__ c13 = A.cols[dz.col_m13];
+    double s = 0.0, c = 0.0;
+    for(int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e0 = base + threadIdx.x;
+        const bool valid = e0 < n;
+        const int64_t e  = valid ? e0 : n - 1;
+        double2 w[GF_MAX_RES];
+#pragma unroll
+        for(int i = 0; i < GF_MAX_RES; ++i)
+            w[i] = (i < nres) ? dz.wave[(int64_t)i * dz.ncache + e] : make_double2(0.0, 0.0);
+        const double m12 = c12[e];
+        const double m13 = c13[e];
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for(int i = 0; i < GF_MAX_RES; ++i) {
+            if(i < nres) {
+                const double ar = P[dzi.p + 2 * i], ai = P[dzi.p + 2 * i + 1];
+                re += ar * w[i].x - ai * w[i].y;
+                im += ar * w[i].y + ai * w[i].x;
+            }
+        }
+        const bool inside = gfd::in_dalitz(m12, m13, D.motherMass, D.daug1Mass, D.daug2Mass, D.daug3Mass);
+        // a degree-0 polynomial is its constant coefficient (numBoxes == 1)
+        const double eff = CONST_EFF ? P[effi.p + effi.a] : multipoly_at(effi, A, P, e);
+        const double T   = inside ? (re * re + im * im) * eff : 0.0;
+        const double v   = gfd::metric(metric, T, 0.0, 0.0, norm, A.total_events);
+        if(WRITE) {
+            if(valid)
+                out[e] = v;
+        } else if(valid) {
+            gfd::kb_add(s, c, v);
+        }
+    }
+    if(!WRITE)
+        store_partial(s, c, partials);
 }
 
 }  // namespace
@@ -465,7 +574,7 @@ namespace {
 
 template <int EPT>
 __global__ __launch_bounds__(GF_BLOCK) void k_reduce_bins(GfProgArgs A, int pc0, GfGrid G, int64_t total,
-                                                          Partial *partials, unsigned *counter, double *result) {
+                                                          Partial *partials) {
     const int64_t tile = (int64_t)GF_BLOCK * EPT;
     double s = 0.0, c = 0.0;
     for(int64_t base = (int64_t)blockIdx.x * tile; base < total; base += (int64_t)gridDim.x * tile) {
@@ -496,13 +605,13 @@ __global__ __launch_bounds__(GF_BLOCK) void k_reduce_bins(GfProgArgs A, int pc0,
             }
         }
         double T[EPT];
-        run<EPT>(A, pc0, src, row, valid, T);
+        run<EPT>(A, A.params, A.norms, pc0, src, row, valid, T);
 #pragma unroll
         for(int j = 0; j < EPT; ++j)
             if(valid[j])
                 gfd::kb_add(s, c, T[j]);
     }
-    finish_reduction(s, c, partials, counter, result);
+    store_partial(s, c, partials);
 }
 
 // ------------------------------------------------------------- Dalitz cache
@@ -626,7 +735,7 @@ __global__ __launch_bounds__(GG_BLOCK) void k_gram(GfProgArgs A, GramSrc S, cons
                 const int64_t r0[1] = {0};
                 const bool v0[1]    = {true};
                 double T[1];
-                run<1>(A, S.eff_pc, ev, r0, v0, T);
+                run<1>(A, A.params, A.norms, S.eff_pc, ev, r0, v0, T);
                 eff = T[0];
             }
             ew = valid ? eff * wt : 0.0;
@@ -670,14 +779,16 @@ __global__ __launch_bounds__(GG_BLOCK) void k_gram(GfProgArgs A, GramSrc S, cons
     }
 }
 
-// Sum the per-block Gram partials in block order (one thread per element).
+// Sum the per-block Gram partials: one block per output element, each thread a
+// fixed stride of blocks, then a fixed-order tree (deterministic for a given grid).
 __global__ __launch_bounds__(256) void k_gram_final(const double *__restrict__ partials, int nblocks, double *out) {
-    for(int e = threadIdx.x; e < 768; e += blockDim.x) {
-        double s = 0.0, c = 0.0;
-        for(int b = 0; b < nblocks; ++b)
-            gfd::kb_add(s, c, partials[(int64_t)b * 768 + e]);
+    const int e = blockIdx.x;  // 0..767
+    double s = 0.0, c = 0.0;
+    for(int b = threadIdx.x; b < nblocks; b += blockDim.x)
+        gfd::kb_add(s, c, partials[(int64_t)b * 768 + e]);
+    block_reduce(s, c);
+    if(threadIdx.x == 0)
         out[e] = s + c;
-    }
 }
 
 // One v_mfma_f64_16x16x4f64 on given per-lane operands: out[4*lane + r] is
@@ -710,27 +821,66 @@ inline int check(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
 
 namespace gfk {
 
-static int ept_choice() { return 2; }
-
-int launch_reduce_metric(const GfProgArgs &a, int metric, Partial *partials, unsigned *counter, double *result,
-                         int nblocks, void *stream) {
-    hipStream_t s = (hipStream_t)stream;
-    switch(ept_choice()) {
-    case 1: hipLaunchKernelGGL(k_reduce_metric<1>, dim3(nblocks), dim3(GF_BLOCK), 0, s, a, metric, partials, counter,
-                               result); break;
-    default: hipLaunchKernelGGL(k_reduce_metric<2>, dim3(nblocks), dim3(GF_BLOCK), 0, s, a, metric, partials,
-                                counter, result); break;
-    }
+// Add the nblocks partials of the launch before it on stream s into *result.
+// A non-null a.mark event is recorded between the two launches (timing).
+static int launch_finish(const GfProgArgs &a, Partial *partials, int nblocks, GfResult *result, hipStream_t s) {
+    int rc = check(hipGetLastError());
+    if(rc)
+        return rc;
+    if(a.mark && (rc = check(hipEventRecord((hipEvent_t)a.mark, s))))
+        return rc;
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(GF_BLOCK), 0, s, partials, nblocks, result, a.err, a.seq);
     return check(hipGetLastError());
 }
 
-int launch_eval_metric(const GfProgArgs &a, int metric, double *out, int nblocks, void *stream) {
-    hipLaunchKernelGGL(k_eval_metric<2>, dim3(nblocks), dim3(GF_BLOCK), 0, (hipStream_t)stream, a, metric, out);
+int launch_reduce_metric(const GfProgArgs &a, const GfInline *inl, int metric, Partial *partials, GfResult *result,
+                         int nblocks, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if(inl)
+        hipLaunchKernelGGL((k_reduce_metric<2, true>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, *inl, metric, partials);
+    else
+        hipLaunchKernelGGL((k_reduce_metric<2, false>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, GfInline{}, metric,
+                           partials);
+    return launch_finish(a, partials, nblocks, result, s);
+}
+
+int launch_eval_metric(const GfProgArgs &a, const GfInline *inl, int metric, double *out, int nblocks, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if(inl)
+        hipLaunchKernelGGL((k_eval_metric<2, true>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, *inl, metric, out);
+    else
+        hipLaunchKernelGGL((k_eval_metric<2, false>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, GfInline{}, metric, out);
     return check(hipGetLastError());
+}
+
+template <bool WRITE, bool INL, bool CE>
+static void launch_fast_t(const GfProgArgs &a, const GfInline *inl, int metric, Partial *partials, double *out,
+                          int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_dalitz_fast<WRITE, INL, CE>), dim3(nblocks), dim3(256), 0, s, a, inl ? *inl : GfInline{},
+                       metric, partials, out);
+}
+
+int launch_dalitz_fast(const GfProgArgs &a, const GfInline *inl, bool const_eff, int metric, Partial *partials,
+                       GfResult *result, double *out, int nblocks, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    const int v   = (out ? 4 : 0) | (inl ? 2 : 0) | (const_eff ? 1 : 0);
+    switch(v) {
+    case 0: launch_fast_t<false, false, false>(a, inl, metric, partials, out, nblocks, s); break;
+    case 1: launch_fast_t<false, false, true>(a, inl, metric, partials, out, nblocks, s); break;
+    case 2: launch_fast_t<false, true, false>(a, inl, metric, partials, out, nblocks, s); break;
+    case 3: launch_fast_t<false, true, true>(a, inl, metric, partials, out, nblocks, s); break;
+    case 4: launch_fast_t<true, false, false>(a, inl, metric, partials, out, nblocks, s); break;
+    case 5: launch_fast_t<true, false, true>(a, inl, metric, partials, out, nblocks, s); break;
+    case 6: launch_fast_t<true, true, false>(a, inl, metric, partials, out, nblocks, s); break;
+    default: launch_fast_t<true, true, true>(a, inl, metric, partials, out, nblocks, s); break;
+    }
+    if(out)
+        return check(hipGetLastError());
+    return launch_finish(a, partials, nblocks, result, s);
 }
 
 int launch_reduce_bins(const GfProgArgs &a, int pc0, const double *ranges, const int *slots, int k, int64_t total,
-                       Partial *partials, unsigned *counter, double *result, int nblocks, void *stream) {
+                       Partial *partials, GfResult *result, int nblocks, void *stream) {
     GfGrid G;
     for(int d = 0; d < GF_MAX_OBS; ++d) {
         G.lo[d]   = d < k ? ranges[3 * d + 0] : 0.0;
@@ -739,9 +889,9 @@ int launch_reduce_bins(const GfProgArgs &a, int pc0, const double *ranges, const
         G.slot[d] = d < k ? slots[d] : 0;
     }
     G.k = k;
-    hipLaunchKernelGGL(k_reduce_bins<2>, dim3(nblocks), dim3(GF_BLOCK), 0, (hipStream_t)stream, a, pc0, G, total,
-                       partials, counter, result);
-    return check(hipGetLastError());
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_reduce_bins<2>, dim3(nblocks), dim3(GF_BLOCK), 0, s, a, pc0, G, total, partials);
+    return launch_finish(a, partials, nblocks, result, s);
 }
 
 int launch_dalitz_cache(const GfProgArgs &a, int slot, int64_t offset, int64_t n, uint32_t mask, const GfResDesc *res,
@@ -779,7 +929,7 @@ int launch_gram(const GfProgArgs &a, int mode, const double *grid6, const double
     int rc = check(hipGetLastError());
     if(rc)
         return rc;
-    hipLaunchKernelGGL(k_gram_final, dim3(1), dim3(256), 0, s, partials, nblocks, tiles_dev);
+    hipLaunchKernelGGL(k_gram_final, dim3(768), dim3(256), 0, s, partials, nblocks, tiles_dev);
     return check(hipGetLastError());
 }
 

@@ -116,9 +116,12 @@ void PdfBase::setIndices() {
     recursiveSetIndices();
 }
 
-void PdfBase::updateVariable(Variable var, fptype newValue) {
+// PdfBase::updateVariable, src/PDFs/PdfBase.cu:121-129: every parameter of the
+// tree with var's name takes the value.  Copies of one Variable share their value,
+// so the storage test settles most entries without comparing names.
+void PdfBase::updateVariable(const Variable &var, fptype newValue) {
     for(auto &p : parametersList)
-        if(p.getName() == var.getName())
+        if(p == var || p.getName() == var.getName())
             p.setValue(newValue);
     for(auto *c : components)
         c->updateVariable(var, newValue);

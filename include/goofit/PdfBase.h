@@ -124,7 +124,7 @@ class PdfBase {
 
     void setIntegrationFineness(int i);
     bool parametersChanged() const;
-    void updateVariable(Variable v, fptype newValue);
+    void updateVariable(const Variable &v, fptype newValue);
     void updateParameters();
 
     void SigGenSetIndices() {

@@ -93,6 +93,24 @@ void gf_ctx_destroy(gf_ctx *ctx);
 /* Message of the last failing call on this thread ("" if none). */
 const char *gf_last_error(void);
 
+/* Launch-shape knobs of a context.  No reference counterpart: the reference
+ * leaves the grid to thrust::transform_reduce (src/PDFs/GooPdf.cu:65-70,100-105);
+ * an autotuner, or a caller sharing the device with other work, sets them. */
+enum gf_tune {
+    GF_TUNE_INLINE_ARGS        = 0, /* 1: parameter and normalisation values travel in the kernel
+                                       arguments (no upload copy); 0: device buffers (default 1) */
+    GF_TUNE_FAST_DALITZ        = 1, /* 1: dedicated coherent-sum kernel for a lone Amp3Body (default 1) */
+    GF_TUNE_BLOCKS_PER_CU      = 2, /* interpreter reduction grid cap, workgroups per CU (1..64, default 4) */
+    GF_TUNE_FAST_BLOCKS_PER_CU = 3, /* coherent-sum kernel grid cap, workgroups per CU (1..64, default 4) */
+    GF_TUNE_SPIN_US            = 4, /* a reduction's caller polls the result for up to this many
+                                       microseconds before blocking on the stream (0..1e6, default 2000) */
+    GF_TUNE_COUNT              = 5
+};
+
+/* Set / read one knob (gf_tune); values out of range give GF_ERR_INVALID. */
+gf_status gf_ctx_set_tuning(gf_ctx *ctx, int key, int value);
+gf_status gf_ctx_get_tuning(gf_ctx *ctx, int key, int *value);
+
 /* ---- events ------------------------------------------------------------- */
 
 /* Upload n events as nobs host columns (cols[k] = observable with index k).
@@ -219,6 +237,12 @@ gf_status gf_dalitz_cache_gram(gf_ctx *ctx, int dalitz_func, double *out_re_im);
  * return the elapsed device time between HIP events recorded on that stream
  * before the first and after the last launch.  No reference counterpart. */
 gf_status gf_time_reduce_metric(gf_ctx *ctx, int metric, int reps, double *ms_total);
+
+/* Same launches (reps <= 4096), with events around each event sweep: ms_sweep is
+ * the summed device time of the sweep kernels alone (without the kernel that
+ * adds the per-workgroup partials), ms_total (optional) the whole span.
+ * No reference counterpart. */
+gf_status gf_time_reduce_sweep(gf_ctx *ctx, int metric, int reps, double *ms_sweep, double *ms_total);
 
 /* Same for gf_dalitz_cache (all resonances) and gf_dalitz_gram. */
 gf_status gf_time_dalitz_cache(gf_ctx *ctx, int dalitz_func, int reps, double *ms_total);

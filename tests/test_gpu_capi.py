@@ -282,3 +282,38 @@ def test_bad_program_rejected(ctx):
     f2[1] = 99
     with pytest.raises(capi.GooFitError):
         ctx.set_program(f2, p, c, o, n)
+
+
+def test_dalitz_nll_paths(ctx):
+    """Both Dalitz NLL paths against the oracle: the coherent-sum fast path with a
+    degree-1 efficiency polynomial, and the interpreter when the event numbers are
+    not the row index (a permutation)."""
+    n = 30_000
+    m12o = O.Obs("m12", 0.0, 3.0, 100)
+    m13o = O.Obs("m13", 0.0, 3.0, 100)
+    eff = O.MultiPolynomial("eff", [m12o, m13o], [O.Var("k0", 0.8), O.Var("k1", 0.1), O.Var("k2", -0.05)],
+                            [O.Var("o0", 0.5), O.Var("o1", 0.2)], 1)
+    top, _ = dalitz(nbins=100, eff=eff)
+    top.m12, top.m13 = m12o, m13o
+    top.observables[0], top.observables[1] = m12o, m13o
+    aos = dalitz_events(n, 17)
+    for permute in (False, True):
+        a = aos.copy()
+        if permute:
+            perm = np.random.default_rng(3).permutation(n)
+            a[:, 2] = perm.astype(float)
+        M = O.Model(top, a, decay=W.DALITZ_DECAY)
+        ref = M.calculate_nll()
+        _install(ctx, top, decay=W.DALITZ_DECAY)
+        ctx.upload_columns([a[:, 0], a[:, 1], a[:, 2]])
+        if not permute:
+            ctx.set_identity_column(2)
+        ctx.dalitz_set_data_size(top.function_idx, n)
+        M.op.sync_params()
+        ctx.update_params(M.op.p)
+        ctx.dalitz_cache(top.function_idx)
+        _sync(ctx, M)
+        got = ctx.reduce_metric("NLL")[0]
+        assert math.isclose(got, ref, rel_tol=NLL_RTOL), (permute, got, ref)
+        probs = ctx.eval_metric("PROB")
+        np.testing.assert_allclose(probs, M.op.eval_metric(a, "PROB"), rtol=1e-12, atol=0)

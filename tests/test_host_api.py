@@ -1,0 +1,134 @@
+"""The GooFit C++ API's host logic on the CPU: data sets, grids, the flattened
+tree, parameter handling, and that the device path fails loudly without a GPU.
+
+Known answers from the reference's tests/simple/SimpleTest.cpp and
+NormalizeTest.cpp; the flattened arrays are compared with the oracle's
+restatement of PdfBase::populateArrays (src/PDFs/PdfBase.cu:268-311)."""
+import numpy as np
+import pytest
+
+import goofit_amd as G
+import oracle as O
+from goofit_amd import workloads as W
+
+from _models import add1d, dalitz as oracle_dalitz, prod3d
+
+FLT_EPS = float(np.finfo(np.float32).eps)
+
+
+def catch_approx(a, e):
+    return abs(a - e) <= 100 * FLT_EPS * abs(e)
+
+
+def test_add_events_by_value():
+    """SimpleTest.cpp:11-45: addEvent() copies the observables' current values."""
+    x = G.Observable("xvar", 0, 10)
+    y = G.Observable("yvar", 0, 10)
+    d = G.UnbinnedDataSet([x, y])
+    for a, b in ((1, 2), (3, 4), (5, 6)):
+        x.value, y.value = a, b
+        d.addEvent()
+    assert d.getNumEvents() == 3
+    d.loadEvent(1)
+    assert x.value == 3 and y.value == 4
+    assert [d.getValue(x, i) for i in range(3)] == [1, 3, 5]
+    assert [d.getValue(y, i) for i in range(3)] == [2, 4, 6]
+
+
+def test_make_grid_order():
+    """SimpleTest.cpp:81-127: 100 x 100 grid, first observable fastest; EventNumber counts rows."""
+    x = G.Observable("xvar", 0, 10)
+    y = G.Observable("yvar", 0, 10)
+    n = G.EventNumber("eventNumber")
+    d = G.UnbinnedDataSet([x, y, n])
+    d.fillWithGrid()
+    assert d.getNumEvents() == 100 * 100
+    for row, (xv, yv) in {0: (0.05, 0.05), 1: (0.15, 0.05), 100: (0.05, 0.15)}.items():
+        d.loadEvent(row)
+        assert catch_approx(x.value, xv) and catch_approx(y.value, yv)
+        assert n.value == float(row)
+
+
+def test_pdf_make_grid_kat():
+    """NormalizeTest.cpp:57-91 and 105-140: makeGrid over each PDF's observables."""
+    x = G.Observable("xvar", -5, 10)
+    y = G.Observable("yvar", -5, 10)
+    g1 = G.GaussianPdf("g1", x, G.Variable("mean1", 1.5), G.Variable("sigma1", 2.5))
+    g2 = G.GaussianPdf("g2", y, G.Variable("mean2", -1), G.Variable("sigma2", 0.5))
+    d1 = g1.makeGrid()
+    for i, v in {0: -4.925, 20: -1.925, 40: 1.075, 60: 4.075}.items():
+        assert catch_approx(d1.getValue(x, i), v)
+    p = G.ProdPdf("product", [g1, g2])
+    d2 = p.makeGrid()
+    assert d2.getNumEvents() == 10000
+    for i, v in {0: -4.925, 20: -1.925, 40: 1.075, 60: 4.075}.items():
+        assert catch_approx(d2.getValue(x, i), v)
+        assert catch_approx(d2.getValue(y, 100 * i), v)
+
+
+def _flat_equal(top_oracle):
+    f, p, c, o, n = G.engine.flattened()
+    rf, rp, rc, ro, rn = O.flatten(top_oracle).arrays()
+    assert list(f) == list(rf)
+    assert np.array_equal(np.asarray(p), rp)
+    assert np.array_equal(np.asarray(c), rc)
+    assert np.array_equal(np.asarray(o), ro)
+    assert len(n) == len(rn)
+
+
+def test_flattened_tree_add1d():
+    m = W.build_add1d()
+    m.pdf.SigGenSetIndices()
+    top, _ = add1d(mu=m.params["mu"].value, sigma=m.params["sigma"].value, alpha=m.params["alpha"].value,
+                   frac=m.params["frac"].value)
+    _flat_equal(top)
+
+
+def test_flattened_tree_prod3d():
+    m = W.build_prod3d()
+    m.pdf.SigGenSetIndices()
+    top, _ = prod3d(mu=m.params["mu"].value, sigma=m.params["sigma"].value, alpha=m.params["alpha"].value,
+                    rate=m.params["rate"].value)
+    _flat_equal(top)
+
+
+def test_flattened_tree_dalitz():
+    """Amp3Body with 16 resonances (Amp3Body.cu:127-196): resonance blocks, efficiency last."""
+    m = W.build_dalitz(nbins=200)
+    m.pdf.SigGenSetIndices()
+    top, _ = oracle_dalitz(nbins=200)
+    _flat_equal(top)
+    names = G.engine.function_names()
+    assert len(names) == 18  # Amp3Body + 16 resonances + efficiency
+
+
+def test_minuit_vector_roundtrip_and_blinding():
+    """Params::from_minuit_vector (Params.cpp:47-57): values minus the blind offset reach the tree, and
+    only changed values are flagged (BlindTest.cpp:48-91 semantics)."""
+    m = W.build_add1d()
+    fm = G.FitManager(m.pdf)
+    par = fm.params
+    x = par.make_minuit_vector()
+    assert len(x) == par.size()
+    x2 = list(x)
+    x2[0] += 0.25
+    par.from_minuit_vector(x2)
+    assert par.make_minuit_vector()[0] == pytest.approx(x2[0])
+    f, p, _, _, _ = G.engine.flattened()
+    assert x2[0] in list(p)  # host_parameters updated in place (PdfBase::updateParameters)
+
+
+def test_device_path_fails_loudly_without_gpu():
+    """No CPU fallback: without a GPU every device entry point raises."""
+    from goofit_amd import capi
+    if capi.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(capi.GooFitError) as e:
+        capi.Context(0)
+    assert e.value.status == capi.GF_ERR_NODEVICE
+    m = W.build_add1d()
+    d = G.UnbinnedDataSet(m.observables)
+    d.from_columns([W.sample_add1d(100, seed=1)])
+    with pytest.raises(Exception):
+        m.pdf.setData(d)
+        m.pdf.calculateNLL()
