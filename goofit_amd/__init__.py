@@ -18,8 +18,8 @@ try:
     from . import _goofit  # noqa: F401
     from ._goofit import (  # noqa: F401
         AddPdf, Amp3Body, DataSet, DaughterPair, DecayInfo3, EvalFunc, EventNumber, ExpGausPdf, ExpPdf,
-        FitManager, FunctionMinimum, GaussianPdf, GeneralError, GooPdf, Observable, PAIR_12, PAIR_13, PAIR_23,
-        PdfBase, PolynomialPdf, ProdPdf, Resonances, UnbinnedDataSet, Variable, engine, inDalitz,
+        FCN, FitManager, FunctionMinimum, GaussianPdf, GeneralError, GooPdf, Observable, PAIR_12, PAIR_13, PAIR_23,
+        Params, PdfBase, PolynomialPdf, ProdPdf, Resonances, UnbinnedDataSet, Variable, engine, inDalitz,
     )
     HAVE_API = True
 except ImportError as _e:  # the binding is not built yet
