@@ -32,6 +32,9 @@ EXPORTS = [
     "gf_dalitz_set_data_size", "gf_dalitz_cache", "gf_dalitz_gram", "gf_dalitz_gram_points",
     "gf_dalitz_gram_points_device", "gf_dalitz_get_cache", "gf_dalitz_cache_gram",
     "gf_time_reduce_metric", "gf_time_reduce_sweep", "gf_time_dalitz_cache", "gf_time_dalitz_gram",
+    "gf_sample_uniform", "gf_mlp_param_count", "gf_mlp_create", "gf_mlp_destroy", "gf_mlp_get_params",
+    "gf_mlp_set_params", "gf_mlp_forward", "gf_mlp_gradient", "gf_mlp_train", "gf_mlp_build_proposal",
+    "gf_mlp_sample", "gf_mlp_train_host", "gf_dalitz_set_norm_points", "gf_dalitz_gram_norm_points",
 ]
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -85,6 +88,24 @@ def lib():
         L.gf_dalitz_cache_gram.argtypes = [c_p, ctypes.c_int, _D]
         L.gf_time_reduce_metric.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D]
         L.gf_time_reduce_sweep.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D, _D]
+        _F = ctypes.POINTER(ctypes.c_float)
+        L.gf_sample_uniform.argtypes = [c_p, _D, ctypes.c_int64, ctypes.c_uint64, c_p, c_p, c_p]
+        L.gf_mlp_param_count.restype = ctypes.c_int
+        L.gf_mlp_create.argtypes = [c_p, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(c_p)]
+        L.gf_mlp_destroy.argtypes = [c_p]
+        L.gf_mlp_destroy.restype = None
+        L.gf_mlp_get_params.argtypes = [c_p, _F, ctypes.c_int]
+        L.gf_mlp_set_params.argtypes = [c_p, _F, ctypes.c_int]
+        L.gf_mlp_forward.argtypes = [c_p, _F, ctypes.c_int64, _F]
+        L.gf_mlp_gradient.argtypes = [c_p, _F, _F, ctypes.c_int64, _F]
+        L.gf_mlp_train.argtypes = [c_p, c_p, c_p, ctypes.c_int64, _D, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                   ctypes.c_uint64, _F, _F]
+        L.gf_mlp_build_proposal.argtypes = [c_p, _D, ctypes.c_int, ctypes.c_double]
+        L.gf_mlp_sample.argtypes = [c_p, ctypes.c_int64, ctypes.c_uint64, c_p, c_p, c_p]
+        L.gf_mlp_train_host.argtypes = [c_p, _D, _D, ctypes.c_int64, _D, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                        ctypes.c_uint64, _F, _F]
+        L.gf_dalitz_set_norm_points.argtypes = [c_p, ctypes.c_int, c_p, ctypes.c_int64, ctypes.c_uint64, _D]
+        L.gf_dalitz_gram_norm_points.argtypes = [c_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _D]
         L.gf_time_dalitz_cache.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D]
         L.gf_time_dalitz_gram.argtypes = [c_p, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int, _D]
         _lib = L
@@ -187,6 +208,17 @@ class Context:
         d = _f64(decay6)
         _check(lib().gf_decay3_set(self.h, _dp(d)))
 
+    # -- Monte-Carlo normalisation points ---------------------------------------
+    def dalitz_set_norm_points(self, dalitz_func, n, seed, box=None, proposal=None):
+        b = np.ascontiguousarray(box if box is not None else [0, 0, 0, 0], np.float64)
+        _check(lib().gf_dalitz_set_norm_points(self.h, dalitz_func, proposal.h if proposal is not None else None,
+                                               n, seed, _dp(b)))
+
+    def dalitz_gram_norm_points(self, dalitz_func, eff_func, offset, count, nres):
+        out = np.zeros(2 * nres * nres)
+        _check(lib().gf_dalitz_gram_norm_points(self.h, dalitz_func, eff_func, offset, count, _dp(out)))
+        return out.view(np.complex128).reshape(nres, nres)
+
     # -- reductions -------------------------------------------------------
     def reduce_metric(self, metric="NLL"):
         s, c = ctypes.c_double(), ctypes.c_double()
@@ -272,3 +304,77 @@ class Context:
         g = _f64(grid6)
         _check(lib().gf_time_dalitz_gram(self.h, dalitz_func, _dp(g), eff_func, reps, ctypes.byref(ms)))
         return ms.value
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Proposal:
+    """A gf_mlp: the learned importance-sampling proposal of a Context."""
+
+    def __init__(self, ctx: Context, seed: int = 1, batch_cap: int = 65536):
+        h = ctypes.c_void_p()
+        _check(lib().gf_mlp_create(ctx.h, seed, batch_cap, ctypes.byref(h)))
+        self.h, self.ctx = h, ctx
+        self.nparam = lib().gf_mlp_param_count()
+
+    def close(self):
+        if self.h:
+            lib().gf_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.empty(self.nparam, np.float32)
+        _check(lib().gf_mlp_get_params(self.h, _fp(p), p.size))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().gf_mlp_set_params(self.h, _fp(p), p.size))
+
+    def forward(self, xy):
+        xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+        out = np.empty(xy.shape[0], np.float32)
+        _check(lib().gf_mlp_forward(self.h, _fp(xy), xy.shape[0], _fp(out)))
+        return out
+
+    def gradient(self, xy, coef):
+        xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+        coef = np.ascontiguousarray(coef, np.float32)
+        g = np.empty(self.nparam, np.float32)
+        _check(lib().gf_mlp_gradient(self.h, _fp(xy), _fp(coef), xy.shape[0], _fp(g)))
+        return g
+
+    def train(self, dev_m12: int, dev_m13: int, n: int, box, steps, batch, lr, seed):
+        b = np.ascontiguousarray(box, np.float64)
+        l0, l1 = ctypes.c_float(), ctypes.c_float()
+        _check(lib().gf_mlp_train(self.h, dev_m12, dev_m13, n, _dp(b), steps, batch, lr, seed, ctypes.byref(l0),
+                                  ctypes.byref(l1)))
+        return l0.value, l1.value
+
+    def train_host(self, m12, m13, box, steps, batch, lr, seed):
+        a, b = _f64(m12), _f64(m13)
+        bx = np.ascontiguousarray(box, np.float64)
+        l0, l1 = ctypes.c_float(), ctypes.c_float()
+        _check(lib().gf_mlp_train_host(self.h, _dp(a), _dp(b), a.size, _dp(bx), steps, batch, lr, seed,
+                                       ctypes.byref(l0), ctypes.byref(l1)))
+        return l0.value, l1.value
+
+    def build(self, box, grid=512, alpha=0.1):
+        b = np.ascontiguousarray(box, np.float64)
+        _check(lib().gf_mlp_build_proposal(self.h, _dp(b), grid, alpha))
+
+    def sample(self, n, seed, dev_m12: int, dev_m13: int, dev_w: int):
+        _check(lib().gf_mlp_sample(self.h, n, seed, dev_m12, dev_m13, dev_w))
+
+
+def sample_uniform(ctx: Context, box, n, seed, dev_m12: int, dev_m13: int, dev_w: int):
+    b = np.ascontiguousarray(box, np.float64)
+    _check(lib().gf_sample_uniform(ctx.h, _dp(b), n, seed, dev_m12, dev_m13, dev_w))

@@ -14,6 +14,9 @@
 
 #include "../../include/goofit_hip.h"
 #include "gf_internal.h"
+#include "gf_mlp.h"
+
+#include <random>
 
 namespace {
 
@@ -88,6 +91,14 @@ struct gf_ctx {
     GfDecay decay{};
 
     std::map<int, WaveCache> caches;
+
+    // Monte-Carlo normalisation points per Amp3Body (keyed by cacheToUse):
+    // buf = m12[n] | m13[n] | w[n]
+    struct NormPoints {
+        double *buf = nullptr;
+        int64_t n = 0, cap = 0;
+    };
+    std::map<int, NormPoints> norm_points;
 
     // reduction scratch
     gfk::Partial *d_partials = nullptr;
@@ -979,6 +990,9 @@ void gf_ctx_destroy(gf_ctx *c) {
     for(auto &kv : c->caches)
         if(kv.second.wave)
             (void)hipFree(kv.second.wave);
+    for(auto &kv : c->norm_points)
+        if(kv.second.buf)
+            (void)hipFree(kv.second.buf);
     void *dev[] = {c->col_base,  c->d_cols,   c->d_params,   c->d_consts,        c->d_norms,
                    c->d_obsid,   c->d_ins,    c->d_dz,       c->d_res,           c->d_partials,
                    c->d_counter, c->d_result, c->d_err, c->d_gram_partials, c->d_gram_tiles};
@@ -1619,6 +1633,453 @@ gf_status gf_time_dalitz_gram(gf_ctx *ctx, int dalitz_func, const double grid6[6
     GF_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     *ms_total = ms;
     return GF_OK;
+}
+
+
+// ------------------------------------------------------------ learned proposal
+
+}  // extern "C"
+
+struct gf_mlp {
+    gf_ctx *ctx = nullptr;
+    GfMlpDev D{};
+    int adam_t = 0;
+    // tabulated proposal
+    double *d_cdf = nullptr, *d_pcell = nullptr;
+    int G         = 0;
+    GfBox box{};
+};
+
+namespace {
+
+void mlp_free(gf_mlp *m) {
+    void *p[] = {m->D.P,     m->D.m,       m->D.v,       m->D.grad,     m->D.w2b,     m->D.w2t,   m->D.xy,
+                 m->D.z1,    m->D.z2,      m->D.h1,      m->D.delta2,   m->D.out,     m->D.coef,  m->D.loss,
+                 m->D.part_in, m->D.part_w2, m->D.part_out, m->d_cdf, m->d_pcell};
+    for(void *q : p)
+        if(q)
+            (void)hipFree(q);
+}
+
+// The kernels assume lane l of v_mfma_f32_16x16x32_bf16 takes A[l%16][8(l/16)+t],
+// B[8(l/16)+t][l%16] and returns D[GF_BF16_D_ROW(l, r)][l%16]: verify on the device.
+gf_status mlp_layout_check(gf_ctx *ctx) {
+    std::vector<uint16_t> a8(64 * 8), b8(64 * 8);
+    std::vector<float> A(16 * 32), B(32 * 16), d(256);
+    std::mt19937 g(7);
+    auto bf = [](float x) {
+        uint32_t u;
+        memcpy(&u, &x, 4);
+        return (uint16_t)(u >> 16);  // small integers: exact
+    };
+    for(auto &x : A)
+        x = (float)((int)(g() % 7) - 3);
+    for(auto &x : B)
+        x = (float)((int)(g() % 7) - 3);
+    for(int l = 0; l < 64; l++)
+        for(int t = 0; t < 8; t++) {
+            a8[8 * l + t] = bf(A[(l % 16) * 32 + 8 * (l / 16) + t]);
+            b8[8 * l + t] = bf(B[(8 * (l / 16) + t) * 16 + l % 16]);
+        }
+    uint16_t *da = nullptr;
+    float *dd    = nullptr;
+    GF_HIP(hipMalloc((void **)&da, 2 * 64 * 8 * sizeof(uint16_t)));
+    if(hipMalloc((void **)&dd, 256 * sizeof(float)) != hipSuccess) {
+        (void)hipFree(da);
+        return fail(GF_ERR_NOMEM, "probe buffer");
+    }
+    gf_status st = GF_OK;
+    if(hipMemcpy(da, a8.data(), 512 * 2, hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(da + 512, b8.data(), 512 * 2, hipMemcpyHostToDevice) != hipSuccess
+       || gfm::launch_bf16_probe(da, da + 512, dd, ctx->stream)
+       || hipStreamSynchronize(ctx->stream) != hipSuccess
+       || hipMemcpy(d.data(), dd, 256 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        st = fail(GF_ERR_HIP, "BF16 MFMA probe failed");
+    (void)hipFree(da);
+    (void)hipFree(dd);
+    if(st)
+        return st;
+    for(int l = 0; l < 64; l++)
+        for(int r = 0; r < 4; r++) {
+            const int i = GF_BF16_D_ROW(l, r), j = l % 16;
+            float want  = 0.f;
+            for(int k = 0; k < 32; k++)
+                want += A[i * 32 + k] * B[k * 16 + j];
+            if(d[4 * l + r] != want)
+                return fail(GF_ERR_HIP, "BF16 MFMA result layout differs from the compiled one (lane %d reg %d)", l, r);
+        }
+    return GF_OK;
+}
+
+inline int64_t round64(int64_t n) { return (n + 63) / 64 * 64; }
+
+gf_status mlp_copy_xy(gf_mlp *m, const float *xy, int64_t n) {
+    GF_HIP(hipMemcpyAsync(m->D.xy, xy, (size_t)n * 2 * sizeof(float), hipMemcpyHostToDevice, m->ctx->stream));
+    const int64_t np = round64(n);
+    if(np > n)
+        GF_HIP(hipMemsetAsync(m->D.xy + n, 0, (size_t)(np - n) * 2 * sizeof(float), m->ctx->stream));
+    return GF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gf_mlp_param_count(void) { return GF_MLP_NPARAM; }
+
+gf_status gf_mlp_create(gf_ctx *ctx, uint64_t seed, int batch_cap, gf_mlp **out) {
+    if(!out)
+        return fail(GF_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(batch_cap < 64 || batch_cap > (1 << 22))
+        return fail(GF_ERR_INVALID, "batch_cap %d outside 64..4194304", batch_cap);
+    gf_status st = mlp_layout_check(ctx);
+    if(st)
+        return st;
+    gf_mlp *m  = new gf_mlp();
+    m->ctx     = ctx;
+    GfMlpDev &D = m->D;
+    D.cap       = round64(batch_cap);
+    const size_t cap = (size_t)D.cap, H = GF_MLP_H;
+    const size_t nb = cap / 64;
+    bool ok = hipMalloc((void **)&D.P, GF_MLP_NPARAM * 4) == hipSuccess
+              && hipMalloc((void **)&D.m, GF_MLP_NPARAM * 4) == hipSuccess
+              && hipMalloc((void **)&D.v, GF_MLP_NPARAM * 4) == hipSuccess
+              && hipMalloc((void **)&D.grad, GF_MLP_NPARAM * 4) == hipSuccess
+              && hipMalloc((void **)&D.w2b, H * H * 2) == hipSuccess
+              && hipMalloc((void **)&D.w2t, H * H * 2) == hipSuccess
+              && hipMalloc((void **)&D.xy, cap * 8) == hipSuccess
+              && hipMalloc((void **)&D.z1, cap * H * 4) == hipSuccess
+              && hipMalloc((void **)&D.z2, cap * H * 4) == hipSuccess
+              && hipMalloc((void **)&D.h1, cap * H * 2) == hipSuccess
+              && hipMalloc((void **)&D.delta2, cap * H * 2) == hipSuccess
+              && hipMalloc((void **)&D.out, cap * 4) == hipSuccess
+              && hipMalloc((void **)&D.coef, cap * 4) == hipSuccess
+              && hipMalloc((void **)&D.loss, 64) == hipSuccess
+              && hipMalloc((void **)&D.part_in, nb * GF_MLP_PART_IN * 4) == hipSuccess
+              && hipMalloc((void **)&D.part_w2, (size_t)GF_MLP_W2_BLOCKS * H * H * 4) == hipSuccess
+              && hipMalloc((void **)&D.part_out, nb * GF_MLP_PART_OUT * 4) == hipSuccess;
+    if(!ok) {
+        mlp_free(m);
+        delete m;
+        return fail(GF_ERR_NOMEM, "cannot allocate the network buffers (batch_cap %d)", batch_cap);
+    }
+    // seeded initial weights: first-layer ridges spread over the square, Kaiming-uniform
+    // second layer, small output layer
+    std::vector<float> P(GF_MLP_NPARAM, 0.f);
+    std::mt19937_64 g(seed);
+    std::uniform_real_distribution<float> u1(-3.f, 3.f);
+    const float b2 = std::sqrt(6.f / GF_MLP_H);
+    std::uniform_real_distribution<float> u2(-b2, b2), u3(-0.05f, 0.05f);
+    for(int i = GF_MLP_W1; i < GF_MLP_W2; i++)
+        P[i] = u1(g);
+    for(int i = GF_MLP_W2; i < GF_MLP_B2; i++)
+        P[i] = u2(g);
+    for(int i = GF_MLP_W3; i < GF_MLP_B3; i++)
+        P[i] = u3(g);
+    st = gf_mlp_set_params(m, P.data(), GF_MLP_NPARAM);
+    if(st) {
+        gf_mlp_destroy(m);
+        return st;
+    }
+    *out = m;
+    return GF_OK;
+}
+
+void gf_mlp_destroy(gf_mlp *m) {
+    if(!m)
+        return;
+    if(m->ctx) {
+        (void)hipSetDevice(m->ctx->device);
+        (void)hipStreamSynchronize(m->ctx->stream);
+    }
+    mlp_free(m);
+    delete m;
+}
+
+gf_status gf_mlp_set_params(gf_mlp *m, const float *in, int n) {
+    if(!m || !in || n != GF_MLP_NPARAM)
+        return fail(GF_ERR_INVALID, "parameter vector must have %d entries", GF_MLP_NPARAM);
+    if(!valid_ctx(m->ctx))
+        return GF_ERR_INVALID;
+    hipStream_t s = m->ctx->stream;
+    GF_HIP(hipMemcpyAsync(m->D.P, in, GF_MLP_NPARAM * 4, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemsetAsync(m->D.m, 0, GF_MLP_NPARAM * 4, s));
+    GF_HIP(hipMemsetAsync(m->D.v, 0, GF_MLP_NPARAM * 4, s));
+    m->adam_t = 0;
+    if(gfm::launch_w2_bf16(m->D, s))
+        return fail(GF_ERR_HIP, "bf16 conversion launch failed");
+    GF_HIP(hipStreamSynchronize(s));
+    return GF_OK;
+}
+
+gf_status gf_mlp_get_params(gf_mlp *m, float *out, int n) {
+    if(!m || !out || n != GF_MLP_NPARAM)
+        return fail(GF_ERR_INVALID, "parameter vector must have %d entries", GF_MLP_NPARAM);
+    if(!valid_ctx(m->ctx))
+        return GF_ERR_INVALID;
+    GF_HIP(hipMemcpyAsync(out, m->D.P, GF_MLP_NPARAM * 4, hipMemcpyDeviceToHost, m->ctx->stream));
+    GF_HIP(hipStreamSynchronize(m->ctx->stream));
+    return GF_OK;
+}
+
+gf_status gf_mlp_forward(gf_mlp *m, const float *xy, int64_t n, float *out) {
+    if(!m || !xy || !out || n < 1)
+        return fail(GF_ERR_INVALID, "null or empty arrays");
+    if(!valid_ctx(m->ctx))
+        return GF_ERR_INVALID;
+    hipStream_t s = m->ctx->stream;
+    for(int64_t off = 0; off < n; off += m->D.cap) {
+        const int64_t c = std::min<int64_t>(m->D.cap, n - off);
+        gf_status st    = mlp_copy_xy(m, xy + 2 * off, c);
+        if(st)
+            return st;
+        if(gfm::launch_forward(m->D, m->D.xy, (int)round64(c), m->D.out, s))
+            return fail(GF_ERR_HIP, "forward launch failed");
+        GF_HIP(hipMemcpyAsync(out + off, m->D.out, (size_t)c * 4, hipMemcpyDeviceToHost, s));
+    }
+    GF_HIP(hipStreamSynchronize(s));
+    return GF_OK;
+}
+
+gf_status gf_mlp_gradient(gf_mlp *m, const float *xy, const float *coef, int64_t n, float *grad) {
+    if(!m || !xy || !coef || !grad || n < 1)
+        return fail(GF_ERR_INVALID, "null or empty arrays");
+    if(n > m->D.cap)
+        return fail(GF_ERR_INVALID, "%lld points exceed batch_cap %lld", (long long)n, (long long)m->D.cap);
+    if(!valid_ctx(m->ctx))
+        return GF_ERR_INVALID;
+    hipStream_t s    = m->ctx->stream;
+    const int64_t np = round64(n);
+    gf_status st     = mlp_copy_xy(m, xy, n);
+    if(st)
+        return st;
+    GF_HIP(hipMemcpyAsync(m->D.coef, coef, (size_t)n * 4, hipMemcpyHostToDevice, s));
+    if(np > n)
+        GF_HIP(hipMemsetAsync(m->D.coef + n, 0, (size_t)(np - n) * 4, s));
+    if(gfm::launch_forward(m->D, m->D.xy, (int)np, m->D.out, s)
+       || gfm::launch_backward(m->D, m->D.xy, m->D.coef, (int)np, m->D.grad, s))
+        return fail(GF_ERR_HIP, "forward/backward launch failed");
+    GF_HIP(hipMemcpyAsync(grad, m->D.grad, GF_MLP_NPARAM * 4, hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    return GF_OK;
+}
+
+gf_status gf_mlp_train(gf_mlp *m, const double *dev_m12, const double *dev_m13, int64_t ndata, const double box[4],
+                       int steps, int batch, float lr, uint64_t seed, float *loss_first, float *loss_last) {
+    if(!m || !dev_m12 || !dev_m13 || !box || ndata < 1 || steps < 1)
+        return fail(GF_ERR_INVALID, "bad training arguments");
+    if(batch < 128 || batch % 128 || batch > m->D.cap)
+        return fail(GF_ERR_INVALID, "batch %d must be a multiple of 128 and <= batch_cap %lld", batch,
+                    (long long)m->D.cap);
+    if(!(box[1] > box[0]) || !(box[3] > box[2]))
+        return fail(GF_ERR_INVALID, "empty box");
+    if(!valid_ctx(m->ctx))
+        return GF_ERR_INVALID;
+    hipStream_t s = m->ctx->stream;
+    const GfBox B{box[0], box[1], box[2], box[3]};
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const int half = batch / 2;
+    for(int k = 0; k < steps; k++) {
+        const uint64_t key = seed * 0x9E3779B97F4A7C15ull + 2ull * (uint64_t)k + 1ull;
+        const double cosf_ = 0.5 * (1.0 + std::cos(M_PI * (double)k / (double)steps));
+        const float lr_k   = (float)(lr * (0.05 + 0.95 * cosf_));
+        m->adam_t++;
+        const float c1 = 1.f - std::pow(b1, (float)m->adam_t), c2 = 1.f - std::pow(b2, (float)m->adam_t);
+        if(gfm::launch_batch_train(dev_m12, dev_m13, ndata, B, half, half, key, m->D.xy, s)
+           || gfm::launch_forward(m->D, m->D.xy, batch, m->D.out, s)
+           || gfm::launch_loss(m->D.out, half, half, m->D.coef, m->D.loss, s)
+           || gfm::launch_backward(m->D, m->D.xy, m->D.coef, batch, m->D.grad, s)
+           || gfm::launch_adam(m->D, m->D.grad, lr_k, b1, b2, eps, c1, c2, s))
+            return fail(GF_ERR_HIP, "training step %d launch failed", k);
+        if(k == 0 && loss_first) {
+            GF_HIP(hipMemcpyAsync(loss_first, m->D.loss, 4, hipMemcpyDeviceToHost, s));
+            GF_HIP(hipStreamSynchronize(s));
+        }
+    }
+    if(loss_last)
+        GF_HIP(hipMemcpyAsync(loss_last, m->D.loss, 4, hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    return GF_OK;
+}
+
+gf_status gf_mlp_build_proposal(gf_mlp *m, const double box[4], int grid, double alpha) {
+    if(!m || !box)
+        return fail(GF_ERR_INVALID, "null argument");
+    if(grid < 8 || grid > 4096 || grid % 8)
+        return fail(GF_ERR_INVALID, "grid %d must be a multiple of 8 in 8..4096", grid);
+    if(!(alpha > 0.0 && alpha <= 1.0))
+        return fail(GF_ERR_INVALID, "alpha %g outside (0, 1]", alpha);
+    if(!(box[1] > box[0]) || !(box[3] > box[2]))
+        return fail(GF_ERR_INVALID, "empty box");
+    if(!valid_ctx(m->ctx))
+        return GF_ERR_INVALID;
+    hipStream_t s      = m->ctx->stream;
+    const int64_t nc   = (int64_t)grid * grid;
+    float2 *gxy        = nullptr;
+    float *gout        = nullptr;
+    GF_HIP(hipMalloc((void **)&gxy, (size_t)nc * 8));
+    if(hipMalloc((void **)&gout, (size_t)nc * 4) != hipSuccess) {
+        (void)hipFree(gxy);
+        return fail(GF_ERR_NOMEM, "proposal table");
+    }
+    std::vector<float> f(nc);
+    gf_status st = GF_OK;
+    if(gfm::launch_batch_grid(grid, gxy, s))
+        st = fail(GF_ERR_HIP, "grid launch failed");
+    for(int64_t off = 0; !st && off < nc; off += m->D.cap) {
+        const int64_t c = std::min<int64_t>(m->D.cap, nc - off);  // multiple of 64: grid % 8 == 0
+        if(gfm::launch_forward(m->D, gxy + off, (int)c, gout + off, s))
+            st = fail(GF_ERR_HIP, "forward launch failed");
+    }
+    if(!st && (hipMemcpyAsync(f.data(), gout, (size_t)nc * 4, hipMemcpyDeviceToHost, s) != hipSuccess
+               || hipStreamSynchronize(s) != hipSuccess))
+        st = fail(GF_ERR_HIP, "copy of the table failed");
+    (void)hipFree(gxy);
+    (void)hipFree(gout);
+    if(st)
+        return st;
+    double fmax = -INFINITY;
+    for(float v : f)
+        if(std::isfinite(v))
+            fmax = std::max(fmax, (double)v);
+    if(!std::isfinite(fmax))
+        return fail(GF_ERR_INVALID, "the network output is not finite anywhere");
+    std::vector<double> p(nc), cdf(nc);
+    double z = 0.0;
+    for(int64_t c = 0; c < nc; c++) {
+        p[c] = std::isfinite(f[c]) ? std::exp((double)f[c] - fmax) : 0.0;
+        z += p[c];
+    }
+    double acc = 0.0;
+    for(int64_t c = 0; c < nc; c++) {
+        p[c] = (1.0 - alpha) * p[c] / z + alpha / (double)nc;
+        acc += p[c];
+        cdf[c] = acc;
+    }
+    for(int64_t c = 0; c < nc; c++)  // exact 1 at the end; p matches the normalised table
+        cdf[c] /= acc;
+    for(int64_t c = 0; c < nc; c++)
+        p[c] /= acc;
+    cdf[nc - 1] = 1.0;
+    if(m->G != grid) {
+        if(m->d_cdf)
+            (void)hipFree(m->d_cdf);
+        if(m->d_pcell)
+            (void)hipFree(m->d_pcell);
+        m->d_cdf = m->d_pcell = nullptr;
+        m->G                  = 0;
+        GF_HIP(hipMalloc((void **)&m->d_cdf, (size_t)nc * 8));
+        GF_HIP(hipMalloc((void **)&m->d_pcell, (size_t)nc * 8));
+    }
+    GF_HIP(hipMemcpyAsync(m->d_cdf, cdf.data(), (size_t)nc * 8, hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(m->d_pcell, p.data(), (size_t)nc * 8, hipMemcpyHostToDevice, s));
+    GF_HIP(hipStreamSynchronize(s));
+    m->G   = grid;
+    m->box = GfBox{box[0], box[1], box[2], box[3]};
+    return GF_OK;
+}
+
+gf_status gf_mlp_sample(gf_mlp *m, int64_t n, uint64_t seed, double *dev_m12, double *dev_m13, double *dev_w) {
+    if(!m || !dev_m12 || !dev_m13 || !dev_w || n < 1)
+        return fail(GF_ERR_INVALID, "null or empty arrays");
+    if(!m->G)
+        return fail(GF_ERR_STATE, "no proposal table (gf_mlp_build_proposal)");
+    if(!valid_ctx(m->ctx))
+        return GF_ERR_INVALID;
+    hipStream_t s = m->ctx->stream;
+    if(gfm::launch_sample(m->d_cdf, m->d_pcell, m->G, m->box, n, seed, dev_m12, dev_m13, dev_w, s))
+        return fail(GF_ERR_HIP, "sampling launch failed");
+    GF_HIP(hipStreamSynchronize(s));
+    return GF_OK;
+}
+
+gf_status gf_sample_uniform(gf_ctx *ctx, const double box[4], int64_t n, uint64_t seed, double *dev_m12,
+                            double *dev_m13, double *dev_w) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(!box || !dev_m12 || !dev_m13 || !dev_w || n < 1)
+        return fail(GF_ERR_INVALID, "null or empty arrays");
+    if(!(box[1] > box[0]) || !(box[3] > box[2]))
+        return fail(GF_ERR_INVALID, "empty box");
+    if(gfm::launch_sample_uniform(GfBox{box[0], box[1], box[2], box[3]}, n, seed, dev_m12, dev_m13, dev_w,
+                                  ctx->stream))
+        return fail(GF_ERR_HIP, "sampling launch failed");
+    GF_HIP(hipStreamSynchronize(ctx->stream));
+    return GF_OK;
+}
+
+gf_status gf_mlp_train_host(gf_mlp *m, const double *m12, const double *m13, int64_t ndata, const double box[4],
+                            int steps, int batch, float lr, uint64_t seed, float *loss_first, float *loss_last) {
+    if(!m || !m12 || !m13 || ndata < 1)
+        return fail(GF_ERR_INVALID, "null or empty event arrays");
+    if(!valid_ctx(m->ctx))
+        return GF_ERR_INVALID;
+    double *buf = nullptr;
+    GF_HIP(hipMalloc((void **)&buf, (size_t)ndata * 2 * sizeof(double)));
+    gf_status st = GF_OK;
+    if(hipMemcpy(buf, m12, (size_t)ndata * 8, hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(buf + ndata, m13, (size_t)ndata * 8, hipMemcpyHostToDevice) != hipSuccess)
+        st = fail(GF_ERR_HIP, "copy of the training events failed");
+    if(!st)
+        st = gf_mlp_train(m, buf, buf + ndata, ndata, box, steps, batch, lr, seed, loss_first, loss_last);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    (void)hipFree(buf);
+    return st;
+}
+
+gf_status gf_dalitz_set_norm_points(gf_ctx *ctx, int dalitz_func, gf_mlp *proposal, int64_t n, uint64_t seed,
+                                    const double box[4]) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    DalitzNode *d = find_dalitz(ctx, dalitz_func, nullptr);
+    if(!d)
+        return fail(GF_ERR_INVALID, "function %d is not an Amp3Body node", dalitz_func);
+    if(n < 1)
+        return fail(GF_ERR_INVALID, "need at least one point");
+    if(proposal && proposal->ctx != ctx)
+        return fail(GF_ERR_INVALID, "the proposal belongs to another context");
+    if(!proposal && !box)
+        return fail(GF_ERR_INVALID, "uniform points need the box");
+    auto &np = ctx->norm_points[d->cache_key];
+    if(np.cap < n) {
+        if(np.buf) {
+            GF_HIP(hipStreamSynchronize(ctx->stream));
+            GF_HIP(hipFree(np.buf));
+        }
+        np.buf = nullptr;
+        np.cap = np.n = 0;
+        if(hipMalloc((void **)&np.buf, (size_t)n * 3 * sizeof(double)) != hipSuccess)
+            return fail(GF_ERR_NOMEM, "cannot allocate %lld normalisation points", (long long)n);
+        np.cap = n;
+    }
+    np.n         = 0;
+    gf_status st = proposal ? gf_mlp_sample(proposal, n, seed, np.buf, np.buf + n, np.buf + 2 * n)
+                            : gf_sample_uniform(ctx, box, n, seed, np.buf, np.buf + n, np.buf + 2 * n);
+    if(st)
+        return st;
+    np.n = n;
+    return GF_OK;
+}
+
+gf_status gf_dalitz_gram_norm_points(gf_ctx *ctx, int dalitz_func, int eff_func, int64_t offset, int64_t count,
+                                     double *out_re_im) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    DalitzNode *d = find_dalitz(ctx, dalitz_func, nullptr);
+    if(!d)
+        return fail(GF_ERR_INVALID, "function %d is not an Amp3Body node", dalitz_func);
+    auto it = ctx->norm_points.find(d->cache_key);
+    if(it == ctx->norm_points.end() || it->second.n == 0)
+        return fail(GF_ERR_STATE, "no normalisation points (gf_dalitz_set_norm_points)");
+    const auto &np = it->second;
+    if(offset < 0 || count < 1 || offset + count > np.n)
+        return fail(GF_ERR_INVALID, "points [%lld, %lld) outside the %lld stored", (long long)offset,
+                    (long long)(offset + count), (long long)np.n);
+    return gf_dalitz_gram_points_device(ctx, dalitz_func, np.buf + offset, np.buf + np.n + offset,
+                                        np.buf + 2 * np.n + offset, count, eff_func, out_re_im);
 }
 
 }  // extern "C"

@@ -102,7 +102,7 @@ def dalitz_couplings(seed=3, n=len(DALITZ_RESONANCES), fixed_index=1):
     mag = rng.uniform(0.1, 2.0, size=n)
     ph = rng.uniform(0.0, 2 * math.pi, size=n)
     c = mag * np.exp(1j * ph)
-    if fixed_index is not None:
+    if fixed_index is not None and fixed_index < n:
         c[fixed_index] = 1.0 + 0.0j
     return c
 

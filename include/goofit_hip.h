@@ -231,6 +231,66 @@ gf_status gf_dalitz_get_cache(gf_ctx *ctx, int dalitz_func, int res, double *out
  * Amp3Body::fit_fractions (src/PDFs/physics/Amp3Body.cu:376-428). */
 gf_status gf_dalitz_cache_gram(gf_ctx *ctx, int dalitz_func, double *out_re_im);
 
+/* ---- normalisation by Monte Carlo: uniform and learned proposals ------------
+ * No reference counterpart (the reference integrates Amp3Body on a fixed grid,
+ * src/PDFs/physics/detail/SpecialResonanceIntegrator.cu:52-121).  Point sets go
+ * to gf_dalitz_gram_points_device: with weights w_p = 1 / (N q(x_p)) the Gram
+ * sum is an unbiased estimate of the integrals over the box. */
+
+/* n uniform points of the box {l12, u12, l13, u13} into device arrays, w = area / n.
+ * Counter-based generator: the same seed gives the same points. */
+gf_status gf_sample_uniform(gf_ctx *ctx, const double box[4], int64_t n, uint64_t seed, double *dev_m12,
+                            double *dev_m13, double *dev_w);
+
+typedef struct gf_mlp gf_mlp;
+
+/* Number of parameters of the proposal network (2 -> 128 -> 128 -> 1, SiLU):
+ * W1[128][2], b1[128], W2[128][128] (row = output neuron), b2[128], w3[128], b3. */
+int gf_mlp_param_count(void);
+
+/* A network on ctx's device with seeded initial weights; batch_cap bounds the
+ * samples of one forward/backward pass (rounded up to a multiple of 64).
+ * Fails with GF_ERR_HIP if the device's BF16 MFMA result layout is not the one
+ * the kernels were written for. */
+gf_status gf_mlp_create(gf_ctx *ctx, uint64_t seed, int batch_cap, gf_mlp **out);
+void gf_mlp_destroy(gf_mlp *m);
+gf_status gf_mlp_get_params(gf_mlp *m, float *out, int n);
+gf_status gf_mlp_set_params(gf_mlp *m, const float *in, int n);
+
+/* f(x, y) for n host points xy[2i], xy[2i+1] in [-1, 1]^2 (u = (x+1)/2 of the box). */
+gf_status gf_mlp_forward(gf_mlp *m, const float *xy, int64_t n, float *out);
+
+/* grad = sum_i coef[i] * d f(x_i) / d theta over n host points (n <= batch_cap). */
+gf_status gf_mlp_gradient(gf_mlp *m, const float *xy, const float *coef, int64_t n, float *grad);
+
+/* Maximum-likelihood training on ndata events (device arrays m12, m13 inside box):
+ * `steps` Adam steps (cosine-decayed learning rate from lr), each on batch/2 data
+ * events drawn with replacement and batch/2 uniform points of the box
+ * (loss = -mean_data f + log mean_uniform exp f).  loss_first/loss_last may be NULL. */
+gf_status gf_mlp_train(gf_mlp *m, const double *dev_m12, const double *dev_m13, int64_t ndata, const double box[4],
+                       int steps, int batch, float lr, uint64_t seed, float *loss_first, float *loss_last);
+
+/* Same, with the events in host arrays. */
+gf_status gf_mlp_train_host(gf_mlp *m, const double *m12, const double *m13, int64_t ndata, const double box[4],
+                            int steps, int batch, float lr, uint64_t seed, float *loss_first, float *loss_last);
+
+/* Tabulate the proposal on a grid x grid cell-centre table over box:
+ * p_cell = (1 - alpha) exp(f) / sum exp(f) + alpha / grid^2 (0 < alpha <= 1). */
+gf_status gf_mlp_build_proposal(gf_mlp *m, const double box[4], int grid, double alpha);
+
+/* n points of the tabulated proposal and their weights 1 / (n q(x)). */
+gf_status gf_mlp_sample(gf_mlp *m, int64_t n, uint64_t seed, double *dev_m12, double *dev_m13, double *dev_w);
+
+/* Draw and keep n normalisation points for Amp3Body dalitz_func: from the
+ * tabulated proposal when proposal is not NULL, else uniform over box. */
+gf_status gf_dalitz_set_norm_points(gf_ctx *ctx, int dalitz_func, gf_mlp *proposal, int64_t n, uint64_t seed,
+                                    const double box[4]);
+
+/* Integrals I_ij = sum_p w_p eff_p B_i(p) conj(B_j(p)) over the kept points
+ * [offset, offset + count) (weights as drawn for all n points). */
+gf_status gf_dalitz_gram_norm_points(gf_ctx *ctx, int dalitz_func, int eff_func, int64_t offset, int64_t count,
+                                     double *out_re_im);
+
 /* ---- timing ------------------------------------------------------------- */
 
 /* Run gf_reduce_metric `reps` times back to back on the context stream and
