@@ -80,6 +80,81 @@ def time_steps(pdf, params, steps, warmup, sync):
     return t1 - t0
 
 
+def norm_error_study(model, pdf, train_events, sync, target=1e-4, seeds=6):
+    """Normalisation-integral error at equal Monte-Carlo points, uniform phase-space MC vs the
+    learned proposal, against a 1e8-point uniform FP64 judge; then a MIGRAD fit with each method
+    at the point count that reaches `target` relative error.  Leaves pdf on its midpoint grid."""
+    import goofit_amd as G
+    from goofit_amd import workloads as W
+    out = {}
+    truth = {n: v.value for n, v in model.params.items()}
+    grid_norm = pdf.normalize()
+    t0 = time.perf_counter()
+    pdf.setNormalisationPoints(100_000_000, 991)
+    judge = pdf.normalize()
+    sync()
+    out["judge"] = dict(points=100_000_000, seconds=round(time.perf_counter() - t0, 3),
+                        rel_stat_err=math.sqrt(pdf.normalisationRelativeVariance(16) / 1e8),
+                        grid_vs_judge=grid_norm / judge - 1)
+    log(f"judge: 1e8 uniform points, grid/judge - 1 = {grid_norm / judge - 1:.2e}")
+
+    def errors(n, proposal, base):
+        e, v = [], []
+        for k in range(seeds):
+            pdf.setNormalisationPoints(n, base + k, proposal)
+            e.append(pdf.normalize() / judge - 1)
+            v.append(pdf.normalisationRelativeVariance(16))
+        e = np.array(e)
+        return dict(points=n, rms_rel_err=float(np.sqrt((e ** 2).mean())), mean_rel_err=float(e.mean()),
+                    rel_var_per_point=float(np.mean(v)))
+
+    uni = [errors(n, None, 1000 + 10 * i) for i, n in enumerate((100_000, 1_000_000))]
+    t0 = time.perf_counter()
+    q = G.DalitzProposal(model.observables[0], model.observables[1], seed=7)
+    l0, l1 = q.train(train_events, steps=2000, batch=16384, lr=3e-3, seed=3)
+    q.build(512, 0.1)
+    sync()
+    t_train = time.perf_counter() - t0
+    lrn = [errors(n, q, 2000 + 10 * i) for i, n in enumerate((100_000, 1_000_000))]
+    q.setReferenceVariance(lrn[-1]["rel_var_per_point"])
+    out.update(uniform=uni, learned=lrn, train=dict(events=len(train_events), steps=2000, batch=16384,
+                                                    seconds=round(t_train, 3), loss_first=l0, loss_last=l1))
+    log(f"norm error at 1e6 points: uniform {uni[1]['rms_rel_err']:.2e}, learned {lrn[1]['rms_rel_err']:.2e} "
+        f"(training {t_train:.2f} s)")
+
+    # fits at equal target error
+    fits = {}
+    names = sorted(model.params)
+    for label, var, prop in (("uniform", uni[1]["rel_var_per_point"], None),
+                             ("learned", lrn[1]["rel_var_per_point"], q)):
+        n = int(min(max(math.ceil(var / target ** 2), 100_000), 1_000_000_000))
+        rng = np.random.default_rng(11)
+        for k in names:
+            model.params[k].value = truth[k] * (1.0 + 0.05 * rng.standard_normal())
+        fm = G.FitManager(pdf)
+        fm.setVerbosity(0)
+        sync()
+        t0 = time.perf_counter()
+        pdf.setNormalisationPoints(n, 4242, prop)
+        m = fm.fit()
+        sync()
+        wall = time.perf_counter() - t0
+        fits[label] = dict(points=n, expected_rel_err=math.sqrt(var / n), wall_s=round(wall, 3), nfcn=m.NFcn(),
+                           valid=bool(m.IsValid()), values=list(m.values()), errors=list(m.errors()))
+        log(f"{label} fit at {n:.3g} points: {wall:.2f} s, {m.NFcn()} calls")
+    vu, vl = np.array(fits["uniform"]["values"]), np.array(fits["learned"]["values"])
+    eu = np.array(fits["uniform"]["errors"])
+    ok = eu > 0
+    fits["max_param_shift_sigma"] = float(np.max(np.abs(vu - vl)[ok] / eu[ok])) if ok.any() else None
+    fits["learned_incl_training_s"] = round(fits["learned"]["wall_s"] + t_train, 3)
+    for k in names:
+        model.params[k].value = truth[k]
+    pdf.copyParams()
+    pdf.setNormalisationGrid()
+    out["fits"] = fits
+    return out
+
+
 def cpu_baseline_dalitz(flat, decay, m12, m13, target_s=10.0):
     """Oracle (C/OpenMP restatement of the reference algorithm) NLL over the same events."""
     import oracle as O
@@ -119,6 +194,7 @@ def main():
     ap.add_argument("--no-fit", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the normalisation-error study")
     ap.add_argument("--profile-only", action="store_true",
                     help="set up the Dalitz workload, run 20 NLL reductions and a 256 MiB device copy "
                          "(HBM counter calibration), print nothing else; for rocprofv3 runs")
@@ -220,6 +296,18 @@ def main():
             model.params[n].value = truth[n]
         pdf.copyParams()
 
+    norm_study = None
+    if not args.no_learned:
+        log("normalisation-integral error study: uniform MC vs learned proposal")
+        t12, t13 = W.generate_dalitz(model, 2_000_000, seed=8)
+        train_events = W.dalitz_dataset(model, t12, t13)
+        pdf.setData(data)
+        pdf.setDataSize(len(m12))
+        pdf.setForceIntegrals()
+        norm_study = norm_error_study(model, pdf, train_events, sync)
+        pdf.setForceIntegrals()
+        pdf.calculateNLL()
+
     workloads = {}
     if not args.no_extra:
         for name, builder, sampler, nevt, bpe in (
@@ -254,7 +342,8 @@ def main():
                 vs_baseline=None, dtype="f64", data="synthetic (GPU accept-reject on the model)",
                 config=dict(workload="dalitz16_d0_pipipi0", events=args.events, norm_grid=f"{args.grid}x{args.grid}",
                             resonances=16, free_params=30, step="calculateNLL with new couplings"),
-                roofline=roofline, cpu_baseline=cpu, fit=fit, dalitz=extra, workloads=workloads)
+                roofline=roofline, cpu_baseline=cpu, fit=fit, dalitz=extra, norm_error=norm_study,
+                workloads=workloads)
     print(json.dumps(line), flush=True)
 
 

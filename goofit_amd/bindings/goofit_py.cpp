@@ -19,6 +19,8 @@
 #include <goofit/PDFs/combine/AddPdf.h>
 #include <goofit/PDFs/combine/ProdPdf.h>
 #include <goofit/PDFs/physics/Amp3Body.h>
+#include <goofit/PDFs/physics/DalitzProposal.h>
+#include <goofit/fitting/BatchFitter.h>
 #include <goofit/PDFs/physics/ResonancePdf.h>
 #include <goofit/UnbinnedDataSet.h>
 #include <goofit/Variable.h>
@@ -264,7 +266,35 @@ PYBIND11_MODULE(_goofit, m) {
         .def("fit_fractions", &Amp3Body::fit_fractions)
         .def("getIntegrals", &Amp3Body::getIntegrals)
         .def("getEfficiencyIndex", &Amp3Body::getEfficiencyIndex)
+        .def("setNormalisationPoints", &Amp3Body::setNormalisationPoints, py::arg("n"), py::arg("seed"),
+             py::arg("proposal") = nullptr, py::keep_alive<1, 4>())
+        .def("setNormalisationGrid", &Amp3Body::setNormalisationGrid)
+        .def("getNormalisationPointCount", &Amp3Body::getNormalisationPointCount)
+        .def("normalisationRelativeVariance", &Amp3Body::normalisationRelativeVariance, py::arg("batches") = 16)
+        .def("generateEvents", &Amp3Body::generateEvents, py::arg("out"), py::arg("n"), py::arg("seed"),
+             py::arg("fmax") = 0.0)
         .def_static("resetCacheCounter", &Amp3Body::resetCacheCounter);
+
+    py::class_<DalitzProposal>(m, "DalitzProposal")
+        .def(py::init<Observable, Observable, unsigned long long, int>(), py::arg("m12"), py::arg("m13"),
+             py::arg("seed") = 1, py::arg("batch_cap") = 65536)
+        .def("train", &DalitzProposal::train, py::arg("events"), py::arg("steps") = 2000, py::arg("batch") = 8192,
+             py::arg("lr") = 3e-3f, py::arg("seed") = 1)
+        .def("build", &DalitzProposal::build, py::arg("grid") = 512, py::arg("alpha") = 0.1)
+        .def("isBuilt", &DalitzProposal::isBuilt)
+        .def("evaluate",
+             [](const DalitzProposal &q, py::array_t<float, py::array::c_style | py::array::forcecast> xy) {
+                 std::vector<float> v(xy.data(), xy.data() + xy.size());
+                 auto out = q.evaluate(v);
+                 py::array_t<float> a(out.size());
+                 std::copy(out.begin(), out.end(), a.mutable_data());
+                 return a;
+             })
+        .def("getParameters", &DalitzProposal::getParameters)
+        .def("setParameters", &DalitzProposal::setParameters)
+        .def("setReferenceVariance", &DalitzProposal::setReferenceVariance)
+        .def("getReferenceVariance", &DalitzProposal::getReferenceVariance)
+        .def("needsRetraining", &DalitzProposal::needsRetraining, py::arg("relvar"), py::arg("factor") = 4.0);
 
     m.def("inDalitz", &inDalitz);
 
@@ -304,6 +334,30 @@ PYBIND11_MODULE(_goofit, m) {
         .def(py::init<Params &>(), py::arg("params"), py::keep_alive<1, 2>())
         .def("__call__", [](const FCN &f, const std::vector<double> &x) { return f(x); })
         .def("Up", &FCN::Up);
+
+    py::class_<BatchFitter>(m, "BatchFitter")
+        .def(py::init<GooPdf *, const std::vector<int64_t> &, const std::vector<int64_t> &>(), py::arg("pdf"),
+             py::arg("first"), py::arg("count"), py::keep_alive<1, 2>())
+        .def("fit", &BatchFitter::fit, py::arg("threads") = 256, py::arg("strategy") = 1, py::arg("tolerance") = 0.1,
+             py::arg("autotune") = true, py::call_guard<py::gil_scoped_release>())
+        .def("evaluate", &BatchFitter::evaluate)
+        .def("useGraph", &BatchFitter::useGraph)
+        .def("__len__", &BatchFitter::size)
+        .def("values", &BatchFitter::values)
+        .def("errors", &BatchFitter::errors)
+        .def("valid", &BatchFitter::valid)
+        .def("nfcn", &BatchFitter::nfcn)
+        .def("fval", &BatchFitter::fval)
+        .def("stats", [](const BatchFitter &b) {
+            const BatchFitStats &s = b.stats();
+            py::dict d;
+            d["launches"]   = s.launches;
+            d["fcn_calls"]  = s.fcn_calls;
+            d["seconds"]    = s.seconds;
+            d["gpu_wait_s"] = s.gpu_wait_s;
+            d["chunk"]      = s.chunk;
+            return d;
+        });
 
     py::class_<FitManagerMinuit2>(m, "FitManager")
         .def(py::init<PdfBase *>(), py::arg("pdf"), py::keep_alive<1, 2>())

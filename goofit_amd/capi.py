@@ -35,6 +35,8 @@ EXPORTS = [
     "gf_sample_uniform", "gf_mlp_param_count", "gf_mlp_create", "gf_mlp_destroy", "gf_mlp_get_params",
     "gf_mlp_set_params", "gf_mlp_forward", "gf_mlp_gradient", "gf_mlp_train", "gf_mlp_build_proposal",
     "gf_mlp_sample", "gf_mlp_train_host", "gf_dalitz_set_norm_points", "gf_dalitz_gram_norm_points",
+    "gf_dalitz_generate", "gf_batch_create", "gf_batch_destroy", "gf_batch_set_toy", "gf_batch_reduce",
+    "gf_batch_set_chunk", "gf_batch_get_chunk", "gf_batch_use_graph", "gf_batch_autotune",
 ]
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -106,6 +108,18 @@ def lib():
                                         ctypes.c_uint64, _F, _F]
         L.gf_dalitz_set_norm_points.argtypes = [c_p, ctypes.c_int, c_p, ctypes.c_int64, ctypes.c_uint64, _D]
         L.gf_dalitz_gram_norm_points.argtypes = [c_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _D]
+        _L = ctypes.POINTER(ctypes.c_int64)
+        L.gf_batch_create.argtypes = [c_p, ctypes.c_int, _L, _L, ctypes.POINTER(c_p)]
+        L.gf_batch_destroy.argtypes = [c_p]
+        L.gf_batch_destroy.restype = None
+        L.gf_batch_set_toy.argtypes = [c_p, ctypes.c_int, _D, _D]
+        L.gf_batch_reduce.argtypes = [c_p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int, _D]
+        L.gf_batch_set_chunk.argtypes = [c_p, ctypes.c_int]
+        L.gf_batch_get_chunk.argtypes = [c_p, _I]
+        L.gf_batch_use_graph.argtypes = [c_p, ctypes.c_int]
+        L.gf_batch_autotune.argtypes = [c_p, ctypes.c_int, _I, _D]
+        L.gf_dalitz_generate.argtypes = [c_p, ctypes.c_int, ctypes.c_int64, ctypes.c_uint64, _D, ctypes.c_double,
+                                         ctypes.c_int, c_p, c_p, ctypes.POINTER(ctypes.c_int64), _D]
         L.gf_time_dalitz_cache.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D]
         L.gf_time_dalitz_gram.argtypes = [c_p, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int, _D]
         _lib = L
@@ -218,6 +232,15 @@ class Context:
         out = np.zeros(2 * nres * nres)
         _check(lib().gf_dalitz_gram_norm_points(self.h, dalitz_func, eff_func, offset, count, _dp(out)))
         return out.view(np.complex128).reshape(nres, nres)
+
+    def dalitz_generate(self, dalitz_func, n, seed, box, fmax=0.0):
+        """n events (host arrays m12, m13), the number of candidates tried and the envelope."""
+        b = np.ascontiguousarray(box, np.float64)
+        a12, a13 = np.empty(n), np.empty(n)
+        tried, fm = ctypes.c_int64(), ctypes.c_double()
+        _check(lib().gf_dalitz_generate(self.h, dalitz_func, n, seed, _dp(b), fmax, 1, a12.ctypes.data,
+                                        a13.ctypes.data, ctypes.byref(tried), ctypes.byref(fm)))
+        return a12, a13, tried.value, fm.value
 
     # -- reductions -------------------------------------------------------
     def reduce_metric(self, metric="NLL"):
@@ -378,3 +401,50 @@ class Proposal:
 def sample_uniform(ctx: Context, box, n, seed, dev_m12: int, dev_m13: int, dev_w: int):
     b = np.ascontiguousarray(box, np.float64)
     _check(lib().gf_sample_uniform(ctx.h, _dp(b), n, seed, dev_m12, dev_m13, dev_w))
+
+
+class Batch:
+    """A gf_batch: toys over row ranges of a Context's events, reduced in one launch."""
+
+    def __init__(self, ctx: Context, first, count):
+        f = np.ascontiguousarray(first, np.int64)
+        c = np.ascontiguousarray(count, np.int64)
+        h = ctypes.c_void_p()
+        L64 = ctypes.POINTER(ctypes.c_int64)
+        _check(lib().gf_batch_create(ctx.h, f.size, f.ctypes.data_as(L64), c.ctypes.data_as(L64), ctypes.byref(h)))
+        self.h, self.n = h, f.size
+
+    def close(self):
+        if self.h:
+            lib().gf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_toy(self, t, params=None, norms=None):
+        p = _f64(params) if params is not None else None
+        n = _f64(norms) if norms is not None else None
+        _check(lib().gf_batch_set_toy(self.h, t, _dp(p) if p is not None else None, _dp(n) if n is not None else None))
+
+    def reduce(self, active=None, metric="NLL"):
+        a = np.ones(self.n, np.uint8) if active is None else np.ascontiguousarray(active, np.uint8)
+        out = np.full(self.n, np.nan)
+        m = METRIC[metric] if isinstance(metric, str) else metric
+        _check(lib().gf_batch_reduce(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), m, _dp(out)))
+        return out
+
+    def set_chunk(self, chunk):
+        _check(lib().gf_batch_set_chunk(self.h, chunk))
+
+    def use_graph(self, on):
+        _check(lib().gf_batch_use_graph(self.h, 1 if on else 0))
+
+    def autotune(self, metric="NLL"):
+        c, us = ctypes.c_int32(), ctypes.c_double()
+        m = METRIC[metric] if isinstance(metric, str) else metric
+        _check(lib().gf_batch_autotune(self.h, m, ctypes.byref(c), ctypes.byref(us)))
+        return c.value, us.value

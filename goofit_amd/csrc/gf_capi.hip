@@ -14,6 +14,7 @@
 
 #include "../../include/goofit_hip.h"
 #include "gf_internal.h"
+#include "gf_device.h"
 #include "gf_mlp.h"
 
 #include <random>
@@ -56,6 +57,7 @@ struct DalitzNode {
     int eff_func = -1;    // efficiencyFunction (last function of the subtree, Amp3Body.cu:198-202)
     int eff_root = -1;    // first function of the efficiency subtree
     int col_m12 = 0, col_m13 = 1, col_num = 2;
+    int p_coup = 0;       // params index of (re, im) of the first coupling
 };
 
 }  // namespace
@@ -488,6 +490,9 @@ gf_status scan_dalitz(gf_ctx *ctx) {
         d.col_m12 = c12;
         d.col_m13 = c13;
         d.col_num = cnum;
+        d.p_coup  = q.p + 1;
+        if(C.np(q) < 2 * d.nres)
+            return fail(GF_ERR_INVALID, "Amp3Body at %d: %d parameters for %d couplings", f, C.np(q), d.nres);
         if(!C.inc(q))
             return fail(GF_ERR_INVALID, "%s", C.err.c_str());
         for(int i = 0; i < d.nres; i++) {
@@ -2080,6 +2085,433 @@ gf_status gf_dalitz_gram_norm_points(gf_ctx *ctx, int dalitz_func, int eff_func,
                     (long long)(offset + count), (long long)np.n);
     return gf_dalitz_gram_points_device(ctx, dalitz_func, np.buf + offset, np.buf + np.n + offset,
                                         np.buf + 2 * np.n + offset, count, eff_func, out_re_im);
+}
+
+gf_status gf_dalitz_generate(gf_ctx *ctx, int dalitz_func, int64_t n, uint64_t seed, const double box[4],
+                             double fmax, int host_out, double *m12, double *m13, int64_t *tried,
+                             double *fmax_used) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    int slot      = 0;
+    DalitzNode *d = find_dalitz(ctx, dalitz_func, &slot);
+    if(!d)
+        return fail(GF_ERR_INVALID, "function %d is not an Amp3Body node", dalitz_func);
+    if(n < 1 || !m12 || !m13 || !box)
+        return fail(GF_ERR_INVALID, "null output or n < 1");
+    if(!(box[1] > box[0]) || !(box[3] > box[2]))
+        return fail(GF_ERR_INVALID, "empty box");
+    int eff_pc   = -1;
+    gf_status st = subprogram(ctx, d->eff_func, eff_pc);
+    if(st)
+        return st;
+    if((st = flush_uploads(ctx)))
+        return st;
+    const int64_t B  = 1 << 22;  // candidates per pass
+    const int64_t nb = B / 256;
+    double *c = nullptr, *o = nullptr, *bmax = nullptr;
+    uint8_t *flag = nullptr;
+    int *bcount = nullptr;
+    int64_t *boff = nullptr;
+    auto release = [&]() {
+        (void)hipStreamSynchronize(ctx->stream);
+        void *p[] = {c, o, bmax, flag, bcount, boff};
+        for(void *q : p)
+            if(q)
+                (void)hipFree(q);
+    };
+    if(hipMalloc((void **)&c, (size_t)B * 2 * 8) != hipSuccess || hipMalloc((void **)&flag, (size_t)B) != hipSuccess
+       || hipMalloc((void **)&bcount, (size_t)nb * 4) != hipSuccess
+       || hipMalloc((void **)&bmax, (size_t)nb * 8) != hipSuccess
+       || hipMalloc((void **)&boff, (size_t)nb * 8) != hipSuccess
+       || (host_out && hipMalloc((void **)&o, (size_t)n * 2 * 8) != hipSuccess)) {
+        release();
+        return fail(GF_ERR_NOMEM, "cannot allocate the generator buffers");
+    }
+    double *o12 = host_out ? o : m12, *o13 = host_out ? o + n : m13;
+    GfProgArgs a = prog_args(ctx);
+    std::vector<int> hc(nb);
+    std::vector<double> hm(nb);
+    std::vector<int64_t> off(nb);
+    const uint64_t key = gfd::mix64(seed ^ 0x6A09E667F3BCC909ull);
+    int64_t have = 0, base = 0;
+    auto pass = [&](double env) -> gf_status {
+        if(gfk::launch_dalitz_candidates(a, ctx->d_res + slot * GF_MAX_RES, d->nres, d->p_coup, eff_pc, d->col_m12,
+                                         d->col_m13, box, key, base, B, env, c, c + B, flag, bcount, bmax,
+                                         ctx->stream))
+            return fail(GF_ERR_HIP, "candidate launch failed");
+        GF_HIP(hipMemcpyAsync(hc.data(), bcount, (size_t)nb * 4, hipMemcpyDeviceToHost, ctx->stream));
+        GF_HIP(hipMemcpyAsync(hm.data(), bmax, (size_t)nb * 8, hipMemcpyDeviceToHost, ctx->stream));
+        GF_HIP(hipStreamSynchronize(ctx->stream));
+        return GF_OK;
+    };
+    if(!(fmax > 0.0)) {
+        // envelope: 1.25 x the largest value over one pass of candidates
+        st = pass(INFINITY);
+        if(st) {
+            release();
+            return st;
+        }
+        double mx = 0.0;
+        for(double v : hm)
+            mx = std::max(mx, v);
+        if(!(mx > 0.0)) {
+            release();
+            return fail(GF_ERR_INVALID, "|A|^2 * eff is zero on %lld candidates", (long long)B);
+        }
+        fmax = 1.25 * mx;
+        base += B;
+    }
+    while(have < n) {
+        if((st = pass(fmax))) {
+            release();
+            return st;
+        }
+        double mx    = 0.0;
+        int64_t acc  = 0;
+        for(int64_t b = 0; b < nb; b++) {
+            off[b] = have + acc;
+            acc += hc[b];
+            mx = std::max(mx, hm[b]);
+        }
+        if(mx > fmax) {
+            release();
+            return fail(GF_ERR_INVALID, "envelope %g below the value %g found: raise fmax", fmax, mx);
+        }
+        if(acc > 0) {
+            if(hipMemcpyAsync(boff, off.data(), (size_t)nb * 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess
+               || gfk::launch_compact(c, c + B, flag, B, boff, n, o12, o13, ctx->stream)) {
+                release();
+                return fail(GF_ERR_HIP, "compaction failed");
+            }
+        }
+        have += acc;
+        base += B;
+        if(base > (int64_t)1 << 40) {
+            release();
+            return fail(GF_ERR_INVALID, "acceptance too low: %lld of %lld events after %lld candidates",
+                        (long long)have, (long long)n, (long long)base);
+        }
+    }
+    if(host_out) {
+        if(hipMemcpyAsync(m12, o12, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess
+           || hipMemcpyAsync(m13, o13, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) {
+            release();
+            return fail(GF_ERR_HIP, "copy of the events failed");
+        }
+    }
+    release();
+    if(tried)
+        *tried = base;
+    if(fmax_used)
+        *fmax_used = fmax;
+    return GF_OK;
+}
+
+// ------------------------------------------------------------ toy batches
+
+}  // extern "C"
+
+struct gf_batch {
+    gf_ctx *ctx = nullptr;
+    int ntoys = 0, np = 0, nn = 0;
+    std::vector<int64_t> first, count;
+    // pinned staging: params [T][np] | norms [T][nn] | active [T] (bytes, padded to 8)
+    double *h_stage = nullptr, *d_stage = nullptr;
+    size_t stage_bytes = 0;
+    // work table and partials for the current chunk
+    int chunk = 0, nblocks = 0;
+    GfBatchWork *d_work = nullptr;
+    int *d_first = nullptr;
+    gfk::Partial *d_partials = nullptr;
+    // results: mapped [2T] doubles + stamp
+    double *h_res = nullptr, *d_res = nullptr;
+    uint32_t *h_stamp = nullptr, *d_stamp = nullptr, *d_counter = nullptr;
+    uint32_t expect = 0;
+    // graph of (stage copy, sweep, per-toy finish, stamp) for one metric
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_metric = -1, graph_chunk = 0;
+    bool use_graph = true;
+};
+
+namespace {
+
+void batch_drop_graph(gf_batch *b) {
+    if(b->exec)
+        (void)hipGraphExecDestroy(b->exec);
+    if(b->graph)
+        (void)hipGraphDestroy(b->graph);
+    b->exec  = nullptr;
+    b->graph = nullptr;
+    b->graph_metric = -1;
+}
+
+double *batch_params(gf_batch *b) { return b->h_stage; }
+double *batch_norms(gf_batch *b) { return b->h_stage + (size_t)b->ntoys * b->np; }
+uint8_t *batch_active(gf_batch *b) { return (uint8_t *)(b->h_stage + (size_t)b->ntoys * (b->np + b->nn)); }
+
+// Work table for `chunk` events per workgroup: each toy's rows split into
+// ceil(count / chunk) pieces, toys in order.
+gf_status batch_layout(gf_batch *b, int chunk) {
+    std::vector<GfBatchWork> w;
+    std::vector<int> first(b->ntoys + 1);
+    for(int t = 0; t < b->ntoys; t++) {
+        first[t] = (int)w.size();
+        const int64_t n = b->count[t];
+        for(int64_t s = 0; s < n || (s == 0 && n == 0); s += chunk) {
+            GfBatchWork e;
+            e.toy   = t;
+            e.start = b->first[t] + s;
+            e.end   = b->first[t] + std::min<int64_t>(n, s + chunk);
+            e.total = (double)n;
+            w.push_back(e);
+            if(n == 0)
+                break;
+        }
+    }
+    first[b->ntoys] = (int)w.size();
+    if(w.size() > (size_t)INT32_MAX / 2)
+        return fail(GF_ERR_INVALID, "too many workgroups");
+    batch_drop_graph(b);
+    if(b->d_work)
+        GF_HIP(hipFree(b->d_work));
+    if(b->d_partials)
+        GF_HIP(hipFree(b->d_partials));
+    b->d_work     = nullptr;
+    b->d_partials = nullptr;
+    GF_HIP(hipMalloc((void **)&b->d_work, w.size() * sizeof(GfBatchWork)));
+    GF_HIP(hipMalloc((void **)&b->d_partials, w.size() * sizeof(gfk::Partial)));
+    GF_HIP(hipMemcpy(b->d_work, w.data(), w.size() * sizeof(GfBatchWork), hipMemcpyHostToDevice));
+    GF_HIP(hipMemcpy(b->d_first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice));
+    b->chunk   = chunk;
+    b->nblocks = (int)w.size();
+    return GF_OK;
+}
+
+// The launch sequence of one batched reduction on the context stream.
+gf_status batch_enqueue(gf_batch *b, int metric) {
+    gf_ctx *ctx  = b->ctx;
+    hipStream_t s = ctx->stream;
+    GF_HIP(hipMemcpyAsync(b->d_stage, b->h_stage, b->stage_bytes, hipMemcpyHostToDevice, s));
+    GfProgArgs a        = prog_args(ctx);
+    const double *dpar  = b->d_stage;
+    const double *dnorm = b->d_stage + (size_t)b->ntoys * b->np;
+    const uint8_t *dact = (const uint8_t *)(b->d_stage + (size_t)b->ntoys * (b->np + b->nn));
+    if(gfk::launch_batch(a, b->d_work, b->nblocks, dpar, b->np, dnorm, b->nn, dact, metric, b->d_partials, b->d_first,
+                         b->ntoys, b->d_res, b->d_counter, b->d_stamp, s))
+        return fail(GF_ERR_HIP, "batch launch failed");
+    return GF_OK;
+}
+
+gf_status batch_wait(gf_batch *b) {
+    const volatile uint32_t *stamp = b->h_stamp;
+    const auto t0 = std::chrono::steady_clock::now();
+    for(unsigned it = 0;; ++it) {
+        if(*stamp == b->expect)
+            return GF_OK;
+        if((it & 255u) == 255u
+           && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(b->ctx->tune[GF_TUNE_SPIN_US]))
+            break;
+        __builtin_ia32_pause();
+    }
+    GF_HIP(hipStreamSynchronize(b->ctx->stream));
+    if(*stamp != b->expect)
+        return fail(GF_ERR_HIP, "batched reduction finished without its stamp");
+    return GF_OK;
+}
+
+gf_status batch_run(gf_batch *b, int metric) {
+    if(!b->use_graph) {
+        gf_status st = batch_enqueue(b, metric);
+        if(st)
+            return st;
+    } else {
+        if(!b->exec || b->graph_metric != metric || b->graph_chunk != b->chunk) {
+            batch_drop_graph(b);
+            GF_HIP(hipStreamBeginCapture(b->ctx->stream, hipStreamCaptureModeThreadLocal));
+            gf_status st = batch_enqueue(b, metric);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(b->ctx->stream, &g);
+            if(st) {
+                if(g)
+                    (void)hipGraphDestroy(g);
+                return st;
+            }
+            if(e != hipSuccess)
+                return fail(GF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+            b->graph = g;
+            GF_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+            b->graph_metric = metric;
+            b->graph_chunk  = b->chunk;
+        }
+        GF_HIP(hipGraphLaunch(b->exec, b->ctx->stream));
+    }
+    b->expect++;
+    return batch_wait(b);
+}
+
+}  // namespace
+
+extern "C" {
+
+gf_status gf_batch_create(gf_ctx *ctx, int ntoys, const int64_t *first, const int64_t *count, gf_batch **out) {
+    if(!out)
+        return fail(GF_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(ntoys < 1 || !first || !count)
+        return fail(GF_ERR_INVALID, "need at least one toy");
+    if(ctx->ins.empty())
+        return fail(GF_ERR_STATE, "no program installed (gf_program_set)");
+    for(int t = 0; t < ntoys; t++)
+        if(first[t] < 0 || count[t] < 0 || first[t] + count[t] > ctx->n)
+            return fail(GF_ERR_INVALID, "toy %d rows [%lld, %lld) outside the %lld events", t, (long long)first[t],
+                        (long long)(first[t] + count[t]), (long long)ctx->n);
+    gf_batch *b = new gf_batch();
+    b->ctx      = ctx;
+    b->ntoys    = ntoys;
+    b->np       = (int)ctx->params.size();
+    b->nn       = (int)ctx->norms.size();
+    b->first.assign(first, first + ntoys);
+    b->count.assign(count, count + ntoys);
+    const size_t nd = (size_t)ntoys * (b->np + b->nn) + ((size_t)ntoys + 7) / 8;
+    b->stage_bytes  = nd * sizeof(double);
+    gf_status st    = GF_OK;
+    if(hipHostMalloc((void **)&b->h_stage, b->stage_bytes, hipHostMallocDefault) != hipSuccess
+       || hipMalloc((void **)&b->d_stage, b->stage_bytes) != hipSuccess
+       || hipMalloc((void **)&b->d_first, (size_t)(ntoys + 1) * sizeof(int)) != hipSuccess
+       || hipHostMalloc((void **)&b->h_res, (size_t)2 * ntoys * sizeof(double),
+                        hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess
+       || hipHostGetDevicePointer((void **)&b->d_res, b->h_res, 0) != hipSuccess
+       || hipHostMalloc((void **)&b->h_stamp, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess
+       || hipHostGetDevicePointer((void **)&b->d_stamp, b->h_stamp, 0) != hipSuccess
+       || hipMalloc((void **)&b->d_counter, 64) != hipSuccess || hipMemset(b->d_counter, 0, 64) != hipSuccess)
+        st = fail(GF_ERR_NOMEM, "cannot allocate the batch buffers");
+    if(!st) {
+        *b->h_stamp = 0;
+        memset(b->h_stage, 0, b->stage_bytes);
+        for(int t = 0; t < ntoys; t++) {  // start from the context's arrays
+            memcpy(batch_params(b) + (size_t)t * b->np, ctx->params.data(), b->np * sizeof(double));
+            memcpy(batch_norms(b) + (size_t)t * b->nn, ctx->norms.data(), b->nn * sizeof(double));
+        }
+        st = batch_layout(b, 8192);
+    }
+    if(st) {
+        gf_batch_destroy(b);
+        return st;
+    }
+    *out = b;
+    return GF_OK;
+}
+
+void gf_batch_destroy(gf_batch *b) {
+    if(!b)
+        return;
+    if(b->ctx) {
+        (void)hipSetDevice(b->ctx->device);
+        (void)hipStreamSynchronize(b->ctx->stream);
+    }
+    batch_drop_graph(b);
+    void *dev[] = {b->d_stage, b->d_first, b->d_work, b->d_partials, b->d_counter};
+    for(void *p : dev)
+        if(p)
+            (void)hipFree(p);
+    void *host[] = {b->h_stage, b->h_res, b->h_stamp};
+    for(void *p : host)
+        if(p)
+            (void)hipHostFree(p);
+    delete b;
+}
+
+gf_status gf_batch_set_toy(gf_batch *b, int toy, const double *params, const double *norms) {
+    if(!b || toy < 0 || toy >= b->ntoys)
+        return fail(GF_ERR_INVALID, "toy index out of range");
+    if(params)
+        memcpy(batch_params(b) + (size_t)toy * b->np, params, b->np * sizeof(double));
+    if(norms)
+        memcpy(batch_norms(b) + (size_t)toy * b->nn, norms, b->nn * sizeof(double));
+    return GF_OK;
+}
+
+gf_status gf_batch_reduce(gf_batch *b, const uint8_t *active, int metric, double *out) {
+    if(!b || !active || !out)
+        return fail(GF_ERR_INVALID, "null argument");
+    if(!valid_ctx(b->ctx))
+        return GF_ERR_INVALID;
+    gf_status st = check_events_for_program(b->ctx, metric);
+    if(st)
+        return st;
+    if(b->np != (int)b->ctx->params.size() || b->nn != (int)b->ctx->norms.size())
+        return fail(GF_ERR_STATE, "the context's program changed since the batch was created");
+    if((st = flush_uploads(b->ctx)))  // device copies used by the wave cache and constants
+        return st;
+    memcpy(batch_active(b), active, b->ntoys);
+    if((st = batch_run(b, metric)))
+        return st;
+    for(int t = 0; t < b->ntoys; t++)
+        if(active[t])
+            out[t] = ((const volatile double *)b->h_res)[2 * t];
+    return GF_OK;
+}
+
+gf_status gf_batch_set_chunk(gf_batch *b, int chunk) {
+    if(!b || chunk < 256 || chunk % 256)
+        return fail(GF_ERR_INVALID, "chunk must be a positive multiple of 256");
+    if(!valid_ctx(b->ctx))
+        return GF_ERR_INVALID;
+    GF_HIP(hipStreamSynchronize(b->ctx->stream));
+    return batch_layout(b, chunk);
+}
+
+gf_status gf_batch_get_chunk(gf_batch *b, int *chunk) {
+    if(!b || !chunk)
+        return fail(GF_ERR_INVALID, "null argument");
+    *chunk = b->chunk;
+    return GF_OK;
+}
+
+gf_status gf_batch_use_graph(gf_batch *b, int on) {
+    if(!b)
+        return fail(GF_ERR_INVALID, "null batch");
+    b->use_graph = on != 0;
+    return GF_OK;
+}
+
+gf_status gf_batch_autotune(gf_batch *b, int metric, int *chunk_out, double *us_out) {
+    if(!b)
+        return fail(GF_ERR_INVALID, "null batch");
+    if(!valid_ctx(b->ctx))
+        return GF_ERR_INVALID;
+    std::vector<uint8_t> all(b->ntoys, 1);
+    std::vector<double> tmp(b->ntoys);
+    const int cand[] = {1024, 2048, 4096, 8192, 16384, 32768};
+    int best = 0;
+    double best_us = INFINITY;
+    for(int c : cand) {
+        gf_status st = gf_batch_set_chunk(b, c);
+        if(st)
+            return st;
+        if((st = gf_batch_reduce(b, all.data(), metric, tmp.data())))  // warm-up (and graph build)
+            return st;
+        const auto t0 = std::chrono::steady_clock::now();
+        for(int r = 0; r < 5; r++)
+            if((st = gf_batch_reduce(b, all.data(), metric, tmp.data())))
+                return st;
+        const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() / 5;
+        if(us < best_us) {
+            best_us = us;
+            best    = c;
+        }
+    }
+    gf_status st = gf_batch_set_chunk(b, best);
+    if(st)
+        return st;
+    if(chunk_out)
+        *chunk_out = best;
+    if(us_out)
+        *us_out = best_us;
+    return GF_OK;
 }
 
 }  // extern "C"

@@ -321,4 +321,17 @@ __device__ __forceinline__ double2 lineshape(const GfResDesc &r, const double *_
     }
 }
 
+// Counter-based random numbers: splitmix64 finaliser of (key, counter).  Any
+// (key, counter) pair gives the same value on every run and device.
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// uniform in [0, 1) with 53 random bits
+__host__ __device__ __forceinline__ double unif01(uint64_t key, uint64_t ctr) {
+    return (double)(mix64(key ^ mix64(ctr)) >> 11) * (1.0 / 9007199254740992.0);
+}
+
 }  // namespace gfd

@@ -127,6 +127,15 @@ struct GfResult {
     int32_t pad[2];
 };
 
+// One workgroup's share of a toy batch: rows [start, end) of toy `toy`
+// (total = the toy's event count, for the binned metrics).
+struct GfBatchWork {
+    int32_t toy;
+    int32_t pad;
+    int64_t start, end;
+    double total;
+};
+
 // Host-side launchers (gf_kernels.hip).
 namespace gfk {
 struct Partial {
@@ -144,6 +153,18 @@ int launch_dalitz_fast(const GfProgArgs &a, const GfInline *inl, bool const_eff,
                        GfResult *result, double *out, int nblocks, void *stream);
 int launch_reduce_bins(const GfProgArgs &a, int pc0, const double *ranges, const int *slots, int k, int64_t total,
                        Partial *partials, GfResult *result, int nblocks, void *stream);
+// Toy batch: per-workgroup sweep with per-toy parameters, per-toy finishing
+// reduction into result[2t..2t+1] (host-mapped), then the completion stamp.
+int launch_batch(const GfProgArgs &a, const GfBatchWork *work, int nblocks, const double *params, int np,
+                 const double *norms, int nn, const uint8_t *active, int metric, Partial *partials, const int *first,
+                 int ntoys, double *result, uint32_t *counter, uint32_t *stamp, void *stream);
+// Accept-reject generation: candidates (uniform in box) with accept flags, and
+// the ordered compaction of the accepted ones.
+int launch_dalitz_candidates(const GfProgArgs &a, const GfResDesc *res, int nres, int p_coup, int eff_pc, int id12,
+                             int id13, const double box[4], uint64_t key, int64_t base, int64_t nc, double fmax,
+                             double *c12, double *c13, uint8_t *flag, int *bcount, double *bmax, void *stream);
+int launch_compact(const double *c12, const double *c13, const uint8_t *flag, int64_t nc, const int64_t *boff,
+                   int64_t limit, double *o12, double *o13, void *stream);
 int launch_dalitz_cache(const GfProgArgs &a, int slot, int64_t offset, int64_t n, uint32_t mask,
                         const GfResDesc *res, int nres, int nblocks, void *stream);
 // mode 0: grid cells (grid6 host array); 1: points (m12, m13, w device arrays); 2: data cache of `slot`.

@@ -641,6 +641,190 @@ __global__ __launch_bounds__(256) void k_dalitz_cache(GfProgArgs A, int slot, in
     }
 }
 
+// ------------------------------------------------------------- toy batches
+
+// One entry of a batch's work table: rows [start, end) of toy `toy`.
+// (GfBatchWork is declared in gf_internal.h)
+
+// Metric sums of many toys in one launch: workgroup b evaluates the rows of
+// work[b] with its toy's parameter and normalisation arrays (P = params +
+// toy * np, Nm = norms + toy * nn) and stores one partial; workgroups of
+// inactive toys store zero.
+template <int EPT>
+__global__ __launch_bounds__(GF_BLOCK) void k_batch_metric(GfProgArgs A, const GfBatchWork *__restrict__ work,
+                                                           const double *__restrict__ params, int np,
+                                                           const double *__restrict__ norms, int nn,
+                                                           const uint8_t *__restrict__ active, int metric,
+                                                           Partial *partials) {
+    const GfBatchWork w = work[blockIdx.x];
+    double s = 0.0, c = 0.0;
+    if(active[w.toy]) {
+        const double *__restrict__ P  = params + (int64_t)w.toy * np;
+        const double *__restrict__ Nm = norms + (int64_t)w.toy * nn;
+        const double norm             = Nm[1];
+        const bool binned             = metric >= GF_METRIC_BINAVG_CODE;
+        const int64_t tile            = (int64_t)GF_BLOCK * EPT;
+        for(int64_t base = w.start; base < w.end; base += tile) {
+            ColSrc<EPT> src;
+            src.cols = A.cols;
+            int64_t row[EPT];
+            bool valid[EPT];
+#pragma unroll
+            for(int j = 0; j < EPT; ++j) {
+                const int64_t r = base + (int64_t)j * GF_BLOCK + threadIdx.x;
+                valid[j]        = r < w.end;
+                row[j]          = valid[j] ? r : w.start;
+                src.idx[j]      = row[j];
+            }
+            double T[EPT];
+            run<EPT>(A, P, Nm, 0, src, row, valid, T);
+#pragma unroll
+            for(int j = 0; j < EPT; ++j) {
+                if(!valid[j])
+                    continue;
+                double e0 = 0.0, e1 = 0.0;
+                if(binned) {
+                    e0 = A.cols[A.nobs - 2][row[j]];
+                    e1 = A.cols[A.nobs - 1][row[j]];
+                }
+                gfd::kb_add(s, c, gfd::metric(metric, T[j], e0, e1, norm, w.total));
+            }
+        }
+    }
+    store_partial(s, c, partials);
+}
+
+// One workgroup per toy: the toy's partials [first[t], first[t + 1]) in order
+// into result[2t] = sum, result[2t + 1] = residual (host-mapped).
+__global__ __launch_bounds__(GF_BLOCK) void k_finish_toys(const Partial *__restrict__ partials,
+                                                          const int *__restrict__ first,
+                                                          const uint8_t *__restrict__ active,
+                                                          double *__restrict__ result) {
+    const int t = blockIdx.x;
+    if(!active[t])
+        return;
+    double s = 0.0, c = 0.0;
+    for(int i = first[t] + threadIdx.x; i < first[t + 1]; i += GF_BLOCK)
+        gfd::kb_merge(s, c, partials[i].s, partials[i].c);
+    block_reduce(s, c);
+    if(threadIdx.x == 0) {
+        result[2 * t]     = s + c;
+        result[2 * t + 1] = c;
+    }
+}
+
+// Completion stamp after the toys' results: counter + 1 into the mapped word.
+__global__ void k_stamp(uint32_t *__restrict__ counter, uint32_t *__restrict__ mapped) {
+    const uint32_t v = *counter + 1u;
+    *counter         = v;
+    __threadfence_system();
+    __hip_atomic_store(mapped, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// ------------------------------------------------------------- event generation
+
+// Candidates for accept-reject generation from an Amp3Body: point i of the
+// counter-based stream is uniform in the box; its value is
+// |sum_r c_r B_r(m12, m13)|^2 * eff (0 outside the Dalitz plot, as
+// device_DalitzPlot, Amp3Body.cu:60-122, with lineshapes computed directly);
+// accepted when u_i * fmax < value.  Per block: accepted count and the largest
+// value (to check the envelope).
+__global__ __launch_bounds__(256) void k_dalitz_candidates(GfProgArgs A, const GfResDesc *__restrict__ res, int nres,
+                                                           int p_coup, int eff_pc, int id12, int id13, double l12,
+                                                           double u12, double l13, double u13, uint64_t key,
+                                                           int64_t base, int64_t nc, double fmax,
+                                                           double *__restrict__ c12, double *__restrict__ c13,
+                                                           uint8_t *__restrict__ flag, int *__restrict__ bcount,
+                                                           double *__restrict__ bmax) {
+    __shared__ int scount[4];
+    __shared__ double smax[4];
+    const int64_t i   = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid  = i < nc;
+    const GfDecay D   = A.decay;
+    const uint64_t ct = (uint64_t)(base + (valid ? i : 0));
+    const double m12  = l12 + gfd::unif01(key, 2 * ct) * (u12 - l12);
+    const double m13  = l13 + gfd::unif01(key, 2 * ct + 1) * (u13 - l13);
+    double value      = 0.0;
+    if(valid && gfd::in_dalitz(m12, m13, D.motherMass, D.daug1Mass, D.daug2Mass, D.daug3Mass)) {
+        const double m23 = D.motherMass * D.motherMass + D.daug1Mass * D.daug1Mass + D.daug2Mass * D.daug2Mass
+                           + D.daug3Mass * D.daug3Mass - m12 - m13;
+        double re = 0.0, im = 0.0;
+        for(int r = 0; r < nres; ++r) {
+            const double2 b = gfd::lineshape(res[r], A.params, A.consts, D, m12, m13, m23);
+            const double ar = A.params[p_coup + 2 * r], ai = A.params[p_coup + 2 * r + 1];
+            re += ar * b.x - ai * b.y;
+            im += ar * b.y + ai * b.x;
+        }
+        double eff = 1.0;
+        if(eff_pc >= 0) {
+            RegSrc<1> ev;
+#pragma unroll
+            for(int o = 0; o < GF_MAX_OBS; ++o)
+                ev.v[0][o] = 0.0;
+            ev.set(id12, 0, m12);
+            ev.set(id13, 0, m13);
+            const int64_t r0[1] = {0};
+            const bool v0[1]    = {true};
+            double T[1];
+            run<1>(A, A.params, A.norms, eff_pc, ev, r0, v0, T);
+            eff = T[0];
+        }
+        value = (re * re + im * im) * eff;
+    }
+    const bool acc = valid && gfd::unif01(key ^ 0xD1B54A32D192ED03ull, ct) * fmax < value;
+    if(valid) {
+        c12[i]  = m12;
+        c13[i]  = m13;
+        flag[i] = acc ? 1 : 0;
+    }
+    // block totals: count of accepted, largest value
+    const unsigned long long ball = __ballot(acc);
+    double vm = value;
+    for(int off = 32; off > 0; off >>= 1)
+        vm = ::fmax(vm, __shfl_xor(vm, off, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if(lane == 0) {
+        scount[wave] = __popcll(ball);
+        smax[wave]   = vm;
+    }
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        int c    = 0;
+        double m = 0.0;
+        for(int w = 0; w < 4; ++w) {
+            c += scount[w];
+            m = ::fmax(m, smax[w]);
+        }
+        bcount[blockIdx.x] = c;
+        bmax[blockIdx.x]   = m;
+    }
+}
+
+// Write the accepted candidates in order: block b's go to boff[b] + their rank
+// in the block (waves in order, lanes in order); rows >= limit are dropped.
+__global__ __launch_bounds__(256) void k_compact(const double *__restrict__ c12, const double *__restrict__ c13,
+                                                 const uint8_t *__restrict__ flag, int64_t nc,
+                                                 const int64_t *__restrict__ boff, int64_t limit,
+                                                 double *__restrict__ o12, double *__restrict__ o13) {
+    __shared__ int wcount[4];
+    const int64_t i  = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool acc   = i < nc && flag[i];
+    const unsigned long long ball = __ballot(acc);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if(lane == 0)
+        wcount[wave] = __popcll(ball);
+    __syncthreads();
+    int before = 0;
+    for(int w = 0; w < wave; ++w)
+        before += wcount[w];
+    const unsigned long long lower = lane ? (ball & ((1ull << lane) - 1ull)) : 0ull;
+    const int64_t dst = boff[blockIdx.x] + before + __popcll(lower);
+    if(acc && dst < limit) {
+        o12[dst] = c12[i];
+        o13[dst] = c13[i];
+    }
+}
+
 // ------------------------------------------------------------- FP64-MFMA Gram
 
 typedef double gf_double4 __attribute__((ext_vector_type(4)));
@@ -892,6 +1076,39 @@ int launch_reduce_bins(const GfProgArgs &a, int pc0, const double *ranges, const
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_reduce_bins<2>, dim3(nblocks), dim3(GF_BLOCK), 0, s, a, pc0, G, total, partials);
     return launch_finish(a, partials, nblocks, result, s);
+}
+
+int launch_batch(const GfProgArgs &a, const GfBatchWork *work, int nblocks, const double *params, int np,
+                 const double *norms, int nn, const uint8_t *active, int metric, Partial *partials, const int *first,
+                 int ntoys, double *result, uint32_t *counter, uint32_t *stamp, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_batch_metric<2>, dim3(nblocks), dim3(GF_BLOCK), 0, s, a, work, params, np, norms, nn, active,
+                       metric, partials);
+    int rc = check(hipGetLastError());
+    if(rc)
+        return rc;
+    hipLaunchKernelGGL(k_finish_toys, dim3(ntoys), dim3(GF_BLOCK), 0, s, partials, first, active, result);
+    if((rc = check(hipGetLastError())))
+        return rc;
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, s, counter, stamp);
+    return check(hipGetLastError());
+}
+
+int launch_dalitz_candidates(const GfProgArgs &a, const GfResDesc *res, int nres, int p_coup, int eff_pc, int id12,
+                             int id13, const double box[4], uint64_t key, int64_t base, int64_t nc, double fmax,
+                             double *c12, double *c13, uint8_t *flag, int *bcount, double *bmax, void *stream) {
+    const int nb = (int)((nc + 255) / 256);
+    hipLaunchKernelGGL(k_dalitz_candidates, dim3(nb), dim3(256), 0, (hipStream_t)stream, a, res, nres, p_coup, eff_pc,
+                       id12, id13, box[0], box[1], box[2], box[3], key, base, nc, fmax, c12, c13, flag, bcount, bmax);
+    return check(hipGetLastError());
+}
+
+int launch_compact(const double *c12, const double *c13, const uint8_t *flag, int64_t nc, const int64_t *boff,
+                   int64_t limit, double *o12, double *o13, void *stream) {
+    const int nb = (int)((nc + 255) / 256);
+    hipLaunchKernelGGL(k_compact, dim3(nb), dim3(256), 0, (hipStream_t)stream, c12, c13, flag, nc, boff, limit, o12,
+                       o13);
+    return check(hipGetLastError());
 }
 
 int launch_dalitz_cache(const GfProgArgs &a, int slot, int64_t offset, int64_t n, uint32_t mask, const GfResDesc *res,

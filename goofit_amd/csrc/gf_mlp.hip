@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdint>
 
+#include "gf_device.h"
 #include "gf_mlp.h"
 
 namespace {
@@ -49,16 +50,8 @@ __device__ __forceinline__ float dsilu(float z) {
     return s * (1.0f + z * (1.0f - s));
 }
 
-// counter-based generator: splitmix64 of (key, counter)
-__device__ __forceinline__ uint64_t mix64(uint64_t z) {
-    z += 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-__device__ __forceinline__ double unif(uint64_t key, uint64_t ctr) {
-    return (double)(mix64(key ^ mix64(ctr)) >> 11) * (1.0 / 9007199254740992.0);
-}
+using gfd::mix64;
+__device__ __forceinline__ double unif(uint64_t key, uint64_t ctr) { return gfd::unif01(key, ctr); }
 
 __device__ __forceinline__ bf16x8 load8(const uint16_t *p) {
     const uint4 q = *reinterpret_cast<const uint4 *>(p);  // 16-byte aligned

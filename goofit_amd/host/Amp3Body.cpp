@@ -4,6 +4,8 @@
 // the host assembly of the normalisation.
 #include <goofit/Error.h>
 #include <goofit/PDFs/physics/Amp3Body.h>
+#include <goofit/PDFs/physics/DalitzProposal.h>
+#include <goofit/UnbinnedDataSet.h>
 #include <goofit/PDFs/physics/resonances/GS.h>
 #include <goofit/PDFs/physics/resonances/NonRes.h>
 #include <goofit/PDFs/physics/resonances/RBW.h>
@@ -137,11 +139,17 @@ fptype Amp3Body::normalize() {
         engine::flush();
         if(cacheReady)
             engine::check(gf_dalitz_cache(engine::ctx(), (int)functionIdx, mask), "gf_dalitz_cache");
-        const double grid6[6] = {_m12.getLowerLimit(), _m12.getUpperLimit(), (double)_m12.getNumBins(),
-                                 _m13.getLowerLimit(), _m13.getUpperLimit(), (double)_m13.getNumBins()};
         std::vector<double> out(2 * nres * nres);
-        engine::check(gf_dalitz_gram(engine::ctx(), (int)functionIdx, grid6, (int)efficiencyFunction, out.data()),
-                      "gf_dalitz_gram");
+        if(normPoints > 0) {
+            engine::check(gf_dalitz_gram_norm_points(engine::ctx(), (int)functionIdx, (int)efficiencyFunction, 0,
+                                                     normPoints, out.data()),
+                          "gf_dalitz_gram_norm_points");
+        } else {
+            const double grid6[6] = {_m12.getLowerLimit(), _m12.getUpperLimit(), (double)_m12.getNumBins(),
+                                     _m13.getLowerLimit(), _m13.getUpperLimit(), (double)_m13.getNumBins()};
+            engine::check(gf_dalitz_gram(engine::ctx(), (int)functionIdx, grid6, (int)efficiencyFunction, out.data()),
+                          "gf_dalitz_gram");
+        }
         for(size_t k = 0; k < nres * nres; k++)
             integrals[k] = {out[2 * k], out[2 * k + 1]};
     }
@@ -157,11 +165,100 @@ fptype Amp3Body::normalize() {
     }
     fptype ret           = sumIntegral.real();
     double binSizeFactor = 1;
-    binSizeFactor *= _m12.getBinSize();
-    binSizeFactor *= _m13.getBinSize();
+    if(normPoints == 0) {  // Monte-Carlo weights already carry the measure
+        binSizeFactor *= _m12.getBinSize();
+        binSizeFactor *= _m13.getBinSize();
+    }
     ret *= binSizeFactor;
     setNormalization(1.0 / ret);
     return ret;
+}
+
+void Amp3Body::setNormalisationPoints(int64_t n, unsigned long long seed, DalitzProposal *proposal) {
+    if(n < 1)
+        throw GeneralError("Amp3Body::setNormalisationPoints: need at least one point");
+    if(proposal && !proposal->isBuilt())
+        throw GeneralError("Amp3Body::setNormalisationPoints: the proposal has no table (DalitzProposal::build)");
+    const double box[4] = {_m12.getLowerLimit(), _m12.getUpperLimit(), _m13.getLowerLimit(), _m13.getUpperLimit()};
+    engine::flush();
+    engine::check(gf_dalitz_set_norm_points(engine::ctx(), (int)functionIdx, proposal ? proposal->handle() : nullptr,
+                                            n, seed, box),
+                  "gf_dalitz_set_norm_points");
+    normPoints         = n;
+    forceRedoIntegrals = true;
+}
+
+void Amp3Body::setNormalisationGrid() {
+    normPoints         = 0;
+    forceRedoIntegrals = true;
+}
+
+fptype Amp3Body::normalisationRelativeVariance(int batches) {
+    if(normPoints == 0)
+        throw GeneralError("Amp3Body::normalisationRelativeVariance: normalisation is on the grid");
+    if(batches < 2 || normPoints / batches < 1)
+        throw GeneralError("Amp3Body::normalisationRelativeVariance: need at least 2 non-empty batches");
+    engine::flush();
+    const size_t nres   = decayInfo.resonances.size();
+    const int64_t count = normPoints / batches;
+    std::vector<double> out(2 * nres * nres);
+    std::vector<double> est(batches);
+    for(int b = 0; b < batches; b++) {
+        engine::check(gf_dalitz_gram_norm_points(engine::ctx(), (int)functionIdx, (int)efficiencyFunction,
+                                                 (int64_t)b * count, count, out.data()),
+                      "gf_dalitz_gram_norm_points");
+        std::complex<fptype> sum(0, 0);
+        for(size_t i = 0; i < nres; ++i) {
+            std::complex<fptype> ai(host_parameters.at(parametersIdx + i * 2 + 1),
+                                    host_parameters.at(parametersIdx + i * 2 + 2));
+            for(size_t j = 0; j < nres; ++j) {
+                std::complex<fptype> aj(host_parameters.at(parametersIdx + j * 2 + 1),
+                                        -host_parameters.at(parametersIdx + j * 2 + 2));
+                sum += ai * aj * std::complex<fptype>(out[2 * (i * nres + j)], out[2 * (i * nres + j) + 1]);
+            }
+        }
+        est[b] = sum.real() * (double)normPoints / (double)count;  // the batch's own estimate of the integral
+    }
+    double mean = 0, var = 0;
+    for(double v : est)
+        mean += v;
+    mean /= batches;
+    for(double v : est)
+        var += (v - mean) * (v - mean);
+    var /= (batches - 1);
+    return var / (mean * mean) * (double)count;
+}
+
+int64_t Amp3Body::generateEvents(UnbinnedDataSet &out, size_t n, unsigned long long seed, double fmax) {
+    if(n == 0)
+        return 0;
+    const double box[4] = {_m12.getLowerLimit(), _m12.getUpperLimit(), _m13.getLowerLimit(), _m13.getUpperLimit()};
+    std::vector<fptype> a(n), b(n), num(n);
+    engine::flush();
+    int64_t tried = 0;
+    engine::check(gf_dalitz_generate(engine::ctx(), (int)functionIdx, (int64_t)n, seed, box, fmax, 1, a.data(),
+                                     b.data(), &tried, nullptr),
+                  "gf_dalitz_generate");
+    const size_t first = out.getNumEvents();
+    for(size_t i = 0; i < n; i++)
+        num[i] = (fptype)(first + i);
+    std::vector<const fptype *> cols;
+    for(const Observable &o : out.getObservables()) {
+        if(o == _m12)
+            cols.push_back(a.data());
+        else if(o == _m13)
+            cols.push_back(b.data());
+        else if(o == _eventNumber)
+            cols.push_back(num.data());
+        else
+            throw GeneralError("Amp3Body::generateEvents: the data set has observable {} besides m12, m13 and the "
+                               "event number",
+                               o.getName());
+    }
+    if(cols.size() != 3)
+        throw GeneralError("Amp3Body::generateEvents: the data set needs m12, m13 and the event number");
+    out.appendColumns(cols, n);
+    return tried;
 }
 
 const std::vector<std::complex<fptype>> Amp3Body::getCachedWave(size_t i) const {

@@ -1,0 +1,207 @@
+// BatchFitter: many MIGRAD fits whose FCN calls are coalesced into batched GPU
+// reductions (gf_batch_* of the C ABI).
+#include <goofit/Engine.h>
+#include <goofit/Error.h>
+#include <goofit/PDFs/GooPdf.h>
+#include <goofit/fitting/BatchFitter.h>
+#include <goofit/fitting/Params.h>
+
+#include <Minuit2/FCNBase.h>
+#include <Minuit2/FunctionMinimum.h>
+#include <Minuit2/MnMigrad.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <mutex>
+#include <thread>
+
+namespace GooFit {
+
+namespace {
+
+// Shared state of one fit(): pending FCN requests and their answers.
+struct Batcher {
+    std::mutex mu;
+    std::condition_variable cv_req, cv_done;
+    std::vector<std::vector<double>> x;  // per toy: the requested Minuit vector
+    std::vector<uint8_t> pending, done;
+    std::vector<double> result;
+    int running = 0;   // toys whose MIGRAD is in progress
+    int waiting = 0;   // of those, blocked in an FCN call
+    bool stop   = false;
+};
+
+class ToyFCN : public Minuit2::FCNBase {
+  public:
+    ToyFCN(Batcher &b, int toy)
+        : b_(b)
+        , toy_(toy) {}
+    double operator()(const std::vector<double> &x) const override {
+        std::unique_lock<std::mutex> lk(b_.mu);
+        b_.x[toy_]       = x;
+        b_.done[toy_]    = 0;
+        b_.pending[toy_] = 1;
+        b_.waiting++;
+        b_.cv_req.notify_one();
+        b_.cv_done.wait(lk, [&] { return b_.done[toy_] != 0; });
+        return b_.result[toy_];
+    }
+    double Up() const override { return 0.5; }
+
+  private:
+    Batcher &b_;
+    int toy_;
+};
+
+}  // namespace
+
+BatchFitter::BatchFitter(GooPdf *pdf, const std::vector<int64_t> &first, const std::vector<int64_t> &count)
+    : pdf_(pdf)
+    , first_(first)
+    , count_(count) {
+    if(!pdf || first.size() != count.size() || first.empty())
+        throw GeneralError("BatchFitter: need a PDF and one (first, count) per toy");
+    engine::flush();
+    engine::check(gf_batch_create(engine::ctx(), (int)first.size(), first.data(), count.data(), &batch_),
+                  "gf_batch_create");
+}
+
+BatchFitter::~BatchFitter() { gf_batch_destroy(batch_); }
+
+void BatchFitter::useGraph(bool on) { engine::check(gf_batch_use_graph(batch_, on ? 1 : 0), "gf_batch_use_graph"); }
+
+std::vector<double> BatchFitter::evaluate() {
+    pdf_->normalize();
+    engine::flush();
+    std::vector<uint8_t> all(size(), 1);
+    std::vector<double> out(size());
+    for(size_t t = 0; t < size(); t++)
+        engine::check(gf_batch_set_toy(batch_, (int)t, host_parameters.data(), host_normalizations.data()),
+                      "gf_batch_set_toy");
+    engine::check(gf_batch_reduce(batch_, all.data(), GF_METRIC_NLL, out.data()), "gf_batch_reduce");
+    return out;
+}
+
+void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool autotune) {
+    const int T = (int)size();
+    Params params(*pdf_);
+    std::vector<Variable> vars = params.GetGooFitParams();
+    const std::vector<double> start = params.make_minuit_vector();
+    const auto t0 = std::chrono::steady_clock::now();
+    pdf_->normalize();
+    engine::flush();
+    if(autotune) {
+        for(int t = 0; t < T; t++)
+            engine::check(gf_batch_set_toy(batch_, t, host_parameters.data(), host_normalizations.data()),
+                          "gf_batch_set_toy");
+        engine::check(gf_batch_autotune(batch_, GF_METRIC_NLL, &stats_.chunk, nullptr), "gf_batch_autotune");
+    }
+    Batcher B;
+    B.x.assign(T, {});
+    B.pending.assign(T, 0);
+    B.done.assign(T, 0);
+    B.result.assign(T, 0.0);
+    values_.assign(T, {});
+    errors_.assign(T, {});
+    valid_.assign(T, 0);
+    nfcn_.assign(T, 0);
+    fval_.assign(T, 0.0);
+    int next    = 0;  // next toy to start (under B.mu)
+    const int W = std::max(1, std::min(threads, T));
+    B.running   = 0;
+    auto worker = [&]() {
+        for(;;) {
+            int t;
+            {
+                std::lock_guard<std::mutex> lk(B.mu);
+                if(next >= T)
+                    break;
+                t = next++;
+                B.running++;
+            }
+            ToyFCN fcn(B, t);
+            Minuit2::MnUserParameters up = static_cast<const Minuit2::MnUserParameters &>(params);
+            Minuit2::MnMigrad migrad(fcn, up, Minuit2::MnStrategy(strategy));
+            Minuit2::FunctionMinimum m = migrad(0, tolerance);
+            values_[t] = m.UserParameters().Params();
+            errors_[t] = m.UserParameters().Errors();
+            valid_[t]  = m.IsValid() ? 1 : 0;
+            nfcn_[t]   = (int)m.NFcn();
+            fval_[t]   = m.Fval();
+            std::lock_guard<std::mutex> lk(B.mu);
+            B.running--;
+            B.cv_req.notify_one();
+        }
+    };
+    std::vector<std::thread> pool;
+    pool.reserve(W);
+    for(int i = 0; i < W; i++)
+        pool.emplace_back(worker);
+
+    // batcher: serve every pending call of the running toys in one launch
+    std::vector<uint8_t> act(T);
+    std::vector<double> out(T);
+    std::vector<int> serve;
+    std::exception_ptr failure;
+    for(;;) {
+        std::unique_lock<std::mutex> lk(B.mu);
+        // wait until every running toy is blocked in an FCN call (or all done)
+        B.cv_req.wait_for(lk, std::chrono::milliseconds(50), [&] {
+            return (B.running > 0 && B.waiting == B.running) || (B.running == 0 && next >= T);
+        });
+        if(B.running == 0 && next >= T)
+            break;
+        if(B.waiting == 0)
+            continue;
+        if(B.waiting < B.running) {
+            // some toys are in host-side Minuit work: give them a moment, then serve what is there
+            B.cv_req.wait_for(lk, std::chrono::microseconds(200),
+                              [&] { return B.waiting == B.running; });
+        }
+        serve.clear();
+        for(int t = 0; t < T; t++)
+            if(B.pending[t])
+                serve.push_back(t);
+        std::fill(act.begin(), act.end(), 0);
+        try {
+            for(int t : serve) {
+                // push the toy's values into the shared Variables, then the host arrays
+                const std::vector<double> &x = B.x[t];
+                for(Variable &v : vars)
+                    v.setValue(x.at(v.getFitterIndex()) - v.getBlind(Variable::Key()));
+                pdf_->updateParameters();
+                pdf_->normalize();
+                engine::check(gf_batch_set_toy(batch_, t, host_parameters.data(), host_normalizations.data()),
+                              "gf_batch_set_toy");
+                act[t] = 1;
+            }
+            const auto g0 = std::chrono::steady_clock::now();
+            engine::check(gf_batch_reduce(batch_, act.data(), GF_METRIC_NLL, out.data()), "gf_batch_reduce");
+            stats_.gpu_wait_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - g0).count();
+        } catch(...) {
+            failure = std::current_exception();
+            for(int t : serve)
+                out[t] = 1e300;
+        }
+        stats_.launches++;
+        stats_.fcn_calls += (int64_t)serve.size();
+        for(int t : serve) {
+            B.result[t]  = out[t];
+            B.pending[t] = 0;
+            B.done[t]    = 1;
+            B.waiting--;
+        }
+        B.cv_done.notify_all();
+    }
+    for(auto &th : pool)
+        th.join();
+    // restore the starting values in the tree
+    for(Variable &v : vars)
+        v.setValue(start.at(v.getFitterIndex()));
+    pdf_->updateParameters();
+    stats_.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if(failure)
+        std::rethrow_exception(failure);
+}
+
+}  // namespace GooFit

@@ -260,25 +260,12 @@ def dalitz_dataset(model, m12, m13):
     return d
 
 
-def generate_dalitz(model, n, seed=6, batch=4_000_000, safety=1.05):
-    """n events of the model by accept-reject on the GPU-evaluated |A|^2 * eff over
-    points uniform in the Dalitz plot.  Returns (m12, m13)."""
-    rng = np.random.default_rng(seed)
+def generate_dalitz(model, n, seed=6):
+    """n events of the model (current parameters) by accept-reject on the GPU
+    (Amp3Body.generateEvents).  Returns (m12, m13)."""
+    import goofit_amd as G
     pdf = model.pdf
-    acc12, acc13, fmax = [], [], None
-    have = 0
-    while have < n:
-        c12, c13 = dalitz_uniform(batch, int(rng.integers(1 << 31)))
-        data = dalitz_dataset(model, c12, c13)
-        pdf.setData(data)
-        pdf.setDataSize(batch)
-        pdf.setForceIntegrals()
-        from goofit_amd import EvalFunc
-        f = pdf.evaluateAtData(EvalFunc.Eval)
-        if fmax is None:
-            fmax = safety * float(f.max())
-        keep = rng.random(batch) * fmax < f
-        acc12.append(c12[keep])
-        acc13.append(c13[keep])
-        have += int(keep.sum())
-    return np.concatenate(acc12)[:n], np.concatenate(acc13)[:n]
+    pdf.SigGenSetIndices()
+    d = G.UnbinnedDataSet(model.observables)
+    pdf.generateEvents(d, int(n), int(seed))
+    return d.column(model.observables[0]), d.column(model.observables[1])

@@ -16,6 +16,8 @@
 
 namespace GooFit {
 
+class DalitzProposal;
+
 class Amp3Body : public Amp3BodyBase {
   public:
     Amp3Body(std::string n, Observable m12, Observable m13, EventNumber eventNumber, DecayInfo3 decay,
@@ -45,6 +47,26 @@ class Amp3Body : public Amp3BodyBase {
     /// Function index of the efficiency (the last function of this subtree).
     unsigned int getEfficiencyIndex() const { return efficiencyFunction; }
 
+    /// Normalise by n Monte-Carlo points of the (m12, m13) box instead of the midpoint
+    /// grid: uniform (proposal == nullptr) or drawn from a learned proposal with
+    /// importance weights.  The integrals are then sum_p w_p eff B_i conj(B_j) (no bin
+    /// area factor).  No counterpart in the reference.
+    void setNormalisationPoints(int64_t n, unsigned long long seed, DalitzProposal *proposal = nullptr);
+    /// Back to the reference's midpoint grid (the default).
+    void setNormalisationGrid();
+    int64_t getNormalisationPointCount() const { return normPoints; }
+    /// Relative variance per point of the normalisation estimate at the current
+    /// couplings, from `batches` disjoint batches of the points (Monte-Carlo mode).
+    fptype normalisationRelativeVariance(int batches = 16);
+
+    /// Append n events of this PDF's density at the current parameters to `out`
+    /// (observables m12, m13 and the event number, numbered from out's current
+    /// size), generated on the GPU by accept-reject with a seeded counter-based
+    /// stream.  fmax <= 0 estimates the envelope.  Returns the candidates tried.
+    /// Takes the place of the reference's MCBooster-based GenerateSig
+    /// (src/PDFs/physics/Amp3Body.cu:430-547).
+    int64_t generateEvents(UnbinnedDataSet &out, size_t n, unsigned long long seed, double fmax = 0);
+
   protected:
     DecayInfo3 decayInfo;
     Observable _m12;
@@ -60,6 +82,7 @@ class Amp3Body : public Amp3BodyBase {
     static int cacheCount;
     unsigned int efficiencyFunction{0};
     bool cacheReady{false};
+    int64_t normPoints{0};  // 0: midpoint grid
 };
 
 }  // namespace GooFit

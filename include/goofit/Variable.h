@@ -15,6 +15,7 @@
 namespace GooFit {
 
 class Params;
+class BatchFitter;
 
 /// A named, limited, indexable quantity (min == max means "no limits").
 class Indexable {
@@ -93,6 +94,7 @@ class Variable : public Indexable {
     /// Key to the blinding offset (only the fitter may read it).
     class Key {
         friend Params;
+        friend BatchFitter;
         Key() = default;
     };
 

@@ -1,0 +1,69 @@
+// Many toy fits of one PDF model at once (no counterpart in the reference,
+// which fits one data set per FitManager).
+//
+// Toy t owns rows [first[t], first[t] + count[t]) of one data set.  One Minuit2
+// MIGRAD per toy runs on a pool of host threads; every FCN call posts the toy's
+// parameter vector to a batcher, which turns the pending calls of all toys into
+// one GPU launch (a captured hipGraph: staging copy, per-toy event sweep,
+// per-toy finishing reduction) and hands each toy its NLL.  The normalisations
+// are computed on the host per toy (analytic integrals, or an Amp3Body's cached
+// integral matrix), so the model's non-analytic parts must not depend on the
+// floating parameters.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include <goofit/GlobalCudaDefines.h>
+#include <goofit_hip.h>
+
+namespace GooFit {
+
+class GooPdf;
+class UnbinnedDataSet;
+
+struct BatchFitStats {
+    int64_t launches   = 0;  // batched GPU reductions
+    int64_t fcn_calls  = 0;  // FCN calls served
+    double seconds     = 0;  // wall time of fit()
+    double gpu_wait_s  = 0;  // time spent in the batched reductions
+    int chunk          = 0;  // events per workgroup chosen by the autotuner
+};
+
+class BatchFitter {
+  public:
+    /// pdf must already hold `data` (setData; for an Amp3Body also setDataSize).
+    BatchFitter(GooPdf *pdf, const std::vector<int64_t> &first, const std::vector<int64_t> &count);
+    ~BatchFitter();
+    BatchFitter(const BatchFitter &)            = delete;
+    BatchFitter &operator=(const BatchFitter &) = delete;
+
+    /// Fit every toy with MIGRAD from the Variables' current values using `threads`
+    /// concurrent Minuit instances (strategy, maxfcn 0 = Minuit's default, tolerance).
+    void fit(int threads = 256, unsigned int strategy = 1, double tolerance = 0.1, bool autotune = true);
+
+    /// NLL of every toy at the Variables' current values (one batched launch).
+    std::vector<double> evaluate();
+
+    size_t size() const { return first_.size(); }
+    /// Per toy: fitted values / errors of the Minuit parameters (Minuit order), validity, calls.
+    const std::vector<std::vector<double>> &values() const { return values_; }
+    const std::vector<std::vector<double>> &errors() const { return errors_; }
+    const std::vector<int> &valid() const { return valid_; }
+    const std::vector<int> &nfcn() const { return nfcn_; }
+    const std::vector<double> &fval() const { return fval_; }
+    const BatchFitStats &stats() const { return stats_; }
+
+    void useGraph(bool on);
+
+  private:
+    GooPdf *pdf_;
+    std::vector<int64_t> first_, count_;
+    gf_batch *batch_{nullptr};
+    std::vector<std::vector<double>> values_, errors_;
+    std::vector<int> valid_, nfcn_;
+    std::vector<double> fval_;
+    BatchFitStats stats_;
+};
+
+}  // namespace GooFit

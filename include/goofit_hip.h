@@ -291,6 +291,44 @@ gf_status gf_dalitz_set_norm_points(gf_ctx *ctx, int dalitz_func, gf_mlp *propos
 gf_status gf_dalitz_gram_norm_points(gf_ctx *ctx, int dalitz_func, int eff_func, int64_t offset, int64_t count,
                                      double *out_re_im);
 
+/* n events of Amp3Body dalitz_func's density |sum c_i B_i|^2 * eff (current
+ * parameters) by accept-reject on uniform candidates of box: candidate k of the
+ * seeded counter-based stream is accepted when u_k * fmax < value_k; accepted
+ * events keep the stream's order, so (seed, fmax) fix the events.  fmax <= 0:
+ * 1.25 x the largest value over the first 2^22 candidates of the stream (which
+ * are then skipped).  Fails if a candidate exceeds fmax.  host_out != 0:
+ * m12/m13 are host arrays, else device arrays.  tried / fmax_used may be NULL.
+ * Takes the place of the reference's MCBooster generation
+ * (Amp3Body::GenerateSig, src/PDFs/physics/Amp3Body.cu:430-547) and of its
+ * grid-based DalitzPlotter::fillDataSetMC (include/goofit/PDFs/physics/DalitzPlotter.h:64-95). */
+gf_status gf_dalitz_generate(gf_ctx *ctx, int dalitz_func, int64_t n, uint64_t seed, const double box[4],
+                             double fmax, int host_out, double *m12, double *m13, int64_t *tried,
+                             double *fmax_used);
+
+/* ---- toy batches ----------------------------------------------------------
+ * Many fits of one model at once (no reference counterpart; the reference fits
+ * one data set per FitManager, src/goofit/FitManagerMinuit2.cpp:31-119).  Toy t
+ * owns rows [first[t], first[t] + count[t]) of the context's events and its own
+ * copies of the parameter and normalisation arrays (layout of gf_program_set).
+ * One gf_batch_reduce is one captured hipGraph: the staged arrays are copied in,
+ * one sweep covers every active toy's rows, and one workgroup per toy adds its
+ * partials in a fixed order. */
+typedef struct gf_batch gf_batch;
+
+gf_status gf_batch_create(gf_ctx *ctx, int ntoys, const int64_t *first, const int64_t *count, gf_batch **out);
+void gf_batch_destroy(gf_batch *b);
+/* Stage toy's parameter / normalisation arrays (either may be NULL: unchanged). */
+gf_status gf_batch_set_toy(gf_batch *b, int toy, const double *params, const double *norms);
+/* Metric sums of the toys with active[t] != 0 into out[t] (others untouched). */
+gf_status gf_batch_reduce(gf_batch *b, const uint8_t *active, int metric, double *out);
+/* Events per workgroup (a multiple of 256); default 8192. */
+gf_status gf_batch_set_chunk(gf_batch *b, int chunk);
+gf_status gf_batch_get_chunk(gf_batch *b, int *chunk);
+/* on = 0: plain stream launches instead of the captured graph. */
+gf_status gf_batch_use_graph(gf_batch *b, int on);
+/* Time every active-all reduction for chunk in {1024 ... 32768}, keep the fastest. */
+gf_status gf_batch_autotune(gf_batch *b, int metric, int *chunk_out, double *us_out);
+
 /* ---- timing ------------------------------------------------------------- */
 
 /* Run gf_reduce_metric `reps` times back to back on the context stream and

@@ -317,3 +317,46 @@ def test_dalitz_nll_paths(ctx):
         assert math.isclose(got, ref, rel_tol=NLL_RTOL), (permute, got, ref)
         probs = ctx.eval_metric("PROB")
         np.testing.assert_allclose(probs, M.op.eval_metric(a, "PROB"), rtol=1e-12, atol=0)
+
+
+def test_dalitz_generate(ctx):
+    """Accept-reject generation on the GPU: reproducible for a seed, every event inside the plot,
+    and the histogram over cells fully inside the plot follows |A|^2 (chi^2 per degree of freedom)."""
+    top, M, aos = _dalitz_setup(ctx, 1000, nbins=64)
+    M.op.sync_params()
+    ctx.update_params(M.op.p)
+    n = 200_000
+    box = [0.0, 3.0, 0.0, 3.0]
+    _, _, tried, fmax = ctx.dalitz_generate(top.function_idx, 1000, 5, box)  # envelope estimate
+    assert tried >= 1000 and fmax > 0
+    a12, a13, _, _ = ctx.dalitz_generate(top.function_idx, n, 5, box, fmax)
+    b12, b13, _, _ = ctx.dalitz_generate(top.function_idx, n, 5, box, fmax)
+    c12, _, _, _ = ctx.dalitz_generate(top.function_idx, n, 6, box, fmax)
+    assert np.array_equal(a12, b12) and np.array_equal(a13, b13)
+    assert not np.array_equal(a12, c12)
+    assert W.in_dalitz_np(a12, a13).all()
+    # expected content of 20 x 20 coarse cells from |A|^2 on a 400 x 400 midpoint grid
+    F, C = 400, 20
+    fc = (np.arange(F) + 0.5) * 3.0 / F
+    g12, g13 = np.meshgrid(fc, fc, indexing="ij")
+    pts = np.stack([g12.ravel(), g13.ravel()], 1)
+    ctx.upload_columns([pts[:, 0], pts[:, 1], np.arange(len(pts), dtype=np.float64)])
+    ctx.set_identity_column(2)
+    ctx.dalitz_set_data_size(top.function_idx, len(pts))
+    ctx.dalitz_cache(top.function_idx)
+    val = ctx.eval_metric("EVAL").reshape(F, F)
+    S = val.reshape(C, F // C, C, F // C).sum(axis=(1, 3))
+    edges = np.linspace(0, 3, C + 1)
+    inside = np.ones((C, C), bool)
+    for i in range(C):
+        for j in range(C):
+            for x in (edges[i], edges[i + 1]):
+                for y in (edges[j], edges[j + 1]):
+                    inside[i, j] &= bool(W.in_dalitz_np(np.array([x]), np.array([y]))[0])
+    H, _, _ = np.histogram2d(a12, a13, bins=[edges, edges])
+    obs, exp = H[inside], S[inside]
+    exp = exp / exp.sum() * obs.sum()
+    chi2 = ((obs - exp) ** 2 / exp).sum()
+    ndf = inside.sum() - 1
+    assert ndf > 50
+    assert chi2 / ndf < 1.35, (chi2, ndf)
