@@ -89,14 +89,22 @@ def norm_error_study(model, pdf, train_events, sync, target=1e-4, seeds=6):
     out = {}
     truth = {n: v.value for n, v in model.params.items()}
     grid_norm = pdf.normalize()
-    t0 = time.perf_counter()
-    pdf.setNormalisationPoints(100_000_000, 991)
-    judge = pdf.normalize()
-    sync()
-    out["judge"] = dict(points=100_000_000, seconds=round(time.perf_counter() - t0, 3),
-                        rel_stat_err=math.sqrt(pdf.normalisationRelativeVariance(16) / 1e8),
-                        grid_vs_judge=grid_norm / judge - 1)
-    log(f"judge: 1e8 uniform points, grid/judge - 1 = {grid_norm / judge - 1:.2e}")
+    # judge: uniform FP64 Monte Carlo, 1e8 points as specified and 1e9 for a tighter reference
+    judges = {}
+    for npts, seed in ((100_000_000, 991), (1_000_000_000, 992)):
+        t0 = time.perf_counter()
+        pdf.setNormalisationPoints(npts, seed)
+        val = pdf.normalize()
+        sync()
+        judges[npts] = dict(points=npts, value=val, seconds=round(time.perf_counter() - t0, 3),
+                            rel_stat_err=math.sqrt(pdf.normalisationRelativeVariance(16) / npts),
+                            grid_vs_judge=grid_norm / val - 1)
+    judge = judges[1_000_000_000]["value"]
+    out["judge"] = judges[1_000_000_000]
+    out["judge_1e8"] = judges[100_000_000]
+    out["judge_1e8"]["vs_1e9"] = judges[100_000_000]["value"] / judge - 1
+    log(f"judge: 1e9 uniform points (rel. stat. error {out['judge']['rel_stat_err']:.1e}); "
+        f"1e8-point judge / 1e9 - 1 = {out['judge_1e8']['vs_1e9']:.2e}; grid / judge - 1 = {grid_norm / judge - 1:.2e}")
 
     def errors(n, proposal, base):
         e, v = [], []
@@ -155,6 +163,129 @@ def norm_error_study(model, pdf, train_events, sync, target=1e-4, seeds=6):
     return out
 
 
+def toy_counts(n, seed0):
+    """Per-toy event counts: Poisson with mean uniform in [5e3, 8e4], each from its own seed."""
+    out = []
+    for t in range(n):
+        rng = np.random.default_rng(seed0 + t)
+        out.append(int(rng.poisson(rng.uniform(5e3, 8e4))))
+    return np.array(out, np.int64)
+
+
+def fit_toys(model, data, counts, truth, sync, seq_toys, threads):
+    """Batched MIGRAD fits of every toy, sequential fits of the first seq_toys; pulls."""
+    import goofit_amd as G
+    pdf = model.pdf
+    names = sorted(model.params)
+    for k in names:
+        model.params[k].value = truth[k]
+    first = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)
+    bf = G.BatchFitter(pdf, list(first), list(counts))
+    sync()
+    t0 = time.perf_counter()
+    bf.fit(threads=threads)
+    sync()
+    t_batch = time.perf_counter() - t0
+    st = bf.stats()
+    vals, errs, valid = np.array(bf.values()), np.array(bf.errors()), np.array(bf.valid())
+    fm0 = G.FitManager(pdf)
+    mv = fm0.minuit_vector()
+    free = np.array(fm0.free_mask(), bool)
+    tv = np.array(mv)  # truth in Minuit order (the Variables hold the truth)
+    ok = valid.astype(bool)
+    pulls = (vals[ok][:, free] - tv[free]) / errs[ok][:, free]
+    pm, pw = pulls.mean(0), pulls.std(0, ddof=1)
+    # sequential fits of the first toys, one data set at a time
+    seq_vals, t_seq = [], 0.0
+    cols = [data.column(o) for o in model.observables]
+    for t in range(seq_toys):
+        for k in names:
+            model.params[k].value = truth[k]
+        d = G.UnbinnedDataSet(model.observables)
+        sl = slice(int(first[t]), int(first[t] + counts[t]))
+        parts = [c[sl] for c in cols]
+        if len(parts) == 3:  # Dalitz: renumber the events of the toy
+            parts[2] = np.arange(int(counts[t]), dtype=np.float64)
+        d.from_columns(parts)
+        sync()
+        t0 = time.perf_counter()
+        pdf.setData(d)
+        if hasattr(pdf, "setDataSize"):
+            pdf.setDataSize(int(counts[t]))
+        fm = G.FitManager(pdf)
+        fm.setVerbosity(0)
+        r = fm.fit()
+        sync()
+        t_seq += time.perf_counter() - t0
+        seq_vals.append((np.array(r.values()), np.array(r.errors())))
+    shifts = [np.max(np.abs(vals[t][free] - sv[free]) / se[free]) for t, (sv, se) in enumerate(seq_vals)]
+    for k in names:
+        model.params[k].value = truth[k]
+    pdf.setData(data)
+    if hasattr(pdf, "setDataSize"):
+        pdf.setDataSize(int(counts.sum()))
+    n = len(counts)
+    return dict(toys=n, events=int(counts.sum()), free_params=int(free.sum()), valid=int(ok.sum()),
+                batched_s=round(t_batch, 3), batched_toys_per_hour=round(n / t_batch * 3600, 1),
+                sequential_toys=seq_toys, sequential_s=round(t_seq, 3),
+                sequential_toys_per_hour=round(seq_toys / t_seq * 3600, 1) if t_seq > 0 else None,
+                speedup=round((n / t_batch) / (seq_toys / t_seq), 3) if t_seq > 0 else None,
+                launches=st["launches"], fcn_calls=st["fcn_calls"], gpu_wait_s=round(st["gpu_wait_s"], 3),
+                chunk=st["chunk"], pull_mean=[round(float(v), 4) for v in pm],
+                pull_width=[round(float(v), 4) for v in pw],
+                max_abs_pull_mean=round(float(np.abs(pm).max()), 4),
+                max_abs_pull_width_minus_1=round(float(np.abs(pw - 1).max()), 4),
+                batched_vs_sequential_max_shift_sigma=round(float(max(shifts)), 5) if shifts else None)
+
+
+def toy_study(ntoys, sync, seq_toys=16, threads=256):
+    """cfg: ntoys/2 toys of the 1-D AddPdf model and ntoys/2 of an 8-resonance cut of the Dalitz model,
+    Poisson-varied sizes, fitted in batches and (a few) sequentially."""
+    import goofit_amd as G
+    from goofit_amd import workloads as W
+    half = ntoys // 2
+    out = {}
+    # 1-D AddPdf toys
+    counts = toy_counts(half, 10_000)
+    t0 = time.perf_counter()
+    xs = np.concatenate([W.sample_add1d(int(n), seed=50_000 + t) for t, n in enumerate(counts)])
+    t_gen = time.perf_counter() - t0
+    m = W.build_add1d(start=1.0)
+    truth = {k: v.value for k, v in m.params.items()}
+    d = G.UnbinnedDataSet(m.observables)
+    d.from_columns([xs])
+    m.pdf.setData(d)
+    out["add1d"] = fit_toys(m, d, counts, truth, sync, seq_toys, threads)
+    out["add1d"]["generation_s"] = round(t_gen, 2)
+    log("add1d toys", {k: out["add1d"][k] for k in ("batched_s", "batched_toys_per_hour",
+                                                    "sequential_toys_per_hour", "max_abs_pull_mean",
+                                                    "max_abs_pull_width_minus_1")})
+    del d
+    # 8-resonance Dalitz toys
+    counts = toy_counts(half, 20_000)
+    md = W.build_dalitz(nbins=400, resonances=W.DALITZ_RESONANCES[:8])
+    truth = {k: v.value for k, v in md.params.items()}
+    t0 = time.perf_counter()
+    g12, g13 = W.generate_dalitz(md, int(counts.sum()), seed=77)
+    sync()
+    t_gen = time.perf_counter() - t0
+    dd = W.dalitz_dataset(md, g12, g13)
+    md.pdf.setData(dd)
+    md.pdf.setDataSize(int(counts.sum()))
+    md.pdf.setForceIntegrals()
+    md.pdf.calculateNLL()
+    out["dalitz8"] = fit_toys(md, dd, counts, truth, sync, seq_toys, threads)
+    out["dalitz8"]["generation_s"] = round(t_gen, 2)
+    log("dalitz8 toys", {k: out["dalitz8"][k] for k in ("batched_s", "batched_toys_per_hour",
+                                                        "sequential_toys_per_hour", "max_abs_pull_mean",
+                                                        "max_abs_pull_width_minus_1")})
+    tot_t = out["add1d"]["batched_s"] + out["dalitz8"]["batched_s"]
+    seq_rate = 2.0 / (1 / out["add1d"]["sequential_toys_per_hour"] + 1 / out["dalitz8"]["sequential_toys_per_hour"])
+    out["all"] = dict(toys=2 * half, batched_s=round(tot_t, 3), batched_toys_per_hour=round(2 * half / tot_t * 3600, 1),
+                      sequential_toys_per_hour=round(seq_rate, 1))
+    return out
+
+
 def cpu_baseline_dalitz(flat, decay, m12, m13, target_s=10.0):
     """Oracle (C/OpenMP restatement of the reference algorithm) NLL over the same events."""
     import oracle as O
@@ -195,6 +326,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the normalisation-error study")
+    ap.add_argument("--toys", type=int, default=4096, help="batched toy fits (0: skip)")
     ap.add_argument("--profile-only", action="store_true",
                     help="set up the Dalitz workload, run 20 NLL reductions and a 256 MiB device copy "
                          "(HBM counter calibration), print nothing else; for rocprofv3 runs")
@@ -308,6 +440,11 @@ def main():
         pdf.setForceIntegrals()
         pdf.calculateNLL()
 
+    toys = None
+    if args.toys > 0:
+        log(f"toy study: {args.toys} toys")
+        toys = toy_study(args.toys, sync)
+
     workloads = {}
     if not args.no_extra:
         for name, builder, sampler, nevt, bpe in (
@@ -343,7 +480,7 @@ def main():
                 config=dict(workload="dalitz16_d0_pipipi0", events=args.events, norm_grid=f"{args.grid}x{args.grid}",
                             resonances=16, free_params=30, step="calculateNLL with new couplings"),
                 roofline=roofline, cpu_baseline=cpu, fit=fit, dalitz=extra, norm_error=norm_study,
-                workloads=workloads)
+                toys=toys, workloads=workloads)
     print(json.dumps(line), flush=True)
 
 

@@ -166,9 +166,14 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
         try {
             for(int t : serve) {
                 // push the toy's values into the shared Variables, then the host arrays
+                // as Params::from_minuit_vector: flag the values that move, so that an
+                // Amp3Body recomputes its integrals only when a mass or width changes
                 const std::vector<double> &x = B.x[t];
-                for(Variable &v : vars)
-                    v.setValue(x.at(v.getFitterIndex()) - v.getBlind(Variable::Key()));
+                for(Variable &v : vars) {
+                    const double nv = x.at(v.getFitterIndex()) - v.getBlind(Variable::Key());
+                    v.setChanged(v.getValue() != nv);
+                    v.setValue(nv);
+                }
                 pdf_->updateParameters();
                 pdf_->normalize();
                 engine::check(gf_batch_set_toy(batch_, t, host_parameters.data(), host_normalizations.data()),
