@@ -17,7 +17,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 try:
     from . import _goofit  # noqa: F401
     from ._goofit import (  # noqa: F401
-        AddPdf, Amp3Body, BatchFitter, DalitzProposal, DataSet, DaughterPair, DecayInfo3, EvalFunc, EventNumber, ExpGausPdf, ExpPdf,
+        AddPdf, Amp3Body, BatchFitter, BinnedChisqFit, BinnedDataSet, BinnedErrorFit, BinnedNllFit, DalitzProposal,
+        DataSet, EvalFit, FitControl, ProbFit, UnbinnedNllFit, DaughterPair, DecayInfo3, EvalFunc, EventNumber, ExpGausPdf, ExpPdf,
         FCN, FitManager, FunctionMinimum, GaussianPdf, GeneralError, GooPdf, Observable, PAIR_12, PAIR_13, PAIR_23,
         Params, PdfBase, PolynomialPdf, ProdPdf, Resonances, UnbinnedDataSet, Variable, engine, inDalitz,
     )

@@ -22,6 +22,8 @@
 #include <goofit/PDFs/physics/DalitzProposal.h>
 #include <goofit/fitting/BatchFitter.h>
 #include <goofit/PDFs/physics/ResonancePdf.h>
+#include <goofit/BinnedDataSet.h>
+#include <goofit/FitControl.h>
 #include <goofit/UnbinnedDataSet.h>
 #include <goofit/Variable.h>
 
@@ -128,6 +130,35 @@ PYBIND11_MODULE(_goofit, m) {
              })
         .def("__len__", &UnbinnedDataSet::getNumEvents);
 
+    // python/goofit/BinnedDataSet.cpp of the reference
+    py::class_<BinnedDataSet, DataSet>(m, "BinnedDataSet")
+        .def(py::init<const Observable &, std::string>(), py::arg("var"), py::arg("name") = "")
+        .def(py::init<const std::vector<Observable> &, std::string>(), py::arg("vars"), py::arg("name") = "")
+        .def(py::init([](py::args args) {
+            std::vector<Observable> obs;
+            for(auto a : args)
+                obs.push_back(a.cast<Observable>());
+            return new BinnedDataSet(obs);
+        }))
+        .def("addEvent", [](BinnedDataSet &d) { d.addEvent(); })
+        .def("addWeightedEvent", &BinnedDataSet::addWeightedEvent, py::arg("weight"))
+        .def("getBinContent", &BinnedDataSet::getBinContent, py::arg("bin"))
+        .def("setBinContent", &BinnedDataSet::setBinContent, py::arg("bin"), py::arg("value"))
+        .def("getBinError", &BinnedDataSet::getBinError, py::arg("bin"))
+        .def("setBinError", &BinnedDataSet::setBinError, py::arg("bin"), py::arg("value"))
+        .def("getBinCenter", (fptype(BinnedDataSet::*)(size_t, size_t) const) & BinnedDataSet::getBinCenter)
+        .def("getBinCenter", (fptype(BinnedDataSet::*)(const Observable &, size_t) const) & BinnedDataSet::getBinCenter)
+        .def("getBinSize", &BinnedDataSet::getBinSize, py::arg("ivar"))
+        .def("getBinNumber", &BinnedDataSet::getBinNumber)
+        .def("getBinVolume", &BinnedDataSet::getBinVolume, py::arg("bin"))
+        .def("getNumBins", &BinnedDataSet::getNumBins)
+        .def("getNumWeightedEvents", &BinnedDataSet::getNumWeightedEvents)
+        .def("getDiminsions", &BinnedDataSet::getDiminsions)
+        .def_property_readonly("num_bins", &BinnedDataSet::getNumBins)
+        .def_property_readonly("num_events", &BinnedDataSet::getNumEvents)
+        .def_property_readonly("diminsions", &BinnedDataSet::getDiminsions)
+        .def("__len__", &BinnedDataSet::getNumBins);
+
     // ------------------------------------------------------------ PDFs
     py::enum_<EvalFunc>(m, "EvalFunc")
         .value("Eval", EvalFunc::Eval)
@@ -136,6 +167,20 @@ PYBIND11_MODULE(_goofit, m) {
         .value("BinAvg", EvalFunc::BinAvg)
         .value("BinWithError", EvalFunc::BinWithError)
         .value("Chisq", EvalFunc::Chisq);
+
+    // python/goofit/FitControl.cpp of the reference
+    py::class_<FitControl, std::shared_ptr<FitControl>>(m, "FitControl")
+        .def("binnedFit", &FitControl::binnedFit)
+        .def("binErrors", &FitControl::binErrors)
+        .def("metricIsPdf", &FitControl::metricIsPdf)
+        .def("getMetric", &FitControl::getMetric)
+        .def("getName", &FitControl::getName);
+    py::class_<UnbinnedNllFit, FitControl, std::shared_ptr<UnbinnedNllFit>>(m, "UnbinnedNllFit").def(py::init<>());
+    py::class_<BinnedNllFit, FitControl, std::shared_ptr<BinnedNllFit>>(m, "BinnedNllFit").def(py::init<>());
+    py::class_<BinnedErrorFit, FitControl, std::shared_ptr<BinnedErrorFit>>(m, "BinnedErrorFit").def(py::init<>());
+    py::class_<BinnedChisqFit, FitControl, std::shared_ptr<BinnedChisqFit>>(m, "BinnedChisqFit").def(py::init<>());
+    py::class_<ProbFit, FitControl, std::shared_ptr<ProbFit>>(m, "ProbFit").def(py::init<>());
+    py::class_<EvalFit, FitControl, std::shared_ptr<EvalFit>>(m, "EvalFit").def(py::init<>());
 
     py::class_<PdfBase>(m, "PdfBase")
         .def("getName", &PdfBase::getName)
@@ -161,6 +206,7 @@ PYBIND11_MODULE(_goofit, m) {
         .def("__repr__", [](const PdfBase &p) { return repr_of(p); });
 
     py::class_<GooPdf, PdfBase>(m, "GooPdf")
+        .def("setFitControl", &GooPdf::setFitControl, py::arg("fc"))
         .def("integrate", &GooPdf::integrate)
         .def("hasAnalyticIntegral", &GooPdf::hasAnalyticIntegral)
         .def("getValue", &GooPdf::getValue, py::arg("evalfunc") = EvalFunc::Eval)

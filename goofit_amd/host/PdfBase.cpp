@@ -1,5 +1,6 @@
 // PdfBase.cpp -- tree registration, flattening and data upload
 // (src/goofit/PdfBase.cpp and src/PDFs/PdfBase.cu of the reference).
+#include <goofit/BinnedDataSet.h>
 #include <goofit/Error.h>
 #include <goofit/FitControl.h>
 #include <goofit/FitManager.h>
@@ -9,6 +10,7 @@
 
 #include <algorithm>
 #include <iostream>
+#include <memory>
 #include <numeric>
 #include <random>
 
@@ -230,28 +232,49 @@ void PdfBase::setData(DataSet *data) {
     data_ = data;
     if(data == nullptr)
         return;
-    auto *unbinned = dynamic_cast<UnbinnedDataSet *>(data);
-    if(!unbinned)
-        throw GeneralError("Dataset must be binned or unbinned!");
-    numEntries = (unsigned int)data->getNumEvents();
-    numEvents  = numEntries;
     const size_t dims = observablesList.size();
-    if(dims > 8)
-        throw GeneralError("{}: {} observables (the engine handles at most 8)", getName(), dims);
-    std::vector<const double *> cols(dims, nullptr);
-    std::vector<std::vector<double>> owned;
-    for(const Observable &v : observablesList) {
-        const std::vector<fptype> &col = unbinned->getColumn(v);
-        cols.at(v.getIndex())          = col.data();
-    }
+    if(dims + 2 > 8)
+        throw GeneralError("{}: {} observables (the engine handles at most 6 with bins, 8 without)", getName(), dims);
     gf_ctx *c = engine::ctx();
-    engine::check(gf_events_upload(c, (int)dims, (int64_t)numEntries, cols.data()), "gf_events_upload");
-    for(const Observable &v : observablesList)
-        if(v.isEventNumber() && numEntries > 0) {
-            if(gf_events_set_identity_column(c, v.getIndex()) != GF_OK) {
-                // the event numbers are not the row index: the column is read as is
-            }
+    if(auto *unbinned = dynamic_cast<UnbinnedDataSet *>(data)) {
+        numEntries = (unsigned int)data->getNumEvents();
+        numEvents  = numEntries;
+        std::vector<const double *> cols(dims, nullptr);
+        for(const Observable &v : observablesList) {
+            const std::vector<fptype> &col = unbinned->getColumn(v);
+            cols.at(v.getIndex())          = col.data();
         }
+        engine::check(gf_events_upload(c, (int)dims, (int64_t)numEntries, cols.data()), "gf_events_upload");
+        for(const Observable &v : observablesList)
+            if(v.isEventNumber() && numEntries > 0) {
+                if(gf_events_set_identity_column(c, v.getIndex()) != GF_OK) {
+                    // the event numbers are not the row index: the column is read as is
+                }
+            }
+        return;
+    }
+    auto *binned = dynamic_cast<BinnedDataSet *>(data);
+    if(!binned)
+        throw GeneralError("Dataset must be binned or unbinned!");
+    // PdfBase.cu:439-527: one row per bin: the bin centre of each observable at its
+    // index, then the content, then the volume (or the error for BinnedErrorFit);
+    // c_totalEvents is the number of bins.
+    numEvents  = 0;
+    numEntries = (unsigned int)binned->getNumBins();
+    if(!fitControl || !fitControl->binnedFit())
+        setFitControl(std::make_shared<BinnedNllFit>());
+    std::vector<std::vector<double>> cols(dims + 2, std::vector<double>(numEntries));
+    for(unsigned int i = 0; i < numEntries; ++i) {
+        for(const Observable &v : observablesList)
+            cols[v.getIndex()][i] = binned->getBinCenter(v, i);
+        cols[dims][i]     = binned->getBinContent(i);
+        cols[dims + 1][i] = fitControl->binErrors() ? binned->getBinError(i) : binned->getBinVolume(i);
+        numEvents += binned->getBinContent(i);
+    }
+    std::vector<const double *> ptrs;
+    for(auto &col : cols)
+        ptrs.push_back(col.data());
+    engine::check(gf_events_upload(c, (int)(dims + 2), (int64_t)numEntries, ptrs.data()), "gf_events_upload");
 }
 
 void PdfBase::fillMCDataSimple(size_t events, unsigned int seed) {

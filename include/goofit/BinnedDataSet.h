@@ -1,0 +1,56 @@
+// Binned data set of the GooFit API (include/goofit/BinnedDataSet.h of the
+// reference): a histogram over the observables' bins, global bin number with
+// the first observable fastest.
+#pragma once
+
+#include <initializer_list>
+#include <set>
+#include <string>
+#include <vector>
+
+#include <goofit/DataSet.h>
+
+namespace GooFit {
+
+class BinnedDataSet : public DataSet {
+  public:
+    using DataSet::addEvent;
+
+    BinnedDataSet(const Observable &var, std::string n = "");
+    BinnedDataSet(const std::vector<Observable> &vars, std::string n = "");
+    BinnedDataSet(const std::set<Observable> &vars, std::string n = "");
+    BinnedDataSet(std::initializer_list<Observable> vars, std::string n = "");
+    ~BinnedDataSet() override = default;
+
+    /// One entry in the bin of the observables' current values.
+    void addEvent() override;
+    void addWeightedEvent(double weight) override;
+
+    fptype getBinContent(size_t bin) const { return binvalues.at(bin); }
+    fptype getBinCenter(size_t ivar, size_t bin) const;
+    fptype getBinCenter(const Observable &var, size_t bin) const;
+    fptype getBinSize(size_t ivar) const;
+    /// Bin of the observables' current values.
+    size_t getBinNumber() const;
+    fptype getBinVolume(size_t bin) const;
+    /// sqrt(content) unless errors were set.
+    fptype getBinError(size_t bin) const;
+    std::vector<size_t> getDiminsions() const { return binsizes; }
+    size_t getNumBins() const;
+    fptype getNumWeightedEvents() const;
+
+    void setBinContent(unsigned int bin, fptype value) { binvalues.at(bin) = value; }
+    void setBinError(unsigned int bin, fptype value);
+
+  private:
+    std::vector<size_t> convertValuesToBins(const std::vector<fptype> &vals) const;
+    size_t localToGlobal(const std::vector<size_t> &locals) const;
+    std::vector<size_t> globalToLocal(size_t global) const;
+    void collectBins();
+
+    std::vector<size_t> binsizes;
+    std::vector<fptype> binvalues;
+    std::vector<fptype> binerrors;
+};
+
+}  // namespace GooFit

@@ -206,3 +206,65 @@ def test_dalitz_mc_normalisation_and_learned_proposal():
     pdf.setNormalisationGrid()
     nll_g = pdf.calculateNLL()
     assert abs((nll_q - nll_g) - len(ev12) * math.log(got_q / ref)) < 1e-6 * abs(nll_g)
+
+
+def _exp_sample(rate, n, hi, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.exponential(1 / rate, n)
+    return v[v < hi]
+
+
+def test_binned_exp_fit():
+    """BinningTest.cpp:16-59 (binned section): 1e5 events of exp(-1.5 x) in 100 bins of [0, 10]."""
+    N = 100_000
+    x = G.Observable("xvar", 0, 10)
+    d = G.BinnedDataSet(x)
+    for v in _exp_sample(1.5, N, 10, 137):
+        x.value = v
+        d.addEvent()
+    alpha = G.Variable("alpha", -2, 0.1, -10, 10)
+    pdf = G.ExpPdf("exppdf", x, alpha)
+    pdf.setData(d)
+    fm = G.FitManager(pdf)
+    fm.setVerbosity(0)
+    r = fm.fit()
+    assert r.IsValid()
+    err = 1 / math.sqrt(N) * abs(alpha.value)
+    assert alpha.error < 3 * err
+    assert abs(alpha.value + 1.5) <= 3 / math.sqrt(N) * 1.5
+
+
+def test_binned_nll_parity_and_metrics():
+    """Binned NLL (BinAvg), Chisq and BinWithError against the oracle on the same bins
+    (c_totalEvents = number of bins, PdfBase.cu:527)."""
+    x = G.Observable("x", -5, 5)
+    x.numbins = 60
+    mu, sig = G.Variable("mu", 0.2, 0.1, -1, 1), G.Variable("sig", 1.1, 0.1, 0.1, 3)
+    pdf = G.GaussianPdf("g", x, mu, sig)
+    d = G.BinnedDataSet(x)
+    rng = np.random.default_rng(5)
+    for v in rng.normal(0.0, 1.0, 20_000):
+        if -5 < v < 5:
+            x.value = v
+            d.addEvent()
+    for fc, metric, with_err in ((None, "BINAVG", False), (G.BinnedChisqFit(), "CHISQ", False),
+                                 (G.BinnedErrorFit(), "BINWITHERROR", True)):
+        if fc is not None:
+            pdf.setFitControl(fc)
+        pdf.setData(d)
+        got = pdf.calculateNLL()
+        centres = np.array([d.getBinCenter(0, i) for i in range(60)])
+        cont = np.array([d.getBinContent(i) for i in range(60)])
+        last = np.array([d.getBinError(i) if with_err else d.getBinVolume(i) for i in range(60)])
+        xo = O.Obs("x", -5.0, 5.0, 60)
+        g = O.Gaussian("g", xo, O.Var("mu", 0.2), O.Var("sig", 1.1))
+        model = O.Model(g)
+        if not with_err:
+            model.normalize()
+        else:
+            g.cached_norm = 1.0  # BinWithError: the metric is not a probability
+        model.op.sync_params()
+        model.op.sync_norms()
+        model.op.set_total_events(60)
+        ref = model.op.reduce_metric(np.stack([centres, cont, last], 1), metric)
+        assert math.isclose(got, ref, rel_tol=1e-11), (metric, got, ref)

@@ -132,3 +132,34 @@ def test_device_path_fails_loudly_without_gpu():
     with pytest.raises(Exception):
         m.pdf.setData(d)
         m.pdf.calculateNLL()
+
+
+def test_binned_dataset_numbering():
+    """BinnedDataSet.cpp:37-160: floor((clamped value - lower) / size), first observable fastest,
+    centres size * (local + 0.5) + lower, errors sqrt(content) unless set."""
+    x = G.Observable("x", 0, 10)
+    y = G.Observable("y", -1, 1)
+    x.numbins, y.numbins = 10, 4
+    d = G.BinnedDataSet(x, y)
+    assert d.getNumBins() == 40 and d.getDiminsions() == [10, 4]
+    for xv, yv, want in ((0.0, -1.0, 0), (9.999, -1.0, 9), (0.5, -0.49, 10), (3.2, 0.7, 3 + 3 * 10)):
+        x.value, y.value = xv, yv
+        assert d.getBinNumber() == want
+        d.addEvent()
+    x.value, y.value = 3.3, 0.9
+    d.addWeightedEvent(2.5)
+    assert d.getBinContent(33) == 3.5 and d.getNumEvents() == 5
+    assert d.getNumWeightedEvents() == 6  # accumulated into an integer, as the reference does
+    assert catch_approx(d.getBinCenter(0, 33), 3.5) and catch_approx(d.getBinCenter(1, 33), 0.75)
+    assert d.getBinVolume(7) == pytest.approx(1.0 * 0.5)
+    assert d.getBinError(33) == pytest.approx(3.5 ** 0.5)
+    d.setBinError(33, 0.25)
+    assert d.getBinError(33) == 0.25
+    # hazard A14: a value equal to the upper limit gets local bin n.  In 2-D that is
+    # another bin's number (10 + 2 * 10 = 30, i.e. x-bin 0 of the next y row), as in
+    # the reference; in 1-D it is outside the histogram and addEvent throws.
+    x.value, y.value = 10.0, 0.0
+    assert d.getBinNumber() == 30
+    d1 = G.BinnedDataSet(x)
+    with pytest.raises(IndexError):
+        d1.addEvent()
