@@ -347,6 +347,15 @@ PYBIND11_MODULE(_goofit, m) {
     // ------------------------------------------------------------ fitting
     py::class_<M2::FunctionMinimum>(m, "FunctionMinimum")
         .def("IsValid", &M2::FunctionMinimum::IsValid)
+        .def("UserState", &M2::FunctionMinimum::UserState, py::return_value_policy::reference_internal)
+        .def("UserParameters", &M2::FunctionMinimum::UserParameters, py::return_value_policy::reference_internal)
+        .def("UserCovariance", &M2::FunctionMinimum::UserCovariance)
+        .def("Up", &M2::FunctionMinimum::Up)
+        .def("HasValidParameters", &M2::FunctionMinimum::HasValidParameters)
+        .def("HasValidCovariance", &M2::FunctionMinimum::HasValidCovariance)
+        .def("HasPosDefCovar", &M2::FunctionMinimum::HasPosDefCovar)
+        .def("HasMadePosDefCovar", &M2::FunctionMinimum::HasMadePosDefCovar)
+        .def("HesseFailed", &M2::FunctionMinimum::HesseFailed)
         .def("HasCovariance", &M2::FunctionMinimum::HasCovariance)
         .def("HasAccurateCovar", &M2::FunctionMinimum::HasAccurateCovar)
         .def("IsAboveMaxEdm", &M2::FunctionMinimum::IsAboveMaxEdm)
@@ -421,6 +430,7 @@ PYBIND11_MODULE(_goofit, m) {
         .def_property_readonly("fcn", &FitManagerMinuit2::getFCN, py::return_value_policy::reference_internal)
         .def("__nonzero__", [](const FitManagerMinuit2 &f) { return (bool)f; })
         .def("getLastFitTime", &FitManagerMinuit2::getLastFitTime)
+        .def("getMnScan", &FitManagerMinuit2::getMnScan, py::keep_alive<0, 1>())
         .def("minuit_vector", [](FitManagerMinuit2 &f) { return f.getParams()->make_minuit_vector(); })
         .def("free_mask",
              [](FitManagerMinuit2 &f) {
@@ -433,6 +443,102 @@ PYBIND11_MODULE(_goofit, m) {
              "One FCN call: push the Minuit vector into the PDF and return its NLL")
         .def("__bool__", [](const FitManagerMinuit2 &f) { return (bool)f; })
         .def("__int__", [](const FitManagerMinuit2 &f) { return (int)f; });
+
+    // ------------------------------------------------------------ Minuit2 (python/Minuit2/*.cpp of the reference)
+    {
+        auto mm = m.def_submodule("Minuit2", "The Minuit2 API used by GooFit, on this build's minimiser");
+        struct PyFCN : M2::FCNBase {
+            using M2::FCNBase::FCNBase;
+            double operator()(const std::vector<double> &v) const override {
+                PYBIND11_OVERRIDE_PURE_NAME(double, M2::FCNBase, "__call__", operator(), v);
+            }
+            double Up() const override { PYBIND11_OVERRIDE_PURE(double, M2::FCNBase, Up, ); }
+            double ErrorDef() const override { PYBIND11_OVERRIDE(double, M2::FCNBase, ErrorDef, ); }
+        };
+        py::class_<M2::FCNBase, PyFCN>(mm, "FCNBase")
+            .def(py::init<>())
+            .def("__call__", &M2::FCNBase::operator())
+            .def("ErrorDef", &M2::FCNBase::ErrorDef)
+            .def("Up", &M2::FCNBase::Up);
+        py::class_<M2::MinuitParameter>(mm, "MinuitParameter")
+            .def("Name", [](const M2::MinuitParameter &p) { return p.name; })
+            .def("GetName", [](const M2::MinuitParameter &p) { return p.name; })
+            .def("Value", [](const M2::MinuitParameter &p) { return p.value; })
+            .def("Error", [](const M2::MinuitParameter &p) { return p.error; })
+            .def("IsFixed", [](const M2::MinuitParameter &p) { return p.fixed; })
+            .def("IsConst", [](const M2::MinuitParameter &p) { return p.fixed && p.error == 0; })
+            .def("HasLimits", [](const M2::MinuitParameter &p) { return p.has_lower && p.has_upper; })
+            .def("HasLowerLimit", [](const M2::MinuitParameter &p) { return p.has_lower; })
+            .def("HasUpperLimit", [](const M2::MinuitParameter &p) { return p.has_upper; })
+            .def("LowerLimit", [](const M2::MinuitParameter &p) { return p.lower; })
+            .def("UpperLimit", [](const M2::MinuitParameter &p) { return p.upper; });
+        py::class_<M2::MnUserParameters>(mm, "MnUserParameters")
+            .def(py::init<>())
+            .def("Add", (bool(M2::MnUserParameters::*)(const std::string &, double)) & M2::MnUserParameters::Add)
+            .def("Add",
+                 (bool(M2::MnUserParameters::*)(const std::string &, double, double)) & M2::MnUserParameters::Add)
+            .def("Add", (bool(M2::MnUserParameters::*)(const std::string &, double, double, double, double))
+                            & M2::MnUserParameters::Add)
+            .def("Parameter", [](const M2::MnUserParameters &u, unsigned int i) { return u.Parameters().at(i); })
+            .def("Parameters", &M2::MnUserParameters::Parameters)
+            .def("Params", &M2::MnUserParameters::Params)
+            .def("Errors", &M2::MnUserParameters::Errors)
+            .def("Value", &M2::MnUserParameters::Value)
+            .def("Error", &M2::MnUserParameters::Error)
+            .def("SetValue", &M2::MnUserParameters::SetValue)
+            .def("SetError", &M2::MnUserParameters::SetError)
+            .def("SetLimits", &M2::MnUserParameters::SetLimits)
+            .def("SetLowerLimit", &M2::MnUserParameters::SetLowerLimit)
+            .def("SetUpperLimit", &M2::MnUserParameters::SetUpperLimit)
+            .def("RemoveLimits", &M2::MnUserParameters::RemoveLimits)
+            .def("Fix", &M2::MnUserParameters::Fix)
+            .def("Release", &M2::MnUserParameters::Release)
+            .def("Index", &M2::MnUserParameters::Index)
+            .def("GetName", &M2::MnUserParameters::GetName)
+            .def("NParameters", &M2::MnUserParameters::NParameters)
+            .def("VariableParameters", &M2::MnUserParameters::VariableParameters);
+        py::class_<M2::MnUserParameterState>(mm, "MnUserParameterState")
+            .def("Value", (double(M2::MnUserParameterState::*)(unsigned int) const) & M2::MnUserParameterState::Value,
+                 py::arg("n"))
+            .def("Error", (double(M2::MnUserParameterState::*)(unsigned int) const) & M2::MnUserParameterState::Error,
+                 py::arg("n"))
+            .def("Covariance", &M2::MnUserParameterState::Covariance)
+            .def("Fval", &M2::MnUserParameterState::Fval)
+            .def("Edm", &M2::MnUserParameterState::Edm)
+            .def("NFcn", &M2::MnUserParameterState::NFcn)
+            .def("IsValid", &M2::MnUserParameterState::IsValid);
+        py::class_<M2::MnUserCovariance>(mm, "MnUserCovariance")
+            .def("Nrow", &M2::MnUserCovariance::Nrow)
+            .def("size", &M2::MnUserCovariance::size)
+            .def("__call__", &M2::MnUserCovariance::operator())
+            .def("to_matrix", [](const M2::MnUserCovariance &c) {
+                py::array_t<double> a({(py::ssize_t)c.Nrow(), (py::ssize_t)c.Nrow()});
+                std::copy(c.Data().begin(), c.Data().end(), a.mutable_data());
+                return a;
+            });
+        py::class_<M2::MnMigrad>(mm, "MnMigrad")
+            .def(py::init([](const M2::FCNBase &fcn, const M2::MnUserParameters &par, unsigned int stra) {
+                     return new M2::MnMigrad(fcn, par, stra);
+                 }),
+                 py::arg("fcn"), py::arg("par"), py::arg("stra") = 1, py::keep_alive<1, 2>())
+            .def("__call__", &M2::MnMigrad::operator(), py::arg("maxfcn") = 0, py::arg("tolerance") = 0.1);
+        py::class_<M2::MnMinos>(mm, "MnMinos")
+            .def(py::init<const M2::FCNBase &, const M2::FunctionMinimum &, unsigned int>(), py::arg("fcn"),
+                 py::arg("min"), py::arg("stra") = 1, py::keep_alive<1, 2>(), py::keep_alive<1, 3>())
+            .def("__call__", &M2::MnMinos::operator(), py::arg("par"), py::arg("maxcalls") = 0,
+                 "(lower, upper) error of parameter par")
+            .def("Lower", &M2::MnMinos::Lower, py::arg("par"), py::arg("maxcalls") = 0)
+            .def("Upper", &M2::MnMinos::Upper, py::arg("par"), py::arg("maxcalls") = 0);
+        py::class_<M2::MnScan>(mm, "MnScan")
+            .def(py::init<const M2::FCNBase &, const M2::MnUserParameters &, unsigned int>(), py::arg("fcn"),
+                 py::arg("par"), py::arg("stra") = 1, py::keep_alive<1, 2>())
+            .def("Scan", &M2::MnScan::Scan, "Scan parameter.", py::arg("par"), py::arg("maxsteps") = 41,
+                 py::arg("low") = 0.0, py::arg("high") = 0.0);
+        mm.attr("FunctionMinimum") = m.attr("FunctionMinimum");
+        auto mp = mm.def_submodule("MnPrint");
+        mp.def("GlobalLevel", &M2::MnPrint::GlobalLevel);
+        mp.def("SetGlobalLevel", &M2::MnPrint::SetGlobalLevel);
+    }
 
     // ------------------------------------------------------------ engine state
     auto me = m.def_submodule("engine");

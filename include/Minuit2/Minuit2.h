@@ -111,6 +111,23 @@ class MnStrategy {
     unsigned int level_;
 };
 
+/// Covariance of the variable (non-fixed) parameters, row-major (Minuit2 MnUserCovariance).
+class MnUserCovariance {
+  public:
+    MnUserCovariance() = default;
+    MnUserCovariance(unsigned int n, std::vector<double> data)
+        : n_(n)
+        , data_(std::move(data)) {}
+    unsigned int Nrow() const { return n_; }
+    unsigned int size() const { return (unsigned int)data_.size(); }
+    double operator()(unsigned int i, unsigned int j) const { return data_.at((size_t)i * n_ + j); }
+    const std::vector<double> &Data() const { return data_; }
+
+  private:
+    unsigned int n_ = 0;
+    std::vector<double> data_;
+};
+
 /// Result of a minimisation (Minuit2 FunctionMinimum).
 class FunctionMinimum {
   public:
@@ -130,6 +147,19 @@ class FunctionMinimum {
     double Up() const { return up_; }
     const MnUserParameterState &UserState() const { return state_; }
     const MnUserParameters &UserParameters() const { return state_.pars_; }
+    /// Covariance over the variable parameters (in their order of appearance).
+    MnUserCovariance UserCovariance() const {
+        std::vector<unsigned int> var;
+        for(unsigned int i = 0; i < state_.pars_.NParameters(); i++)
+            if(!state_.pars_.Parameters()[i].fixed)
+                var.push_back(i);
+        std::vector<double> c(var.size() * var.size(), 0.0);
+        if(state_.HasCovariance())
+            for(size_t a = 0; a < var.size(); a++)
+                for(size_t b = 0; b < var.size(); b++)
+                    c[a * var.size() + b] = state_.Covariance(var[a], var[b]);
+        return MnUserCovariance((unsigned int)var.size(), c);
+    }
 
     // filled by the minimiser
     MnUserParameterState state_;
