@@ -74,6 +74,14 @@ PYBIND11_MODULE(_goofit, m) {
         .def("getNumBins", &Observable::getNumBins)
         .def("setNumBins", &Observable::setNumBins)
         .def("getBinSize", &Observable::getBinSize)
+        .def_property_readonly("bin_size", &Observable::getBinSize)
+        .def_property_readonly("bin_centers",
+                               [](const Observable &o) {
+                                   std::vector<double> c(o.getNumBins());
+                                   for(size_t i = 0; i < c.size(); i++)
+                                       c[i] = o.getLowerLimit() + (i + 0.5) * o.getBinSize();
+                                   return c;
+                               })
         .def("isEventNumber", &Observable::isEventNumber)
         .def("__repr__", [](const Observable &o) { return repr_of(o); });
 
@@ -91,6 +99,8 @@ PYBIND11_MODULE(_goofit, m) {
         .def_property("error", &Variable::getError, &Variable::setError)
         .def_property("fixed", &Variable::IsFixed, &Variable::setFixed)
         .def_property("fitterindex", &Variable::getFitterIndex, &Variable::setFitterIndex)
+        .def_property_readonly("fitterIndex", &Variable::getFitterIndex)
+        .def_property("blind", nullptr, &Variable::setBlind)
         .def("getError", &Variable::getError)
         .def("setBlind", &Variable::setBlind)
         .def("__repr__", [](const Variable &v) { return repr_of(v); });
@@ -100,7 +110,10 @@ PYBIND11_MODULE(_goofit, m) {
         .def("getNumEvents", &DataSet::getNumEvents)
         .def("getObservables", &DataSet::getObservables)
         .def("addEvent", [](DataSet &d) { d.addEvent(); })
-        .def("getName", &DataSet::getName);
+        .def("getName", &DataSet::getName)
+        .def_property_readonly("name", &DataSet::getName)
+        .def_property_readonly("variables", &DataSet::getObservables)
+        .def("__len__", &DataSet::getNumEvents);
 
     py::class_<UnbinnedDataSet, DataSet>(m, "UnbinnedDataSet")
         .def(py::init<const Observable &, std::string>(), py::arg("var"), py::arg("name") = "")
@@ -128,7 +141,76 @@ PYBIND11_MODULE(_goofit, m) {
                  }
                  d.appendColumns(ptrs, n);
              })
-        .def("__len__", &UnbinnedDataSet::getNumEvents);
+        .def("__len__", &UnbinnedDataSet::getNumEvents)
+        // python/goofit/UnbinnedDataSet.cpp of the reference: observables x events
+        .def("to_numpy",
+             [](const UnbinnedDataSet &d) {
+                 const auto &obs = d.getObservables();
+                 py::array_t<double> a({(py::ssize_t)obs.size(), (py::ssize_t)d.getNumEvents()});
+                 for(size_t i = 0; i < obs.size(); i++) {
+                     const auto &col = d.getColumn(obs[i]);
+                     std::copy(col.begin(), col.end(), a.mutable_data() + i * d.getNumEvents());
+                 }
+                 return a;
+             })
+        .def("to_matrix",
+             [](const UnbinnedDataSet &d) {
+                 const auto &obs = d.getObservables();
+                 py::array_t<double> a({(py::ssize_t)obs.size(), (py::ssize_t)d.getNumEvents()});
+                 for(size_t i = 0; i < obs.size(); i++) {
+                     const auto &col = d.getColumn(obs[i]);
+                     std::copy(col.begin(), col.end(), a.mutable_data() + i * d.getNumEvents());
+                 }
+                 return a;
+             })
+        .def(
+            "from_matrix",
+            [](UnbinnedDataSet &d, py::array_t<double, py::array::c_style | py::array::forcecast> m, bool filter) {
+                // rows = observables; one row fewer: the last observable counts the events
+                auto obs          = d.getObservables();
+                const size_t rows = (size_t)m.shape(0), n = (size_t)m.shape(1);
+                if(rows != obs.size() && rows + 1 != obs.size())
+                    throw GeneralError("from_matrix: {} rows for {} observables", rows, obs.size());
+                const size_t first = d.getNumEvents();
+                for(size_t j = 0; j < n; j++) {
+                    for(size_t i = 0; i < rows; i++)
+                        obs[i].setValue(m.at(i, j));
+                    if(rows + 1 == obs.size())
+                        obs.back().setValue((double)(first + j));
+                    bool ok = true;
+                    if(filter)
+                        for(const auto &o : obs)
+                            ok = ok && (bool)o;
+                    if(ok)
+                        d.addEvent();
+                }
+            },
+            py::arg("matrix"), py::arg("filter") = false)
+        .def(
+            "from_numpy",
+            [](UnbinnedDataSet &d, py::array_t<double, py::array::c_style | py::array::forcecast> m, bool filter) {
+                auto obs = d.getObservables();
+                for(py::ssize_t j = 0; j < m.shape(1); j++) {
+                    for(py::ssize_t i = 0; i < m.shape(0); i++)
+                        obs.at(i).setValue(m.at(i, j));
+                    bool ok = true;
+                    if(filter)
+                        for(const auto &o : obs)
+                            ok = ok && (bool)o;
+                    if(ok)
+                        d.addEvent();
+                }
+            },
+            py::arg("array"), py::arg("filter") = false)
+        .def("__getitem__", [](const UnbinnedDataSet &d, py::object idx) {
+            const auto &obs = d.getObservables();
+            py::array_t<double> a({(py::ssize_t)obs.size(), (py::ssize_t)d.getNumEvents()});
+            for(size_t i = 0; i < obs.size(); i++) {
+                const auto &col = d.getColumn(obs[i]);
+                std::copy(col.begin(), col.end(), a.mutable_data() + i * d.getNumEvents());
+            }
+            return a.attr("__getitem__")(idx);
+        });
 
     // python/goofit/BinnedDataSet.cpp of the reference
     py::class_<BinnedDataSet, DataSet>(m, "BinnedDataSet")

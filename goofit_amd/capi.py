@@ -34,7 +34,7 @@ EXPORTS = [
     "gf_time_reduce_metric", "gf_time_reduce_sweep", "gf_time_dalitz_cache", "gf_time_dalitz_gram",
     "gf_sample_uniform", "gf_mlp_param_count", "gf_mlp_create", "gf_mlp_destroy", "gf_mlp_get_params",
     "gf_mlp_set_params", "gf_mlp_forward", "gf_mlp_gradient", "gf_mlp_train", "gf_mlp_build_proposal",
-    "gf_mlp_sample", "gf_mlp_train_host", "gf_dalitz_set_norm_points", "gf_dalitz_gram_norm_points",
+    "gf_mlp_sample", "gf_mlp_train_host", "gf_mlp_use_graph", "gf_dalitz_set_norm_points", "gf_dalitz_gram_norm_points",
     "gf_dalitz_generate", "gf_batch_create", "gf_batch_destroy", "gf_batch_set_toy", "gf_batch_reduce",
     "gf_batch_set_chunk", "gf_batch_get_chunk", "gf_batch_use_graph", "gf_batch_autotune",
 ]
@@ -104,6 +104,7 @@ def lib():
                                    ctypes.c_uint64, _F, _F]
         L.gf_mlp_build_proposal.argtypes = [c_p, _D, ctypes.c_int, ctypes.c_double]
         L.gf_mlp_sample.argtypes = [c_p, ctypes.c_int64, ctypes.c_uint64, c_p, c_p, c_p]
+        L.gf_mlp_use_graph.argtypes = [c_p, ctypes.c_int]
         L.gf_mlp_train_host.argtypes = [c_p, _D, _D, ctypes.c_int64, _D, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                         ctypes.c_uint64, _F, _F]
         L.gf_dalitz_set_norm_points.argtypes = [c_p, ctypes.c_int, c_p, ctypes.c_int64, ctypes.c_uint64, _D]
@@ -381,6 +382,9 @@ class Proposal:
         _check(lib().gf_mlp_train(self.h, dev_m12, dev_m13, n, _dp(b), steps, batch, lr, seed, ctypes.byref(l0),
                                   ctypes.byref(l1)))
         return l0.value, l1.value
+
+    def use_graph(self, on):
+        _check(lib().gf_mlp_use_graph(self.h, 1 if on else 0))
 
     def train_host(self, m12, m13, box, steps, batch, lr, seed):
         a, b = _f64(m12), _f64(m13)

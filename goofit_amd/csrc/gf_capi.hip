@@ -728,6 +728,30 @@ bool use_fast_dalitz(const gf_ctx *ctx, int metric) {
 // The fast path's efficiency polynomial has maximum degree 0.
 bool const_efficiency(const gf_ctx *ctx) { return (int)ctx->consts[ctx->ins[1].c] == 0; }
 
+// Efficiency handling of the point kernels (Gram, generator): a MultiPolynomial
+// of maximum degree 0 is its single coefficient (device_MultiPolynomial,
+// PolynomialPdf.cu:47-118, with numBoxes = 1), passed as a constant (mode -2);
+// any other efficiency runs as a compiled subprogram (pc >= 0); none: -1.
+gf_status efficiency_mode(gf_ctx *ctx, int eff_func, int &pc, double &value) {
+    pc    = -1;
+    value = 1.0;
+    if(eff_func < 0)
+        return GF_OK;
+    if(ctx->funcs.at(eff_func) == GF_FN_MULTI_POLYNOMIAL) {
+        Compiler C{ctx, {}, {}, -1};
+        Cur q;
+        if(cursor_at(C, eff_func, q) && C.nc(q) >= 1 && (int)ctx->consts[q.c + 1] == 0) {
+            const int nobs = C.no(q);
+            if(C.np(q) >= nobs + 1) {
+                pc    = -2;
+                value = ctx->params[q.p + 1 + nobs];
+                return GF_OK;
+            }
+        }
+    }
+    return subprogram(ctx, eff_func, pc);
+}
+
 int fast_blocks(const gf_ctx *ctx, int64_t n) {
     int64_t nb  = (n + 255) / 256;
     int64_t cap = (int64_t)ctx->cu_count * ctx->tune[GF_TUNE_FAST_BLOCKS_PER_CU];
@@ -851,9 +875,10 @@ gf_status run_gram(gf_ctx *ctx, const DalitzNode &d, int slot, int mode, const d
     gf_status st = ensure_layout(ctx);
     if(st)
         return st;
-    int eff_pc = -1;
+    int eff_pc      = -1;
+    double eff_cval = 1.0;
     if(mode != 2 && eff_func >= 0) {
-        st = subprogram(ctx, eff_func, eff_pc);
+        st = efficiency_mode(ctx, eff_func, eff_pc, eff_cval);
         if(st)
             return st;
     }
@@ -870,7 +895,7 @@ gf_status run_gram(gf_ctx *ctx, const DalitzNode &d, int slot, int mode, const d
     if(st)
         return st;
     GfProgArgs a = prog_args(ctx);
-    int rc = gfk::launch_gram(a, mode, grid6, m12, m13, w, np, slot, d.col_m12, d.col_m13, eff_pc,
+    int rc = gfk::launch_gram(a, mode, grid6, m12, m13, w, np, slot, d.col_m12, d.col_m13, eff_pc, eff_cval,
                               ctx->d_res + slot * GF_MAX_RES, d.nres, ctx->d_gram_partials, ctx->d_gram_tiles,
                               (int)nb, ctx->stream);
     if(rc)
@@ -1648,7 +1673,8 @@ gf_status gf_time_dalitz_gram(gf_ctx *ctx, int dalitz_func, const double grid6[6
 struct gf_mlp {
     gf_ctx *ctx = nullptr;
     GfMlpDev D{};
-    int adam_t = 0;
+    int adam_t     = 0;
+    bool use_graph = true;  // replay one captured training step
     // tabulated proposal
     double *d_cdf = nullptr, *d_pcell = nullptr;
     int G         = 0;
@@ -1660,7 +1686,7 @@ namespace {
 void mlp_free(gf_mlp *m) {
     void *p[] = {m->D.P,     m->D.m,       m->D.v,       m->D.grad,     m->D.w2b,     m->D.w2t,   m->D.xy,
                  m->D.z1,    m->D.z2,      m->D.h1,      m->D.delta2,   m->D.out,     m->D.coef,  m->D.loss,
-                 m->D.part_in, m->D.part_w2, m->D.part_out, m->d_cdf, m->d_pcell};
+                 m->D.part_in, m->D.part_w2, m->D.part_out, m->d_cdf, m->d_pcell, m->D.state};
     for(void *q : p)
         if(q)
             (void)hipFree(q);
@@ -1765,7 +1791,8 @@ gf_status gf_mlp_create(gf_ctx *ctx, uint64_t seed, int batch_cap, gf_mlp **out)
               && hipMalloc((void **)&D.loss, 64) == hipSuccess
               && hipMalloc((void **)&D.part_in, nb * GF_MLP_PART_IN * 4) == hipSuccess
               && hipMalloc((void **)&D.part_w2, (size_t)GF_MLP_W2_BLOCKS * H * H * 4) == hipSuccess
-              && hipMalloc((void **)&D.part_out, nb * GF_MLP_PART_OUT * 4) == hipSuccess;
+              && hipMalloc((void **)&D.part_out, nb * GF_MLP_PART_OUT * 4) == hipSuccess
+              && hipMalloc((void **)&D.state, sizeof(GfTrainState)) == hipSuccess;
     if(!ok) {
         mlp_free(m);
         delete m;
@@ -1885,28 +1912,64 @@ gf_status gf_mlp_train(gf_mlp *m, const double *dev_m12, const double *dev_m13, 
         return GF_ERR_INVALID;
     hipStream_t s = m->ctx->stream;
     const GfBox B{box[0], box[1], box[2], box[3]};
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    const int half = batch / 2;
-    for(int k = 0; k < steps; k++) {
-        const uint64_t key = seed * 0x9E3779B97F4A7C15ull + 2ull * (uint64_t)k + 1ull;
-        const double cosf_ = 0.5 * (1.0 + std::cos(M_PI * (double)k / (double)steps));
-        const float lr_k   = (float)(lr * (0.05 + 0.95 * cosf_));
-        m->adam_t++;
-        const float c1 = 1.f - std::pow(b1, (float)m->adam_t), c2 = 1.f - std::pow(b2, (float)m->adam_t);
-        if(gfm::launch_batch_train(dev_m12, dev_m13, ndata, B, half, half, key, m->D.xy, s)
-           || gfm::launch_forward(m->D, m->D.xy, batch, m->D.out, s)
-           || gfm::launch_loss(m->D.out, half, half, m->D.coef, m->D.loss, s)
-           || gfm::launch_backward(m->D, m->D.xy, m->D.coef, batch, m->D.grad, s)
-           || gfm::launch_adam(m->D, m->D.grad, lr_k, b1, b2, eps, c1, c2, s))
-            return fail(GF_ERR_HIP, "training step %d launch failed", k);
-        if(k == 0 && loss_first) {
-            GF_HIP(hipMemcpyAsync(loss_first, m->D.loss, 4, hipMemcpyDeviceToHost, s));
-            GF_HIP(hipStreamSynchronize(s));
-        }
+    GfTrainState st{};
+    st.seed   = seed;
+    st.k      = 0;
+    st.steps  = steps;
+    st.adam_t = m->adam_t;
+    st.lr0    = lr;
+    st.b1     = 0.9f;
+    st.b2     = 0.999f;
+    st.eps    = 1e-8f;
+    GF_HIP(hipMemcpyAsync(m->D.state, &st, sizeof(st), hipMemcpyHostToDevice, s));
+    // step 0 eagerly (its loss is the first loss)
+    if(gfm::launch_train_step(m->D, dev_m12, dev_m13, ndata, B, batch, s))
+        return fail(GF_ERR_HIP, "training step launch failed");
+    if(loss_first) {
+        GF_HIP(hipMemcpyAsync(loss_first, m->D.loss, 4, hipMemcpyDeviceToHost, s));
+        GF_HIP(hipStreamSynchronize(s));
     }
+    if(steps > 1 && m->use_graph) {
+        // one captured step replayed: all per-step values come from the device state
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        GF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc   = gfm::launch_train_step(m->D, dev_m12, dev_m13, ndata, B, batch, s);
+        hipError_t e   = hipStreamEndCapture(s, &g);
+        if(rc || e != hipSuccess) {
+            if(g)
+                (void)hipGraphDestroy(g);
+            return fail(GF_ERR_HIP, "training step capture failed");
+        }
+        if(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return fail(GF_ERR_HIP, "training graph instantiation failed");
+        }
+        gf_status gst = GF_OK;
+        for(int k = 1; k < steps && !gst; k++)
+            if(hipGraphLaunch(ge, s) != hipSuccess)
+                gst = fail(GF_ERR_HIP, "training graph launch failed at step %d", k);
+        (void)hipStreamSynchronize(s);
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+        if(gst)
+            return gst;
+    } else {
+        for(int k = 1; k < steps; k++)
+            if(gfm::launch_train_step(m->D, dev_m12, dev_m13, ndata, B, batch, s))
+                return fail(GF_ERR_HIP, "training step %d launch failed", k);
+    }
+    m->adam_t += steps;
     if(loss_last)
         GF_HIP(hipMemcpyAsync(loss_last, m->D.loss, 4, hipMemcpyDeviceToHost, s));
     GF_HIP(hipStreamSynchronize(s));
+    return GF_OK;
+}
+
+gf_status gf_mlp_use_graph(gf_mlp *m, int on) {
+    if(!m)
+        return fail(GF_ERR_INVALID, "null network");
+    m->use_graph = on != 0;
     return GF_OK;
 }
 
@@ -2100,8 +2163,9 @@ gf_status gf_dalitz_generate(gf_ctx *ctx, int dalitz_func, int64_t n, uint64_t s
         return fail(GF_ERR_INVALID, "null output or n < 1");
     if(!(box[1] > box[0]) || !(box[3] > box[2]))
         return fail(GF_ERR_INVALID, "empty box");
-    int eff_pc   = -1;
-    gf_status st = subprogram(ctx, d->eff_func, eff_pc);
+    int eff_pc      = -1;
+    double eff_cval = 1.0;
+    gf_status st    = efficiency_mode(ctx, d->eff_func, eff_pc, eff_cval);
     if(st)
         return st;
     if((st = flush_uploads(ctx)))
@@ -2135,7 +2199,8 @@ gf_status gf_dalitz_generate(gf_ctx *ctx, int dalitz_func, int64_t n, uint64_t s
     const uint64_t key = gfd::mix64(seed ^ 0x6A09E667F3BCC909ull);
     int64_t have = 0, base = 0;
     auto pass = [&](double env) -> gf_status {
-        if(gfk::launch_dalitz_candidates(a, ctx->d_res + slot * GF_MAX_RES, d->nres, d->p_coup, eff_pc, d->col_m12,
+        if(gfk::launch_dalitz_candidates(a, ctx->d_res + slot * GF_MAX_RES, d->nres, d->p_coup, eff_pc, eff_cval,
+                                         d->col_m12,
                                          d->col_m13, box, key, base, B, env, c, c + B, flag, bcount, bmax,
                                          ctx->stream))
             return fail(GF_ERR_HIP, "candidate launch failed");

@@ -123,11 +123,22 @@ __device__ __forceinline__ double two_body_cm_mother_mom(double rMassSq, double 
     return 0.5 * rMassSq * kin1 * kin2 / dm;
 }
 
+// x^(2 spin + 1) for spin 0, 1, 2 by multiplication (the reference calls pow with
+// the double exponent 2 spin + 1; products differ from pow by a few ulp, far
+// inside the likelihood tolerance, and avoid FP64 pow's log/exp sequence).
+__device__ __forceinline__ double odd_power(double x, int spin) {
+    const double x2 = x * x;
+    double r        = x;
+    for(int k = 0; k < spin; ++k)
+        r *= x2;
+    return r;
+}
+
 // dampingFactorSquare, Resonance.cu:77-86 (spin-2 form as written there)
 __device__ __forceinline__ double damping_factor_square(double cmmom, int spin, double mRadius) {
     double square  = mRadius * mRadius * cmmom * cmmom;
     double dfsq    = 2 * square;
-    double dfsqres = 13 * square / pow(square - 3, 2.0) + 9 * square * square;
+    double dfsqres = 13 * square / ((square - 3) * (square - 3)) + 9 * square * square;
     return (spin == 2) ? dfsqres : dfsq;
 }
 
@@ -201,7 +212,7 @@ __device__ __forceinline__ double2 rbw(const GfDecay &d, double resmass, double 
             }
         }
         double A  = (resmass2 - rMassSq);
-        double B  = resmass2 * reswidth * pow(measureDaughterMoms / nominalDaughterMoms, 2.0 * spin + 1) * frFactor
+        double B  = resmass2 * reswidth * odd_power(measureDaughterMoms / nominalDaughterMoms, spin) * frFactor
                    / sqrt(rMassSq);
         double C  = 1.0 / (A * A + B * B);
         double re = A * C, im = B * C;
@@ -287,7 +298,7 @@ __device__ __forceinline__ double2 gounaris_sakurai(const GfDecay &d, double res
     }
     double D = (1.0 + d_fun(resmass, d.daug2Mass, d.daug3Mass) * reswidth / sqrt(resmass));
     double E = resmass - rMassSq + fs_fun(rMassSq, resmass, reswidth, d.daug2Mass, d.daug3Mass);
-    double F = sqrt(resmass) * reswidth * pow(measureDaughterMoms / nominalDaughterMoms, 2.0 * spin + 1) * frFactor;
+    double F = sqrt(resmass) * reswidth * odd_power(measureDaughterMoms / nominalDaughterMoms, spin) * frFactor;
     D /= (E * E + F * F);
     double re = D * E, im = D * F;
     double s1 = sqrt(frFactor);

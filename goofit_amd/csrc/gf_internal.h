@@ -160,9 +160,11 @@ int launch_batch(const GfProgArgs &a, const GfBatchWork *work, int nblocks, cons
                  int ntoys, double *result, uint32_t *counter, uint32_t *stamp, void *stream);
 // Accept-reject generation: candidates (uniform in box) with accept flags, and
 // the ordered compaction of the accepted ones.
-int launch_dalitz_candidates(const GfProgArgs &a, const GfResDesc *res, int nres, int p_coup, int eff_pc, int id12,
-                             int id13, const double box[4], uint64_t key, int64_t base, int64_t nc, double fmax,
-                             double *c12, double *c13, uint8_t *flag, int *bcount, double *bmax, void *stream);
+// eff_pc >= 0: efficiency subprogram; -1: efficiency 1; -2: the constant eff_const.
+int launch_dalitz_candidates(const GfProgArgs &a, const GfResDesc *res, int nres, int p_coup, int eff_pc,
+                             double eff_const, int id12, int id13, const double box[4], uint64_t key, int64_t base,
+                             int64_t nc, double fmax, double *c12, double *c13, uint8_t *flag, int *bcount,
+                             double *bmax, void *stream);
 int launch_compact(const double *c12, const double *c13, const uint8_t *flag, int64_t nc, const int64_t *boff,
                    int64_t limit, double *o12, double *o13, void *stream);
 int launch_dalitz_cache(const GfProgArgs &a, int slot, int64_t offset, int64_t n, uint32_t mask,
@@ -170,7 +172,7 @@ int launch_dalitz_cache(const GfProgArgs &a, int slot, int64_t offset, int64_t n
 // mode 0: grid cells (grid6 host array); 1: points (m12, m13, w device arrays); 2: data cache of `slot`.
 // Writes 768 doubles (tiles RR, II, RI in MFMA register layout) to tiles_dev.
 int launch_gram(const GfProgArgs &a, int mode, const double *grid6, const double *m12, const double *m13,
-                const double *w, int64_t npoints, int slot, int id_m12, int id_m13, int eff_pc,
+                const double *w, int64_t npoints, int slot, int id_m12, int id_m13, int eff_pc, double eff_const,
                 const GfResDesc *res, int nres, double *partials, double *tiles_dev, int nblocks, void *stream);
 int launch_identity_check(const double *col, int64_t n, int32_t *flag, void *stream);
 int mfma_probe(const double *dev_a, const double *dev_b, double *dev_out, void *stream);

@@ -729,8 +729,10 @@ __global__ void k_stamp(uint32_t *__restrict__ counter, uint32_t *__restrict__ m
 // device_DalitzPlot, Amp3Body.cu:60-122, with lineshapes computed directly);
 // accepted when u_i * fmax < value.  Per block: accepted count and the largest
 // value (to check the envelope).
+template <bool GEN_EFF>
 __global__ __launch_bounds__(256) void k_dalitz_candidates(GfProgArgs A, const GfResDesc *__restrict__ res, int nres,
-                                                           int p_coup, int eff_pc, int id12, int id13, double l12,
+                                                           int p_coup, int eff_pc, double eff_const, int id12,
+                                                           int id13, double l12,
                                                            double u12, double l13, double u13, uint64_t key,
                                                            int64_t base, int64_t nc, double fmax,
                                                            double *__restrict__ c12, double *__restrict__ c13,
@@ -749,14 +751,15 @@ __global__ __launch_bounds__(256) void k_dalitz_candidates(GfProgArgs A, const G
         const double m23 = D.motherMass * D.motherMass + D.daug1Mass * D.daug1Mass + D.daug2Mass * D.daug2Mass
                            + D.daug3Mass * D.daug3Mass - m12 - m13;
         double re = 0.0, im = 0.0;
+#pragma unroll 1
         for(int r = 0; r < nres; ++r) {
             const double2 b = gfd::lineshape(res[r], A.params, A.consts, D, m12, m13, m23);
             const double ar = A.params[p_coup + 2 * r], ai = A.params[p_coup + 2 * r + 1];
             re += ar * b.x - ai * b.y;
             im += ar * b.y + ai * b.x;
         }
-        double eff = 1.0;
-        if(eff_pc >= 0) {
+        double eff = eff_pc == -2 ? eff_const : 1.0;
+        if constexpr(GEN_EFF) {
             RegSrc<1> ev;
 #pragma unroll
             for(int o = 0; o < GF_MAX_OBS; ++o)
@@ -840,7 +843,8 @@ struct GramSrc {
     int64_t npoints;
     int slot;          // mode 2
     int id_m12, id_m13;
-    int eff_pc;        // -1: efficiency 1
+    int eff_pc;        // >= 0: efficiency subprogram; -1: efficiency 1; -2: constant eff_const
+    double eff_const;
 };
 
 // One block: 128 points per pass; each lane computes the 16 complex amplitudes of
@@ -848,6 +852,7 @@ struct GramSrc {
 // and each wave multiplies its 64 points as 16 MFMA k-steps for each of three
 // 16x16 output tiles: RR = R^T E R, II = I^T E I, RI = R^T E I, with
 // E = diag(eff * weight).  Partials: 3 tiles x 64 lanes x 4 values per block.
+template <bool GEN_EFF>
 __global__ __launch_bounds__(GG_BLOCK) void k_gram(GfProgArgs A, GramSrc S, const GfResDesc *__restrict__ res,
                                                    int nres, double *partials) {
     __shared__ double Xr[2][16][GG_LD];
@@ -898,6 +903,8 @@ __global__ __launch_bounds__(GG_BLOCK) void k_gram(GfProgArgs A, GramSrc S, cons
             const bool inside = valid && gfd::in_dalitz(m12, m13, D.motherMass, D.daug1Mass, D.daug2Mass, D.daug3Mass);
             const double m23  = D.motherMass * D.motherMass + D.daug1Mass * D.daug1Mass + D.daug2Mass * D.daug2Mass
                                + D.daug3Mass * D.daug3Mass - m12 - m13;
+            // one lineshape body in the loop (not 16 inlined copies): registers for occupancy
+#pragma unroll 1
             for(int i = 0; i < GF_MAX_RES; ++i) {
                 double2 b = make_double2(0.0, 0.0);
                 if(i < nres && inside)
@@ -905,8 +912,8 @@ __global__ __launch_bounds__(GG_BLOCK) void k_gram(GfProgArgs A, GramSrc S, cons
                 Xr[wave][i][lane] = b.x;
                 Xi[wave][i][lane] = b.y;
             }
-            double eff = 1.0;
-            if(S.eff_pc >= 0) {
+            double eff = S.eff_pc == -2 ? S.eff_const : 1.0;
+            if constexpr(GEN_EFF) {
                 // The fake event of SpecialResonanceIntegrator.cu:97-101: slot 0 = 2,
                 // then m12 and m13 at their observable slots; other slots 0.
                 RegSrc<1> ev;
@@ -1094,12 +1101,19 @@ int launch_batch(const GfProgArgs &a, const GfBatchWork *work, int nblocks, cons
     return check(hipGetLastError());
 }
 
-int launch_dalitz_candidates(const GfProgArgs &a, const GfResDesc *res, int nres, int p_coup, int eff_pc, int id12,
-                             int id13, const double box[4], uint64_t key, int64_t base, int64_t nc, double fmax,
-                             double *c12, double *c13, uint8_t *flag, int *bcount, double *bmax, void *stream) {
+int launch_dalitz_candidates(const GfProgArgs &a, const GfResDesc *res, int nres, int p_coup, int eff_pc,
+                             double eff_const, int id12, int id13, const double box[4], uint64_t key, int64_t base,
+                             int64_t nc, double fmax, double *c12, double *c13, uint8_t *flag, int *bcount,
+                             double *bmax, void *stream) {
     const int nb = (int)((nc + 255) / 256);
-    hipLaunchKernelGGL(k_dalitz_candidates, dim3(nb), dim3(256), 0, (hipStream_t)stream, a, res, nres, p_coup, eff_pc,
-                       id12, id13, box[0], box[1], box[2], box[3], key, base, nc, fmax, c12, c13, flag, bcount, bmax);
+    if(eff_pc >= 0)
+        hipLaunchKernelGGL(k_dalitz_candidates<true>, dim3(nb), dim3(256), 0, (hipStream_t)stream, a, res, nres,
+                           p_coup, eff_pc, eff_const, id12, id13, box[0], box[1], box[2], box[3], key, base, nc, fmax,
+                           c12, c13, flag, bcount, bmax);
+    else
+        hipLaunchKernelGGL(k_dalitz_candidates<false>, dim3(nb), dim3(256), 0, (hipStream_t)stream, a, res, nres,
+                           p_coup, eff_pc, eff_const, id12, id13, box[0], box[1], box[2], box[3], key, base, nc, fmax,
+                           c12, c13, flag, bcount, bmax);
     return check(hipGetLastError());
 }
 
@@ -1119,7 +1133,8 @@ int launch_dalitz_cache(const GfProgArgs &a, int slot, int64_t offset, int64_t n
 }
 
 int launch_gram(const GfProgArgs &a, int mode, const double *grid6, const double *m12, const double *m13,
-                const double *w, int64_t npoints, int slot, int id_m12, int id_m13, int eff_pc, const GfResDesc *res,
+                const double *w, int64_t npoints, int slot, int id_m12, int id_m13, int eff_pc, double eff_const,
+                const GfResDesc *res,
                 int nres, double *partials, double *tiles_dev, int nblocks, void *stream) {
     GramSrc S;
     S.mode = mode;
@@ -1140,9 +1155,13 @@ int launch_gram(const GfProgArgs &a, int mode, const double *grid6, const double
     S.slot    = slot;
     S.id_m12  = id_m12;
     S.id_m13  = id_m13;
-    S.eff_pc  = eff_pc;
+    S.eff_pc    = eff_pc;
+    S.eff_const = eff_const;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_gram, dim3(nblocks), dim3(GG_BLOCK), 0, s, a, S, res, nres, partials);
+    if(eff_pc >= 0)
+        hipLaunchKernelGGL(k_gram<true>, dim3(nblocks), dim3(GG_BLOCK), 0, s, a, S, res, nres, partials);
+    else
+        hipLaunchKernelGGL(k_gram<false>, dim3(nblocks), dim3(GG_BLOCK), 0, s, a, S, res, nres, partials);
     int rc = check(hipGetLastError());
     if(rc)
         return rc;

@@ -29,6 +29,18 @@ struct GfBox {
     double l12, u12, l13, u13;
 };
 
+// Per-step training state kept on the device, so that a captured training step
+// can be replayed: k_step_begin advances it (batch key, learning rate, Adam bias
+// corrections) before the step's kernels read it.
+struct GfTrainState {
+    uint64_t seed;
+    int32_t k, steps;      // step within this training call, number of steps
+    int32_t adam_t, pad;   // Adam step count over the network's life
+    float lr0, b1, b2, eps;
+    float lr, c1, c2, pad2;
+    uint64_t key;
+};
+
 // Device buffers of one network (sized for batches of up to cap samples).
 struct GfMlpDev {
     float *P, *m, *v, *grad;   // parameters, Adam moments, gradient
@@ -38,20 +50,21 @@ struct GfMlpDev {
     uint16_t *h1, *delta2;     // bf16 activations / layer-2 deltas [cap][H]
     float *out, *coef, *loss;  // f per sample, dL/df per sample, loss
     float *part_in, *part_w2, *part_out;
+    GfTrainState *state;
     int64_t cap;
 };
 
 namespace gfm {
-int launch_batch_train(const double *m12, const double *m13, int64_t ndata, GfBox box, int nd, int nu, uint64_t key,
-                       float2 *xy, hipStream_t s);
+// one training step, all parameters from the device state
+int launch_train_step(const GfMlpDev &D, const double *m12, const double *m13, int64_t ndata, GfBox box, int batch,
+                      hipStream_t s);
 int launch_batch_grid(int G, float2 *xy, hipStream_t s);
 // n a multiple of 64, n <= cap
 int launch_forward(const GfMlpDev &D, const float2 *xy, int n, float *out, hipStream_t s);
 int launch_loss(const float *out, int nd, int nu, float *coef, float *loss, hipStream_t s);
 // gradient of sum_s coef[s] f(x_s) into grad[GF_MLP_NPARAM] (after launch_forward on the same xy)
 int launch_backward(const GfMlpDev &D, const float2 *xy, const float *coef, int n, float *grad, hipStream_t s);
-int launch_adam(const GfMlpDev &D, const float *grad, float lr, float b1, float b2, float eps, float c1, float c2,
-                hipStream_t s);
+
 int launch_w2_bf16(const GfMlpDev &D, hipStream_t s);
 int launch_sample(const double *cdf, const double *pcell, int G, GfBox box, int64_t n, uint64_t key, double *m12,
                   double *m13, double *w, hipStream_t s);

@@ -71,11 +71,12 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t *p) {
 // [nd, nd + nu) uniform points of the unit square.  Inputs to the network are
 // x = 2u - 1, y = 2v - 1.
 __global__ __launch_bounds__(256) void k_batch_train(const double *__restrict__ m12, const double *__restrict__ m13,
-                                                     int64_t ndata, GfBox box, int nd, int nu, uint64_t key,
-                                                     float2 *__restrict__ xy) {
+                                                     int64_t ndata, GfBox box, int nd, int nu,
+                                                     const GfTrainState *__restrict__ st, float2 *__restrict__ xy) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if(s >= nd + nu)
         return;
+    const uint64_t key = st->key;
     double u, v;
     if(s < nd) {
         const int64_t e = (int64_t)(mix64(key ^ mix64(2 * (uint64_t)s)) % (uint64_t)ndata);
@@ -409,14 +410,27 @@ __global__ __launch_bounds__(256) void k_grad_sum(const float *__restrict__ p_in
     g[i] = s;
 }
 
+// Advance the training state by one step: batch key, cosine-decayed learning
+// rate lr0 (0.05 + 0.95 (1 + cos(pi k / steps)) / 2), Adam bias corrections.
+__global__ void k_step_begin(GfTrainState *st) {
+    const int k = st->k;
+    st->adam_t += 1;
+    st->key = st->seed * 0x9E3779B97F4A7C15ull + 2ull * (uint64_t)k + 1ull;
+    const double cs = 0.5 * (1.0 + cos(3.141592653589793 * (double)k / (double)st->steps));
+    st->lr = (float)(st->lr0 * (0.05 + 0.95 * cs));
+    st->c1 = 1.f - powf(st->b1, (float)st->adam_t);
+    st->c2 = 1.f - powf(st->b2, (float)st->adam_t);
+    st->k  = k + 1;
+}
+
 // Adam step on the FP32 master parameters, then the bf16 copies of W2 and W2^T.
 __global__ __launch_bounds__(256) void k_adam(float *__restrict__ P, float *__restrict__ m, float *__restrict__ v,
-                                              const float *__restrict__ g, float lr, float b1, float b2, float eps,
-                                              float c1, float c2, uint16_t *__restrict__ w2b,
-                                              uint16_t *__restrict__ w2t) {
+                                              const float *__restrict__ g, const GfTrainState *__restrict__ st,
+                                              uint16_t *__restrict__ w2b, uint16_t *__restrict__ w2t) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= GF_MLP_NPARAM)
         return;
+    const float lr = st->lr, b1 = st->b1, b2 = st->b2, eps = st->eps, c1 = st->c1, c2 = st->c2;
     const float gi = g[i];
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
@@ -511,10 +525,21 @@ namespace gfm {
 
 int check(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
 
-int launch_batch_train(const double *m12, const double *m13, int64_t ndata, GfBox box, int nd, int nu, uint64_t key,
-                       float2 *xy, hipStream_t s) {
-    hipLaunchKernelGGL(k_batch_train, dim3(nblk(nd + nu, 256)), dim3(256), 0, s, m12, m13, ndata, box, nd, nu, key,
-                       xy);
+int launch_train_step(const GfMlpDev &D, const double *m12, const double *m13, int64_t ndata, GfBox box, int batch,
+                      hipStream_t s) {
+    const int half = batch / 2;
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, s, D.state);
+    int rc = check(hipGetLastError());
+    if(rc)
+        return rc;
+    hipLaunchKernelGGL(k_batch_train, dim3(nblk(batch, 256)), dim3(256), 0, s, m12, m13, ndata, box, half, half,
+                       (const GfTrainState *)D.state, D.xy);
+    if((rc = check(hipGetLastError())) || (rc = launch_forward(D, D.xy, batch, D.out, s))
+       || (rc = launch_loss(D.out, half, half, D.coef, D.loss, s))
+       || (rc = launch_backward(D, D.xy, D.coef, batch, D.grad, s)))
+        return rc;
+    hipLaunchKernelGGL(k_adam, dim3(nblk(GF_MLP_NPARAM, 256)), dim3(256), 0, s, D.P, D.m, D.v, D.grad,
+                       (const GfTrainState *)D.state, D.w2b, D.w2t);
     return check(hipGetLastError());
 }
 
@@ -558,12 +583,6 @@ int launch_backward(const GfMlpDev &D, const float2 *xy, const float *coef, int 
     return check(hipGetLastError());
 }
 
-int launch_adam(const GfMlpDev &D, const float *grad, float lr, float b1, float b2, float eps, float c1, float c2,
-                hipStream_t s) {
-    hipLaunchKernelGGL(k_adam, dim3(nblk(GF_MLP_NPARAM, 256)), dim3(256), 0, s, D.P, D.m, D.v, grad, lr, b1, b2, eps,
-                       c1, c2, D.w2b, D.w2t);
-    return check(hipGetLastError());
-}
 
 int launch_w2_bf16(const GfMlpDev &D, hipStream_t s) {
     hipLaunchKernelGGL(k_w2_bf16, dim3(nblk(GF_MLP_H * GF_MLP_H, 256)), dim3(256), 0, s, D.P, D.w2b, D.w2t);

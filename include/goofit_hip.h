@@ -270,6 +270,10 @@ gf_status gf_mlp_gradient(gf_mlp *m, const float *xy, const float *coef, int64_t
 gf_status gf_mlp_train(gf_mlp *m, const double *dev_m12, const double *dev_m13, int64_t ndata, const double box[4],
                        int steps, int batch, float lr, uint64_t seed, float *loss_first, float *loss_last);
 
+/* on = 0: launch each training step's kernels from the host instead of
+ * replaying one captured step (the results are identical). */
+gf_status gf_mlp_use_graph(gf_mlp *m, int on);
+
 /* Same, with the events in host arrays. */
 gf_status gf_mlp_train_host(gf_mlp *m, const double *m12, const double *m13, int64_t ndata, const double box[4],
                             int steps, int batch, float lr, uint64_t seed, float *loss_first, float *loss_last);

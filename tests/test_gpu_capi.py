@@ -317,6 +317,12 @@ def test_dalitz_nll_paths(ctx):
         assert math.isclose(got, ref, rel_tol=NLL_RTOL), (permute, got, ref)
         probs = ctx.eval_metric("PROB")
         np.testing.assert_allclose(probs, M.op.eval_metric(a, "PROB"), rtol=1e-12, atol=0)
+    # the grid integrals with a non-constant efficiency (the general efficiency path)
+    res_funcs, effi = M.op.dalitz_layout(top.function_idx)
+    grid6 = [0.0, 3.0, 100, 0.0, 3.0, 100]
+    ref_i = M.op.dalitz_integrals(top.function_idx, grid6, effi)
+    got_i = ctx.dalitz_gram(top.function_idx, grid6, effi, len(res_funcs))
+    np.testing.assert_allclose(got_i, ref_i, rtol=1e-11, atol=1e-13 * np.abs(ref_i).max())
 
 
 def test_dalitz_generate(ctx):

@@ -147,3 +147,22 @@ def test_proposal_beats_uniform_at_equal_points(ctx):
         assert abs(errs_q.mean()) < 4 * errs_q.std(ddof=1) / math.sqrt(errs_q.size) + 2e-4
     finally:
         q.close()
+
+
+def test_training_graph_replays_match_eager(ctx):
+    """The captured training step replayed N times gives the same network as N eager steps
+    (bitwise: deterministic kernels, same device-side step state)."""
+    rng = np.random.default_rng(8)
+    ev12, ev13 = rng.uniform(0, 3, 20_000), rng.uniform(0, 3, 20_000)
+    nets = []
+    for graph in (False, True):
+        q = capi.Proposal(ctx, seed=12, batch_cap=4096)
+        q.use_graph(graph)
+        l0, l1 = q.train_host(ev12, ev13, BOX, steps=25, batch=2048, lr=3e-3, seed=6)
+        nets.append((q.get_params(), l0, l1))
+        # a second call continues the Adam step count
+        q.train_host(ev12, ev13, BOX, steps=5, batch=2048, lr=1e-3, seed=7)
+        nets.append((q.get_params(), 0.0, 0.0))
+        q.close()
+    assert np.array_equal(nets[0][0], nets[2][0]) and nets[0][1:] == nets[2][1:]
+    assert np.array_equal(nets[1][0], nets[3][0])

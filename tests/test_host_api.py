@@ -163,3 +163,23 @@ def test_binned_dataset_numbering():
     d1 = G.BinnedDataSet(x)
     with pytest.raises(IndexError):
         d1.addEvent()
+
+
+def test_python_dataset_surface():
+    """python/goofit/{UnbinnedDataSet,Variable,DataSet}.cpp of the reference: numpy round trips,
+    from_matrix with the event number as the missing last row, properties."""
+    x = G.Observable("x", 0, 1)
+    y = G.Observable("y", -1, 1)
+    n = G.EventNumber("n")
+    d = G.UnbinnedDataSet(x, y, n)
+    m = np.array([[0.1, 0.2, 1.5, 0.3], [0.0, 0.5, 0.2, -2.0]])
+    d.from_matrix(m, filter=True)  # events 3 and 4 are out of range
+    assert len(d) == 2 and d.name != "" and [v.name for v in d.variables] == ["x", "y", "n"]
+    a = d.to_numpy()
+    np.testing.assert_array_equal(a, [[0.1, 0.2], [0.0, 0.5], [0.0, 1.0]])
+    np.testing.assert_array_equal(d[1], [0.0, 0.5])
+    x.numbins = 4
+    assert x.bin_size == 0.25 and x.bin_centers == [0.125, 0.375, 0.625, 0.875]
+    v = G.Variable("v", 1.0, 0.1, 0, 2)
+    v.blind = 0.5
+    assert v.fitterIndex == -1
