@@ -43,6 +43,10 @@ __device__ __forceinline__ void kb_merge(double &s, double &c, double s2, double
     kb_add(s, c, s2);
     c += c2;
 }
+// The sum a compensated pair stands for.  Once s is infinite or NaN the
+// correction is meaningless (inf - inf = NaN), and the plain sum's value -- the
+// one the reference's uncompensated reduction gives -- is s itself.
+__device__ __forceinline__ double kb_total(double s, double c) { return isfinite(s) ? s + c : s; }
 
 // ------------------------------------------------------------------- metrics
 

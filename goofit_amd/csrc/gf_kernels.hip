@@ -387,7 +387,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_finish(const Partial *partials, in
     block_reduce(ts, tc);
     if(threadIdx.x == 0) {
         const int32_t e = *err;
-        result->sum     = ts + tc;
+        result->sum     = gfd::kb_total(ts, tc);
         result->comp    = tc;
         result->err     = e;
         if(e)
@@ -708,7 +708,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_finish_toys(const Partial *__restr
         gfd::kb_merge(s, c, partials[i].s, partials[i].c);
     block_reduce(s, c);
     if(threadIdx.x == 0) {
-        result[2 * t]     = s + c;
+        result[2 * t]     = gfd::kb_total(s, c);
         result[2 * t + 1] = c;
     }
 }
@@ -979,7 +979,7 @@ __global__ __launch_bounds__(256) void k_gram_final(const double *__restrict__ p
         gfd::kb_add(s, c, partials[(int64_t)b * 768 + e]);
     block_reduce(s, c);
     if(threadIdx.x == 0)
-        out[e] = s + c;
+        out[e] = gfd::kb_total(s, c);
 }
 
 // One v_mfma_f64_16x16x4f64 on given per-lane operands: out[4*lane + r] is

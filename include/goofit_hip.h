@@ -60,7 +60,10 @@ enum gf_func {
     GF_FN_RBW_SYM           = 15, /* plainBW<2>             src/PDFs/physics/resonances/RBW.cu:89 */
     GF_FN_GS                = 16, /* gouSak                 src/PDFs/physics/resonances/GS.cu:9-75 */
     GF_FN_NONRES            = 17, /* nonres                 src/PDFs/physics/resonances/NonRes.cu:8-11 */
-    GF_FN_COUNT             = 18
+    GF_FN_TDDP              = 18, /* device_Tddp            src/PDFs/physics/Amp3Body_TD.cu:41-257 */
+    GF_FN_TRUTH_RES         = 19, /* device_truth_resolution src/PDFs/physics/TruthResolution.cu:6-27 */
+    GF_FN_THREEGAUSS_RES    = 20, /* device_threegauss_resolution src/PDFs/physics/ThreeGaussResolution.cu:63-127 */
+    GF_FN_COUNT             = 21
 };
 
 /* Metric codes: same numbering as the reference's EvalFunc

@@ -33,7 +33,7 @@ LIB_PATH = os.path.join(HERE, "lib", "liboracle.so")
 # Function codes: the gf_func enum of include/goofit_hip.h.
 FN = dict(NONE=0, GAUSSIAN=1, EXP=2, EXP_OFFSET=3, EXP_POLY=4, EXP_POLY_OFFSET=5, EXPGAUS=6,
           POLYNOMIAL=7, OFFSET_POLYNOMIAL=8, MULTI_POLYNOMIAL=9, ADD=10, ADD_EXT=11, PROD=12,
-          DALITZ=13, RBW=14, RBW_SYM=15, GS=16, NONRES=17)
+          DALITZ=13, RBW=14, RBW_SYM=15, GS=16, NONRES=17, TDDP=18, TRUTH_RES=19, THREEGAUSS_RES=20)
 METRIC = dict(EVAL=0, NLL=1, PROB=2, BINAVG=3, BINWITHERROR=4, CHISQ=5)
 
 _lib = None
@@ -93,6 +93,20 @@ def lib():
         L.gfo_dalitz_integrals.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int, _D]
         L.gfo_dalitz_norm_from_integrals.argtypes = [ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double]
         L.gfo_dalitz_norm_from_integrals.restype = ctypes.c_double
+        L.gfo_prog_set_cache_b.argtypes = [ctypes.c_void_p, ctypes.c_int, _D]
+        L.gfo_tddp_layout.argtypes = [ctypes.c_void_p, ctypes.c_int, _I, _I, _I, _I]
+        L.gfo_tddp_layout.restype = ctypes.c_int
+        L.gfo_tddp_wave.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int64, ctypes.c_int,
+                                    ctypes.c_int64, _D, _D]
+        L.gfo_tddp_integrals.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int, _D, _D, _D]
+        L.gfo_resolution_norm.argtypes = [ctypes.c_int] + [ctypes.c_double] * 8
+        L.gfo_resolution_norm.restype = ctypes.c_double
+        L.gfo_resolution_eval.argtypes = [ctypes.c_int, _D] + [ctypes.c_double] * 9
+        L.gfo_resolution_eval.restype = ctypes.c_double
+        L.gfo_tddp_norm_from_integrals.argtypes = [ctypes.c_int, _D, _D, _D, _D, _D, ctypes.c_int, ctypes.c_double,
+                                                   ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                   ctypes.c_double]
+        L.gfo_tddp_norm_from_integrals.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -133,6 +147,7 @@ class Obs:
 
     def __init__(self, name, lo, hi, nbins=100, event_number=False):
         self.name, self.lo, self.hi, self.nbins = name, float(lo), float(hi), int(nbins)
+        self.value = 0.0  # host value (Indexable(name, 0, lo, hi), include/goofit/Variable.h:120-121)
         self.index = -1
         self.event_number = event_number
 
@@ -336,6 +351,53 @@ class Dalitz(Pdf):
         self.cache_to_use = cache_to_use
 
 
+class TruthResolution(Pdf):
+    """TruthResolution (src/PDFs/physics/TruthResolution.cu:48-53): no parameters."""
+    func = FN["TRUTH_RES"]
+
+    def __init__(self, name="truth"):
+        super().__init__(name)
+
+
+class ThreeGaussResolution(Pdf):
+    """ThreeGaussResolution (src/PDFs/physics/ThreeGaussResolution.cu:131-151): parameters
+    core/tail fractions, core bias/scale, tail bias/scale, outlier bias/scale,
+    selection bias, then the two common bias offsets and width scales."""
+    func = FN["THREEGAUSS_RES"]
+
+    def __init__(self, name, cf, tf, cb, cs, tb, ts, ob, os_, sb, cbo, cws, cbo2, cws2):
+        super().__init__(name, [], [cf, tf, cb, cs, tb, ts, ob, os_, sb, cbo, cws, cbo2, cws2])
+        self.selection_bias = sb
+
+
+class Tddp(Pdf):
+    """Amp3Body_TD (src/PDFs/physics/Amp3Body_TD.cu:262-359): parameters tau, x, y,
+    dx, dy and the couplings; constants nres, cacheToUse, mistag flag, end of the
+    efficiency subtree; observables dtime, sigma_t, m12, m13, event number
+    [, mistag[, charm tag]].  mix = (tau, x, y, dx, dy) Vars."""
+    func = FN["TDDP"]
+
+    def __init__(self, name, dtime, sigmat, m12, m13, evtnum, decay, mix, resonances, resolution, eff,
+                 mistag=None, charmtag=None, cache_to_use=0):
+        params = list(mix)
+        for r in resonances:
+            params += [r.amp_real, r.amp_imag]
+        obs = [dtime, sigmat, m12, m13, evtnum]
+        if mistag is not None:
+            obs.append(mistag)
+        if charmtag is not None:
+            obs.append(charmtag)
+        super().__init__(name, obs, params, [len(resonances), cache_to_use, 0 if mistag is not None else 1, 0],
+                         list(resonances) + [resolution, eff])
+        self.decay = list(decay)
+        self.m12, self.m13 = m12, m13
+        self.resonances = list(resonances)
+        self.resolution = resolution
+        self.mistag = mistag
+        self.cache_to_use = cache_to_use
+        self.d0_fraction = 0.5
+
+
 class OracleProg:
     """A gfo_prog (the oracle's flattened-tree evaluator) for one top-level PDF."""
 
@@ -438,6 +500,35 @@ class OracleProg:
         lib().gfo_dalitz_wave(self.h, dalitz_func, res_func, _dp(aos), n, k, offset, _dp(out))
         return out.view(np.complex128)
 
+    def set_cache_b(self, slot, waves):
+        w = np.ascontiguousarray(waves.view(np.float64) if np.iscomplexobj(waves) else waves, dtype=np.float64)
+        self._caches[(slot, "b")] = w
+        lib().gfo_prog_set_cache_b(self.h, slot, _dp(w))
+
+    def tddp_layout(self, tddp_func):
+        res = np.zeros(64, np.int32)
+        out = np.zeros(3, np.int32)
+        n = lib().gfo_tddp_layout(self.h, tddp_func, res.ctypes.data_as(_I), out[0:].ctypes.data_as(_I),
+                                  out[1:].ctypes.data_as(_I), out[2:].ctypes.data_as(_I))
+        return [int(x) for x in res[:n]], int(out[0]), int(out[1]), int(out[2])
+
+    def tddp_wave(self, tddp_func, res_func, aos, offset=0, n=None):
+        aos = np.ascontiguousarray(aos, np.float64)
+        rows, k = aos.shape
+        n = rows - offset if n is None else n
+        a = np.zeros(2 * n)
+        b = np.zeros(2 * n)
+        lib().gfo_tddp_wave(self.h, tddp_func, res_func, _dp(aos), n, k, offset, _dp(a), _dp(b))
+        return a.view(np.complex128), b.view(np.complex128)
+
+    def tddp_integrals(self, tddp_func, grid6, nchunks=None):
+        res = self.tddp_layout(tddp_func)[0]
+        g = np.asarray(grid6, np.float64)
+        m = len(res)
+        outs = [np.zeros(2 * m * m) for _ in range(3)]
+        lib().gfo_tddp_integrals(self.h, tddp_func, _dp(g), nchunks or num_procs(), *(_dp(o) for o in outs))
+        return tuple(o.view(np.complex128).reshape(m, m) for o in outs)
+
     def dalitz_integrals(self, dalitz_func, grid6, eff_func, nchunks=None):
         res, _ = self.dalitz_layout(dalitz_func)
         g = np.asarray(grid6, np.float64)
@@ -467,6 +558,26 @@ class RawProg(OracleProg):
         lib().gfo_prog_set_params(self.h, _dp(self.p), self.p.size)
 
 
+def resolution_norm(kind, di, tau, x, y, sel_bias=0.0):
+    """MixingTimeResolution::normalization of kind FN["TRUTH_RES"] / FN["THREEGAUSS_RES"]."""
+    return lib().gfo_resolution_norm(kind, *[float(v) for v in di], tau, x, y, sel_bias)
+
+
+def resolution_eval(kind, params13, t1, t2, abr, abi, tau, dtime, x, y, sigma):
+    """Resolution function value (device_truth_resolution / device_threegauss_resolution)."""
+    p = np.ascontiguousarray(params13 if params13 is not None else np.zeros(13), np.float64)
+    return lib().gfo_resolution_eval(kind, _dp(p), t1, t2, abr, abi, tau, dtime, x, y, sigma)
+
+
+def tddp_norm_from_integrals(coups, Iaa, Iab, Ibb, mix5, res_kind, sel_bias, d0_fraction, mistag_on, mistag,
+                             bs12, bs13):
+    c = np.ascontiguousarray(np.asarray(coups, np.complex128).view(np.float64))
+    arrs = [np.ascontiguousarray(np.asarray(I, np.complex128).view(np.float64)) for I in (Iaa, Iab, Ibb)]
+    m = np.ascontiguousarray(mix5, np.float64)
+    return lib().gfo_tddp_norm_from_integrals(len(coups), _dp(c), *(_dp(a) for a in arrs), _dp(m), res_kind,
+                                              sel_bias, d0_fraction, int(bool(mistag_on)), mistag, bs12, bs13)
+
+
 def dalitz_norm_from_integrals(coups, I, bs12, bs13):
     c = np.ascontiguousarray(np.asarray(coups, np.complex128).view(np.float64))
     Ia = np.ascontiguousarray(np.asarray(I, np.complex128).view(np.float64))
@@ -486,14 +597,18 @@ class Model:
         self.nchunks = nchunks or num_procs()
         self.aos = None
         self.integrals = {}
+        self._reset_caches()
         if aos is not None:
             self.set_data(aos)
 
     def set_data(self, aos):
         self.aos = np.ascontiguousarray(aos, np.float64)
         self.op.set_total_events(self.aos.shape[0])
+        self._reset_caches()
+
+    def _reset_caches(self):
         for node in self.op.prog.nodes:
-            if isinstance(node, Dalitz):
+            if isinstance(node, (Dalitz, Tddp)):
                 node.redo = [True] * len(node.resonances)
                 node.cached_mw = [None] * len(node.resonances)
                 node.waves = [None] * len(node.resonances)
@@ -510,6 +625,8 @@ class Model:
             return 1.0
         if isinstance(pdf, Dalitz):
             return self._normalize_dalitz(pdf)
+        if isinstance(pdf, Tddp):
+            return self._normalize_tddp(pdf)
         return self._normalize_goopdf(pdf)
 
     def _normalize_goopdf(self, pdf):
@@ -584,6 +701,49 @@ class Model:
         I = self.integrals[pdf.function_idx]
         coups = [complex(r.amp_real.value, r.amp_imag.value) for r in pdf.resonances]
         ret = dalitz_norm_from_integrals(coups, I, pdf.m12.bin_size(), pdf.m13.bin_size())
+        pdf.cached_norm = 1.0 / ret
+        return ret
+
+    def _normalize_tddp(self, pdf: Tddp):
+        # Amp3Body_TD::normalize, src/PDFs/physics/Amp3Body_TD.cu:552-748
+        def set1(node):
+            node.cached_norm = 1.0
+            for c in node.components:
+                set1(c)
+        set1(pdf)
+        self.op.sync_params()
+        self.op.sync_norms()
+        nres = len(pdf.resonances)
+        res_funcs, _, _, _ = self.op.tddp_layout(pdf.function_idx)
+        for i, r in enumerate(pdf.resonances):
+            mw = tuple(q.value for q in r.params)
+            if pdf.cached_mw[i] != mw:
+                pdf.redo[i] = True
+                pdf.cached_mw[i] = mw
+        if any(pdf.redo):
+            if self.aos is not None:
+                for i in range(nres):
+                    if pdf.redo[i]:
+                        a, b = self.op.tddp_wave(pdf.function_idx, res_funcs[i], self.aos)
+                        pdf.waves[i] = (a, b)
+                        self.op.set_cache(16 * pdf.cache_to_use + i, a)
+                        self.op.set_cache_b(16 * pdf.cache_to_use + i, b)
+            grid6 = [pdf.m12.lo, pdf.m12.hi, pdf.m12.nbins, pdf.m13.lo, pdf.m13.hi, pdf.m13.nbins]
+            self.integrals[pdf.function_idx] = self.op.tddp_integrals(pdf.function_idx, grid6, self.nchunks)
+            pdf.redo = [False] * nres
+        Iaa, Iab, Ibb = self.integrals[pdf.function_idx]
+        coups = [complex(r.amp_real.value, r.amp_imag.value) for r in pdf.resonances]
+        mix5 = [q.value for q in pdf.params[:5]]
+        res = pdf.resolution
+        sel = res.selection_bias.value if isinstance(res, ThreeGaussResolution) else 0.0
+        mistag_on = False
+        if pdf.mistag is not None:
+            # Observable::operator bool (include/goofit/Variable.h:103-105)
+            v, lo, hi = pdf.mistag.value, pdf.mistag.lo, pdf.mistag.hi
+            mistag_on = hi == lo or (lo <= v <= hi)
+        ret = tddp_norm_from_integrals(coups, Iaa, Iab, Ibb, mix5, res.func, sel, pdf.d0_fraction, mistag_on,
+                                       pdf.mistag.value if pdf.mistag is not None else 0.0, pdf.m12.bin_size(),
+                                       pdf.m13.bin_size())
         pdf.cached_norm = 1.0 / ret
         return ret
 

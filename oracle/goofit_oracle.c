@@ -83,6 +83,8 @@ typedef struct gfo_prog {
     /* cResonances[16 * cacheToUse + i][evtNum] as interleaved (re, im) */
     const double *cache[GFO_MAX_CACHE];
     int64_t cache_len[GFO_MAX_CACHE];
+    /* Amp3Body_TD: cWaves[16 * cacheToUse + i][evtNum] holds (ai, bi); ai above, bi here */
+    const double *cache_b[GFO_MAX_CACHE];
     double total_events; /* c_totalEvents (src/PDFs/PdfBase.cu:438) */
     int error;           /* set when an evaluation walks outside the arrays */
 } gfo_prog;
@@ -604,6 +606,198 @@ static double f_dalitz(gfo_prog *g, const double *evt, pcur *pc) {
     return ret;
 }
 
+/* ------------------------------------------------------- time-dependent Dalitz */
+
+/* device_truth_resolution, src/PDFs/physics/TruthResolution.cu:6-27 */
+static double res_truth(double coshterm, double costerm, double sinhterm, double sinterm, double tau, double dtime,
+                        double xmixing, double ymixing) {
+    double ret = 0;
+    dtime /= tau;
+    ret += coshterm * cosh(ymixing * dtime);
+    ret += costerm * cos(xmixing * dtime);
+    ret -= 2 * sinhterm * sinh(ymixing * dtime);
+    ret -= 2 * sinterm * sin(xmixing * dtime);
+    ret *= exp(-dtime);
+    return ret;
+}
+
+/* gaussian(), src/PDFs/physics/ThreeGaussResolution.cu:11-61 */
+static void tg_gaussian(double *P1, double *P2, double *P3, double *P4, double tau, double adjTime, double xmixing,
+                        double ymixing, double adjSigma, double selbias) {
+    const double SQRTPIo2  = 1.0 / M_2_SQRTPI;
+    const double SQRT1o2PI = sqrt(0.5 * M_1_PI);
+    const double R1o6      = 1.0 / 6.0;
+    double _1oSqrtA  = adjSigma * M_SQRT2;
+    double _1oSigma  = 1 / adjSigma;
+    double _1o2SqrtA = 0.5 * _1oSqrtA;
+    double _1oSigma2 = _1oSigma * _1oSigma;
+    double _NormG    = SQRT1o2PI * _1oSigma;
+    double _C        = 0.5 * adjTime * adjTime * _1oSigma2;
+    double _Bgn      = -adjTime * _1oSigma2;
+    double _Gamma    = 1 / tau;
+    double _B        = _Gamma + _Bgn;
+    double _u0       = _1o2SqrtA * (_B + selbias);
+    double _u02      = _u0 * _u0;
+    double _F        = _1oSqrtA * exp(-_C + _u02);
+    double _Ig0      = SQRTPIo2 * erfc(_u0);
+    double _Ig1      = 0.5 * exp(-_u02);
+    double _Ig2      = _Ig1 * _u0 + 0.5 * _Ig0;
+    double _Ig3      = _Ig1 * (_u02 + 1);
+    double _R        = xmixing * _Gamma * _1oSqrtA;
+    double _R2       = _R * _R;
+    double _It0      = _F * _Ig0;
+    double _It1      = _F * _R * (_Ig1 - _u0 * _Ig0);
+    double _It2      = _F * _R2 * (_Ig2 - _u0 * _Ig1 * 2 + _u02 * _Ig0);
+    double _It3      = _F * _R2 * _R * (_Ig3 - _u0 * _Ig2 * 3 + _u02 * _Ig1 * 3 - _u0 * _u02 * _Ig0);
+    *P2              = _NormG * (_It0 - 0.5 * _It2);
+    *P4              = _NormG * (_It1 - R1o6 * _It3);
+    double _u0py = _1o2SqrtA * (_B - ymixing * _Gamma + selbias);
+    double _u0my = _1o2SqrtA * (_B + ymixing * _Gamma + selbias);
+    double _Fpy  = _1oSqrtA * exp(-_C + _u0py * _u0py);
+    double _Fmy  = _1oSqrtA * exp(-_C + _u0my * _u0my);
+    double _Ipy  = _Fpy * SQRTPIo2 * erfc(_u0py);
+    double _Imy  = _Fmy * SQRTPIo2 * erfc(_u0my);
+    *P1          = _NormG * 0.5 * (_Ipy + _Imy);
+    *P3          = _NormG * 0.5 * (_Ipy - _Imy);
+}
+
+/* device_threegauss_resolution, src/PDFs/physics/ThreeGaussResolution.cu:63-127 */
+static double res_threegauss(const gfo_prog *g, const pcur *pc, double coshterm, double costerm, double sinhterm,
+                             double sinterm, double tau, double dtime, double xmixing, double ymixing,
+                             double sigma) {
+    double coreFraction    = P(g, pc, 0);
+    double tailFraction    = P(g, pc, 1);
+    double outlFraction    = 1 - coreFraction - tailFraction;
+    double coreBias        = P(g, pc, 2);
+    double coreScaleFactor = P(g, pc, 3);
+    double tailBias        = P(g, pc, 4);
+    double tailScaleFactor = P(g, pc, 5);
+    double outlBias        = P(g, pc, 6);
+    double outlScaleFactor = P(g, pc, 7);
+    double selbias         = P(g, pc, 8);
+    double cbo = P(g, pc, 9), cws = P(g, pc, 10), cbo2 = P(g, pc, 11), cws2 = P(g, pc, 12);
+    coreBias += cbo;
+    tailBias += cbo2;
+    outlBias += cbo2;
+    coreScaleFactor *= cws;
+    tailScaleFactor *= cws2;
+    outlScaleFactor *= cws2;
+    double cp1 = 0, cp2 = 0, cp3 = 0, cp4 = 0, tp1 = 0, tp2 = 0, tp3 = 0, tp4 = 0, op1 = 0, op2 = 0, op3 = 0, op4 = 0;
+    tg_gaussian(&cp1, &cp2, &cp3, &cp4, tau, dtime - coreBias * sigma, xmixing, ymixing, coreScaleFactor * sigma,
+                selbias);
+    tg_gaussian(&tp1, &tp2, &tp3, &tp4, tau, dtime - tailBias * sigma, xmixing, ymixing, tailScaleFactor * sigma,
+                selbias);
+    tg_gaussian(&op1, &op2, &op3, &op4, tau, dtime - outlBias * sigma, xmixing, ymixing, outlScaleFactor * sigma,
+                selbias);
+    double _P1 = coreFraction * cp1 + tailFraction * tp1 + outlFraction * op1;
+    double _P2 = coreFraction * cp2 + tailFraction * tp2 + outlFraction * op2;
+    double _P3 = coreFraction * cp3 + tailFraction * tp3 + outlFraction * op3;
+    double _P4 = coreFraction * cp4 + tailFraction * tp4 + outlFraction * op4;
+    double ret = 0;
+    ret += coshterm * _P1;
+    ret += costerm * _P2;
+    ret -= 2 * sinhterm * _P3;
+    ret -= 2 * sinterm * _P4;
+    return ret;
+}
+
+static double call_resolution(gfo_prog *g, const pcur *pc, double t1, double t2, double abr, double abi, double tau,
+                              double dtime, double x, double y, double sigma) {
+    switch(g->funcs[pc->funcIdx]) {
+    case GF_FN_TRUTH_RES: return res_truth(t1, t2, abr, abi, tau, dtime, x, y);
+    case GF_FN_THREEGAUSS_RES: return res_threegauss(g, pc, t1, t2, abr, abi, tau, dtime, x, y, sigma);
+    default: g->error = 5; return 0;
+    }
+}
+
+/* device_Tddp, src/PDFs/physics/Amp3Body_TD.cu:41-257: params {tau, x, y, dx, dy,
+ * (re_i, im_i)...}, constants {nres, cacheToUse, mistag flag, end of efficiency},
+ * observables {dtime, sigma_t, m12, m13, event number[, mistag[, charm tag]]};
+ * components: the resonances, the resolution, the efficiency. */
+static double f_tddp(gfo_prog *g, const double *evt, pcur *pc) {
+    int num_parameters  = NP(g, pc);
+    int num_constants   = NC(g, pc);
+    int num_observables = NO(g, pc);
+    int id_m12 = (int)O(g, pc, 2), id_m13 = (int)O(g, pc, 3), id_num = (int)O(g, pc, 4);
+    int id_mis = 0, id_tag = 0;
+    if(num_observables > 5)
+        id_mis = (int)O(g, pc, 5);
+    double m12 = evt[id_m12], m13 = evt[id_m13];
+    unsigned numResonances = (unsigned)C(g, pc, 0);
+    if(!gfo_in_dalitz(m12, m13, g->motherMass, g->daug1Mass, g->daug2Mass, g->daug3Mass)) {
+        unsigned endEfficiencyFunc = (unsigned)C(g, pc, 3);
+        inc5(pc, 1, num_parameters, num_constants, num_observables, 1);
+        for(unsigned i = 0; i < numResonances; i++)
+            inc(g, pc);
+        inc(g, pc);
+        while(pc->funcIdx < (int)endEfficiencyFunc)
+            inc(g, pc);
+        return 0;
+    }
+    int evtNum = (int)floor(0.5 + evt[id_num]);
+    cplx sumWavesA = cmake(0, 0), sumWavesB = cmake(0, 0);
+    unsigned cacheToUse = (unsigned)C(g, pc, 1);
+    for(unsigned i = 0; i < numResonances; ++i) {
+        cplx amp      = cmake(P(g, pc, (int)i * 2 + 5), P(g, pc, (int)i * 2 + 6));
+        unsigned slot = i + 16 * cacheToUse;
+        const double *wa = slot < GFO_MAX_CACHE ? g->cache[slot] : NULL;
+        const double *wb = slot < GFO_MAX_CACHE ? g->cache_b[slot] : NULL;
+        if(!wa || !wb || evtNum < 0 || evtNum >= g->cache_len[slot]) {
+            g->error = 2;
+            continue;
+        }
+        cplx ai   = cmake(wa[2 * (int64_t)evtNum], wa[2 * (int64_t)evtNum + 1]);
+        cplx bi   = cmake(wb[2 * (int64_t)evtNum], wb[2 * (int64_t)evtNum + 1]);
+        sumWavesA = cadd(sumWavesA, cmul(ai, amp));
+        sumWavesB = cadd(sumWavesB, cmul(bi, amp));
+    }
+    int id_time = (int)O(g, pc, 0), id_sigma = (int)O(g, pc, 1);
+    if(num_observables > 6)
+        id_tag = (int)O(g, pc, 6);
+    double _tau = P(g, pc, 0), _xmixing0 = P(g, pc, 1), _ymixing0 = P(g, pc, 2), _deltax = P(g, pc, 3),
+           _deltay = P(g, pc, 4);
+    double _xmixing, _ymixing;
+    int _charmtag = 0;
+    if(num_observables > 6)
+        _charmtag = (int)evt[id_tag];
+    if(_charmtag == 1) {
+        _xmixing = _xmixing0 + _deltax;
+        _ymixing = _ymixing0 + _deltay;
+    } else if(_charmtag == -1) {
+        _xmixing = _xmixing0 - _deltax;
+        _ymixing = _ymixing0 - _deltay;
+    } else {
+        _xmixing = _xmixing0;
+        _ymixing = _ymixing0;
+    }
+    double _time = evt[id_time], _sigma = evt[id_sigma];
+    double nA = sumWavesA.re * sumWavesA.re + sumWavesA.im * sumWavesA.im;
+    double nB = sumWavesB.re * sumWavesB.re + sumWavesB.im * sumWavesB.im;
+    double term1 = nA + nB;
+    double term2 = nA - nB;
+    sumWavesA    = cmul(sumWavesA, cconj(sumWavesB));
+    inc5(pc, 1, num_parameters, num_constants, num_observables, 1);
+    for(unsigned i = 0; i < numResonances; i++)
+        inc(g, pc);
+    double ret = call_resolution(g, pc, term1, term2, sumWavesA.re, sumWavesA.im, _tau, _time, _xmixing, _ymixing,
+                                 _sigma);
+    double mistag = 0;
+    if(num_observables > 5)
+        mistag = evt[id_mis];
+    const double xfix = 0.0039, yfix = 0.0065, taufix = 0.4101;
+    if(mistag > 0) {
+        ret *= (1 - 2 * mistag);
+        ret += mistag * call_resolution(g, pc, term1, -term2, sumWavesA.re, -sumWavesA.im, taufix, _time, xfix, yfix,
+                                        _sigma);
+        ret += mistag * call_resolution(g, pc, term1, term2, sumWavesA.re, sumWavesA.im, taufix, _time, xfix, yfix,
+                                        _sigma);
+    }
+    inc(g, pc);
+    double eff = call_function(g, evt, pc);
+    ret *= eff;
+    return ret;
+}
+
 /* callFunction, src/PDFs/GooPdf.cu:398-400: dispatch on the function table entry */
 static double call_function(gfo_prog *g, const double *evt, pcur *pc) {
     if(pc->funcIdx < 0 || pc->funcIdx >= g->nfunc) {
@@ -624,6 +818,7 @@ static double call_function(gfo_prog *g, const double *evt, pcur *pc) {
     case GF_FN_ADD_EXT: return f_add_ext(g, evt, pc);
     case GF_FN_PROD: return f_prod(g, evt, pc);
     case GF_FN_DALITZ: return f_dalitz(g, evt, pc);
+    case GF_FN_TDDP: return f_tddp(g, evt, pc);
     default: g->error = 4; return 0;
     }
 }
@@ -725,6 +920,11 @@ void gfo_prog_set_cache(gfo_prog *g, int slot, const double *re_im, int64_t len)
         g->cache[slot]     = re_im;
         g->cache_len[slot] = len;
     }
+}
+/* second wave of an Amp3Body_TD cache slot: B(m13, m12) beside A(m12, m13) */
+void gfo_prog_set_cache_b(gfo_prog *g, int slot, const double *re_im) {
+    if(slot >= 0 && slot < GFO_MAX_CACHE)
+        g->cache_b[slot] = re_im;
 }
 int gfo_prog_error(const gfo_prog *g) { return g->error; }
 void gfo_prog_set_total_events(gfo_prog *g, double n) { g->total_events = n; }
@@ -956,6 +1156,8 @@ static void skip_subtree(gfo_prog *g, pcur *pc) {
         nchild = (int)C(g, pc, 0);
     else if(f == GF_FN_DALITZ)
         nchild = (int)C(g, pc, 0) + 1;
+    else if(f == GF_FN_TDDP)
+        nchild = (int)C(g, pc, 0) + 2; /* resonances, resolution, efficiency */
     inc(g, pc);
     for(int i = 0; i < nchild; i++)
         skip_subtree(g, pc);
@@ -1090,6 +1292,260 @@ double gfo_dalitz_norm_from_integrals(int nres, const double *coup, const double
         }
     }
     double ret           = sumIntegral.re;
+    double binSizeFactor = 1;
+    binSizeFactor *= binSize12;
+    binSizeFactor *= binSize13;
+    ret *= binSizeFactor;
+    return ret;
+}
+
+/* ------------------------------------------------------- Amp3Body_TD host side */
+
+/* Layout of an Amp3Body_TD node: its resonances, the resolution function, the
+ * root of the efficiency subtree (efficiencyFunction, Amp3Body_TD.cu:505-512)
+ * and the function index after that subtree (the constant the device function
+ * walks to, Amp3Body_TD.cu:514-519).  Returns the number of resonances. */
+int gfo_tddp_layout(gfo_prog *g, int tddp_func, int *res_funcs, int *res_func, int *eff_func, int *end_func) {
+    pcur pc = {0, 0, 0, 0, 0};
+    while(pc.funcIdx < tddp_func)
+        inc(g, &pc);
+    int nres = (int)C(g, &pc, 0);
+    inc(g, &pc);
+    for(int i = 0; i < nres; i++) {
+        res_funcs[i] = pc.funcIdx;
+        inc(g, &pc);
+    }
+    *res_func = pc.funcIdx;
+    skip_subtree(g, &pc);
+    *eff_func = pc.funcIdx;
+    skip_subtree(g, &pc);
+    *end_func = pc.funcIdx;
+    return nres;
+}
+
+/* (A, B) waves of one resonance at one point: A = BW(m12, m13), B = BW(m13, m12)
+ * with m23 taken once from (m12, m13); both 0 outside the Dalitz region.
+ * SpecialWaveCalculator::operator(), detail/SpecialWaveCalculator.cu:13-63. */
+static void tddp_ab(gfo_prog *g, const pcur *rpc, double m12, double m13, cplx *a, cplx *b) {
+    *a = cmake(0, 0);
+    *b = cmake(0, 0);
+    if(!gfo_in_dalitz(m12, m13, g->motherMass, g->daug1Mass, g->daug2Mass, g->daug3Mass))
+        return;
+    double m23 = g->motherMass * g->motherMass + g->daug1Mass * g->daug1Mass + g->daug2Mass * g->daug2Mass
+                 + g->daug3Mass * g->daug3Mass - m12 - m13;
+    pcur t = *rpc;
+    *a     = resonance_amplitude(g, m12, m13, m23, &t);
+    t      = *rpc;
+    *b     = resonance_amplitude(g, m13, m12, m23, &t);
+}
+
+/* Wave cache of one resonance of a TD node over event rows offset..offset+n-1:
+ * the m12/m13 slots are the node's observables 2 and 3 (Amp3Body_TD.cu:581-620). */
+void gfo_tddp_wave(gfo_prog *g, int tddp_func, int res_func, const double *aos, int64_t n, int evtSize,
+                   int64_t offset, double *out_a, double *out_b) {
+    pcur pc = {0, 0, 0, 0, 0};
+    while(pc.funcIdx < tddp_func)
+        inc(g, &pc);
+    int id_m12 = (int)O(g, &pc, 2);
+    int id_m13 = (int)O(g, &pc, 3);
+    pcur rpc   = pc;
+    while(rpc.funcIdx < res_func)
+        inc(g, &rpc);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for(int64_t k = 0; k < n; k++) {
+        const double *evt = aos + (offset + k) * evtSize;
+        cplx a, b;
+        tddp_ab(g, &rpc, evt[id_m12], evt[id_m13], &a, &b);
+        out_a[2 * k]     = a.re;
+        out_a[2 * k + 1] = a.im;
+        out_b[2 * k]     = b.re;
+        out_b[2 * k + 1] = b.im;
+    }
+}
+
+/* ThreeComplex integrals of every resonance pair over the m12 x m13 midpoint
+ * grid (Amp3Body_TD.cu:620-645 with SpecialDalitzIntegrator.cu:9-126): per cell
+ * (ai aj*, ai bj*, bi bj*) times the efficiency evaluated on a fake event
+ * {2, ..., m12, m13} (other slots 0 here); the m13 bin count is grid6[5].  Each
+ * pair's sum is OMP-chunked like the other oracle sums.  out_*: nres*nres
+ * complex, row-major. */
+void gfo_tddp_integrals(gfo_prog *g, int tddp_func, const double *grid6, int P, double *out_aa, double *out_ab,
+                        double *out_bb) {
+    int res_funcs[64], res_func, eff_func, end_func;
+    int nres = gfo_tddp_layout(g, tddp_func, res_funcs, &res_func, &eff_func, &end_func);
+    pcur dpc = {0, 0, 0, 0, 0};
+    while(dpc.funcIdx < tddp_func)
+        inc(g, &dpc);
+    int id_m12 = (int)O(g, &dpc, 2);
+    int id_m13 = (int)O(g, &dpc, 3);
+    pcur rpc[64];
+    for(int r = 0; r < nres; r++) {
+        rpc[r] = dpc;
+        while(rpc[r].funcIdx < res_funcs[r])
+            inc(g, &rpc[r]);
+    }
+    pcur epc = dpc;
+    while(epc.funcIdx < eff_func)
+        inc(g, &epc);
+    int n12           = (int)floor(grid6[2] + 0.5);
+    int n13           = (int)floor(grid6[5] + 0.5);
+    int64_t totalBins = (int64_t)n12 * (int64_t)n13;
+    int nch           = gfo_num_chunks(totalBins, P);
+    int npair         = nres * nres;
+    cplx *chunk       = (cplx *)calloc((size_t)nch * npair * 3, sizeof(cplx));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for(int c = 0; c < nch; c++) {
+        int64_t b0, e0;
+        chunk_bounds(totalBins, P, c, &b0, &e0);
+        cplx A[64], B[64];
+        cplx *acc = chunk + (size_t)c * npair * 3;
+        for(int64_t bin = b0; bin < e0; bin++) {
+            int globalBinNumber  = (int)bin;
+            double lowerBoundM12 = grid6[0];
+            double upperBoundM12 = grid6[1];
+            int numBinsM12       = (int)floor(grid6[2] + 0.5);
+            int binNumberM12     = globalBinNumber % numBinsM12;
+            double binCenterM12  = upperBoundM12 - lowerBoundM12;
+            binCenterM12 /= numBinsM12;
+            binCenterM12 *= (binNumberM12 + 0.5);
+            binCenterM12 += lowerBoundM12;
+            globalBinNumber /= numBinsM12;
+            double lowerBoundM13 = grid6[3];
+            double upperBoundM13 = grid6[4];
+            int numBinsM13       = (int)floor(grid6[5] + 0.5);
+            double binCenterM13  = upperBoundM13 - lowerBoundM13;
+            binCenterM13 /= numBinsM13;
+            binCenterM13 *= (globalBinNumber + 0.5);
+            binCenterM13 += lowerBoundM13;
+            for(int r = 0; r < nres; r++)
+                tddp_ab(g, &rpc[r], binCenterM12, binCenterM13, &A[r], &B[r]);
+            double events[16];
+            memset(events, 0, sizeof(events));
+            events[0] = 2;
+            if(id_m12 >= 0 && id_m12 < 16)
+                events[id_m12] = binCenterM12;
+            if(id_m13 >= 0 && id_m13 < 16)
+                events[id_m13] = binCenterM13;
+            pcur e  = epc;
+            double eff = call_function(g, events, &e);
+            for(int i = 0; i < nres; i++) {
+                for(int j = 0; j < nres; j++) {
+                    cplx v[3];
+                    v[0] = cscale(cmul(A[i], cconj(A[j])), eff);
+                    v[1] = cscale(cmul(A[i], cconj(B[j])), eff);
+                    v[2] = cscale(cmul(B[i], cconj(B[j])), eff);
+                    for(int t = 0; t < 3; t++) {
+                        cplx *dst = acc + (size_t)t * npair + i * nres + j;
+                        *dst      = (bin == b0) ? v[t] : cadd(*dst, v[t]);
+                    }
+                }
+            }
+        }
+    }
+    double *outs[3] = {out_aa, out_ab, out_bb};
+    for(int t = 0; t < 3; t++)
+        for(int p = 0; p < npair; p++) {
+            cplx s = cmake(0, 0);
+            for(int c = 0; c < nch; c++)
+                s = cadd(s, chunk[((size_t)c * 3 + t) * npair + p]);
+            outs[t][2 * p]     = s.re;
+            outs[t][2 * p + 1] = s.im;
+        }
+    free(chunk);
+}
+
+/* TruthResolution::normalization, src/PDFs/physics/TruthResolution.cu:58-71 */
+static double norm_truth(double di1, double di2, double di3, double di4, double tau, double xmixing,
+                         double ymixing) {
+    double timeIntegralOne = tau / (1 - ymixing * ymixing);
+    double timeIntegralTwo = tau / (1 + xmixing * xmixing);
+    double timeIntegralThr = ymixing * timeIntegralOne;
+    double timeIntegralFou = xmixing * timeIntegralTwo;
+    double ret             = timeIntegralOne * (di1 + di2);
+    ret += timeIntegralTwo * (di1 - di2);
+    ret -= 2 * timeIntegralThr * di3;
+    ret -= 2 * timeIntegralFou * di4;
+    return ret;
+}
+
+/* ThreeGaussResolution::normalization, src/PDFs/physics/ThreeGaussResolution.cu:153-182 */
+static double norm_threegauss(double di1, double di2, double di3, double di4, double tau, double xmixing,
+                              double ymixing, double selBias) {
+    double timeIntegralOne
+        = (selBias + 1 / tau) / (selBias * selBias + 2 * selBias / tau + (1 - ymixing * ymixing) / (tau * tau));
+    double timeIntegralTwo
+        = (selBias + 1 / tau) / (selBias * selBias + 2 * selBias / tau + (1 + xmixing * xmixing) / (tau * tau));
+    double timeIntegralThr
+        = (ymixing / tau) / (selBias * selBias + 2 * selBias / tau + (1 - ymixing * ymixing) / (tau * tau));
+    double timeIntegralFou
+        = (xmixing / tau) / (selBias * selBias + 2 * selBias / tau + (1 + xmixing * xmixing) / (tau * tau));
+    double ret = timeIntegralOne * (di1 + di2);
+    ret += timeIntegralTwo * (di1 - di2);
+    ret -= 2 * timeIntegralThr * di3;
+    ret -= 2 * timeIntegralFou * di4;
+    return ret;
+}
+
+/* MixingTimeResolution::normalization of a resolution kind (a GF_FN_*_RES code);
+ * selBias is the ThreeGauss selection-bias value. */
+double gfo_resolution_norm(int kind, double di1, double di2, double di3, double di4, double tau, double x, double y,
+                           double selBias) {
+    if(kind == GF_FN_THREEGAUSS_RES)
+        return norm_threegauss(di1, di2, di3, di4, tau, x, y, selBias);
+    return norm_truth(di1, di2, di3, di4, tau, x, y);
+}
+
+/* Resolution function value at one decay time (the device functions above):
+ * params13 are the ThreeGauss parameters (ignored for the truth resolution). */
+double gfo_resolution_eval(int kind, const double *params13, double t1, double t2, double abr, double abi,
+                           double tau, double dtime, double x, double y, double sigma) {
+    if(kind == GF_FN_THREEGAUSS_RES) {
+        gfo_prog g;
+        memset(&g, 0, sizeof(g));
+        double pblock[14];
+        pblock[0] = 13;
+        memcpy(pblock + 1, params13, 13 * sizeof(double));
+        g.params = pblock;
+        pcur pc  = {0, 0, 0, 0, 0};
+        return res_threegauss(&g, &pc, t1, t2, abr, abi, tau, dtime, x, y, sigma);
+    }
+    return res_truth(t1, t2, abr, abi, tau, dtime, x, y);
+}
+
+/* Amp3Body_TD::normalize host assembly (Amp3Body_TD.cu:677-748).  coup: nres
+ * (re, im); I*: the ThreeComplex integrals; mix5 = {tau, x, y, dx, dy};
+ * mistag_on / mistag: whether the mistag observable tests true (its value is
+ * in range, include/goofit/Variable.h:103-105) and its value. */
+double gfo_tddp_norm_from_integrals(int nres, const double *coup, const double *Iaa, const double *Iab,
+                                    const double *Ibb, const double *mix5, int res_kind, double selBias,
+                                    double d0fraction, int mistag_on, double mistag, double binSize12,
+                                    double binSize13) {
+    cplx integralA_2 = cmake(0, 0), integralB_2 = cmake(0, 0), integralABs = cmake(0, 0);
+    for(int i = 0; i < nres; ++i) {
+        cplx amplitude_i = cmake(coup[2 * i], coup[2 * i + 1]);
+        for(int j = 0; j < nres; ++j) {
+            cplx amplitude_j = cmake(coup[2 * j], -coup[2 * j + 1]);
+            int k            = 2 * (i * nres + j);
+            integralA_2 = cadd(integralA_2, cmul(cmul(amplitude_i, amplitude_j), cmake(Iaa[k], Iaa[k + 1])));
+            integralABs = cadd(integralABs, cmul(cmul(amplitude_i, amplitude_j), cmake(Iab[k], Iab[k + 1])));
+            integralB_2 = cadd(integralB_2, cmul(cmul(amplitude_i, amplitude_j), cmake(Ibb[k], Ibb[k + 1])));
+        }
+    }
+    double one = integralA_2.re, two = integralB_2.re, thr = integralABs.re, fou = integralABs.im;
+    double tau = mix5[0], x0 = mix5[1], y0 = mix5[2], dx = mix5[3], dy = mix5[4];
+    double ret_D0    = gfo_resolution_norm(res_kind, one, two, thr, fou, tau, x0 + dx, y0 + dy, selBias);
+    double ret_D0bar = gfo_resolution_norm(res_kind, one, two, thr, fou, tau, x0 - dx, y0 - dy, selBias);
+    double ret       = d0fraction * ret_D0 + (1. - d0fraction) * ret_D0bar;
+    const double xfix = 0.0039, yfix = 0.0065, taufix = 0.4101;
+    if(mistag_on) {
+        ret *= (1 - 2 * mistag);
+        ret += mistag * gfo_resolution_norm(res_kind, one, two, thr, fou, taufix, xfix, yfix, selBias);
+        ret += mistag * gfo_resolution_norm(res_kind, two, one, thr, -fou, taufix, xfix, yfix, selBias);
+    }
     double binSizeFactor = 1;
     binSizeFactor *= binSize12;
     binSizeFactor *= binSize13;

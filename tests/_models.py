@@ -66,3 +66,50 @@ def dalitz_events(n, seed=5, outside=0.1):
 def arrays(top):
     prog = O.flatten(top)
     return prog, prog.arrays()
+
+
+TD_MIX = (0.4101, 0.0016, 0.0055, 0.0, 0.0)  # tau, x, y, dx, dy (examples/pipipi0DPFit/pipipi0DPFit.cpp:526-530)
+
+
+def threegauss(cf=0.7, tf=0.2, cb=0.05, cs=1.1, tb=-0.1, ts=2.0, ob=0.0, os_=6.0, sb=0.0, cbo=0.01, cws=1.05,
+               cbo2=0.0, cws2=1.0):
+    v = [O.Var(n, x) for n, x in (("cf", cf), ("tf", tf), ("cb", cb), ("cs", cs), ("tb", tb), ("ts", ts),
+                                  ("ob", ob), ("os", os_), ("sb", sb), ("cbo", cbo), ("cws", cws),
+                                  ("cbo2", cbo2), ("cws2", cws2))]
+    return O.ThreeGaussResolution("tg", *v)
+
+
+def tddp(coup_seed=3, nbins=120, resonances=None, mix=TD_MIX, resolution=None, mistag=False, charmtag=False,
+         eff=None):
+    """Amp3Body_TD over (dtime, sigma_t, m12, m13, evt[, mistag][, tag]) with the config-2 resonances."""
+    dtime = O.Obs("dtime", -2.0, 4.0, 100)
+    sigma = O.Obs("sigmat", 0.0, 0.8, 100)
+    m12 = O.Obs("m12", 0.0, 3.0, nbins)
+    m13 = O.Obs("m13", 0.0, 3.0, nbins)
+    num = O.Obs("evt", 0.0, 1e9, 1, event_number=True)
+    mt = O.Obs("mistag", 0.0, 0.5, 10) if mistag else None
+    tag = O.Obs("tag", -1.0, 1.0, 2) if charmtag else None
+    table = resonances if resonances is not None else W.DALITZ_RESONANCES[:6]
+    base, _ = dalitz(coup_seed=coup_seed, nbins=nbins, resonances=table)
+    res = base.resonances
+    if eff is None:
+        eff = O.MultiPolynomial("eff", [m12, m13], [O.Var("one", 1.0)], [O.Var("z0", 0.0), O.Var("z1", 0.0)], 0)
+    mixv = [O.Var(n, v) for n, v in zip(("tau", "xmix", "ymix", "dx", "dy"), mix)]
+    resolution = resolution if resolution is not None else O.TruthResolution()
+    top = O.Tddp("td", dtime, sigma, m12, m13, num, W.DALITZ_DECAY, mixv, res, resolution, eff, mistag=mt,
+                 charmtag=tag)
+    return top, (dtime, sigma, m12, m13, num, mt, tag)
+
+
+def tddp_events(n, seed=7, tau=0.4101, sigma=0.05, mistag=None, tag=None, outside=0.05):
+    """AoS rows (dtime, sigma_t, m12, m13, evt[, mistag][, tag]): Dalitz points uniform,
+    decay times exponential smeared by sigma_t, event number = row."""
+    d = dalitz_events(n, seed=seed, outside=outside)
+    rng = np.random.default_rng(seed + 7)
+    t = rng.exponential(tau, size=n) + sigma * rng.standard_normal(n)
+    cols = [t, np.full(n, sigma), d[:, 0], d[:, 1], d[:, 2]]
+    if mistag is not None:
+        cols.append(np.full(n, mistag) if np.isscalar(mistag) else np.asarray(mistag, np.float64))
+    if tag is not None:
+        cols.append(np.asarray(tag, np.float64) if not np.isscalar(tag) else np.full(n, float(tag)))
+    return np.stack(cols, axis=1)
