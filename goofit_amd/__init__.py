@@ -1,6 +1,6 @@
 """goofit_amd -- GooFit's likelihood hot path on AMD Instinct MI355X (gfx950).
 
-The GooFit C++ API (GooPdf, AddPdf, ProdPdf, Amp3Body, UnbinnedDataSet,
+The GooFit C++ API (GooPdf, AddPdf, ProdPdf, Amp3Body, Amp3Body_TD, UnbinnedDataSet,
 FitManager, ...) is built as libgoofit.so on top of the C ABI of
 include/goofit_hip.h (libgoofit_hip.so, hand-written HIP kernels).  This package
 exposes that API to Python (the `_goofit` binding) and the raw C ABI (`capi`).
@@ -17,10 +17,11 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 try:
     from . import _goofit  # noqa: F401
     from ._goofit import (  # noqa: F401
-        AddPdf, Amp3Body, BatchFitter, BinnedChisqFit, BinnedDataSet, BinnedErrorFit, BinnedNllFit, DalitzProposal,
-        DataSet, EvalFit, FitControl, ProbFit, UnbinnedNllFit, DaughterPair, DecayInfo3, EvalFunc, EventNumber, ExpGausPdf, ExpPdf,
+        AddPdf, Amp3Body, Amp3Body_TD, BatchFitter, BinnedChisqFit, BinnedDataSet, BinnedErrorFit, BinnedNllFit, DalitzProposal,
+        DataSet, EvalFit, FitControl, ProbFit, UnbinnedNllFit, DaughterPair, DecayInfo3, DecayInfo3t, EvalFunc, EventNumber, ExpGausPdf, ExpPdf,
         FCN, FitManager, FunctionMinimum, GaussianPdf, GeneralError, GooPdf, Observable, PAIR_12, PAIR_13, PAIR_23,
-        Params, PdfBase, PolynomialPdf, ProdPdf, Resonances, UnbinnedDataSet, Variable, engine, inDalitz,
+        MixingTimeResolution, Params, PdfBase, PolynomialPdf, ProdPdf, Resonances, ThreeGaussResolution,
+        TruthResolution, UnbinnedDataSet, Variable, WaveHolder_s, engine, inDalitz,
     )
     HAVE_API = True
 except ImportError as _e:  # the binding is not built yet

@@ -19,6 +19,9 @@
 #include <goofit/PDFs/combine/AddPdf.h>
 #include <goofit/PDFs/combine/ProdPdf.h>
 #include <goofit/PDFs/physics/Amp3Body.h>
+#include <goofit/PDFs/physics/Amp3Body_TD.h>
+#include <goofit/PDFs/physics/ThreeGaussResolution.h>
+#include <goofit/PDFs/physics/TruthResolution.h>
 #include <goofit/PDFs/physics/DalitzProposal.h>
 #include <goofit/fitting/BatchFitter.h>
 #include <goofit/PDFs/physics/ResonancePdf.h>
@@ -402,6 +405,64 @@ PYBIND11_MODULE(_goofit, m) {
         .def("generateEvents", &Amp3Body::generateEvents, py::arg("out"), py::arg("n"), py::arg("seed"),
              py::arg("fmax") = 0.0)
         .def_static("resetCacheCounter", &Amp3Body::resetCacheCounter);
+
+    py::class_<DecayInfo3t, DecayInfo3>(m, "DecayInfo3t", py::dynamic_attr())
+        .def(py::init<Variable, Variable, Variable, Variable, Variable>(), py::arg("tau"), py::arg("xmixing"),
+             py::arg("ymixing"), py::arg("deltax"), py::arg("deltay"))
+        .def_readonly("_tau", &DecayInfo3t::_tau)
+        .def_readonly("_xmixing", &DecayInfo3t::_xmixing)
+        .def_readonly("_ymixing", &DecayInfo3t::_ymixing)
+        .def_readonly("_deltax", &DecayInfo3t::_deltax)
+        .def_readonly("_deltay", &DecayInfo3t::_deltay);
+
+    py::class_<WaveHolder_s>(m, "WaveHolder_s")
+        .def_readonly("ai_real", &WaveHolder_s::ai_real)
+        .def_readonly("ai_imag", &WaveHolder_s::ai_imag)
+        .def_readonly("bi_real", &WaveHolder_s::bi_real)
+        .def_readonly("bi_imag", &WaveHolder_s::bi_imag);
+
+    py::class_<MixingTimeResolution, GooPdf>(m, "MixingTimeResolution")
+        .def("normalization", &MixingTimeResolution::normalization, py::arg("di1"), py::arg("di2"), py::arg("di3"),
+             py::arg("di4"), py::arg("tau"), py::arg("xmixing"), py::arg("ymixing"))
+        .def("getDeviceFunction", &MixingTimeResolution::getDeviceFunction)
+        .def("getCalcTauIdx", &MixingTimeResolution::getCalcTauIdx)
+        .def("setCalcTauIdx", &MixingTimeResolution::setCalcTauIdx);
+    py::class_<TruthResolution, MixingTimeResolution>(m, "TruthResolution").def(py::init<>());
+    py::class_<ThreeGaussResolution, MixingTimeResolution>(m, "ThreeGaussResolution")
+        .def(py::init<Variable, Variable, Variable, Variable, Variable, Variable, Variable, Variable, Variable,
+                      Variable, Variable, Variable, Variable>(),
+             py::arg("cf"), py::arg("tf"), py::arg("cb"), py::arg("cs"), py::arg("tb"), py::arg("ts"), py::arg("ob"),
+             py::arg("os"), py::arg("sb"), py::arg("common_bias_offset"), py::arg("common_width_scale"),
+             py::arg("common_bias_offset2"), py::arg("common_width_scale2"));
+
+    py::class_<Amp3Body_TD, CombinePdf>(m, "Amp3Body_TD")
+        .def(py::init<std::string, Observable, Observable, Observable, Observable, EventNumber, DecayInfo3t,
+                      MixingTimeResolution *, GooPdf *>(),
+             py::arg("name"), py::arg("dtime"), py::arg("sigmat"), py::arg("m12"), py::arg("m13"),
+             py::arg("eventNumber"), py::arg("decay"), py::arg("r"), py::arg("eff"), py::keep_alive<1, 8>(),
+             py::keep_alive<1, 9>(), py::keep_alive<1, 10>())
+        .def(py::init<std::string, Observable, Observable, Observable, Observable, EventNumber, DecayInfo3t,
+                      MixingTimeResolution *, GooPdf *, Observable *, Observable *>(),
+             py::arg("name"), py::arg("dtime"), py::arg("sigmat"), py::arg("m12"), py::arg("m13"),
+             py::arg("eventNumber"), py::arg("decay"), py::arg("r"), py::arg("eff"), py::arg("mistag"),
+             py::arg("charmtag"), py::keep_alive<1, 8>(), py::keep_alive<1, 9>(), py::keep_alive<1, 10>())
+        .def(py::init<std::string, Observable, Observable, Observable, Observable, EventNumber, DecayInfo3t,
+                      std::vector<MixingTimeResolution *> &, GooPdf *, Observable, Observable *, Observable *>(),
+             py::arg("name"), py::arg("dtime"), py::arg("sigmat"), py::arg("m12"), py::arg("m13"),
+             py::arg("eventNumber"), py::arg("decay"), py::arg("r"), py::arg("eff"), py::arg("md0"),
+             py::arg("mistag") = nullptr, py::arg("charmtag") = nullptr)
+        .def("setDataSize", &Amp3Body_TD::setDataSize, py::arg("dataSize"), py::arg("evtSize") = 5,
+             py::arg("offset") = 0)
+        .def("setD0Fraction", &Amp3Body_TD::setD0Fraction)
+        .def("getD0Fraction", &Amp3Body_TD::getD0Fraction)
+        .def("setForceIntegrals", &Amp3Body_TD::setForceIntegrals, py::arg("f") = true)
+        .def("getCachedWaveNoCopy", &Amp3Body_TD::getCachedWaveNoCopy)
+        .def("getFractions", &Amp3Body_TD::getFractions)
+        .def("getIntegralsAA", &Amp3Body_TD::getIntegralsAA)
+        .def("getIntegralsAB", &Amp3Body_TD::getIntegralsAB)
+        .def("getIntegralsBB", &Amp3Body_TD::getIntegralsBB)
+        .def("getDecayInfo", &Amp3Body_TD::getDecayInfo, py::return_value_policy::reference_internal)
+        .def_static("resetCacheCounter", &Amp3Body_TD::resetCacheCounter);
 
     py::class_<DalitzProposal>(m, "DalitzProposal")
         .def(py::init<Observable, Observable, unsigned long long, int>(), py::arg("m12"), py::arg("m13"),

@@ -30,7 +30,7 @@ EXPORTS = [
     "gf_events_set_identity_column", "gf_program_set", "gf_params_update", "gf_norms_update",
     "gf_decay3_set", "gf_reduce_metric", "gf_eval_metric", "gf_eval_metric_device", "gf_reduce_bins",
     "gf_dalitz_set_data_size", "gf_dalitz_cache", "gf_dalitz_gram", "gf_dalitz_gram_points",
-    "gf_dalitz_gram_points_device", "gf_dalitz_get_cache", "gf_dalitz_cache_gram",
+    "gf_dalitz_gram_points_device", "gf_dalitz_get_cache", "gf_dalitz_cache_gram", "gf_tddp_gram",
     "gf_time_reduce_metric", "gf_time_reduce_sweep", "gf_time_dalitz_cache", "gf_time_dalitz_gram",
     "gf_sample_uniform", "gf_mlp_param_count", "gf_mlp_create", "gf_mlp_destroy", "gf_mlp_get_params",
     "gf_mlp_set_params", "gf_mlp_forward", "gf_mlp_gradient", "gf_mlp_train", "gf_mlp_build_proposal",
@@ -88,6 +88,7 @@ def lib():
                                                    _D]
         L.gf_dalitz_get_cache.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int64]
         L.gf_dalitz_cache_gram.argtypes = [c_p, ctypes.c_int, _D]
+        L.gf_tddp_gram.argtypes = [c_p, ctypes.c_int, _D, _D, _D, _D]
         L.gf_time_reduce_metric.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D]
         L.gf_time_reduce_sweep.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D, _D]
         _F = ctypes.POINTER(ctypes.c_float)
@@ -298,6 +299,13 @@ class Context:
         out = np.zeros(2 * n)
         _check(lib().gf_dalitz_get_cache(self.h, dalitz_func, res, _dp(out), n))
         return out.view(np.complex128)
+
+    def tddp_gram(self, tddp_func, grid6, nres):
+        """ThreeComplex integrals (sum eff a_i a_j*, a_i b_j*, b_i b_j*) of a time-dependent node."""
+        g = np.ascontiguousarray(grid6, np.float64)
+        outs = [np.zeros(2 * nres * nres) for _ in range(3)]
+        _check(lib().gf_tddp_gram(self.h, tddp_func, _dp(g), *(_dp(o) for o in outs)))
+        return tuple(o.view(np.complex128).reshape(nres, nres) for o in outs)
 
     def dalitz_cache_gram(self, dalitz_func, nres):
         out = np.zeros(2 * nres * nres)

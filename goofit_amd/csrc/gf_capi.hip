@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <chrono>
+#include <complex>
 #include <map>
 #include <string>
 #include <vector>
@@ -43,7 +44,8 @@ gf_status fail(gf_status code, const char *fmt, ...) {
 // Wave-cache storage of one Dalitz node, keyed by its cacheToUse constant as the
 // reference keys cResonances[16 * cacheToUse + i] (Amp3Body.cu:54, 222-233).
 struct WaveCache {
-    double2 *wave = nullptr;  // GF_MAX_RES x n complex
+    double2 *wave = nullptr;    // GF_MAX_RES x n complex
+    double2 *wave_b = nullptr;  // time-dependent nodes: the B waves, same layout
     int64_t n = 0, offset = 0;
 };
 
@@ -58,6 +60,13 @@ struct DalitzNode {
     int eff_root = -1;    // first function of the efficiency subtree
     int col_m12 = 0, col_m13 = 1, col_num = 2;
     int p_coup = 0;       // params index of (re, im) of the first coupling
+    // time-dependent node (Amp3Body_TD): columns, mixing parameters, resolution
+    bool td = false;
+    int col_time = -1, col_sigma = -1, col_mis = -1, col_tag = -1;
+    int p_mix = 0;        // params index of tau (x, y, dx, dy follow)
+    int res_func = -1;    // resolution function index
+    int res_kind = GT_TRUTH;
+    int res_p = 0;        // params index of the resolution's first parameter
 };
 
 }  // namespace
@@ -82,6 +91,7 @@ struct gf_ctx {
     std::map<int, int> sub_pc;        // function index -> first instruction
     int max_col = -1;                 // largest observable column any instruction reads
     std::vector<DalitzNode> dalitz;   // Dalitz nodes of the program, slot order
+    bool has_td = false;              // one of them is time-dependent
 
     // device copies
     double *d_params = nullptr, *d_consts = nullptr, *d_norms = nullptr;
@@ -421,11 +431,49 @@ struct Compiler {
             push(GF_LVL_OP(GK_DALITZ_END, lvl), lvl, 0, 0, 0, 0, 0, 0);
             return true;
         }
+        case GF_FN_TDDP: {
+            // device_Tddp walks incrementIndex(1, nP, nC, nO, 1), the resonances and the
+            // resolution, then evaluates the efficiency (Amp3Body_TD.cu:208-253)
+            if(!expect(q, -1, 4, -1, "Amp3Body_TD"))
+                return false;
+            const int nres = (int)ctx->consts[q.c + 1];
+            if(nres < 0 || nres > GF_MAX_RES || np(q) != 5 + 2 * nres || no(q) < 5 || no(q) > 7) {
+                err = "Amp3Body_TD: bad resonance/parameter/observable counts";
+                return false;
+            }
+            if(lvl >= GF_MAX_LVL) {
+                err = "PDF tree nests deeper than " + std::to_string(GF_MAX_LVL) + " combining levels";
+                return false;
+            }
+            int slot = -1;
+            for(size_t s = 0; s < ctx->dalitz.size(); s++)
+                if(ctx->dalitz[s].func == q.f)
+                    slot = (int)s;
+            if(slot < 0) {
+                err = "Amp3Body_TD node not registered";
+                return false;
+            }
+            const int pbase = q.p + 1;
+            if(!inc(q))
+                return false;
+            for(int i = 0; i < nres + 1; i++)  // the resonances, then the resolution
+                if(!inc(q))
+                    return false;
+            push(GF_LVL_OP(GK_TDDP, lvl), lvl, nres, pbase, 0, 0, 0, slot);
+            if(!emit(q, lvl + 1))
+                return false;
+            push(GF_LVL_OP(GK_TDDP_END, lvl), lvl, 0, 0, 0, 0, 0, slot);
+            return true;
+        }
         case GF_FN_RBW:
         case GF_FN_RBW_SYM:
         case GF_FN_GS:
         case GF_FN_NONRES:
-            err = "a resonance is only valid as a component of Amp3Body";
+            err = "a resonance is only valid as a component of Amp3Body or Amp3Body_TD";
+            return false;
+        case GF_FN_TRUTH_RES:
+        case GF_FN_THREEGAUSS_RES:
+            err = "a time resolution is only valid as a component of Amp3Body_TD";
             return false;
         default: err = "unknown function code " + std::to_string(f); return false;
         }
@@ -448,6 +496,8 @@ bool skip_subtree(Compiler &C, Cur &q) {
         nchild = C.nc(q) >= 1 ? (int)C.ctx->consts[q.c + 1] : 0;
     else if(f == GF_FN_DALITZ)
         nchild = C.nc(q) >= 1 ? (int)C.ctx->consts[q.c + 1] + 1 : 0;
+    else if(f == GF_FN_TDDP)  // resonances, resolution, efficiency
+        nchild = C.nc(q) >= 1 ? (int)C.ctx->consts[q.c + 1] + 2 : 0;
     if(!C.inc(q))
         return false;
     for(int i = 0; i < nchild; i++)
@@ -467,10 +517,12 @@ bool cursor_at(Compiler &C, int func, Cur &q) {
 // Discover the Dalitz nodes (resonance layout, efficiency index, observable columns).
 gf_status scan_dalitz(gf_ctx *ctx) {
     ctx->dalitz.clear();
+    ctx->has_td = false;
     Compiler C{ctx, {}, {}, -1};
     for(int f = 0; f < (int)ctx->funcs.size(); f++) {
-        if(ctx->funcs[f] != GF_FN_DALITZ)
+        if(ctx->funcs[f] != GF_FN_DALITZ && ctx->funcs[f] != GF_FN_TDDP)
             continue;
+        const bool td = ctx->funcs[f] == GF_FN_TDDP;
         if((int)ctx->dalitz.size() >= GF_MAX_DALITZ)
             return fail(GF_ERR_INVALID, "more than %d Amp3Body nodes in one tree", GF_MAX_DALITZ);
         Cur q;
@@ -478,21 +530,49 @@ gf_status scan_dalitz(gf_ctx *ctx) {
             return fail(GF_ERR_INVALID, "%s", C.err.c_str());
         DalitzNode d;
         d.func = f;
-        if(C.nc(q) != 2 || C.no(q) < 3)
+        d.td   = td;
+        if(!td && (C.nc(q) != 2 || C.no(q) < 3))
             return fail(GF_ERR_INVALID, "Amp3Body at %d: expected 2 constants and >= 3 observables", f);
-        d.nres      = (int)ctx->consts[q.c + 1];
-        d.cache_key = (int)ctx->consts[q.c + 2];
+        if(td && (C.nc(q) != 4 || C.no(q) < 5 || C.no(q) > 7))
+            return fail(GF_ERR_INVALID, "Amp3Body_TD at %d: expected 4 constants and 5 to 7 observables", f);
+        d.nres = (int)ctx->consts[q.c + 1];
+        // cacheToUse counts Amp3Body and Amp3Body_TD objects separately (Amp3Body.cu:157,
+        // Amp3Body_TD.cu:313), so the two kinds key their caches apart
+        d.cache_key = (int)ctx->consts[q.c + 2] + (td ? (1 << 20) : 0);
         if(d.nres < 0 || d.nres > GF_MAX_RES)
             return fail(GF_ERR_INVALID, "Amp3Body at %d: %d resonances (at most %d)", f, d.nres, GF_MAX_RES);
-        int c12, c13, cnum;
-        if(!C.col_of(q, 0, c12) || !C.col_of(q, 1, c13) || !C.col_of(q, 2, cnum))
-            return fail(GF_ERR_INVALID, "%s", C.err.c_str());
-        d.col_m12 = c12;
-        d.col_m13 = c13;
-        d.col_num = cnum;
-        d.p_coup  = q.p + 1;
-        if(C.np(q) < 2 * d.nres)
-            return fail(GF_ERR_INVALID, "Amp3Body at %d: %d parameters for %d couplings", f, C.np(q), d.nres);
+        if(td) {
+            // observables dtime, sigma_t, m12, m13, event number[, mistag[, charm tag]]
+            // (Amp3Body_TD.cu:262-295)
+            int ct, cs, c12, c13, cnum;
+            if(!C.col_of(q, 0, ct) || !C.col_of(q, 1, cs) || !C.col_of(q, 2, c12) || !C.col_of(q, 3, c13)
+               || !C.col_of(q, 4, cnum))
+                return fail(GF_ERR_INVALID, "%s", C.err.c_str());
+            d.col_time  = ct;
+            d.col_sigma = cs;
+            d.col_m12   = c12;
+            d.col_m13   = c13;
+            d.col_num   = cnum;
+            if(C.no(q) > 5 && !C.col_of(q, 5, d.col_mis))
+                return fail(GF_ERR_INVALID, "%s", C.err.c_str());
+            if(C.no(q) > 6 && !C.col_of(q, 6, d.col_tag))
+                return fail(GF_ERR_INVALID, "%s", C.err.c_str());
+            if(C.np(q) != 5 + 2 * d.nres)
+                return fail(GF_ERR_INVALID, "Amp3Body_TD at %d: %d parameters for %d couplings", f, C.np(q), d.nres);
+            d.p_mix  = q.p + 1;
+            d.p_coup = q.p + 1 + 5;
+            ctx->has_td = true;
+        } else {
+            int c12, c13, cnum;
+            if(!C.col_of(q, 0, c12) || !C.col_of(q, 1, c13) || !C.col_of(q, 2, cnum))
+                return fail(GF_ERR_INVALID, "%s", C.err.c_str());
+            d.col_m12 = c12;
+            d.col_m13 = c13;
+            d.col_num = cnum;
+            d.p_coup  = q.p + 1;
+            if(C.np(q) < 2 * d.nres)
+                return fail(GF_ERR_INVALID, "Amp3Body at %d: %d parameters for %d couplings", f, C.np(q), d.nres);
+        }
         if(!C.inc(q))
             return fail(GF_ERR_INVALID, "%s", C.err.c_str());
         for(int i = 0; i < d.nres; i++) {
@@ -516,10 +596,33 @@ gf_status scan_dalitz(gf_ctx *ctx) {
             if(!C.inc(q))
                 return fail(GF_ERR_INVALID, "%s", C.err.c_str());
         }
+        if(td) {
+            // the resolution follows the resonances (Amp3Body_TD.cu:514-515)
+            if(!C.in_range(q))
+                return fail(GF_ERR_INVALID, "Amp3Body_TD at %d: no resolution function", f);
+            const int rf = ctx->funcs[q.f];
+            if(rf == GF_FN_TRUTH_RES) {
+                if(C.np(q) != 0 || C.nc(q) != 0 || C.no(q) != 0)
+                    return fail(GF_ERR_INVALID, "TruthResolution at %d: expected no parameters", q.f);
+                d.res_kind = GT_TRUTH;
+            } else if(rf == GF_FN_THREEGAUSS_RES) {
+                if(C.np(q) != 13 || C.nc(q) != 0 || C.no(q) != 0)
+                    return fail(GF_ERR_INVALID, "ThreeGaussResolution at %d: expected 13 parameters", q.f);
+                d.res_kind = GT_THREEGAUSS;
+            } else {
+                return fail(GF_ERR_INVALID, "Amp3Body_TD at %d: component %d is not a time resolution", f, q.f);
+            }
+            d.res_func = q.f;
+            d.res_p    = q.p + 1;
+            if(!C.inc(q))
+                return fail(GF_ERR_INVALID, "%s", C.err.c_str());
+        }
         d.eff_root = q.f;
         if(!skip_subtree(C, q))
             return fail(GF_ERR_INVALID, "%s", C.err.c_str());
-        d.eff_func = q.f - 1;
+        // Amp3Body: the last function of the subtree (Amp3Body.cu:198-202);
+        // Amp3Body_TD: the root of the efficiency subtree (Amp3Body_TD.cu:518)
+        d.eff_func = td ? d.eff_root : q.f - 1;
         ctx->dalitz.push_back(d);
     }
     return GF_OK;
@@ -539,6 +642,13 @@ gf_status upload_dalitz_table(gf_ctx *ctx) {
         slots[s].col_m13    = d.col_m13;
         slots[s].col_num    = d.col_num;
         slots[s].nres       = d.nres;
+        slots[s].wave_b     = it != ctx->caches.end() ? it->second.wave_b : nullptr;
+        slots[s].col_time   = d.col_time;
+        slots[s].col_sigma  = d.col_sigma;
+        slots[s].col_mis    = d.col_mis;
+        slots[s].col_tag    = d.col_tag;
+        slots[s].res_kind   = d.res_kind;
+        slots[s].res_p      = d.res_p;
         for(int i = 0; i < d.nres; i++)
             res[s * GF_MAX_RES + i] = d.res[i];
     }
@@ -626,6 +736,7 @@ GfProgArgs prog_args(gf_ctx *ctx) {
     a.nobs         = ctx->ncols;
     a.total_events = (double)ctx->n;
     a.decay        = ctx->decay;
+    a.td           = ctx->has_td ? 1 : 0;
     a.err          = ctx->d_err;
     a.seq          = ctx->res_seq;
     a.mark         = nullptr;
@@ -870,8 +981,13 @@ void tiles_to_matrix(gf_ctx *ctx, const double *tiles, int nres, double *out) {
         }
 }
 
+// Gram of the node's lineshapes over grid cells (mode 0), points (1) or its data
+// cache (2) into out (nres x nres complex).  stack/nstack (device array of at most
+// GF_MAX_RES descriptors) replaces the node's resonance list when given;
+// td_grid selects the time-dependent grid's m13 bin count.
 gf_status run_gram(gf_ctx *ctx, const DalitzNode &d, int slot, int mode, const double *grid6, const double *m12,
-                   const double *m13, const double *w, int64_t np, int eff_func, double *out) {
+                   const double *m13, const double *w, int64_t np, int eff_func, double *out,
+                   const GfResDesc *stack = nullptr, int nstack = 0, int td_grid = 0) {
     gf_status st = ensure_layout(ctx);
     if(st)
         return st;
@@ -894,17 +1010,18 @@ gf_status run_gram(gf_ctx *ctx, const DalitzNode &d, int slot, int mode, const d
     st = ensure_dev(ctx->d_gram_partials, ctx->cap_gram_partials, (size_t)nb * 768, ctx->stream);
     if(st)
         return st;
-    GfProgArgs a = prog_args(ctx);
-    int rc = gfk::launch_gram(a, mode, grid6, m12, m13, w, np, slot, d.col_m12, d.col_m13, eff_pc, eff_cval,
-                              ctx->d_res + slot * GF_MAX_RES, d.nres, ctx->d_gram_partials, ctx->d_gram_tiles,
-                              (int)nb, ctx->stream);
+    GfProgArgs a         = prog_args(ctx);
+    const GfResDesc *res = stack ? stack : ctx->d_res + slot * GF_MAX_RES;
+    const int nres       = stack ? nstack : d.nres;
+    int rc = gfk::launch_gram(a, mode, grid6, m12, m13, w, np, slot, d.col_m12, d.col_m13, eff_pc, eff_cval, td_grid,
+                              res, nres, ctx->d_gram_partials, ctx->d_gram_tiles, (int)nb, ctx->stream);
     if(rc)
         return fail(GF_ERR_HIP, "Gram launch failed (%d)", rc);
     GF_HIP(hipMemcpyAsync(ctx->h_gram_tiles, ctx->d_gram_tiles, 768 * sizeof(double), hipMemcpyDeviceToHost,
                           ctx->stream));
     GF_HIP(hipStreamSynchronize(ctx->stream));
     ctx->stage_in_flight = false;
-    tiles_to_matrix(ctx, ctx->h_gram_tiles, d.nres, out);
+    tiles_to_matrix(ctx, ctx->h_gram_tiles, nres, out);
     return GF_OK;
 }
 
@@ -996,6 +1113,10 @@ gf_status gf_ctx_create(gf_ctx **out, int device) {
     TRY(hipMalloc((void **)&c->d_err, 64));
     TRY(hipHostMalloc((void **)&c->h_res, sizeof(GfResult), hipHostMallocMapped | hipHostMallocCoherent));
     TRY(hipHostGetDevicePointer((void **)&c->d_res_mapped, c->h_res, 0));
+    // Pinned blocks are recycled between contexts: a block freed by an earlier context
+    // still holds that context's last {sum, seq}, and this context's first reduction
+    // (seq 1) would read it as its own.  Start from seq 0.
+    memset((void *)c->h_res, 0, sizeof(GfResult));
     TRY(hipMalloc((void **)&c->d_gram_tiles, 768 * sizeof(double)));
     TRY(hipHostMalloc((void **)&c->h_result, 64, hipHostMallocDefault));
     TRY(hipHostMalloc((void **)&c->h_gram_tiles, 768 * sizeof(double), hipHostMallocDefault));
@@ -1017,9 +1138,12 @@ void gf_ctx_destroy(gf_ctx *c) {
     (void)hipSetDevice(c->device);
     if(c->stream)
         (void)hipStreamSynchronize(c->stream);
-    for(auto &kv : c->caches)
+    for(auto &kv : c->caches) {
         if(kv.second.wave)
             (void)hipFree(kv.second.wave);
+        if(kv.second.wave_b)
+            (void)hipFree(kv.second.wave_b);
+    }
     for(auto &kv : c->norm_points)
         if(kv.second.buf)
             (void)hipFree(kv.second.buf);
@@ -1190,7 +1314,7 @@ gf_status gf_program_set(gf_ctx *ctx, const int32_t *funcs, int nfunc, const dou
                        && ctx->ins[1].op == GI_MULTI_POLY && ctx->ins[2].op == GF_LVL_OP(GK_DALITZ_END, 0)
                        && ctx->ins[3].op == GI_END;
     for(const DalitzNode &d : ctx->dalitz) {
-        int cols[3] = {d.col_m12, d.col_m13, d.col_num};
+        int cols[7] = {d.col_m12, d.col_m13, d.col_num, d.col_time, d.col_sigma, d.col_mis, d.col_tag};
         for(int c : cols)
             if(c > ctx->max_col)
                 ctx->max_col = c;
@@ -1362,6 +1486,13 @@ gf_status gf_reduce_bins(gf_ctx *ctx, int func_idx, const double *ranges, int k,
     }
     for(const DalitzNode &d : ctx->dalitz)
         if(d.func >= func_idx) {
+            if(d.td) {
+                // the bin kernel runs without the time-dependent code: not for a subtree holding one
+                Cur r;
+                Compiler C2{ctx, {}, {}, -1};
+                if(cursor_at(C2, func_idx, r) && skip_subtree(C2, r) && d.func < r.f)
+                    return fail(GF_ERR_INVALID, "binned integral over a subtree holding Amp3Body_TD at %d", d.func);
+            }
             auto it = ctx->caches.find(d.cache_key);
             if(it == ctx->caches.end() || !it->second.wave) {
                 Cur r;
@@ -1398,13 +1529,28 @@ gf_status gf_dalitz_set_data_size(gf_ctx *ctx, int dalitz_func, int64_t n, int64
         GF_HIP(hipFree(wc.wave));
         wc.wave = nullptr;
     }
+    if(wc.wave_b) {
+        GF_HIP(hipFree(wc.wave_b));
+        wc.wave_b = nullptr;
+    }
+    wc.n = 0;
+    // GF_MAX_RES waves per event (Amp3Body.cu:222-233); a time-dependent node keeps
+    // its B waves beside them (WaveHolder_s, Amp3Body_TD.cu:531-580)
     size_t bytes = (size_t)GF_MAX_RES * (size_t)n * sizeof(double2);
     if(hipMalloc((void **)&wc.wave, bytes) != hipSuccess) {
         wc.wave = nullptr;
-        wc.n    = 0;
         return fail(GF_ERR_NOMEM, "cannot allocate a wave cache of %zu bytes", bytes);
     }
     GF_HIP(hipMemset(wc.wave, 0, bytes));
+    if(d->td) {
+        if(hipMalloc((void **)&wc.wave_b, bytes) != hipSuccess) {
+            wc.wave_b = nullptr;
+            (void)hipFree(wc.wave);
+            wc.wave = nullptr;
+            return fail(GF_ERR_NOMEM, "cannot allocate a wave cache of %zu bytes", bytes);
+        }
+        GF_HIP(hipMemset(wc.wave_b, 0, bytes));
+    }
     wc.n      = n;
     wc.offset = offset;
     return upload_dalitz_table(ctx);
@@ -1452,6 +1598,8 @@ gf_status gf_dalitz_gram(gf_ctx *ctx, int dalitz_func, const double grid6[6], in
     DalitzNode *d = find_dalitz(ctx, dalitz_func, &slot);
     if(!d)
         return fail(GF_ERR_INVALID, "function %d is not an Amp3Body node", dalitz_func);
+    if(d->td)
+        return fail(GF_ERR_INVALID, "function %d is an Amp3Body_TD node (gf_tddp_gram)", dalitz_func);
     if(!grid6 || !out_re_im)
         return fail(GF_ERR_INVALID, "null grid or output");
     double n12 = floor(grid6[2] + 0.5), n13 = floor(grid6[5] + 0.5);
@@ -1471,6 +1619,8 @@ gf_status gf_dalitz_gram_points_device(gf_ctx *ctx, int dalitz_func, const doubl
     DalitzNode *d = find_dalitz(ctx, dalitz_func, &slot);
     if(!d)
         return fail(GF_ERR_INVALID, "function %d is not an Amp3Body node", dalitz_func);
+    if(d->td)
+        return fail(GF_ERR_INVALID, "point normalisation is not available for Amp3Body_TD");
     if(!dev_m12 || !dev_m13 || !out_re_im || np < 0)
         return fail(GF_ERR_INVALID, "null point arrays or output");
     if(eff_func >= (int)ctx->funcs.size())
@@ -1510,11 +1660,13 @@ gf_status gf_dalitz_get_cache(gf_ctx *ctx, int dalitz_func, int res, double *out
     auto it = ctx->caches.find(d->cache_key);
     if(it == ctx->caches.end() || !it->second.wave)
         return fail(GF_ERR_STATE, "no wave cache");
-    if(res < 0 || res >= d->nres || !out_re_im || n != it->second.n)
-        return fail(GF_ERR_INVALID, "resonance %d / size %lld do not match the cache (%d, %lld)", res, (long long)n,
-                    d->nres, (long long)it->second.n);
-    GF_HIP(hipMemcpyAsync(out_re_im, it->second.wave + (size_t)res * it->second.n, (size_t)n * sizeof(double2),
-                          hipMemcpyDeviceToHost, ctx->stream));
+    const int nwaves = d->td ? 2 * d->nres : d->nres;  // time-dependent: A waves, then B waves
+    if(res < 0 || res >= nwaves || !out_re_im || n != it->second.n)
+        return fail(GF_ERR_INVALID, "wave %d / size %lld do not match the cache (%d, %lld)", res, (long long)n,
+                    nwaves, (long long)it->second.n);
+    const double2 *src = res < d->nres ? it->second.wave + (size_t)res * it->second.n
+                                       : it->second.wave_b + (size_t)(res - d->nres) * it->second.n;
+    GF_HIP(hipMemcpyAsync(out_re_im, src, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
     GF_HIP(hipStreamSynchronize(ctx->stream));
     return GF_OK;
 }
@@ -1526,12 +1678,99 @@ gf_status gf_dalitz_cache_gram(gf_ctx *ctx, int dalitz_func, double *out_re_im) 
     DalitzNode *d = find_dalitz(ctx, dalitz_func, &slot);
     if(!d)
         return fail(GF_ERR_INVALID, "function %d is not an Amp3Body node", dalitz_func);
+    if(d->td)
+        return fail(GF_ERR_INVALID, "function %d is an Amp3Body_TD node", dalitz_func);
     auto it = ctx->caches.find(d->cache_key);
     if(it == ctx->caches.end() || !it->second.wave)
         return fail(GF_ERR_STATE, "no wave cache");
     if(!out_re_im)
         return fail(GF_ERR_INVALID, "null output");
     return run_gram(ctx, *d, slot, 2, nullptr, nullptr, nullptr, nullptr, it->second.n, -1, out_re_im);
+}
+
+gf_status gf_tddp_gram(gf_ctx *ctx, int tddp_func, const double grid6[6], double *out_aa, double *out_ab,
+                       double *out_bb) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    int slot      = 0;
+    DalitzNode *d = find_dalitz(ctx, tddp_func, &slot);
+    if(!d || !d->td)
+        return fail(GF_ERR_INVALID, "function %d is not an Amp3Body_TD node", tddp_func);
+    if(!grid6 || !out_aa || !out_ab || !out_bb)
+        return fail(GF_ERR_INVALID, "null grid or output");
+    double n12 = floor(grid6[2] + 0.5), n13 = floor(grid6[5] + 0.5);
+    if(!(n12 >= 1) || !(n13 >= 1) || n12 * n13 > 4.0e9)
+        return fail(GF_ERR_INVALID, "bad grid bin counts %g x %g", grid6[2], grid6[5]);
+    const int n = d->nres;
+    if(n == 0)
+        return GF_OK;
+    // Stacked waves v = (a_0..a_{n-1}, b_0..b_{n-1}); the Hermitian Gram
+    // G_kl = sum_cells eff v_k conj(v_l) holds ai aj*, ai bj* and bi bj* as its blocks.
+    // One launch covers 16 entries; above that, groups of 8 are taken in pairs.
+    const int m = 2 * n;
+    std::vector<GfResDesc> all(m);
+    for(int k = 0; k < n; k++) {
+        all[k]          = d->res[k];
+        all[k].swap     = 0;
+        all[n + k]      = d->res[k];
+        all[n + k].swap = 1;
+    }
+    std::vector<int> groups;  // start of each group; the last group ends at m
+    const int gsize = m <= GF_MAX_RES ? m : GF_MAX_RES / 2;
+    for(int g = 0; g < m; g += gsize)
+        groups.push_back(g);
+    const int ng = (int)groups.size();
+    std::vector<std::complex<double>> G((size_t)m * m);
+    std::vector<char> have((size_t)m * m, 0);
+    GfResDesc *dstack = nullptr;
+    if(hipMalloc((void **)&dstack, GF_MAX_RES * sizeof(GfResDesc)) != hipSuccess)
+        return fail(GF_ERR_NOMEM, "cannot allocate the Gram stack");
+    gf_status st = GF_OK;
+    for(int u = 0; u < ng && !st; u++) {
+        for(int v = (ng == 1 ? 0 : u + 1); v < ng && !st; v++) {
+            std::vector<int> idx;  // stacked entries of this launch
+            for(int g : {u, v}) {
+                if(g == v && u == v && !idx.empty())
+                    break;
+                const int lo = groups[g], hi = std::min(m, lo + gsize);
+                for(int k = lo; k < hi; k++)
+                    idx.push_back(k);
+            }
+            std::vector<GfResDesc> st_res(idx.size());
+            for(size_t k = 0; k < idx.size(); k++)
+                st_res[k] = all[idx[k]];
+            if(hipMemcpy(dstack, st_res.data(), st_res.size() * sizeof(GfResDesc), hipMemcpyHostToDevice)
+               != hipSuccess) {
+                st = fail(GF_ERR_HIP, "copy of the Gram stack failed");
+                break;
+            }
+            const int ns = (int)idx.size();
+            std::vector<double> out(2 * (size_t)ns * ns);
+            st = run_gram(ctx, *d, slot, 0, grid6, nullptr, nullptr, nullptr, (int64_t)n12 * (int64_t)n13,
+                          d->eff_func, out.data(), dstack, ns, 1);
+            for(int i = 0; i < ns && !st; i++)
+                for(int j = 0; j < ns; j++) {
+                    const size_t at = (size_t)idx[i] * m + idx[j];
+                    if(!have[at]) {
+                        G[at]    = {out[2 * (i * ns + j)], out[2 * (i * ns + j) + 1]};
+                        have[at] = 1;
+                    }
+                }
+        }
+    }
+    (void)hipFree(dstack);
+    if(st)
+        return st;
+    for(int i = 0; i < n; i++)
+        for(int j = 0; j < n; j++) {
+            const size_t o = 2 * ((size_t)i * n + j);
+            const std::complex<double> aa = G[(size_t)i * m + j], ab = G[(size_t)i * m + n + j],
+                                       bb = G[(size_t)(n + i) * m + n + j];
+            out_aa[o] = aa.real(), out_aa[o + 1] = aa.imag();
+            out_ab[o] = ab.real(), out_ab[o + 1] = ab.imag();
+            out_bb[o] = bb.real(), out_bb[o + 1] = bb.imag();
+        }
+    return GF_OK;
 }
 
 // Internal hook (not in the public header): one FP64 MFMA on host operands.
@@ -2105,6 +2344,8 @@ gf_status gf_dalitz_set_norm_points(gf_ctx *ctx, int dalitz_func, gf_mlp *propos
     DalitzNode *d = find_dalitz(ctx, dalitz_func, nullptr);
     if(!d)
         return fail(GF_ERR_INVALID, "function %d is not an Amp3Body node", dalitz_func);
+    if(d->td)
+        return fail(GF_ERR_INVALID, "point normalisation is not available for Amp3Body_TD");
     if(n < 1)
         return fail(GF_ERR_INVALID, "need at least one point");
     if(proposal && proposal->ctx != ctx)
@@ -2159,6 +2400,8 @@ gf_status gf_dalitz_generate(gf_ctx *ctx, int dalitz_func, int64_t n, uint64_t s
     DalitzNode *d = find_dalitz(ctx, dalitz_func, &slot);
     if(!d)
         return fail(GF_ERR_INVALID, "function %d is not an Amp3Body node", dalitz_func);
+    if(d->td)
+        return fail(GF_ERR_INVALID, "event generation is not available for Amp3Body_TD");
     if(n < 1 || !m12 || !m13 || !box)
         return fail(GF_ERR_INVALID, "null output or n < 1");
     if(!(box[1] > box[0]) || !(box[3] > box[2]))
@@ -2429,6 +2672,8 @@ gf_status gf_batch_create(gf_ctx *ctx, int ntoys, const int64_t *first, const in
         return fail(GF_ERR_INVALID, "need at least one toy");
     if(ctx->ins.empty())
         return fail(GF_ERR_STATE, "no program installed (gf_program_set)");
+    if(ctx->has_td)
+        return fail(GF_ERR_INVALID, "toy batches do not take Amp3Body_TD programs");
     for(int t = 0; t < ntoys; t++)
         if(first[t] < 0 || count[t] < 0 || first[t] + count[t] > ctx->n)
             return fail(GF_ERR_INVALID, "toy %d rows [%lld, %lld) outside the %lld events", t, (long long)first[t],

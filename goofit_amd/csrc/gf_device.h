@@ -336,6 +336,97 @@ __device__ __forceinline__ double2 lineshape(const GfResDesc &r, const double *_
     }
 }
 
+// ------------------------------------------------------- time resolutions
+
+// device_truth_resolution, src/PDFs/physics/TruthResolution.cu:6-27: the decay-time
+// terms without smearing.
+__device__ __forceinline__ double truth_resolution(double coshterm, double costerm, double sinhterm, double sinterm,
+                                                   double tau, double dtime, double xmixing, double ymixing) {
+    double ret = 0;
+    dtime /= tau;
+    ret += coshterm * cosh(ymixing * dtime);
+    ret += costerm * cos(xmixing * dtime);
+    ret -= 2 * sinhterm * sinh(ymixing * dtime);
+    ret -= 2 * sinterm * sin(xmixing * dtime);
+    ret *= exp(-dtime);
+    return ret;
+}
+
+// One Gaussian of the three-Gaussian model (gaussian(), ThreeGaussResolution.cu:11-61):
+// the decay-time terms convolved with a Gaussian of width adjSigma centred on adjTime;
+// cos and sin are expanded to second and third order in x t / tau.
+__device__ __forceinline__ void tg_gaussian(double &P1, double &P2, double &P3, double &P4, double tau, double adjTime,
+                                            double xmixing, double ymixing, double adjSigma, double selbias) {
+    const double sqrt_pi_over_2 = 1.0 / M_2_SQRTPI;
+    const double inv_sqrt_2pi   = sqrt(0.5 * M_1_PI);
+    const double sixth          = 1.0 / 6.0;
+    const double a1  = adjSigma * M_SQRT2;  // 1 / sqrt(A)
+    const double is  = 1 / adjSigma;
+    const double a2  = 0.5 * a1;
+    const double is2 = is * is;
+    const double ng  = inv_sqrt_2pi * is;
+    const double Cc  = 0.5 * adjTime * adjTime * is2;
+    const double Bgn = -adjTime * is2;
+    const double G   = 1 / tau;
+    const double B   = G + Bgn;
+    const double u0  = a2 * (B + selbias);
+    const double u02 = u0 * u0;
+    const double F   = a1 * exp(-Cc + u02);
+    const double Ig0 = sqrt_pi_over_2 * erfc(u0);
+    const double Ig1 = 0.5 * exp(-u02);
+    const double Ig2 = Ig1 * u0 + 0.5 * Ig0;
+    const double Ig3 = Ig1 * (u02 + 1);
+    const double R   = xmixing * G * a1;
+    const double R2  = R * R;
+    const double It0 = F * Ig0;
+    const double It1 = F * R * (Ig1 - u0 * Ig0);
+    const double It2 = F * R2 * (Ig2 - u0 * Ig1 * 2 + u02 * Ig0);
+    const double It3 = F * R2 * R * (Ig3 - u0 * Ig2 * 3 + u02 * Ig1 * 3 - u0 * u02 * Ig0);
+    P2 = ng * (It0 - 0.5 * It2);
+    P4 = ng * (It1 - sixth * It3);
+    const double upy = a2 * (B - ymixing * G + selbias);
+    const double umy = a2 * (B + ymixing * G + selbias);
+    const double Ipy = a1 * exp(-Cc + upy * upy) * sqrt_pi_over_2 * erfc(upy);
+    const double Imy = a1 * exp(-Cc + umy * umy) * sqrt_pi_over_2 * erfc(umy);
+    P1 = ng * 0.5 * (Ipy + Imy);
+    P3 = ng * 0.5 * (Ipy - Imy);
+}
+
+// device_threegauss_resolution, ThreeGaussResolution.cu:63-127: core, tail and
+// outlier Gaussians (fractions f_c, f_t, 1 - f_c - f_t) with per-event width sigma.
+// R = the 13 parameters: fractions, biases and scales, selection bias, common offsets.
+__device__ __forceinline__ double threegauss_resolution(const double *__restrict__ R, double coshterm,
+                                                        double costerm, double sinhterm, double sinterm, double tau,
+                                                        double dtime, double xmixing, double ymixing, double sigma) {
+    const double fc = R[0], ft = R[1], fo = 1 - fc - ft;
+    const double bc = R[2] + R[9], bt = R[4] + R[11], bo = R[6] + R[11];
+    const double sc = R[3] * R[10], st = R[5] * R[12], so = R[7] * R[12];
+    const double sb = R[8];
+    double c1, c2, c3, c4, t1, t2, t3, t4, o1, o2, o3, o4;
+    tg_gaussian(c1, c2, c3, c4, tau, dtime - bc * sigma, xmixing, ymixing, sc * sigma, sb);
+    tg_gaussian(t1, t2, t3, t4, tau, dtime - bt * sigma, xmixing, ymixing, st * sigma, sb);
+    tg_gaussian(o1, o2, o3, o4, tau, dtime - bo * sigma, xmixing, ymixing, so * sigma, sb);
+    const double P1 = fc * c1 + ft * t1 + fo * o1;
+    const double P2 = fc * c2 + ft * t2 + fo * o2;
+    const double P3 = fc * c3 + ft * t3 + fo * o3;
+    const double P4 = fc * c4 + ft * t4 + fo * o4;
+    double ret = 0;
+    ret += coshterm * P1;
+    ret += costerm * P2;
+    ret -= 2 * sinhterm * P3;
+    ret -= 2 * sinterm * P4;
+    return ret;
+}
+
+// The resolution function of a time-dependent node, dispatched on its wave-uniform kind.
+__device__ __forceinline__ double time_resolution(int kind, const double *__restrict__ R, double t1, double t2,
+                                                  double abr, double abi, double tau, double dtime, double x,
+                                                  double y, double sigma) {
+    if(kind == GT_THREEGAUSS)
+        return threegauss_resolution(R, t1, t2, abr, abi, tau, dtime, x, y, sigma);
+    return truth_resolution(t1, t2, abr, abi, tau, dtime, x, y);
+}
+
 // Counter-based random numbers: splitmix64 finaliser of (key, counter).  Any
 // (key, counter) pair gives the same value on every run and device.
 __host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {

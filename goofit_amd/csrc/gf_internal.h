@@ -45,6 +45,8 @@ enum GfLvlKind : int32_t {
     GK_POP,            // T = acc
     GK_DALITZ,         // acc = inside ? |sum_i c_i W_i|^2 : -1
     GK_DALITZ_END,     // T = (acc == -1) ? 0 : acc * T
+    GK_TDDP,           // acc = time-dependent rate of A = sum c_i W_i, B = sum c_i W'_i
+    GK_TDDP_END,       // T = inside ? acc * T : 0
     GK_COUNT
 };
 #define GF_LVL_OP(kind, lvl) (GI_LVL + 4 * (kind) + (lvl))
@@ -67,19 +69,28 @@ struct GfResDesc {
     int32_t kind;  // GfRes
     int32_t p;     // index in params[] of mass (width at p+1)
     int32_t c;     // index in consts[] of spin (cyclic index at c+1, norm flag at c+2)
-    int32_t pad;
+    int32_t swap;  // 1: evaluated at (m13, m12) -- the B wave of a time-dependent node
 };
+
+// Time-resolution models of a time-dependent Dalitz node.
+enum GfTimeRes : int32_t { GT_TRUTH = 0, GT_THREEGAUSS = 1 };
 
 struct GfDecay {
     double motherMass, daug1Mass, daug2Mass, daug3Mass, meson_radius, mother_meson_radius;
 };
 
-// Per-Dalitz-node data the coherent sum needs.
+// Per-Dalitz-node data the coherent sum needs.  A time-dependent node also has
+// the B waves W'_i(m12, m13) = W_i(m13, m12) in wave_b (same layout), its time,
+// time-error, mistag and charm-tag columns (-1: absent) and its resolution model.
 struct GfDalitzSlot {
     const double2 *wave;  // nres x ncache complex, resonance-major (W[i*ncache + e])
     int64_t ncache;
     int32_t col_m12, col_m13, col_num;
     int32_t nres;
+    const double2 *wave_b;
+    int32_t col_time, col_sigma, col_mis, col_tag;
+    int32_t res_kind;     // GfTimeRes
+    int32_t res_p;        // index in params[] of the resolution's first parameter
 };
 
 // Everything an event-evaluating kernel needs, passed by value as the kernel
@@ -99,6 +110,7 @@ struct GfProgArgs {
     int32_t nobs;
     double total_events;         // c_totalEvents
     GfDecay decay;
+    int32_t td;                  // 1: the program holds a time-dependent Dalitz node
     int32_t *err;                // device error word (bit 0: cache index out of range)
     uint32_t seq;                // sequence number k_finish stamps on its GfResult
     void *mark;                  // host only: hipEvent_t recorded between a sweep and its k_finish
@@ -171,9 +183,12 @@ int launch_dalitz_cache(const GfProgArgs &a, int slot, int64_t offset, int64_t n
                         const GfResDesc *res, int nres, int nblocks, void *stream);
 // mode 0: grid cells (grid6 host array); 1: points (m12, m13, w device arrays); 2: data cache of `slot`.
 // Writes 768 doubles (tiles RR, II, RI in MFMA register layout) to tiles_dev.
+// td_grid: the grid of a time-dependent node (m13 bin count grid6[5]); res[i].swap
+// evaluates entry i at (m13, m12).
 int launch_gram(const GfProgArgs &a, int mode, const double *grid6, const double *m12, const double *m13,
                 const double *w, int64_t npoints, int slot, int id_m12, int id_m13, int eff_pc, double eff_const,
-                const GfResDesc *res, int nres, double *partials, double *tiles_dev, int nblocks, void *stream);
+                int td_grid, const GfResDesc *res, int nres, double *partials, double *tiles_dev, int nblocks,
+                void *stream);
 int launch_identity_check(const double *col, int64_t n, int32_t *flag, void *stream);
 int mfma_probe(const double *dev_a, const double *dev_b, double *dev_out, void *stream);
 }  // namespace gfk

@@ -84,8 +84,10 @@ __device__ __forceinline__ double (&level(Acc<EPT> &a))[EPT] {
 
 // ------------------------------------------------------------ interpreter
 
-// One combining step on the accumulator of level L.
-template <int L, int EPT, class Src>
+// One combining step on the accumulator of level L.  TD: the program may hold a
+// time-dependent Dalitz node (its code is compiled only into the kernels that
+// need it, keeping the others' register budget).
+template <int L, int EPT, bool TD, class Src>
 __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs &A, const double *__restrict__ P,
                                      const double *__restrict__ Nm, const Src &src, const int64_t (&row)[EPT],
                                      const bool (&valid)[EPT], Acc<EPT> &acc, double (&T)[EPT]) {
@@ -199,6 +201,80 @@ __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs
         for(int j = 0; j < EPT; ++j)
             T[j] = (a[j] == -1.0) ? 0.0 : a[j] * T[j];
         break;
+    case GK_TDDP:
+        if constexpr(TD) {
+            // device_Tddp, src/PDFs/physics/Amp3Body_TD.cu:41-257.  in.p: tau (then x, y,
+            // dx, dy and the couplings); in.a: resonance count; in.x: node slot.
+            const GfDalitzSlot dz = A.dz[in.x];
+            const GfDecay &D      = A.decay;
+            int64_t en[EPT];
+            double sar[EPT], sai[EPT], sbr[EPT], sbi[EPT];
+#pragma unroll
+            for(int j = 0; j < EPT; ++j) {
+                const double m12 = src.get(dz.col_m12, j), m13 = src.get(dz.col_m13, j);
+                const bool inside = gfd::in_dalitz(m12, m13, D.motherMass, D.daug1Mass, D.daug2Mass, D.daug3Mass);
+                int64_t e = A.identity_num ? row[j] : (int64_t)floor(0.5 + src.get(dz.col_num, j));
+                if(e < 0 || e >= dz.ncache) {
+                    if(inside && valid[j])
+                        atomicOr(A.err, 1);
+                    e = 0;
+                }
+                en[j]  = e;
+                sar[j] = sai[j] = sbr[j] = sbi[j] = 0.0;
+            }
+            for(int i = 0; i < in.a; ++i) {
+                // A += a_i * c_i, B += b_i * c_i (Amp3Body_TD.cu:91-110)
+                const double cr = P[in.p + 5 + 2 * i], ci = P[in.p + 6 + 2 * i];
+                const double2 *__restrict__ wa = dz.wave + (int64_t)i * dz.ncache;
+                const double2 *__restrict__ wb = dz.wave_b + (int64_t)i * dz.ncache;
+#pragma unroll
+                for(int j = 0; j < EPT; ++j) {
+                    const double2 u = wa[en[j]], v = wb[en[j]];
+                    sar[j] += u.x * cr - u.y * ci;
+                    sai[j] += u.x * ci + u.y * cr;
+                    sbr[j] += v.x * cr - v.y * ci;
+                    sbi[j] += v.x * ci + v.y * cr;
+                }
+            }
+            const double tau = P[in.p], x0 = P[in.p + 1], y0 = P[in.p + 2], dx = P[in.p + 3], dy = P[in.p + 4];
+            const double *__restrict__ R = P + dz.res_p;
+            // mistag terms: A <-> B with fixed lifetime and mixing (Amp3Body_TD.cu:226-248)
+            const double xfix = 0.0039, yfix = 0.0065, taufix = 0.4101;
+#pragma unroll
+            for(int j = 0; j < EPT; ++j) {
+                const int tag = dz.col_tag >= 0 ? (int)src.get(dz.col_tag, j) : 0;
+                const double x = tag == 1 ? x0 + dx : (tag == -1 ? x0 - dx : x0);
+                const double y = tag == 1 ? y0 + dy : (tag == -1 ? y0 - dy : y0);
+                const double t = src.get(dz.col_time, j), sg = src.get(dz.col_sigma, j);
+                const double nA = sar[j] * sar[j] + sai[j] * sai[j];
+                const double nB = sbr[j] * sbr[j] + sbi[j] * sbi[j];
+                const double t1 = nA + nB, t2 = nA - nB;
+                const double abr = sar[j] * sbr[j] + sai[j] * sbi[j];  // A conj(B)
+                const double abi = sai[j] * sbr[j] - sar[j] * sbi[j];
+                double ret = gfd::time_resolution(dz.res_kind, R, t1, t2, abr, abi, tau, t, x, y, sg);
+                const double mis = dz.col_mis >= 0 ? src.get(dz.col_mis, j) : 0.0;
+                if(mis > 0) {
+                    ret *= (1 - 2 * mis);
+                    ret += mis * gfd::time_resolution(dz.res_kind, R, t1, -t2, abr, -abi, taufix, t, xfix, yfix, sg);
+                    ret += mis * gfd::time_resolution(dz.res_kind, R, t1, t2, abr, abi, taufix, t, xfix, yfix, sg);
+                }
+                a[j] = ret;
+            }
+        }
+        break;
+    case GK_TDDP_END:
+        if constexpr(TD) {
+            // Amp3Body_TD.cu:61-76, 253-255: ret *= eff inside, 0 outside
+            const GfDalitzSlot dz = A.dz[in.x];
+            const GfDecay &D      = A.decay;
+#pragma unroll
+            for(int j = 0; j < EPT; ++j) {
+                const bool inside = gfd::in_dalitz(src.get(dz.col_m12, j), src.get(dz.col_m13, j), D.motherMass,
+                                                   D.daug1Mass, D.daug2Mass, D.daug3Mass);
+                T[j] = inside ? a[j] * T[j] : 0.0;
+            }
+        }
+        break;
     default: break;
     }
 }
@@ -207,7 +283,7 @@ __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs
 // Run the compiled program from instruction pc0 for EPT events per lane; the
 // value of the tree ends up in T.  row[] are the event rows (used as wave-cache
 // index when the event-number column is the identity), valid[] masks tail lanes.
-template <int EPT, class Src>
+template <int EPT, bool TD = false, class Src>
 __device__ __forceinline__ void run(const GfProgArgs &A, const double *__restrict__ P, const double *__restrict__ Nm,
                                     int pc0, const Src &src, const int64_t (&row)[EPT], const bool (&valid)[EPT],
                                     double (&T)[EPT]) {
@@ -324,10 +400,10 @@ __device__ __forceinline__ void run(const GfProgArgs &A, const double *__restric
             if(in.op >= GI_LVL) {
                 const int kind = (in.op - GI_LVL) >> 2;
                 switch((in.op - GI_LVL) & 3) {
-                case 0: step<0>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
-                case 1: step<1>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
-                case 2: step<2>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
-                default: step<3>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
+                case 0: step<0, EPT, TD>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
+                case 1: step<1, EPT, TD>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
+                case 2: step<2, EPT, TD>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
+                default: step<3, EPT, TD>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
                 }
                 break;
             }
@@ -399,7 +475,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_finish(const Partial *partials, in
 
 // ------------------------------------------------------------- event kernels
 
-template <int EPT, bool INL>
+template <int EPT, bool INL, bool TD = false>
 __global__ __launch_bounds__(GF_BLOCK) void k_reduce_metric(GfProgArgs A, GfInline I, int metric, Partial *partials) {
     const double *__restrict__ P  = INL ? I.p : A.params;
     const double *__restrict__ Nm = INL ? I.n : A.norms;
@@ -421,7 +497,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_reduce_metric(GfProgArgs A, GfInli
             src.idx[j] = row[j];
         }
         double T[EPT];
-        run<EPT>(A, P, Nm, 0, src, row, valid, T);
+        run<EPT, TD>(A, P, Nm, 0, src, row, valid, T);
 #pragma unroll
         for(int j = 0; j < EPT; ++j) {
             double e0 = 0.0, e1 = 0.0;
@@ -437,7 +513,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_reduce_metric(GfProgArgs A, GfInli
     store_partial(s, c, partials);
 }
 
-template <int EPT, bool INL>
+template <int EPT, bool INL, bool TD = false>
 __global__ __launch_bounds__(GF_BLOCK) void k_eval_metric(GfProgArgs A, GfInline I, int metric, double *out) {
     const double *__restrict__ P  = INL ? I.p : A.params;
     const double *__restrict__ Nm = INL ? I.n : A.norms;
@@ -458,7 +534,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_eval_metric(GfProgArgs A, GfInline
             src.idx[j] = row[j];
         }
         double T[EPT];
-        run<EPT>(A, P, Nm, 0, src, row, valid, T);
+        run<EPT, TD>(A, P, Nm, 0, src, row, valid, T);
 #pragma unroll
         for(int j = 0; j < EPT; ++j) {
             double e0 = 0.0, e1 = 0.0;
@@ -621,6 +697,7 @@ __global__ __launch_bounds__(256) void k_dalitz_cache(GfProgArgs A, int slot, in
     const GfDalitzSlot dz = A.dz[slot];
     const GfDecay D       = A.decay;
     double2 *wave         = const_cast<double2 *>(dz.wave);
+    double2 *wave_b       = const_cast<double2 *>(dz.wave_b);  // time-dependent nodes only
     const double *c12     = A.cols[dz.col_m12];
     const double *c13     = A.cols[dz.col_m13];
     for(int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
@@ -637,6 +714,13 @@ __global__ __launch_bounds__(256) void k_dalitz_cache(GfProgArgs A, int slot, in
             if(inside)
                 w = gfd::lineshape(res[i], A.params, A.consts, D, m12, m13, m23);
             wave[(int64_t)i * dz.ncache + e] = w;
+            if(wave_b) {
+                // SpecialWaveCalculator.cu:55-58: the B wave at the reversed point, same m23
+                double2 v = make_double2(0.0, 0.0);
+                if(inside)
+                    v = gfd::lineshape(res[i], A.params, A.consts, D, m13, m12, m23);
+                wave_b[(int64_t)i * dz.ncache + e] = v;
+            }
         }
     }
 }
@@ -845,6 +929,7 @@ struct GramSrc {
     int id_m12, id_m13;
     int eff_pc;        // >= 0: efficiency subprogram; -1: efficiency 1; -2: constant eff_const
     double eff_const;
+    int td;            // grid of a time-dependent node: m13 bin count n13 (SpecialDalitzIntegrator.cu:73-80)
 };
 
 // One block: 128 points per pass; each lane computes the 16 complex amplitudes of
@@ -877,7 +962,7 @@ __global__ __launch_bounds__(GG_BLOCK) void k_gram(GfProgArgs A, GramSrc S, cons
             c12 *= (binNumberM12 + 0.5);
             c12 += S.l12;
             globalBinNumber /= numBinsM12;
-            int numBinsM13 = S.n12;
+            int numBinsM13 = S.td ? S.n13 : S.n12;
             double c13     = S.u13 - S.l13;
             c13 /= numBinsM13;
             c13 *= (globalBinNumber + 0.5);
@@ -907,8 +992,10 @@ __global__ __launch_bounds__(GG_BLOCK) void k_gram(GfProgArgs A, GramSrc S, cons
 #pragma unroll 1
             for(int i = 0; i < GF_MAX_RES; ++i) {
                 double2 b = make_double2(0.0, 0.0);
-                if(i < nres && inside)
-                    b = gfd::lineshape(res[i], A.params, A.consts, D, m12, m13, m23);
+                if(i < nres && inside) {
+                    const bool sw = res[i].swap != 0;  // B wave of a time-dependent node
+                    b = gfd::lineshape(res[i], A.params, A.consts, D, sw ? m13 : m12, sw ? m12 : m13, m23);
+                }
                 Xr[wave][i][lane] = b.x;
                 Xi[wave][i][lane] = b.y;
             }
@@ -1027,7 +1114,15 @@ static int launch_finish(const GfProgArgs &a, Partial *partials, int nblocks, Gf
 int launch_reduce_metric(const GfProgArgs &a, const GfInline *inl, int metric, Partial *partials, GfResult *result,
                          int nblocks, void *stream) {
     hipStream_t s = (hipStream_t)stream;
-    if(inl)
+    if(a.td) {
+        // time-dependent nodes: one event per lane (the resolution code is large)
+        if(inl)
+            hipLaunchKernelGGL((k_reduce_metric<1, true, true>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, *inl, metric,
+                               partials);
+        else
+            hipLaunchKernelGGL((k_reduce_metric<1, false, true>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, GfInline{},
+                               metric, partials);
+    } else if(inl)
         hipLaunchKernelGGL((k_reduce_metric<2, true>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, *inl, metric, partials);
     else
         hipLaunchKernelGGL((k_reduce_metric<2, false>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, GfInline{}, metric,
@@ -1037,7 +1132,14 @@ int launch_reduce_metric(const GfProgArgs &a, const GfInline *inl, int metric, P
 
 int launch_eval_metric(const GfProgArgs &a, const GfInline *inl, int metric, double *out, int nblocks, void *stream) {
     hipStream_t s = (hipStream_t)stream;
-    if(inl)
+    if(a.td) {
+        if(inl)
+            hipLaunchKernelGGL((k_eval_metric<1, true, true>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, *inl, metric,
+                               out);
+        else
+            hipLaunchKernelGGL((k_eval_metric<1, false, true>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, GfInline{},
+                               metric, out);
+    } else if(inl)
         hipLaunchKernelGGL((k_eval_metric<2, true>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, *inl, metric, out);
     else
         hipLaunchKernelGGL((k_eval_metric<2, false>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, GfInline{}, metric, out);
@@ -1134,10 +1236,11 @@ int launch_dalitz_cache(const GfProgArgs &a, int slot, int64_t offset, int64_t n
 
 int launch_gram(const GfProgArgs &a, int mode, const double *grid6, const double *m12, const double *m13,
                 const double *w, int64_t npoints, int slot, int id_m12, int id_m13, int eff_pc, double eff_const,
-                const GfResDesc *res,
-                int nres, double *partials, double *tiles_dev, int nblocks, void *stream) {
+                int td_grid, const GfResDesc *res, int nres, double *partials, double *tiles_dev, int nblocks,
+                void *stream) {
     GramSrc S;
     S.mode = mode;
+    S.td   = td_grid;
     S.l12 = S.u12 = S.l13 = S.u13 = 0.0;
     S.n12 = S.n13 = 1;
     if(grid6) {

@@ -253,6 +253,71 @@ def build_dalitz(nbins=1414, coup_seed=3, resonances=None, float_couplings=True,
     return Model(amp, [m12, m13, evt], params, keep=res + [decay, eff])
 
 
+TD_MIX = (0.4101, 0.0016, 0.0055, 0.0, 0.0)  # tau [ps], x, y, dx, dy of the pipipi0 example
+
+
+def build_tddp(nbins=240, coup_seed=3, resonances=None, mix=TD_MIX, resolution="truth", mistag=False,
+               charmtag=False, float_mixing=True, tg=None):
+    """Amp3Body_TD of D0 -> pi0 pi+ pi- (the first six resonances of DALITZ_RESONANCES by
+    default, couplings fixed), observables (dtime, sigma_t, m12, m13, evt[, mistag][, tag]);
+    tau, x, y free when float_mixing.  resolution: "truth" or "threegauss" (parameters tg)."""
+    import goofit_amd as G
+    G.require_api()
+    table = resonances if resonances is not None else DALITZ_RESONANCES[:6]
+    dtime = G.Observable("dtime", -2.0, 4.0)
+    sigma = G.Observable("sigmat", 0.0, 0.8)
+    m12 = G.Observable("m12", 0.0, 3.0)
+    m13 = G.Observable("m13", 0.0, 3.0)
+    m12.numbins = nbins
+    m13.numbins = nbins
+    evt = G.EventNumber("eventNumber")
+    coups = dalitz_couplings(coup_seed, len(table))
+    res = []
+    for (name, kind, mass, width, spin, pair), c in zip(table, coups):
+        ar, ai = G.Variable(name + "_re", c.real), G.Variable(name + "_im", c.imag)
+        if kind == "NONRES":
+            res.append(G.Resonances.NonRes(name, ar, ai))
+        elif kind == "GS":
+            res.append(G.Resonances.GS(name, ar, ai, G.Variable(name + "_mass", mass),
+                                       G.Variable(name + "_width", width), spin, pair))
+        else:
+            res.append(G.Resonances.RBW(name, ar, ai, G.Variable(name + "_mass", mass),
+                                        G.Variable(name + "_width", width), spin, pair))
+    if float_mixing:
+        tau = G.Variable("tau", mix[0], 0.001, 0.3, 0.5)
+        x = G.Variable("xmixing", mix[1], 0.001, 0.0, 0.0)
+        y = G.Variable("ymixing", mix[2], 0.001, 0.0, 0.0)
+    else:
+        tau, x, y = G.Variable("tau", mix[0]), G.Variable("xmixing", mix[1]), G.Variable("ymixing", mix[2])
+    dx, dy = G.Variable("deltax", mix[3]), G.Variable("deltay", mix[4])
+    decay = G.DecayInfo3t(tau, x, y, dx, dy)
+    decay.motherMass, decay.daug1Mass, decay.daug2Mass, decay.daug3Mass = DALITZ_DECAY[:4]
+    decay.meson_radius, decay.mother_meson_radius = DALITZ_DECAY[4:]
+    decay.resonances = res
+    if resolution == "threegauss":
+        vals = tg if tg is not None else (0.7, 0.2, 0.05, 1.1, -0.1, 2.0, 0.0, 6.0, 0.0, 0.01, 1.05, 0.0, 1.0)
+        names = ("cf", "tf", "cb", "cs", "tb", "ts", "ob", "os", "sb", "cbo", "cws", "cbo2", "cws2")
+        rv = [G.Variable(n, v) for n, v in zip(names, vals)]
+        r = G.ThreeGaussResolution(*rv)
+    else:
+        rv = []
+        r = G.TruthResolution()
+    eff = G.PolynomialPdf("constantEff", [m12, m13], [G.Variable("constantOne", 1.0)],
+                          [G.Variable("constantZero0", 0.0), G.Variable("constantZero1", 0.0)], 0)
+    obs = [dtime, sigma, m12, m13, evt]
+    mt = G.Observable("mistag", 0.0, 0.5) if mistag else None
+    tag = G.Observable("charmtag", -1.0, 1.0) if charmtag else None
+    if mt is not None:
+        obs.append(mt)
+        if tag is not None:
+            obs.append(tag)
+        td = G.Amp3Body_TD("td", dtime, sigma, m12, m13, evt, decay, r, eff, mt, tag)
+    else:
+        td = G.Amp3Body_TD("td", dtime, sigma, m12, m13, evt, decay, r, eff)
+    params = dict(tau=tau, xmixing=x, ymixing=y)
+    return Model(td, obs, params, keep=res + rv + [decay, eff, r])
+
+
 def dalitz_dataset(model, m12, m13):
     import goofit_amd as G
     d = G.UnbinnedDataSet(model.observables)

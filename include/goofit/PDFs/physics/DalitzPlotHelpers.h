@@ -28,4 +28,26 @@ struct DecayInfo3 {
     std::vector<ResonancePdf *> resonances;
 };
 
+/// DecayInfo3 plus the D0 lifetime and mixing parameters of the time-dependent
+/// PDF (DalitzPlotHelpers.h:99-112 of the reference).
+struct DecayInfo3t : public DecayInfo3 {
+    Variable _tau;
+    Variable _xmixing;
+    Variable _ymixing;
+    Variable _deltax;
+    Variable _deltay;
+
+    DecayInfo3t(Variable tau, Variable xmixing, Variable ymixing, Variable deltax, Variable deltay)
+        : _tau(tau)
+        , _xmixing(xmixing)
+        , _ymixing(ymixing)
+        , _deltax(deltax)
+        , _deltay(deltay) {}
+};
+
+/// Cached (A, B) waves of one event for one resonance of the time-dependent PDF.
+struct WaveHolder_s {
+    fptype ai_real, ai_imag, bi_real, bi_imag;
+};
+
 }  // namespace GooFit

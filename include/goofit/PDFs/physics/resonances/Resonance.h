@@ -10,6 +10,7 @@ namespace GooFit {
 
 class ResonancePdf : public AmpComponent {
     friend class Amp3Body;
+    friend class Amp3Body_TD;
 
   public:
     ~ResonancePdf() override = default;

@@ -226,13 +226,33 @@ gf_status gf_dalitz_gram_points_device(gf_ctx *ctx, int dalitz_func, const doubl
                                        int eff_func, double *out_re_im);
 
 /* Copy cached wave `res` (n complex values, re/im interleaved) to the host.
- * Replaces Amp3Body::getCachedWave (src/PDFs/physics/Amp3Body.cu:370-375). */
+ * Replaces Amp3Body::getCachedWave (src/PDFs/physics/Amp3Body.cu:370-375).  For an
+ * Amp3Body_TD node res in [nres, 2 nres) selects B wave res - nres
+ * (Amp3Body_TD::getCachedWaveNoCopy, include/goofit/PDFs/physics/Amp3Body_TD.h:136-138). */
 gf_status gf_dalitz_get_cache(gf_ctx *ctx, int dalitz_func, int res, double *out_re_im, int64_t n);
 
 /* Data-event Gram G_ij = sum_e W_i[e] conj(W_j[e]) over the wave cache, used for
  * fit fractions.  Replaces the n*n transform_reduce calls of
  * Amp3Body::fit_fractions (src/PDFs/physics/Amp3Body.cu:376-428). */
 gf_status gf_dalitz_cache_gram(gf_ctx *ctx, int dalitz_func, double *out_re_im);
+
+/* ---- time-dependent three-body Dalitz (Amp3Body_TD) -------------------------
+ * A GF_FN_TDDP node takes the Amp3Body entry points above for its wave cache:
+ * gf_dalitz_set_data_size allocates the A waves W_i(m12, m13) and the B waves
+ * W_i(m13, m12) (WaveHolder_s, Amp3Body_TD.cu:531-580) and gf_dalitz_cache fills
+ * both (SpecialWaveCalculator.cu:13-63).  Its normalisation integrals: */
+
+/* ThreeComplex integrals of every resonance pair over the m12 x m13 midpoint grid
+ * grid6 = {l12, u12, n12, l13, u13, n13}: out_aa = sum eff a_i conj(a_j), out_ab =
+ * sum eff a_i conj(b_j), out_bb = sum eff b_i conj(b_j), each nres x nres complex
+ * (re, im interleaved, row-major), with the efficiency of the node evaluated on
+ * the fake event {2, .., m12, m13} at each cell centre.  One fused lineshape +
+ * FP64-MFMA Gram of the stacked waves (a, b); above 8 resonances, pairs of
+ * 8-wave groups.  Replaces the nres^2 transform_reduce calls with
+ * SpecialDalitzIntegrator (src/PDFs/physics/Amp3Body_TD.cu:664-682,
+ * detail/SpecialDalitzIntegrator.cu:9-126). */
+gf_status gf_tddp_gram(gf_ctx *ctx, int tddp_func, const double grid6[6], double *out_aa, double *out_ab,
+                       double *out_bb);
 
 /* ---- normalisation by Monte Carlo: uniform and learned proposals ------------
  * No reference counterpart (the reference integrates Amp3Body on a fixed grid,

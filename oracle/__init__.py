@@ -215,6 +215,11 @@ class Program:
         self.norms += [1.0, pdf.cached_norm]
         for c in pdf.components:
             self.populate(c)
+        if pdf.func == FN["TDDP"]:
+            # Amp3Body_TD::populateArrays (Amp3Body_TD.cu:523-527): the last constant is
+            # the function index after the efficiency subtree
+            pdf.consts[-1] = float(len(self.funcs))
+            self.consts[pdf.constants_idx + len(pdf.consts)] = pdf.consts[-1]
 
     def arrays(self):
         return (np.asarray(self.funcs, np.int32), np.asarray(self.params, np.float64),
@@ -705,7 +710,7 @@ class Model:
         return ret
 
     def _normalize_tddp(self, pdf: Tddp):
-        # Amp3Body_TD::normalize, src/PDFs/physics/Amp3Body_TD.cu:552-748
+        # Amp3Body_TD::normalize, src/PDFs/physics/Amp3Body_TD.cu:590-748
         def set1(node):
             node.cached_norm = 1.0
             for c in node.components:

@@ -1302,9 +1302,9 @@ double gfo_dalitz_norm_from_integrals(int nres, const double *coup, const double
 /* ------------------------------------------------------- Amp3Body_TD host side */
 
 /* Layout of an Amp3Body_TD node: its resonances, the resolution function, the
- * root of the efficiency subtree (efficiencyFunction, Amp3Body_TD.cu:505-512)
+ * root of the efficiency subtree (efficiencyFunction, Amp3Body_TD.cu:518)
  * and the function index after that subtree (the constant the device function
- * walks to, Amp3Body_TD.cu:514-519).  Returns the number of resonances. */
+ * walks to, Amp3Body_TD.cu:523-527).  Returns the number of resonances. */
 int gfo_tddp_layout(gfo_prog *g, int tddp_func, int *res_funcs, int *res_func, int *eff_func, int *end_func) {
     pcur pc = {0, 0, 0, 0, 0};
     while(pc.funcIdx < tddp_func)
@@ -1340,7 +1340,7 @@ static void tddp_ab(gfo_prog *g, const pcur *rpc, double m12, double m13, cplx *
 }
 
 /* Wave cache of one resonance of a TD node over event rows offset..offset+n-1:
- * the m12/m13 slots are the node's observables 2 and 3 (Amp3Body_TD.cu:581-620). */
+ * the m12/m13 slots are the node's observables 2 and 3 (Amp3Body_TD.cu:639-661). */
 void gfo_tddp_wave(gfo_prog *g, int tddp_func, int res_func, const double *aos, int64_t n, int evtSize,
                    int64_t offset, double *out_a, double *out_b) {
     pcur pc = {0, 0, 0, 0, 0};
@@ -1366,7 +1366,7 @@ void gfo_tddp_wave(gfo_prog *g, int tddp_func, int res_func, const double *aos, 
 }
 
 /* ThreeComplex integrals of every resonance pair over the m12 x m13 midpoint
- * grid (Amp3Body_TD.cu:620-645 with SpecialDalitzIntegrator.cu:9-126): per cell
+ * grid (Amp3Body_TD.cu:664-682 with SpecialDalitzIntegrator.cu:9-126): per cell
  * (ai aj*, ai bj*, bi bj*) times the efficiency evaluated on a fake event
  * {2, ..., m12, m13} (other slots 0 here); the m13 bin count is grid6[5].  Each
  * pair's sum is OMP-chunked like the other oracle sums.  out_*: nres*nres
@@ -1516,7 +1516,7 @@ double gfo_resolution_eval(int kind, const double *params13, double t1, double t
     return res_truth(t1, t2, abr, abi, tau, dtime, x, y);
 }
 
-/* Amp3Body_TD::normalize host assembly (Amp3Body_TD.cu:677-748).  coup: nres
+/* Amp3Body_TD::normalize host assembly (Amp3Body_TD.cu:686-751).  coup: nres
  * (re, im); I*: the ThreeComplex integrals; mix5 = {tau, x, y, dx, dy};
  * mistag_on / mistag: whether the mistag observable tests true (its value is
  * in range, include/goofit/Variable.h:103-105) and its value. */

@@ -53,6 +53,22 @@ def test_add1d_nll(ctx, n):
     assert math.isclose(got, exact, rel_tol=1e-13), (got, exact)
 
 
+def test_first_result_of_a_new_context_is_its_own():
+    """Contexts created one after another (pinned result blocks recycled): each first
+    reduction reports its own sum, not the last one a destroyed context left behind."""
+    for k, n in enumerate((3000, 5000, 7000)):
+        top, x = add1d()
+        data = W.sample_add1d(n, seed=40 + k)
+        M = O.Model(top, data.reshape(-1, 1))
+        ref = M.calculate_nll()
+        with capi.Context(0) as c:
+            _install(c, top)
+            c.upload_columns([data])
+            _sync(c, M)
+            got, _ = c.reduce_metric("NLL")
+        assert math.isclose(got, ref, rel_tol=NLL_RTOL), (k, got, ref)
+
+
 def test_add1d_per_event_prob(ctx):
     top, x = add1d()
     data = W.sample_add1d(4099, seed=12)

@@ -11,6 +11,7 @@ import goofit_amd as G
 import oracle as O
 from goofit_amd import workloads as W
 
+import _models
 from _models import add1d, dalitz as oracle_dalitz, prod3d
 
 FLT_EPS = float(np.finfo(np.float32).eps)
@@ -100,6 +101,25 @@ def test_flattened_tree_dalitz():
     _flat_equal(top)
     names = G.engine.function_names()
     assert len(names) == 18  # Amp3Body + 16 resonances + efficiency
+
+
+@pytest.mark.parametrize("kind", ["truth", "threegauss"])
+def test_flattened_tree_tddp(kind):
+    """Amp3Body_TD (Amp3Body_TD.cu:262-359, 472-529): mixing parameters and couplings,
+    constants (nres, cacheToUse, mistag flag, function index after the efficiency),
+    resonance blocks, the resolution, then the efficiency."""
+    G.Amp3Body_TD.resetCacheCounter()
+    m = W.build_tddp(nbins=150, resolution=kind, mistag=True, charmtag=True)
+    m.pdf.SigGenSetIndices()
+    res = _models.threegauss() if kind == "threegauss" else O.TruthResolution()
+    top, _ = _models.tddp(nbins=150, mix=W.TD_MIX, resolution=res, mistag=True, charmtag=True)
+    _flat_equal(top)
+    f, _, c, _, _ = G.engine.flattened()
+    assert c[4] == len(f) and c[3] == 0  # end of the efficiency subtree; mistag observable given
+    # the per-D0-mass-bin resolution variant is refused (its device code never selects)
+    with pytest.raises(G.GeneralError):
+        G.Amp3Body_TD("bad", *m.observables[:4], m.observables[4], m.pdf.getDecayInfo(), [G.TruthResolution()],
+                      None, G.Observable("md0", 1.8, 1.9))
 
 
 def test_minuit_vector_roundtrip_and_blinding():

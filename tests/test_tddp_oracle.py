@@ -24,7 +24,7 @@ sys.path.insert(0, os.path.dirname(__file__))
 import _models as M  # noqa: E402
 
 TRUTH, TG = O.FN["TRUTH_RES"], O.FN["THREEGAUSS_RES"]
-FIX = (0.4101, 0.0039, 0.0065)  # taufix, xfix, yfix of the mistag terms (Amp3Body_TD.cu:227-229)
+FIX = (0.4101, 0.0039, 0.0065)  # taufix, xfix, yfix of the mistag terms (Amp3Body_TD.cu:229-231)
 
 
 def _tg_params(cf=1.0, tf=0.0, cb=0.0, cs=1.0, tb=0.0, ts=1.0, ob=0.0, os_=1.0, sb=0.0, cbo=0.0, cws=1.0,
@@ -50,7 +50,7 @@ def test_truth_time_integrals_closed_form(x, y):
 @pytest.mark.parametrize("kind", [TRUTH, TG])
 def test_resolution_normalisation_is_time_integral(kind):
     """normalization(di1..di4) = integral over t of the resolution function fed
-    term1 = di1 + di2, term2 = di1 - di2, Re/Im(AB*) = di3, di4 (Amp3Body_TD.cu:183-186)."""
+    term1 = di1 + di2, term2 = di1 - di2, Re/Im(AB*) = di3, di4 (Amp3Body_TD.cu:165-214)."""
     rng = np.random.default_rng(11)
     tau, x, y, sb = 0.41, 0.004, 0.006, 0.0
     p = M.threegauss().params
@@ -148,7 +148,7 @@ def test_no_mixing_reduces_to_dalitz_density():
 
 
 def test_charm_tag_selects_mixing_parameters():
-    """tag +1 / -1 use (x + dx, y + dy) / (x - dx, y - dy); tag 0 uses (x, y) (Amp3Body_TD.cu:141-150)."""
+    """tag +1 / -1 use (x + dx, y + dy) / (x - dx, y - dy); tag 0 uses (x, y) (Amp3Body_TD.cu:132-145)."""
     mix = (0.41, 0.02, 0.03, 0.005, -0.004)
     aos = M.tddp_events(800, seed=3)
     n = len(aos)
