@@ -233,7 +233,9 @@ void PdfBase::setData(DataSet *data) {
     if(data == nullptr)
         return;
     const size_t dims = observablesList.size();
-    if(dims + 2 > 8)
+    // GF_MAX_OBS = 8 event columns: a binned row adds the content and the volume/error
+    const bool is_binned = dynamic_cast<BinnedDataSet *>(data) != nullptr;
+    if(dims + (is_binned ? 2 : 0) > 8)
         throw GeneralError("{}: {} observables (the engine handles at most 6 with bins, 8 without)", getName(), dims);
     gf_ctx *c = engine::ctx();
     if(auto *unbinned = dynamic_cast<UnbinnedDataSet *>(data)) {

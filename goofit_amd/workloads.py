@@ -264,7 +264,7 @@ def build_tddp(nbins=240, coup_seed=3, resonances=None, mix=TD_MIX, resolution="
     import goofit_amd as G
     G.require_api()
     table = resonances if resonances is not None else DALITZ_RESONANCES[:6]
-    dtime = G.Observable("dtime", -2.0, 4.0)
+    dtime = G.Observable("dtime", -2.0, 20.0)
     sigma = G.Observable("sigmat", 0.0, 0.8)
     m12 = G.Observable("m12", 0.0, 3.0)
     m13 = G.Observable("m13", 0.0, 3.0)

@@ -82,7 +82,7 @@ def threegauss(cf=0.7, tf=0.2, cb=0.05, cs=1.1, tb=-0.1, ts=2.0, ob=0.0, os_=6.0
 def tddp(coup_seed=3, nbins=120, resonances=None, mix=TD_MIX, resolution=None, mistag=False, charmtag=False,
          eff=None):
     """Amp3Body_TD over (dtime, sigma_t, m12, m13, evt[, mistag][, tag]) with the config-2 resonances."""
-    dtime = O.Obs("dtime", -2.0, 4.0, 100)
+    dtime = O.Obs("dtime", -2.0, 20.0, 100)
     sigma = O.Obs("sigmat", 0.0, 0.8, 100)
     m12 = O.Obs("m12", 0.0, 3.0, nbins)
     m13 = O.Obs("m13", 0.0, 3.0, nbins)

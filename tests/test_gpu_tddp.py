@@ -20,13 +20,13 @@ pytestmark = pytest.mark.gpu
 NLL_RTOL = 1e-11
 
 
-def _install(ctx, top, aos):
+def _install(ctx, top, aos, td=None):
     prog, (f, p, c, o, n) = arrays(top)
     ctx.set_program(f, p, c, o, n)
     ctx.set_decay(W.DALITZ_DECAY)
     ctx.upload_columns([np.ascontiguousarray(aos[:, k]) for k in range(aos.shape[1])])
     ctx.set_identity_column(4)
-    ctx.dalitz_set_data_size(top.function_idx, len(aos))
+    ctx.dalitz_set_data_size((td or top).function_idx, len(aos))
     return prog
 
 
@@ -97,7 +97,7 @@ def test_tddp_nll(ctx, kind, tags):
     got = ctx.reduce_metric("NLL")[0]
     assert math.isclose(got, ref, rel_tol=NLL_RTOL), (got, ref)
     # per-event values
-    got_e = ctx.eval_metric("EVAL", n)
+    got_e = ctx.eval_metric("EVAL")
     ref_e = M.op.eval_metric(aos, "EVAL")
     np.testing.assert_allclose(got_e, ref_e, rtol=1e-12, atol=1e-14 * np.abs(ref_e).max())
 
@@ -114,7 +114,7 @@ def test_tddp_in_addpdf(ctx):
     aos = tddp_events(n, seed=12)
     M = O.Model(top, aos, decay=W.DALITZ_DECAY)
     ref = M.calculate_nll()
-    _install(ctx, top, aos)
+    _install(ctx, top, aos, td=top_td)
     M.op.sync_params()
     ctx.update_params(M.op.p)
     ctx.dalitz_cache(top_td.function_idx)
