@@ -163,6 +163,44 @@ def norm_error_study(model, pdf, train_events, sync, target=1e-4, seeds=6):
     return out
 
 
+def tddp_workload(args, sync, n=1_000_000, grid=1414):
+    """Time-dependent Dalitz NLL: the 16 resonances, three-Gaussian resolution, per-event
+    mistag and charm tag, 1e6 events (uniform Dalitz points, exponential decay times);
+    a step moves tau, x and y (no cache or integral redo)."""
+    import goofit_amd as G
+    from goofit_amd import workloads as W
+    mdl = W.build_tddp(nbins=grid, resonances=W.DALITZ_RESONANCES, resolution="threegauss", mistag=True,
+                       charmtag=True)
+    rng = np.random.default_rng(31)
+    m12, m13 = W.dalitz_uniform(n, seed=32)
+    cols = [rng.exponential(W.TD_MIX[0], n), np.full(n, 0.05), m12, m13, np.arange(n, dtype=np.float64),
+            rng.uniform(0.0, 0.3, n), rng.choice([-1.0, 1.0], n)]
+    d = G.UnbinnedDataSet(mdl.observables)
+    d.from_columns(cols)
+    mdl.observables[5].value = 0.1
+    pdf = mdl.pdf
+    pdf.setData(d)
+    pdf.setDataSize(n, 7)
+    sync()
+    t0 = time.perf_counter()
+    pdf.calculateNLL()  # A/B wave caches of 16 resonances + ThreeComplex Gram (6 pair launches) + NLL
+    sync()
+    t_first = time.perf_counter() - t0
+    steps = max(args.steps, 20)
+    dtw = time_steps(pdf, mdl.params, steps, 3, sync)
+    k_ms = G.engine.time_reduce_metric(1, 50) / 50
+    bpe = 2 * 16 * 16 + 6 * 8  # A and B waves of 16 resonances + 6 columns (event number = row)
+    out = dict(events=n, resonances=16, resolution="ThreeGaussResolution", mistag=True, charm_tag=True,
+               nll_events_per_s=round(n * steps / dtw, 1), ms_per_step=round(dtw / steps * 1e3, 4),
+               kernel_ms=round(k_ms, 5), bytes_per_event=bpe, hbm_gbs=round(bpe * n / (k_ms * 1e-3) / 1e9, 1),
+               first_nll_ms=round(t_first * 1e3, 2), norm_grid=f"{grid}x{grid}")
+    del d
+    return out
+
+
+TOY_RAW = {}  # per-toy pulls, sizes, values and errors of the last toy study (not printed)
+
+
 def toy_counts(n, seed0):
     """Per-toy event counts: Poisson with mean uniform in [5e3, 8e4], each from its own seed."""
     out = []
@@ -195,6 +233,12 @@ def fit_toys(model, data, counts, truth, sync, seq_toys, threads):
     ok = valid.astype(bool)
     pulls = (vals[ok][:, free] - tv[free]) / errs[ok][:, free]
     pm, pw = pulls.mean(0), pulls.std(0, ddof=1)
+    # robust width (inter-quartile range / 1.349), tails, and the toys above the median size
+    q75, q25 = np.percentile(pulls, [75, 25], axis=0)
+    pw_robust = (q75 - q25) / 1.349
+    big = counts[ok] >= np.median(counts)
+    pw_big = pulls[big].std(0, ddof=1)
+    n_tail = int(np.sum(np.any(np.abs(pulls) > 5, axis=1)))
     # sequential fits of the first toys, one data set at a time
     seq_vals, t_seq = [], 0.0
     cols = [data.column(o) for o in model.observables]
@@ -235,7 +279,11 @@ def fit_toys(model, data, counts, truth, sync, seq_toys, threads):
                 pull_width=[round(float(v), 4) for v in pw],
                 max_abs_pull_mean=round(float(np.abs(pm).max()), 4),
                 max_abs_pull_width_minus_1=round(float(np.abs(pw - 1).max()), 4),
-                batched_vs_sequential_max_shift_sigma=round(float(max(shifts)), 5) if shifts else None)
+                pull_width_robust=[round(float(v), 4) for v in pw_robust],
+                pull_width_large_toys=[round(float(v), 4) for v in pw_big],
+                toys_with_pull_beyond_5=n_tail,
+                batched_vs_sequential_max_shift_sigma=round(float(max(shifts)), 5) if shifts else None,
+                _raw=dict(pulls=pulls, counts=counts[ok], vals=vals[ok], errs=errs[ok], truth=tv, free=free))
 
 
 def toy_study(ntoys, sync, seq_toys=16, threads=256):
@@ -256,6 +304,7 @@ def toy_study(ntoys, sync, seq_toys=16, threads=256):
     d.from_columns([xs])
     m.pdf.setData(d)
     out["add1d"] = fit_toys(m, d, counts, truth, sync, seq_toys, threads)
+    TOY_RAW["add1d"] = out["add1d"].pop("_raw")
     out["add1d"]["generation_s"] = round(t_gen, 2)
     log("add1d toys", {k: out["add1d"][k] for k in ("batched_s", "batched_toys_per_hour",
                                                     "sequential_toys_per_hour", "max_abs_pull_mean",
@@ -275,6 +324,7 @@ def toy_study(ntoys, sync, seq_toys=16, threads=256):
     md.pdf.setForceIntegrals()
     md.pdf.calculateNLL()
     out["dalitz8"] = fit_toys(md, dd, counts, truth, sync, seq_toys, threads)
+    TOY_RAW["dalitz8"] = out["dalitz8"].pop("_raw")
     out["dalitz8"]["generation_s"] = round(t_gen, 2)
     log("dalitz8 toys", {k: out["dalitz8"][k] for k in ("batched_s", "batched_toys_per_hour",
                                                         "sequential_toys_per_hour", "max_abs_pull_mean",
@@ -463,6 +513,8 @@ def main():
                                    hbm_gbs=round(bpe * nevt / (k_ms * 1e-3) / 1e9, 1))
             log(name, workloads[name])
             del d
+        workloads["tddp16_1e6_threegauss"] = tddp_workload(args, sync)
+        log("tddp16_1e6_threegauss", workloads["tddp16_1e6_threegauss"])
 
     cpu = None
     if not args.no_cpu:

@@ -392,12 +392,15 @@ __device__ __forceinline__ void tg_gaussian(double &P1, double &P2, double &P3, 
     P3 = ng * 0.5 * (Ipy - Imy);
 }
 
-// device_threegauss_resolution, ThreeGaussResolution.cu:63-127: core, tail and
-// outlier Gaussians (fractions f_c, f_t, 1 - f_c - f_t) with per-event width sigma.
-// R = the 13 parameters: fractions, biases and scales, selection bias, common offsets.
-__device__ __forceinline__ double threegauss_resolution(const double *__restrict__ R, double coshterm,
-                                                        double costerm, double sinhterm, double sinterm, double tau,
-                                                        double dtime, double xmixing, double ymixing, double sigma) {
+// The decay-time terms P1..P4 (cosh, cos, sinh, sin parts) of the three-Gaussian model,
+// device_threegauss_resolution, ThreeGaussResolution.cu:63-115: core, tail and outlier
+// Gaussians (fractions f_c, f_t, 1 - f_c - f_t) with per-event width sigma.  R = the 13
+// parameters: fractions, biases and scales, selection bias, common offsets.
+struct TimeTerms {
+    double P1, P2, P3, P4;
+};
+__device__ __forceinline__ TimeTerms threegauss_terms(const double *__restrict__ R, double tau, double dtime,
+                                                      double xmixing, double ymixing, double sigma) {
     const double fc = R[0], ft = R[1], fo = 1 - fc - ft;
     const double bc = R[2] + R[9], bt = R[4] + R[11], bo = R[6] + R[11];
     const double sc = R[3] * R[10], st = R[5] * R[12], so = R[7] * R[12];
@@ -406,15 +409,21 @@ __device__ __forceinline__ double threegauss_resolution(const double *__restrict
     tg_gaussian(c1, c2, c3, c4, tau, dtime - bc * sigma, xmixing, ymixing, sc * sigma, sb);
     tg_gaussian(t1, t2, t3, t4, tau, dtime - bt * sigma, xmixing, ymixing, st * sigma, sb);
     tg_gaussian(o1, o2, o3, o4, tau, dtime - bo * sigma, xmixing, ymixing, so * sigma, sb);
-    const double P1 = fc * c1 + ft * t1 + fo * o1;
-    const double P2 = fc * c2 + ft * t2 + fo * o2;
-    const double P3 = fc * c3 + ft * t3 + fo * o3;
-    const double P4 = fc * c4 + ft * t4 + fo * o4;
+    TimeTerms P;
+    P.P1 = fc * c1 + ft * t1 + fo * o1;
+    P.P2 = fc * c2 + ft * t2 + fo * o2;
+    P.P3 = fc * c3 + ft * t3 + fo * o3;
+    P.P4 = fc * c4 + ft * t4 + fo * o4;
+    return P;
+}
+// ThreeGaussResolution.cu:117-122: the rate from the terms
+__device__ __forceinline__ double threegauss_rate(const TimeTerms &P, double coshterm, double costerm,
+                                                  double sinhterm, double sinterm) {
     double ret = 0;
-    ret += coshterm * P1;
-    ret += costerm * P2;
-    ret -= 2 * sinhterm * P3;
-    ret -= 2 * sinterm * P4;
+    ret += coshterm * P.P1;
+    ret += costerm * P.P2;
+    ret -= 2 * sinhterm * P.P3;
+    ret -= 2 * sinterm * P.P4;
     return ret;
 }
 
@@ -423,7 +432,7 @@ __device__ __forceinline__ double time_resolution(int kind, const double *__rest
                                                   double abr, double abi, double tau, double dtime, double x,
                                                   double y, double sigma) {
     if(kind == GT_THREEGAUSS)
-        return threegauss_resolution(R, t1, t2, abr, abi, tau, dtime, x, y, sigma);
+        return threegauss_rate(threegauss_terms(R, tau, dtime, x, y, sigma), t1, t2, abr, abi);
     return truth_resolution(t1, t2, abr, abi, tau, dtime, x, y);
 }
 

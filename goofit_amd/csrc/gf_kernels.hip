@@ -255,8 +255,16 @@ __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs
                 const double mis = dz.col_mis >= 0 ? src.get(dz.col_mis, j) : 0.0;
                 if(mis > 0) {
                     ret *= (1 - 2 * mis);
-                    ret += mis * gfd::time_resolution(dz.res_kind, R, t1, -t2, abr, -abi, taufix, t, xfix, yfix, sg);
-                    ret += mis * gfd::time_resolution(dz.res_kind, R, t1, t2, abr, abi, taufix, t, xfix, yfix, sg);
+                    if(dz.res_kind == GT_THREEGAUSS) {
+                        // both mistag calls take the same (taufix, t, xfix, yfix, sigma): their
+                        // Gaussian terms are evaluated once and combined twice (same values)
+                        const gfd::TimeTerms Pf = gfd::threegauss_terms(R, taufix, t, xfix, yfix, sg);
+                        ret += mis * gfd::threegauss_rate(Pf, t1, -t2, abr, -abi);
+                        ret += mis * gfd::threegauss_rate(Pf, t1, t2, abr, abi);
+                    } else {
+                        ret += mis * gfd::truth_resolution(t1, -t2, abr, -abi, taufix, t, xfix, yfix);
+                        ret += mis * gfd::truth_resolution(t1, t2, abr, abi, taufix, t, xfix, yfix);
+                    }
                 }
                 a[j] = ret;
             }
