@@ -222,18 +222,35 @@ __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs
                 en[j]  = e;
                 sar[j] = sai[j] = sbr[j] = sbi[j] = 0.0;
             }
-            for(int i = 0; i < in.a; ++i) {
-                // A += a_i * c_i, B += b_i * c_i (Amp3Body_TD.cu:91-110)
-                const double cr = P[in.p + 5 + 2 * i], ci = P[in.p + 6 + 2 * i];
-                const double2 *__restrict__ wa = dz.wave + (int64_t)i * dz.ncache;
-                const double2 *__restrict__ wb = dz.wave_b + (int64_t)i * dz.ncache;
+            // A += a_i * c_i, B += b_i * c_i (Amp3Body_TD.cu:91-110).  The A and B waves of
+            // GF_TD_CHUNK resonances are loaded before any is consumed (2 x 4 x EPT 16-byte
+            // loads in flight per lane); the sums still run over i = 0, 1, ... in order.
+            constexpr int GF_TD_CHUNK = 4;
+            for(int i0 = 0; i0 < in.a; i0 += GF_TD_CHUNK) {
+                double2 wa[GF_TD_CHUNK][EPT], wb[GF_TD_CHUNK][EPT];
 #pragma unroll
-                for(int j = 0; j < EPT; ++j) {
-                    const double2 u = wa[en[j]], v = wb[en[j]];
-                    sar[j] += u.x * cr - u.y * ci;
-                    sai[j] += u.x * ci + u.y * cr;
-                    sbr[j] += v.x * cr - v.y * ci;
-                    sbi[j] += v.x * ci + v.y * cr;
+                for(int u = 0; u < GF_TD_CHUNK; ++u) {
+                    const bool use = i0 + u < in.a;
+                    const int64_t off = (int64_t)(use ? i0 + u : 0) * dz.ncache;
+#pragma unroll
+                    for(int j = 0; j < EPT; ++j) {
+                        wa[u][j] = use ? dz.wave[off + en[j]] : make_double2(0.0, 0.0);
+                        wb[u][j] = use ? dz.wave_b[off + en[j]] : make_double2(0.0, 0.0);
+                    }
+                }
+#pragma unroll
+                for(int u = 0; u < GF_TD_CHUNK; ++u) {
+                    if(i0 + u < in.a) {
+                        const double cr = P[in.p + 5 + 2 * (i0 + u)], ci = P[in.p + 6 + 2 * (i0 + u)];
+#pragma unroll
+                        for(int j = 0; j < EPT; ++j) {
+                            const double2 a_ = wa[u][j], b_ = wb[u][j];
+                            sar[j] += a_.x * cr - a_.y * ci;
+                            sai[j] += a_.x * ci + a_.y * cr;
+                            sbr[j] += b_.x * cr - b_.y * ci;
+                            sbi[j] += b_.x * ci + b_.y * cr;
+                        }
+                    }
                 }
             }
             const double tau = P[in.p], x0 = P[in.p + 1], y0 = P[in.p + 2], dx = P[in.p + 3], dy = P[in.p + 4];
