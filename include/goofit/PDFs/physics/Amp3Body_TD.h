@@ -31,7 +31,7 @@ class Amp3Body_TD : public Amp3BodyBase {
                 EventNumber eventNumber, DecayInfo3t decay, MixingTimeResolution *r, GooPdf *eff,
                 Observable *mistag = nullptr, Observable *charmtag = nullptr);
     /// The reference's variant with one resolution per bin of the D0 mass; its
-    /// device function never selects among them (Amp3Body_TD.cu:171-208 are
+    /// device function never selects among them (Amp3Body_TD.cu:176-205 are
     /// commented out), so this constructor is refused with an error.
     Amp3Body_TD(std::string n, Observable _dtime, Observable _sigmat, Observable m12, Observable m13,
                 EventNumber eventNumber, DecayInfo3t decay, std::vector<MixingTimeResolution *> &r, GooPdf *eff,
