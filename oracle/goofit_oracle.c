@@ -21,7 +21,13 @@
  *     lineshapes (src/PDFs/physics/resonances/{Resonance,RBW,GS,NonRes}.cu),
  *     the wave cache (detail/SpecialResonanceCalculator.cu:12-46), the grid
  *     integrals (detail/SpecialResonanceIntegrator.cu:7-121) and the coherent
- *     sum (src/PDFs/physics/Amp3Body.cu:60-122).
+ *     sum (src/PDFs/physics/Amp3Body.cu:60-122),
+ *   - the time-dependent Dalitz amplitude: the rate with mistag and charm tag
+ *     (src/PDFs/physics/Amp3Body_TD.cu:41-257), the truth and three-Gaussian
+ *     resolutions and their normalisations (TruthResolution.cu, ThreeGaussResolution.cu),
+ *     the A/B waves (detail/SpecialWaveCalculator.cu:13-63), the ThreeComplex grid
+ *     integrals (detail/SpecialDalitzIntegrator.cu:9-126) and the host assembly of
+ *     the normalisation (Amp3Body_TD.cu:686-751).
  *
  * Built with -ffp-contract=off so that every product and sum rounds exactly
  * as written (no fused multiply-add), as on the reference's x86 OMP build.
@@ -35,8 +41,15 @@
  *   RBW/GS: independent Python formulas of the published lineshapes plus
  *   the quirks A5/A6 reproduced on purpose; integrals: Hermitian and positive
  *   semi-definite, and the closed-form area for a flat amplitude; the chunked
- *   summation: compared with exact (fsum) sums.
- *   Parity unpinned: none of the functions here; see the tests for each check.
+ *   summation: compared with exact (fsum) sums.  Time-dependent Dalitz: decay-time
+ *   integrals in closed form, the resolutions' normalisations against numerical
+ *   integrals of their event functions, the narrow-resolution limit, the x = y = 0
+ *   reduction to the Amp3Body density, the charm-tag and mistag identities, and the
+ *   normalisation as the integral of the event density (tests/test_tddp_oracle.py).
+ *   Parity unpinned: the GS lineshape away from its pole, RBW spin 1 and 2 away from
+ *   the pole (the Blatt-Weisskopf ratios), the MultiPolynomial evaluation order
+ *   (checked only through degree-0 and degree-1 cases), and quirks A5/A6 beyond
+ *   their restatement.
  */
 #include <math.h>
 #include <stdint.h>
