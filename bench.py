@@ -443,7 +443,11 @@ def main():
     reps = 200
     ms_sweep, ms_span = G.engine.time_reduce_sweep(1, reps)
     ms_kernel = ms_sweep / reps
-    bytes_per_event = 16 * 16 + 16  # 16 complex waves + (m12, m13); event number = row index
+    # 16 complex waves; event number = row index; m12 and m13 are not read when the cache
+    # (filled from these events, 0 outside the Dalitz region) settles inDalitz and the
+    # efficiency is constant (GF_TUNE_ZERO_OUTSIDE, on by default)
+    zero_outside = G.engine.get_tuning(G.engine.TUNE_ZERO_OUTSIDE) == 1
+    bytes_per_event = 16 * 16 + (0 if zero_outside else 16)
     achieved = bytes_per_event * args.events / (ms_kernel * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic() if args.events == 1_000_000 else (None, None)
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",

@@ -703,6 +703,7 @@ PYBIND11_MODULE(_goofit, m) {
     me.attr("TUNE_BLOCKS_PER_CU")       = (int)GF_TUNE_BLOCKS_PER_CU;
     me.attr("TUNE_FAST_BLOCKS_PER_CU")  = (int)GF_TUNE_FAST_BLOCKS_PER_CU;
     me.attr("TUNE_SPIN_US")             = (int)GF_TUNE_SPIN_US;
+    me.attr("TUNE_ZERO_OUTSIDE")        = (int)GF_TUNE_ZERO_OUTSIDE;
     me.def("callnumber", []() { return host_callnumber; });
     me.def("flattened", []() {
         return py::make_tuple(host_function_table, host_parameters, host_constants, host_observables,

@@ -47,6 +47,10 @@ struct WaveCache {
     double2 *wave = nullptr;    // GF_MAX_RES x n complex
     double2 *wave_b = nullptr;  // time-dependent nodes: the B waves, same layout
     int64_t n = 0, offset = 0;
+    // resonances whose waves were computed from the current event columns (events_gen,
+    // m12/m13 columns): with all of them filled, the cache is 0 exactly outside the plot
+    uint64_t filled = 0, gen = 0;
+    int col12 = -1, col13 = -1;
 };
 
 // A Dalitz node of the installed program.
@@ -139,7 +143,9 @@ struct gf_ctx {
     uint32_t res_seq = 0;  // sequence number of the last reduction launched
 
     // launch-shape knobs (gf_tune)
-    int tune[GF_TUNE_COUNT] = {1, 1, 4, 4, 2000};
+    int tune[GF_TUNE_COUNT] = {1, 1, 4, 4, 2000, 1};
+
+    uint64_t events_gen = 1;  // bumped whenever the event columns may change
 };
 
 namespace {
@@ -839,6 +845,23 @@ bool use_fast_dalitz(const gf_ctx *ctx, int metric) {
 // The fast path's efficiency polynomial has maximum degree 0.
 bool const_efficiency(const gf_ctx *ctx) { return (int)ctx->consts[ctx->ins[1].c] == 0; }
 
+// The fast path may skip m12/m13: constant finite efficiency, and every resonance's wave
+// computed from the current event columns starting at row 0, so the cache is exactly 0
+// outside the Dalitz region (k_dalitz_cache writes 0 there) and |A|^2 eff is 0 there too.
+bool zero_outside(const gf_ctx *ctx) {
+    if(!ctx->tune[GF_TUNE_ZERO_OUTSIDE] || !const_efficiency(ctx) || ctx->dalitz.empty())
+        return false;
+    const DalitzNode &d = ctx->dalitz[0];
+    auto it             = ctx->caches.find(d.cache_key);
+    if(it == ctx->caches.end())
+        return false;
+    const WaveCache &w  = it->second;
+    const uint64_t all = d.nres >= 64 ? ~0ull : ((1ull << d.nres) - 1ull);
+    const double eff   = ctx->params[ctx->ins[1].p + ctx->ins[1].a];
+    return w.offset == 0 && w.gen == ctx->events_gen && w.col12 == d.col_m12 && w.col13 == d.col_m13
+           && (w.filled & all) == all && std::isfinite(eff);
+}
+
 // Efficiency handling of the point kernels (Gram, generator): a MultiPolynomial
 // of maximum degree 0 is its single coefficient (device_MultiPolynomial,
 // PolynomialPdf.cu:47-118, with numBoxes = 1), passed as a constant (mode -2);
@@ -884,7 +907,7 @@ gf_status launch_reduce(gf_ctx *ctx, int metric, hipEvent_t mark = nullptr) {
     a.mark       = mark;
     int rc;
     if(use_fast_dalitz(ctx, metric))
-        rc = gfk::launch_dalitz_fast(a, have_inl ? &inl : nullptr, const_efficiency(ctx), metric, ctx->d_partials,
+        rc = gfk::launch_dalitz_fast(a, have_inl ? &inl : nullptr, const_efficiency(ctx), zero_outside(ctx), metric, ctx->d_partials,
                                      ctx->d_res_mapped, nullptr, fast_blocks(ctx, ctx->n),
                                      ctx->stream);
     else
@@ -1046,6 +1069,7 @@ gf_status gf_ctx_set_tuning(gf_ctx *ctx, int key, int value) {
     switch(key) {
     case GF_TUNE_INLINE_ARGS:
     case GF_TUNE_FAST_DALITZ:
+    case GF_TUNE_ZERO_OUTSIDE:
         if(value != 0 && value != 1)
             return fail(GF_ERR_INVALID, "tuning key %d takes 0 or 1 (got %d)", key, value);
         break;
@@ -1186,6 +1210,7 @@ static gf_status alloc_columns(gf_ctx *ctx, int ncols, int64_t n) {
     }
     ctx->ncols       = ncols;
     ctx->n           = n;
+    ctx->events_gen++;
     ctx->identity_ok = false;
     ctx->identity_col = -1;
     size_t bytes     = (size_t)ncols * (size_t)(n > 0 ? n : 1) * sizeof(double);
@@ -1246,6 +1271,7 @@ gf_status gf_events_column(gf_ctx *ctx, int k, double **dev_ptr) {
     if(!dev_ptr || k < 0 || k >= ctx->ncols || !ctx->col_base)
         return fail(GF_ERR_INVALID, "column %d not allocated", k);
     *dev_ptr = ctx->col_base + (size_t)k * ctx->n;
+    ctx->events_gen++;  // the caller may write the column
     return GF_OK;
 }
 
@@ -1414,7 +1440,7 @@ gf_status gf_eval_metric_device(gf_ctx *ctx, int metric, double *dev_out, int64_
     GfProgArgs a = prog_args(ctx);
     int rc;
     if(use_fast_dalitz(ctx, metric))
-        rc = gfk::launch_dalitz_fast(a, have_inl ? &inl : nullptr, const_efficiency(ctx), metric, ctx->d_partials,
+        rc = gfk::launch_dalitz_fast(a, have_inl ? &inl : nullptr, const_efficiency(ctx), zero_outside(ctx), metric, ctx->d_partials,
                                      ctx->d_res_mapped, dev_out, fast_blocks(ctx, ctx->n),
                                      ctx->stream);
     else
@@ -1553,6 +1579,7 @@ gf_status gf_dalitz_set_data_size(gf_ctx *ctx, int dalitz_func, int64_t n, int64
     }
     wc.n      = n;
     wc.offset = offset;
+    wc.filled = 0;
     return upload_dalitz_table(ctx);
 }
 
@@ -1588,6 +1615,14 @@ gf_status gf_dalitz_cache(gf_ctx *ctx, int dalitz_func, uint64_t res_mask) {
         return fail(GF_ERR_HIP, "cache launch failed (%d)", rc);
     GF_HIP(hipStreamSynchronize(ctx->stream));
     ctx->stage_in_flight = false;
+    WaveCache &w = it->second;
+    if(w.gen != ctx->events_gen || w.col12 != d->col_m12 || w.col13 != d->col_m13) {
+        w.filled = 0;
+        w.gen    = ctx->events_gen;
+        w.col12  = d->col_m12;
+        w.col13  = d->col_m13;
+    }
+    w.filled |= mask;
     return GF_OK;
 }
 

@@ -161,8 +161,10 @@ int launch_eval_metric(const GfProgArgs &a, const GfInline *inl, int metric, dou
 // Coherent-sum fast path: program [DALITZ, MULTI_POLY efficiency, DALITZ_END, END],
 // event number = row index, every row inside the wave cache.
 // const_eff: the efficiency polynomial has degree 0 (a constant coefficient).
-int launch_dalitz_fast(const GfProgArgs &a, const GfInline *inl, bool const_eff, int metric, Partial *partials,
-                       GfResult *result, double *out, int nblocks, void *stream);
+// zero_outside (with const_eff): the wave cache was filled from these events, so m12/m13
+// are not read (the waves are 0 outside the Dalitz region).
+int launch_dalitz_fast(const GfProgArgs &a, const GfInline *inl, bool const_eff, bool zero_outside, int metric,
+                       Partial *partials, GfResult *result, double *out, int nblocks, void *stream);
 int launch_reduce_bins(const GfProgArgs &a, int pc0, const double *ranges, const int *slots, int k, int64_t total,
                        Partial *partials, GfResult *result, int nblocks, void *stream);
 // Toy batch: per-workgroup sweep with per-toy parameters, per-toy finishing

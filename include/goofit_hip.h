@@ -107,7 +107,11 @@ enum gf_tune {
     GF_TUNE_FAST_BLOCKS_PER_CU = 3, /* coherent-sum kernel grid cap, workgroups per CU (1..64, default 4) */
     GF_TUNE_SPIN_US            = 4, /* a reduction's caller polls the result for up to this many
                                        microseconds before blocking on the stream (0..1e6, default 2000) */
-    GF_TUNE_COUNT              = 5
+    GF_TUNE_ZERO_OUTSIDE       = 5, /* 1: with a constant efficiency and a wave cache filled from the
+                                       current events, the coherent-sum kernel takes |sum c_i W_i|^2 as 0
+                                       outside the Dalitz region from the cache (which holds 0 there)
+                                       instead of reading m12 and m13 (default 1) */
+    GF_TUNE_COUNT              = 6
 };
 
 /* Set / read one knob (gf_tune); values out of range give GF_ERR_INVALID. */

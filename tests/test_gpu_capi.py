@@ -341,6 +341,39 @@ def test_dalitz_nll_paths(ctx):
     np.testing.assert_allclose(got_i, ref_i, rtol=1e-11, atol=1e-13 * np.abs(ref_i).max())
 
 
+def test_dalitz_zero_outside_cache(ctx):
+    """The coherent-sum kernel that takes the cache's zeros outside the Dalitz region
+    (constant efficiency, cache filled from these events) gives the same NLL and per-event
+    values as the one that reads m12/m13; after new events are uploaded without refilling
+    the cache it is not used."""
+    n = 40_000
+    top, M, aos = _dalitz_setup(ctx, n, nbins=80, seed=23)
+    assert (~W.in_dalitz_np(aos[:, 0], aos[:, 1])).sum() > 1000
+    ref = M.calculate_nll()
+    M.op.sync_params()
+    ctx.update_params(M.op.p)
+    ctx.dalitz_cache(top.function_idx)
+    _sync(ctx, M)
+    out = {}
+    for zo in (1, 0):
+        ctx.set_tuning(capi.TUNE_ZERO_OUTSIDE, zo)
+        out[zo] = (ctx.reduce_metric("NLL")[0], ctx.eval_metric("NLL"))
+    ctx.set_tuning(capi.TUNE_ZERO_OUTSIDE, 1)
+    assert out[1][0] == out[0][0]
+    assert math.isclose(out[1][0], ref, rel_tol=NLL_RTOL), (out[1][0], ref)
+    np.testing.assert_array_equal(out[1][1], out[0][1])
+    # new m12/m13 values with the old cache: the kernel must read the masses again
+    other = dalitz_events(n, 24)
+    ctx.upload_columns([other[:, 0], other[:, 1], other[:, 2]])
+    ctx.set_identity_column(2)
+    stale = {}
+    for zo in (1, 0):
+        ctx.set_tuning(capi.TUNE_ZERO_OUTSIDE, zo)
+        stale[zo] = ctx.reduce_metric("NLL")[0]
+    ctx.set_tuning(capi.TUNE_ZERO_OUTSIDE, 1)
+    assert stale[1] == stale[0]
+
+
 def test_dalitz_generate(ctx):
     """Accept-reject generation on the GPU: reproducible for a seed, every event inside the plot,
     and the histogram over cells fully inside the plot follows |A|^2 (chi^2 per degree of freedom)."""
