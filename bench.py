@@ -456,6 +456,25 @@ def main():
                     reduction_ms=round(ms_span / reps, 5), bytes_per_event=bytes_per_event,
                     bytes_per_launch=bytes_per_event * args.events)
     log(f"NLL sweep {ms_kernel * 1e3:.1f} us -> {achieved:.0f} GB/s; whole reduction {ms_span / reps * 1e3:.1f} us")
+    # practical streaming-read rate of this device: torch's own reduction over 256 MiB,
+    # timed the same way (events around each launch on the launching stream)
+    buf = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+    for _ in range(5):
+        buf.sum()
+    sync()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    tot = 0.0
+    for _ in range(50):
+        e0.record()
+        buf.sum()
+        e1.record()
+        e1.synchronize()
+        tot += e0.elapsed_time(e1)
+    del buf
+    ref_gbs = (32 << 20) * 8 / (tot / 50 * 1e-3) / 1e9
+    roofline["device_read_gbs_torch_sum"] = round(ref_gbs, 1)
+    roofline["frac_of_device_read"] = round(achieved / ref_gbs, 4)
+    log(f"torch.sum over 256 MiB: {ref_gbs:.0f} GB/s (the sweep reaches {achieved / ref_gbs:.0%} of it)")
 
     extra = dict(events=len(m12), generation_s=round(t_gen, 2), first_nll_ms=round(t_first * 1e3, 2), nll=nll0)
     grid6 = [0.0, 3.0, float(args.grid), 0.0, 3.0, float(args.grid)]
