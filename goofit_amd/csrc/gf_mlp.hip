@@ -227,28 +227,50 @@ __global__ __launch_bounds__(1024) void k_loss(const float *__restrict__ out, in
 // ----------------------------------------------------------------- backward
 
 // delta2 = coef * w3 * silu'(z2) (stored as bf16) and this block's partial sums
-// over its 64 samples of dL/dw3, dL/db2 and dL/db3.  One lane per neuron.
-__global__ __launch_bounds__(GF_MLP_H) void k_back_out(const float *__restrict__ z2, const float *__restrict__ coef,
-                                                       const float *__restrict__ P, int n,
-                                                       uint16_t *__restrict__ delta2, float *__restrict__ part) {
-    const int j = threadIdx.x;
-    const int s0 = blockIdx.x * 64;
+// over its 64 samples of dL/dw3, dL/db2 and dL/db3.  Lane j of each of the
+// GF_BO_SPLIT lane groups takes neuron j on a contiguous quarter of the samples;
+// the groups' sums are added in group order.
+#define GF_BO_SPLIT 4
+__global__ __launch_bounds__(GF_MLP_H *GF_BO_SPLIT) void k_back_out(const float *__restrict__ z2,
+                                                                   const float *__restrict__ coef,
+                                                                   const float *__restrict__ P, int n,
+                                                                   uint16_t *__restrict__ delta2,
+                                                                   float *__restrict__ part) {
+    __shared__ float red[GF_BO_SPLIT][2 * GF_MLP_H + 1];
+    const int j  = threadIdx.x % GF_MLP_H, g = threadIdx.x / GF_MLP_H;
+    const int s0 = blockIdx.x * 64 + g * (64 / GF_BO_SPLIT);
     const float wj = P[GF_MLP_W3 + j];
     float gw3 = 0.f, gb2 = 0.f, gb3 = 0.f;
-    for(int s = s0; s < s0 + 64 && s < n; ++s) {
-        const float z  = z2[(int64_t)s * GF_MLP_H + j];
-        const float cs = coef[s];
-        const float d  = cs * wj * dsilu(z);
-        delta2[(int64_t)s * GF_MLP_H + j] = f2bf(d);
-        gw3 += cs * silu(z);
-        gb2 += d;
-        gb3 += cs;
+#pragma unroll 4
+    for(int s = s0; s < s0 + 64 / GF_BO_SPLIT; ++s) {
+        if(s < n) {
+            const float z  = z2[(int64_t)s * GF_MLP_H + j];
+            const float cs = coef[s];
+            const float d  = cs * wj * dsilu(z);
+            delta2[(int64_t)s * GF_MLP_H + j] = f2bf(d);
+            gw3 += cs * silu(z);
+            gb2 += d;
+            gb3 += cs;
+        }
     }
-    float *p = part + (int64_t)blockIdx.x * GF_MLP_PART_OUT;
-    p[j]            = gw3;
-    p[GF_MLP_H + j] = gb2;
+    red[g][j]            = gw3;
+    red[g][GF_MLP_H + j] = gb2;
     if(j == 0)
-        p[2 * GF_MLP_H] = gb3;
+        red[g][2 * GF_MLP_H] = gb3;
+    __syncthreads();
+    if(g == 0) {
+        float a = red[0][j], b = red[0][GF_MLP_H + j], c = red[0][2 * GF_MLP_H];
+        for(int q = 1; q < GF_BO_SPLIT; ++q) {
+            a += red[q][j];
+            b += red[q][GF_MLP_H + j];
+            c += red[q][2 * GF_MLP_H];
+        }
+        float *p = part + (int64_t)blockIdx.x * GF_MLP_PART_OUT;
+        p[j]            = a;
+        p[GF_MLP_H + j] = b;
+        if(j == 0)
+            p[2 * GF_MLP_H] = c;
+    }
 }
 
 // dW2[j][k] = sum_s delta2[s][j] h1[s][k] over this block's samples (BF16 MFMA,
@@ -381,33 +403,52 @@ __global__ __launch_bounds__(256) void k_back_h1(const uint16_t *__restrict__ de
     }
 }
 
-// Gradient vector: sum of the per-block partials, in block order.
+// Gradient vector: sum of the per-block partials.  A workgroup takes 64
+// parameters; its 4 waves sum the partials b = w, w + 4, w + 8, ... (two
+// interleaved accumulators each, so consecutive loads are independent), then
+// the four sums are added in wave order: a fixed order for a given launch.
 __global__ __launch_bounds__(256) void k_grad_sum(const float *__restrict__ p_in, int nb_in,
                                                   const float *__restrict__ p_w2, int nb_w2,
                                                   const float *__restrict__ p_out, int nb_out, float *__restrict__ g) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= GF_MLP_NPARAM)
-        return;
-    float s = 0.f;
+    __shared__ float red[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i    = blockIdx.x * 64 + lane;
+    const float *base = p_in;
+    int64_t stride    = GF_MLP_PART_IN;
+    int nb = nb_in, o = 0;
     if(i < GF_MLP_W2) {  // W1 (interleaved k, input) then b1
-        for(int b = 0; b < nb_in; ++b)
-            s += p_in[(int64_t)b * GF_MLP_PART_IN + i];
+        o = i;
     } else if(i < GF_MLP_B2) {
-        for(int b = 0; b < nb_w2; ++b)
-            s += p_w2[(int64_t)b * GF_MLP_H * GF_MLP_H + (i - GF_MLP_W2)];
+        base   = p_w2;
+        stride = (int64_t)GF_MLP_H * GF_MLP_H;
+        nb     = nb_w2;
+        o      = i - GF_MLP_W2;
     } else {
         // b2, w3, b3 live in the output partials as [w3 | b2 | b3]
-        int o;
+        base   = p_out;
+        stride = GF_MLP_PART_OUT;
+        nb     = nb_out;
         if(i < GF_MLP_W3)
             o = GF_MLP_H + (i - GF_MLP_B2);
         else if(i < GF_MLP_B3)
             o = i - GF_MLP_W3;
         else
             o = 2 * GF_MLP_H;
-        for(int b = 0; b < nb_out; ++b)
-            s += p_out[(int64_t)b * GF_MLP_PART_OUT + o];
     }
-    g[i] = s;
+    float s0 = 0.f, s1 = 0.f;
+    if(i < GF_MLP_NPARAM) {
+        int b = w;
+        for(; b + 4 < nb; b += 8) {
+            s0 += base[(int64_t)b * stride + o];
+            s1 += base[(int64_t)(b + 4) * stride + o];
+        }
+        if(b < nb)
+            s0 += base[(int64_t)b * stride + o];
+    }
+    red[w][lane] = s0 + s1;
+    __syncthreads();
+    if(w == 0 && i < GF_MLP_NPARAM)
+        g[i] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
 
 // Advance the training state by one step: batch key, cosine-decayed learning
@@ -565,7 +606,8 @@ int launch_loss(const float *out, int nd, int nu, float *coef, float *loss, hipS
 
 int launch_backward(const GfMlpDev &D, const float2 *xy, const float *coef, int n, float *grad, hipStream_t s) {
     const int nb = n / 64;
-    hipLaunchKernelGGL(k_back_out, dim3(nb), dim3(GF_MLP_H), 0, s, D.z2, coef, D.P, n, D.delta2, D.part_out);
+    hipLaunchKernelGGL(k_back_out, dim3(nb), dim3(GF_MLP_H * GF_BO_SPLIT), 0, s, D.z2, coef, D.P, n, D.delta2,
+                       D.part_out);
     int rc = check(hipGetLastError());
     if(rc)
         return rc;
@@ -578,7 +620,7 @@ int launch_backward(const GfMlpDev &D, const float2 *xy, const float *coef, int 
     hipLaunchKernelGGL(k_back_h1, dim3(nb), dim3(256), 0, s, D.delta2, D.w2t, D.z1, xy, n, D.part_in);
     if((rc = check(hipGetLastError())))
         return rc;
-    hipLaunchKernelGGL(k_grad_sum, dim3(nblk(GF_MLP_NPARAM, 256)), dim3(256), 0, s, D.part_in, nb, D.part_w2, nbw,
+    hipLaunchKernelGGL(k_grad_sum, dim3(nblk(GF_MLP_NPARAM, 64)), dim3(256), 0, s, D.part_in, nb, D.part_w2, nbw,
                        D.part_out, nb, grad);
     return check(hipGetLastError());
 }
