@@ -462,14 +462,13 @@ def main():
     for _ in range(5):
         buf.sum()
     sync()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    tot = 0.0
-    for _ in range(50):
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(50)]
+    for e0, e1 in evs:  # back to back, as the sweep's launches are timed
         e0.record()
         buf.sum()
         e1.record()
-        e1.synchronize()
-        tot += e0.elapsed_time(e1)
+    sync()
+    tot = sum(e0.elapsed_time(e1) for e0, e1 in evs)
     del buf
     ref_gbs = (32 << 20) * 8 / (tot / 50 * 1e-3) / 1e9
     roofline["device_read_gbs_torch_sum"] = round(ref_gbs, 1)
