@@ -285,6 +285,7 @@ def fit_toys(model, data, counts, truth, sync, seq_toys, threads):
                 sequential_toys_per_hour=round(seq_toys / t_seq * 3600, 1) if t_seq > 0 else None,
                 speedup=round((n / t_batch) / (seq_toys / t_seq), 3) if t_seq > 0 else None,
                 launches=st["launches"], fcn_calls=st["fcn_calls"], gpu_wait_s=round(st["gpu_wait_s"], 3),
+                host_prep_s=round(st["host_prep_s"], 3), gather_s=round(st["gather_s"], 3),
                 chunk=st["chunk"], pull_mean=[round(float(v), 4) for v in pm],
                 pull_width=[round(float(v), 4) for v in pw],
                 max_abs_pull_mean=round(float(np.abs(pm).max()), 4),
@@ -379,7 +380,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--events", type=int, default=1_000_000)
     ap.add_argument("--grid", type=int, default=1414)
     ap.add_argument("--no-fit", action="store_true")
@@ -432,16 +433,17 @@ def main():
         log("profile-only run done: 20 NLL reductions, 3 copies of 256 MiB")
         return
 
+    # dominant kernel: the coherent-sum NLL sweep over the events, timed with HIP
+    # events on the engine's stream around each sweep launch (the one-workgroup
+    # kernel that adds the per-workgroup partials is timed separately); measured
+    # first, so that the device is at its working clocks for the timed steps
+    reps = 200
+    ms_sweep, ms_span = G.engine.time_reduce_sweep(1, reps)
+
     dt = time_steps(pdf, model.params, args.steps, args.warmup, sync)
     ms_per_step = dt / args.steps * 1e3
     value = args.events * args.steps / dt
     log(f"{args.steps} NLL steps: {ms_per_step:.4f} ms/step, {value:.4e} events/s")
-
-    # dominant kernel: the coherent-sum NLL sweep over the events, timed with HIP
-    # events on the engine's stream around each sweep launch (the one-workgroup
-    # kernel that adds the per-workgroup partials is timed separately)
-    reps = 200
-    ms_sweep, ms_span = G.engine.time_reduce_sweep(1, reps)
     ms_kernel = ms_sweep / reps
     # 16 complex waves; event number = row index; m12 and m13 are not read when the cache
     # (filled from these events, 0 outside the Dalitz region) settles inDalitz and the
@@ -493,6 +495,7 @@ def main():
     if not args.no_fit:
         names = sorted(model.params)
         truth = {n: model.params[n].value for n in names}
+        err0 = {n: model.params[n].error for n in names}
         rng = np.random.default_rng(11)
         for n in names:
             model.params[n].value = truth[n] * (1.0 + 0.05 * rng.standard_normal())
@@ -504,10 +507,30 @@ def main():
         sync()
         t_fit = time.perf_counter() - t0
         fit = dict(wall_s=round(t_fit, 3), nfcn=m.NFcn(), valid=bool(m.IsValid()), fval=m.Fval(), edm=m.Edm(),
-                   free_params=len(names))
-        log(f"MIGRAD fit: {t_fit:.2f} s, {m.NFcn()} calls, valid={m.IsValid()}")
+                   free_params=len(names), batched_derivatives=True)
+        log(f"MIGRAD fit: {t_fit:.3f} s, {m.NFcn()} calls, valid={m.IsValid()}")
+        vals_b = list(m.values())
+        # the same fit with one request per function value (the reference's call pattern),
+        # from the same start values and step sizes
+        rng = np.random.default_rng(11)
+        for n in names:
+            model.params[n].value = truth[n] * (1.0 + 0.05 * rng.standard_normal())
+            model.params[n].error = err0[n]
+        fm1 = G.FitManager(pdf)
+        fm1.setVerbosity(0)
+        fm1.setBatchedDerivatives(False)
+        sync()
+        t0 = time.perf_counter()
+        m1 = fm1.fit()
+        sync()
+        t_fit1 = time.perf_counter() - t0
+        fit["one_point_per_request"] = dict(wall_s=round(t_fit1, 3), nfcn=m1.NFcn(),
+                                            identical_values=list(m1.values()) == vals_b)
+        log(f"MIGRAD fit with one request per function value: {t_fit1:.3f} s, {m1.NFcn()} calls "
+            f"(identical result: {list(m1.values()) == vals_b})")
         for n in names:
             model.params[n].value = truth[n]
+            model.params[n].error = err0[n]
         pdf.copyParams()
 
     norm_study = None

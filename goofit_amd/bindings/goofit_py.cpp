@@ -553,6 +553,8 @@ PYBIND11_MODULE(_goofit, m) {
             d["fcn_calls"]  = s.fcn_calls;
             d["seconds"]    = s.seconds;
             d["gpu_wait_s"] = s.gpu_wait_s;
+            d["host_prep_s"] = s.host_prep_s;
+            d["gather_s"]   = s.gather_s;
             d["chunk"]      = s.chunk;
             return d;
         });
@@ -567,6 +569,8 @@ PYBIND11_MODULE(_goofit, m) {
         .def("getMinosErrors", &FitManagerMinuit2::getMinosErrors)
         .def("setStrategy", &FitManagerMinuit2::setStrategy)
         .def("setTolerance", &FitManagerMinuit2::setTolerance)
+        .def("setBatchedDerivatives", &FitManagerMinuit2::setBatchedDerivatives, py::arg("on") = true)
+        .def("getBatchedDerivatives", &FitManagerMinuit2::getBatchedDerivatives)
         .def_property("verbosity", &FitManagerMinuit2::getVerbosity, &FitManagerMinuit2::setVerbosity)
         .def_property("max_calls", nullptr, [](FitManagerMinuit2 &f, unsigned v) { f.setMaxCalls(v); })
         .def_property_readonly("params", &FitManagerMinuit2::getParams, py::return_value_policy::reference_internal)

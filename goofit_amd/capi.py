@@ -29,7 +29,7 @@ EXPORTS = [
     "gf_ctx_get_tuning",
     "gf_events_upload", "gf_events_upload_aos", "gf_events_alloc", "gf_events_column",
     "gf_events_set_identity_column", "gf_program_set", "gf_params_update", "gf_norms_update",
-    "gf_decay3_set", "gf_reduce_metric", "gf_eval_metric", "gf_eval_metric_device", "gf_reduce_bins",
+    "gf_decay3_set", "gf_reduce_metric", "gf_reduce_metric_multi", "gf_eval_metric", "gf_eval_metric_device", "gf_reduce_bins",
     "gf_dalitz_set_data_size", "gf_dalitz_cache", "gf_dalitz_gram", "gf_dalitz_gram_points",
     "gf_dalitz_gram_points_device", "gf_dalitz_get_cache", "gf_dalitz_cache_gram", "gf_tddp_gram",
     "gf_time_reduce_metric", "gf_time_reduce_sweep", "gf_time_dalitz_cache", "gf_time_dalitz_gram",
@@ -90,6 +90,7 @@ def lib():
         L.gf_dalitz_get_cache.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int64]
         L.gf_dalitz_cache_gram.argtypes = [c_p, ctypes.c_int, _D]
         L.gf_tddp_gram.argtypes = [c_p, ctypes.c_int, _D, _D, _D, _D]
+        L.gf_reduce_metric_multi.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D]
         L.gf_time_reduce_metric.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D]
         L.gf_time_reduce_sweep.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D, _D]
         _F = ctypes.POINTER(ctypes.c_float)
@@ -251,6 +252,16 @@ class Context:
         m = METRIC[metric] if isinstance(metric, str) else metric
         _check(lib().gf_reduce_metric(self.h, m, ctypes.byref(s), ctypes.byref(c)))
         return s.value, c.value
+
+    def reduce_metric_multi(self, metric, params_list, norms_list):
+        """Metric sums for several (params, norms) sets over the same events."""
+        P = np.ascontiguousarray(np.asarray(params_list, np.float64))
+        N = np.ascontiguousarray(np.asarray(norms_list, np.float64))
+        k = P.shape[0]
+        out = np.zeros(k)
+        m = METRIC[metric] if isinstance(metric, str) else metric
+        _check(lib().gf_reduce_metric_multi(self.h, m, k, _dp(P), P.shape[1], _dp(N), N.shape[1], _dp(out)))
+        return out
 
     def eval_metric(self, metric="EVAL"):
         out = np.empty(self.nevents)

@@ -146,6 +146,11 @@ struct gf_ctx {
     int tune[GF_TUNE_COUNT] = {1, 1, 4, 4, 2000, 1};
 
     uint64_t events_gen = 1;  // bumped whenever the event columns may change
+
+    // gf_reduce_metric_multi: parameter/norm sets, partials and results
+    double *d_multi = nullptr, *h_multi = nullptr, *d_multi_out = nullptr, *h_multi_out = nullptr;
+    size_t cap_multi = 0;
+    gfk::Partial *d_multi_partials = nullptr;
 };
 
 namespace {
@@ -1171,6 +1176,16 @@ void gf_ctx_destroy(gf_ctx *c) {
     for(auto &kv : c->norm_points)
         if(kv.second.buf)
             (void)hipFree(kv.second.buf);
+    if(c->d_multi)
+        (void)hipFree(c->d_multi);
+    if(c->d_multi_out)
+        (void)hipFree(c->d_multi_out);
+    if(c->d_multi_partials)
+        (void)hipFree(c->d_multi_partials);
+    if(c->h_multi)
+        (void)hipHostFree(c->h_multi);
+    if(c->h_multi_out)
+        (void)hipHostFree(c->h_multi_out);
     void *dev[] = {c->col_base,  c->d_cols,   c->d_params,   c->d_consts,        c->d_norms,
                    c->d_obsid,   c->d_ins,    c->d_dz,       c->d_res,           c->d_partials,
                    c->d_counter, c->d_result, c->d_err, c->d_gram_partials, c->d_gram_tiles};
@@ -1423,6 +1438,92 @@ gf_status gf_reduce_metric(gf_ctx *ctx, int metric, double *sum, double *comp) {
     if((st = launch_reduce(ctx, metric)))
         return st;
     return read_result(ctx, sum, comp);
+}
+
+gf_status gf_reduce_metric_multi(gf_ctx *ctx, int metric, int npts, const double *params, int np,
+                                 const double *norms, int nn, double *out) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    gf_status st = check_events_for_program(ctx, metric);
+    if(st)
+        return st;
+    if(npts < 0 || (npts > 0 && (!params || !norms || !out)))
+        return fail(GF_ERR_INVALID, "null arrays or negative point count");
+    if(np != (int)ctx->params.size() || nn != (int)ctx->norms.size())
+        return fail(GF_ERR_INVALID, "arrays of %d / %d entries, the program has %zu / %zu", np, nn,
+                    ctx->params.size(), ctx->norms.size());
+    if(npts == 0)
+        return GF_OK;
+    if(!use_fast_dalitz(ctx, metric)) {
+        // one reduction per set, then the context's own arrays back
+        const std::vector<double> p0 = ctx->params, n0 = ctx->norms;
+        for(int k = 0; k < npts && !st; k++) {
+            if(!(st = gf_params_update(ctx, params + (size_t)k * np, np))
+               && !(st = gf_norms_update(ctx, norms + (size_t)k * nn, nn)) && !(st = launch_reduce(ctx, metric)))
+                st = read_result(ctx, out + k, nullptr);
+        }
+        gf_params_update(ctx, p0.data(), np);
+        gf_norms_update(ctx, n0.data(), nn);
+        return st;
+    }
+    // coherent-sum path: GF_MULTI_PTS sets per sweep, the waves read once per sweep
+    const size_t vals = (size_t)npts * (size_t)(np + nn);
+    if(vals > ctx->cap_multi) {
+        GF_HIP(hipStreamSynchronize(ctx->stream));
+        if(ctx->d_multi)
+            GF_HIP(hipFree(ctx->d_multi));
+        if(ctx->h_multi)
+            GF_HIP(hipHostFree(ctx->h_multi));
+        if(ctx->d_multi_out)
+            GF_HIP(hipFree(ctx->d_multi_out));
+        if(ctx->h_multi_out)
+            GF_HIP(hipHostFree(ctx->h_multi_out));
+        ctx->d_multi = ctx->h_multi = ctx->d_multi_out = ctx->h_multi_out = nullptr;
+        ctx->cap_multi = 0;
+        const size_t c = vals + vals / 2 + 64;
+        const size_t o = 2 * (c / 2 + 64);  // result pairs: at least 2 per set
+        if(hipMalloc((void **)&ctx->d_multi, c * sizeof(double)) != hipSuccess
+           || hipHostMalloc((void **)&ctx->h_multi, c * sizeof(double), hipHostMallocDefault) != hipSuccess
+           || hipMalloc((void **)&ctx->d_multi_out, o * sizeof(double)) != hipSuccess
+           || hipHostMalloc((void **)&ctx->h_multi_out, o * sizeof(double), hipHostMallocDefault) != hipSuccess)
+            return fail(GF_ERR_NOMEM, "cannot allocate %zu parameter values for %d sets", c, npts);
+        ctx->cap_multi = c;
+    }
+    if(!ctx->d_multi_partials
+       && hipMalloc((void **)&ctx->d_multi_partials, (size_t)GF_MULTI_PTS * GF_MAX_PARTIALS * sizeof(gfk::Partial))
+              != hipSuccess) {
+        ctx->d_multi_partials = nullptr;
+        return fail(GF_ERR_NOMEM, "cannot allocate the multi-set partials");
+    }
+    if(ctx->stage_in_flight) {
+        GF_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->stage_in_flight = false;
+    }
+    double *hp = ctx->h_multi, *hn = ctx->h_multi + (size_t)npts * np;
+    memcpy(hp, params, (size_t)npts * np * sizeof(double));
+    memcpy(hn, norms, (size_t)npts * nn * sizeof(double));
+    GF_HIP(hipMemcpyAsync(ctx->d_multi, ctx->h_multi, vals * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    // the zero-outside sweep needs every set's efficiency constant finite
+    bool zo = zero_outside(ctx);
+    for(int k = 0; k < npts && zo; k++)
+        zo = std::isfinite(params[(size_t)k * np + ctx->ins[1].p + ctx->ins[1].a]);
+    const bool ce = const_efficiency(ctx);
+    GfProgArgs a  = prog_args(ctx);
+    const int nb  = fast_blocks(ctx, ctx->n);
+    for(int k0 = 0; k0 < npts; k0 += GF_MULTI_PTS) {
+        const int cnt = std::min(GF_MULTI_PTS, npts - k0);
+        int rc = gfk::launch_dalitz_multi(a, ctx->d_multi + (size_t)k0 * np, np,
+                                          ctx->d_multi + (size_t)npts * np + (size_t)k0 * nn, nn, cnt, ce, zo, metric,
+                                          ctx->d_multi_partials, ctx->d_multi_out + 2 * (size_t)k0, nb, ctx->stream);
+        if(rc)
+            return fail(GF_ERR_HIP, "multi-set launch failed (%d)", rc);
+    }
+    GF_HIP(hipMemcpyAsync(ctx->h_multi_out, ctx->d_multi_out, 2 * (size_t)npts * sizeof(double),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    GF_HIP(hipStreamSynchronize(ctx->stream));
+    for(int k = 0; k < npts; k++)
+        out[k] = ctx->h_multi_out[2 * k];
+    return GF_OK;
 }
 
 gf_status gf_eval_metric_device(gf_ctx *ctx, int metric, double *dev_out, int64_t n) {

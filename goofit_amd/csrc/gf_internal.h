@@ -161,6 +161,13 @@ int launch_eval_metric(const GfProgArgs &a, const GfInline *inl, int metric, dou
 // Coherent-sum fast path: program [DALITZ, MULTI_POLY efficiency, DALITZ_END, END],
 // event number = row index, every row inside the wave cache.
 // const_eff: the efficiency polynomial has degree 0 (a constant coefficient).
+// The coherent-sum metric of npts <= GF_MULTI_PTS parameter sets (params: npts x np,
+// norms: npts x nn, device) over the same events in one sweep; out[2p], out[2p+1] =
+// sum and residual of set p (device).  partials: npts x nblocks.
+#define GF_MULTI_PTS 16
+int launch_dalitz_multi(const GfProgArgs &a, const double *params, int np, const double *norms, int nn, int npts,
+                        bool const_eff, bool zero_outside, int metric, Partial *partials, double *out, int nblocks,
+                        void *stream);
 // zero_outside (with const_eff): the wave cache was filled from these events, so m12/m13
 // are not read (the waves are 0 outside the Dalitz region).
 int launch_dalitz_fast(const GfProgArgs &a, const GfInline *inl, bool const_eff, bool zero_outside, int metric,

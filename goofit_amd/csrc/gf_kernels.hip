@@ -665,6 +665,90 @@ __global__ __launch_bounds__(256) void k_dalitz_fast(GfProgArgs A, GfInline I, i
         store_partial(s, c, partials);
 }
 
+// The coherent-sum metric sum of up to GF_MULTI_PTS parameter sets over the same
+// events in one sweep: an event's waves are loaded once and every set's term is
+// taken with the arithmetic, the event-to-lane assignment and the reduction
+// order of k_dalitz_fast on the same grid, so each set's partials equal that
+// kernel's for the same parameters bit for bit.  params / norms: npts
+// consecutive copies of the flattened arrays (np and nn doubles each).
+// partials: [npts][gridDim.x].
+template <bool CONST_EFF, bool ZERO_OUT>
+__global__ __launch_bounds__(256) void k_dalitz_multi(GfProgArgs A, const double *__restrict__ params, int np,
+                                                      const double *__restrict__ norms, int nn, int npts, int metric,
+                                                      Partial *partials) {
+    const GfIns dzi       = A.ins[0];
+    const GfIns effi      = A.ins[1];
+    const GfDalitzSlot dz = A.dz[dzi.x];
+    const GfDecay &D      = A.decay;
+    const int nres        = dzi.a;
+    const int64_t n       = A.nevents;
+    const double *__restrict__ c12 = A.cols[dz.col_m12];
+    const double *__restrict__ c13 = A.cols[dz.col_m13];
+    double s[GF_MULTI_PTS], c[GF_MULTI_PTS];
+#pragma unroll
+    for(int p = 0; p < GF_MULTI_PTS; ++p)
+        s[p] = c[p] = 0.0;
+    for(int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e0 = base + threadIdx.x;
+        const bool valid = e0 < n;
+        const int64_t e  = valid ? e0 : n - 1;
+        double2 w[GF_MAX_RES];
+#pragma unroll
+        for(int i = 0; i < GF_MAX_RES; ++i)
+            w[i] = (i < nres) ? dz.wave[(int64_t)i * dz.ncache + e] : make_double2(0.0, 0.0);
+        const double m12  = ZERO_OUT ? 0.0 : c12[e];
+        const double m13  = ZERO_OUT ? 0.0 : c13[e];
+        const bool inside =
+            ZERO_OUT || gfd::in_dalitz(m12, m13, D.motherMass, D.daug1Mass, D.daug2Mass, D.daug3Mass);
+#pragma unroll
+        for(int p = 0; p < GF_MULTI_PTS; ++p) {
+            if(p < npts) {
+                const double *__restrict__ P = params + (int64_t)p * np;
+                const double norm            = norms[(int64_t)p * nn + 1];
+                double re = 0.0, im = 0.0;
+#pragma unroll
+                for(int i = 0; i < GF_MAX_RES; ++i) {
+                    if(i < nres) {
+                        const double ar = P[dzi.p + 2 * i], ai = P[dzi.p + 2 * i + 1];
+                        re += ar * w[i].x - ai * w[i].y;
+                        im += ar * w[i].y + ai * w[i].x;
+                    }
+                }
+                const double eff = CONST_EFF ? P[effi.p + effi.a] : multipoly_at(effi, A, P, e);
+                const double T   = inside ? (re * re + im * im) * eff : 0.0;
+                const double v   = gfd::metric(metric, T, 0.0, 0.0, norm, A.total_events);
+                if(valid)
+                    gfd::kb_add(s[p], c[p], v);
+            }
+        }
+    }
+#pragma unroll
+    for(int p = 0; p < GF_MULTI_PTS; ++p) {
+        if(p < npts) {
+            double ss = s[p], cc = c[p];
+            block_reduce(ss, cc);
+            if(threadIdx.x == 0) {
+                partials[(int64_t)p * gridDim.x + blockIdx.x].s = ss;
+                partials[(int64_t)p * gridDim.x + blockIdx.x].c = cc;
+            }
+        }
+    }
+}
+
+// k_finish for several sums: workgroup p adds the nparts partials of set p in the
+// order k_finish uses and writes {sum, residual} to out[2p].
+__global__ __launch_bounds__(GF_BLOCK) void k_finish_multi(const Partial *partials, int nparts, double *out) {
+    const Partial *q = partials + (int64_t)blockIdx.x * nparts;
+    double ts = 0.0, tc = 0.0;
+    for(int i = threadIdx.x; i < nparts; i += GF_BLOCK)
+        gfd::kb_merge(ts, tc, q[i].s, q[i].c);
+    block_reduce(ts, tc);
+    if(threadIdx.x == 0) {
+        out[2 * blockIdx.x]     = gfd::kb_total(ts, tc);
+        out[2 * blockIdx.x + 1] = tc;
+    }
+}
+
 }  // namespace
 
 // Grid description of k_reduce_bins (by value; indexed only by unrolled constants).
@@ -1181,6 +1265,28 @@ static void launch_fast_t(const GfProgArgs &a, const GfInline *inl, int metric, 
                           int nblocks, hipStream_t s) {
     hipLaunchKernelGGL((k_dalitz_fast<WRITE, INL, CE, ZO>), dim3(nblocks), dim3(256), 0, s, a,
                        inl ? *inl : GfInline{}, metric, partials, out);
+}
+
+int launch_dalitz_multi(const GfProgArgs &a, const double *params, int np, const double *norms, int nn, int npts,
+                        bool const_eff, bool zero_outside, int metric, Partial *partials, double *out, int nblocks,
+                        void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if(npts < 1 || npts > GF_MULTI_PTS)
+        return -1;
+    if(const_eff && zero_outside)
+        hipLaunchKernelGGL((k_dalitz_multi<true, true>), dim3(nblocks), dim3(256), 0, s, a, params, np, norms, nn,
+                           npts, metric, partials);
+    else if(const_eff)
+        hipLaunchKernelGGL((k_dalitz_multi<true, false>), dim3(nblocks), dim3(256), 0, s, a, params, np, norms, nn,
+                           npts, metric, partials);
+    else
+        hipLaunchKernelGGL((k_dalitz_multi<false, false>), dim3(nblocks), dim3(256), 0, s, a, params, np, norms,
+                           nn, npts, metric, partials);
+    int rc = check(hipGetLastError());
+    if(rc)
+        return rc;
+    hipLaunchKernelGGL(k_finish_multi, dim3(npts), dim3(GF_BLOCK), 0, s, partials, nblocks, out);
+    return check(hipGetLastError());
 }
 
 int launch_dalitz_fast(const GfProgArgs &a, const GfInline *inl, bool const_eff, bool zero_outside, int metric,

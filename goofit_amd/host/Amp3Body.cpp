@@ -137,8 +137,10 @@ fptype Amp3Body::normalize() {
     forceRedoIntegrals = false;
     if(mask) {
         engine::flush();
-        if(cacheReady)
+        if(cacheReady) {
             engine::check(gf_dalitz_cache(engine::ctx(), (int)functionIdx, mask), "gf_dalitz_cache");
+            engine::bump_cache_epoch();
+        }
         std::vector<double> out(2 * nres * nres);
         if(normPoints > 0) {
             engine::check(gf_dalitz_gram_norm_points(engine::ctx(), (int)functionIdx, (int)efficiencyFunction, 0,

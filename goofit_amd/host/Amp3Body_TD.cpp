@@ -185,8 +185,10 @@ fptype Amp3Body_TD::getD0Fraction() { return _D0Fraction; }
 void Amp3Body_TD::computeIntegrals(uint64_t mask) {
     const size_t nres = decayInfo.resonances.size();
     engine::flush();
-    if(cacheReady && mask)
+    if(cacheReady && mask) {
         engine::check(gf_dalitz_cache(engine::ctx(), (int)functionIdx, mask), "gf_dalitz_cache");
+        engine::bump_cache_epoch();
+    }
     const double grid6[6] = {_m12.getLowerLimit(), _m12.getUpperLimit(), (double)_m12.getNumBins(),
                              _m13.getLowerLimit(), _m13.getUpperLimit(), (double)_m13.getNumBins()};
     std::vector<double> a(2 * nres * nres), b(2 * nres * nres), c(2 * nres * nres);

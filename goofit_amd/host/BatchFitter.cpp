@@ -145,6 +145,7 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
     std::exception_ptr failure;
     for(;;) {
         std::unique_lock<std::mutex> lk(B.mu);
+        const auto w0 = std::chrono::steady_clock::now();
         // wait until every running toy is blocked in an FCN call (or all done)
         B.cv_req.wait_for(lk, std::chrono::milliseconds(50), [&] {
             return (B.running > 0 && B.waiting == B.running) || (B.running == 0 && next >= T);
@@ -163,6 +164,8 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
             if(B.pending[t])
                 serve.push_back(t);
         std::fill(act.begin(), act.end(), 0);
+        const auto p0 = std::chrono::steady_clock::now();
+        stats_.gather_s += std::chrono::duration<double>(p0 - w0).count();
         try {
             for(int t : serve) {
                 // push the toy's values into the shared Variables, then the host arrays
@@ -181,6 +184,7 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
                 act[t] = 1;
             }
             const auto g0 = std::chrono::steady_clock::now();
+            stats_.host_prep_s += std::chrono::duration<double>(g0 - p0).count();
             engine::check(gf_batch_reduce(batch_, act.data(), GF_METRIC_NLL, out.data()), "gf_batch_reduce");
             stats_.gpu_wait_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - g0).count();
         } catch(...) {

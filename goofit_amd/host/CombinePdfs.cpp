@@ -75,13 +75,17 @@ fptype AddPdf::normalize() {
 // AddPdf::calculateNLL, AddPdf.cu:187-204: extended likelihood adds E - N log E.
 double AddPdf::calculateNLL() {
     double ret = GooPdf::calculateNLL();
-    if(extended) {
-        fptype expEvents = 0;
-        for(size_t i = 0; i < components.size(); ++i)
-            expEvents += parametersList[i].getValue();
-        ret += (expEvents - numEvents * std::log(expEvents));
-    }
+    ret += nllExtraTerm();
     return ret;
+}
+
+double AddPdf::nllExtraTerm() const {
+    if(!extended)
+        return 0.0;
+    fptype expEvents = 0;
+    for(size_t i = 0; i < components.size(); ++i)
+        expEvents += parametersList[i].getValue();
+    return expEvents - numEvents * std::log(expEvents);
 }
 
 // ---------------------------------------------------------------- ProdPdf

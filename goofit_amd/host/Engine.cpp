@@ -60,6 +60,12 @@ void release() {
 
 void mark_program_dirty() { t_state.program_dirty = true; }
 
+namespace {
+thread_local unsigned long long t_cache_epoch = 0;
+}
+unsigned long long cache_epoch() { return t_cache_epoch; }
+void bump_cache_epoch() { ++t_cache_epoch; }
+
 void check(gf_status status, const char *what) {
     if(status != GF_OK)
         throw GeneralError("{}: {}", what, gf_last_error());

@@ -1,6 +1,8 @@
 // Fitting.cpp -- FCN, Params and FitManagerMinuit2 (src/goofit/{FCN,Params,
 // FitManagerMinuit2}.cpp of the reference) on the Minuit2-compatible minimiser.
 #include <goofit/Error.h>
+#include <goofit/Engine.h>
+#include <goofit/PDFs/GooPdf.h>
 #include <goofit/PdfBase.h>
 #include <goofit/fitting/FCN.h>
 #include <goofit/fitting/FitManagerMinuit2.h>
@@ -89,6 +91,53 @@ double FCN::operator()() const {
 }
 
 Params *FCN::GetParams() { return params_; }
+
+std::vector<double> FCN::EvaluateMany(const std::vector<std::vector<double>> &xs) const {
+    auto *pdf = dynamic_cast<GooPdf *>(params_->pdf_);
+    if(!batched_ || xs.size() < 2 || !pdf)
+        return Minuit2::FCNBase::EvaluateMany(xs);
+    const size_t npts = xs.size();
+    std::vector<double> P, N, extra(npts);
+    size_t np = 0, nn = 0;
+    unsigned long long epoch = 0;
+    for(size_t k = 0; k < npts; k++) {
+        // what operator() does up to the reduction: set the point, normalise, check
+        params_->from_minuit_vector(xs[k]);
+        pdf->normalize();
+        if(k == 0) {
+            epoch = engine::cache_epoch();
+        } else if(engine::cache_epoch() != epoch) {
+            // this point recomputed a wave cache: the points do not share the caches
+            std::vector<double> out(npts);
+            for(size_t j = 0; j < npts; j++)
+                out[j] = (*this)(xs[j]);
+            return out;
+        }
+        if(!pdf->normalisationPositive())
+            GooFit::abort(__FILE__, __LINE__, pdf->getName() + " non-positive normalization", pdf);
+        if(k == 0) {
+            np = host_parameters.size();
+            nn = host_normalizations.size();
+            P.reserve(np * npts);
+            N.reserve(nn * npts);
+        }
+        P.insert(P.end(), host_parameters.begin(), host_parameters.end());
+        N.insert(N.end(), host_normalizations.begin(), host_normalizations.end());
+        extra[k] = pdf->nllExtraTerm();
+    }
+    engine::flush();
+    std::vector<double> out(npts);
+    engine::check(gf_reduce_metric_multi(engine::ctx(), pdf->metricCode(), (int)npts, P.data(), (int)np, N.data(),
+                                         (int)nn, out.data()),
+                  "gf_reduce_metric_multi");
+    for(size_t k = 0; k < npts; k++) {
+        if(0.0 == out[k])
+            GooFit::abort(__FILE__, __LINE__, pdf->getName() + " zero NLL", pdf);
+        out[k] += extra[k];
+    }
+    host_callnumber += (unsigned int)npts;
+    return out;
+}
 
 // ---------------------------------------------------------------- FitManagerMinuit2
 

@@ -67,6 +67,8 @@ double GooPdf::calculateNLL() {
     return ret;
 }
 
+bool GooPdf::normalisationPositive() const { return host_normalizations.at(normalIdx + 1) > 0; }
+
 // GooPdf::normalize, src/PDFs/GooPdf.cu:340-396
 fptype GooPdf::normalize() {
     if(fitControl && !fitControl->metricIsPdf()) {

@@ -131,6 +131,20 @@ struct IntFcn {
         nfcn++;
         return fcn(ext(x));
     }
+    // independent points in one request (FCNBase::EvaluateMany)
+    std::vector<double> many(const std::vector<std::vector<double>> &xs) const {
+        if(xs.empty())
+            return {};
+        nfcn += (unsigned int)xs.size();
+        std::vector<std::vector<double>> e;
+        e.reserve(xs.size());
+        for(const auto &x : xs)
+            e.push_back(ext(x));
+        std::vector<double> f = fcn.EvaluateMany(e);
+        if(f.size() != xs.size())
+            throw std::runtime_error("FCNBase::EvaluateMany returned a wrong number of values");
+        return f;
+    }
     std::vector<double> start() const {
         std::vector<double> x(tr.size());
         for(size_t k = 0; k < tr.size(); k++)
@@ -250,6 +264,10 @@ struct Grad {
 // Central-difference gradient with step refinement (cf. Minuit2's numerical
 // two-point gradient): the step of parameter i aims at sqrt(dfmin / |g2_i|),
 // dfmin being the rounding level 8 eps2 (|f| + Up) of the function value.
+// Each parameter's refinement depends only on its own two function values, so
+// the cycles run parameter-parallel: cycle c evaluates f(x +- step_i e_i) of
+// every parameter still refining as one request of 2 n_active points.  Given a
+// function without side effects this is the parameter-by-parameter loop.
 Grad numerical_gradient(const IntFcn &F, const std::vector<double> &x, double f0, const Grad &prev, double up,
                         unsigned int ncycles) {
     const size_t n = x.size();
@@ -259,39 +277,61 @@ Grad numerical_gradient(const IntFcn &F, const std::vector<double> &x, double f0
     G.step.assign(n, 0);
     const double dfmin  = 8.0 * kEps2 * (std::fabs(f0) + up);
     const double vrysml = 8.0 * kEps * kEps;
-    std::vector<double> xx = x;
+    std::vector<double> step(n), stepb4(n, 0.0), g(n), g2(n);
+    std::vector<char> active(n, 1);
     for(size_t i = 0; i < n; i++) {
-        const double xtf    = x[i];
-        const double epspri = kEps2 + std::fabs(prev.g[i] * kEps2);
-        double step         = prev.step[i];
-        double stepb4       = 0;
-        double g = prev.g[i], g2 = prev.g2[i];
-        for(unsigned int c = 0; c < ncycles; c++) {
-            double optstp = std::sqrt(dfmin / (std::fabs(g2) + epspri));
-            step          = std::max(optstp, std::fabs(0.1 * step));
-            double stpmax = 10.0 * std::fabs(prev.step[i] > 0 ? prev.step[i] : step);
+        step[i] = prev.step[i];
+        g[i]    = prev.g[i];
+        g2[i]   = prev.g2[i];
+    }
+    std::vector<size_t> idx;
+    std::vector<std::vector<double>> pts;
+    for(unsigned int c = 0; c < ncycles; c++) {
+        idx.clear();
+        pts.clear();
+        for(size_t i = 0; i < n; i++) {
+            if(!active[i])
+                continue;
+            const double xtf    = x[i];
+            const double epspri = kEps2 + std::fabs(prev.g[i] * kEps2);
+            double optstp       = std::sqrt(dfmin / (std::fabs(g2[i]) + epspri));
+            double st           = std::max(optstp, std::fabs(0.1 * step[i]));
+            double stpmax       = 10.0 * std::fabs(prev.step[i] > 0 ? prev.step[i] : st);
             if(F.tr[i].kind == 1)
                 stpmax = std::min(stpmax, 0.5);
             double stpmin = std::max(vrysml, 8.0 * std::fabs(kEps2 * xtf));
-            step          = std::min(step, stpmax);
-            step          = std::max(step, stpmin);
-            if(c > 0 && std::fabs((step - stepb4) / step) < 0.3)
-                break;
-            stepb4    = step;
-            xx[i]     = xtf + step;
-            double fs1 = F(xx);
-            xx[i]     = xtf - step;
-            double fs2 = F(xx);
-            xx[i]     = xtf;
-            double grdb4 = g;
-            g  = 0.5 * (fs1 - fs2) / step;
-            g2 = (fs1 + fs2 - 2.0 * f0) / step / step;
-            if(c > 0 && std::fabs(grdb4 - g) / (std::fabs(g) + dfmin / step) < 0.05)
-                break;
+            st            = std::min(st, stpmax);
+            st            = std::max(st, stpmin);
+            step[i]       = st;
+            if(c > 0 && std::fabs((st - stepb4[i]) / st) < 0.3) {
+                active[i] = 0;
+                continue;
+            }
+            stepb4[i] = st;
+            idx.push_back(i);
+            std::vector<double> xp = x, xm = x;
+            xp[i] = xtf + st;
+            xm[i] = xtf - st;
+            pts.push_back(std::move(xp));
+            pts.push_back(std::move(xm));
         }
-        G.g[i]    = g;
-        G.g2[i]   = g2;
-        G.step[i] = step;
+        if(idx.empty())
+            break;
+        const std::vector<double> f = F.many(pts);
+        for(size_t k = 0; k < idx.size(); k++) {
+            const size_t i     = idx[k];
+            const double fs1   = f[2 * k], fs2 = f[2 * k + 1];
+            const double grdb4 = g[i];
+            g[i]               = 0.5 * (fs1 - fs2) / step[i];
+            g2[i]              = (fs1 + fs2 - 2.0 * f0) / step[i] / step[i];
+            if(c > 0 && std::fabs(grdb4 - g[i]) / (std::fabs(g[i]) + dfmin / step[i]) < 0.05)
+                active[i] = 0;
+        }
+    }
+    for(size_t i = 0; i < n; i++) {
+        G.g[i]    = g[i];
+        G.g2[i]   = g2[i];
+        G.step[i] = step[i];
     }
     return G;
 }
@@ -306,44 +346,78 @@ bool numerical_hessian_once(const IntFcn &F, const std::vector<double> &x, doubl
     g_out.assign(n, 0.0);
     g2_out.assign(n, 0.0);
     d_out.assign(n, 0.0);
-    std::vector<double> xx = x, fp(n, 0.0);
+    std::vector<double> fp(n, 0.0);
     const double dfmin = 8.0 * kEps2 * (std::fabs(f0) + up);
+    // diagonal: each parameter's step search (up to 6 tries) depends only on its
+    // own two values, so the tries run parameter-parallel, one request per round
+    std::vector<double> d(n), fs1(n, 0.0), fs2(n, 0.0);
+    std::vector<char> active(n, 1);
     for(size_t i = 0; i < n; i++) {
         double g2 = grad.g2[i];
-        double d  = (std::fabs(g2) > 0) ? std::sqrt(2.0 * target * up / std::fabs(g2)) : std::fabs(grad.step[i]) * 10;
-        if(!(d > 0))
-            d = 1e-3;
+        d[i] = (std::fabs(g2) > 0) ? std::sqrt(2.0 * target * up / std::fabs(g2)) : std::fabs(grad.step[i]) * 10;
+        if(!(d[i] > 0))
+            d[i] = 1e-3;
         if(F.tr[i].kind == 1)
-            d = std::min(d, 0.5);
-        double fs1 = 0, fs2 = 0;
-        for(int c = 0; c < 6; c++) {
-            xx[i] = x[i] + d;
-            fs1   = F(xx);
-            xx[i] = x[i] - d;
-            fs2   = F(xx);
-            xx[i] = x[i];
-            double sag = 0.5 * (fs1 + fs2 - 2.0 * f0);
-            if(sag > 4.0 * dfmin && sag < 100.0 * target * up)
-                break;
-            if(sag <= 4.0 * dfmin)
-                d *= 4.0;
-            else
-                d *= 0.3;
-        }
-        H[i * n + i] = (fs1 + fs2 - 2.0 * f0) / (d * d);
-        g_out[i]     = 0.5 * (fs1 - fs2) / d;
-        g2_out[i]    = H[i * n + i];
-        d_out[i]     = d;
-        fp[i]        = fs1;
+            d[i] = std::min(d[i], 0.5);
     }
+    std::vector<size_t> idx;
+    std::vector<std::vector<double>> pts;
+    for(int c = 0; c < 6; c++) {
+        idx.clear();
+        pts.clear();
+        for(size_t i = 0; i < n; i++) {
+            if(!active[i])
+                continue;
+            idx.push_back(i);
+            std::vector<double> xp = x, xm = x;
+            xp[i] = x[i] + d[i];
+            xm[i] = x[i] - d[i];
+            pts.push_back(std::move(xp));
+            pts.push_back(std::move(xm));
+        }
+        if(idx.empty())
+            break;
+        const std::vector<double> f = F.many(pts);
+        for(size_t k = 0; k < idx.size(); k++) {
+            const size_t i = idx[k];
+            fs1[i]         = f[2 * k];
+            fs2[i]         = f[2 * k + 1];
+            double sag     = 0.5 * (fs1[i] + fs2[i] - 2.0 * f0);
+            if(sag > 4.0 * dfmin && sag < 100.0 * target * up) {
+                active[i] = 0;
+                continue;
+            }
+            if(c == 5) {
+                active[i] = 0;  // the last try keeps its values and step
+                continue;
+            }
+            if(sag <= 4.0 * dfmin)
+                d[i] *= 4.0;
+            else
+                d[i] *= 0.3;
+        }
+    }
+    for(size_t i = 0; i < n; i++) {
+        H[i * n + i] = (fs1[i] + fs2[i] - 2.0 * f0) / (d[i] * d[i]);
+        g_out[i]     = 0.5 * (fs1[i] - fs2[i]) / d[i];
+        g2_out[i]    = H[i * n + i];
+        d_out[i]     = d[i];
+        fp[i]        = fs1[i];
+    }
+    // off-diagonal: every f(x + d_i e_i + d_j e_j) in one request
+    pts.clear();
     for(size_t i = 0; i < n; i++)
         for(size_t j = i + 1; j < n; j++) {
-            xx[i]      = x[i] + d_out[i];
-            xx[j]      = x[j] + d_out[j];
-            double fij = F(xx);
-            xx[i]      = x[i];
-            xx[j]      = x[j];
-            double h   = (fij + f0 - fp[i] - fp[j]) / (d_out[i] * d_out[j]);
+            std::vector<double> xx = x;
+            xx[i] = x[i] + d_out[i];
+            xx[j] = x[j] + d_out[j];
+            pts.push_back(std::move(xx));
+        }
+    const std::vector<double> fo = F.many(pts);
+    size_t k = 0;
+    for(size_t i = 0; i < n; i++)
+        for(size_t j = i + 1; j < n; j++, k++) {
+            double h     = (fo[k] + f0 - fp[i] - fp[j]) / (d_out[i] * d_out[j]);
             H[i * n + j] = H[j * n + i] = h;
         }
     return true;

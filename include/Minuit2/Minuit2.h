@@ -25,6 +25,18 @@ class FCNBase {
     /// Change of the function value that defines one standard deviation (0.5 for -log L).
     virtual double Up() const = 0;
     virtual double ErrorDef() const { return Up(); }
+    /// The function at several independent points, in order.  No counterpart in
+    /// Minuit2: the minimiser hands over the points of a derivative step together
+    /// (the 2n points of one gradient cycle, the points of a Hessian) so that a
+    /// function evaluated on a GPU can serve them in one launch.  The default
+    /// evaluates them one by one; an override must return what operator() would.
+    virtual std::vector<double> EvaluateMany(const std::vector<std::vector<double>> &xs) const {
+        std::vector<double> out;
+        out.reserve(xs.size());
+        for(const auto &x : xs)
+            out.push_back((*this)(x));
+        return out;
+    }
 };
 
 /// One user parameter: value, step/error, optional limits, fixed flag.

@@ -40,6 +40,10 @@ void flush();
 void check(gf_status status, const char *what);
 /// Three-body decay constants (c_motherMass .. c_mother_meson_radius).
 void set_decay(const double decay6[6]);
+/// Counter of wave-cache recomputations on this thread: derivative points that
+/// see it unchanged share the caches and may be reduced in one request.
+unsigned long long cache_epoch();
+void bump_cache_epoch();
 }  // namespace engine
 
 }  // namespace GooFit

@@ -36,6 +36,13 @@ class GooPdf : public PdfBase {
     double reduce_with_bins() const;
 
     double calculateNLL() override;
+    /// The metric code (gf_metric) this PDF's fit control selects.
+    int metricCode() const { return metric_code(); }
+    /// Term calculateNLL adds to the metric sum (the extended-likelihood term of an
+    /// extended AddPdf), at the current parameters.
+    virtual double nllExtraTerm() const { return 0.0; }
+    /// The normalisation of this node is positive (calculateNLL aborts otherwise).
+    bool normalisationPositive() const;
     std::vector<fptype> evaluateAtPoints(Observable var);
     fptype normalize() override;
     fptype normalise() { return normalize(); }

@@ -11,6 +11,7 @@ class AddPdf : public CombinePdf {
     AddPdf(std::string n, Variable frac1, PdfBase *func1, PdfBase *func2);
     fptype normalize() override;
     double calculateNLL() override;
+    double nllExtraTerm() const override;
     bool hasAnalyticIntegral() const override { return false; }
 
   private:

@@ -27,6 +27,8 @@ struct BatchFitStats {
     int64_t fcn_calls  = 0;  // FCN calls served
     double seconds     = 0;  // wall time of fit()
     double gpu_wait_s  = 0;  // time spent in the batched reductions
+    double host_prep_s = 0;  // time the batcher spends setting each served toy's parameters and norms
+    double gather_s    = 0;  // time the batcher waits for the toys' next FCN requests
     int chunk          = 0;  // events per workgroup chosen by the autotuner
 };
 

@@ -33,6 +33,10 @@ class FitManagerMinuit2 {
     void setTolerance(double tol) { tolerance_ = tol; }
     /// Wall time of the last fit() in seconds.
     double getLastFitTime() const { return last_time_; }
+    /// Hand the points of each derivative step to the GPU together (default on):
+    /// the same MIGRAD with the same function values, fewer launches.
+    void setBatchedDerivatives(bool on) { fcn_.setBatched(on); }
+    bool getBatchedDerivatives() const { return fcn_.isBatched(); }
 
   private:
     Params upar_;

@@ -182,6 +182,19 @@ gf_status gf_decay3_set(gf_ctx *ctx, const double decay[6]);
  * NULL) the residual correction term that was folded in. */
 gf_status gf_reduce_metric(gf_ctx *ctx, int metric, double *sum, double *comp);
 
+/* The metric sums for npts parameter sets over the same events: params holds
+ * npts consecutive copies of the flattened parameter array (np values each, np
+ * as in gf_params_update), norms npts copies of the normalisation array;
+ * out[k] = the sum gf_reduce_metric returns with set k installed.  For a lone
+ * Amp3Body (the coherent-sum path) one sweep serves 16 sets, reading each
+ * event's waves once, with each set's result bit-identical to its own
+ * gf_reduce_metric; otherwise the sets are reduced one after another.  The
+ * context's own arrays are left as they were.  No reference counterpart: it
+ * serves the independent points of a numerical derivative
+ * (Minuit2::FCNBase::EvaluateMany of include/Minuit2/Minuit2.h) in one request. */
+gf_status gf_reduce_metric_multi(gf_ctx *ctx, int metric, int npts, const double *params, int np,
+                                 const double *norms, int nn, double *out);
+
 /* Per-event metric values into host array out[n] (n = number of events).
  * Replaces GooPdf::evaluate_with_metric (src/PDFs/GooPdf.cu:112-173). */
 gf_status gf_eval_metric(gf_ctx *ctx, int metric, double *out, int64_t n);

@@ -151,6 +151,37 @@ def _scipy_fit(model: O.Model, names, x0):
     return r2.x, r2.fun
 
 
+def test_batched_derivatives_reproduce_the_fit():
+    """MIGRAD with each derivative step's points served in one request (FCN::EvaluateMany ->
+    gf_reduce_metric_multi) makes the same function values as one request per point, so the
+    fits agree exactly: values, errors and call counts."""
+    table = W.DALITZ_RESONANCES[:6]
+    m = W.build_dalitz(nbins=150, resonances=table)
+    g12, g13 = W.generate_dalitz(m, 100_000, seed=41)
+    d = W.dalitz_dataset(m, g12, g13)
+    m.pdf.setData(d)
+    m.pdf.setDataSize(len(g12))
+    truth = {k: v.value for k, v in m.params.items()}
+    err0 = {k: v.error for k, v in m.params.items()}  # a fit writes its errors back: restore the steps
+    out = {}
+    for batched in (False, True):
+        rng = np.random.default_rng(3)
+        for k in sorted(m.params):
+            m.params[k].value = truth[k] * (1 + 0.05 * rng.standard_normal())
+            m.params[k].error = err0[k]
+        m.pdf.setForceIntegrals()
+        fm = G.FitManager(m.pdf)
+        fm.setVerbosity(0)
+        fm.setBatchedDerivatives(batched)
+        r = fm.fit()
+        out[batched] = (list(r.values()), list(r.errors()), r.NFcn(), r.Fval(), r.IsValid())
+    assert out[True][4] and out[False][4]
+    assert out[True][2] == out[False][2]
+    assert out[True][3] == out[False][3]
+    assert out[True][0] == out[False][0]
+    assert out[True][1] == out[False][1]
+
+
 def test_add1d_fit_parity():
     """MIGRAD on the GPU NLL vs scipy on the oracle NLL: parameters within 0.01 sigma."""
     truth = W.sample_add1d(20_000, seed=41)

@@ -374,6 +374,53 @@ def test_dalitz_zero_outside_cache(ctx):
     assert stale[1] == stale[0]
 
 
+def test_reduce_metric_multi_equals_single(ctx):
+    """gf_reduce_metric_multi: each set's sum equals gf_reduce_metric's with that set
+    installed, bit for bit, on the coherent-sum path (20 sets: one full sweep of 16
+    and one of 4) and on the interpreter (one reduction per set)."""
+    n = 50_000
+    top, M, aos = _dalitz_setup(ctx, n, nbins=64, seed=31)
+    M.calculate_nll()
+    M.op.sync_params()
+    ctx.update_params(M.op.p)
+    ctx.dalitz_cache(top.function_idx)
+    _sync(ctx, M)
+    p0, n0 = M.op.p.copy(), M.op.n.copy()
+    rng = np.random.default_rng(5)
+    res = top.resonances
+    sets_p, sets_n = [], []
+    for k in range(20):
+        p = p0.copy()
+        for r in res:  # perturb every coupling (the waves do not depend on them)
+            p[top.parameters_idx + 1 + 2 * res.index(r)] *= 1 + 1e-3 * rng.standard_normal()
+        nn_ = n0.copy()
+        nn_[top.normal_idx + 1] *= 1 + 1e-4 * rng.standard_normal()
+        sets_p.append(p)
+        sets_n.append(nn_)
+    got = ctx.reduce_metric_multi("NLL", sets_p, sets_n)
+    for k in range(20):
+        ctx.update_params(sets_p[k])
+        ctx.update_norms(sets_n[k])
+        single = ctx.reduce_metric("NLL")[0]
+        assert got[k] == single, (k, got[k], single)
+    # the interpreter path: an AddPdf model
+    top1, x = add1d()
+    data = W.sample_add1d(20_000, seed=9)
+    M1 = O.Model(top1, data.reshape(-1, 1))
+    M1.calculate_nll()
+    _install(ctx, top1)
+    ctx.upload_columns([data])
+    _sync(ctx, M1)
+    q0, m0 = M1.op.p.copy(), M1.op.n.copy()
+    sp = [q0 * (1 + 1e-3 * k) for k in range(3)]
+    sn = [m0 for _ in range(3)]
+    got1 = ctx.reduce_metric_multi("NLL", sp, sn)
+    for k in range(3):
+        ctx.update_params(sp[k])
+        ctx.update_norms(sn[k])
+        assert got1[k] == ctx.reduce_metric("NLL")[0]
+
+
 def test_dalitz_generate(ctx):
     """Accept-reject generation on the GPU: reproducible for a seed, every event inside the plot,
     and the histogram over cells fully inside the plot follows |A|^2 (chi^2 per degree of freedom)."""
