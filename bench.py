@@ -286,7 +286,7 @@ def fit_toys(model, data, counts, truth, sync, seq_toys, threads):
                 speedup=round((n / t_batch) / (seq_toys / t_seq), 3) if t_seq > 0 else None,
                 launches=st["launches"], fcn_calls=st["fcn_calls"], gpu_wait_s=round(st["gpu_wait_s"], 3),
                 host_prep_s=round(st["host_prep_s"], 3), gather_s=round(st["gather_s"], 3),
-                chunk=st["chunk"], pull_mean=[round(float(v), 4) for v in pm],
+                chunk=st["chunk"], multi_point=bool(bf.multiPoint()), pull_mean=[round(float(v), 4) for v in pm],
                 pull_width=[round(float(v), 4) for v in pw],
                 max_abs_pull_mean=round(float(np.abs(pm).max()), 4),
                 max_abs_pull_width_minus_1=round(float(np.abs(pw - 1).max()), 4),
@@ -295,6 +295,16 @@ def fit_toys(model, data, counts, truth, sync, seq_toys, threads):
                 toys_with_pull_beyond_5=n_tail,
                 batched_vs_sequential_max_shift_sigma=round(float(max(shifts)), 5) if shifts else None,
                 _raw=dict(pulls=pulls, counts=counts[ok], vals=vals[ok], errs=errs[ok], truth=tv, free=free))
+
+
+# Normalisation grid of the Dalitz toy fits.  The fits integrate the density with the
+# midpoint rule over the bins whose centres lie inside the Dalitz region, while the toys
+# are drawn from the exact density; the staircase along the kinematic boundary leaves a
+# normalisation error of order one bin width, which at 400 x 400 shifted the pull means
+# of 2e5-event toys by up to 0.1 sigma.  At 1600 x 1600 the shift is a quarter of that,
+# and the integrals are computed once per toy batch (lineshapes are fixed), so the finer
+# grid costs nothing per fit.
+TOY_GRID = 1600
 
 
 def toy_study(ntoys, sync, seq_toys=16, threads=256):
@@ -323,7 +333,7 @@ def toy_study(ntoys, sync, seq_toys=16, threads=256):
     del d
     # 8-resonance Dalitz toys
     counts = toy_counts(half, 20_000)
-    md = W.build_dalitz(nbins=400, resonances=W.DALITZ_RESONANCES[:8])
+    md = W.build_dalitz(nbins=TOY_GRID, resonances=W.DALITZ_RESONANCES[:8])
     truth = {k: v.value for k, v in md.params.items()}
     t0 = time.perf_counter()
     g12, g13 = W.generate_dalitz(md, int(counts.sum()), seed=77)
@@ -344,6 +354,33 @@ def toy_study(ntoys, sync, seq_toys=16, threads=256):
     seq_rate = 2.0 / (1 / out["add1d"]["sequential_toys_per_hour"] + 1 / out["dalitz8"]["sequential_toys_per_hour"])
     out["all"] = dict(toys=2 * half, batched_s=round(tot_t, 3), batched_toys_per_hour=round(2 * half / tot_t * 3600, 1),
                       sequential_toys_per_hour=round(seq_rate, 1))
+    return out
+
+
+def pull_study(ntoys, mean_events, sync, threads=512, nbins=1600):
+    """Pulls of the 8-resonance Dalitz toys at a larger size: ntoys toys, Poisson sizes of mean
+    mean_events, fitted in batches with an nbins x nbins normalisation grid (not part of the
+    default run: --pull-study)."""
+    import goofit_amd as G
+    from goofit_amd import workloads as W
+    rng = np.random.default_rng(90_000)
+    counts = rng.poisson(mean_events, size=ntoys).astype(np.int64)
+    md = W.build_dalitz(nbins=nbins, resonances=W.DALITZ_RESONANCES[:8])
+    truth = {k: v.value for k, v in md.params.items()}
+    t0 = time.perf_counter()
+    g12, g13 = W.generate_dalitz(md, int(counts.sum()), seed=91)
+    sync()
+    t_gen = time.perf_counter() - t0
+    dd = W.dalitz_dataset(md, g12, g13)
+    md.pdf.setData(dd)
+    md.pdf.setDataSize(int(counts.sum()))
+    md.pdf.setForceIntegrals()
+    md.pdf.calculateNLL()
+    out = fit_toys(md, dd, counts, truth, sync, 4, threads)
+    TOY_RAW["dalitz8_large"] = out.pop("_raw")
+    out["generation_s"] = round(t_gen, 2)
+    out["mean_events"] = mean_events
+    out["norm_grid"] = f"{nbins}x{nbins}"
     return out
 
 
@@ -388,6 +425,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the normalisation-error study")
     ap.add_argument("--toys", type=int, default=4096, help="batched toy fits (0: skip)")
+    ap.add_argument("--pull-study", type=int, default=0, metavar="NTOYS",
+                    help="also fit NTOYS 8-resonance Dalitz toys of 2e5 events each for their pulls")
+    ap.add_argument("--pull-grid", type=int, default=TOY_GRID, help="normalisation grid of the pull study's fits")
     ap.add_argument("--profile-only", action="store_true",
                     help="set up the Dalitz workload, run 20 NLL reductions and a 256 MiB device copy "
                          "(HBM counter calibration), print nothing else; for rocprofv3 runs")
@@ -553,6 +593,13 @@ def main():
     if args.toys > 0:
         log(f"toy study: {args.toys} toys")
         toys = toy_study(args.toys, sync)
+
+    if args.pull_study > 0:
+        log(f"pull study: {args.pull_study} toys of 2e5 events, {args.pull_grid}^2 normalisation grid")
+        ps = pull_study(args.pull_study, 200_000, sync, nbins=args.pull_grid)
+        toys = dict(toys or {}, dalitz8_2e5=ps)
+        log("dalitz8 2e5-event toys", {k: ps[k] for k in ("batched_s", "max_abs_pull_mean",
+                                                         "max_abs_pull_width_minus_1", "pull_width")})
 
     workloads = {}
     if not args.no_extra:

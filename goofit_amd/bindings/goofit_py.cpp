@@ -540,6 +540,8 @@ PYBIND11_MODULE(_goofit, m) {
              py::arg("autotune") = true, py::call_guard<py::gil_scoped_release>())
         .def("evaluate", &BatchFitter::evaluate)
         .def("useGraph", &BatchFitter::useGraph)
+        .def("useMultiPoint", &BatchFitter::useMultiPoint)
+        .def("multiPoint", &BatchFitter::multiPoint)
         .def("__len__", &BatchFitter::size)
         .def("values", &BatchFitter::values)
         .def("errors", &BatchFitter::errors)

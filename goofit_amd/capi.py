@@ -38,6 +38,7 @@ EXPORTS = [
     "gf_mlp_sample", "gf_mlp_train_host", "gf_mlp_use_graph", "gf_dalitz_set_norm_points", "gf_dalitz_gram_norm_points",
     "gf_dalitz_generate", "gf_batch_create", "gf_batch_destroy", "gf_batch_set_toy", "gf_batch_reduce",
     "gf_batch_set_chunk", "gf_batch_get_chunk", "gf_batch_use_graph", "gf_batch_autotune",
+    "gf_batch_add_point", "gf_batch_reduce_points", "gf_batch_clear_points",
 ]
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -122,6 +123,9 @@ def lib():
         L.gf_batch_get_chunk.argtypes = [c_p, _I]
         L.gf_batch_use_graph.argtypes = [c_p, ctypes.c_int]
         L.gf_batch_autotune.argtypes = [c_p, ctypes.c_int, _I, _D]
+        L.gf_batch_add_point.argtypes = [c_p, ctypes.c_int, _D, _D]
+        L.gf_batch_reduce_points.argtypes = [c_p, ctypes.c_int, _D]
+        L.gf_batch_clear_points.argtypes = [c_p]
         L.gf_dalitz_generate.argtypes = [c_p, ctypes.c_int, ctypes.c_int64, ctypes.c_uint64, _D, ctypes.c_double,
                                          ctypes.c_int, c_p, c_p, ctypes.POINTER(ctypes.c_int64), _D]
         L.gf_time_dalitz_cache.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D]
@@ -427,6 +431,9 @@ def sample_uniform(ctx: Context, box, n, seed, dev_m12: int, dev_m13: int, dev_w
     _check(lib().gf_sample_uniform(ctx.h, _dp(b), n, seed, dev_m12, dev_m13, dev_w))
 
 
+BATCH_MAX_POINTS = 64  # GF_BATCH_MAX_POINTS
+
+
 class Batch:
     """A gf_batch: toys over row ranges of a Context's events, reduced in one launch."""
 
@@ -460,6 +467,20 @@ class Batch:
         m = METRIC[metric] if isinstance(metric, str) else metric
         _check(lib().gf_batch_reduce(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), m, _dp(out)))
         return out
+
+    def add_point(self, t, params, norms):
+        p, n = _f64(params), _f64(norms)
+        _check(lib().gf_batch_add_point(self.h, t, _dp(p), _dp(n)))
+
+    def reduce_points(self, metric="NLL"):
+        """Evaluate every added point: [ntoys, BATCH_MAX_POINTS] array, NaN where no point was added."""
+        out = np.full(self.n * BATCH_MAX_POINTS, np.nan)
+        m = METRIC[metric] if isinstance(metric, str) else metric
+        _check(lib().gf_batch_reduce_points(self.h, m, _dp(out)))
+        return out.reshape(self.n, BATCH_MAX_POINTS)
+
+    def clear_points(self):
+        _check(lib().gf_batch_clear_points(self.h))
 
     def set_chunk(self, chunk):
         _check(lib().gf_batch_set_chunk(self.h, chunk))

@@ -2676,7 +2676,16 @@ struct gf_batch {
     hipGraphExec_t exec = nullptr;
     int graph_metric = -1, graph_chunk = 0;
     bool use_graph = true;
+    // several points per toy: pinned stage = table (int pairs {first slot, points}
+    // per toy, ntoys doubles) then slots of np + nn doubles; device copy alike
+    double *h_pts = nullptr, *d_pts = nullptr;
+    size_t pts_cap = 0;  // slots
+    int pts_slots = 0, pts_last = -1, pts_max = 0;
+    gfk::Partial *d_pts_partials = nullptr;  // [nblocks][GF_BATCH_PTS]
+    int pts_partials_blocks = 0;
+    double *h_pts_res = nullptr, *d_pts_res = nullptr;  // mapped [ntoys][GF_BATCH_PTS][2]
 };
+static_assert(GF_BATCH_PTS == GF_BATCH_MAX_POINTS, "point limits of the C ABI and the kernels");
 
 namespace {
 
@@ -2859,11 +2868,11 @@ void gf_batch_destroy(gf_batch *b) {
         (void)hipStreamSynchronize(b->ctx->stream);
     }
     batch_drop_graph(b);
-    void *dev[] = {b->d_stage, b->d_first, b->d_work, b->d_partials, b->d_counter};
+    void *dev[] = {b->d_stage, b->d_first, b->d_work, b->d_partials, b->d_counter, b->d_pts, b->d_pts_partials};
     for(void *p : dev)
         if(p)
             (void)hipFree(p);
-    void *host[] = {b->h_stage, b->h_res, b->h_stamp};
+    void *host[] = {b->h_stage, b->h_res, b->h_stamp, b->h_pts, b->h_pts_res};
     for(void *p : host)
         if(p)
             (void)hipHostFree(p);
@@ -2899,6 +2908,143 @@ gf_status gf_batch_reduce(gf_batch *b, const uint8_t *active, int metric, double
         if(active[t])
             out[t] = ((const volatile double *)b->h_res)[2 * t];
     return GF_OK;
+}
+
+namespace {
+
+int *batch_table(gf_batch *b) { return (int *)b->h_pts; }
+
+// Room for `slots` points in the pinned stage and its device copy (contents kept).
+gf_status batch_points_reserve(gf_batch *b, size_t slots) {
+    if(slots <= b->pts_cap && b->h_pts)
+        return GF_OK;
+    size_t cap = std::max<size_t>(b->pts_cap ? b->pts_cap : (size_t)std::max(b->ntoys, 256), 1);
+    while(cap < slots)
+        cap *= 2;
+    const size_t W     = (size_t)(b->np + b->nn);
+    const size_t bytes = ((size_t)b->ntoys + cap * W) * sizeof(double);
+    double *h = nullptr, *d = nullptr;
+    if(hipHostMalloc((void **)&h, bytes, hipHostMallocDefault) != hipSuccess) {
+        return fail(GF_ERR_NOMEM, "cannot allocate a stage of %zu points", cap);
+    }
+    if(hipMalloc((void **)&d, bytes) != hipSuccess) {
+        (void)hipHostFree(h);
+        return fail(GF_ERR_NOMEM, "cannot allocate a device stage of %zu points", cap);
+    }
+    if(b->h_pts) {
+        memcpy(h, b->h_pts, ((size_t)b->ntoys + (size_t)b->pts_slots * W) * sizeof(double));
+        GF_HIP(hipStreamSynchronize(b->ctx->stream));  // no copy from the old stage in flight
+        (void)hipHostFree(b->h_pts);
+        (void)hipFree(b->d_pts);
+    } else {
+        memset(h, 0, (size_t)b->ntoys * sizeof(double));
+    }
+    b->h_pts   = h;
+    b->d_pts   = d;
+    b->pts_cap = cap;
+    return GF_OK;
+}
+
+}  // namespace
+
+gf_status gf_batch_add_point(gf_batch *b, int toy, const double *params, const double *norms) {
+    if(!b || toy < 0 || toy >= b->ntoys)
+        return fail(GF_ERR_INVALID, "toy index out of range");
+    if(!params || !norms)
+        return fail(GF_ERR_INVALID, "null parameter or normalisation array");
+    gf_status st = batch_points_reserve(b, (size_t)b->pts_slots + 1);
+    if(st)
+        return st;
+    int *tbl = batch_table(b);
+    if(tbl[2 * toy + 1] > 0 && b->pts_last != toy)
+        return fail(GF_ERR_INVALID, "the points of toy %d must be added one after the other", toy);
+    if(tbl[2 * toy + 1] >= GF_BATCH_PTS)
+        return fail(GF_ERR_INVALID, "toy %d already has %d points", toy, GF_BATCH_PTS);
+    if(tbl[2 * toy + 1] == 0)
+        tbl[2 * toy] = b->pts_slots;
+    const size_t W = (size_t)(b->np + b->nn);
+    double *slot   = b->h_pts + b->ntoys + (size_t)b->pts_slots * W;
+    memcpy(slot, params, (size_t)b->np * sizeof(double));
+    memcpy(slot + b->np, norms, (size_t)b->nn * sizeof(double));
+    tbl[2 * toy + 1]++;
+    b->pts_slots++;
+    b->pts_last = toy;
+    b->pts_max  = std::max(b->pts_max, tbl[2 * toy + 1]);
+    return GF_OK;
+}
+
+gf_status gf_batch_clear_points(gf_batch *b) {
+    if(!b)
+        return fail(GF_ERR_INVALID, "null batch");
+    if(b->h_pts)
+        memset(b->h_pts, 0, (size_t)b->ntoys * sizeof(double));
+    b->pts_slots = 0;
+    b->pts_last  = -1;
+    b->pts_max   = 0;
+    return GF_OK;
+}
+
+gf_status gf_batch_reduce_points(gf_batch *b, int metric, double *out) {
+    if(!b || !out)
+        return fail(GF_ERR_INVALID, "null argument");
+    if(!valid_ctx(b->ctx))
+        return GF_ERR_INVALID;
+    gf_ctx *ctx  = b->ctx;
+    gf_status st = check_events_for_program(ctx, metric);
+    if(st)
+        return st;
+    if(b->np != (int)ctx->params.size() || b->nn != (int)ctx->norms.size())
+        return fail(GF_ERR_STATE, "the context's program changed since the batch was created");
+    if(b->pts_slots == 0)
+        return GF_OK;
+    if((st = flush_uploads(ctx)))
+        return st;
+    if(b->pts_partials_blocks != b->nblocks) {
+        GF_HIP(hipStreamSynchronize(ctx->stream));
+        if(b->d_pts_partials)
+            GF_HIP(hipFree(b->d_pts_partials));
+        b->d_pts_partials      = nullptr;
+        b->pts_partials_blocks = 0;
+        if(hipMalloc((void **)&b->d_pts_partials, (size_t)b->nblocks * GF_BATCH_PTS * sizeof(gfk::Partial))
+           != hipSuccess) {
+            b->d_pts_partials = nullptr;
+            return fail(GF_ERR_NOMEM, "cannot allocate the partials of %d workgroups", b->nblocks);
+        }
+        b->pts_partials_blocks = b->nblocks;
+    }
+    if(!b->h_pts_res) {
+        if(hipHostMalloc((void **)&b->h_pts_res, (size_t)2 * b->ntoys * GF_BATCH_PTS * sizeof(double),
+                         hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess
+           || hipHostGetDevicePointer((void **)&b->d_pts_res, b->h_pts_res, 0) != hipSuccess) {
+            if(b->h_pts_res)
+                (void)hipHostFree(b->h_pts_res);
+            b->h_pts_res = b->d_pts_res = nullptr;
+            return fail(GF_ERR_NOMEM, "cannot allocate the batch results");
+        }
+    }
+    const size_t W  = (size_t)(b->np + b->nn);
+    const bool fast = use_fast_dalitz(ctx, metric);
+    const bool ce   = fast && const_efficiency(ctx);
+    bool zo         = ce && zero_outside(ctx);
+    for(int k = 0; k < b->pts_slots && zo; k++)  // every point's efficiency constant finite
+        zo = std::isfinite(b->h_pts[b->ntoys + (size_t)k * W + ctx->ins[1].p + ctx->ins[1].a]);
+    GF_HIP(hipMemcpyAsync(b->d_pts, b->h_pts, ((size_t)b->ntoys + (size_t)b->pts_slots * W) * sizeof(double),
+                          hipMemcpyHostToDevice, ctx->stream));
+    const int groups = (b->pts_max + GF_MULTI_PTS - 1) / GF_MULTI_PTS;
+    GfProgArgs a     = prog_args(ctx);
+    if(gfk::launch_batch_points(a, b->d_work, b->nblocks, groups, b->d_pts + b->ntoys, b->np, b->nn,
+                                (const int *)b->d_pts, fast, ce, zo, metric, b->d_pts_partials, b->d_first,
+                                b->ntoys, b->d_pts_res, b->d_counter, b->d_stamp, ctx->stream))
+        return fail(GF_ERR_HIP, "batch launch failed");
+    b->expect++;
+    if((st = batch_wait(b)))
+        return st;
+    const int *tbl = batch_table(b);
+    const volatile double *res = b->h_pts_res;
+    for(int t = 0; t < b->ntoys; t++)
+        for(int k = 0; k < tbl[2 * t + 1]; k++)
+            out[(size_t)t * GF_BATCH_PTS + k] = res[2 * ((size_t)t * GF_BATCH_PTS + k)];
+    return gf_batch_clear_points(b);
 }
 
 gf_status gf_batch_set_chunk(gf_batch *b, int chunk) {

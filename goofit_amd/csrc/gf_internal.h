@@ -179,6 +179,16 @@ int launch_reduce_bins(const GfProgArgs &a, int pc0, const double *ranges, const
 int launch_batch(const GfProgArgs &a, const GfBatchWork *work, int nblocks, const double *params, int np,
                  const double *norms, int nn, const uint8_t *active, int metric, Partial *partials, const int *first,
                  int ntoys, double *result, uint32_t *counter, uint32_t *stamp, void *stream);
+// Toy batch with several points per toy: tbl[2t], tbl[2t + 1] = first stage slot, points;
+// each stage slot holds np parameters then nn normalisation values.  The grid is
+// nblocks x ceil(max points / GF_MULTI_PTS); partials[b * GF_BATCH_PTS + p];
+// result[2 (t GF_BATCH_PTS + p) ..] (host-mapped), then the stamp.
+// fast: the coherent-sum program of launch_dalitz_fast.
+#define GF_BATCH_PTS 64
+int launch_batch_points(const GfProgArgs &a, const GfBatchWork *work, int nblocks, int groups, const double *stage,
+                        int np, int nn, const int *tbl, bool fast, bool const_eff, bool zero_outside, int metric,
+                        Partial *partials, const int *first, int ntoys, double *result, uint32_t *counter,
+                        uint32_t *stamp, void *stream);
 // Accept-reject generation: candidates (uniform in box) with accept flags, and
 // the ordered compaction of the accepted ones.
 // eff_pc >= 0: efficiency subprogram; -1: efficiency 1; -2: the constant eff_const.

@@ -843,6 +843,43 @@ __global__ __launch_bounds__(256) void k_dalitz_cache(GfProgArgs A, int slot, in
 // One entry of a batch's work table: rows [start, end) of toy `toy`.
 // (GfBatchWork is declared in gf_internal.h)
 
+// The metric sum over rows [w.start, w.end) of one toy with parameter array P and
+// normalisation array Nm, accumulated per thread in the tile order of
+// k_batch_metric (EPT rows per thread per tile).
+template <int EPT>
+__device__ __forceinline__ void batch_rows(const GfProgArgs &A, const GfBatchWork &w, const double *__restrict__ P,
+                                           const double *__restrict__ Nm, int metric, double &s, double &c) {
+    const double norm  = Nm[1];
+    const bool binned  = metric >= GF_METRIC_BINAVG_CODE;
+    const int64_t tile = (int64_t)GF_BLOCK * EPT;
+    for(int64_t base = w.start; base < w.end; base += tile) {
+        ColSrc<EPT> src;
+        src.cols = A.cols;
+        int64_t row[EPT];
+        bool valid[EPT];
+#pragma unroll
+        for(int j = 0; j < EPT; ++j) {
+            const int64_t r = base + (int64_t)j * GF_BLOCK + threadIdx.x;
+            valid[j]        = r < w.end;
+            row[j]          = valid[j] ? r : w.start;
+            src.idx[j]      = row[j];
+        }
+        double T[EPT];
+        run<EPT>(A, P, Nm, 0, src, row, valid, T);
+#pragma unroll
+        for(int j = 0; j < EPT; ++j) {
+            if(!valid[j])
+                continue;
+            double e0 = 0.0, e1 = 0.0;
+            if(binned) {
+                e0 = A.cols[A.nobs - 2][row[j]];
+                e1 = A.cols[A.nobs - 1][row[j]];
+            }
+            gfd::kb_add(s, c, gfd::metric(metric, T[j], e0, e1, norm, w.total));
+        }
+    }
+}
+
 // Metric sums of many toys in one launch: workgroup b evaluates the rows of
 // work[b] with its toy's parameter and normalisation arrays (P = params +
 // toy * np, Nm = norms + toy * nn) and stores one partial; workgroups of
@@ -855,40 +892,130 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_metric(GfProgArgs A, const G
                                                            Partial *partials) {
     const GfBatchWork w = work[blockIdx.x];
     double s = 0.0, c = 0.0;
-    if(active[w.toy]) {
-        const double *__restrict__ P  = params + (int64_t)w.toy * np;
-        const double *__restrict__ Nm = norms + (int64_t)w.toy * nn;
-        const double norm             = Nm[1];
-        const bool binned             = metric >= GF_METRIC_BINAVG_CODE;
-        const int64_t tile            = (int64_t)GF_BLOCK * EPT;
-        for(int64_t base = w.start; base < w.end; base += tile) {
-            ColSrc<EPT> src;
-            src.cols = A.cols;
-            int64_t row[EPT];
-            bool valid[EPT];
+    if(active[w.toy])
+        batch_rows<EPT>(A, w, params + (int64_t)w.toy * np, norms + (int64_t)w.toy * nn, metric, s, c);
+    store_partial(s, c, partials);
+}
+
+// Several points per toy, any program: workgroup (b, g) evaluates the rows of
+// work[b] once per point p in [16 g, min(16 g + 16, npts)) of its toy and stores
+// partial (b, p).  Each point's sum takes the path of k_batch_metric.
+template <int EPT>
+__global__ __launch_bounds__(GF_BLOCK) void k_batch_points(GfProgArgs A, const GfBatchWork *__restrict__ work,
+                                                           const double *__restrict__ stage, int np, int nn,
+                                                           const int *__restrict__ tbl, int metric,
+                                                           Partial *partials) {
+    const GfBatchWork w = work[blockIdx.x];
+    const int slot0     = tbl[2 * w.toy];
+    const int npts      = tbl[2 * w.toy + 1];
+    const int p0        = (int)blockIdx.y * GF_MULTI_PTS;
+    const int p1        = min(npts, p0 + GF_MULTI_PTS);
+    for(int p = p0; p < p1; ++p) {
+        const double *__restrict__ P = stage + (int64_t)(slot0 + p) * (np + nn);
+        double s = 0.0, c = 0.0;
+        batch_rows<EPT>(A, w, P, P + np, metric, s, c);
+        block_reduce(s, c);
+        if(threadIdx.x == 0) {
+            partials[(int64_t)blockIdx.x * GF_BATCH_PTS + p].s = s;
+            partials[(int64_t)blockIdx.x * GF_BATCH_PTS + p].c = c;
+        }
+    }
+}
+
+// Several points per toy, coherent-sum program (as k_dalitz_multi): workgroup
+// (b, g) reads the waves of each row of work[b] once and adds the row's metric
+// for every point p in [16 g, min(16 g + 16, npts)) of its toy.
+template <bool CONST_EFF, bool ZERO_OUT>
+__global__ __launch_bounds__(GF_BLOCK) void k_batch_dalitz_points(GfProgArgs A, const GfBatchWork *__restrict__ work,
+                                                                  const double *__restrict__ stage, int np, int nn,
+                                                                  const int *__restrict__ tbl, int metric,
+                                                                  Partial *partials) {
+    const GfBatchWork w = work[blockIdx.x];
+    const int slot0     = tbl[2 * w.toy];
+    const int npts      = tbl[2 * w.toy + 1];
+    const int p0        = (int)blockIdx.y * GF_MULTI_PTS;
+    const int cnt       = min(npts - p0, GF_MULTI_PTS);
+    if(cnt <= 0)
+        return;  // uniform over the workgroup
+    const GfIns dzi       = A.ins[0];
+    const GfIns effi      = A.ins[1];
+    const GfDalitzSlot dz = A.dz[dzi.x];
+    const GfDecay &D      = A.decay;
+    const int nres        = dzi.a;
+    const int W           = np + nn;
+    const double *__restrict__ S0  = stage + (int64_t)(slot0 + p0) * W;
+    const double *__restrict__ c12 = A.cols[dz.col_m12];
+    const double *__restrict__ c13 = A.cols[dz.col_m13];
+    double s[GF_MULTI_PTS], c[GF_MULTI_PTS];
 #pragma unroll
-            for(int j = 0; j < EPT; ++j) {
-                const int64_t r = base + (int64_t)j * GF_BLOCK + threadIdx.x;
-                valid[j]        = r < w.end;
-                row[j]          = valid[j] ? r : w.start;
-                src.idx[j]      = row[j];
-            }
-            double T[EPT];
-            run<EPT>(A, P, Nm, 0, src, row, valid, T);
+    for(int p = 0; p < GF_MULTI_PTS; ++p)
+        s[p] = c[p] = 0.0;
+    for(int64_t base = w.start; base < w.end; base += GF_BLOCK) {
+        const int64_t e0 = base + threadIdx.x;
+        const bool valid = e0 < w.end;
+        const int64_t e  = valid ? e0 : w.start;
+        double2 wv[GF_MAX_RES];
 #pragma unroll
-            for(int j = 0; j < EPT; ++j) {
-                if(!valid[j])
-                    continue;
-                double e0 = 0.0, e1 = 0.0;
-                if(binned) {
-                    e0 = A.cols[A.nobs - 2][row[j]];
-                    e1 = A.cols[A.nobs - 1][row[j]];
+        for(int i = 0; i < GF_MAX_RES; ++i)
+            wv[i] = (i < nres) ? dz.wave[(int64_t)i * dz.ncache + e] : make_double2(0.0, 0.0);
+        const double m12  = ZERO_OUT ? 0.0 : c12[e];
+        const double m13  = ZERO_OUT ? 0.0 : c13[e];
+        const bool inside =
+            ZERO_OUT || gfd::in_dalitz(m12, m13, D.motherMass, D.daug1Mass, D.daug2Mass, D.daug3Mass);
+#pragma unroll
+        for(int p = 0; p < GF_MULTI_PTS; ++p) {
+            if(p < cnt) {
+                const double *__restrict__ P = S0 + (int64_t)p * W;
+                const double norm            = P[np + 1];
+                double re = 0.0, im = 0.0;
+#pragma unroll
+                for(int i = 0; i < GF_MAX_RES; ++i) {
+                    if(i < nres) {
+                        const double ar = P[dzi.p + 2 * i], ai = P[dzi.p + 2 * i + 1];
+                        re += ar * wv[i].x - ai * wv[i].y;
+                        im += ar * wv[i].y + ai * wv[i].x;
+                    }
                 }
-                gfd::kb_add(s, c, gfd::metric(metric, T[j], e0, e1, norm, w.total));
+                const double eff = CONST_EFF ? P[effi.p + effi.a] : multipoly_at(effi, A, P, e);
+                const double T   = inside ? (re * re + im * im) * eff : 0.0;
+                const double v   = gfd::metric(metric, T, 0.0, 0.0, norm, w.total);
+                if(valid)
+                    gfd::kb_add(s[p], c[p], v);
             }
         }
     }
-    store_partial(s, c, partials);
+#pragma unroll
+    for(int p = 0; p < GF_MULTI_PTS; ++p) {
+        if(p < cnt) {
+            double ss = s[p], cc = c[p];
+            block_reduce(ss, cc);
+            if(threadIdx.x == 0) {
+                partials[(int64_t)blockIdx.x * GF_BATCH_PTS + p0 + p].s = ss;
+                partials[(int64_t)blockIdx.x * GF_BATCH_PTS + p0 + p].c = cc;
+            }
+        }
+    }
+}
+
+// One workgroup per toy: for each of its points, the partials of the toy's work
+// entries [first[t], first[t + 1]) in the order of k_finish_toys into
+// result[2 (t GF_BATCH_PTS + p) ..] (host-mapped).
+__global__ __launch_bounds__(GF_BLOCK) void k_finish_toy_points(const Partial *__restrict__ partials,
+                                                                const int *__restrict__ first,
+                                                                const int *__restrict__ tbl,
+                                                                double *__restrict__ result) {
+    const int t    = blockIdx.x;
+    const int npts = tbl[2 * t + 1];
+    for(int p = 0; p < npts; ++p) {
+        double s = 0.0, c = 0.0;
+        for(int i = first[t] + threadIdx.x; i < first[t + 1]; i += GF_BLOCK)
+            gfd::kb_merge(s, c, partials[(int64_t)i * GF_BATCH_PTS + p].s, partials[(int64_t)i * GF_BATCH_PTS + p].c);
+        block_reduce(s, c);
+        if(threadIdx.x == 0) {
+            result[2 * ((int64_t)t * GF_BATCH_PTS + p)]     = gfd::kb_total(s, c);
+            result[2 * ((int64_t)t * GF_BATCH_PTS + p) + 1] = c;
+        }
+    }
 }
 
 // One workgroup per toy: the toy's partials [first[t], first[t + 1]) in order
@@ -1344,6 +1471,36 @@ int launch_batch(const GfProgArgs &a, const GfBatchWork *work, int nblocks, cons
     if(rc)
         return rc;
     hipLaunchKernelGGL(k_finish_toys, dim3(ntoys), dim3(GF_BLOCK), 0, s, partials, first, active, result);
+    if((rc = check(hipGetLastError())))
+        return rc;
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, s, counter, stamp);
+    return check(hipGetLastError());
+}
+
+int launch_batch_points(const GfProgArgs &a, const GfBatchWork *work, int nblocks, int groups, const double *stage,
+                        int np, int nn, const int *tbl, bool fast, bool const_eff, bool zero_outside, int metric,
+                        Partial *partials, const int *first, int ntoys, double *result, uint32_t *counter,
+                        uint32_t *stamp, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if(groups < 1 || groups * GF_MULTI_PTS > GF_BATCH_PTS)
+        return -1;
+    const dim3 grid(nblocks, groups);
+    if(!fast)
+        hipLaunchKernelGGL(k_batch_points<2>, grid, dim3(GF_BLOCK), 0, s, a, work, stage, np, nn, tbl, metric,
+                           partials);
+    else if(const_eff && zero_outside)
+        hipLaunchKernelGGL((k_batch_dalitz_points<true, true>), grid, dim3(GF_BLOCK), 0, s, a, work, stage, np, nn,
+                           tbl, metric, partials);
+    else if(const_eff)
+        hipLaunchKernelGGL((k_batch_dalitz_points<true, false>), grid, dim3(GF_BLOCK), 0, s, a, work, stage, np, nn,
+                           tbl, metric, partials);
+    else
+        hipLaunchKernelGGL((k_batch_dalitz_points<false, false>), grid, dim3(GF_BLOCK), 0, s, a, work, stage, np, nn,
+                           tbl, metric, partials);
+    int rc = check(hipGetLastError());
+    if(rc)
+        return rc;
+    hipLaunchKernelGGL(k_finish_toy_points, dim3(ntoys), dim3(GF_BLOCK), 0, s, partials, first, tbl, result);
     if((rc = check(hipGetLastError())))
         return rc;
     hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, s, counter, stamp);

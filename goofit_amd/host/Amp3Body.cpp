@@ -155,17 +155,24 @@ fptype Amp3Body::normalize() {
         for(size_t k = 0; k < nres * nres; k++)
             integrals[k] = {out[2 * k], out[2 * k + 1]};
     }
-    std::complex<fptype> sumIntegral(0, 0);
+    // sum_ij c_i conj(c_j) I_ij in the reference's order, each complex product
+    // written out as (a + ib)(c + id) = (ac - bd) + i(ad + bc), which is what the
+    // std::complex products compute for finite values (this runs once per FCN point)
+    if(parametersIdx + 2 * nres >= host_parameters.size())
+        throw GeneralError("Amp3Body::normalize: parameter block outside the parameter array");
+    const fptype *cp = host_parameters.data() + parametersIdx + 1;
+    const auto *I    = reinterpret_cast<const fptype *>(integrals.data());
+    fptype sum_re = 0;  // the imaginary part is not used
     for(size_t i = 0; i < nres; ++i) {
-        std::complex<fptype> ai(host_parameters.at(parametersIdx + i * 2 + 1),
-                                host_parameters.at(parametersIdx + i * 2 + 2));
+        const fptype ar = cp[2 * i], ai = cp[2 * i + 1];
         for(size_t j = 0; j < nres; ++j) {
-            std::complex<fptype> aj(host_parameters.at(parametersIdx + j * 2 + 1),
-                                    -host_parameters.at(parametersIdx + j * 2 + 2));
-            sumIntegral += ai * aj * integrals[i * nres + j];
+            const fptype br = cp[2 * j], bi = -cp[2 * j + 1];
+            const fptype pr = ar * br - ai * bi, pi = ar * bi + ai * br;
+            const fptype ir = I[2 * (i * nres + j)], ii = I[2 * (i * nres + j) + 1];
+            sum_re += pr * ir - pi * ii;
         }
     }
-    fptype ret           = sumIntegral.real();
+    fptype ret           = sum_re;
     double binSizeFactor = 1;
     if(normPoints == 0) {  // Monte-Carlo weights already carry the measure
         binSizeFactor *= _m12.getBinSize();

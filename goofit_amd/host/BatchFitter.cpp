@@ -23,22 +23,46 @@ namespace {
 struct Batcher {
     std::mutex mu;
     std::condition_variable cv_req, cv_done;
-    std::vector<std::vector<double>> x;  // per toy: the requested Minuit vector
+    std::vector<const std::vector<std::vector<double>> *> x;  // per toy: the requested Minuit vectors
+    std::vector<size_t> x0, xn;                               // of those, the points of this request
     std::vector<uint8_t> pending, done;
-    std::vector<double> result;
+    std::vector<std::vector<double>> result;
     int running = 0;   // toys whose MIGRAD is in progress
     int waiting = 0;   // of those, blocked in an FCN call
     bool stop   = false;
 };
 
+// A toy's FCN: each call, or each group of independent points (a derivative
+// round, FCNBase::EvaluateMany), becomes one request to the batcher; with
+// `multi` false a group is served one point per request.
 class ToyFCN : public Minuit2::FCNBase {
   public:
-    ToyFCN(Batcher &b, int toy)
+    ToyFCN(Batcher &b, int toy, bool multi)
         : b_(b)
-        , toy_(toy) {}
+        , toy_(toy)
+        , multi_(multi) {}
     double operator()(const std::vector<double> &x) const override {
+        std::vector<std::vector<double>> one{x};
+        return request(one, 0, 1)[0];
+    }
+    std::vector<double> EvaluateMany(const std::vector<std::vector<double>> &xs) const override {
+        std::vector<double> f;
+        f.reserve(xs.size());
+        const size_t step = multi_ ? (size_t)GF_BATCH_MAX_POINTS : 1;
+        for(size_t k = 0; k < xs.size(); k += step) {
+            const std::vector<double> r = request(xs, k, std::min(step, xs.size() - k));
+            f.insert(f.end(), r.begin(), r.end());
+        }
+        return f;
+    }
+    double Up() const override { return 0.5; }
+
+  private:
+    std::vector<double> request(const std::vector<std::vector<double>> &xs, size_t k0, size_t n) const {
         std::unique_lock<std::mutex> lk(b_.mu);
-        b_.x[toy_]       = x;
+        b_.x[toy_]       = &xs;
+        b_.x0[toy_]      = k0;
+        b_.xn[toy_]      = n;
         b_.done[toy_]    = 0;
         b_.pending[toy_] = 1;
         b_.waiting++;
@@ -46,11 +70,9 @@ class ToyFCN : public Minuit2::FCNBase {
         b_.cv_done.wait(lk, [&] { return b_.done[toy_] != 0; });
         return b_.result[toy_];
     }
-    double Up() const override { return 0.5; }
-
-  private:
     Batcher &b_;
     int toy_;
+    bool multi_;
 };
 
 }  // namespace
@@ -87,7 +109,11 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
     Params params(*pdf_);
     std::vector<Variable> vars = params.GetGooFitParams();
     const std::vector<double> start = params.make_minuit_vector();
+    // updateParameters() as a flat copy: the (Variable, slot) pairs of the tree
+    std::vector<std::pair<Variable, unsigned int>> slots;
+    pdf_->parameterSlots(slots);
     const auto t0 = std::chrono::steady_clock::now();
+    stats_        = BatchFitStats{};
     pdf_->normalize();
     engine::flush();
     if(autotune) {
@@ -97,10 +123,12 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
         engine::check(gf_batch_autotune(batch_, GF_METRIC_NLL, &stats_.chunk, nullptr), "gf_batch_autotune");
     }
     Batcher B;
-    B.x.assign(T, {});
+    B.x.assign(T, nullptr);
+    B.x0.assign(T, 0);
+    B.xn.assign(T, 0);
     B.pending.assign(T, 0);
     B.done.assign(T, 0);
-    B.result.assign(T, 0.0);
+    B.result.assign(T, {});
     values_.assign(T, {});
     errors_.assign(T, {});
     valid_.assign(T, 0);
@@ -119,7 +147,7 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
                 t = next++;
                 B.running++;
             }
-            ToyFCN fcn(B, t);
+            ToyFCN fcn(B, t, multi_);
             Minuit2::MnUserParameters up = static_cast<const Minuit2::MnUserParameters &>(params);
             Minuit2::MnMigrad migrad(fcn, up, Minuit2::MnStrategy(strategy));
             Minuit2::FunctionMinimum m = migrad(0, tolerance);
@@ -138,9 +166,9 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
     for(int i = 0; i < W; i++)
         pool.emplace_back(worker);
 
-    // batcher: serve every pending call of the running toys in one launch
+    // batcher: serve every pending request of the running toys in one launch
     std::vector<uint8_t> act(T);
-    std::vector<double> out(T);
+    std::vector<double> out(multi_ ? (size_t)T * GF_BATCH_MAX_POINTS : (size_t)T);
     std::vector<int> serve;
     std::exception_ptr failure;
     for(;;) {
@@ -166,36 +194,53 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
         std::fill(act.begin(), act.end(), 0);
         const auto p0 = std::chrono::steady_clock::now();
         stats_.gather_s += std::chrono::duration<double>(p0 - w0).count();
+        int64_t points = 0;
         try {
             for(int t : serve) {
-                // push the toy's values into the shared Variables, then the host arrays
-                // as Params::from_minuit_vector: flag the values that move, so that an
-                // Amp3Body recomputes its integrals only when a mass or width changes
-                const std::vector<double> &x = B.x[t];
-                for(Variable &v : vars) {
-                    const double nv = x.at(v.getFitterIndex()) - v.getBlind(Variable::Key());
-                    v.setChanged(v.getValue() != nv);
-                    v.setValue(nv);
+                for(size_t k = 0; k < B.xn[t]; k++) {
+                    // push the point into the shared Variables, then the host arrays as
+                    // Params::from_minuit_vector: flag the values that move, so that an
+                    // Amp3Body recomputes its integrals only when a mass or width changes
+                    const std::vector<double> &x = (*B.x[t])[B.x0[t] + k];
+                    for(Variable &v : vars) {
+                        const double nv = x.at(v.getFitterIndex()) - v.getBlind(Variable::Key());
+                        v.setChanged(v.getValue() != nv);
+                        v.setValue(nv);
+                    }
+                    for(const auto &sl : slots)
+                        host_parameters[sl.second] = sl.first.getValue();
+                    pdf_->normalize();
+                    if(multi_)
+                        engine::check(gf_batch_add_point(batch_, t, host_parameters.data(), host_normalizations.data()),
+                                      "gf_batch_add_point");
+                    else
+                        engine::check(gf_batch_set_toy(batch_, t, host_parameters.data(), host_normalizations.data()),
+                                      "gf_batch_set_toy");
                 }
-                pdf_->updateParameters();
-                pdf_->normalize();
-                engine::check(gf_batch_set_toy(batch_, t, host_parameters.data(), host_normalizations.data()),
-                              "gf_batch_set_toy");
+                points += (int64_t)B.xn[t];
                 act[t] = 1;
             }
             const auto g0 = std::chrono::steady_clock::now();
             stats_.host_prep_s += std::chrono::duration<double>(g0 - p0).count();
-            engine::check(gf_batch_reduce(batch_, act.data(), GF_METRIC_NLL, out.data()), "gf_batch_reduce");
+            if(multi_)
+                engine::check(gf_batch_reduce_points(batch_, GF_METRIC_NLL, out.data()), "gf_batch_reduce_points");
+            else
+                engine::check(gf_batch_reduce(batch_, act.data(), GF_METRIC_NLL, out.data()), "gf_batch_reduce");
             stats_.gpu_wait_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - g0).count();
         } catch(...) {
             failure = std::current_exception();
-            for(int t : serve)
-                out[t] = 1e300;
+            if(multi_)
+                (void)gf_batch_clear_points(batch_);
+            std::fill(out.begin(), out.end(), 1e300);
         }
         stats_.launches++;
-        stats_.fcn_calls += (int64_t)serve.size();
+        stats_.fcn_calls += points;
         for(int t : serve) {
-            B.result[t]  = out[t];
+            if(multi_)
+                B.result[t].assign(out.begin() + (size_t)t * GF_BATCH_MAX_POINTS,
+                                   out.begin() + (size_t)t * GF_BATCH_MAX_POINTS + B.xn[t]);
+            else
+                B.result[t].assign(1, out[t]);
             B.pending[t] = 0;
             B.done[t]    = 1;
             B.waiting--;

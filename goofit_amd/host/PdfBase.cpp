@@ -138,6 +138,14 @@ void PdfBase::updateParameters() {
     pre_call();
 }
 
+void PdfBase::parameterSlots(std::vector<std::pair<Variable, unsigned int>> &out) const {
+    for(size_t i = 0; i < parametersList.size(); i++)
+        if(parametersIdx + i + 1 < host_parameters.size())
+            out.emplace_back(parametersList[i], (unsigned int)(parametersIdx + i + 1));
+    for(auto *c : components)
+        c->parameterSlots(out);
+}
+
 void PdfBase::recursiveSetNormalization(fptype norm, bool subpdf) {
     host_normalizations.at(normalIdx + 1) = norm;
     cachedNormalization                   = norm;

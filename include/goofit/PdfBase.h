@@ -11,6 +11,7 @@
 #include <memory>
 #include <string>
 #include <vector>
+#include <utility>
 
 #include <goofit/Engine.h>
 #include <goofit/Error.h>
@@ -126,6 +127,10 @@ class PdfBase {
     bool parametersChanged() const;
     void updateVariable(const Variable &v, fptype newValue);
     void updateParameters();
+    /// Every (Variable, host_parameters slot) pair that updateParameters() copies,
+    /// this PDF's first and then its components' in the same recursion, so that a
+    /// caller that sets parameters many times can copy them in a flat loop.
+    void parameterSlots(std::vector<std::pair<Variable, unsigned int>> &out) const;
 
     void SigGenSetIndices() {
         setupObservables();

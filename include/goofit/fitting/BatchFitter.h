@@ -2,10 +2,13 @@
 // which fits one data set per FitManager).
 //
 // Toy t owns rows [first[t], first[t] + count[t]) of one data set.  One Minuit2
-// MIGRAD per toy runs on a pool of host threads; every FCN call posts the toy's
-// parameter vector to a batcher, which turns the pending calls of all toys into
-// one GPU launch (a captured hipGraph: staging copy, per-toy event sweep,
-// per-toy finishing reduction) and hands each toy its NLL.  The normalisations
+// MIGRAD per toy runs on a pool of host threads; every FCN call, and every group
+// of independent points (one derivative round, up to 64 points), posts the toy's
+// parameter vectors to a batcher, which turns the pending requests of all toys
+// into one GPU launch sequence (staging copy, per-toy event sweep over all of a
+// toy's points, per-toy finishing reduction) and hands each toy its NLLs.  With
+// useMultiPoint(false) every point is its own request, served by a captured
+// hipGraph with one parameter set per toy.  The normalisations
 // are computed on the host per toy (analytic integrals, or an Amp3Body's cached
 // integral matrix), so the model's non-analytic parts must not depend on the
 // floating parameters.
@@ -24,7 +27,7 @@ class UnbinnedDataSet;
 
 struct BatchFitStats {
     int64_t launches   = 0;  // batched GPU reductions
-    int64_t fcn_calls  = 0;  // FCN calls served
+    int64_t fcn_calls  = 0;  // FCN points served
     double seconds     = 0;  // wall time of fit()
     double gpu_wait_s  = 0;  // time spent in the batched reductions
     double host_prep_s = 0;  // time the batcher spends setting each served toy's parameters and norms
@@ -57,6 +60,9 @@ class BatchFitter {
     const BatchFitStats &stats() const { return stats_; }
 
     void useGraph(bool on);
+    /// Several points per toy and launch (default) or one.
+    void useMultiPoint(bool on) { multi_ = on; }
+    bool multiPoint() const { return multi_; }
 
   private:
     GooPdf *pdf_;
@@ -66,6 +72,7 @@ class BatchFitter {
     std::vector<int> valid_, nfcn_;
     std::vector<double> fval_;
     BatchFitStats stats_;
+    bool multi_{true};
 };
 
 }  // namespace GooFit

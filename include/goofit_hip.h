@@ -373,6 +373,23 @@ gf_status gf_batch_use_graph(gf_batch *b, int on);
 /* Time every active-all reduction for chunk in {1024 ... 32768}, keep the fastest. */
 gf_status gf_batch_autotune(gf_batch *b, int metric, int *chunk_out, double *us_out);
 
+/* Several parameter points per toy in one launch (no reference counterpart: the
+ * points of one derivative round of every running fit, FCNBase::EvaluateMany).
+ * gf_batch_add_point appends a point (params: np values, norms: nn values, the
+ * layout of gf_program_set) to toy `toy`; a toy's points are added one after the
+ * other, at most GF_BATCH_MAX_POINTS per toy.  gf_batch_reduce_points evaluates
+ * every added point in one launch sequence (plain stream launches, the staged
+ * points copied in with one copy), writes the metric sum of toy t's k-th point to
+ * out[t * GF_BATCH_MAX_POINTS + k] (other entries untouched) and clears the
+ * points; gf_batch_clear_points drops them without a launch.  A lone Amp3Body
+ * reads each event's waves once for up to 16 points.  Each sum is added in the
+ * order of gf_batch_reduce at the same chunk, so a lone point gives the same
+ * value as gf_batch_reduce for the same arrays. */
+#define GF_BATCH_MAX_POINTS 64
+gf_status gf_batch_add_point(gf_batch *b, int toy, const double *params, const double *norms);
+gf_status gf_batch_reduce_points(gf_batch *b, int metric, double *out);
+gf_status gf_batch_clear_points(gf_batch *b);
+
 /* ---- timing ------------------------------------------------------------- */
 
 /* Run gf_reduce_metric `reps` times back to back on the context stream and

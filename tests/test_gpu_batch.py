@@ -94,3 +94,196 @@ def test_batch_fitter_matches_sequential_fits():
         free = se > 0
         shift = np.abs(np.array(vals[t]) - sv)[free] / se[free]
         assert shift.max() < 0.01, (t, shift)
+
+
+def _add1d_point(rng):
+    """Parameter and normalisation arrays of the AddPdf model at a random point, and the model."""
+    mu, sig = 0.3 + 0.1 * rng.normal(), 0.8 * (1 + 0.1 * rng.random())
+    tt, _ = add1d(mu=mu, sigma=sig, alpha=-0.4 + 0.05 * rng.normal(), frac=0.3 + 0.05 * rng.random())
+    return tt
+
+
+def test_batch_points_equal_single_point_launches(ctx):
+    """gf_batch_reduce_points on the interpreter path: every point's sum equals
+    gf_batch_reduce's for the same arrays bit for bit (0, 1, 3, 5, 17 and 64 points per
+    toy: one to four point groups), the first points of each toy agree with the oracle,
+    and toys without points keep their output entries."""
+    counts = [3000, 0, 777, 12_345, 64, 5000]
+    npts = [1, 3, 0, 17, 64, 5]
+    data = _toys(counts, 41)
+    first = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    top, _ = add1d()
+    f, p, c, o, n = O.flatten(top).arrays()
+    ctx.set_program(f, p, c, o, n)
+    ctx.upload_columns([np.concatenate(data)])
+    b = capi.Batch(ctx, first, counts)
+    rng = np.random.default_rng(8)
+    try:
+        pts = {}
+        for t in range(len(counts)):
+            for k in range(npts[t]):
+                m = O.Model(_add1d_point(rng), data[t].reshape(-1, 1) if counts[t] else np.zeros((1, 1)))
+                m.normalize()
+                m.op.sync_params()
+                m.op.sync_norms()
+                ref = None
+                if k < 2 and counts[t]:
+                    ref = m.op.reduce_metric(data[t].reshape(-1, 1), "NLL")
+                pts[t, k] = (m.op.p.copy(), m.op.n.copy(), ref)
+                b.add_point(t, m.op.p, m.op.n)
+        got = b.reduce_points()
+        assert got.shape == (len(counts), capi.BATCH_MAX_POINTS)
+        for t in range(len(counts)):
+            assert np.isnan(got[t, npts[t]:]).all()
+        for (t, k), (_, _, ref) in pts.items():
+            if counts[t] == 0:
+                assert got[t, k] == 0.0
+            elif ref is not None:
+                assert math.isclose(got[t, k], ref, rel_tol=1e-11), (t, k, got[t, k], ref)
+        b.use_graph(False)
+        for k in range(max(npts)):
+            active = np.array([k < m for m in npts], np.uint8)
+            for t in range(len(counts)):
+                if active[t]:
+                    b.set_toy(t, pts[t, k][0], pts[t, k][1])
+            single = b.reduce(active)
+            for t in range(len(counts)):
+                if active[t]:
+                    assert got[t, k] == single[t], (t, k, got[t, k], single[t])
+        # a second request after the first: the points were cleared
+        b.add_point(3, pts[3, 5][0], pts[3, 5][1])
+        again = b.reduce_points()
+        assert again[3, 0] == got[3, 5] and np.isnan(again[0]).all()
+    finally:
+        b.close()
+
+
+def test_batch_points_rejects_bad_requests(ctx):
+    top, _ = add1d()
+    f, p, c, o, n = O.flatten(top).arrays()
+    ctx.set_program(f, p, c, o, n)
+    ctx.upload_columns([W.sample_add1d(2000, seed=3)])
+    b = capi.Batch(ctx, [0, 1000], [1000, 1000])
+    try:
+        b.add_point(0, p, n)
+        b.add_point(1, p, n)
+        with pytest.raises(RuntimeError, match="one after the other"):
+            b.add_point(0, p, n)
+        b.clear_points()
+        for _ in range(capi.BATCH_MAX_POINTS):
+            b.add_point(1, p, n)
+        with pytest.raises(RuntimeError, match="already has"):
+            b.add_point(1, p, n)
+        got = b.reduce_points()
+        assert np.isfinite(got[1]).all() and np.isnan(got[0]).all()
+        assert np.all(got[1] == got[1, 0])
+    finally:
+        b.close()
+
+
+def _dalitz_batch_setup(ctx, n, eff=None, masses=None, nbins=64, seed=5):
+    from _models import arrays, dalitz, dalitz_events
+    top, _ = dalitz(nbins=nbins, eff=eff)
+    if masses is not None:  # the efficiency's observables are the Dalitz PDF's
+        top.m12, top.m13 = masses
+        top.observables[0], top.observables[1] = masses
+    aos = dalitz_events(n, seed)
+    M = O.Model(top, aos, decay=W.DALITZ_DECAY)
+    _, (f, p, c, o, nn) = arrays(top)
+    ctx.set_program(f, p, c, o, nn)
+    ctx.set_decay(W.DALITZ_DECAY)
+    ctx.upload_columns([aos[:, 0], aos[:, 1], aos[:, 2]])
+    ctx.set_identity_column(2)
+    ctx.dalitz_set_data_size(top.function_idx, n)
+    ref = M.calculate_nll()
+    M.op.sync_params()
+    M.op.sync_norms()
+    ctx.update_params(M.op.p)
+    ctx.update_norms(M.op.n)
+    ctx.dalitz_cache(top.function_idx)
+    return top, M, ref
+
+
+def _coupling_sets(top, M, k, rng):
+    sets = []
+    for _ in range(k):
+        p = M.op.p.copy()
+        for i in range(len(top.resonances)):
+            p[top.parameters_idx + 1 + 2 * i] *= 1 + 1e-3 * rng.standard_normal()
+            p[top.parameters_idx + 2 + 2 * i] *= 1 + 1e-3 * rng.standard_normal()
+        nn = M.op.n.copy()
+        nn[top.normal_idx + 1] *= 1 + 1e-4 * rng.standard_normal()
+        sets.append((p, nn))
+    return sets
+
+
+@pytest.mark.parametrize("const_eff", [True, False])
+def test_batch_points_dalitz(ctx, const_eff):
+    """The coherent-sum form of gf_batch_reduce_points (waves read once per 16 points):
+    a toy over all rows reproduces the oracle's NLL at its first point; every point agrees
+    with the interpreter's gf_batch_reduce within 1e-12 relative; with a constant
+    efficiency the kernel that skips m12/m13 gives the same bits as the one that reads them."""
+    n = 40_000
+    eff = masses = None
+    if not const_eff:
+        masses = (O.Obs("m12", 0.0, 3.0, 64), O.Obs("m13", 0.0, 3.0, 64))
+        eff = O.MultiPolynomial("eff", list(masses), [O.Var("k0", 0.8), O.Var("k1", 0.1), O.Var("k2", -0.05)],
+                                [O.Var("o0", 0.5), O.Var("o1", 0.2)], 1)
+    top, M, ref = _dalitz_batch_setup(ctx, n, eff=eff, masses=masses)
+    first, counts = [0, 0, 10_000, 33_333], [n, 10_000, 23_333, n - 33_333]
+    npts = [20, 1, 16, 40]
+    rng = np.random.default_rng(12)
+    sets = {t: _coupling_sets(top, M, npts[t], rng) for t in range(4)}
+    sets[0][0] = (M.op.p.copy(), M.op.n.copy())
+    b = capi.Batch(ctx, first, counts)
+    try:
+        def request():
+            for t in range(4):
+                for p, nn in sets[t]:
+                    b.add_point(t, p, nn)
+            return b.reduce_points()
+        got = request()
+        assert math.isclose(got[0, 0], ref, rel_tol=1e-11), (got[0, 0], ref)
+        b.use_graph(False)
+        for k in range(max(npts)):
+            active = np.array([k < m for m in npts], np.uint8)
+            for t in range(4):
+                if active[t]:
+                    b.set_toy(t, *sets[t][k])
+            single = b.reduce(active)
+            for t in range(4):
+                if active[t]:
+                    assert math.isclose(got[t, k], single[t], rel_tol=1e-12), (t, k, got[t, k], single[t])
+        if const_eff:
+            ctx.set_tuning(capi.TUNE_ZERO_OUTSIDE, 0)
+            try:
+                plain = request()
+            finally:
+                ctx.set_tuning(capi.TUNE_ZERO_OUTSIDE, 1)
+            np.testing.assert_array_equal(plain, got)
+    finally:
+        b.close()
+
+
+def test_batch_fitter_multi_point_equals_single_point():
+    """BatchFitter with derivative rounds in one request against one point per request:
+    the same function values (interpreter path), hence the same fits and call counts."""
+    counts = [4000, 9000, 6000, 15_000, 7000, 5000]
+    data = _toys(counts, 90)
+    m = W.build_add1d(start=1.1)
+    d = G.UnbinnedDataSet(m.observables)
+    d.from_columns([np.concatenate(data)])
+    m.pdf.setData(d)
+    first = list(np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(int))
+    out = {}
+    for multi in (True, False):
+        bf = G.BatchFitter(m.pdf, first, counts)
+        bf.useMultiPoint(multi)
+        assert bf.multiPoint() == multi
+        bf.fit(threads=3, autotune=False)
+        out[multi] = (bf.values(), bf.nfcn(), bf.valid(), bf.stats())
+    assert out[True][0] == out[False][0]
+    assert out[True][1] == out[False][1]
+    assert all(out[True][2])
+    assert out[True][3]["fcn_calls"] == out[False][3]["fcn_calls"]
+    assert out[True][3]["launches"] < out[False][3]["launches"]
