@@ -123,7 +123,7 @@ struct gf_ctx {
     GfResult *h_res          = nullptr;  // host-mapped result block written by the last block
     GfResult *d_res_mapped   = nullptr;  // its device address
     double *h_result         = nullptr;  // pinned staging for small copies
-    int32_t *d_err           = nullptr;  // device error word (bit 0: cache index out of range)
+    int32_t *d_err           = nullptr;  // device error word (bit 0: cache index out of range, bit 1: node kind not compiled in)
     bool fast_dalitz         = false;    // program is [DALITZ, MULTI_POLY, DALITZ_END, END]
     double *d_gram_partials  = nullptr;
     size_t cap_gram_partials = 0;
@@ -748,6 +748,7 @@ GfProgArgs prog_args(gf_ctx *ctx) {
     a.total_events = (double)ctx->n;
     a.decay        = ctx->decay;
     a.td           = ctx->has_td ? 1 : 0;
+    a.has_dz       = ctx->dalitz.empty() ? 0 : 1;
     a.err          = ctx->d_err;
     a.seq          = ctx->res_seq;
     a.mark         = nullptr;
@@ -821,6 +822,9 @@ gf_status read_result(gf_ctx *ctx, double *sum, double *comp) {
     r.sum  = h->sum;
     r.comp = h->comp;
     r.err  = h->err;
+    if(r.err & 2)
+        return fail(GF_ERR_STATE, "internal: a Dalitz node reached a kernel compiled without it (error word %d)",
+                    r.err);
     if(r.err)
         return fail(GF_ERR_INVALID, "Dalitz event number outside the wave cache (error word %d)", r.err);
     if(sum)
@@ -1569,7 +1573,9 @@ gf_status gf_eval_metric(gf_ctx *ctx, int metric, double *out, int64_t n) {
             st = fail(GF_ERR_HIP, "copy of the evaluated events failed");
         else if(err) {
             (void)hipMemset(ctx->d_err, 0, sizeof(int32_t));
-            st = fail(GF_ERR_INVALID, "Dalitz event number outside the wave cache (error word %d)", err);
+            st = (err & 2) ? fail(GF_ERR_STATE, "internal: a Dalitz node reached a kernel compiled without it "
+                                                "(error word %d)", err)
+                           : fail(GF_ERR_INVALID, "Dalitz event number outside the wave cache (error word %d)", err);
         }
     }
     (void)hipStreamSynchronize(ctx->stream);

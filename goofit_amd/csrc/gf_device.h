@@ -12,6 +12,23 @@
 
 namespace gfd {
 
+// Wave-uniform read-only data (the program, parameter / normalisation /
+// constant arrays, descriptor tables) is read through the constant address
+// space: a uniform index then compiles to scalar loads through the scalar
+// cache instead of per-lane vector loads followed by lane reads.  The data must
+// not change while a kernel runs (it never does: it is uploaded between launches).
+// The qualifier exists only in the device pass (the host pass has no address
+// spaces and only parses these functions).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GF_CONST __attribute__((address_space(4)))
+#else
+#define GF_CONST
+#endif
+template <class T>
+__device__ __forceinline__ const GF_CONST T *as_const(const T *p) {
+    return (const GF_CONST T *)p;
+}
+
 // ---------------------------------------------------------------- leaf PDFs
 
 // device_Gaussian, src/PDFs/basic/GaussianPdf.cu:7-17
@@ -399,7 +416,7 @@ __device__ __forceinline__ void tg_gaussian(double &P1, double &P2, double &P3, 
 struct TimeTerms {
     double P1, P2, P3, P4;
 };
-__device__ __forceinline__ TimeTerms threegauss_terms(const double *__restrict__ R, double tau, double dtime,
+__device__ __forceinline__ TimeTerms threegauss_terms(const GF_CONST double *__restrict__ R, double tau, double dtime,
                                                       double xmixing, double ymixing, double sigma) {
     const double fc = R[0], ft = R[1], fo = 1 - fc - ft;
     const double bc = R[2] + R[9], bt = R[4] + R[11], bo = R[6] + R[11];
@@ -428,7 +445,7 @@ __device__ __forceinline__ double threegauss_rate(const TimeTerms &P, double cos
 }
 
 // The resolution function of a time-dependent node, dispatched on its wave-uniform kind.
-__device__ __forceinline__ double time_resolution(int kind, const double *__restrict__ R, double t1, double t2,
+__device__ __forceinline__ double time_resolution(int kind, const GF_CONST double *__restrict__ R, double t1, double t2,
                                                   double abr, double abi, double tau, double dtime, double x,
                                                   double y, double sigma) {
     if(kind == GT_THREEGAUSS)

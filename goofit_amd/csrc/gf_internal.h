@@ -111,7 +111,8 @@ struct GfProgArgs {
     double total_events;         // c_totalEvents
     GfDecay decay;
     int32_t td;                  // 1: the program holds a time-dependent Dalitz node
-    int32_t *err;                // device error word (bit 0: cache index out of range)
+    int32_t has_dz;              // 1: the program holds a Dalitz node (either kind)
+    int32_t *err;                // device error word (bit 0: cache index out of range, bit 1: node kind not compiled in)
     uint32_t seq;                // sequence number k_finish stamps on its GfResult
     void *mark;                  // host only: hipEvent_t recorded between a sweep and its k_finish
 };

@@ -31,7 +31,7 @@ template <int EPT>
 struct ColSrc {
     const double *const *cols;
     int64_t idx[EPT];
-    __device__ __forceinline__ double get(int a, int j) const { return cols[a][idx[j]]; }
+    __device__ __forceinline__ double get(int a, int j) const { return gfd::as_const(cols)[a][idx[j]]; }
 };
 
 // Synthetic events held in registers (grid points); slot chosen by a
@@ -84,12 +84,14 @@ __device__ __forceinline__ double (&level(Acc<EPT> &a))[EPT] {
 
 // ------------------------------------------------------------ interpreter
 
-// One combining step on the accumulator of level L.  TD: the program may hold a
-// time-dependent Dalitz node (its code is compiled only into the kernels that
-// need it, keeping the others' register budget).
-template <int L, int EPT, bool TD, class Src>
-__device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs &A, const double *__restrict__ P,
-                                     const double *__restrict__ Nm, const Src &src, const int64_t (&row)[EPT],
+// One combining step on the accumulator of level L.  FEAT: the node kinds compiled
+// in -- 0 none of the Dalitz nodes, 1 the time-integrated Dalitz node, 2 also the
+// time-dependent one -- so that each kernel carries only the register budget of
+// the programs it runs (a Dalitz node met by a kernel without it sets bit 1 of
+// the error word instead of being skipped silently).
+template <int L, int EPT, int FEAT, class Src>
+__device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs &A, const GF_CONST double *__restrict__ P,
+                                     const GF_CONST double *__restrict__ Nm, const Src &src, const int64_t (&row)[EPT],
                                      const bool (&valid)[EPT], Acc<EPT> &acc, double (&T)[EPT]) {
     double(&a)[EPT] = level<L>(acc);
     switch(kind) {
@@ -144,8 +146,12 @@ __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs
             T[j] = a[j];
         break;
     case GK_DALITZ: {
+        if constexpr(FEAT < 1) {
+            atomicOr(A.err, 2);
+            break;
+        }
         // device_DalitzPlot, src/PDFs/physics/Amp3Body.cu:60-122
-        const GfDalitzSlot dz = A.dz[in.x];
+        const GfDalitzSlot dz = gfd::as_const(A.dz)[in.x];
         const GfDecay &D      = A.decay;
         int64_t en[EPT];
         bool inside[EPT];
@@ -202,10 +208,10 @@ __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs
             T[j] = (a[j] == -1.0) ? 0.0 : a[j] * T[j];
         break;
     case GK_TDDP:
-        if constexpr(TD) {
+        if constexpr(FEAT >= 2) {
             // device_Tddp, src/PDFs/physics/Amp3Body_TD.cu:41-257.  in.p: tau (then x, y,
             // dx, dy and the couplings); in.a: resonance count; in.x: node slot.
-            const GfDalitzSlot dz = A.dz[in.x];
+            const GfDalitzSlot dz = gfd::as_const(A.dz)[in.x];
             const GfDecay &D      = A.decay;
             int64_t en[EPT];
             double sar[EPT], sai[EPT], sbr[EPT], sbi[EPT];
@@ -254,7 +260,7 @@ __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs
                 }
             }
             const double tau = P[in.p], x0 = P[in.p + 1], y0 = P[in.p + 2], dx = P[in.p + 3], dy = P[in.p + 4];
-            const double *__restrict__ R = P + dz.res_p;
+            const GF_CONST double *__restrict__ R = P + dz.res_p;
             // mistag terms: A <-> B with fixed lifetime and mixing (Amp3Body_TD.cu:226-248)
             const double xfix = 0.0039, yfix = 0.0065, taufix = 0.4101;
 #pragma unroll
@@ -285,12 +291,14 @@ __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs
                 }
                 a[j] = ret;
             }
+        } else {
+            atomicOr(A.err, 2);
         }
         break;
     case GK_TDDP_END:
-        if constexpr(TD) {
+        if constexpr(FEAT >= 2) {
             // Amp3Body_TD.cu:61-76, 253-255: ret *= eff inside, 0 outside
-            const GfDalitzSlot dz = A.dz[in.x];
+            const GfDalitzSlot dz = gfd::as_const(A.dz)[in.x];
             const GfDecay &D      = A.decay;
 #pragma unroll
             for(int j = 0; j < EPT; ++j) {
@@ -298,6 +306,8 @@ __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs
                                                    D.daug1Mass, D.daug2Mass, D.daug3Mass);
                 T[j] = inside ? a[j] * T[j] : 0.0;
             }
+        } else {
+            atomicOr(A.err, 2);
         }
         break;
     default: break;
@@ -308,11 +318,14 @@ __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs
 // Run the compiled program from instruction pc0 for EPT events per lane; the
 // value of the tree ends up in T.  row[] are the event rows (used as wave-cache
 // index when the event-number column is the identity), valid[] masks tail lanes.
-template <int EPT, bool TD = false, class Src>
-__device__ __forceinline__ void run(const GfProgArgs &A, const double *__restrict__ P, const double *__restrict__ Nm,
+template <int EPT, int FEAT = 1, class Src>
+__device__ __forceinline__ void run(const GfProgArgs &A, const double *__restrict__ P_, const double *__restrict__ Nm_,
                                     int pc0, const Src &src, const int64_t (&row)[EPT], const bool (&valid)[EPT],
                                     double (&T)[EPT]) {
-    const double *__restrict__ Cn = A.consts;
+    const GF_CONST double *__restrict__ P  = gfd::as_const(P_);
+    const GF_CONST double *__restrict__ Nm = gfd::as_const(Nm_);
+    const GF_CONST double *__restrict__ Cn = gfd::as_const(A.consts);
+    const GF_CONST GfIns *__restrict__ prog = gfd::as_const(A.ins);
     Acc<EPT> acc;
 #pragma unroll
     for(int j = 0; j < EPT; ++j) {
@@ -320,7 +333,7 @@ __device__ __forceinline__ void run(const GfProgArgs &A, const double *__restric
         T[j]                                          = 0.0;
     }
     for(int pc = pc0;; ++pc) {
-        const GfIns in = A.ins[pc];
+        const GfIns in = prog[pc];
         switch(in.op) {
         case GI_END: return;
         case GI_GAUSS: {
@@ -405,7 +418,7 @@ __device__ __forceinline__ void run(const GfProgArgs &A, const double *__restric
                     double term = 1;
                     int ci = i, sum = 0;
                     for(int o = 0; o < nobs; ++o) {
-                        double x = src.get(A.obsid[in.x + o], j);
+                        double x = src.get(gfd::as_const(A.obsid)[in.x + o], j);
                         x -= P[in.p + o];
                         int pw = ci % maxDegree;
                         ci /= maxDegree;
@@ -425,10 +438,10 @@ __device__ __forceinline__ void run(const GfProgArgs &A, const double *__restric
             if(in.op >= GI_LVL) {
                 const int kind = (in.op - GI_LVL) >> 2;
                 switch((in.op - GI_LVL) & 3) {
-                case 0: step<0, EPT, TD>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
-                case 1: step<1, EPT, TD>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
-                case 2: step<2, EPT, TD>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
-                default: step<3, EPT, TD>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
+                case 0: step<0, EPT, FEAT>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
+                case 1: step<1, EPT, FEAT>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
+                case 2: step<2, EPT, FEAT>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
+                default: step<3, EPT, FEAT>(kind, in, A, P, Nm, src, row, valid, acc, T); break;
                 }
                 break;
             }
@@ -500,7 +513,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_finish(const Partial *partials, in
 
 // ------------------------------------------------------------- event kernels
 
-template <int EPT, bool INL, bool TD = false>
+template <int EPT, bool INL, int FEAT>
 __global__ __launch_bounds__(GF_BLOCK) void k_reduce_metric(GfProgArgs A, GfInline I, int metric, Partial *partials) {
     const double *__restrict__ P  = INL ? I.p : A.params;
     const double *__restrict__ Nm = INL ? I.n : A.norms;
@@ -522,7 +535,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_reduce_metric(GfProgArgs A, GfInli
             src.idx[j] = row[j];
         }
         double T[EPT];
-        run<EPT, TD>(A, P, Nm, 0, src, row, valid, T);
+        run<EPT, FEAT>(A, P, Nm, 0, src, row, valid, T);
 #pragma unroll
         for(int j = 0; j < EPT; ++j) {
             double e0 = 0.0, e1 = 0.0;
@@ -538,7 +551,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_reduce_metric(GfProgArgs A, GfInli
     store_partial(s, c, partials);
 }
 
-template <int EPT, bool INL, bool TD = false>
+template <int EPT, bool INL, int FEAT>
 __global__ __launch_bounds__(GF_BLOCK) void k_eval_metric(GfProgArgs A, GfInline I, int metric, double *out) {
     const double *__restrict__ P  = INL ? I.p : A.params;
     const double *__restrict__ Nm = INL ? I.n : A.norms;
@@ -559,7 +572,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_eval_metric(GfProgArgs A, GfInline
             src.idx[j] = row[j];
         }
         double T[EPT];
-        run<EPT, TD>(A, P, Nm, 0, src, row, valid, T);
+        run<EPT, FEAT>(A, P, Nm, 0, src, row, valid, T);
 #pragma unroll
         for(int j = 0; j < EPT; ++j) {
             double e0 = 0.0, e1 = 0.0;
@@ -578,10 +591,10 @@ __global__ __launch_bounds__(GF_BLOCK) void k_eval_metric(GfProgArgs A, GfInline
 
 // Multivariate polynomial of one event (device_MultiPolynomial,
 // src/PDFs/basic/PolynomialPdf.cu:47-118), columns read at row e.
-__device__ __forceinline__ double multipoly_at(const GfIns &in, const GfProgArgs &A, const double *__restrict__ P,
-                                               int64_t e) {
+__device__ __forceinline__ double multipoly_at(const GfIns &in, const GfProgArgs &A,
+                                               const GF_CONST double *__restrict__ P, int64_t e) {
     const int nobs      = in.a;
-    const int maxDegree = (int)A.consts[in.c] + 1;
+    const int maxDegree = (int)gfd::as_const(A.consts)[in.c] + 1;
     int numBoxes        = 1;
     for(int i = 0; i < nobs; ++i)
         numBoxes *= maxDegree;
@@ -592,7 +605,7 @@ __device__ __forceinline__ double multipoly_at(const GfIns &in, const GfProgArgs
         double term = 1;
         int ci = i, sum = 0;
         for(int o = 0; o < nobs; ++o) {
-            double x = A.cols[A.obsid[in.x + o]][e];
+            double x = gfd::as_const(A.cols)[gfd::as_const(A.obsid)[in.x + o]][e];
             x -= P[in.p + o];
             int pw = ci % maxDegree;
             ci /= maxDegree;
@@ -651,7 +664,7 @@ __global__ __launch_bounds__(256) void k_dalitz_fast(GfProgArgs A, GfInline I, i
         const bool inside =
             ZERO_OUT || gfd::in_dalitz(m12, m13, D.motherMass, D.daug1Mass, D.daug2Mass, D.daug3Mass);
         // a degree-0 polynomial is its constant coefficient (numBoxes == 1)
-        const double eff = CONST_EFF ? P[effi.p + effi.a] : multipoly_at(effi, A, P, e);
+        const double eff = CONST_EFF ? P[effi.p + effi.a] : multipoly_at(effi, A, gfd::as_const(P), e);
         const double T   = inside ? (re * re + im * im) * eff : 0.0;
         const double v   = gfd::metric(metric, T, 0.0, 0.0, norm, A.total_events);
         if(WRITE) {
@@ -678,7 +691,7 @@ __global__ __launch_bounds__(256) void k_dalitz_multi(GfProgArgs A, const double
                                                       Partial *partials) {
     const GfIns dzi       = A.ins[0];
     const GfIns effi      = A.ins[1];
-    const GfDalitzSlot dz = A.dz[dzi.x];
+    const GfDalitzSlot dz = gfd::as_const(A.dz)[dzi.x];
     const GfDecay &D      = A.decay;
     const int nres        = dzi.a;
     const int64_t n       = A.nevents;
@@ -703,8 +716,8 @@ __global__ __launch_bounds__(256) void k_dalitz_multi(GfProgArgs A, const double
 #pragma unroll
         for(int p = 0; p < GF_MULTI_PTS; ++p) {
             if(p < npts) {
-                const double *__restrict__ P = params + (int64_t)p * np;
-                const double norm            = norms[(int64_t)p * nn + 1];
+                const GF_CONST double *__restrict__ P = gfd::as_const(params + (int64_t)p * np);
+                const double norm                     = gfd::as_const(norms)[(int64_t)p * nn + 1];
                 double re = 0.0, im = 0.0;
 #pragma unroll
                 for(int i = 0; i < GF_MAX_RES; ++i) {
@@ -939,11 +952,11 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_dalitz_points(GfProgArgs A, 
         return;  // uniform over the workgroup
     const GfIns dzi       = A.ins[0];
     const GfIns effi      = A.ins[1];
-    const GfDalitzSlot dz = A.dz[dzi.x];
+    const GfDalitzSlot dz = gfd::as_const(A.dz)[dzi.x];
     const GfDecay &D      = A.decay;
     const int nres        = dzi.a;
     const int W           = np + nn;
-    const double *__restrict__ S0  = stage + (int64_t)(slot0 + p0) * W;
+    const GF_CONST double *__restrict__ S0 = gfd::as_const(stage + (int64_t)(slot0 + p0) * W);
     const double *__restrict__ c12 = A.cols[dz.col_m12];
     const double *__restrict__ c13 = A.cols[dz.col_m13];
     double s[GF_MULTI_PTS], c[GF_MULTI_PTS];
@@ -965,7 +978,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_dalitz_points(GfProgArgs A, 
 #pragma unroll
         for(int p = 0; p < GF_MULTI_PTS; ++p) {
             if(p < cnt) {
-                const double *__restrict__ P = S0 + (int64_t)p * W;
+                const GF_CONST double *__restrict__ P = S0 + (int64_t)p * W;
                 const double norm            = P[np + 1];
                 double re = 0.0, im = 0.0;
 #pragma unroll
@@ -1351,39 +1364,39 @@ static int launch_finish(const GfProgArgs &a, Partial *partials, int nblocks, Gf
     return check(hipGetLastError());
 }
 
+// Node kinds a program needs compiled in (FEAT of the interpreter kernels).
+static int feature_level(const GfProgArgs &a) { return a.td ? 2 : (a.has_dz ? 1 : 0); }
+
 int launch_reduce_metric(const GfProgArgs &a, const GfInline *inl, int metric, Partial *partials, GfResult *result,
                          int nblocks, void *stream) {
     hipStream_t s = (hipStream_t)stream;
-    if(a.td) {
-        // time-dependent nodes: the kernels compiled with the TD ops (two events per lane
-        // measured 1.5 % faster than one on the 16-resonance three-Gaussian workload)
-        if(inl)
-            hipLaunchKernelGGL((k_reduce_metric<2, true, true>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, *inl, metric,
-                               partials);
-        else
-            hipLaunchKernelGGL((k_reduce_metric<2, false, true>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, GfInline{},
-                               metric, partials);
-    } else if(inl)
-        hipLaunchKernelGGL((k_reduce_metric<2, true>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, *inl, metric, partials);
-    else
-        hipLaunchKernelGGL((k_reduce_metric<2, false>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, GfInline{}, metric,
-                           partials);
+    // time-dependent nodes: the kernels compiled with the TD ops (two events per lane
+    // measured 1.5 % faster than one on the 16-resonance three-Gaussian workload);
+    // programs without Dalitz nodes: the kernels compiled without them
+    const dim3 g(nblocks), b(GF_BLOCK);
+    const GfInline none{};
+    switch(feature_level(a) * 2 + (inl ? 1 : 0)) {
+    case 0: hipLaunchKernelGGL((k_reduce_metric<2, false, 0>), g, b, 0, s, a, none, metric, partials); break;
+    case 1: hipLaunchKernelGGL((k_reduce_metric<2, true, 0>), g, b, 0, s, a, *inl, metric, partials); break;
+    case 2: hipLaunchKernelGGL((k_reduce_metric<2, false, 1>), g, b, 0, s, a, none, metric, partials); break;
+    case 3: hipLaunchKernelGGL((k_reduce_metric<2, true, 1>), g, b, 0, s, a, *inl, metric, partials); break;
+    case 4: hipLaunchKernelGGL((k_reduce_metric<2, false, 2>), g, b, 0, s, a, none, metric, partials); break;
+    default: hipLaunchKernelGGL((k_reduce_metric<2, true, 2>), g, b, 0, s, a, *inl, metric, partials); break;
+    }
     return launch_finish(a, partials, nblocks, result, s);
 }
-
 int launch_eval_metric(const GfProgArgs &a, const GfInline *inl, int metric, double *out, int nblocks, void *stream) {
     hipStream_t s = (hipStream_t)stream;
-    if(a.td) {
-        if(inl)
-            hipLaunchKernelGGL((k_eval_metric<2, true, true>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, *inl, metric,
-                               out);
-        else
-            hipLaunchKernelGGL((k_eval_metric<2, false, true>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, GfInline{},
-                               metric, out);
-    } else if(inl)
-        hipLaunchKernelGGL((k_eval_metric<2, true>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, *inl, metric, out);
-    else
-        hipLaunchKernelGGL((k_eval_metric<2, false>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, GfInline{}, metric, out);
+    const dim3 g(nblocks), b(GF_BLOCK);
+    const GfInline none{};
+    switch(feature_level(a) * 2 + (inl ? 1 : 0)) {
+    case 0: hipLaunchKernelGGL((k_eval_metric<2, false, 0>), g, b, 0, s, a, none, metric, out); break;
+    case 1: hipLaunchKernelGGL((k_eval_metric<2, true, 0>), g, b, 0, s, a, *inl, metric, out); break;
+    case 2: hipLaunchKernelGGL((k_eval_metric<2, false, 1>), g, b, 0, s, a, none, metric, out); break;
+    case 3: hipLaunchKernelGGL((k_eval_metric<2, true, 1>), g, b, 0, s, a, *inl, metric, out); break;
+    case 4: hipLaunchKernelGGL((k_eval_metric<2, false, 2>), g, b, 0, s, a, none, metric, out); break;
+    default: hipLaunchKernelGGL((k_eval_metric<2, true, 2>), g, b, 0, s, a, *inl, metric, out); break;
+    }
     return check(hipGetLastError());
 }
 
