@@ -771,7 +771,7 @@ int event_blocks(gf_ctx *ctx, int64_t n) {
 gf_status check_events_for_program(gf_ctx *ctx, int metric) {
     if(ctx->ins.empty())
         return fail(GF_ERR_STATE, "no program installed (gf_program_set)");
-    if(!ctx->col_base || ctx->n <= 0)
+    if(!ctx->col_base)  // an upload of zero events is a valid empty set: every sum is 0
         return fail(GF_ERR_STATE, "no events uploaded");
     if(ctx->max_col >= ctx->ncols)
         return fail(GF_ERR_STATE, "program reads observable column %d but only %d columns are uploaded", ctx->max_col,

@@ -1,0 +1,182 @@
+"""Every launch variant of the likelihood sweep against the oracle: parameters in
+the kernel arguments or in device arrays, grid caps of 1, 4 and 16 workgroups per
+CU, the interpreter compiled per node feature level (trees without a Dalitz node,
+with the time-integrated one, with the time-dependent one), the coherent-sum Dalitz
+kernel against the interpreter; an empty event set; and the two abort conditions of
+GooPdf::calculateNLL (src/PDFs/GooPdf.cu:254-273).
+
+Tolerance: the NLL within 1e-11 relative of the oracle; variants that differ only in
+where the parameters are read give the same bits.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import goofit_amd as G
+import oracle as O
+from goofit_amd import capi
+from goofit_amd import workloads as W
+
+from _models import add1d, arrays, dalitz, dalitz_events, prod3d, tddp, tddp_events
+
+pytestmark = pytest.mark.gpu
+
+NLL_RTOL = 1e-11
+
+
+def _install(ctx, top, cols, decay=None):
+    prog, (f, p, c, o, n) = arrays(top)
+    ctx.set_program(f, p, c, o, n)
+    if decay is not None:
+        ctx.set_decay(decay)
+    ctx.upload_columns(cols)
+    return prog
+
+
+def _sync(ctx, M):
+    M.op.sync_params()
+    M.op.sync_norms()
+    ctx.update_params(M.op.p)
+    ctx.update_norms(M.op.n)
+
+
+def _variants(ctx, ref):
+    """NLL under every (inline, workgroups per CU) setting; the defaults restored after."""
+    out = {}
+    try:
+        for blocks in (1, 4, 16):
+            ctx.set_tuning(capi.TUNE_BLOCKS_PER_CU, blocks)
+            for inline in (1, 0):
+                ctx.set_tuning(capi.TUNE_INLINE_ARGS, inline)
+                got = ctx.reduce_metric("NLL")[0]
+                assert math.isclose(got, ref, rel_tol=NLL_RTOL), (blocks, inline, got, ref)
+                out[blocks, inline] = got
+            assert out[blocks, 1] == out[blocks, 0], blocks
+    finally:
+        ctx.set_tuning(capi.TUNE_BLOCKS_PER_CU, 4)
+        ctx.set_tuning(capi.TUNE_INLINE_ARGS, 1)
+    return out
+
+
+def test_variants_without_dalitz_nodes(ctx):
+    """AddPdf(Gauss, Exp) and ProdPdf(Gauss, Exp, ExpGaus): the kernels without Dalitz nodes."""
+    top, _ = add1d()
+    data = W.sample_add1d(50_000, seed=31)
+    M = O.Model(top, data.reshape(-1, 1))
+    ref = M.calculate_nll()
+    _install(ctx, top, [data])
+    _sync(ctx, M)
+    _variants(ctx, ref)
+    top3, _ = prod3d()
+    xs, ys, zs = W.sample_prod3d(100_000, seed=32)
+    M3 = O.Model(top3, np.stack([xs, ys, zs], axis=1))
+    ref3 = M3.calculate_nll()
+    _install(ctx, top3, [xs, ys, zs])
+    _sync(ctx, M3)
+    _variants(ctx, ref3)
+
+
+def test_variants_with_a_dalitz_node(ctx):
+    """A lone Amp3Body: the coherent-sum kernel and the interpreter (fast path off), each
+    under every inline / grid setting."""
+    n = 30_000
+    top, _ = dalitz(nbins=64)
+    aos = dalitz_events(n, 33)
+    M = O.Model(top, aos, decay=W.DALITZ_DECAY)
+    ref = M.calculate_nll()
+    _install(ctx, top, [aos[:, 0], aos[:, 1], aos[:, 2]], decay=W.DALITZ_DECAY)
+    ctx.set_identity_column(2)
+    ctx.dalitz_set_data_size(top.function_idx, n)
+    M.op.sync_params()
+    ctx.update_params(M.op.p)
+    ctx.dalitz_cache(top.function_idx)
+    _sync(ctx, M)
+    fast = _variants(ctx, ref)
+    try:
+        ctx.set_tuning(capi.TUNE_FAST_DALITZ, 0)
+        slow = _variants(ctx, ref)
+    finally:
+        ctx.set_tuning(capi.TUNE_FAST_DALITZ, 1)
+    assert math.isclose(fast[4, 1], slow[4, 1], rel_tol=1e-13)
+
+
+def test_variants_with_a_time_dependent_node(ctx):
+    n = 20_000
+    top, _ = tddp(nbins=80)
+    aos = tddp_events(n, seed=34)
+    M = O.Model(top, aos, decay=W.DALITZ_DECAY)
+    ref = M.calculate_nll()
+    prog, (f, p, c, o, nn) = arrays(top)
+    ctx.set_program(f, p, c, o, nn)
+    ctx.set_decay(W.DALITZ_DECAY)
+    ctx.upload_columns([np.ascontiguousarray(aos[:, k]) for k in range(aos.shape[1])])
+    ctx.set_identity_column(4)
+    ctx.dalitz_set_data_size(top.function_idx, n)
+    M.op.sync_params()
+    ctx.update_params(M.op.p)
+    ctx.dalitz_cache(top.function_idx)
+    _sync(ctx, M)
+    _variants(ctx, ref)
+
+
+def test_empty_event_set(ctx):
+    """No events: every sum is 0 (the reference then aborts in calculateNLL, below)."""
+    top, _ = add1d()
+    M = O.Model(top, np.zeros((1, 1)))
+    M.normalize()
+    _install(ctx, top, [np.zeros(0)])
+    _sync(ctx, M)
+    got, comp = ctx.reduce_metric("NLL")
+    assert got == 0.0 and comp == 0.0
+    assert ctx.eval_metric("NLL").size == 0
+
+
+def _gauss_model():
+    x = G.Observable("x", -5.0, 5.0)
+    x.numbins = 100
+    mu, sigma = G.Variable("mu", 0.0, 0.1, -1, 1), G.Variable("sigma", 1.0, 0.1, 0.1, 3)
+    return x, mu, sigma, G.GaussianPdf("g", x, mu, sigma)
+
+
+def test_calculate_nll_aborts_on_zero_nll():
+    """GooPdf.cu:269-270: a likelihood sum of exactly 0 aborts -- here every event
+    contributes -log(pdf * norm) = 0 because pdf * norm = 1."""
+    x, mu, sigma, g = _gauss_model()
+    # a 1-bin flat polynomial over a unit-width range: pdf * norm = 1 at every event
+    y = G.Observable("y", 0.0, 1.0)
+    y.numbins = 1
+    c0 = G.Variable("c0", 1.0)
+    flat = G.PolynomialPdf("flat", y, [c0], 0)
+    d = G.UnbinnedDataSet([y])
+    d.from_columns([np.linspace(0.1, 0.9, 50)])
+    flat.setData(d)
+    with pytest.raises(G.GeneralError, match="zero NLL"):
+        flat.calculateNLL()
+    # no events at all: the empty sum is 0 as well (GooPdf::reduce_with_metric over an
+    # empty range)
+    empty = G.UnbinnedDataSet([x])
+    empty.from_columns([np.zeros(0)])
+    g.setData(empty)
+    with pytest.raises(G.GeneralError, match="zero NLL"):
+        g.calculateNLL()
+    # and the normal case on the same engine still works
+    dg = G.UnbinnedDataSet([x])
+    dg.from_columns([np.random.default_rng(1).normal(0, 1, 1000).clip(-4.9, 4.9)])
+    g.setData(dg)
+    assert g.calculateNLL() > 0
+
+
+def test_calculate_nll_aborts_on_non_positive_normalisation():
+    """A PDF whose grid integral is negative aborts before the sum: in
+    GooPdf::normalize (GooPdf.cu:383-384) when the integral is numeric, as here, and
+    otherwise in calculateNLL (GooPdf.cu:260-261)."""
+    y = G.Observable("y", 0.0, 2.0)
+    y.numbins = 50
+    c0 = G.Variable("c0", -1.0)
+    neg = G.PolynomialPdf("neg", y, [c0], 0)
+    d = G.UnbinnedDataSet([y])
+    d.from_columns([np.linspace(0.1, 1.9, 20)])
+    neg.setData(d)
+    with pytest.raises(G.GeneralError, match="(?i)non-positive normalization"):
+        neg.calculateNLL()
