@@ -1516,7 +1516,7 @@ gf_status gf_reduce_metric_multi(gf_ctx *ctx, int metric, int npts, const double
     const int nb  = fast_blocks(ctx, ctx->n);
     for(int k0 = 0; k0 < npts; k0 += GF_MULTI_PTS) {
         const int cnt = std::min(GF_MULTI_PTS, npts - k0);
-        int rc = gfk::launch_dalitz_multi(a, ctx->d_multi + (size_t)k0 * np, np,
+        int rc = gfk::launch_dalitz_multi(a, ctx->dalitz[0].nres, ctx->d_multi + (size_t)k0 * np, np,
                                           ctx->d_multi + (size_t)npts * np + (size_t)k0 * nn, nn, cnt, ce, zo, metric,
                                           ctx->d_multi_partials, ctx->d_multi_out + 2 * (size_t)k0, nb, ctx->stream);
         if(rc)
@@ -3038,7 +3038,8 @@ gf_status gf_batch_reduce_points(gf_batch *b, int metric, double *out) {
                           hipMemcpyHostToDevice, ctx->stream));
     const int groups = (b->pts_max + GF_MULTI_PTS - 1) / GF_MULTI_PTS;
     GfProgArgs a     = prog_args(ctx);
-    if(gfk::launch_batch_points(a, b->d_work, b->nblocks, groups, b->d_pts + b->ntoys, b->np, b->nn,
+    const int nres = fast ? ctx->dalitz[0].nres : 0;
+    if(gfk::launch_batch_points(a, nres, b->d_work, b->nblocks, groups, b->d_pts + b->ntoys, b->np, b->nn,
                                 (const int *)b->d_pts, fast, ce, zo, metric, b->d_pts_partials, b->d_first,
                                 b->ntoys, b->d_pts_res, b->d_counter, b->d_stamp, ctx->stream))
         return fail(GF_ERR_HIP, "batch launch failed");

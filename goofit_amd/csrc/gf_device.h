@@ -29,6 +29,14 @@ __device__ __forceinline__ const GF_CONST T *as_const(const T *p) {
     return (const GF_CONST T *)p;
 }
 
+// (re, im) += (cr + i ci) * (w.x + i w.y) as four fused multiply-adds (the code is
+// built with contraction off, so the coherent sums ask for the fusion explicitly:
+// half the instructions of separate products and sums, one rounding per step).
+__device__ __forceinline__ void cmac(double &re, double &im, double cr, double ci, double2 w) {
+    re = fma(cr, w.x, fma(-ci, w.y, re));
+    im = fma(cr, w.y, fma(ci, w.x, im));
+}
+
 // ---------------------------------------------------------------- leaf PDFs
 
 // device_Gaussian, src/PDFs/basic/GaussianPdf.cu:7-17

@@ -166,9 +166,9 @@ int launch_eval_metric(const GfProgArgs &a, const GfInline *inl, int metric, dou
 // norms: npts x nn, device) over the same events in one sweep; out[2p], out[2p+1] =
 // sum and residual of set p (device).  partials: npts x nblocks.
 #define GF_MULTI_PTS 16
-int launch_dalitz_multi(const GfProgArgs &a, const double *params, int np, const double *norms, int nn, int npts,
-                        bool const_eff, bool zero_outside, int metric, Partial *partials, double *out, int nblocks,
-                        void *stream);
+int launch_dalitz_multi(const GfProgArgs &a, int nres, const double *params, int np, const double *norms, int nn,
+                        int npts, bool const_eff, bool zero_outside, int metric, Partial *partials, double *out,
+                        int nblocks, void *stream);
 // zero_outside (with const_eff): the wave cache was filled from these events, so m12/m13
 // are not read (the waves are 0 outside the Dalitz region).
 int launch_dalitz_fast(const GfProgArgs &a, const GfInline *inl, bool const_eff, bool zero_outside, int metric,
@@ -186,10 +186,10 @@ int launch_batch(const GfProgArgs &a, const GfBatchWork *work, int nblocks, cons
 // result[2 (t GF_BATCH_PTS + p) ..] (host-mapped), then the stamp.
 // fast: the coherent-sum program of launch_dalitz_fast.
 #define GF_BATCH_PTS 64
-int launch_batch_points(const GfProgArgs &a, const GfBatchWork *work, int nblocks, int groups, const double *stage,
-                        int np, int nn, const int *tbl, bool fast, bool const_eff, bool zero_outside, int metric,
-                        Partial *partials, const int *first, int ntoys, double *result, uint32_t *counter,
-                        uint32_t *stamp, void *stream);
+int launch_batch_points(const GfProgArgs &a, int nres, const GfBatchWork *work, int nblocks, int groups,
+                        const double *stage, int np, int nn, const int *tbl, bool fast, bool const_eff,
+                        bool zero_outside, int metric, Partial *partials, const int *first, int ntoys, double *result,
+                        uint32_t *counter, uint32_t *stamp, void *stream);
 // Accept-reject generation: candidates (uniform in box) with accept flags, and
 // the ordered compaction of the accepted ones.
 // eff_pc >= 0: efficiency subprogram; -1: efficiency 1; -2: the constant eff_const.

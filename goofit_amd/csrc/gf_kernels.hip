@@ -191,8 +191,7 @@ __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs
 #pragma unroll
                     for(int j = 0; j < EPT; ++j) {
                         const double2 me = w[u][j];
-                        re[j] += ar * me.x - ai * me.y;
-                        im[j] += ar * me.y + ai * me.x;
+                        gfd::cmac(re[j], im[j], ar, ai, me);
                     }
                 }
             }
@@ -657,8 +656,7 @@ __global__ __launch_bounds__(256) void k_dalitz_fast(GfProgArgs A, GfInline I, i
         for(int i = 0; i < GF_MAX_RES; ++i) {
             if(i < nres) {
                 const double ar = P[dzi.p + 2 * i], ai = P[dzi.p + 2 * i + 1];
-                re += ar * w[i].x - ai * w[i].y;
-                im += ar * w[i].y + ai * w[i].x;
+                gfd::cmac(re, im, ar, ai, w[i]);
             }
         }
         const bool inside =
@@ -685,10 +683,52 @@ __global__ __launch_bounds__(256) void k_dalitz_fast(GfProgArgs A, GfInline I, i
 // kernel's for the same parameters bit for bit.  params / norms: npts
 // consecutive copies of the flattened arrays (np and nn doubles each).
 // partials: [npts][gridDim.x].
-template <bool CONST_EFF, bool ZERO_OUT>
+// The parameter sets of one multi-set sweep as the event loop reads them: the
+// couplings of set p at c[p][i] (zero for resonances i >= nres, so that a resonance
+// loop unrolled to NR >= nres adds exact zeros past nres), each set's constant
+// efficiency and normalisation.  Staged once per workgroup in LDS (4.3 KB): the
+// loop then reads them with broadcast LDS loads instead of one scalar load and
+// wait per coupling.
+struct MultiSets {
+    double2 c[GF_MULTI_PTS][GF_MAX_RES];
+    double eff[GF_MULTI_PTS], norm[GF_MULTI_PTS];
+};
+static_assert(GF_MULTI_PTS * GF_MAX_RES == 256, "one staging thread per (set, resonance)");
+
+// Fill S from cnt sets: set p's parameter array at P0 + p * pstride, its
+// normalisation array at N0 + p * nstride; couplings from index pc, the constant
+// efficiency (CONST_EFF) at index pe.  All 256 threads of the workgroup take part.
+template <bool CONST_EFF>
+__device__ __forceinline__ void stage_sets(MultiSets &S, const double *__restrict__ P0, int64_t pstride,
+                                           const double *__restrict__ N0, int64_t nstride, int cnt, int pc, int pe,
+                                           int nres) {
+    const int t = threadIdx.x, p = t >> 4, i = t & 15;
+    double2 v   = make_double2(0.0, 0.0);
+    if(p < cnt && i < nres) {
+        const GF_CONST double *__restrict__ P = gfd::as_const(P0 + p * pstride);
+        v                                     = make_double2(P[pc + 2 * i], P[pc + 2 * i + 1]);
+    }
+    S.c[p][i] = v;
+    if(t < GF_MULTI_PTS) {
+        S.eff[t]  = (CONST_EFF && t < cnt) ? gfd::as_const(P0 + t * pstride)[pe] : 0.0;
+        S.norm[t] = t < cnt ? gfd::as_const(N0 + t * nstride)[1] : 0.0;
+    }
+    __syncthreads();
+}
+
+// The coherent-sum metric of up to GF_MULTI_PTS parameter sets over the same
+// events in one sweep: an event's waves are loaded once and every set's term is
+// taken with the arithmetic, the event-to-lane assignment and the reduction
+// order of k_dalitz_fast on the same grid, so each set's partials equal that
+// kernel's for the same parameters bit for bit (the resonance loop runs to
+// NR >= nres with zero couplings and waves past nres, whose fused terms add
+// exact zeros).  params / norms: npts consecutive copies of the flattened arrays
+// (np and nn doubles each).  partials: [npts][gridDim.x].
+template <bool CONST_EFF, bool ZERO_OUT, int NR>
 __global__ __launch_bounds__(256) void k_dalitz_multi(GfProgArgs A, const double *__restrict__ params, int np,
                                                       const double *__restrict__ norms, int nn, int npts, int metric,
                                                       Partial *partials) {
+    __shared__ MultiSets S;
     const GfIns dzi       = A.ins[0];
     const GfIns effi      = A.ins[1];
     const GfDalitzSlot dz = gfd::as_const(A.dz)[dzi.x];
@@ -697,17 +737,21 @@ __global__ __launch_bounds__(256) void k_dalitz_multi(GfProgArgs A, const double
     const int64_t n       = A.nevents;
     const double *__restrict__ c12 = A.cols[dz.col_m12];
     const double *__restrict__ c13 = A.cols[dz.col_m13];
+    stage_sets<CONST_EFF>(S, params, np, norms, nn, npts, dzi.p, effi.p + effi.a, nres);
     double s[GF_MULTI_PTS], c[GF_MULTI_PTS];
 #pragma unroll
     for(int p = 0; p < GF_MULTI_PTS; ++p)
         s[p] = c[p] = 0.0;
     for(int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+        // keep the LDS reads of the sets inside the loop: hoisted, the 16 x NR
+        // loop-invariant couplings would not fit in registers
+        asm volatile("" ::: "memory");
         const int64_t e0 = base + threadIdx.x;
         const bool valid = e0 < n;
         const int64_t e  = valid ? e0 : n - 1;
-        double2 w[GF_MAX_RES];
+        double2 w[NR];
 #pragma unroll
-        for(int i = 0; i < GF_MAX_RES; ++i)
+        for(int i = 0; i < NR; ++i)
             w[i] = (i < nres) ? dz.wave[(int64_t)i * dz.ncache + e] : make_double2(0.0, 0.0);
         const double m12  = ZERO_OUT ? 0.0 : c12[e];
         const double m13  = ZERO_OUT ? 0.0 : c13[e];
@@ -716,20 +760,16 @@ __global__ __launch_bounds__(256) void k_dalitz_multi(GfProgArgs A, const double
 #pragma unroll
         for(int p = 0; p < GF_MULTI_PTS; ++p) {
             if(p < npts) {
-                const GF_CONST double *__restrict__ P = gfd::as_const(params + (int64_t)p * np);
-                const double norm                     = gfd::as_const(norms)[(int64_t)p * nn + 1];
                 double re = 0.0, im = 0.0;
 #pragma unroll
-                for(int i = 0; i < GF_MAX_RES; ++i) {
-                    if(i < nres) {
-                        const double ar = P[dzi.p + 2 * i], ai = P[dzi.p + 2 * i + 1];
-                        re += ar * w[i].x - ai * w[i].y;
-                        im += ar * w[i].y + ai * w[i].x;
-                    }
+                for(int i = 0; i < NR; ++i) {
+                    const double2 cp = S.c[p][i];
+                    gfd::cmac(re, im, cp.x, cp.y, w[i]);
                 }
-                const double eff = CONST_EFF ? P[effi.p + effi.a] : multipoly_at(effi, A, P, e);
-                const double T   = inside ? (re * re + im * im) * eff : 0.0;
-                const double v   = gfd::metric(metric, T, 0.0, 0.0, norm, A.total_events);
+                const double eff =
+                    CONST_EFF ? S.eff[p] : multipoly_at(effi, A, gfd::as_const(params + (int64_t)p * np), e);
+                const double T = inside ? (re * re + im * im) * eff : 0.0;
+                const double v = gfd::metric(metric, T, 0.0, 0.0, S.norm[p], A.total_events);
                 if(valid)
                     gfd::kb_add(s[p], c[p], v);
             }
@@ -938,11 +978,12 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_points(GfProgArgs A, const G
 // Several points per toy, coherent-sum program (as k_dalitz_multi): workgroup
 // (b, g) reads the waves of each row of work[b] once and adds the row's metric
 // for every point p in [16 g, min(16 g + 16, npts)) of its toy.
-template <bool CONST_EFF, bool ZERO_OUT>
+template <bool CONST_EFF, bool ZERO_OUT, int NR>
 __global__ __launch_bounds__(GF_BLOCK) void k_batch_dalitz_points(GfProgArgs A, const GfBatchWork *__restrict__ work,
                                                                   const double *__restrict__ stage, int np, int nn,
                                                                   const int *__restrict__ tbl, int metric,
                                                                   Partial *partials) {
+    __shared__ MultiSets S;
     const GfBatchWork w = work[blockIdx.x];
     const int slot0     = tbl[2 * w.toy];
     const int npts      = tbl[2 * w.toy + 1];
@@ -955,21 +996,23 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_dalitz_points(GfProgArgs A, 
     const GfDalitzSlot dz = gfd::as_const(A.dz)[dzi.x];
     const GfDecay &D      = A.decay;
     const int nres        = dzi.a;
-    const int W           = np + nn;
-    const GF_CONST double *__restrict__ S0 = gfd::as_const(stage + (int64_t)(slot0 + p0) * W);
+    const int64_t W       = np + nn;
+    const double *__restrict__ S0  = stage + (int64_t)(slot0 + p0) * W;
     const double *__restrict__ c12 = A.cols[dz.col_m12];
     const double *__restrict__ c13 = A.cols[dz.col_m13];
+    stage_sets<CONST_EFF>(S, S0, W, S0 + np, W, cnt, dzi.p, effi.p + effi.a, nres);
     double s[GF_MULTI_PTS], c[GF_MULTI_PTS];
 #pragma unroll
     for(int p = 0; p < GF_MULTI_PTS; ++p)
         s[p] = c[p] = 0.0;
     for(int64_t base = w.start; base < w.end; base += GF_BLOCK) {
+        asm volatile("" ::: "memory");  // the set reads stay in the loop (as k_dalitz_multi)
         const int64_t e0 = base + threadIdx.x;
         const bool valid = e0 < w.end;
         const int64_t e  = valid ? e0 : w.start;
-        double2 wv[GF_MAX_RES];
+        double2 wv[NR];
 #pragma unroll
-        for(int i = 0; i < GF_MAX_RES; ++i)
+        for(int i = 0; i < NR; ++i)
             wv[i] = (i < nres) ? dz.wave[(int64_t)i * dz.ncache + e] : make_double2(0.0, 0.0);
         const double m12  = ZERO_OUT ? 0.0 : c12[e];
         const double m13  = ZERO_OUT ? 0.0 : c13[e];
@@ -978,20 +1021,15 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_dalitz_points(GfProgArgs A, 
 #pragma unroll
         for(int p = 0; p < GF_MULTI_PTS; ++p) {
             if(p < cnt) {
-                const GF_CONST double *__restrict__ P = S0 + (int64_t)p * W;
-                const double norm            = P[np + 1];
                 double re = 0.0, im = 0.0;
 #pragma unroll
-                for(int i = 0; i < GF_MAX_RES; ++i) {
-                    if(i < nres) {
-                        const double ar = P[dzi.p + 2 * i], ai = P[dzi.p + 2 * i + 1];
-                        re += ar * wv[i].x - ai * wv[i].y;
-                        im += ar * wv[i].y + ai * wv[i].x;
-                    }
+                for(int i = 0; i < NR; ++i) {
+                    const double2 cp = S.c[p][i];
+                    gfd::cmac(re, im, cp.x, cp.y, wv[i]);
                 }
-                const double eff = CONST_EFF ? P[effi.p + effi.a] : multipoly_at(effi, A, P, e);
+                const double eff = CONST_EFF ? S.eff[p] : multipoly_at(effi, A, gfd::as_const(S0 + p * W), e);
                 const double T   = inside ? (re * re + im * im) * eff : 0.0;
-                const double v   = gfd::metric(metric, T, 0.0, 0.0, norm, w.total);
+                const double v   = gfd::metric(metric, T, 0.0, 0.0, S.norm[p], w.total);
                 if(valid)
                     gfd::kb_add(s[p], c[p], v);
             }
@@ -1407,21 +1445,36 @@ static void launch_fast_t(const GfProgArgs &a, const GfInline *inl, int metric, 
                        inl ? *inl : GfInline{}, metric, partials, out);
 }
 
-int launch_dalitz_multi(const GfProgArgs &a, const double *params, int np, const double *norms, int nn, int npts,
-                        bool const_eff, bool zero_outside, int metric, Partial *partials, double *out, int nblocks,
-                        void *stream) {
+// The multi-set kernels for nres resonances: the resonance loop unrolled to the
+// next multiple of 4 (NR), the variant by efficiency and zero-outside use.
+#define GF_MULTI_SWITCH(KERNEL, NRES, CE, ZO, ...)                                                                  \
+    do {                                                                                                             \
+        const int nr_ = (NRES) <= 4 ? 4 : (NRES) <= 8 ? 8 : (NRES) <= 12 ? 12 : 16;                                  \
+        const int v_  = (CE) ? ((ZO) ? 2 : 1) : 0;                                                                   \
+        switch(v_ * 4 + nr_ / 4 - 1) {                                                                               \
+        case 0: hipLaunchKernelGGL((KERNEL<false, false, 4>), __VA_ARGS__); break;                                  \
+        case 1: hipLaunchKernelGGL((KERNEL<false, false, 8>), __VA_ARGS__); break;                                  \
+        case 2: hipLaunchKernelGGL((KERNEL<false, false, 12>), __VA_ARGS__); break;                                 \
+        case 3: hipLaunchKernelGGL((KERNEL<false, false, 16>), __VA_ARGS__); break;                                 \
+        case 4: hipLaunchKernelGGL((KERNEL<true, false, 4>), __VA_ARGS__); break;                                   \
+        case 5: hipLaunchKernelGGL((KERNEL<true, false, 8>), __VA_ARGS__); break;                                   \
+        case 6: hipLaunchKernelGGL((KERNEL<true, false, 12>), __VA_ARGS__); break;                                  \
+        case 7: hipLaunchKernelGGL((KERNEL<true, false, 16>), __VA_ARGS__); break;                                  \
+        case 8: hipLaunchKernelGGL((KERNEL<true, true, 4>), __VA_ARGS__); break;                                    \
+        case 9: hipLaunchKernelGGL((KERNEL<true, true, 8>), __VA_ARGS__); break;                                    \
+        case 10: hipLaunchKernelGGL((KERNEL<true, true, 12>), __VA_ARGS__); break;                                  \
+        default: hipLaunchKernelGGL((KERNEL<true, true, 16>), __VA_ARGS__); break;                                  \
+        }                                                                                                            \
+    } while(0)
+
+int launch_dalitz_multi(const GfProgArgs &a, int nres, const double *params, int np, const double *norms, int nn,
+                        int npts, bool const_eff, bool zero_outside, int metric, Partial *partials, double *out,
+                        int nblocks, void *stream) {
     hipStream_t s = (hipStream_t)stream;
-    if(npts < 1 || npts > GF_MULTI_PTS)
+    if(npts < 1 || npts > GF_MULTI_PTS || nres < 1 || nres > GF_MAX_RES)
         return -1;
-    if(const_eff && zero_outside)
-        hipLaunchKernelGGL((k_dalitz_multi<true, true>), dim3(nblocks), dim3(256), 0, s, a, params, np, norms, nn,
-                           npts, metric, partials);
-    else if(const_eff)
-        hipLaunchKernelGGL((k_dalitz_multi<true, false>), dim3(nblocks), dim3(256), 0, s, a, params, np, norms, nn,
-                           npts, metric, partials);
-    else
-        hipLaunchKernelGGL((k_dalitz_multi<false, false>), dim3(nblocks), dim3(256), 0, s, a, params, np, norms,
-                           nn, npts, metric, partials);
+    GF_MULTI_SWITCH(k_dalitz_multi, nres, const_eff, zero_outside && const_eff, dim3(nblocks), dim3(256), 0, s, a,
+                    params, np, norms, nn, npts, metric, partials);
     int rc = check(hipGetLastError());
     if(rc)
         return rc;
@@ -1490,26 +1543,20 @@ int launch_batch(const GfProgArgs &a, const GfBatchWork *work, int nblocks, cons
     return check(hipGetLastError());
 }
 
-int launch_batch_points(const GfProgArgs &a, const GfBatchWork *work, int nblocks, int groups, const double *stage,
-                        int np, int nn, const int *tbl, bool fast, bool const_eff, bool zero_outside, int metric,
-                        Partial *partials, const int *first, int ntoys, double *result, uint32_t *counter,
-                        uint32_t *stamp, void *stream) {
+int launch_batch_points(const GfProgArgs &a, int nres, const GfBatchWork *work, int nblocks, int groups,
+                        const double *stage, int np, int nn, const int *tbl, bool fast, bool const_eff,
+                        bool zero_outside, int metric, Partial *partials, const int *first, int ntoys, double *result,
+                        uint32_t *counter, uint32_t *stamp, void *stream) {
     hipStream_t s = (hipStream_t)stream;
-    if(groups < 1 || groups * GF_MULTI_PTS > GF_BATCH_PTS)
+    if(groups < 1 || groups * GF_MULTI_PTS > GF_BATCH_PTS || (fast && (nres < 1 || nres > GF_MAX_RES)))
         return -1;
     const dim3 grid(nblocks, groups);
     if(!fast)
         hipLaunchKernelGGL(k_batch_points<2>, grid, dim3(GF_BLOCK), 0, s, a, work, stage, np, nn, tbl, metric,
                            partials);
-    else if(const_eff && zero_outside)
-        hipLaunchKernelGGL((k_batch_dalitz_points<true, true>), grid, dim3(GF_BLOCK), 0, s, a, work, stage, np, nn,
-                           tbl, metric, partials);
-    else if(const_eff)
-        hipLaunchKernelGGL((k_batch_dalitz_points<true, false>), grid, dim3(GF_BLOCK), 0, s, a, work, stage, np, nn,
-                           tbl, metric, partials);
     else
-        hipLaunchKernelGGL((k_batch_dalitz_points<false, false>), grid, dim3(GF_BLOCK), 0, s, a, work, stage, np, nn,
-                           tbl, metric, partials);
+        GF_MULTI_SWITCH(k_batch_dalitz_points, nres, const_eff, zero_outside && const_eff, grid, dim3(GF_BLOCK), 0, s,
+                        a, work, stage, np, nn, tbl, metric, partials);
     int rc = check(hipGetLastError());
     if(rc)
         return rc;

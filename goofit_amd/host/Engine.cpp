@@ -65,6 +65,11 @@ thread_local unsigned long long t_cache_epoch = 0;
 }
 unsigned long long cache_epoch() { return t_cache_epoch; }
 void bump_cache_epoch() { ++t_cache_epoch; }
+namespace {
+thread_local unsigned long long t_layout_epoch = 0;
+}
+unsigned long long layout_epoch() { return t_layout_epoch; }
+void bump_layout_epoch() { ++t_layout_epoch; }
 
 void check(gf_status status, const char *what) {
     if(status != GF_OK)

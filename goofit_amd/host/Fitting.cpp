@@ -56,14 +56,32 @@ std::vector<double> Params::make_minuit_vector() const {
 }
 
 // Params::from_minuit_vector, src/goofit/Params.cpp:47-57
+// Params::from_minuit_vector, src/goofit/Params.cpp:47-57: the changed flags, the
+// values minus the blinding offsets into every parameter updateVariable would set,
+// then the parameter array as updateParameters fills it -- the tree walks of those
+// two calls planned once per layout and replayed as flat loops.
 void Params::from_minuit_vector(const std::vector<double> &values, bool force_changed) {
-    for(Variable &var : vars_) {
-        var.setChanged(force_changed ? true : var.getValue() != values.at(var.getFitterIndex()));
-        pdf_->updateVariable(var, values.at(var.getFitterIndex()) - var.getBlind(Variable::Key()));
+    if(plan_epoch_ != engine::layout_epoch() || plan_nhost_ != host_parameters.size()) {
+        targets_.assign(vars_.size(), {});
+        for(size_t k = 0; k < vars_.size(); k++)
+            pdf_->matchingParameters(vars_[k], targets_[k]);
+        slots_.clear();
+        pdf_->parameterSlots(slots_);
+        plan_epoch_ = engine::layout_epoch();
+        plan_nhost_ = host_parameters.size();
+    }
+    for(size_t k = 0; k < vars_.size(); k++) {
+        Variable &var  = vars_[k];
+        const double v = values.at(var.getFitterIndex());
+        var.setChanged(force_changed ? true : var.getValue() != v);
+        const double nv = v - var.getBlind(Variable::Key());
+        for(Variable &t : targets_[k])
+            t.setValue(nv);
     }
     if(do_record_)
         recorded_.push_back(values);
-    pdf_->updateParameters();
+    for(const auto &sl : slots_)
+        host_parameters[sl.second] = sl.first.getValue();
 }
 
 // ---------------------------------------------------------------- FCN

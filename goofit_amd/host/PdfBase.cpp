@@ -108,6 +108,7 @@ void PdfBase::populateArrays() {
 }
 
 void PdfBase::setIndices() {
+    engine::bump_layout_epoch();
     host_parameters.clear();
     host_constants.clear();
     host_observables.clear();
@@ -136,6 +137,14 @@ void PdfBase::updateParameters() {
     for(auto *c : components)
         c->updateParameters();
     pre_call();
+}
+
+void PdfBase::matchingParameters(const Variable &var, std::vector<Variable> &out) const {
+    for(const auto &p : parametersList)
+        if(p == var || p.getName() == var.getName())
+            out.push_back(p);
+    for(auto *c : components)
+        c->matchingParameters(var, out);
 }
 
 void PdfBase::parameterSlots(std::vector<std::pair<Variable, unsigned int>> &out) const {

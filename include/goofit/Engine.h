@@ -44,6 +44,11 @@ void set_decay(const double decay6[6]);
 /// see it unchanged share the caches and may be reduced in one request.
 unsigned long long cache_epoch();
 void bump_cache_epoch();
+/// Counter of rebuilds of the flattened arrays' layout on this thread
+/// (PdfBase::setIndices): plans of slots into host_parameters stay valid while
+/// it is unchanged.
+unsigned long long layout_epoch();
+void bump_layout_epoch();
 }  // namespace engine
 
 }  // namespace GooFit

@@ -131,6 +131,9 @@ class PdfBase {
     /// this PDF's first and then its components' in the same recursion, so that a
     /// caller that sets parameters many times can copy them in a flat loop.
     void parameterSlots(std::vector<std::pair<Variable, unsigned int>> &out) const;
+    /// Every parameter of the tree that updateVariable(var, ...) sets (the same
+    /// variable or one of the same name), in the order it visits them.
+    void matchingParameters(const Variable &var, std::vector<Variable> &out) const;
 
     void SigGenSetIndices() {
         setupObservables();

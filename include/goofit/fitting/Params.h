@@ -24,6 +24,13 @@ class Params : public Minuit2::MnUserParameters {
     size_t num_;
     bool do_record_{false};
     std::vector<std::vector<double>> recorded_;
+    // from_minuit_vector's two tree walks (PdfBase::updateVariable per variable,
+    // then PdfBase::updateParameters), done once per layout of the flattened
+    // arrays: the parameters each variable sets, and the (parameter, slot) copies
+    std::vector<std::vector<Variable>> targets_;
+    std::vector<std::pair<Variable, unsigned int>> slots_;
+    unsigned long long plan_epoch_{~0ull};
+    size_t plan_nhost_{0};
 
   public:
     using MnUserParameters::MnUserParameters;
