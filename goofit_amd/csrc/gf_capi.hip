@@ -3084,7 +3084,7 @@ gf_status gf_batch_autotune(gf_batch *b, int metric, int *chunk_out, double *us_
         return GF_ERR_INVALID;
     std::vector<uint8_t> all(b->ntoys, 1);
     std::vector<double> tmp(b->ntoys);
-    const int cand[] = {1024, 2048, 4096, 8192, 16384, 32768};
+    const int cand[] = {2048, 4096, 8192, 16384};
     int best = 0;
     double best_us = INFINITY;
     for(int c : cand) {
@@ -3094,10 +3094,10 @@ gf_status gf_batch_autotune(gf_batch *b, int metric, int *chunk_out, double *us_
         if((st = gf_batch_reduce(b, all.data(), metric, tmp.data())))  // warm-up (and graph build)
             return st;
         const auto t0 = std::chrono::steady_clock::now();
-        for(int r = 0; r < 5; r++)
+        for(int r = 0; r < 2; r++)
             if((st = gf_batch_reduce(b, all.data(), metric, tmp.data())))
                 return st;
-        const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() / 5;
+        const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() / 2;
         if(us < best_us) {
             best_us = us;
             best    = c;

@@ -249,11 +249,8 @@ __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs
                         const double cr = P[in.p + 5 + 2 * (i0 + u)], ci = P[in.p + 6 + 2 * (i0 + u)];
 #pragma unroll
                         for(int j = 0; j < EPT; ++j) {
-                            const double2 a_ = wa[u][j], b_ = wb[u][j];
-                            sar[j] += a_.x * cr - a_.y * ci;
-                            sai[j] += a_.x * ci + a_.y * cr;
-                            sbr[j] += b_.x * cr - b_.y * ci;
-                            sbi[j] += b_.x * ci + b_.y * cr;
+                            gfd::cmac(sar[j], sai[j], cr, ci, wa[u][j]);
+                            gfd::cmac(sbr[j], sbi[j], cr, ci, wb[u][j]);
                         }
                     }
                 }
@@ -899,7 +896,7 @@ __global__ __launch_bounds__(256) void k_dalitz_cache(GfProgArgs A, int slot, in
 // The metric sum over rows [w.start, w.end) of one toy with parameter array P and
 // normalisation array Nm, accumulated per thread in the tile order of
 // k_batch_metric (EPT rows per thread per tile).
-template <int EPT>
+template <int EPT, int FEAT>
 __device__ __forceinline__ void batch_rows(const GfProgArgs &A, const GfBatchWork &w, const double *__restrict__ P,
                                            const double *__restrict__ Nm, int metric, double &s, double &c) {
     const double norm  = Nm[1];
@@ -918,7 +915,7 @@ __device__ __forceinline__ void batch_rows(const GfProgArgs &A, const GfBatchWor
             src.idx[j]      = row[j];
         }
         double T[EPT];
-        run<EPT>(A, P, Nm, 0, src, row, valid, T);
+        run<EPT, FEAT>(A, P, Nm, 0, src, row, valid, T);
 #pragma unroll
         for(int j = 0; j < EPT; ++j) {
             if(!valid[j])
@@ -936,8 +933,9 @@ __device__ __forceinline__ void batch_rows(const GfProgArgs &A, const GfBatchWor
 // Metric sums of many toys in one launch: workgroup b evaluates the rows of
 // work[b] with its toy's parameter and normalisation arrays (P = params +
 // toy * np, Nm = norms + toy * nn) and stores one partial; workgroups of
-// inactive toys store zero.
-template <int EPT>
+// inactive toys store zero.  FEAT as for k_reduce_metric (toy batches hold no
+// time-dependent node).
+template <int EPT, int FEAT>
 __global__ __launch_bounds__(GF_BLOCK) void k_batch_metric(GfProgArgs A, const GfBatchWork *__restrict__ work,
                                                            const double *__restrict__ params, int np,
                                                            const double *__restrict__ norms, int nn,
@@ -946,14 +944,14 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_metric(GfProgArgs A, const G
     const GfBatchWork w = work[blockIdx.x];
     double s = 0.0, c = 0.0;
     if(active[w.toy])
-        batch_rows<EPT>(A, w, params + (int64_t)w.toy * np, norms + (int64_t)w.toy * nn, metric, s, c);
+        batch_rows<EPT, FEAT>(A, w, params + (int64_t)w.toy * np, norms + (int64_t)w.toy * nn, metric, s, c);
     store_partial(s, c, partials);
 }
 
 // Several points per toy, any program: workgroup (b, g) evaluates the rows of
 // work[b] once per point p in [16 g, min(16 g + 16, npts)) of its toy and stores
 // partial (b, p).  Each point's sum takes the path of k_batch_metric.
-template <int EPT>
+template <int EPT, int FEAT>
 __global__ __launch_bounds__(GF_BLOCK) void k_batch_points(GfProgArgs A, const GfBatchWork *__restrict__ work,
                                                            const double *__restrict__ stage, int np, int nn,
                                                            const int *__restrict__ tbl, int metric,
@@ -966,7 +964,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_points(GfProgArgs A, const G
     for(int p = p0; p < p1; ++p) {
         const double *__restrict__ P = stage + (int64_t)(slot0 + p) * (np + nn);
         double s = 0.0, c = 0.0;
-        batch_rows<EPT>(A, w, P, P + np, metric, s, c);
+        batch_rows<EPT, FEAT>(A, w, P, P + np, metric, s, c);
         block_reduce(s, c);
         if(threadIdx.x == 0) {
             partials[(int64_t)blockIdx.x * GF_BATCH_PTS + p].s = s;
@@ -1531,8 +1529,12 @@ int launch_batch(const GfProgArgs &a, const GfBatchWork *work, int nblocks, cons
                  const double *norms, int nn, const uint8_t *active, int metric, Partial *partials, const int *first,
                  int ntoys, double *result, uint32_t *counter, uint32_t *stamp, void *stream) {
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_batch_metric<2>, dim3(nblocks), dim3(GF_BLOCK), 0, s, a, work, params, np, norms, nn, active,
-                       metric, partials);
+    if(a.has_dz)
+        hipLaunchKernelGGL((k_batch_metric<2, 1>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, work, params, np, norms, nn,
+                           active, metric, partials);
+    else
+        hipLaunchKernelGGL((k_batch_metric<2, 0>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, work, params, np, norms, nn,
+                           active, metric, partials);
     int rc = check(hipGetLastError());
     if(rc)
         return rc;
@@ -1551,8 +1553,11 @@ int launch_batch_points(const GfProgArgs &a, int nres, const GfBatchWork *work, 
     if(groups < 1 || groups * GF_MULTI_PTS > GF_BATCH_PTS || (fast && (nres < 1 || nres > GF_MAX_RES)))
         return -1;
     const dim3 grid(nblocks, groups);
-    if(!fast)
-        hipLaunchKernelGGL(k_batch_points<2>, grid, dim3(GF_BLOCK), 0, s, a, work, stage, np, nn, tbl, metric,
+    if(!fast && a.has_dz)
+        hipLaunchKernelGGL((k_batch_points<2, 1>), grid, dim3(GF_BLOCK), 0, s, a, work, stage, np, nn, tbl, metric,
+                           partials);
+    else if(!fast)
+        hipLaunchKernelGGL((k_batch_points<2, 0>), grid, dim3(GF_BLOCK), 0, s, a, work, stage, np, nn, tbl, metric,
                            partials);
     else
         GF_MULTI_SWITCH(k_batch_dalitz_points, nres, const_eff, zero_outside && const_eff, grid, dim3(GF_BLOCK), 0, s,

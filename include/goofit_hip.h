@@ -370,7 +370,8 @@ gf_status gf_batch_set_chunk(gf_batch *b, int chunk);
 gf_status gf_batch_get_chunk(gf_batch *b, int *chunk);
 /* on = 0: plain stream launches instead of the captured graph. */
 gf_status gf_batch_use_graph(gf_batch *b, int on);
-/* Time every active-all reduction for chunk in {1024 ... 32768}, keep the fastest. */
+/* Time an active-all reduction for chunk in {2048, 4096, 8192, 16384} (one
+ * warm-up and two timed launches each), keep the fastest. */
 gf_status gf_batch_autotune(gf_batch *b, int metric, int *chunk_out, double *us_out);
 
 /* Several parameter points per toy in one launch (no reference counterpart: the
