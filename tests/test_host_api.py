@@ -138,6 +138,43 @@ def test_minuit_vector_roundtrip_and_blinding():
     assert x2[0] in list(p)  # host_parameters updated in place (PdfBase::updateParameters)
 
 
+def test_params_plan_sets_every_slot_and_follows_relayouts():
+    """Params::from_minuit_vector replays the tree walks of PdfBase::updateVariable and
+    updateParameters (PdfBase.cu:121-140) from a plan: a variable shared by two components
+    reaches both parameter slots, blinded values arrive minus the offset, and a new layout
+    of the flattened arrays (setIndices, here with a third component in front) is planned
+    anew instead of writing the old slots."""
+    x = G.Observable("x", -5.0, 5.0)
+    mu = G.Variable("mu", 0.1, 0.01, -1.0, 1.0)
+    s1 = G.Variable("s1", 1.0, 0.01, 0.1, 3.0)
+    s2 = G.Variable("s2", 1.5, 0.01, 0.1, 3.0)
+    f = G.Variable("f", 0.4, 0.01, 0.0, 1.0)
+    g1, g2 = G.GaussianPdf("g1", x, mu, s1), G.GaussianPdf("g2", x, mu, s2)
+    top = G.AddPdf("sum", [f], [g1, g2])
+    top.SigGenSetIndices()
+    par = G.Params(top)
+    names = [v.name for v in top.getParameters()]
+    v = list(par.make_minuit_vector())
+    v[names.index("mu")] = 0.37
+    v[names.index("s1")] = 1.25
+    s2.blind = 0.5
+    v[names.index("s2")] = 2.0
+    par.from_minuit_vector(v)
+    assert mu.value == 0.37 and s1.value == 1.25 and s2.value == 1.5
+    p = list(G.engine.flattened()[1])
+    assert p.count(0.37) == 2 and 1.25 in p and 1.5 in p
+    # a new layout: the plan is rebuilt, every slot of the new arrays receives its value
+    e0 = G.Variable("e0", -0.2)
+    ex = G.ExpPdf("ex", x, e0)
+    top2 = G.AddPdf("sum2", [G.Variable("f2", 0.3, 0.01, 0.0, 1.0)], [ex, top])
+    top2.SigGenSetIndices()
+    v[names.index("mu")] = -0.11
+    par.from_minuit_vector(v)
+    p2 = list(G.engine.flattened()[1])
+    assert p2.count(-0.11) == 2 and 0.37 not in p2
+    assert -0.2 in p2  # the new component's own block is intact
+
+
 def test_device_path_fails_loudly_without_gpu():
     """No CPU fallback: without a GPU every device entry point raises."""
     from goofit_amd import capi
