@@ -27,8 +27,9 @@ struct Batcher {
     std::vector<size_t> x0, xn;                               // of those, the points of this request
     std::vector<uint8_t> pending, done;
     std::vector<std::vector<double>> result;
-    int running = 0;   // toys whose MIGRAD is in progress
-    int waiting = 0;   // of those, blocked in an FCN call
+    int running    = 0;  // toys whose MIGRAD is in progress
+    int waiting    = 0;  // of those, blocked in an FCN call
+    int unprepared = 0;  // of those, whose request the batcher has not taken yet
     bool stop   = false;
 };
 
@@ -66,6 +67,7 @@ class ToyFCN : public Minuit2::FCNBase {
         b_.done[toy_]    = 0;
         b_.pending[toy_] = 1;
         b_.waiting++;
+        b_.unprepared++;
         b_.cv_req.notify_one();
         b_.cv_done.wait(lk, [&] { return b_.done[toy_] != 0; });
         return b_.result[toy_];
@@ -166,86 +168,118 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
     for(int i = 0; i < W; i++)
         pool.emplace_back(worker);
 
-    // batcher: serve every pending request of the running toys in one launch
-    std::vector<uint8_t> act(T);
+    // batcher: prepare each request as it arrives (the Minuit work of the other toys
+    // goes on meanwhile), launch once every running toy waits -- or when no new
+    // request came for 200 us -- and answer the toys of that launch
+    std::vector<uint8_t> act(T, 0), prepared(T, 0);
     std::vector<double> out(multi_ ? (size_t)T * GF_BATCH_MAX_POINTS : (size_t)T);
-    std::vector<int> serve;
+    std::vector<int> taken, serve;
     std::exception_ptr failure;
-    for(;;) {
-        std::unique_lock<std::mutex> lk(B.mu);
-        const auto w0 = std::chrono::steady_clock::now();
-        // wait until every running toy is blocked in an FCN call (or all done)
-        B.cv_req.wait_for(lk, std::chrono::milliseconds(50), [&] {
-            return (B.running > 0 && B.waiting == B.running) || (B.running == 0 && next >= T);
-        });
-        if(B.running == 0 && next >= T)
-            break;
-        if(B.waiting == 0)
-            continue;
-        if(B.waiting < B.running) {
-            // some toys are in host-side Minuit work: give them a moment, then serve what is there
-            B.cv_req.wait_for(lk, std::chrono::microseconds(200),
-                              [&] { return B.waiting == B.running; });
-        }
-        serve.clear();
-        for(int t = 0; t < T; t++)
-            if(B.pending[t])
-                serve.push_back(t);
-        std::fill(act.begin(), act.end(), 0);
-        const auto p0 = std::chrono::steady_clock::now();
-        stats_.gather_s += std::chrono::duration<double>(p0 - w0).count();
-        int64_t points = 0;
-        try {
-            for(int t : serve) {
-                for(size_t k = 0; k < B.xn[t]; k++) {
-                    // push the point into the shared Variables, then the host arrays as
-                    // Params::from_minuit_vector: flag the values that move, so that an
-                    // Amp3Body recomputes its integrals only when a mass or width changes
-                    const std::vector<double> &x = (*B.x[t])[B.x0[t] + k];
-                    for(Variable &v : vars) {
-                        const double nv = x.at(v.getFitterIndex()) - v.getBlind(Variable::Key());
-                        v.setChanged(v.getValue() != nv);
-                        v.setValue(nv);
-                    }
-                    for(const auto &sl : slots)
-                        host_parameters[sl.second] = sl.first.getValue();
-                    pdf_->normalize();
-                    if(multi_)
-                        engine::check(gf_batch_add_point(batch_, t, host_parameters.data(), host_normalizations.data()),
-                                      "gf_batch_add_point");
-                    else
-                        engine::check(gf_batch_set_toy(batch_, t, host_parameters.data(), host_normalizations.data()),
-                                      "gf_batch_set_toy");
-                }
-                points += (int64_t)B.xn[t];
-                act[t] = 1;
+    bool round_failed = false;
+    int64_t points    = 0;
+    auto prepare = [&](int t) {
+        for(size_t k = 0; k < B.xn[t]; k++) {
+            // push the point into the shared Variables, then the host arrays as
+            // Params::from_minuit_vector: flag the values that move, so that an
+            // Amp3Body recomputes its integrals only when a mass or width changes
+            const std::vector<double> &x = (*B.x[t])[B.x0[t] + k];
+            for(Variable &v : vars) {
+                const double nv = x.at(v.getFitterIndex()) - v.getBlind(Variable::Key());
+                v.setChanged(v.getValue() != nv);
+                v.setValue(nv);
             }
-            const auto g0 = std::chrono::steady_clock::now();
-            stats_.host_prep_s += std::chrono::duration<double>(g0 - p0).count();
+            for(const auto &sl : slots)
+                host_parameters[sl.second] = sl.first.getValue();
+            pdf_->normalize();
             if(multi_)
-                engine::check(gf_batch_reduce_points(batch_, GF_METRIC_NLL, out.data()), "gf_batch_reduce_points");
+                engine::check(gf_batch_add_point(batch_, t, host_parameters.data(), host_normalizations.data()),
+                              "gf_batch_add_point");
             else
-                engine::check(gf_batch_reduce(batch_, act.data(), GF_METRIC_NLL, out.data()), "gf_batch_reduce");
-            stats_.gpu_wait_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - g0).count();
-        } catch(...) {
-            failure = std::current_exception();
+                engine::check(gf_batch_set_toy(batch_, t, host_parameters.data(), host_normalizations.data()),
+                              "gf_batch_set_toy");
+        }
+    };
+    for(;;) {
+        taken.clear();
+        bool launch = false;
+        {
+            std::unique_lock<std::mutex> lk(B.mu);
+            const auto w0 = std::chrono::steady_clock::now();
+            // the requests of the toys blocked in an FCN call are all in hand only when
+            // every running toy waits; new ones may still come until then
+            const bool arrived = B.cv_req.wait_for(
+                lk, serve.empty() ? std::chrono::microseconds(50000) : std::chrono::microseconds(200), [&] {
+                    return B.unprepared > 0 || (!serve.empty() && B.waiting == B.running)
+                           || (B.running == 0 && next >= T);
+                });
+            stats_.gather_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
+            if(B.running == 0 && next >= T && serve.empty())
+                break;
+            for(int t = 0; t < T; t++)
+                if(B.pending[t] && !prepared[t]) {
+                    prepared[t] = 1;
+                    taken.push_back(t);
+                }
+            B.unprepared -= (int)taken.size();
+            launch = !(serve.empty() && taken.empty()) && (B.waiting == B.running || !arrived);
+        }
+        // the taken toys' requests stay untouched while those toys wait: prepare them
+        // without holding the lock
+        const auto p0 = std::chrono::steady_clock::now();
+        for(int t : taken) {
+            if(!round_failed) {
+                try {
+                    prepare(t);
+                } catch(...) {
+                    failure      = std::current_exception();
+                    round_failed = true;
+                }
+            }
+            points += (int64_t)B.xn[t];
+            act[t] = 1;
+            serve.push_back(t);
+        }
+        stats_.host_prep_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - p0).count();
+        if(!launch)
+            continue;
+        const auto g0 = std::chrono::steady_clock::now();
+        if(!round_failed) {
+            try {
+                if(multi_)
+                    engine::check(gf_batch_reduce_points(batch_, GF_METRIC_NLL, out.data()), "gf_batch_reduce_points");
+                else
+                    engine::check(gf_batch_reduce(batch_, act.data(), GF_METRIC_NLL, out.data()), "gf_batch_reduce");
+            } catch(...) {
+                failure      = std::current_exception();
+                round_failed = true;
+            }
+        }
+        if(round_failed) {
             if(multi_)
                 (void)gf_batch_clear_points(batch_);
             std::fill(out.begin(), out.end(), 1e300);
         }
+        stats_.gpu_wait_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - g0).count();
         stats_.launches++;
         stats_.fcn_calls += points;
-        for(int t : serve) {
-            if(multi_)
-                B.result[t].assign(out.begin() + (size_t)t * GF_BATCH_MAX_POINTS,
-                                   out.begin() + (size_t)t * GF_BATCH_MAX_POINTS + B.xn[t]);
-            else
-                B.result[t].assign(1, out[t]);
-            B.pending[t] = 0;
-            B.done[t]    = 1;
-            B.waiting--;
+        {
+            std::lock_guard<std::mutex> lk(B.mu);
+            for(int t : serve) {
+                if(multi_)
+                    B.result[t].assign(out.begin() + (size_t)t * GF_BATCH_MAX_POINTS,
+                                       out.begin() + (size_t)t * GF_BATCH_MAX_POINTS + B.xn[t]);
+                else
+                    B.result[t].assign(1, out[t]);
+                B.pending[t] = 0;
+                B.done[t]    = 1;
+                B.waiting--;
+                prepared[t] = 0;
+                act[t]      = 0;
+            }
         }
         B.cv_done.notify_all();
+        serve.clear();
+        points = 0;
     }
     for(auto &th : pool)
         th.join();
