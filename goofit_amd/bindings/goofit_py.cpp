@@ -723,6 +723,12 @@ PYBIND11_MODULE(_goofit, m) {
         engine::check(gf_time_reduce_sweep(engine::ctx(), metric, reps, &sweep, &total), "gf_time_reduce_sweep");
         return py::make_tuple(sweep, total);
     });
+    me.def("time_sweep_back_to_back", [](int metric, int reps) {
+        engine::flush();
+        double ms = 0;
+        engine::check(gf_time_sweep_back_to_back(engine::ctx(), metric, reps, &ms), "gf_time_sweep_back_to_back");
+        return ms;
+    });
     me.def("time_reduce_metric", [](int metric, int reps) {
         engine::flush();
         double ms = 0;

@@ -32,7 +32,8 @@ EXPORTS = [
     "gf_decay3_set", "gf_reduce_metric", "gf_reduce_metric_multi", "gf_eval_metric", "gf_eval_metric_device", "gf_reduce_bins",
     "gf_dalitz_set_data_size", "gf_dalitz_cache", "gf_dalitz_gram", "gf_dalitz_gram_points",
     "gf_dalitz_gram_points_device", "gf_dalitz_get_cache", "gf_dalitz_cache_gram", "gf_tddp_gram",
-    "gf_time_reduce_metric", "gf_time_reduce_sweep", "gf_time_dalitz_cache", "gf_time_dalitz_gram",
+    "gf_time_reduce_metric", "gf_time_reduce_sweep", "gf_time_sweep_back_to_back", "gf_time_dalitz_cache",
+    "gf_time_dalitz_gram",
     "gf_sample_uniform", "gf_mlp_param_count", "gf_mlp_create", "gf_mlp_destroy", "gf_mlp_get_params",
     "gf_mlp_set_params", "gf_mlp_forward", "gf_mlp_gradient", "gf_mlp_train", "gf_mlp_build_proposal",
     "gf_mlp_sample", "gf_mlp_train_host", "gf_mlp_use_graph", "gf_dalitz_set_norm_points", "gf_dalitz_gram_norm_points",
@@ -94,6 +95,7 @@ def lib():
         L.gf_reduce_metric_multi.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D]
         L.gf_time_reduce_metric.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D]
         L.gf_time_reduce_sweep.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D, _D]
+        L.gf_time_sweep_back_to_back.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D]
         _F = ctypes.POINTER(ctypes.c_float)
         L.gf_sample_uniform.argtypes = [c_p, _D, ctypes.c_int64, ctypes.c_uint64, c_p, c_p, c_p]
         L.gf_mlp_param_count.restype = ctypes.c_int
@@ -341,6 +343,13 @@ class Context:
         m = METRIC[metric] if isinstance(metric, str) else metric
         _check(lib().gf_time_reduce_sweep(self.h, m, reps, ctypes.byref(sw), ctypes.byref(tot)))
         return sw.value, tot.value
+
+    def time_sweep_back_to_back(self, metric, reps):
+        """Device ms per event sweep, reps sweeps launched back to back between two events."""
+        ms = ctypes.c_double()
+        m = METRIC[metric] if isinstance(metric, str) else metric
+        _check(lib().gf_time_sweep_back_to_back(self.h, m, reps, ctypes.byref(ms)))
+        return ms.value
 
     def time_dalitz_cache(self, dalitz_func, reps):
         ms = ctypes.c_double()

@@ -752,6 +752,7 @@ GfProgArgs prog_args(gf_ctx *ctx) {
     a.err          = ctx->d_err;
     a.seq          = ctx->res_seq;
     a.mark         = nullptr;
+    a.sweep_only   = 0;
     return a;
 }
 
@@ -905,15 +906,17 @@ int fast_blocks(const gf_ctx *ctx, int64_t n) {
 
 // Launch the likelihood reduction (fast path or interpreter) without waiting;
 // mark (optional) is recorded between the event sweep and k_finish.
-gf_status launch_reduce(gf_ctx *ctx, int metric, hipEvent_t mark = nullptr) {
+gf_status launch_reduce(gf_ctx *ctx, int metric, hipEvent_t mark = nullptr, bool sweep_only = false) {
     GfInline inl;
     const bool have_inl = make_inline(ctx, inl);
     gf_status st        = have_inl ? GF_OK : flush_uploads(ctx);
     if(st)
         return st;
-    ++ctx->res_seq;
+    if(!sweep_only)
+        ++ctx->res_seq;
     GfProgArgs a = prog_args(ctx);
     a.mark       = mark;
+    a.sweep_only = sweep_only ? 1 : 0;
     int rc;
     if(use_fast_dalitz(ctx, metric))
         rc = gfk::launch_dalitz_fast(a, have_inl ? &inl : nullptr, const_efficiency(ctx), zero_outside(ctx), metric, ctx->d_partials,
@@ -1982,6 +1985,31 @@ gf_status gf_time_reduce_sweep(gf_ctx *ctx, int metric, int reps, double *ms_swe
         *ms_total = ms;
     }
     return read_result(ctx, nullptr, nullptr);
+}
+
+gf_status gf_time_sweep_back_to_back(gf_ctx *ctx, int metric, int reps, double *ms_per_sweep) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    gf_status st = check_events_for_program(ctx, metric);
+    if(st)
+        return st;
+    if(reps < 1 || reps > 4096 || !ms_per_sweep)
+        return fail(GF_ERR_INVALID, "reps must be in 1..4096");
+    if(ctx->stage_in_flight) {
+        GF_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->stage_in_flight = false;
+    }
+    GF_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    for(int r = 0; r < reps; r++)
+        if((st = launch_reduce(ctx, metric, nullptr, true)))
+            return st;
+    GF_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    GF_HIP(hipEventSynchronize(ctx->ev1));
+    ctx->stage_in_flight = false;  // every staged upload has been consumed
+    float ms = 0;
+    GF_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *ms_per_sweep = ms / reps;
+    return GF_OK;
 }
 
 gf_status gf_time_dalitz_cache(gf_ctx *ctx, int dalitz_func, int reps, double *ms_total) {

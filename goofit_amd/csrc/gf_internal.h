@@ -115,6 +115,7 @@ struct GfProgArgs {
     int32_t *err;                // device error word (bit 0: cache index out of range, bit 1: node kind not compiled in)
     uint32_t seq;                // sequence number k_finish stamps on its GfResult
     void *mark;                  // host only: hipEvent_t recorded between a sweep and its k_finish
+    int32_t sweep_only;          // host only: launch the sweep without its k_finish (timing)
 };
 
 #define GF_METRIC_BINAVG_CODE 3  // metrics from here on read (bin entry, bin volume/error) columns

@@ -1396,6 +1396,8 @@ static int launch_finish(const GfProgArgs &a, Partial *partials, int nblocks, Gf
         return rc;
     if(a.mark && (rc = check(hipEventRecord((hipEvent_t)a.mark, s))))
         return rc;
+    if(a.sweep_only)
+        return 0;
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(GF_BLOCK), 0, s, partials, nblocks, result, a.err, a.seq);
     return check(hipGetLastError());
 }

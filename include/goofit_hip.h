@@ -404,6 +404,12 @@ gf_status gf_time_reduce_metric(gf_ctx *ctx, int metric, int reps, double *ms_to
  * No reference counterpart. */
 gf_status gf_time_reduce_sweep(gf_ctx *ctx, int metric, int reps, double *ms_sweep, double *ms_total);
 
+/* The event sweep alone, reps <= 4096 launches back to back between two events
+ * (no finishing kernel in between and no result read): ms_per_sweep = elapsed /
+ * reps, the sweep kernel's duration plus the gap between back-to-back dispatches.
+ * The next reduction of the context is unaffected.  No reference counterpart. */
+gf_status gf_time_sweep_back_to_back(gf_ctx *ctx, int metric, int reps, double *ms_per_sweep);
+
 /* Same for gf_dalitz_cache (all resonances) and gf_dalitz_gram. */
 gf_status gf_time_dalitz_cache(gf_ctx *ctx, int dalitz_func, int reps, double *ms_total);
 gf_status gf_time_dalitz_gram(gf_ctx *ctx, int dalitz_func, const double grid6[6], int eff_func, int reps,

@@ -180,3 +180,30 @@ def test_calculate_nll_aborts_on_non_positive_normalisation():
     neg.setData(d)
     with pytest.raises(G.GeneralError, match="(?i)non-positive normalization"):
         neg.calculateNLL()
+
+
+def test_back_to_back_sweep_timing_leaves_the_context_as_it_was(ctx):
+    """gf_time_sweep_back_to_back launches sweeps without their finishing kernel: the
+    reductions before and after it give the same bits, with parameters inline and in
+    device arrays."""
+    n = 20_000
+    top, _ = dalitz(nbins=64)
+    aos = dalitz_events(n, 35)
+    M = O.Model(top, aos, decay=W.DALITZ_DECAY)
+    ref = M.calculate_nll()
+    _install(ctx, top, [aos[:, 0], aos[:, 1], aos[:, 2]], decay=W.DALITZ_DECAY)
+    ctx.set_identity_column(2)
+    ctx.dalitz_set_data_size(top.function_idx, n)
+    M.op.sync_params()
+    ctx.update_params(M.op.p)
+    ctx.dalitz_cache(top.function_idx)
+    _sync(ctx, M)
+    try:
+        for inline in (1, 0):
+            ctx.set_tuning(capi.TUNE_INLINE_ARGS, inline)
+            before = ctx.reduce_metric("NLL")[0]
+            assert ctx.time_sweep_back_to_back("NLL", 16) > 0
+            after = ctx.reduce_metric("NLL")[0]
+            assert before == after and math.isclose(after, ref, rel_tol=NLL_RTOL)
+    finally:
+        ctx.set_tuning(capi.TUNE_INLINE_ARGS, 1)
