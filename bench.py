@@ -85,36 +85,36 @@ TRAIN = dict(steps=500, batch=8192, lr=6e-3)  # proposal training: ~0.06 s, rela
 
 def norm_error_study(model, pdf, train_events, sync, t_train_gen, target=1e-4, seeds=6):
     """Normalisation-integral error at equal Monte-Carlo points, uniform phase-space MC vs the
-    learned proposal, against a 1e8-point uniform FP64 judge; then a MIGRAD fit with each method
-    at the point count that reaches `target` relative error.  The learned method is charged its
+    learned proposal, against a uniform FP64 reference integral of 1e8 and 1e9 points; then a
+    MIGRAD fit with each method at the point count that reaches `target` relative error.  The learned method is charged its
     training-event generation and training.  Leaves pdf on its midpoint grid."""
     import goofit_amd as G
     from goofit_amd import workloads as W
     out = {}
     truth = {n: v.value for n, v in model.params.items()}
     grid_norm = pdf.normalize()
-    # judge: uniform FP64 Monte Carlo, 1e8 points as specified and 1e9 for a tighter reference
-    judges = {}
+    # reference integral: uniform FP64 Monte Carlo, 1e8 points and 1e9 for a tighter one
+    refs = {}
     for npts, seed in ((100_000_000, 991), (1_000_000_000, 992)):
         t0 = time.perf_counter()
         pdf.setNormalisationPoints(npts, seed)
         val = pdf.normalize()
         sync()
-        judges[npts] = dict(points=npts, value=val, seconds=round(time.perf_counter() - t0, 3),
-                            rel_stat_err=math.sqrt(pdf.normalisationRelativeVariance(16) / npts),
-                            grid_vs_judge=grid_norm / val - 1)
-    judge = judges[1_000_000_000]["value"]
-    out["judge"] = judges[1_000_000_000]
-    out["judge_1e8"] = judges[100_000_000]
-    out["judge_1e8"]["vs_1e9"] = judges[100_000_000]["value"] / judge - 1
-    log(f"judge: 1e9 uniform points (rel. stat. error {out['judge']['rel_stat_err']:.1e}); "
-        f"1e8-point judge / 1e9 - 1 = {out['judge_1e8']['vs_1e9']:.2e}; grid / judge - 1 = {grid_norm / judge - 1:.2e}")
+        refs[npts] = dict(points=npts, value=val, seconds=round(time.perf_counter() - t0, 3),
+                          rel_stat_err=math.sqrt(pdf.normalisationRelativeVariance(16) / npts),
+                          grid_vs_reference=grid_norm / val - 1)
+    ref_norm = refs[1_000_000_000]["value"]
+    out["reference"] = refs[1_000_000_000]
+    out["reference_1e8"] = refs[100_000_000]
+    out["reference_1e8"]["vs_1e9"] = refs[100_000_000]["value"] / ref_norm - 1
+    log(f"reference integral: 1e9 uniform points (rel. stat. error {out['reference']['rel_stat_err']:.1e}); "
+        f"1e8-point reference / 1e9 - 1 = {out['reference_1e8']['vs_1e9']:.2e}; grid / reference - 1 = {grid_norm / ref_norm - 1:.2e}")
 
     def errors(n, proposal, base):
         e, v = [], []
         for k in range(seeds):
             pdf.setNormalisationPoints(n, base + k, proposal)
-            e.append(pdf.normalize() / judge - 1)
+            e.append(pdf.normalize() / ref_norm - 1)
             v.append(pdf.normalisationRelativeVariance(16))
         e = np.array(e)
         return dict(points=n, rms_rel_err=float(np.sqrt((e ** 2).mean())), mean_rel_err=float(e.mean()),

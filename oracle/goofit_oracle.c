@@ -1569,7 +1569,7 @@ double gfo_tddp_norm_from_integrals(int nres, const double *coup, const double *
 /* ------------------------------------------------------------ exact sums */
 
 /* Neumaier-compensated sum in long double of an array: a near-exact reference
- * used to judge both the chunked oracle sum and the GPU's compensated sum. */
+ * used to check both the chunked oracle sum and the GPU's compensated sum. */
 double gfo_exact_sum(const double *x, int64_t n) {
     long double s = 0.0L, c = 0.0L;
     for(int64_t i = 0; i < n; i++) {

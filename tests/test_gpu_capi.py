@@ -1,6 +1,6 @@
 """Parity of the gfx950 engine with the oracle, through the C ABI.
 
-Tolerances: the NLL sum within 1e-11 relative (the north-star bar); single
+Tolerances: the NLL sum within 1e-11 relative; single
 event values within 1e-13 relative (libm exp/log/erfc/pow differ by <= a few
 ulp between glibc and the device math library); grid indices bit-exact.
 """

@@ -119,10 +119,10 @@ def test_proposal_beats_uniform_at_equal_points(ctx):
     keep = rng.uniform(0, val.max(), val.size) < val
     ev12, ev13 = m12[keep], m13[keep]
     assert ev12.size > 50_000
-    # judge: fine midpoint grid
+    # reference: fine midpoint grid
     G = 1500
     I_grid = ctx.dalitz_gram(top.function_idx, [0.0, 3.0, G, 0.0, 3.0, G], eff, nres) * (3.0 / G) ** 2
-    judge = O.dalitz_norm_from_integrals(coups, I_grid, 1.0, 1.0)
+    ref_norm = O.dalitz_norm_from_integrals(coups, I_grid, 1.0, 1.0)
 
     q = capi.Proposal(ctx, seed=9, batch_cap=16384)
     try:
@@ -137,8 +137,8 @@ def test_proposal_beats_uniform_at_equal_points(ctx):
             Iu = ctx.dalitz_gram_norm_points(top.function_idx, eff, 0, N, nres)
             ctx.dalitz_set_norm_points(top.function_idx, N, 200 + s, proposal=q)
             Iq = ctx.dalitz_gram_norm_points(top.function_idx, eff, 0, N, nres)
-            errs_u.append(O.dalitz_norm_from_integrals(coups, Iu, 1.0, 1.0) / judge - 1)
-            errs_q.append(O.dalitz_norm_from_integrals(coups, Iq, 1.0, 1.0) / judge - 1)
+            errs_u.append(O.dalitz_norm_from_integrals(coups, Iu, 1.0, 1.0) / ref_norm - 1)
+            errs_q.append(O.dalitz_norm_from_integrals(coups, Iq, 1.0, 1.0) / ref_norm - 1)
         errs_u, errs_q = np.array(errs_u), np.array(errs_q)
         rms_u, rms_q = np.sqrt((errs_u ** 2).mean()), np.sqrt((errs_q ** 2).mean())
         print(f"norm rel. error at N=1e5: uniform {rms_u:.2e}, proposal {rms_q:.2e}; loss {l0:.3f} -> {l1:.3f}")
