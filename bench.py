@@ -434,7 +434,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the path shards by events across GPUs "
-              f"but this build runs on a single MI355X.", file=sys.stderr)
+              f"but this benchmark runs on a single MI355X.", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -595,7 +595,7 @@ PYBIND11_MODULE(_goofit, m) {
 
     // ------------------------------------------------------------ Minuit2 (python/Minuit2/*.cpp of the reference)
     {
-        auto mm = m.def_submodule("Minuit2", "The Minuit2 API used by GooFit, on this build's minimiser");
+        auto mm = m.def_submodule("Minuit2", "The Minuit2 API used by GooFit, on the goofit_amd minimiser");
         struct PyFCN : M2::FCNBase {
             using M2::FCNBase::FCNBase;
             double operator()(const std::vector<double> &v) const override {
