@@ -39,7 +39,8 @@ EXPORTS = [
     "gf_mlp_sample", "gf_mlp_train_host", "gf_mlp_use_graph", "gf_dalitz_set_norm_points", "gf_dalitz_gram_norm_points",
     "gf_dalitz_generate", "gf_batch_create", "gf_batch_destroy", "gf_batch_set_toy", "gf_batch_reduce",
     "gf_batch_set_chunk", "gf_batch_get_chunk", "gf_batch_use_graph", "gf_batch_autotune",
-    "gf_batch_add_point", "gf_batch_reduce_points", "gf_batch_clear_points",
+    "gf_batch_add_point", "gf_batch_reduce_points", "gf_batch_clear_points", "gf_batch_submit_points",
+    "gf_batch_wait_points",
 ]
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -128,6 +129,8 @@ def lib():
         L.gf_batch_add_point.argtypes = [c_p, ctypes.c_int, _D, _D]
         L.gf_batch_reduce_points.argtypes = [c_p, ctypes.c_int, _D]
         L.gf_batch_clear_points.argtypes = [c_p]
+        L.gf_batch_submit_points.argtypes = [c_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]
+        L.gf_batch_wait_points.argtypes = [c_p, ctypes.c_uint32, _D]
         L.gf_dalitz_generate.argtypes = [c_p, ctypes.c_int, ctypes.c_int64, ctypes.c_uint64, _D, ctypes.c_double,
                                          ctypes.c_int, c_p, c_p, ctypes.POINTER(ctypes.c_int64), _D]
         L.gf_time_dalitz_cache.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D]
@@ -490,6 +493,17 @@ class Batch:
 
     def clear_points(self):
         _check(lib().gf_batch_clear_points(self.h))
+
+    def submit_points(self, metric="NLL"):
+        t = ctypes.c_uint32()
+        m = METRIC[metric] if isinstance(metric, str) else metric
+        _check(lib().gf_batch_submit_points(self.h, m, ctypes.byref(t)))
+        return t.value
+
+    def wait_points(self, ticket):
+        out = np.full(self.n * BATCH_MAX_POINTS, np.nan)
+        _check(lib().gf_batch_wait_points(self.h, ticket, _dp(out)))
+        return out.reshape(self.n, BATCH_MAX_POINTS)
 
     def set_chunk(self, chunk):
         _check(lib().gf_batch_set_chunk(self.h, chunk))

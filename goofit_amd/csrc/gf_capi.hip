@@ -2710,14 +2710,20 @@ struct gf_batch {
     hipGraphExec_t exec = nullptr;
     int graph_metric = -1, graph_chunk = 0;
     bool use_graph = true;
-    // several points per toy: pinned stage = table (int pairs {first slot, points}
-    // per toy, ntoys doubles) then slots of np + nn doubles; device copy alike
-    double *h_pts = nullptr, *d_pts = nullptr;
-    size_t pts_cap = 0;  // slots
-    int pts_slots = 0, pts_last = -1, pts_max = 0;
-    gfk::Partial *d_pts_partials = nullptr;  // [nblocks][GF_BATCH_PTS]
-    int pts_partials_blocks = 0;
-    double *h_pts_res = nullptr, *d_pts_res = nullptr;  // mapped [ntoys][GF_BATCH_PTS][2]
+    // several points per toy, two buffers (one filled while the other's launch runs):
+    // pinned stage = table (int pairs {first slot, points} per toy, ntoys doubles)
+    // then slots of np + nn doubles; device copy alike
+    struct Points {
+        double *h = nullptr, *d = nullptr;
+        size_t cap = 0;  // slots
+        int slots = 0, last = -1, max = 0;
+        gfk::Partial *d_partials = nullptr;  // [nblocks][GF_BATCH_PTS]
+        int partials_blocks = 0;
+        double *h_res = nullptr, *d_res = nullptr;  // mapped [ntoys][GF_BATCH_PTS][2]
+        bool in_flight = false;
+        uint32_t ticket = 0;  // stamp value of its launch
+    } pts[2];
+    int pts_cur = 0;  // the buffer gf_batch_add_point fills
 };
 static_assert(GF_BATCH_PTS == GF_BATCH_MAX_POINTS, "point limits of the C ABI and the kernels");
 
@@ -2790,11 +2796,13 @@ gf_status batch_enqueue(gf_batch *b, int metric) {
     return GF_OK;
 }
 
-gf_status batch_wait(gf_batch *b) {
+// Wait until the launch stamped `target` (or a later one on the stream) has finished.
+gf_status batch_wait(gf_batch *b, uint32_t target) {
     const volatile uint32_t *stamp = b->h_stamp;
-    const auto t0 = std::chrono::steady_clock::now();
+    auto reached                   = [&] { return (int32_t)(*stamp - target) >= 0; };
+    const auto t0                  = std::chrono::steady_clock::now();
     for(unsigned it = 0;; ++it) {
-        if(*stamp == b->expect)
+        if(reached())
             return GF_OK;
         if((it & 255u) == 255u
            && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(b->ctx->tune[GF_TUNE_SPIN_US]))
@@ -2802,10 +2810,11 @@ gf_status batch_wait(gf_batch *b) {
         __builtin_ia32_pause();
     }
     GF_HIP(hipStreamSynchronize(b->ctx->stream));
-    if(*stamp != b->expect)
+    if(!reached())
         return fail(GF_ERR_HIP, "batched reduction finished without its stamp");
     return GF_OK;
 }
+gf_status batch_wait(gf_batch *b) { return batch_wait(b, b->expect); }
 
 gf_status batch_run(gf_batch *b, int metric) {
     if(!b->use_graph) {
@@ -2902,11 +2911,12 @@ void gf_batch_destroy(gf_batch *b) {
         (void)hipStreamSynchronize(b->ctx->stream);
     }
     batch_drop_graph(b);
-    void *dev[] = {b->d_stage, b->d_first, b->d_work, b->d_partials, b->d_counter, b->d_pts, b->d_pts_partials};
+    void *dev[] = {b->d_stage,    b->d_first, b->d_work, b->d_partials, b->d_counter, b->pts[0].d, b->pts[0].d_partials,
+                   b->pts[1].d, b->pts[1].d_partials};
     for(void *p : dev)
         if(p)
             (void)hipFree(p);
-    void *host[] = {b->h_stage, b->h_res, b->h_stamp, b->h_pts, b->h_pts_res};
+    void *host[] = {b->h_stage, b->h_res, b->h_stamp, b->pts[0].h, b->pts[0].h_res, b->pts[1].h, b->pts[1].h_res};
     for(void *p : host)
         if(p)
             (void)hipHostFree(p);
@@ -2946,37 +2956,46 @@ gf_status gf_batch_reduce(gf_batch *b, const uint8_t *active, int metric, double
 
 namespace {
 
-int *batch_table(gf_batch *b) { return (int *)b->h_pts; }
+using Points = gf_batch::Points;
 
-// Room for `slots` points in the pinned stage and its device copy (contents kept).
-gf_status batch_points_reserve(gf_batch *b, size_t slots) {
-    if(slots <= b->pts_cap && b->h_pts)
+int *batch_table(Points &q) { return (int *)q.h; }
+
+// Room for `slots` points in the pinned stage of q and its device copy (contents kept).
+gf_status batch_points_reserve(gf_batch *b, Points &q, size_t slots) {
+    if(slots <= q.cap && q.h)
         return GF_OK;
-    size_t cap = std::max<size_t>(b->pts_cap ? b->pts_cap : (size_t)std::max(b->ntoys, 256), 1);
+    size_t cap = std::max<size_t>(q.cap ? q.cap : (size_t)std::max(b->ntoys, 256), 1);
     while(cap < slots)
         cap *= 2;
     const size_t W     = (size_t)(b->np + b->nn);
     const size_t bytes = ((size_t)b->ntoys + cap * W) * sizeof(double);
     double *h = nullptr, *d = nullptr;
-    if(hipHostMalloc((void **)&h, bytes, hipHostMallocDefault) != hipSuccess) {
+    if(hipHostMalloc((void **)&h, bytes, hipHostMallocDefault) != hipSuccess)
         return fail(GF_ERR_NOMEM, "cannot allocate a stage of %zu points", cap);
-    }
     if(hipMalloc((void **)&d, bytes) != hipSuccess) {
         (void)hipHostFree(h);
         return fail(GF_ERR_NOMEM, "cannot allocate a device stage of %zu points", cap);
     }
-    if(b->h_pts) {
-        memcpy(h, b->h_pts, ((size_t)b->ntoys + (size_t)b->pts_slots * W) * sizeof(double));
+    if(q.h) {
+        memcpy(h, q.h, ((size_t)b->ntoys + (size_t)q.slots * W) * sizeof(double));
         GF_HIP(hipStreamSynchronize(b->ctx->stream));  // no copy from the old stage in flight
-        (void)hipHostFree(b->h_pts);
-        (void)hipFree(b->d_pts);
+        (void)hipHostFree(q.h);
+        (void)hipFree(q.d);
     } else {
         memset(h, 0, (size_t)b->ntoys * sizeof(double));
     }
-    b->h_pts   = h;
-    b->d_pts   = d;
-    b->pts_cap = cap;
+    q.h   = h;
+    q.d   = d;
+    q.cap = cap;
     return GF_OK;
+}
+
+void batch_points_clear(gf_batch *b, Points &q) {
+    if(q.h)
+        memset(q.h, 0, (size_t)b->ntoys * sizeof(double));
+    q.slots = 0;
+    q.last  = -1;
+    q.max   = 0;
 }
 
 }  // namespace
@@ -2986,40 +3005,40 @@ gf_status gf_batch_add_point(gf_batch *b, int toy, const double *params, const d
         return fail(GF_ERR_INVALID, "toy index out of range");
     if(!params || !norms)
         return fail(GF_ERR_INVALID, "null parameter or normalisation array");
-    gf_status st = batch_points_reserve(b, (size_t)b->pts_slots + 1);
+    Points &q = b->pts[b->pts_cur];
+    if(q.in_flight)
+        return fail(GF_ERR_STATE, "two submissions in flight: wait for the earlier one first");
+    gf_status st = batch_points_reserve(b, q, (size_t)q.slots + 1);
     if(st)
         return st;
-    int *tbl = batch_table(b);
-    if(tbl[2 * toy + 1] > 0 && b->pts_last != toy)
+    int *tbl = batch_table(q);
+    if(tbl[2 * toy + 1] > 0 && q.last != toy)
         return fail(GF_ERR_INVALID, "the points of toy %d must be added one after the other", toy);
     if(tbl[2 * toy + 1] >= GF_BATCH_PTS)
         return fail(GF_ERR_INVALID, "toy %d already has %d points", toy, GF_BATCH_PTS);
     if(tbl[2 * toy + 1] == 0)
-        tbl[2 * toy] = b->pts_slots;
+        tbl[2 * toy] = q.slots;
     const size_t W = (size_t)(b->np + b->nn);
-    double *slot   = b->h_pts + b->ntoys + (size_t)b->pts_slots * W;
+    double *slot   = q.h + b->ntoys + (size_t)q.slots * W;
     memcpy(slot, params, (size_t)b->np * sizeof(double));
     memcpy(slot + b->np, norms, (size_t)b->nn * sizeof(double));
     tbl[2 * toy + 1]++;
-    b->pts_slots++;
-    b->pts_last = toy;
-    b->pts_max  = std::max(b->pts_max, tbl[2 * toy + 1]);
+    q.slots++;
+    q.last = toy;
+    q.max  = std::max(q.max, tbl[2 * toy + 1]);
     return GF_OK;
 }
 
 gf_status gf_batch_clear_points(gf_batch *b) {
     if(!b)
         return fail(GF_ERR_INVALID, "null batch");
-    if(b->h_pts)
-        memset(b->h_pts, 0, (size_t)b->ntoys * sizeof(double));
-    b->pts_slots = 0;
-    b->pts_last  = -1;
-    b->pts_max   = 0;
+    if(!b->pts[b->pts_cur].in_flight)  // a buffer in flight holds no unsubmitted points
+        batch_points_clear(b, b->pts[b->pts_cur]);
     return GF_OK;
 }
 
-gf_status gf_batch_reduce_points(gf_batch *b, int metric, double *out) {
-    if(!b || !out)
+gf_status gf_batch_submit_points(gf_batch *b, int metric, uint32_t *ticket) {
+    if(!b || !ticket)
         return fail(GF_ERR_INVALID, "null argument");
     if(!valid_ctx(b->ctx))
         return GF_ERR_INVALID;
@@ -3029,30 +3048,32 @@ gf_status gf_batch_reduce_points(gf_batch *b, int metric, double *out) {
         return st;
     if(b->np != (int)ctx->params.size() || b->nn != (int)ctx->norms.size())
         return fail(GF_ERR_STATE, "the context's program changed since the batch was created");
-    if(b->pts_slots == 0)
-        return GF_OK;
+    Points &q = b->pts[b->pts_cur];
+    if(q.in_flight)
+        return fail(GF_ERR_STATE, "two submissions in flight: wait for the earlier one first");
+    if(q.slots == 0)
+        return fail(GF_ERR_STATE, "no points added since the last submission");
     if((st = flush_uploads(ctx)))
         return st;
-    if(b->pts_partials_blocks != b->nblocks) {
+    if(q.partials_blocks != b->nblocks) {
         GF_HIP(hipStreamSynchronize(ctx->stream));
-        if(b->d_pts_partials)
-            GF_HIP(hipFree(b->d_pts_partials));
-        b->d_pts_partials      = nullptr;
-        b->pts_partials_blocks = 0;
-        if(hipMalloc((void **)&b->d_pts_partials, (size_t)b->nblocks * GF_BATCH_PTS * sizeof(gfk::Partial))
-           != hipSuccess) {
-            b->d_pts_partials = nullptr;
+        if(q.d_partials)
+            GF_HIP(hipFree(q.d_partials));
+        q.d_partials      = nullptr;
+        q.partials_blocks = 0;
+        if(hipMalloc((void **)&q.d_partials, (size_t)b->nblocks * GF_BATCH_PTS * sizeof(gfk::Partial)) != hipSuccess) {
+            q.d_partials = nullptr;
             return fail(GF_ERR_NOMEM, "cannot allocate the partials of %d workgroups", b->nblocks);
         }
-        b->pts_partials_blocks = b->nblocks;
+        q.partials_blocks = b->nblocks;
     }
-    if(!b->h_pts_res) {
-        if(hipHostMalloc((void **)&b->h_pts_res, (size_t)2 * b->ntoys * GF_BATCH_PTS * sizeof(double),
+    if(!q.h_res) {
+        if(hipHostMalloc((void **)&q.h_res, (size_t)2 * b->ntoys * GF_BATCH_PTS * sizeof(double),
                          hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess
-           || hipHostGetDevicePointer((void **)&b->d_pts_res, b->h_pts_res, 0) != hipSuccess) {
-            if(b->h_pts_res)
-                (void)hipHostFree(b->h_pts_res);
-            b->h_pts_res = b->d_pts_res = nullptr;
+           || hipHostGetDevicePointer((void **)&q.d_res, q.h_res, 0) != hipSuccess) {
+            if(q.h_res)
+                (void)hipHostFree(q.h_res);
+            q.h_res = q.d_res = nullptr;
             return fail(GF_ERR_NOMEM, "cannot allocate the batch results");
         }
     }
@@ -3060,26 +3081,59 @@ gf_status gf_batch_reduce_points(gf_batch *b, int metric, double *out) {
     const bool fast = use_fast_dalitz(ctx, metric);
     const bool ce   = fast && const_efficiency(ctx);
     bool zo         = ce && zero_outside(ctx);
-    for(int k = 0; k < b->pts_slots && zo; k++)  // every point's efficiency constant finite
-        zo = std::isfinite(b->h_pts[b->ntoys + (size_t)k * W + ctx->ins[1].p + ctx->ins[1].a]);
-    GF_HIP(hipMemcpyAsync(b->d_pts, b->h_pts, ((size_t)b->ntoys + (size_t)b->pts_slots * W) * sizeof(double),
-                          hipMemcpyHostToDevice, ctx->stream));
-    const int groups = (b->pts_max + GF_MULTI_PTS - 1) / GF_MULTI_PTS;
+    for(int k = 0; k < q.slots && zo; k++)  // every point's efficiency constant finite
+        zo = std::isfinite(q.h[b->ntoys + (size_t)k * W + ctx->ins[1].p + ctx->ins[1].a]);
+    GF_HIP(hipMemcpyAsync(q.d, q.h, ((size_t)b->ntoys + (size_t)q.slots * W) * sizeof(double), hipMemcpyHostToDevice,
+                          ctx->stream));
+    const int groups = (q.max + GF_MULTI_PTS - 1) / GF_MULTI_PTS;
     GfProgArgs a     = prog_args(ctx);
-    const int nres = fast ? ctx->dalitz[0].nres : 0;
-    if(gfk::launch_batch_points(a, nres, b->d_work, b->nblocks, groups, b->d_pts + b->ntoys, b->np, b->nn,
-                                (const int *)b->d_pts, fast, ce, zo, metric, b->d_pts_partials, b->d_first,
-                                b->ntoys, b->d_pts_res, b->d_counter, b->d_stamp, ctx->stream))
+    const int nres   = fast ? ctx->dalitz[0].nres : 0;
+    if(gfk::launch_batch_points(a, nres, b->d_work, b->nblocks, groups, q.d + b->ntoys, b->np, b->nn,
+                                (const int *)q.d, fast, ce, zo, metric, q.d_partials, b->d_first, b->ntoys, q.d_res,
+                                b->d_counter, b->d_stamp, ctx->stream))
         return fail(GF_ERR_HIP, "batch launch failed");
-    b->expect++;
-    if((st = batch_wait(b)))
+    q.ticket    = ++b->expect;
+    q.in_flight = true;
+    *ticket     = q.ticket;
+    b->pts_cur ^= 1;
+    return GF_OK;
+}
+
+gf_status gf_batch_wait_points(gf_batch *b, uint32_t ticket, double *out) {
+    if(!b || !out)
+        return fail(GF_ERR_INVALID, "null argument");
+    if(!valid_ctx(b->ctx))
+        return GF_ERR_INVALID;
+    Points *q = nullptr;
+    for(Points &c : b->pts)
+        if(c.in_flight && c.ticket == ticket)
+            q = &c;
+    if(!q)
+        return fail(GF_ERR_INVALID, "no submission with ticket %u in flight", ticket);
+    gf_status st = batch_wait(b, ticket);
+    if(st)
         return st;
-    const int *tbl = batch_table(b);
-    const volatile double *res = b->h_pts_res;
+    // the stage and its copy are no longer in use either: the launch has finished
+    const int *tbl             = batch_table(*q);
+    const volatile double *res = q->h_res;
     for(int t = 0; t < b->ntoys; t++)
         for(int k = 0; k < tbl[2 * t + 1]; k++)
             out[(size_t)t * GF_BATCH_PTS + k] = res[2 * ((size_t)t * GF_BATCH_PTS + k)];
-    return gf_batch_clear_points(b);
+    q->in_flight = false;
+    batch_points_clear(b, *q);
+    return GF_OK;
+}
+
+gf_status gf_batch_reduce_points(gf_batch *b, int metric, double *out) {
+    if(!b || !out)
+        return fail(GF_ERR_INVALID, "null argument");
+    if(b->pts[b->pts_cur].slots == 0)
+        return GF_OK;
+    uint32_t ticket = 0;
+    gf_status st    = gf_batch_submit_points(b, metric, &ticket);
+    if(st)
+        return st;
+    return gf_batch_wait_points(b, ticket, out);
 }
 
 gf_status gf_batch_set_chunk(gf_batch *b, int chunk) {

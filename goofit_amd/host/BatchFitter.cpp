@@ -199,6 +199,57 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
                               "gf_batch_set_toy");
         }
     };
+    // With several points per toy the launches are pipelined: a stage goes to the GPU
+    // once half of the running toys are prepared (or the round is complete), and the
+    // next stage is prepared while it runs; at most one launch is in flight besides
+    // the buffer being filled, and the batcher never blocks on new requests while a
+    // launch is in flight.  When preparing is cheap against the GPU work (few points,
+    // simple trees) a round stays one launch.
+    struct Flight {
+        uint32_t ticket = 0;
+        std::vector<int> serve;
+        int64_t points = 0;
+        bool failed    = false;
+    };
+    Flight flight;
+    bool in_flight     = false;
+    bool split         = false;
+    bool split_decided = false;
+    auto deliver = [&](const std::vector<int> &who, bool failed) {
+        {
+            std::lock_guard<std::mutex> lk(B.mu);
+            for(int t : who) {
+                if(failed)
+                    B.result[t].assign(B.xn[t], 1e300);
+                else if(multi_)
+                    B.result[t].assign(out.begin() + (size_t)t * GF_BATCH_MAX_POINTS,
+                                       out.begin() + (size_t)t * GF_BATCH_MAX_POINTS + B.xn[t]);
+                else
+                    B.result[t].assign(1, out[t]);
+                B.pending[t] = 0;
+                B.done[t]    = 1;
+                B.waiting--;
+                prepared[t] = 0;
+                act[t]      = 0;
+            }
+        }
+        B.cv_done.notify_all();
+    };
+    auto land = [&]() {  // wait for the launch in flight and answer its toys
+        const auto g0 = std::chrono::steady_clock::now();
+        bool failed   = flight.failed;
+        if(!failed) {
+            try {
+                engine::check(gf_batch_wait_points(batch_, flight.ticket, out.data()), "gf_batch_wait_points");
+            } catch(...) {
+                failure = std::current_exception();
+                failed  = true;
+            }
+        }
+        stats_.gpu_wait_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - g0).count();
+        deliver(flight.serve, failed);
+        in_flight = false;
+    };
     for(;;) {
         taken.clear();
         bool launch = false;
@@ -207,13 +258,15 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
             const auto w0 = std::chrono::steady_clock::now();
             // the requests of the toys blocked in an FCN call are all in hand only when
             // every running toy waits; new ones may still come until then
-            const bool arrived = B.cv_req.wait_for(
-                lk, serve.empty() ? std::chrono::microseconds(50000) : std::chrono::microseconds(200), [&] {
-                    return B.unprepared > 0 || (!serve.empty() && B.waiting == B.running)
-                           || (B.running == 0 && next >= T);
-                });
+            bool arrived = B.unprepared > 0;
+            if(!in_flight)
+                arrived = B.cv_req.wait_for(
+                    lk, serve.empty() ? std::chrono::microseconds(50000) : std::chrono::microseconds(200), [&] {
+                        return B.unprepared > 0 || (!serve.empty() && B.waiting == B.running)
+                               || (B.running == 0 && next >= T);
+                    });
             stats_.gather_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
-            if(B.running == 0 && next >= T && serve.empty())
+            if(B.running == 0 && next >= T && serve.empty() && !in_flight)
                 break;
             for(int t = 0; t < T; t++)
                 if(B.pending[t] && !prepared[t]) {
@@ -221,7 +274,18 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
                     taken.push_back(t);
                 }
             B.unprepared -= (int)taken.size();
-            launch = !(serve.empty() && taken.empty()) && (B.waiting == B.running || !arrived);
+            // split rounds in two stages when preparing costs more than the GPU work:
+            // then the GPU work of one stage hides behind the preparation of the other,
+            // which outweighs the second launch's fixed cost (its grid still spans every
+            // toy's rows).  Decided once, from the first twelve rounds, run unsplit (the
+            // first ones, single points, would overstate the preparation)
+            if(multi_ && !split_decided && stats_.launches >= 12) {
+                split         = stats_.host_prep_s > stats_.gpu_wait_s;
+                split_decided = true;
+            }
+            const size_t stage = split ? (size_t)std::max(1, B.running / 2) : (size_t)T + 1;
+            const size_t ready = serve.size() + taken.size();
+            launch = ready > 0 && (B.waiting == B.running || (!in_flight && !arrived) || ready >= stage);
         }
         // the taken toys' requests stay untouched while those toys wait: prepare them
         // without holding the lock
@@ -240,44 +304,45 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
             serve.push_back(t);
         }
         stats_.host_prep_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - p0).count();
-        if(!launch)
+        if(!launch) {
+            if(in_flight && taken.empty())
+                land();
             continue;
-        const auto g0 = std::chrono::steady_clock::now();
-        if(!round_failed) {
-            try {
-                if(multi_)
-                    engine::check(gf_batch_reduce_points(batch_, GF_METRIC_NLL, out.data()), "gf_batch_reduce_points");
-                else
-                    engine::check(gf_batch_reduce(batch_, act.data(), GF_METRIC_NLL, out.data()), "gf_batch_reduce");
-            } catch(...) {
-                failure      = std::current_exception();
-                round_failed = true;
-            }
         }
-        if(round_failed) {
-            if(multi_)
-                (void)gf_batch_clear_points(batch_);
-            std::fill(out.begin(), out.end(), 1e300);
-        }
-        stats_.gpu_wait_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - g0).count();
+        if(in_flight)
+            land();  // frees the other point buffer
         stats_.launches++;
         stats_.fcn_calls += points;
-        {
-            std::lock_guard<std::mutex> lk(B.mu);
-            for(int t : serve) {
-                if(multi_)
-                    B.result[t].assign(out.begin() + (size_t)t * GF_BATCH_MAX_POINTS,
-                                       out.begin() + (size_t)t * GF_BATCH_MAX_POINTS + B.xn[t]);
-                else
-                    B.result[t].assign(1, out[t]);
-                B.pending[t] = 0;
-                B.done[t]    = 1;
-                B.waiting--;
-                prepared[t] = 0;
-                act[t]      = 0;
+        if(multi_) {
+            flight.serve.swap(serve);
+            flight.points = points;
+            flight.failed = round_failed;
+            if(!round_failed) {
+                try {
+                    engine::check(gf_batch_submit_points(batch_, GF_METRIC_NLL, &flight.ticket),
+                                  "gf_batch_submit_points");
+                } catch(...) {
+                    failure       = std::current_exception();
+                    round_failed  = true;
+                    flight.failed = true;
+                }
             }
+            if(flight.failed)
+                (void)gf_batch_clear_points(batch_);
+            in_flight = true;
+        } else {
+            const auto g0 = std::chrono::steady_clock::now();
+            if(!round_failed) {
+                try {
+                    engine::check(gf_batch_reduce(batch_, act.data(), GF_METRIC_NLL, out.data()), "gf_batch_reduce");
+                } catch(...) {
+                    failure      = std::current_exception();
+                    round_failed = true;
+                }
+            }
+            stats_.gpu_wait_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - g0).count();
+            deliver(serve, round_failed);
         }
-        B.cv_done.notify_all();
         serve.clear();
         points = 0;
     }

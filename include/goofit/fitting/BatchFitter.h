@@ -6,7 +6,9 @@
 // of independent points (one derivative round, up to 64 points), posts the toy's
 // parameter vectors to a batcher, which turns the pending requests of all toys
 // into one GPU launch sequence (staging copy, per-toy event sweep over all of a
-// toy's points, per-toy finishing reduction) and hands each toy its NLLs.  With
+// toy's points, per-toy finishing reduction) and hands each toy its NLLs; when
+// preparing the requests costs more than the GPU work, a round goes out in two
+// stages so that one is evaluated while the other is prepared.  With
 // useMultiPoint(false) every point is its own request, served by a captured
 // hipGraph with one parameter set per toy.  The normalisations
 // are computed on the host per toy (analytic integrals, or an Amp3Body's cached

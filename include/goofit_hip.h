@@ -390,6 +390,14 @@ gf_status gf_batch_autotune(gf_batch *b, int metric, int *chunk_out, double *us_
 gf_status gf_batch_add_point(gf_batch *b, int toy, const double *params, const double *norms);
 gf_status gf_batch_reduce_points(gf_batch *b, int metric, double *out);
 gf_status gf_batch_clear_points(gf_batch *b);
+/* The two halves of gf_batch_reduce_points: submit launches the added points and
+ * returns at once with a ticket; points added after it go to a second buffer, so
+ * the next request can be staged while the launch runs.  At most two submissions
+ * are in flight (adding or submitting while both are fails until one is waited
+ * for); wait blocks until the ticket's launch has finished and writes its sums as
+ * gf_batch_reduce_points does. */
+gf_status gf_batch_submit_points(gf_batch *b, int metric, uint32_t *ticket);
+gf_status gf_batch_wait_points(gf_batch *b, uint32_t ticket, double *out);
 
 /* ---- timing ------------------------------------------------------------- */
 

@@ -287,3 +287,51 @@ def test_batch_fitter_multi_point_equals_single_point():
     assert all(out[True][2])
     assert out[True][3]["fcn_calls"] == out[False][3]["fcn_calls"]
     assert out[True][3]["launches"] < out[False][3]["launches"]
+
+
+def test_batch_points_submit_and_wait(ctx):
+    """gf_batch_submit_points / gf_batch_wait_points: two submissions in flight (the
+    second staged while the first runs), a third refused until one is waited for, waits
+    in either order, the same bits as gf_batch_reduce_points, stale tickets refused."""
+    counts = [3000, 5000, 4000]
+    data = _toys(counts, 61)
+    first = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    top, _ = add1d()
+    f, p, c, o, n = O.flatten(top).arrays()
+    ctx.set_program(f, p, c, o, n)
+    ctx.upload_columns([np.concatenate(data)])
+    b = capi.Batch(ctx, first, counts)
+    rng = np.random.default_rng(21)
+    sets = []
+    for _ in range(6):
+        m = O.Model(_add1d_point(rng), np.zeros((1, 1)))
+        m.normalize()
+        m.op.sync_params()
+        m.op.sync_norms()
+        sets.append((m.op.p.copy(), m.op.n.copy()))
+    try:
+        plan1 = [(0, sets[0]), (0, sets[1]), (1, sets[2])]
+        plan2 = [(2, sets[3]), (2, sets[4]), (2, sets[5])]
+        for t, (pp, nn) in plan1:
+            b.add_point(t, pp, nn)
+        t1 = b.submit_points()
+        for t, (pp, nn) in plan2:
+            b.add_point(t, pp, nn)
+        t2 = b.submit_points()
+        with pytest.raises(RuntimeError, match="two submissions in flight"):
+            b.add_point(1, *sets[0])
+        with pytest.raises(RuntimeError, match="two submissions in flight"):
+            b.submit_points()
+        b.clear_points()  # no-op on a buffer in flight
+        r2 = b.wait_points(t2)
+        r1 = b.wait_points(t1)
+        with pytest.raises(RuntimeError, match="no submission"):
+            b.wait_points(t1)
+        for plan, r in ((plan1, r1), (plan2, r2)):
+            for t, (pp, nn) in plan:
+                b.add_point(t, pp, nn)
+            ref = b.reduce_points()
+            np.testing.assert_array_equal(np.nan_to_num(r, nan=-1.0), np.nan_to_num(ref, nan=-1.0))
+        assert np.isfinite(r1[0, :2]).all() and np.isfinite(r1[1, 0]) and np.isnan(r1[2]).all()
+    finally:
+        b.close()
