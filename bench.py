@@ -56,7 +56,9 @@ def measured_traffic(name="dalitz16_1e6_hbm_counters.md"):
 
 def time_steps(pdf, params, steps, warmup, sync):
     """Time `steps` FCN calls (Params::from_minuit_vector + calculateNLL), each with
-    slightly moved free parameters: exactly what MIGRAD requests per call."""
+    slightly moved free parameters: exactly what MIGRAD requests per call, made one after
+    the other from C++ (FitManager.time_calls) as the minimiser makes them; the vectors
+    are built before the timed region."""
     import goofit_amd as G
     fm = G.FitManager(pdf)
     x0 = np.asarray(fm.minuit_vector(), float)
@@ -68,12 +70,11 @@ def time_steps(pdf, params, steps, warmup, sync):
         x[idx] = x0[idx] * (1.0 + 1e-3 * np.sin(0.37 * k + 1.3 * np.arange(idx.size)))
         return x.tolist()
 
-    for k in range(warmup):
-        fm.evaluate(vec(k))
+    fm.time_calls([vec(k) for k in range(warmup)])
+    timed = [vec(warmup + k) for k in range(steps)]
     sync()
     t0 = time.perf_counter()
-    for k in range(steps):
-        fm.evaluate(vec(warmup + k))
+    fm.time_calls(timed)
     sync()
     t1 = time.perf_counter()
     fm.evaluate(x0.tolist())

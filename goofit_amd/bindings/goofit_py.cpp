@@ -4,6 +4,8 @@
 #include <pybind11/complex.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
+
+#include <chrono>
 #include <pybind11/stl.h>
 
 #include <sstream>
@@ -590,6 +592,19 @@ PYBIND11_MODULE(_goofit, m) {
              })
         .def("evaluate", [](FitManagerMinuit2 &f, const std::vector<double> &x) { return (*f.getFCN())(x); },
              "One FCN call: push the Minuit vector into the PDF and return its NLL")
+        .def(
+            "time_calls",
+            [](FitManagerMinuit2 &f, const std::vector<std::vector<double>> &xs) {
+                // one FCN call per vector, in order, from C++ as the minimiser makes them
+                const auto &fcn = *f.getFCN();
+                std::vector<double> out(xs.size());
+                const auto t0 = std::chrono::steady_clock::now();
+                for(size_t k = 0; k < xs.size(); k++)
+                    out[k] = fcn(xs[k]);
+                const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                return py::make_tuple(s, out);
+            },
+            py::arg("xs"), "Seconds for one FCN call per Minuit vector, made one after the other, and the NLLs")
         .def("__bool__", [](const FitManagerMinuit2 &f) { return (bool)f; })
         .def("__int__", [](const FitManagerMinuit2 &f) { return (int)f; });
 
