@@ -291,6 +291,9 @@ def fit_toys(model, data, counts, truth, sync, seq_toys, threads):
                 pull_width=[round(float(v), 4) for v in pw],
                 max_abs_pull_mean=round(float(np.abs(pm).max()), 4),
                 max_abs_pull_width_minus_1=round(float(np.abs(pw - 1).max()), 4),
+                # the same in units of their statistical errors, 1/sqrt(N) and 1/sqrt(2(N-1))
+                max_pull_mean_over_error=round(float(np.abs(pm).max() * math.sqrt(len(pulls))), 2),
+                max_pull_width_dev_over_error=round(float(np.abs(pw - 1).max() * math.sqrt(2 * (len(pulls) - 1))), 2),
                 pull_width_robust=[round(float(v), 4) for v in pw_robust],
                 pull_width_large_toys=[round(float(v), 4) for v in pw_big],
                 toys_with_pull_beyond_5=n_tail,
