@@ -12,6 +12,7 @@
 
 #include <chrono>
 #include <condition_variable>
+#include <memory>
 #include <mutex>
 #include <thread>
 
@@ -22,7 +23,8 @@ namespace {
 // Shared state of one fit(): pending FCN requests and their answers.
 struct Batcher {
     std::mutex mu;
-    std::condition_variable cv_req, cv_done;
+    std::condition_variable cv_req;
+    std::unique_ptr<std::condition_variable[]> cv_toy;  // one per toy: a delivery wakes only its toys
     std::vector<const std::vector<std::vector<double>> *> x;  // per toy: the requested Minuit vectors
     std::vector<size_t> x0, xn;                               // of those, the points of this request
     std::vector<uint8_t> pending, done;
@@ -69,7 +71,7 @@ class ToyFCN : public Minuit2::FCNBase {
         b_.waiting++;
         b_.unprepared++;
         b_.cv_req.notify_one();
-        b_.cv_done.wait(lk, [&] { return b_.done[toy_] != 0; });
+        b_.cv_toy[toy_].wait(lk, [&] { return b_.done[toy_] != 0; });
         return b_.result[toy_];
     }
     Batcher &b_;
@@ -125,6 +127,7 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
         engine::check(gf_batch_autotune(batch_, GF_METRIC_NLL, &stats_.chunk, nullptr), "gf_batch_autotune");
     }
     Batcher B;
+    B.cv_toy.reset(new std::condition_variable[T]);
     B.x.assign(T, nullptr);
     B.x0.assign(T, 0);
     B.xn.assign(T, 0);
@@ -233,7 +236,8 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
                 act[t]      = 0;
             }
         }
-        B.cv_done.notify_all();
+        for(int t : who)
+            B.cv_toy[t].notify_one();
     };
     auto land = [&]() {  // wait for the launch in flight and answer its toys
         const auto g0 = std::chrono::steady_clock::now();
