@@ -311,9 +311,11 @@ def fit_toys(model, data, counts, truth, sync, seq_toys, threads):
 TOY_GRID = 1600
 
 
-def toy_study(ntoys, sync, seq_toys=16, threads=256):
+def toy_study(ntoys, sync, seq_toys=16, threads=256, add1d_threads=512):
     """cfg: ntoys/2 toys of the 1-D AddPdf model and ntoys/2 of an 8-resonance cut of the Dalitz model,
-    Poisson-varied sizes, fitted in batches and (a few) sequentially."""
+    Poisson-varied sizes, fitted in batches (512 / 256 concurrent Minuit instances: the AddPdf
+    fits are GPU-bound and gain from fuller launches, the Dalitz fits host-bound) and (a few)
+    sequentially."""
     import goofit_amd as G
     from goofit_amd import workloads as W
     half = ntoys // 2
@@ -328,7 +330,7 @@ def toy_study(ntoys, sync, seq_toys=16, threads=256):
     d = G.UnbinnedDataSet(m.observables)
     d.from_columns([xs])
     m.pdf.setData(d)
-    out["add1d"] = fit_toys(m, d, counts, truth, sync, seq_toys, threads)
+    out["add1d"] = fit_toys(m, d, counts, truth, sync, seq_toys, add1d_threads)
     TOY_RAW["add1d"] = out["add1d"].pop("_raw")
     out["add1d"]["generation_s"] = round(t_gen, 2)
     log("add1d toys", {k: out["add1d"][k] for k in ("batched_s", "batched_toys_per_hour",
