@@ -246,8 +246,9 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
             try {
                 engine::check(gf_batch_wait_points(batch_, flight.ticket, out.data()), "gf_batch_wait_points");
             } catch(...) {
-                failure = std::current_exception();
-                failed  = true;
+                failure      = std::current_exception();
+                failed       = true;
+                round_failed = true;  // later requests are answered without launches
             }
         }
         stats_.gpu_wait_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - g0).count();
