@@ -2860,8 +2860,6 @@ gf_status gf_batch_create(gf_ctx *ctx, int ntoys, const int64_t *first, const in
         return fail(GF_ERR_INVALID, "need at least one toy");
     if(ctx->ins.empty())
         return fail(GF_ERR_STATE, "no program installed (gf_program_set)");
-    if(ctx->has_td)
-        return fail(GF_ERR_INVALID, "toy batches do not take Amp3Body_TD programs");
     for(int t = 0; t < ntoys; t++)
         if(first[t] < 0 || count[t] < 0 || first[t] + count[t] > ctx->n)
             return fail(GF_ERR_INVALID, "toy %d rows [%lld, %lld) outside the %lld events", t, (long long)first[t],

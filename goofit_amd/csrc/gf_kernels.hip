@@ -933,8 +933,7 @@ __device__ __forceinline__ void batch_rows(const GfProgArgs &A, const GfBatchWor
 // Metric sums of many toys in one launch: workgroup b evaluates the rows of
 // work[b] with its toy's parameter and normalisation arrays (P = params +
 // toy * np, Nm = norms + toy * nn) and stores one partial; workgroups of
-// inactive toys store zero.  FEAT as for k_reduce_metric (toy batches hold no
-// time-dependent node).
+// inactive toys store zero.  FEAT as for k_reduce_metric.
 template <int EPT, int FEAT>
 __global__ __launch_bounds__(GF_BLOCK) void k_batch_metric(GfProgArgs A, const GfBatchWork *__restrict__ work,
                                                            const double *__restrict__ params, int np,
@@ -1531,12 +1530,20 @@ int launch_batch(const GfProgArgs &a, const GfBatchWork *work, int nblocks, cons
                  const double *norms, int nn, const uint8_t *active, int metric, Partial *partials, const int *first,
                  int ntoys, double *result, uint32_t *counter, uint32_t *stamp, void *stream) {
     hipStream_t s = (hipStream_t)stream;
-    if(a.has_dz)
-        hipLaunchKernelGGL((k_batch_metric<2, 1>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, work, params, np, norms, nn,
-                           active, metric, partials);
-    else
+    switch(feature_level(a)) {
+    case 0:
         hipLaunchKernelGGL((k_batch_metric<2, 0>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, work, params, np, norms, nn,
                            active, metric, partials);
+        break;
+    case 1:
+        hipLaunchKernelGGL((k_batch_metric<2, 1>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, work, params, np, norms, nn,
+                           active, metric, partials);
+        break;
+    default:
+        hipLaunchKernelGGL((k_batch_metric<2, 2>), dim3(nblocks), dim3(GF_BLOCK), 0, s, a, work, params, np, norms, nn,
+                           active, metric, partials);
+        break;
+    }
     int rc = check(hipGetLastError());
     if(rc)
         return rc;
@@ -1555,7 +1562,10 @@ int launch_batch_points(const GfProgArgs &a, int nres, const GfBatchWork *work, 
     if(groups < 1 || groups * GF_MULTI_PTS > GF_BATCH_PTS || (fast && (nres < 1 || nres > GF_MAX_RES)))
         return -1;
     const dim3 grid(nblocks, groups);
-    if(!fast && a.has_dz)
+    if(!fast && feature_level(a) == 2)
+        hipLaunchKernelGGL((k_batch_points<2, 2>), grid, dim3(GF_BLOCK), 0, s, a, work, stage, np, nn, tbl, metric,
+                           partials);
+    else if(!fast && feature_level(a) == 1)
         hipLaunchKernelGGL((k_batch_points<2, 1>), grid, dim3(GF_BLOCK), 0, s, a, work, stage, np, nn, tbl, metric,
                            partials);
     else if(!fast)

@@ -191,3 +191,47 @@ def test_api_tddp_fit_lifetime_and_mixing():
     assert abs(x.value) < 3 * x.error, (x.value, x.error)
     assert abs(y.value) < 3 * y.error, (y.value, y.error)
     assert tau.error < 0.01
+
+
+def test_batch_fitter_tddp_toys_match_sequential_fits():
+    """Amp3Body_TD toys over disjoint rows of one data set, fitted together by
+    BatchFitter (tau, x, y free; the host computes each toy's decay-time integrals),
+    against one FitManager fit per toy from the same start: every value within 0.01
+    sigma."""
+    import goofit_amd as G
+    counts = [6000, 9000, 7000, 8000]
+    n = sum(counts)
+    table = W.DALITZ_RESONANCES[:6]
+    dz = W.build_dalitz(nbins=120, resonances=table, float_couplings=False)
+    m12, m13 = W.generate_dalitz(dz, n, seed=41)
+    model = W.build_tddp(nbins=120, resonances=table, mix=(0.42, 0.002, -0.002, 0.0, 0.0))
+    start = {k: v.value for k, v in model.params.items()}
+    rng = np.random.default_rng(42)
+    t_all, m12, m13 = rng.exponential(0.4101, n), np.asarray(m12), np.asarray(m13)
+
+    def dataset(lo, hi):
+        d = G.UnbinnedDataSet(model.observables)
+        d.from_columns([t_all[lo:hi], np.full(hi - lo, 0.05), m12[lo:hi], m13[lo:hi],
+                        np.arange(hi - lo, dtype=np.float64)])
+        return d
+
+    model.pdf.setData(dataset(0, n))
+    model.pdf.setDataSize(n, 5)
+    first = [int(v) for v in np.concatenate([[0], np.cumsum(counts)[:-1]])]
+    bf = G.BatchFitter(model.pdf, first, counts)
+    bf.fit(threads=4)
+    vals, valid = bf.values(), bf.valid()
+    for t in range(len(counts)):
+        for k, v in start.items():
+            model.params[k].value = v
+        model.pdf.setData(dataset(first[t], first[t] + counts[t]))
+        model.pdf.setDataSize(counts[t], 5)
+        fm = G.FitManager(model.pdf)
+        fm.verbosity = 0
+        r = fm.fit()
+        assert valid[t] and r.IsValid(), t
+        sv, se = np.array(r.values()), np.array(r.errors())
+        free = se > 0
+        assert free.sum() == 3, se
+        shift = np.abs(np.array(vals[t]) - sv)[free] / se[free]
+        assert shift.max() < 0.01, (t, shift)
