@@ -46,8 +46,11 @@
  *   integrals of their event functions, the narrow-resolution limit, the x = y = 0
  *   reduction to the Amp3Body density, the charm-tag and mistag identities, and the
  *   normalisation as the integral of the event density (tests/test_tddp_oracle.py).
- *   Parity unpinned: the GS lineshape away from its pole, RBW spin 1 and 2 away from
- *   the pole (the Blatt-Weisskopf ratios), the MultiPolynomial evaluation order
+ *   RBW spin 1 and 2 away from the pole: the normalised Blatt-Weisskopf ratios
+ *   against the spherical Hankel functions (scipy), with the mother-side momentum
+ *   s / (2 M) that twoBodyCMMothermom's clamped square roots give inside the plot.
+ *   Parity unpinned: the GS lineshape away from its pole, RBW without normalised
+ *   barriers (norm = false) away from the pole, the MultiPolynomial evaluation order
  *   (checked only through degree-0 and degree-1 cases), and quirks A5/A6 beyond
  *   their restatement.
  */

@@ -330,6 +330,44 @@ def test_rbw_pole_angular_factor(pair, spin):
         assert got == pytest.approx(1j * z / (m0 * g0), rel=1e-10, abs=1e-12 / (m0 * g0)), (cos, got)
 
 
+def _barrier(L, x):
+    """x^(2L+2) |h_L(x)|^2, h_L the spherical Hankel function of the first kind, from scipy's
+    spherical Bessel functions: the Blatt-Weisskopf denominator of angular momentum L in
+    x = radius * momentum."""
+    return x ** (2 * L + 2) * (special.spherical_jn(L, x) ** 2 + special.spherical_yn(L, x) ** 2)
+
+
+@pytest.mark.parametrize("pair", [0, 1, 2])
+@pytest.mark.parametrize("spin,m0,g0", [(1, 0.98, 0.07), (1, 1.45, 0.3), (2, 1.27, 0.185)])
+def test_rbw_off_pole_blatt_weisskopf(pair, spin, m0, g0):
+    """plainBW with normalised barriers (RBW.cu:9-86) across the Dalitz plot:
+    Z F_R F_D / (m0^2 - s - i m0 Gamma(s)), Gamma(s) = Gamma0 (q/q0)^(2L+1) (m0/sqrt s) F_R^2.
+    F_R^2 = D_L(r q0) / D_L(r q) with D_L from the spherical Hankel functions and q the pair's
+    breakup momentum; F_D^2 the same in the mother radius with the mother-side momentum of
+    twoBodyCMMothermom (Resonance.cu:62-75), whose clamped square roots make it s / (2 M)
+    inside the Dalitz plot; Z the Zemach factor from four-vectors in the pair's rest frame."""
+    M, r, rm = W.DALITZ_DECAY[0], W.DALITZ_DECAY[4], W.DALITZ_DECAY[5]
+    mass = {1: W.DALITZ_DECAY[1], 2: W.DALITZ_DECAY[2], 3: W.DALITZ_DECAY[3]}
+    a, b, c = PAIR_ABC[pair]
+    lo, hi = (mass[a] + mass[b]) ** 2, (M - mass[c]) ** 2
+    op, rf = _lineshape_prog("RBW", m0, g0, spin, pair)
+    q0 = _p_cm(m0 * m0, mass[a], mass[b])
+    wants, gots = [], []
+    for s in np.linspace(lo + 0.03 * (hi - lo), hi - 0.03 * (hi - lo), 7):
+        for cos in (-0.7, 0.2, 0.85):
+            s12, s13, dot, mag2 = _pole_config(pair, math.sqrt(s), cos)
+            q = _p_cm(s, mass[a], mass[b])
+            fr2 = _barrier(spin, r * q0) / _barrier(spin, r * q)
+            fd2 = _barrier(spin, rm * m0 * m0 / (2 * M)) / _barrier(spin, rm * s / (2 * M))
+            gam = g0 * (q / q0) ** (2 * spin + 1) * (m0 / math.sqrt(s)) * fr2
+            z = -4 * dot if spin == 1 else 16 * (dot * dot - mag2 / 3)
+            wants.append(z * math.sqrt(fr2 * fd2) / complex(m0 * m0 - s, -m0 * gam))
+            gots.append(op.lineshape(rf, s12, s13))
+    scale = max(abs(w) for w in wants)
+    for w, g in zip(wants, gots):
+        assert g == pytest.approx(w, rel=1e-10, abs=1e-12 * scale), (w, g)
+
+
 def test_gs_pole_rho():
     """Gounaris-Sakurai (GS.cu:9-75) for the rho0 -> pi+ pi- (spin 1, pair 23) at s = m0^2:
     i (1 + d(m0^2) Gamma0/m0) Z / (m0 Gamma0), with the GS d-function for two equal-mass pions
