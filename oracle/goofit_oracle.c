@@ -48,9 +48,11 @@
  *   normalisation as the integral of the event density (tests/test_tddp_oracle.py).
  *   RBW spin 1 and 2 away from the pole: the normalised Blatt-Weisskopf ratios
  *   against the spherical Hankel functions (scipy), with the mother-side momentum
- *   s / (2 M) that twoBodyCMMothermom's clamped square roots give inside the plot.
- *   Parity unpinned: the GS lineshape away from its pole, RBW without normalised
- *   barriers (norm = false) away from the pole, the MultiPolynomial evaluation order
+ *   s / (2 M) that twoBodyCMMothermom's clamped square roots give inside the plot;
+ *   GS away from its pole: the published Gounaris-Sakurai f(s) with h' by
+ *   complex-step differentiation and the width term as GS.cu:64 writes it.
+ *   Parity unpinned: RBW and GS without normalised barriers (norm = false) away
+ *   from the pole, GS in channels with unequal daughter masses (A6), the MultiPolynomial evaluation order
  *   (checked only through degree-0 and degree-1 cases), and quirks A5/A6 beyond
  *   their restatement.
  */

@@ -384,6 +384,45 @@ def test_gs_pole_rho():
         assert op.lineshape(rf, s12, s13) == pytest.approx(want, rel=1e-10)
 
 
+def test_gs_off_pole_rho():
+    """Gounaris-Sakurai (GS.cu:9-75) for the rho0 -> pi+ pi- (spin 1, pair 23) across the plot:
+    (1 + d Gamma0/m0) Z F_R F_D / (m0^2 - s + f(s) - i m0 Gamma0 (k/k0)^3 F_R^2) with
+    f(s) = Gamma0 m0^2 / k0^3 [k^2 (h(s) - h(m0^2)) + (m0^2 - s) k0^2 h'(m0^2)] and
+    h(s) = (2/pi) (k/sqrt s) ln((sqrt s + 2k) / (2 m_pi)) (Gounaris & Sakurai, PRL 21 (1968)
+    244); h' by complex-step differentiation; the width term without the textbook m0/sqrt(s)
+    factor, as GS.cu:64 writes it; barriers as in the RBW check."""
+    import cmath
+    M, r, rm, mpi = W.DALITZ_DECAY[0], W.DALITZ_DECAY[4], W.DALITZ_DECAY[5], W.DALITZ_DECAY[2]
+    mc = W.DALITZ_DECAY[1]
+    m0, g0 = 0.77526, 0.1491
+    op, rf = _lineshape_prog("GS", m0, g0, 1, 2)
+
+    def h(s):
+        k = cmath.sqrt(s / 4 - mpi * mpi)
+        return (2 / math.pi) * (k / cmath.sqrt(s)) * cmath.log((cmath.sqrt(s) + 2 * k) / (2 * mpi))
+
+    s0 = m0 * m0
+    k0 = math.sqrt(s0 / 4 - mpi * mpi)
+    dh0 = h(complex(s0, 1e-30)).imag / 1e-30
+    d = (3 / math.pi) * mpi ** 2 / k0 ** 2 * math.log((m0 + 2 * k0) / (2 * mpi)) + m0 / (2 * math.pi * k0) \
+        - mpi ** 2 * m0 / (math.pi * k0 ** 3)
+    lo, hi = 4 * mpi * mpi, (M - mc) ** 2
+    wants, gots = [], []
+    for s in np.linspace(lo + 0.03 * (hi - lo), hi - 0.03 * (hi - lo), 9):
+        k = math.sqrt(s / 4 - mpi * mpi)
+        f = g0 * s0 / k0 ** 3 * (k * k * (h(s).real - h(s0).real) + (s0 - s) * k0 * k0 * dh0)
+        fr2 = _barrier(1, r * k0) / _barrier(1, r * k)
+        fd2 = _barrier(1, rm * s0 / (2 * M)) / _barrier(1, rm * s / (2 * M))
+        for cos in (-0.7, 0.2, 0.85):
+            s12, s13, dot, _ = _pole_config(2, math.sqrt(s), cos)
+            den = complex(s0 - s + f, -m0 * g0 * (k / k0) ** 3 * fr2)
+            wants.append((1 + d * g0 / m0) * (-4 * dot) * math.sqrt(fr2 * fd2) / den)
+            gots.append(op.lineshape(rf, s12, s13))
+    scale = max(abs(w) for w in wants)
+    for w, g in zip(wants, gots):
+        assert g == pytest.approx(w, rel=1e-9, abs=1e-12 * scale), (w, g)
+
+
 def test_nonres_is_one():
     op, rf = _lineshape_prog("NONRES", 0, 0, 0, 0)
     assert op.lineshape(rf, 1.0, 1.0) == 1.0
