@@ -3,6 +3,7 @@
 #include <goofit/Engine.h>
 #include <goofit/Error.h>
 #include <goofit/PDFs/GooPdf.h>
+#include <goofit/PDFs/physics/resonances/Resonance.h>
 #include <goofit/fitting/BatchFitter.h>
 #include <goofit/fitting/Params.h>
 
@@ -81,6 +82,19 @@ class ToyFCN : public Minuit2::FCNBase {
 
 }  // namespace
 
+// A resonance's mass and width set the per-event lineshape cache that the rows of
+// every toy share: a toy moving one would rebuild the cache under the other toys'
+// sweeps, so a batched fit refuses them floating.
+static void refuse_floating_lineshapes(PdfBase *p) {
+    if(dynamic_cast<ResonancePdf *>(p) != nullptr)
+        for(const Variable &v : p->getParameters())
+            if(!v.IsFixed())
+                throw GeneralError("BatchFitter: resonance parameter " + v.getName()
+                                   + " floats; its lineshape cache is shared by all toys, so it must be fixed");
+    for(PdfBase *c : p->getComponents())
+        refuse_floating_lineshapes(c);
+}
+
 BatchFitter::BatchFitter(GooPdf *pdf, const std::vector<int64_t> &first, const std::vector<int64_t> &count)
     : pdf_(pdf)
     , first_(first)
@@ -109,6 +123,7 @@ std::vector<double> BatchFitter::evaluate() {
 }
 
 void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool autotune) {
+    refuse_floating_lineshapes(pdf_);
     const int T = (int)size();
     Params params(*pdf_);
     std::vector<Variable> vars = params.GetGooFitParams();

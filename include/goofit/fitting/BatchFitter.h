@@ -13,7 +13,7 @@
 // hipGraph with one parameter set per toy.  The normalisations
 // are computed on the host per toy (analytic integrals, or an Amp3Body's cached
 // integral matrix), so the model's non-analytic parts must not depend on the
-// floating parameters.
+// floating parameters: fit() refuses a floating resonance mass or width.
 #pragma once
 
 #include <cstdint>

@@ -335,3 +335,27 @@ def test_batch_points_submit_and_wait(ctx):
         assert np.isfinite(r1[0, :2]).all() and np.isfinite(r1[1, 0]) and np.isnan(r1[2]).all()
     finally:
         b.close()
+
+
+def test_batch_fitter_refuses_a_floating_resonance_mass():
+    """A resonance's mass sets the lineshape cache that all toys' rows share: a batched
+    fit with it floating is refused before any toy starts; fixed again, the fit runs."""
+    counts = [3000, 4000]
+    n = sum(counts)
+    dz = W.build_dalitz(nbins=64, resonances=W.DALITZ_RESONANCES[:4], float_couplings=True)
+    m12, m13 = W.generate_dalitz(dz, n, seed=51)
+    d = G.UnbinnedDataSet(dz.observables)
+    d.from_columns([np.asarray(m12), np.asarray(m13), np.arange(n, dtype=np.float64)])
+    dz.pdf.setData(d)
+    dz.pdf.setDataSize(n)
+    res = next(c for c in dz._keep if isinstance(c, G.Resonances.RBW))
+    mass = next(v for v in res.getParameters() if v.name.endswith("_mass"))
+    bf = G.BatchFitter(dz.pdf, [0, counts[0]], counts)
+    mass.fixed = False
+    try:
+        with pytest.raises(G.GeneralError, match="must be fixed"):
+            bf.fit(threads=2)
+    finally:
+        mass.fixed = True
+    bf.fit(threads=2)
+    assert min(bf.nfcn()) > 0
