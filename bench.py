@@ -419,6 +419,34 @@ def cpu_baseline_dalitz(flat, decay, m12, m13, target_s=10.0):
                 nll=v, cache_build_s=t_cache)
 
 
+def copy_256mib(sync):
+    """Three device copies of 256 MiB: the reference dispatch for scaling the HBM counters."""
+    import torch
+    src = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    sync()
+
+
+def profile_simple(name, sync):
+    """Set up the 1-D AddPdf (1e5 events) or the 3-D ProdPdf (1e7 events) fit model and run 20
+    NLL reductions plus the calibration copies, for rocprofv3."""
+    import goofit_amd as G
+    from goofit_amd import workloads as W
+    if name == "prod3d":
+        mdl, cols = W.build_prod3d(), list(W.sample_prod3d(10_000_000, seed=2))
+    else:
+        mdl, cols = W.build_add1d(), [W.sample_add1d(100_000, seed=1)]
+    d = G.UnbinnedDataSet(mdl.observables)
+    d.from_columns(cols)
+    mdl.pdf.setData(d)
+    for _ in range(20):
+        mdl.pdf.calculateNLL()
+    copy_256mib(sync)
+    log(f"profile-only run done ({name}): 20 NLL reductions, 3 copies of 256 MiB")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -434,9 +462,12 @@ def main():
     ap.add_argument("--pull-study", type=int, default=0, metavar="NTOYS",
                     help="also fit NTOYS 8-resonance Dalitz toys of 2e5 events each for their pulls")
     ap.add_argument("--pull-grid", type=int, default=TOY_GRID, help="normalisation grid of the pull study's fits")
-    ap.add_argument("--profile-only", action="store_true",
-                    help="set up the Dalitz workload, run 20 NLL reductions and a 256 MiB device copy "
-                         "(HBM counter calibration), print nothing else; for rocprofv3 runs")
+    ap.add_argument("--profile-only", nargs="?", const="dalitz16", default=None,
+                    choices=("dalitz16", "prod3d", "add1d"),
+                    help="set up one workload (default dalitz16: the 16-resonance Dalitz model, whose set-up "
+                         "also runs the lineshape cache and the Gram once; prod3d: the 3-D ProdPdf of 1e7 "
+                         "events; add1d: the 1-D AddPdf of 1e5 events), run 20 NLL reductions and a 256 MiB "
+                         "device copy (HBM counter calibration), print nothing else; for rocprofv3 runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the path shards by events across GPUs "
@@ -449,6 +480,9 @@ def main():
 
     G.require_api()
     sync = torch.cuda.synchronize
+    if args.profile_only in ("prod3d", "add1d"):
+        profile_simple(args.profile_only, sync)
+        return
 
     # ---------------------------------------------------------------- Dalitz
     log("building the 16-resonance Dalitz model")
@@ -471,11 +505,7 @@ def main():
     if args.profile_only:
         for _ in range(20):
             pdf.calculateNLL()
-        src = torch.ones(32 << 20, dtype=torch.float64, device="cuda")  # 256 MiB
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        sync()
+        copy_256mib(sync)
         log("profile-only run done: 20 NLL reductions, 3 copies of 256 MiB")
         return
 
