@@ -51,10 +51,18 @@
  *   s / (2 M) that twoBodyCMMothermom's clamped square roots give inside the plot;
  *   GS away from its pole: the published Gounaris-Sakurai f(s) with h' by
  *   complex-step differentiation and the width term as GS.cu:64 writes it.
- *   Parity unpinned: RBW and GS without normalised barriers (norm = false) away
- *   from the pole, GS in channels with unequal daughter masses (A6), the MultiPolynomial evaluation order
- *   (checked only through degree-0 and degree-1 cases), and quirks A5/A6 beyond
- *   their restatement.
+ *   Polynomial, OffsetPolynomial, ExpOffset, ExpPoly, ExpPolyOffset: numpy
+ *   polynomial closed forms of the parameter layouts the reference's constructors
+ *   register; MultiPolynomial: unit-coefficient monomials in the coefficient order
+ *   the reference documents (PolynomialPdf.h:22-33) and a brute-force monomial sum in
+ *   three observables; the extended AddPdf: the extended NLL from scipy densities;
+ *   RBW_Sym: the sum of its two channels recomputed from four-vectors; RBW and GS
+ *   with unnormalised barriers (norm = false, including A5's spin-2 form): the
+ *   lineshapes recomputed from four-vector momenta and Zemach factors; GS in the
+ *   pi0 pi+- channels (A6): the published functions of two charged pions with the
+ *   pair's own momenta in the width term (tests/test_oracle_pins_more.py).
+ *   Parity unpinned: the summation order inside MultiPolynomial (the terms are the
+ *   documented ones; their order of addition is the restatement's own).
  */
 #include <math.h>
 #include <stdint.h>
