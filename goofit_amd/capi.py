@@ -7,6 +7,7 @@ is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -30,6 +31,7 @@ EXPORTS = [
     "gf_events_upload", "gf_events_upload_aos", "gf_events_alloc", "gf_events_column",
     "gf_events_set_identity_column", "gf_program_set", "gf_params_update", "gf_norms_update",
     "gf_decay3_set", "gf_reduce_metric", "gf_reduce_metric_multi", "gf_eval_metric", "gf_eval_metric_device", "gf_reduce_bins",
+    "gf_eval_bins", "gf_grid_cells", "gf_dalitz_grid_cells",
     "gf_dalitz_set_data_size", "gf_dalitz_cache", "gf_dalitz_gram", "gf_dalitz_gram_points",
     "gf_dalitz_gram_points_device", "gf_dalitz_get_cache", "gf_dalitz_cache_gram", "gf_tddp_gram",
     "gf_time_reduce_metric", "gf_time_reduce_sweep", "gf_time_sweep_back_to_back", "gf_time_dalitz_cache",
@@ -84,6 +86,11 @@ def lib():
         L.gf_eval_metric.argtypes = [c_p, ctypes.c_int, _D, ctypes.c_int64]
         L.gf_eval_metric_device.argtypes = [c_p, ctypes.c_int, c_p, ctypes.c_int64]
         L.gf_reduce_bins.argtypes = [c_p, ctypes.c_int, _D, ctypes.c_int, _D]
+        for name, args in (("gf_eval_bins", [c_p, ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int64]),
+                           ("gf_grid_cells", [c_p, _D, ctypes.c_int, _D, ctypes.c_int64]),
+                           ("gf_dalitz_grid_cells", [c_p, _D, ctypes.c_int, _D, _D, ctypes.c_int64])):
+            if hasattr(L, name):  # a library built before these entry points fails only their callers
+                getattr(L, name).argtypes = args
         L.gf_dalitz_set_data_size.argtypes = [c_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64]
         L.gf_dalitz_cache.argtypes = [c_p, ctypes.c_int, ctypes.c_uint64]
         L.gf_dalitz_gram.argtypes = [c_p, ctypes.c_int, _D, ctypes.c_int, _D]
@@ -287,6 +294,33 @@ class Context:
         s = ctypes.c_double()
         _check(lib().gf_reduce_bins(self.h, func_idx, _dp(r), r.size // 3, ctypes.byref(s)))
         return s.value
+
+    @staticmethod
+    def _grid_bins(r):
+        return int(np.prod([int(math.floor(r[3 * i + 2] + 0.5)) for i in range(r.size // 3)]))
+
+    def eval_bins(self, func_idx, ranges):
+        """Per-bin raw values of function func_idx on the grid (gf_eval_bins)."""
+        r = _f64(ranges)
+        out = np.empty(self._grid_bins(r))
+        _check(lib().gf_eval_bins(self.h, func_idx, _dp(r), r.size // 3, _dp(out), out.size))
+        return out
+
+    def grid_cells(self, ranges):
+        """Cell centres of the grid, shape (k, nbins) (gf_grid_cells)."""
+        r = _f64(ranges)
+        k, n = r.size // 3, self._grid_bins(r)
+        out = np.empty(k * n)
+        _check(lib().gf_grid_cells(self.h, _dp(r), k, _dp(out), n))
+        return out.reshape(k, n)
+
+    def dalitz_grid_cells(self, grid6, time_dependent=False):
+        """Centres (m12, m13) of the integration cells of a Dalitz grid (gf_dalitz_grid_cells)."""
+        g = _f64(grid6)
+        n = int(math.floor(g[2] + 0.5)) * int(math.floor(g[5] + 0.5))
+        m12, m13 = np.empty(n), np.empty(n)
+        _check(lib().gf_dalitz_grid_cells(self.h, _dp(g), 1 if time_dependent else 0, _dp(m12), _dp(m13), n))
+        return m12, m13
 
     # -- Dalitz -----------------------------------------------------------
     def dalitz_set_data_size(self, dalitz_func, n, offset=0):

@@ -1586,14 +1586,11 @@ gf_status gf_eval_metric(gf_ctx *ctx, int metric, double *out, int64_t n) {
     return st;
 }
 
-gf_status gf_reduce_bins(gf_ctx *ctx, int func_idx, const double *ranges, int k, double *sum) {
-    if(!valid_ctx(ctx))
-        return GF_ERR_INVALID;
-    if(ctx->funcs.empty())
-        return fail(GF_ERR_STATE, "no program installed");
-    if(!ranges || k < 1 || k > GF_MAX_OBS || !sum)
+// Total bin count of a grid ranges[3 i + {0, 1, 2}] = {lower, upper, nbins}, i < k.
+static gf_status grid_total(const double *ranges, int k, int64_t &total) {
+    if(!ranges || k < 1 || k > GF_MAX_OBS)
         return fail(GF_ERR_INVALID, "grid needs 1..%d observables", GF_MAX_OBS);
-    int64_t total = 1;
+    total = 1;
     for(int i = 0; i < k; i++) {
         double nb = floor(ranges[3 * i + 2] + 0.5);
         if(!(nb >= 1) || nb > 2147483647.0)
@@ -1602,17 +1599,28 @@ gf_status gf_reduce_bins(gf_ctx *ctx, int func_idx, const double *ranges, int k,
         if(total > ((int64_t)1 << 40))
             return fail(GF_ERR_INVALID, "grid of more than 2^40 bins");
     }
-    int pc       = 0;
-    gf_status st = subprogram(ctx, func_idx, pc);
+    return GF_OK;
+}
+
+// Checks of a grid evaluation of function func_idx (gf_reduce_bins, gf_eval_bins): the
+// subprogram's first instruction, the observable slot of each grid dimension and the
+// bin count; fails for a subtree holding an Amp3Body without wave cache or an Amp3Body_TD.
+static gf_status prepare_bins(gf_ctx *ctx, int func_idx, const double *ranges, int k, int &pc, int *slots,
+                              int64_t &total) {
+    if(ctx->funcs.empty())
+        return fail(GF_ERR_STATE, "no program installed");
+    gf_status st = grid_total(ranges, k, total);
     if(st)
+        return st;
+    pc = 0;
+    if((st = subprogram(ctx, func_idx, pc)))
         return st;
     // observable slot of each grid dimension: the function's own observables in order
     Compiler C{ctx, {}, {}, -1};
     Cur q;
     if(!cursor_at(C, func_idx, q))
         return fail(GF_ERR_INVALID, "%s", C.err.c_str());
-    int slots[GF_MAX_OBS] = {0};
-    int nobs_f             = C.no(q);
+    int nobs_f = C.no(q);
     for(int i = 0; i < k; i++) {
         int idx = (k == 1) ? 0 : i;
         if(idx >= nobs_f)
@@ -1623,7 +1631,7 @@ gf_status gf_reduce_bins(gf_ctx *ctx, int func_idx, const double *ranges, int k,
     for(const DalitzNode &d : ctx->dalitz)
         if(d.func >= func_idx) {
             if(d.td) {
-                // the bin kernel runs without the time-dependent code: not for a subtree holding one
+                // the bin kernels run without the time-dependent code: not for a subtree holding one
                 Cur r;
                 Compiler C2{ctx, {}, {}, -1};
                 if(cursor_at(C2, func_idx, r) && skip_subtree(C2, r) && d.func < r.f)
@@ -1638,7 +1646,18 @@ gf_status gf_reduce_bins(gf_ctx *ctx, int func_idx, const double *ranges, int k,
                     return fail(GF_ERR_STATE, "Amp3Body at %d has no wave cache", d.func);
             }
         }
-    if((st = flush_uploads(ctx)))
+    return flush_uploads(ctx);
+}
+
+gf_status gf_reduce_bins(gf_ctx *ctx, int func_idx, const double *ranges, int k, double *sum) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(!sum)
+        return fail(GF_ERR_INVALID, "null output");
+    int pc = 0, slots[GF_MAX_OBS] = {0};
+    int64_t total = 0;
+    gf_status st  = prepare_bins(ctx, func_idx, ranges, k, pc, slots, total);
+    if(st)
         return st;
     ++ctx->res_seq;
     GfProgArgs a = prog_args(ctx);
@@ -1647,6 +1666,99 @@ gf_status gf_reduce_bins(gf_ctx *ctx, int func_idx, const double *ranges, int k,
     if(rc)
         return fail(GF_ERR_HIP, "grid reduce launch failed (%d)", rc);
     return read_result(ctx, sum, nullptr);
+}
+
+// Copy n doubles from a device buffer the call allocated to the host, then free it.
+static gf_status finish_device_output(gf_ctx *ctx, double *dev, double *host, int64_t n, const char *what) {
+    gf_status st = GF_OK;
+    if(hipMemcpyAsync(host, dev, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess
+       || hipStreamSynchronize(ctx->stream) != hipSuccess)
+        st = fail(GF_ERR_HIP, "copy of the %s failed", what);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dev);
+    return st;
+}
+
+gf_status gf_eval_bins(gf_ctx *ctx, int func_idx, const double *ranges, int k, double *out, int64_t nbins) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(!out)
+        return fail(GF_ERR_INVALID, "null output");
+    int pc = 0, slots[GF_MAX_OBS] = {0};
+    int64_t total = 0;
+    gf_status st  = prepare_bins(ctx, func_idx, ranges, k, pc, slots, total);
+    if(st)
+        return st;
+    if(nbins != total)
+        return fail(GF_ERR_INVALID, "output holds %lld values, the grid has %lld bins", (long long)nbins,
+                    (long long)total);
+    double *d_out = nullptr;
+    if(hipMalloc((void **)&d_out, (size_t)total * sizeof(double)) != hipSuccess)
+        return fail(GF_ERR_NOMEM, "cannot allocate %lld bin values", (long long)total);
+    GfProgArgs a = prog_args(ctx);
+    int rc = gfk::launch_eval_bins(a, pc, ranges, slots, k, total, d_out, event_blocks(ctx, total), ctx->stream);
+    if(rc) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(d_out);
+        return fail(GF_ERR_HIP, "grid evaluation launch failed (%d)", rc);
+    }
+    return finish_device_output(ctx, d_out, out, total, "bin values");
+}
+
+gf_status gf_grid_cells(gf_ctx *ctx, const double *ranges, int k, double *out, int64_t nbins) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(!out)
+        return fail(GF_ERR_INVALID, "null output");
+    int64_t total = 0;
+    gf_status st  = grid_total(ranges, k, total);
+    if(st)
+        return st;
+    if(nbins != total)
+        return fail(GF_ERR_INVALID, "output holds %lld cells, the grid has %lld bins", (long long)nbins,
+                    (long long)total);
+    double *d_out = nullptr;
+    if(hipMalloc((void **)&d_out, (size_t)total * k * sizeof(double)) != hipSuccess)
+        return fail(GF_ERR_NOMEM, "cannot allocate %lld cell centres", (long long)(total * k));
+    int rc = gfk::launch_grid_cells(ranges, k, total, d_out, ctx->stream);
+    if(rc) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(d_out);
+        return fail(GF_ERR_HIP, "grid cell launch failed (%d)", rc);
+    }
+    return finish_device_output(ctx, d_out, out, total * k, "cell centres");
+}
+
+gf_status gf_dalitz_grid_cells(gf_ctx *ctx, const double grid6[6], int time_dependent, double *m12, double *m13,
+                               int64_t ncells) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(!grid6 || !m12 || !m13)
+        return fail(GF_ERR_INVALID, "null grid or output");
+    double n12 = floor(grid6[2] + 0.5), n13 = floor(grid6[5] + 0.5);
+    if(!(n12 >= 1) || !(n13 >= 1) || n12 * n13 > 2147483647.0)
+        return fail(GF_ERR_INVALID, "bad grid bin counts %g x %g", grid6[2], grid6[5]);
+    if(ncells != (int64_t)n12 * (int64_t)n13)
+        return fail(GF_ERR_INVALID, "output holds %lld cells, the grid has %lld", (long long)ncells,
+                    (long long)((int64_t)n12 * (int64_t)n13));
+    double *d_out = nullptr;
+    if(hipMalloc((void **)&d_out, (size_t)ncells * 2 * sizeof(double)) != hipSuccess)
+        return fail(GF_ERR_NOMEM, "cannot allocate %lld cell centres", (long long)ncells);
+    int rc = gfk::launch_dalitz_cells(grid6, time_dependent ? 1 : 0, ncells, d_out, d_out + ncells, ctx->stream);
+    if(rc) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(d_out);
+        return fail(GF_ERR_HIP, "Dalitz cell launch failed (%d)", rc);
+    }
+    gf_status st = GF_OK;
+    if(hipMemcpyAsync(m12, d_out, (size_t)ncells * sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess
+       || hipMemcpyAsync(m13, d_out + ncells, (size_t)ncells * sizeof(double), hipMemcpyDeviceToHost, ctx->stream)
+              != hipSuccess
+       || hipStreamSynchronize(ctx->stream) != hipSuccess)
+        st = fail(GF_ERR_HIP, "copy of the Dalitz cell centres failed");
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_out);
+    return st;
 }
 
 // ---------------------------------------------------------------- Dalitz
@@ -1748,7 +1860,8 @@ gf_status gf_dalitz_gram(gf_ctx *ctx, int dalitz_func, const double grid6[6], in
     if(!grid6 || !out_re_im)
         return fail(GF_ERR_INVALID, "null grid or output");
     double n12 = floor(grid6[2] + 0.5), n13 = floor(grid6[5] + 0.5);
-    if(!(n12 >= 1) || !(n13 >= 1) || n12 * n13 > 4.0e9)
+    // the cell index is an int (SpecialResonanceIntegrator.cu:59)
+    if(!(n12 >= 1) || !(n13 >= 1) || n12 * n13 > 2147483647.0)
         return fail(GF_ERR_INVALID, "bad grid bin counts %g x %g", grid6[2], grid6[5]);
     if(eff_func >= (int)ctx->funcs.size())
         return fail(GF_ERR_INVALID, "efficiency function %d outside the tree", eff_func);
@@ -1844,7 +1957,8 @@ gf_status gf_tddp_gram(gf_ctx *ctx, int tddp_func, const double grid6[6], double
     if(!grid6 || !out_aa || !out_ab || !out_bb)
         return fail(GF_ERR_INVALID, "null grid or output");
     double n12 = floor(grid6[2] + 0.5), n13 = floor(grid6[5] + 0.5);
-    if(!(n12 >= 1) || !(n13 >= 1) || n12 * n13 > 4.0e9)
+    // the cell index is an int (SpecialDalitzIntegrator.cu:65)
+    if(!(n12 >= 1) || !(n13 >= 1) || n12 * n13 > 2147483647.0)
         return fail(GF_ERR_INVALID, "bad grid bin counts %g x %g", grid6[2], grid6[5]);
     const int n = d->nres;
     if(n == 0)

@@ -37,6 +37,20 @@ __device__ __forceinline__ void cmac(double &re, double &im, double cr, double c
     im = fma(cr, w.y, fma(ci, w.x, im));
 }
 
+// ------------------------------------------------------------------ grids
+
+// Centre of local bin b of n equal bins on [lo, hi]: ((hi - lo) / n) * (b + 0.5) + lo,
+// one rounding per operation in this order (MetricTaker.cu:81-84,
+// SpecialResonanceIntegrator.cu:64-76).  Every grid kernel takes its cell centres
+// from here, so the cells of an integral are the cells gf_grid_cells reports.
+__host__ __device__ __forceinline__ double bin_centre(double lo, double hi, int n, int64_t b) {
+    double x = hi - lo;
+    x /= n;
+    x *= ((double)b + 0.5);
+    x += lo;
+    return x;
+}
+
 // ---------------------------------------------------------------- leaf PDFs
 
 // device_Gaussian, src/PDFs/basic/GaussianPdf.cu:7-17

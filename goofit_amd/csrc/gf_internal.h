@@ -176,6 +176,13 @@ int launch_dalitz_fast(const GfProgArgs &a, const GfInline *inl, bool const_eff,
                        Partial *partials, GfResult *result, double *out, int nblocks, void *stream);
 int launch_reduce_bins(const GfProgArgs &a, int pc0, const double *ranges, const int *slots, int k, int64_t total,
                        Partial *partials, GfResult *result, int nblocks, void *stream);
+// Per-bin values of the same grid into out[total] (device).
+int launch_eval_bins(const GfProgArgs &a, int pc0, const double *ranges, const int *slots, int k, int64_t total,
+                     double *out, int nblocks, void *stream);
+// Cell centres of a k-dimensional grid: out[d * total + bin] (device).
+int launch_grid_cells(const double *ranges, int k, int64_t total, double *out, void *stream);
+// Cell centres of a Dalitz integration grid grid6 (host array), cells 0..ncells-1 (device outputs).
+int launch_dalitz_cells(const double *grid6, int td_grid, int64_t ncells, double *m12, double *m13, void *stream);
 // Toy batch: per-workgroup sweep with per-toy parameters, per-toy finishing
 // reduction into result[2t..2t+1] (host-mapped), then the completion stamp.
 int launch_batch(const GfProgArgs &a, const GfBatchWork *work, int nblocks, const double *params, int np,

@@ -811,6 +811,35 @@ struct GfGrid {
 
 namespace {
 
+// The centres of global bin `bin` of grid G in every dimension (MetricTaker.cu:76-86:
+// the first observable varies fastest, local bin = bin % n, then bin /= n).
+__device__ __forceinline__ void grid_point(const GfGrid &G, int64_t bin, double (&x)[GF_MAX_OBS]) {
+#pragma unroll
+    for(int d = 0; d < GF_MAX_OBS; ++d) {
+        x[d] = 0.0;
+        if(d < G.k) {
+            const int64_t localBin = bin % G.nb[d];
+            x[d]                   = gfd::bin_centre(G.lo[d], G.hi[d], G.nb[d], localBin);
+            bin /= G.nb[d];
+        }
+    }
+}
+
+// The synthetic event of grid bin `bin` in slot j of src: every observable slot 0,
+// then each grid dimension's centre at its observable slot.
+template <int EPT>
+__device__ __forceinline__ void grid_event(const GfGrid &G, int64_t bin, RegSrc<EPT> &src, int j) {
+    double x[GF_MAX_OBS];
+    grid_point(G, bin, x);
+#pragma unroll
+    for(int o = 0; o < GF_MAX_OBS; ++o)
+        src.v[j][o] = 0.0;
+#pragma unroll
+    for(int d = 0; d < GF_MAX_OBS; ++d)
+        if(d < G.k)
+            src.set(G.slot[d], j, x[d]);
+}
+
 template <int EPT>
 __global__ __launch_bounds__(GF_BLOCK) void k_reduce_bins(GfProgArgs A, int pc0, GfGrid G, int64_t total,
                                                           Partial *partials) {
@@ -825,23 +854,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_reduce_bins(GfProgArgs A, int pc0,
             int64_t b = base + (int64_t)j * GF_BLOCK + threadIdx.x;
             valid[j]  = b < total;
             row[j]    = valid[j] ? b : 0;
-#pragma unroll
-            for(int o = 0; o < GF_MAX_OBS; ++o)
-                src.v[j][o] = 0.0;
-            int64_t binNumber = row[j];
-            // MetricTaker.cu:76-86: first observable fastest, centre ((u-l)/n)*(b+0.5)+l
-#pragma unroll
-            for(int d = 0; d < GF_MAX_OBS; ++d) {
-                if(d < G.k) {
-                    int64_t localBin = binNumber % G.nb[d];
-                    double x         = G.hi[d] - G.lo[d];
-                    x /= G.nb[d];
-                    x *= ((double)localBin + 0.5);
-                    x += G.lo[d];
-                    src.set(G.slot[d], j, x);
-                    binNumber /= G.nb[d];
-                }
-            }
+            grid_event(G, row[j], src, j);
         }
         double T[EPT];
         run<EPT>(A, A.params, A.norms, pc0, src, row, valid, T);
@@ -851,6 +864,43 @@ __global__ __launch_bounds__(GF_BLOCK) void k_reduce_bins(GfProgArgs A, int pc0,
                 gfd::kb_add(s, c, T[j]);
     }
     store_partial(s, c, partials);
+}
+
+// Per-bin raw values of the subprogram at pc0 over grid G: the transform that
+// k_reduce_bins sums (same cells, same evaluation), written to out[bin].
+template <int EPT>
+__global__ __launch_bounds__(GF_BLOCK) void k_eval_bins(GfProgArgs A, int pc0, GfGrid G, int64_t total, double *out) {
+    const int64_t tile = (int64_t)GF_BLOCK * EPT;
+    for(int64_t base = (int64_t)blockIdx.x * tile; base < total; base += (int64_t)gridDim.x * tile) {
+        RegSrc<EPT> src;
+        int64_t row[EPT];
+        bool valid[EPT];
+#pragma unroll
+        for(int j = 0; j < EPT; ++j) {
+            int64_t b = base + (int64_t)j * GF_BLOCK + threadIdx.x;
+            valid[j]  = b < total;
+            row[j]    = valid[j] ? b : 0;
+            grid_event(G, row[j], src, j);
+        }
+        double T[EPT];
+        run<EPT>(A, A.params, A.norms, pc0, src, row, valid, T);
+#pragma unroll
+        for(int j = 0; j < EPT; ++j)
+            if(valid[j])
+                out[row[j]] = T[j];
+    }
+}
+
+// The cell centres of grid G: out[d * total + bin] = coordinate d of bin.
+__global__ __launch_bounds__(256) void k_grid_cells(GfGrid G, int64_t total, double *out) {
+    for(int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < total; b += (int64_t)gridDim.x * blockDim.x) {
+        double x[GF_MAX_OBS];
+        grid_point(G, b, x);
+#pragma unroll
+        for(int d = 0; d < GF_MAX_OBS; ++d)
+            if(d < G.k)
+                out[(int64_t)d * total + b] = x[d];
+    }
 }
 
 // ------------------------------------------------------------- Dalitz cache
@@ -1217,8 +1267,28 @@ struct GramSrc {
     int id_m12, id_m13;
     int eff_pc;        // >= 0: efficiency subprogram; -1: efficiency 1; -2: constant eff_const
     double eff_const;
-    int td;            // grid of a time-dependent node: m13 bin count n13 (SpecialDalitzIntegrator.cu:73-80)
+    int td;            // grid of a time-dependent node: m13 bin count n13 (SpecialDalitzIntegrator.cu:75-82)
 };
+
+// Centre (m12, m13) of integration cell p of a Dalitz grid (SpecialResonanceIntegrator.cu:61-76:
+// m12 fastest; the m13 bin count is read from the m12 entry there, numBinsM13 = numBinsM12,
+// reproduced here; a time-dependent node's integrator reads its own m13 count,
+// SpecialDalitzIntegrator.cu:75-82).  The cell index is an int there, as here.
+__device__ __forceinline__ void dalitz_cell(const GramSrc &S, int64_t p, double &m12, double &m13) {
+    int globalBinNumber  = (int)p;
+    const int numBinsM12 = S.n12;
+    const int binM12     = globalBinNumber % numBinsM12;
+    m12                  = gfd::bin_centre(S.l12, S.u12, numBinsM12, binM12);
+    globalBinNumber /= numBinsM12;
+    const int numBinsM13 = S.td ? S.n13 : S.n12;
+    m13                  = gfd::bin_centre(S.l13, S.u13, numBinsM13, globalBinNumber);
+}
+
+// The centres of the first npoints cells of the grid in S (mode 0).
+__global__ __launch_bounds__(256) void k_dalitz_cells(GramSrc S, double *m12, double *m13) {
+    for(int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < S.npoints; p += (int64_t)gridDim.x * blockDim.x)
+        dalitz_cell(S, p, m12[p], m13[p]);
+}
 
 // One block: 128 points per pass; each lane computes the 16 complex amplitudes of
 // its point (wave-uniform control flow), stages them in LDS component-major,
@@ -1240,23 +1310,7 @@ __global__ __launch_bounds__(GG_BLOCK) void k_gram(GfProgArgs A, GramSrc S, cons
         const bool valid = p < npts;
         double m12 = 0.0, m13 = 0.0, wt = 1.0;
         if(S.mode == 0) {
-            // SpecialResonanceIntegrator.cu:61-76; the m13 bin count is read from the
-            // m12 entry there (numBinsM13 = numBinsM12), reproduced here.
-            int globalBinNumber = valid ? (int)p : 0;
-            int numBinsM12      = S.n12;
-            int binNumberM12    = globalBinNumber % numBinsM12;
-            double c12          = S.u12 - S.l12;
-            c12 /= numBinsM12;
-            c12 *= (binNumberM12 + 0.5);
-            c12 += S.l12;
-            globalBinNumber /= numBinsM12;
-            int numBinsM13 = S.td ? S.n13 : S.n12;
-            double c13     = S.u13 - S.l13;
-            c13 /= numBinsM13;
-            c13 *= (globalBinNumber + 0.5);
-            c13 += S.l13;
-            m12 = c12;
-            m13 = c13;
+            dalitz_cell(S, valid ? p : 0, m12, m13);
         } else if(S.mode == 1) {
             m12 = valid ? S.m12[p] : 0.0;
             m13 = valid ? S.m13[p] : 0.0;
@@ -1511,19 +1565,42 @@ int launch_dalitz_fast(const GfProgArgs &a, const GfInline *inl, bool const_eff,
     return launch_finish(a, partials, nblocks, result, s);
 }
 
-int launch_reduce_bins(const GfProgArgs &a, int pc0, const double *ranges, const int *slots, int k, int64_t total,
-                       Partial *partials, GfResult *result, int nblocks, void *stream) {
+static GfGrid make_grid(const double *ranges, const int *slots, int k) {
     GfGrid G;
     for(int d = 0; d < GF_MAX_OBS; ++d) {
         G.lo[d]   = d < k ? ranges[3 * d + 0] : 0.0;
         G.hi[d]   = d < k ? ranges[3 * d + 1] : 1.0;
         G.nb[d]   = d < k ? (int)floor(ranges[3 * d + 2] + 0.5) : 1;
-        G.slot[d] = d < k ? slots[d] : 0;
+        G.slot[d] = (d < k && slots) ? slots[d] : 0;
     }
     G.k = k;
-    hipStream_t s = (hipStream_t)stream;
+    return G;
+}
+
+int launch_reduce_bins(const GfProgArgs &a, int pc0, const double *ranges, const int *slots, int k, int64_t total,
+                       Partial *partials, GfResult *result, int nblocks, void *stream) {
+    const GfGrid G = make_grid(ranges, slots, k);
+    hipStream_t s  = (hipStream_t)stream;
     hipLaunchKernelGGL(k_reduce_bins<2>, dim3(nblocks), dim3(GF_BLOCK), 0, s, a, pc0, G, total, partials);
     return launch_finish(a, partials, nblocks, result, s);
+}
+
+int launch_eval_bins(const GfProgArgs &a, int pc0, const double *ranges, const int *slots, int k, int64_t total,
+                     double *out, int nblocks, void *stream) {
+    const GfGrid G = make_grid(ranges, slots, k);
+    hipLaunchKernelGGL(k_eval_bins<2>, dim3(nblocks), dim3(GF_BLOCK), 0, (hipStream_t)stream, a, pc0, G, total, out);
+    return check(hipGetLastError());
+}
+
+static unsigned cell_blocks(int64_t total) {
+    int64_t nb = (total + 255) / 256;
+    return (unsigned)(nb > 8192 ? 8192 : (nb < 1 ? 1 : nb));
+}
+
+int launch_grid_cells(const double *ranges, int k, int64_t total, double *out, void *stream) {
+    const GfGrid G = make_grid(ranges, nullptr, k);
+    hipLaunchKernelGGL(k_grid_cells, dim3(cell_blocks(total)), dim3(256), 0, (hipStream_t)stream, G, total, out);
+    return check(hipGetLastError());
 }
 
 int launch_batch(const GfProgArgs &a, const GfBatchWork *work, int nblocks, const double *params, int np,
@@ -1650,6 +1727,21 @@ int launch_gram(const GfProgArgs &a, int mode, const double *grid6, const double
     if(rc)
         return rc;
     hipLaunchKernelGGL(k_gram_final, dim3(768), dim3(256), 0, s, partials, nblocks, tiles_dev);
+    return check(hipGetLastError());
+}
+
+int launch_dalitz_cells(const double *grid6, int td_grid, int64_t ncells, double *m12, double *m13, void *stream) {
+    GramSrc S{};
+    S.mode    = 0;
+    S.td      = td_grid;
+    S.l12     = grid6[0];
+    S.u12     = grid6[1];
+    S.n12     = (int)floor(grid6[2] + 0.5);
+    S.l13     = grid6[3];
+    S.u13     = grid6[4];
+    S.n13     = (int)floor(grid6[5] + 0.5);
+    S.npoints = ncells;
+    hipLaunchKernelGGL(k_dalitz_cells, dim3(cell_blocks(ncells)), dim3(256), 0, (hipStream_t)stream, S, m12, m13);
     return check(hipGetLastError());
 }
 

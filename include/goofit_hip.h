@@ -209,6 +209,19 @@ gf_status gf_eval_metric_device(gf_ctx *ctx, int metric, double *dev_out, int64_
  * (src/PDFs/MetricTaker.cu:48-96). */
 gf_status gf_reduce_bins(gf_ctx *ctx, int func_idx, const double *ranges, int k, double *sum);
 
+/* The per-bin raw values that gf_reduce_bins adds, into host array out[nbins]
+ * (nbins = the grid's bin count; bin b in the first-observable-fastest order).
+ * Replaces the transform step of GooPdf::reduce_with_bins over the MetricTaker
+ * binned operator (src/PDFs/GooPdf.cu:83-108, src/PDFs/MetricTaker.cu:48-96). */
+gf_status gf_eval_bins(gf_ctx *ctx, int func_idx, const double *ranges, int k, double *out, int64_t nbins);
+
+/* The cell centres every grid kernel uses, into host array out[d * nbins + b]
+ * (coordinate d of bin b; ranges and order as in gf_reduce_bins); each centre is
+ * ((upper - lower) / n) * (local bin + 0.5) + lower, one rounding per operation.
+ * Replaces the bin-centre arithmetic of MetricTaker (src/PDFs/MetricTaker.cu:73-86)
+ * and GooPdf::makeGrid (src/PDFs/GooPdf.cpp:92-99). */
+gf_status gf_grid_cells(gf_ctx *ctx, const double *ranges, int k, double *out, int64_t nbins);
+
 /* ---- three-body Dalitz amplitude ---------------------------------------- */
 
 /* Allocate the wave cache of Dalitz node dalitz_func for n entries, read from
@@ -230,6 +243,15 @@ gf_status gf_dalitz_cache(gf_ctx *ctx, int dalitz_func, uint64_t res_mask);
  * (src/PDFs/physics/Amp3Body.cu:312-330, detail/SpecialResonanceIntegrator.cu:7-121). */
 gf_status gf_dalitz_gram(gf_ctx *ctx, int dalitz_func, const double grid6[6], int eff_func,
                          double *out_re_im);
+
+/* The integration cells of gf_dalitz_gram (time_dependent = 0) or gf_tddp_gram
+ * (time_dependent != 0) on grid6, as those kernels compute them: cell p has m12 bin
+ * p % n12 and m13 bin p / n12 (p < ncells = n12 * n13), centres into host arrays
+ * m12[p], m13[p].  The time-integrated grid takes the m13 bin width from n12 as
+ * SpecialResonanceIntegrator.cu:69-76 does; the time-dependent one its own n13
+ * (SpecialDalitzIntegrator.cu:75-82). */
+gf_status gf_dalitz_grid_cells(gf_ctx *ctx, const double grid6[6], int time_dependent, double *m12, double *m13,
+                               int64_t ncells);
 
 /* Same integrals over a list of np points (m12[p], m13[p]) with weights w[p]
  * (w may be NULL for 1): I_ij = sum_p w_p eff_p B_i(p) conj(B_j(p)).  Host

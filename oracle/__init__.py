@@ -81,6 +81,8 @@ def lib():
         L.gfo_reduce_bins.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int]
         L.gfo_reduce_bins.restype = ctypes.c_double
         L.gfo_eval_bins.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int, _D]
+        L.gfo_grid_cells.argtypes = [_D, ctypes.c_int, _D]
+        L.gfo_dalitz_grid_cells.argtypes = [_D, ctypes.c_int, _D, _D]
         L.gfo_in_dalitz.argtypes = [ctypes.c_double] * 6
         L.gfo_in_dalitz.restype = ctypes.c_int
         L.gfo_two_body_cm_mom.argtypes = [ctypes.c_double] * 3
@@ -128,6 +130,26 @@ def chunked_sum(terms, nchunks=None) -> float:
 def exact_sum(terms) -> float:
     t = np.ascontiguousarray(terms, dtype=np.float64)
     return lib().gfo_exact_sum(_dp(t), t.size)
+
+
+def grid_cells(ranges):
+    """Cell centres of a k-dimensional grid, shape (k, nbins) (MetricTaker.cu:76-86)."""
+    r = np.ascontiguousarray(ranges, np.float64)
+    k = r.size // 3
+    n = int(np.prod([int(math.floor(r[3 * i + 2] + 0.5)) for i in range(k)]))
+    out = np.empty(k * n)
+    lib().gfo_grid_cells(_dp(r), k, _dp(out))
+    return out.reshape(k, n)
+
+
+def dalitz_grid_cells(grid6, time_dependent=False):
+    """Centres (m12, m13) of the Dalitz integration cells (SpecialResonanceIntegrator.cu:59-76,
+    SpecialDalitzIntegrator.cu:75-82)."""
+    g = np.ascontiguousarray(grid6, np.float64)
+    n = int(math.floor(g[2] + 0.5)) * int(math.floor(g[5] + 0.5))
+    m12, m13 = np.empty(n), np.empty(n)
+    lib().gfo_dalitz_grid_cells(_dp(g), 1 if time_dependent else 0, _dp(m12), _dp(m13))
+    return m12, m13
 
 
 def in_dalitz(m12, m13, M, d1, d2, d3) -> bool:

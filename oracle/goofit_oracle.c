@@ -1071,27 +1071,49 @@ double gfo_chunked_sum(const double *terms, int64_t n, int P) {
  * For k > 1 the reference writes every coordinate into the same slot
  * events[getObservable(0)] and leaves the other slots uninitialised; here each
  * coordinate goes to the slot of its own observable (the midpoint rule). */
-static double binned_value(gfo_prog *g, int func_idx, const double *ranges, int k, int64_t binNumber) {
-    pcur pc = {0, 0, 0, 0, 0};
-    double events[16];
-    memset(events, 0, sizeof(events));
-    for(int i = 0; i < func_idx; i++)
-        inc(g, &pc);
+/* Centres of global bin binNumber in each of the k grid dimensions (MetricTaker.cu:76-86:
+ * first dimension fastest, x = ((u - l) / n) * (localBin + 0.5) + l). */
+static void grid_centres(const double *ranges, int k, int64_t binNumber, double *x) {
     for(int i = 0; i < k; ++i) {
         double lowerBound = ranges[3 * i + 0];
         double upperBound = ranges[3 * i + 1];
         int numBins       = (int)floor(ranges[3 * i + 2] + 0.5);
         int localBin      = (int)(binNumber % numBins);
-        double x          = upperBound - lowerBound;
-        x /= numBins;
-        x *= (localBin + 0.5);
-        x += lowerBound;
-        int id = (int)O(g, &pc, k == 1 ? 0 : i);
-        if(id >= 0 && id < 16)
-            events[id] = x;
+        double c          = upperBound - lowerBound;
+        c /= numBins;
+        c *= (localBin + 0.5);
+        c += lowerBound;
+        x[i] = c;
         binNumber /= numBins;
     }
+}
+
+static double binned_value(gfo_prog *g, int func_idx, const double *ranges, int k, int64_t binNumber) {
+    pcur pc = {0, 0, 0, 0, 0};
+    double events[16], x[16];
+    memset(events, 0, sizeof(events));
+    for(int i = 0; i < func_idx; i++)
+        inc(g, &pc);
+    grid_centres(ranges, k, binNumber, x);
+    for(int i = 0; i < k; ++i) {
+        int id = (int)O(g, &pc, k == 1 ? 0 : i);
+        if(id >= 0 && id < 16)
+            events[id] = x[i];
+    }
     return call_function(g, events, &pc);
+}
+
+/* The cell centres of a k-dimensional grid: out[d * total + b] = coordinate d of bin b. */
+void gfo_grid_cells(const double *ranges, int k, double *out) {
+    int64_t total = 1;
+    for(int i = 0; i < k; i++)
+        total *= (int64_t)floor(ranges[3 * i + 2] + 0.5);
+    for(int64_t b = 0; b < total; b++) {
+        double x[16];
+        grid_centres(ranges, k, b, x);
+        for(int d = 0; d < k; d++)
+            out[(int64_t)d * total + b] = x[d];
+    }
 }
 
 double gfo_reduce_bins(gfo_prog *g, int func_idx, const double *ranges, int k, int P) {
@@ -1208,6 +1230,38 @@ int gfo_dalitz_layout(gfo_prog *g, int dalitz_func, int *res_funcs, int *eff_fun
     return nres;
 }
 
+/* Centre of Dalitz integration cell `bin` (SpecialResonanceIntegrator.cu:59-76): m12 bin
+ * bin % n12, m13 bin bin / n12; the time-integrated integrator takes the m13 bin count
+ * from the m12 entry (hazard A3, :72), the time-dependent one its own
+ * (SpecialDalitzIntegrator.cu:75-82).  The cell index is an int there. */
+static void dalitz_cell(const double *grid6, int td, int64_t bin, double *m12, double *m13) {
+    int globalBinNumber  = (int)bin;
+    double lowerBoundM12 = grid6[0];
+    double upperBoundM12 = grid6[1];
+    int numBinsM12       = (int)floor(grid6[2] + 0.5);
+    int binNumberM12     = globalBinNumber % numBinsM12;
+    double binCenterM12  = upperBoundM12 - lowerBoundM12;
+    binCenterM12 /= numBinsM12;
+    binCenterM12 *= (binNumberM12 + 0.5);
+    binCenterM12 += lowerBoundM12;
+    globalBinNumber /= numBinsM12;
+    double lowerBoundM13 = grid6[3];
+    double upperBoundM13 = grid6[4];
+    int numBinsM13       = (int)floor(grid6[td ? 5 : 2] + 0.5);
+    double binCenterM13  = upperBoundM13 - lowerBoundM13;
+    binCenterM13 /= numBinsM13;
+    binCenterM13 *= (globalBinNumber + 0.5);
+    binCenterM13 += lowerBoundM13;
+    *m12 = binCenterM12;
+    *m13 = binCenterM13;
+}
+
+void gfo_dalitz_grid_cells(const double *grid6, int td, double *m12, double *m13) {
+    int64_t total = (int64_t)floor(grid6[2] + 0.5) * (int64_t)floor(grid6[5] + 0.5);
+    for(int64_t b = 0; b < total; b++)
+        dalitz_cell(grid6, td, b, &m12[b], &m13[b]);
+}
+
 /* Grid integrals I_ij for all pairs (Amp3Body.cu:312-330 with
  * SpecialResonanceIntegrator.cu:52-121): the bin centres, the m13 bin count read
  * from the m12 entry (SpecialResonanceIntegrator.cu:72), the fake event for the
@@ -1238,23 +1292,8 @@ void gfo_dalitz_integrals(gfo_prog *g, int dalitz_func, const double *grid6, int
         cplx B[64];
         cplx *acc = chunk + (size_t)c * npair;
         for(int64_t bin = b; bin < e; bin++) {
-            int globalBinNumber  = (int)bin;
-            double lowerBoundM12 = grid6[0];
-            double upperBoundM12 = grid6[1];
-            int numBinsM12       = (int)floor(grid6[2] + 0.5);
-            int binNumberM12     = globalBinNumber % numBinsM12;
-            double binCenterM12  = upperBoundM12 - lowerBoundM12;
-            binCenterM12 /= numBinsM12;
-            binCenterM12 *= (binNumberM12 + 0.5);
-            binCenterM12 += lowerBoundM12;
-            globalBinNumber /= numBinsM12;
-            double lowerBoundM13 = grid6[3];
-            double upperBoundM13 = grid6[4];
-            int numBinsM13       = (int)floor(grid6[2] + 0.5);
-            double binCenterM13  = upperBoundM13 - lowerBoundM13;
-            binCenterM13 /= numBinsM13;
-            binCenterM13 *= (globalBinNumber + 0.5);
-            binCenterM13 += lowerBoundM13;
+            double binCenterM12, binCenterM13;
+            dalitz_cell(grid6, 0, bin, &binCenterM12, &binCenterM13);
 
             int inside = gfo_in_dalitz(binCenterM12, binCenterM13, g->motherMass, g->daug1Mass, g->daug2Mass,
                                        g->daug3Mass);
@@ -1430,23 +1469,8 @@ void gfo_tddp_integrals(gfo_prog *g, int tddp_func, const double *grid6, int P, 
         cplx A[64], B[64];
         cplx *acc = chunk + (size_t)c * npair * 3;
         for(int64_t bin = b0; bin < e0; bin++) {
-            int globalBinNumber  = (int)bin;
-            double lowerBoundM12 = grid6[0];
-            double upperBoundM12 = grid6[1];
-            int numBinsM12       = (int)floor(grid6[2] + 0.5);
-            int binNumberM12     = globalBinNumber % numBinsM12;
-            double binCenterM12  = upperBoundM12 - lowerBoundM12;
-            binCenterM12 /= numBinsM12;
-            binCenterM12 *= (binNumberM12 + 0.5);
-            binCenterM12 += lowerBoundM12;
-            globalBinNumber /= numBinsM12;
-            double lowerBoundM13 = grid6[3];
-            double upperBoundM13 = grid6[4];
-            int numBinsM13       = (int)floor(grid6[5] + 0.5);
-            double binCenterM13  = upperBoundM13 - lowerBoundM13;
-            binCenterM13 /= numBinsM13;
-            binCenterM13 *= (globalBinNumber + 0.5);
-            binCenterM13 += lowerBoundM13;
+            double binCenterM12, binCenterM13;
+            dalitz_cell(grid6, 1, bin, &binCenterM12, &binCenterM13);
             for(int r = 0; r < nres; r++)
                 tddp_ab(g, &rpc[r], binCenterM12, binCenterM13, &A[r], &B[r]);
             double events[16];
