@@ -119,6 +119,9 @@ std::vector<double> BatchFitter::evaluate() {
         engine::check(gf_batch_set_toy(batch_, (int)t, host_parameters.data(), host_normalizations.data()),
                       "gf_batch_set_toy");
     engine::check(gf_batch_reduce(batch_, all.data(), GF_METRIC_NLL, out.data()), "gf_batch_reduce");
+    // calculateNLL's extended term (AddPdf.cu:187-204) with each toy's own event count
+    for(size_t t = 0; t < size(); t++)
+        out[t] += pdf_->nllExtraTermFor((fptype)count_[t]);
     return out;
 }
 
@@ -157,6 +160,7 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
     int next    = 0;  // next toy to start (under B.mu)
     const int W = std::max(1, std::min(threads, T));
     B.running   = 0;
+    std::exception_ptr worker_failure;  // the first exception of a toy's fit (under B.mu)
     auto worker = [&]() {
         for(;;) {
             int t;
@@ -167,15 +171,23 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
                 t = next++;
                 B.running++;
             }
-            ToyFCN fcn(B, t, multi_);
-            Minuit2::MnUserParameters up = static_cast<const Minuit2::MnUserParameters &>(params);
-            Minuit2::MnMigrad migrad(fcn, up, Minuit2::MnStrategy(strategy));
-            Minuit2::FunctionMinimum m = migrad(0, tolerance);
-            values_[t] = m.UserParameters().Params();
-            errors_[t] = m.UserParameters().Errors();
-            valid_[t]  = m.IsValid() ? 1 : 0;
-            nfcn_[t]   = (int)m.NFcn();
-            fval_[t]   = m.Fval();
+            try {
+                ToyFCN fcn(B, t, multi_);
+                Minuit2::MnUserParameters up = static_cast<const Minuit2::MnUserParameters &>(params);
+                Minuit2::MnMigrad migrad(fcn, up, Minuit2::MnStrategy(strategy));
+                Minuit2::FunctionMinimum m = migrad(0, tolerance);
+                values_[t] = m.UserParameters().Params();
+                errors_[t] = m.UserParameters().Errors();
+                valid_[t]  = m.IsValid() ? 1 : 0;
+                nfcn_[t]   = (int)m.NFcn();
+                fval_[t]   = m.Fval();
+            } catch(...) {
+                // the toy ends as an invalid fit; the exception is rethrown after the join
+                valid_[t] = 0;
+                std::lock_guard<std::mutex> lk(B.mu);
+                if(!worker_failure)
+                    worker_failure = std::current_exception();
+            }
             std::lock_guard<std::mutex> lk(B.mu);
             B.running--;
             B.cv_req.notify_one();
@@ -190,12 +202,16 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
     // goes on meanwhile), launch once every running toy waits -- or when no new
     // request came for 200 us -- and answer the toys of that launch
     std::vector<uint8_t> act(T, 0), prepared(T, 0);
+    // calculateNLL's extended term (AddPdf.cu:187-204) of each prepared point, with the
+    // toy's own event count; zero for every other top-level PDF
+    std::vector<std::vector<double>> extra(T);
     std::vector<double> out(multi_ ? (size_t)T * GF_BATCH_MAX_POINTS : (size_t)T);
     std::vector<int> taken, serve;
     std::exception_ptr failure;
     bool round_failed = false;
     int64_t points    = 0;
     auto prepare = [&](int t) {
+        extra[t].assign(B.xn[t], 0.0);
         for(size_t k = 0; k < B.xn[t]; k++) {
             // push the point into the shared Variables, then the host arrays as
             // Params::from_minuit_vector: flag the values that move, so that an
@@ -209,6 +225,7 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
             for(const auto &sl : slots)
                 host_parameters[sl.second] = sl.first.getValue();
             pdf_->normalize();
+            extra[t][k] = pdf_->nllExtraTermFor((fptype)count_[t]);
             if(multi_)
                 engine::check(gf_batch_add_point(batch_, t, host_parameters.data(), host_normalizations.data()),
                               "gf_batch_add_point");
@@ -244,6 +261,9 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
                                        out.begin() + (size_t)t * GF_BATCH_MAX_POINTS + B.xn[t]);
                 else
                     B.result[t].assign(1, out[t]);
+                if(!failed)
+                    for(size_t k = 0; k < B.result[t].size() && k < extra[t].size(); k++)
+                        B.result[t][k] += extra[t][k];
                 B.pending[t] = 0;
                 B.done[t]    = 1;
                 B.waiting--;
@@ -375,6 +395,8 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
     stats_.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if(failure)
         std::rethrow_exception(failure);
+    if(worker_failure)
+        std::rethrow_exception(worker_failure);
 }
 
 }  // namespace GooFit

@@ -79,13 +79,15 @@ double AddPdf::calculateNLL() {
     return ret;
 }
 
-double AddPdf::nllExtraTerm() const {
+double AddPdf::nllExtraTerm() const { return nllExtraTermFor(numEvents); }
+
+double AddPdf::nllExtraTermFor(fptype nevents) const {
     if(!extended)
         return 0.0;
     fptype expEvents = 0;
     for(size_t i = 0; i < components.size(); ++i)
         expEvents += parametersList[i].getValue();
-    return expEvents - numEvents * std::log(expEvents);
+    return expEvents - nevents * std::log(expEvents);
 }
 
 // ---------------------------------------------------------------- ProdPdf

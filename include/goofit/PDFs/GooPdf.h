@@ -41,6 +41,8 @@ class GooPdf : public PdfBase {
     /// Term calculateNLL adds to the metric sum (the extended-likelihood term of an
     /// extended AddPdf), at the current parameters.
     virtual double nllExtraTerm() const { return 0.0; }
+    /// The same term for a data set of nevents events (a toy's share of the rows).
+    virtual double nllExtraTermFor(fptype) const { return 0.0; }
     /// The normalisation of this node is positive (calculateNLL aborts otherwise).
     bool normalisationPositive() const;
     std::vector<fptype> evaluateAtPoints(Observable var);

@@ -12,6 +12,7 @@ class AddPdf : public CombinePdf {
     fptype normalize() override;
     double calculateNLL() override;
     double nllExtraTerm() const override;
+    double nllExtraTermFor(fptype nevents) const override;
     bool hasAnalyticIntegral() const override { return false; }
 
   private:
