@@ -20,42 +20,58 @@ namespace GooFit {
 
 // ---------------------------------------------------------------- Params
 
+// The Minuit side of each free or fixed Variable of the tree.  A Variable enters
+// Minuit2 as a constant when fixed, as an unbounded parameter when its limits
+// coincide, and as a bounded one otherwise (the three MnUserParameters::Add forms
+// the reference uses, src/goofit/Params.cpp:14-35); its Minuit index is stored in
+// the Variable (shared by every copy of it in the tree).
+namespace {
+unsigned int enter_minuit(Minuit2::MnUserParameters &up, Variable &var) {
+    const std::string &name = var.getName();
+    bool fresh;
+    if(var.IsFixed())
+        fresh = up.Add(name, var.getValue());
+    else if(var.getLowerLimit() == var.getUpperLimit())
+        fresh = up.Add(name, var.getValue(), var.getError());
+    else
+        fresh = up.Add(name, var.getValue(), var.getError(), var.getLowerLimit(), var.getUpperLimit());
+    // two distinct parameters with one name would share one Minuit slot
+    if(!fresh)
+        throw std::runtime_error("Params: the parameter name " + name + " is used by two parameters of the tree");
+    const unsigned int k = up.Index(name);
+    var.setFitterIndex((int)k);
+    return k;
+}
+}  // namespace
+
 Params::Params(PdfBase &pdf)
-    : pdf_(&pdf) {
-    vars_ = pdf_->getParameters();
-    for(Variable &var : vars_) {
-        bool added;
-        if(var.IsFixed())
-            added = Add(var.getName(), var.getValue());
-        else if(var.getLowerLimit() == var.getUpperLimit())
-            added = Add(var.getName(), var.getValue(), var.getError());
-        else
-            added = Add(var.getName(), var.getValue(), var.getError(), var.getLowerLimit(), var.getUpperLimit());
-        if(!added)
-            throw std::runtime_error("The name " + var.getName() + " appears more than once!");
-        var.setFitterIndex((int)Index(var.getName()));
-    }
+    : vars_(pdf.getParameters())
+    , pdf_(&pdf) {
+    for(Variable &var : vars_)
+        enter_minuit(*this, var);
     num_ = vars_.size();
 }
 
+// Minuit state -> Variables and this parameter set, index by index.
 void Params::SetGooFitParams(const Minuit2::MnUserParameterState &input) {
     for(Variable &var : vars_) {
-        int k = var.getFitterIndex();
-        var.setValue(input.Value(k));
-        var.setError(input.Error(k));
-        SetValue(k, var.getValue());
-        SetError(k, var.getError());
+        const unsigned int k = (unsigned int)var.getFitterIndex();
+        const double value = input.Value(k), error = input.Error(k);
+        var.setValue(value);
+        var.setError(error);
+        SetValue(k, value);
+        SetError(k, error);
     }
 }
 
+// The Minuit vector of the Variables' current values (slot = Minuit index).
 std::vector<double> Params::make_minuit_vector() const {
-    std::vector<double> v(max_fitter_index(vars_) + 1);
+    std::vector<double> x((size_t)max_fitter_index(vars_) + 1, 0.0);
     for(const Variable &var : vars_)
-        v.at(var.getFitterIndex()) = var.getValue();
-    return v;
+        x.at((size_t)var.getFitterIndex()) = var.getValue();
+    return x;
 }
 
-// Params::from_minuit_vector, src/goofit/Params.cpp:47-57
 // Params::from_minuit_vector, src/goofit/Params.cpp:47-57: the changed flags, the
 // values minus the blinding offsets into every parameter updateVariable would set,
 // then the parameter array as updateParameters fills it -- the tree walks of those

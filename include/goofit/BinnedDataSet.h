@@ -35,7 +35,7 @@ class BinnedDataSet : public DataSet {
     fptype getBinVolume(size_t bin) const;
     /// sqrt(content) unless errors were set.
     fptype getBinError(size_t bin) const;
-    std::vector<size_t> getDiminsions() const { return binsizes; }
+    std::vector<size_t> getDiminsions() const;
     size_t getNumBins() const;
     fptype getNumWeightedEvents() const;
 
@@ -43,14 +43,23 @@ class BinnedDataSet : public DataSet {
     void setBinError(unsigned int bin, fptype value);
 
   private:
-    std::vector<size_t> convertValuesToBins(const std::vector<fptype> &vals) const;
-    size_t localToGlobal(const std::vector<size_t> &locals) const;
-    std::vector<size_t> globalToLocal(size_t global) const;
-    void collectBins();
+    /// One histogram axis: the observable's range, its bin count, and the step of
+    /// its local bin in the global bin number (the product of the counts of the
+    /// axes before it: the first observable varies fastest).
+    struct Axis {
+        fptype lower, upper;
+        size_t bins, stride;
+    };
+    /// Local bin of value v on axis a: v clamped into [lower, upper] (with a
+    /// warning), then floor((v - lower) / bin size); v == upper gives `bins`.
+    size_t localBin(size_t a, fptype v) const;
+    /// Local bin of axis a inside global bin `bin`.
+    size_t localOf(size_t a, size_t bin) const { return (bin / axes[a].stride) % axes[a].bins; }
+    void buildAxes();
 
-    std::vector<size_t> binsizes;
+    std::vector<Axis> axes;
     std::vector<fptype> binvalues;
-    std::vector<fptype> binerrors;
+    std::vector<fptype> binerrors;  // empty until the first setBinError
 };
 
 }  // namespace GooFit
