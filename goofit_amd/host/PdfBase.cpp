@@ -296,36 +296,44 @@ void PdfBase::setData(DataSet *data) {
     engine::check(gf_events_upload(c, (int)(dims + 2), (int64_t)numEntries, ptrs.data()), "gf_events_upload");
 }
 
+// PdfBase::fillMCDataSimple, src/goofit/PdfBase.cpp:154-203.  Events of the lone
+// observable drawn from the PDF tabulated at its bin centres: a cell by inverse CDF
+// of the tabulated values, then a uniform position inside the cell.  Each event takes
+// two draws of a std::mt19937 seeded with `seed` (0: a random seed) through
+// std::uniform_real_distribution -- the cell's uniform in [0, 1), then the offset in
+// [-1/2, 1/2) bin widths -- so a given seed yields the reference's events.  The new
+// events are appended to the PDF's current data set, which is set again at the end
+// so that the device holds them.
 void PdfBase::fillMCDataSimple(size_t events, unsigned int seed) {
     if(observablesList.size() != 1)
         throw GeneralError("You can only fill MC on a 1D dataset with a simple fill");
-    Observable var = observablesList.at(0);
-    UnbinnedDataSet data{var};
-    data.fillWithGrid();
-    auto *origdata = getData();
-    if(origdata == nullptr)
+    DataSet *target = getData();
+    if(target == nullptr)
         throw GeneralError("Can't run on a PDF with no DataSet to fill!");
-    setData(&data);
-    std::vector<double> pdfValues = dynamic_cast<GooPdf *>(this)->getCompProbsAtDataPoints()[0];
-    if(seed == 0) {
-        std::random_device rd;
-        seed = rd();
-    }
-    std::mt19937 gen(seed);
-    std::uniform_real_distribution<> unihalf(-.5, .5);
-    std::uniform_real_distribution<> uniwhole(0.0, 1.0);
-    std::vector<double> integral(pdfValues.size());
-    std::partial_sum(pdfValues.begin(), pdfValues.end(), integral.begin());
-    const double total = integral.back();
-    for(double &v : integral)
-        v /= total;
+    Observable obs = observablesList.front();
+
+    // the PDF at the bin centres (the grid kernel's cells), evaluated on the device
+    UnbinnedDataSet cells{obs};
+    cells.fillWithGrid();
+    setData(&cells);
+    std::vector<double> cdf = dynamic_cast<GooPdf &>(*this).getCompProbsAtDataPoints().front();
+    // running sum in cell order, as a fraction of the total
+    for(size_t k = 1; k < cdf.size(); k++)
+        cdf[k] += cdf[k - 1];
+    const double total = cdf.back();
+    for(double &c : cdf)
+        c /= total;
+
+    std::mt19937 engine(seed != 0 ? seed : std::random_device{}());
+    std::uniform_real_distribution<> pick_cell(0.0, 1.0), within_cell(-.5, .5);
+    const double width = obs.getBinSize();
     for(size_t i = 0; i < events; i++) {
-        double r = uniwhole(gen);
-        size_t j = std::lower_bound(integral.begin(), integral.end(), r) - integral.begin();
-        var.setValue(data.getValue(var, j) + var.getBinSize() * unihalf(gen));
-        origdata->addEvent();
+        const double u    = pick_cell(engine);
+        const size_t cell = (size_t)(std::lower_bound(cdf.begin(), cdf.end(), u) - cdf.begin());
+        obs.setValue(cells.getValue(obs, cell) + width * within_cell(engine));
+        target->addEvent();
     }
-    setData(origdata);
+    setData(target);
 }
 
 ROOT::Minuit2::FunctionMinimum PdfBase::fitTo(DataSet *data, int verbosity) {
