@@ -84,32 +84,60 @@ def time_steps(pdf, params, steps, warmup, sync):
 TRAIN = dict(steps=500, batch=8192, lr=6e-3)  # proposal training: ~0.06 s, relative variance ~0.25/point
 
 
+def grid_integral(model, pdf, nb):
+    """The model's normalisation integral on an nb x nb midpoint grid (numbins restored)."""
+    o12, o13 = model.observables[0], model.observables[1]
+    old = (o12.numbins, o13.numbins)
+    o12.numbins, o13.numbins = nb, nb
+    pdf.setNormalisationGrid()
+    val = pdf.normalize()
+    o12.numbins, o13.numbins = old
+    pdf.setNormalisationGrid()
+    return val
+
+
+JUDGE_GRIDS = (2000, 2828, 4000, 5657, 8000)  # cells per axis; each step halves the cell area
+
+
 def norm_error_study(model, pdf, train_events, sync, t_train_gen, target=1e-4, seeds=6):
     """Normalisation-integral error at equal Monte-Carlo points, uniform phase-space MC vs the
-    learned proposal, against a uniform FP64 reference integral of 1e8 and 1e9 points; then a
-    MIGRAD fit with each method at the point count that reaches `target` relative error.  The learned method is charged its
-    training-event generation and training.  Leaves pdf on its midpoint grid."""
+    learned proposal, judged against a converged deterministic integral: the midpoint grid
+    refined until successive grids agree far below the errors measured (the finest of
+    JUDGE_GRIDS); the 1e8-point uniform FP64 Monte-Carlo integral is reported beside it.  Then
+    a MIGRAD fit with each method at the point count that reaches `target` relative error,
+    and one on the midpoint grid.  The learned method is charged its training-event
+    generation and training.  Leaves pdf on its midpoint grid."""
     import goofit_amd as G
     from goofit_amd import workloads as W
     out = {}
     truth = {n: v.value for n, v in model.params.items()}
     grid_norm = pdf.normalize()
-    # reference integral: uniform FP64 Monte Carlo, 1e8 points and 1e9 for a tighter one
-    refs = {}
-    for npts, seed in ((100_000_000, 991), (1_000_000_000, 992)):
+    # judge: midpoint grids of growing resolution (the boundary cells make the error O(cell size))
+    ladder = []
+    for nb in JUDGE_GRIDS:
         t0 = time.perf_counter()
-        pdf.setNormalisationPoints(npts, seed)
-        val = pdf.normalize()
+        val = grid_integral(model, pdf, nb)
         sync()
-        refs[npts] = dict(points=npts, value=val, seconds=round(time.perf_counter() - t0, 3),
-                          rel_stat_err=math.sqrt(pdf.normalisationRelativeVariance(16) / npts),
-                          grid_vs_reference=grid_norm / val - 1)
-    ref_norm = refs[1_000_000_000]["value"]
-    out["reference"] = refs[1_000_000_000]
-    out["reference_1e8"] = refs[100_000_000]
-    out["reference_1e8"]["vs_1e9"] = refs[100_000_000]["value"] / ref_norm - 1
-    log(f"reference integral: 1e9 uniform points (rel. stat. error {out['reference']['rel_stat_err']:.1e}); "
-        f"1e8-point reference / 1e9 - 1 = {out['reference_1e8']['vs_1e9']:.2e}; grid / reference - 1 = {grid_norm / ref_norm - 1:.2e}")
+        ladder.append(dict(cells_per_axis=nb, value=val, seconds=round(time.perf_counter() - t0, 3)))
+    for a, b in zip(ladder[:-1], ladder[1:]):
+        b["vs_previous"] = b["value"] / a["value"] - 1
+    ref_norm = ladder[-1]["value"]
+    # first-order convergence (cell size falls by sqrt 2 per step): the finest grid's error is
+    # at most the last refinement / (sqrt 2 - 1)
+    out["reference"] = dict(kind="midpoint grid", cells_per_axis=JUDGE_GRIDS[-1], value=ref_norm,
+                            convergence=ladder, est_rel_err=abs(ladder[-1]["vs_previous"]) / (math.sqrt(2) - 1),
+                            fit_grid_vs_reference=grid_norm / ref_norm - 1)
+    # the 1e8-point uniform FP64 Monte-Carlo integral, against the judge
+    t0 = time.perf_counter()
+    pdf.setNormalisationPoints(100_000_000, 991)
+    val = pdf.normalize()
+    sync()
+    out["reference_1e8"] = dict(points=100_000_000, value=val, seconds=round(time.perf_counter() - t0, 3),
+                                rel_stat_err=math.sqrt(pdf.normalisationRelativeVariance(16) / 1e8),
+                                vs_grid_reference=val / ref_norm - 1)
+    log(f"reference integral: {JUDGE_GRIDS[-1]}^2 midpoint grid, last refinement {ladder[-1]['vs_previous']:.1e}; "
+        f"1e8 uniform points / grid - 1 = {out['reference_1e8']['vs_grid_reference']:.2e} "
+        f"(stat. {out['reference_1e8']['rel_stat_err']:.1e}); fit grid / reference - 1 = {grid_norm / ref_norm - 1:.2e}")
 
     def errors(n, proposal, base):
         e, v = [], []
@@ -128,8 +156,11 @@ def norm_error_study(model, pdf, train_events, sync, t_train_gen, target=1e-4, s
     q.build(512, 0.1)
     sync()
     t_train = time.perf_counter() - t0
+    # retraining trigger on: the first points drawn set the reference weight variance; a
+    # later draw above 4x it (the model moved away from the training events) retrains
+    q.setRetraining(events=len(train_events), steps=TRAIN["steps"], batch=TRAIN["batch"], lr=TRAIN["lr"], grid=512,
+                    alpha=0.1, factor=4.0)
     lrn = [errors(n, q, 2000 + 10 * i) for i, n in enumerate((100_000, 1_000_000))]
-    q.setReferenceVariance(lrn[-1]["rel_var_per_point"])
     out.update(uniform=uni, learned=lrn, train=dict(events=len(train_events), steps=TRAIN["steps"],
                                                     batch=TRAIN["batch"], lr=TRAIN["lr"],
                                                     seconds=round(t_train, 3), generation_s=round(t_train_gen, 3),
@@ -157,10 +188,35 @@ def norm_error_study(model, pdf, train_events, sync, t_train_gen, target=1e-4, s
         fits[label] = dict(points=n, expected_rel_err=math.sqrt(var / n), wall_s=round(wall, 3), nfcn=m.NFcn(),
                            valid=bool(m.IsValid()), values=list(m.values()), errors=list(m.errors()))
         log(f"{label} fit at {n:.3g} points: {wall:.2f} s, {m.NFcn()} calls")
+    # the reference's method from the same start: the midpoint-grid normalisation
+    rng = np.random.default_rng(11)
+    for k in names:
+        model.params[k].value = truth[k] * (1.0 + 0.05 * rng.standard_normal())
+    pdf.setNormalisationGrid()
+    fm = G.FitManager(pdf)
+    fm.setVerbosity(0)
+    sync()
+    t0 = time.perf_counter()
+    m = fm.fit()
+    sync()
+    wall = time.perf_counter() - t0
+    fits["grid"] = dict(cells=pdf_grid_cells(model), wall_s=round(wall, 3), nfcn=m.NFcn(), valid=bool(m.IsValid()),
+                        values=list(m.values()), errors=list(m.errors()))
+    log(f"grid fit: {wall:.2f} s, {m.NFcn()} calls")
+    vg, eg = np.array(fits["grid"]["values"]), np.array(fits["grid"]["errors"])
     vu, vl = np.array(fits["uniform"]["values"]), np.array(fits["learned"]["values"])
     eu = np.array(fits["uniform"]["errors"])
     ok = eu > 0
     fits["max_param_shift_sigma"] = float(np.max(np.abs(vu - vl)[ok] / eu[ok])) if ok.any() else None
+    okg = eg > 0
+    for label in ("uniform", "learned"):
+        shift = float(np.max(np.abs(np.array(fits[label]["values"]) - vg)[okg] / eg[okg])) if okg.any() else None
+        fits[label]["max_shift_from_grid_fit_sigma"] = shift
+        # the shift is the normalisation's Monte-Carlo fluctuation, which falls as 1/sqrt(points):
+        # the point count at which it would reach 0.01 sigma
+        if shift is not None and shift > 0:
+            fits[label]["points_for_0p01_sigma"] = int(math.ceil(fits[label]["points"] * (shift / 0.01) ** 2))
+        log(f"{label} fit: largest shift from the grid fit {shift:.3f} sigma")
     fits["learned_incl_training_s"] = round(fits["learned"]["wall_s"] + t_train, 3)
     fits["learned_incl_training_and_generation_s"] = round(fits["learned"]["wall_s"] + t_train + t_train_gen, 3)
     fits["learned_faster_than_uniform"] = fits["learned_incl_training_and_generation_s"] < fits["uniform"]["wall_s"]
@@ -172,6 +228,11 @@ def norm_error_study(model, pdf, train_events, sync, t_train_gen, target=1e-4, s
     pdf.setNormalisationGrid()
     out["fits"] = fits
     return out
+
+
+def pdf_grid_cells(model):
+    """Cells of the model's (m12, m13) normalisation grid."""
+    return int(model.observables[0].numbins) * int(model.observables[1].numbins)
 
 
 def tddp_workload(args, sync, n=1_000_000, grid=1414):
@@ -390,6 +451,24 @@ def pull_study(ntoys, mean_events, sync, threads=512, nbins=1600):
     return out
 
 
+def host_cpu():
+    """The host CPU the baseline ran on: model name, logical CPUs of the machine and of this process."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = None
+    return dict(model=model, logical_cpus=os.cpu_count(), usable_cpus=usable,
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
+
+
 def cpu_baseline_dalitz(flat, decay, m12, m13, target_s=10.0):
     """Oracle (C/OpenMP restatement of the reference algorithm) NLL over the same events."""
     import oracle as O
@@ -413,7 +492,7 @@ def cpu_baseline_dalitz(flat, decay, m12, m13, target_s=10.0):
         v = prog.reduce_metric(aos, "NLL")
     dt = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=n * reps / dt, unit="events/s", cores=threads, kind="port",
+    return dict(value=n * reps / dt, unit="events/s", cores=threads, kind="port", host=host_cpu(),
                 sample=f"{reps} NLL evaluations of the {n}-event 16-resonance Dalitz sample (cached lineshapes; "
                        f"lineshape cache build {t_cache:.2f} s not included)",
                 nll=v, cache_build_s=t_cache)
@@ -458,6 +537,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the normalisation-error study")
+    ap.add_argument("--run-seed", type=int, default=None,
+                    help="seed of the reseeded widths and couplings of the normalisation-error study (default: random)")
     ap.add_argument("--toys", type=int, default=4096, help="batched toy fits (0: skip)")
     ap.add_argument("--pull-study", type=int, default=0, metavar="NTOYS",
                     help="also fit NTOYS 8-resonance Dalitz toys of 2e5 events each for their pulls")
@@ -617,17 +698,28 @@ def main():
 
     norm_study = None
     if not args.no_learned:
-        log("normalisation-integral error study: uniform MC vs learned proposal")
+        # the learned proposal on the 16-resonance model with widths and couplings reseeded per run
+        run_seed = args.run_seed if args.run_seed is not None else int.from_bytes(os.urandom(3), "little")
+        log(f"normalisation-integral error study: uniform MC vs learned proposal (model reseeded with {run_seed})")
+        m3 = W.build_dalitz(nbins=args.grid, coup_seed=run_seed, width_seed=run_seed + 1)
+        d12, d13 = W.generate_dalitz(m3, args.events, seed=run_seed + 2)
+        data3 = W.dalitz_dataset(m3, d12, d13)
         sync()
         t0 = time.perf_counter()
-        t12, t13 = W.generate_dalitz(model, 2_000_000, seed=8)  # the proposal's training events
-        train_events = W.dalitz_dataset(model, t12, t13)
+        t12, t13 = W.generate_dalitz(m3, 2_000_000, seed=run_seed + 3)  # the proposal's training events
+        train_events = W.dalitz_dataset(m3, t12, t13)
         sync()
         t_train_gen = time.perf_counter() - t0
+        m3.pdf.setData(data3)
+        m3.pdf.setDataSize(len(d12))
+        m3.pdf.setForceIntegrals()
+        m3.pdf.calculateNLL()
+        norm_study = norm_error_study(m3, m3.pdf, train_events, sync, t_train_gen)
+        norm_study["run_seed"] = run_seed
+        norm_study["widths"] = {r[0]: r[3] for r in W.reseed_widths(W.DALITZ_RESONANCES, run_seed + 1)}
+        del m3, data3, train_events
         pdf.setData(data)
         pdf.setDataSize(len(m12))
-        pdf.setForceIntegrals()
-        norm_study = norm_error_study(model, pdf, train_events, sync, t_train_gen)
         pdf.setForceIntegrals()
         pdf.calculateNLL()
 
@@ -659,6 +751,20 @@ def main():
             workloads[name] = dict(events=nevt, nll_events_per_s=round(nevt * max(args.steps, 20) / dtw, 1),
                                    ms_per_step=round(dtw / max(args.steps, 20) * 1e3, 4), kernel_ms=round(k_ms, 5),
                                    hbm_gbs=round(bpe * nevt / (k_ms * 1e-3) / 1e9, 1))
+            # the whole MIGRAD fit of the workload's free parameters from +20 % starting values
+            fmdl = builder(start=1.2)
+            fmdl.pdf.setData(d)
+            fm = G.FitManager(fmdl.pdf)
+            fm.setVerbosity(0)
+            sync()
+            t0 = time.perf_counter()
+            m = fm.fit()
+            sync()
+            t_fit = time.perf_counter() - t0
+            workloads[name]["fit"] = dict(
+                wall_s=round(t_fit, 4), nfcn=m.NFcn(), valid=bool(m.IsValid()), start="truth x 1.2",
+                values={k: v.value for k, v in fmdl.params.items()},
+                errors={k: v.error for k, v in fmdl.params.items()})
             log(name, workloads[name])
             del d
         workloads["tddp16_1e6_threegauss"] = tddp_workload(args, sync)

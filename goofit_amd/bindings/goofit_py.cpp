@@ -485,7 +485,26 @@ PYBIND11_MODULE(_goofit, m) {
         .def("setParameters", &DalitzProposal::setParameters)
         .def("setReferenceVariance", &DalitzProposal::setReferenceVariance)
         .def("getReferenceVariance", &DalitzProposal::getReferenceVariance)
-        .def("needsRetraining", &DalitzProposal::needsRetraining, py::arg("relvar"), py::arg("factor") = 4.0);
+        .def("needsRetraining", &DalitzProposal::needsRetraining, py::arg("relvar"), py::arg("factor") = 4.0)
+        .def(
+            "setRetraining",
+            [](DalitzProposal &q, int64_t events, int steps, int batch, float lr, int grid, double alpha,
+               double factor) {
+                DalitzProposal::Retraining r;
+                r.events = events;
+                r.steps  = steps;
+                r.batch  = batch;
+                r.lr     = lr;
+                r.grid   = grid;
+                r.alpha  = alpha;
+                r.factor = factor;
+                q.setRetraining(r);
+            },
+            py::arg("events") = 2000000, py::arg("steps") = 500, py::arg("batch") = 8192, py::arg("lr") = 6e-3f,
+            py::arg("grid") = 512, py::arg("alpha") = 0.1, py::arg("factor") = 4.0)
+        .def("disableRetraining", &DalitzProposal::disableRetraining)
+        .def("retrainingEnabled", &DalitzProposal::retrainingEnabled)
+        .def("retrainCount", &DalitzProposal::retrainCount);
 
     m.def("inDalitz", &inDalitz);
 
@@ -725,6 +744,7 @@ PYBIND11_MODULE(_goofit, m) {
     me.attr("TUNE_FAST_BLOCKS_PER_CU")  = (int)GF_TUNE_FAST_BLOCKS_PER_CU;
     me.attr("TUNE_SPIN_US")             = (int)GF_TUNE_SPIN_US;
     me.attr("TUNE_ZERO_OUTSIDE")        = (int)GF_TUNE_ZERO_OUTSIDE;
+    me.attr("TUNE_FLAT")                = (int)GF_TUNE_FLAT;
     me.def("callnumber", []() { return host_callnumber; });
     me.def("flattened", []() {
         return py::make_tuple(host_function_table, host_parameters, host_constants, host_observables,

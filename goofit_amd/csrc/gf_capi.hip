@@ -125,6 +125,7 @@ struct gf_ctx {
     double *h_result         = nullptr;  // pinned staging for small copies
     int32_t *d_err           = nullptr;  // device error word (bit 0: cache index out of range, bit 1: node kind not compiled in)
     bool fast_dalitz         = false;    // program is [DALITZ, MULTI_POLY, DALITZ_END, END]
+    bool flat                = false;    // program is one level of 1-D leaves (flat_program)
     double *d_gram_partials  = nullptr;
     size_t cap_gram_partials = 0;
     double *d_gram_tiles     = nullptr;  // 768
@@ -143,7 +144,7 @@ struct gf_ctx {
     uint32_t res_seq = 0;  // sequence number of the last reduction launched
 
     // launch-shape knobs (gf_tune)
-    int tune[GF_TUNE_COUNT] = {1, 1, 4, 4, 2000, 1};
+    int tune[GF_TUNE_COUNT] = {1, 1, 4, 4, 2000, 1, 1};
 
     uint64_t events_gen = 1;  // bumped whenever the event columns may change
 
@@ -701,6 +702,29 @@ gf_status subprogram(gf_ctx *ctx, int func, int &pc) {
     return upload_ins(ctx);
 }
 
+// The root program is flat (k_reduce_flat): [leaf, END], or one level-0 AddPdf /
+// ProdPdf whose children are all leaves -- PUSH, (leaf, combine)..., [POP], END --
+// with Gaussian / Exp / Exp-offset / ExpGaus leaves over columns < GF_FLAT_COLS.
+bool flat_program(const gf_ctx *ctx) {
+    if(!ctx->dalitz.empty() || ctx->max_col < 0 || ctx->max_col >= GF_FLAT_COLS)
+        return false;
+    auto leaf = [](int op) { return op == GI_GAUSS || op == GI_EXP || op == GI_EXP_OFF || op == GI_EXPGAUS; };
+    const std::vector<GfIns> &I = ctx->ins;
+    size_t pc = 0;
+    for(; pc < I.size() && I[pc].op != GI_END; ++pc) {
+        const int op = I[pc].op;
+        if(op < GI_LVL) {
+            if(!leaf(op))
+                return false;
+            continue;
+        }
+        const int kind = (op - GI_LVL) >> 2, lvl = (op - GI_LVL) & 3;
+        if(lvl != 0 || kind == GK_DALITZ || kind == GK_DALITZ_END || kind == GK_TDDP || kind == GK_TDDP_END)
+            return false;
+    }
+    return pc < I.size() && pc > 0;
+}
+
 // Flush pending parameter/normalisation updates through the pinned staging buffer.
 gf_status flush_uploads(gf_ctx *ctx) {
     if(!ctx->params_dirty && !ctx->norms_dirty)
@@ -852,6 +876,18 @@ bool use_fast_dalitz(const gf_ctx *ctx, int metric) {
     return it != ctx->caches.end() && it->second.wave && it->second.n >= ctx->n;
 }
 
+// The flat kernel applies to this launch.
+bool use_flat(const gf_ctx *ctx, int metric) {
+    return ctx->tune[GF_TUNE_FLAT] != 0 && ctx->flat && metric < GF_METRIC_BINAVG;
+}
+
+// Events per lane of the flat kernel: 4 hides more latency on large sets (1e7 events of
+// the 3-D ProdPdf: 123 us against 140 us with 2), 2 keeps small sets on more workgroups.
+int flat_ept(const gf_ctx *ctx) {
+    const int t = ctx->tune[GF_TUNE_FLAT];
+    return t == 1 ? (ctx->n >= ((int64_t)1 << 20) ? 4 : 2) : t;
+}
+
 // The fast path's efficiency polynomial has maximum degree 0.
 bool const_efficiency(const gf_ctx *ctx) { return (int)ctx->consts[ctx->ins[1].c] == 0; }
 
@@ -922,6 +958,9 @@ gf_status launch_reduce(gf_ctx *ctx, int metric, hipEvent_t mark = nullptr, bool
         rc = gfk::launch_dalitz_fast(a, have_inl ? &inl : nullptr, const_efficiency(ctx), zero_outside(ctx), metric, ctx->d_partials,
                                      ctx->d_res_mapped, nullptr, fast_blocks(ctx, ctx->n),
                                      ctx->stream);
+    else if(use_flat(ctx, metric))
+        rc = gfk::launch_reduce_flat(a, have_inl ? &inl : nullptr, metric, ctx->max_col + 1, flat_ept(ctx),
+                                     ctx->d_partials, ctx->d_res_mapped, event_blocks(ctx, ctx->n), ctx->stream);
     else
         rc = gfk::launch_reduce_metric(a, have_inl ? &inl : nullptr, metric, ctx->d_partials, ctx->d_res_mapped,
                                        event_blocks(ctx, ctx->n), ctx->stream);
@@ -1079,6 +1118,10 @@ gf_status gf_ctx_set_tuning(gf_ctx *ctx, int key, int value) {
     if(!ctx)
         return fail(GF_ERR_INVALID, "null context");
     switch(key) {
+    case GF_TUNE_FLAT:
+        if(value != 0 && value != 1 && value != 2 && value != 4)
+            return fail(GF_ERR_INVALID, "tuning key %d takes 0, 1, 2 or 4 (got %d)", key, value);
+        break;
     case GF_TUNE_INLINE_ARGS:
     case GF_TUNE_FAST_DALITZ:
     case GF_TUNE_ZERO_OUTSIDE:
@@ -1361,6 +1404,7 @@ gf_status gf_program_set(gf_ctx *ctx, const int32_t *funcs, int nfunc, const dou
     ctx->fast_dalitz = ctx->ins.size() >= 4 && ctx->ins[0].op == GF_LVL_OP(GK_DALITZ, 0)
                        && ctx->ins[1].op == GI_MULTI_POLY && ctx->ins[2].op == GF_LVL_OP(GK_DALITZ_END, 0)
                        && ctx->ins[3].op == GI_END;
+    ctx->flat = flat_program(ctx);
     for(const DalitzNode &d : ctx->dalitz) {
         int cols[7] = {d.col_m12, d.col_m13, d.col_num, d.col_time, d.col_sigma, d.col_mis, d.col_tag};
         for(int c : cols)

@@ -160,6 +160,12 @@ struct Partial {
 int launch_reduce_metric(const GfProgArgs &a, const GfInline *inl, int metric, Partial *partials, GfResult *result,
                          int nblocks, void *stream);
 int launch_eval_metric(const GfProgArgs &a, const GfInline *inl, int metric, double *out, int nblocks, void *stream);
+// Flat program (one leaf, or one AddPdf / ProdPdf level of Gaussian / Exp / Exp-offset /
+// ExpGaus leaves over columns 0..ncols-1, ncols <= GF_FLAT_COLS), unbinned metrics:
+// the same sum as launch_reduce_metric; ept = 2 or 4 events per lane.
+#define GF_FLAT_COLS 4
+int launch_reduce_flat(const GfProgArgs &a, const GfInline *inl, int metric, int ncols, int ept, Partial *partials,
+                       GfResult *result, int nblocks, void *stream);
 // Coherent-sum fast path: program [DALITZ, MULTI_POLY efficiency, DALITZ_END, END],
 // event number = row index, every row inside the wave cache.
 // const_eff: the efficiency polynomial has degree 0 (a constant coefficient).

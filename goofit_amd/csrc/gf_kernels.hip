@@ -547,6 +547,169 @@ __global__ __launch_bounds__(GF_BLOCK) void k_reduce_metric(GfProgArgs A, GfInli
     store_partial(s, c, partials);
 }
 
+// ------------------------------------------------------------ flat trees
+
+// The event columns of a flat program (at most GF_FLAT_COLS), loaded for the whole
+// tile before any of them is used: a lane has every load of the tile in flight at
+// once instead of waiting for one column per leaf.
+template <int EPT>
+struct FlatSrc {
+    double v[EPT][GF_FLAT_COLS];
+    __device__ __forceinline__ double get(int a, int j) const {
+        switch(a) {
+        case 0: return v[j][0];
+        case 1: return v[j][1];
+        case 2: return v[j][2];
+        default: return v[j][3];
+        }
+    }
+};
+
+// A leaf of a flat program into T: the arithmetic of run()'s leaf cases.
+template <int EPT>
+__device__ __forceinline__ void flat_leaf(const GfIns &in, const GF_CONST double *__restrict__ P,
+                                          const FlatSrc<EPT> &src, double (&T)[EPT]) {
+    switch(in.op) {
+    case GI_GAUSS: {
+        const double m = P[in.p], sg = P[in.p + 1];
+#pragma unroll
+        for(int j = 0; j < EPT; ++j)
+            T[j] = gfd::gaussian(src.get(in.a, j), m, sg);
+    } break;
+    case GI_EXP: {
+        const double al = P[in.p];
+#pragma unroll
+        for(int j = 0; j < EPT; ++j)
+            T[j] = exp(al * src.get(in.a, j));
+    } break;
+    case GI_EXP_OFF: {
+        const double x0 = P[in.p], al = P[in.p + 1];
+#pragma unroll
+        for(int j = 0; j < EPT; ++j) {
+            double x = src.get(in.a, j);
+            x -= x0;
+            T[j] = exp(al * x);
+        }
+    } break;
+    default: {  // GI_EXPGAUS
+        const double m = P[in.p], sg = P[in.p + 1], al = P[in.p + 2];
+#pragma unroll
+        for(int j = 0; j < EPT; ++j)
+            T[j] = gfd::expgaus(src.get(in.a, j), m, sg, al);
+    } break;
+    }
+}
+
+// The reduction of a flat program -- one leaf, or one AddPdf / ProdPdf level over
+// Gaussian, Exp, Exp-with-offset and ExpGaus leaves, reading columns 0 .. ncols-1
+// (ncols <= GF_FLAT_COLS) -- with run()'s arithmetic in the same order (the same
+// bits as k_reduce_metric) but one accumulator instead of a stack of levels, no
+// Dalitz or polynomial cases, and the tile's columns loaded up front.
+template <int EPT, bool INL>
+__global__ __launch_bounds__(GF_BLOCK) void k_reduce_flat(GfProgArgs A, GfInline I, int metric, int ncols,
+                                                          Partial *partials) {
+    const GF_CONST double *__restrict__ P    = gfd::as_const(INL ? I.p : A.params);
+    const GF_CONST double *__restrict__ Nm   = gfd::as_const(INL ? I.n : A.norms);
+    const GF_CONST GfIns *__restrict__ prog  = gfd::as_const(A.ins);
+    const auto cols                          = gfd::as_const(A.cols);
+    const int64_t n    = A.nevents;
+    const int64_t tile = (int64_t)GF_BLOCK * EPT;
+    const double norm  = Nm[1];
+    double s = 0.0, c = 0.0;
+    FlatSrc<EPT> src;
+#pragma unroll
+    for(int j = 0; j < EPT; ++j)
+#pragma unroll
+        for(int k = 0; k < GF_FLAT_COLS; ++k)
+            src.v[j][k] = 0.0;
+    for(int64_t base = (int64_t)blockIdx.x * tile; base < n; base += (int64_t)gridDim.x * tile) {
+        int64_t row[EPT];
+        bool valid[EPT];
+#pragma unroll
+        for(int j = 0; j < EPT; ++j) {
+            const int64_t e = base + (int64_t)j * GF_BLOCK + threadIdx.x;
+            valid[j]        = e < n;
+            row[j]          = valid[j] ? e : n - 1;
+        }
+#pragma unroll
+        for(int k = 0; k < GF_FLAT_COLS; ++k)
+            if(k < ncols) {
+                const double *__restrict__ col = cols[k];
+#pragma unroll
+                for(int j = 0; j < EPT; ++j)
+                    src.v[j][k] = col[row[j]];
+            }
+        double T[EPT], acc[EPT];
+#pragma unroll
+        for(int j = 0; j < EPT; ++j)
+            T[j] = acc[j] = 0.0;
+        for(int pc = 0;; ++pc) {
+            const GfIns in = prog[pc];
+            if(in.op == GI_END)
+                break;
+            if(in.op < GI_LVL) {
+                flat_leaf<EPT>(in, P, src, T);
+                continue;
+            }
+            switch((in.op - GI_LVL) >> 2) {
+            case GK_PUSH0:
+#pragma unroll
+                for(int j = 0; j < EPT; ++j)
+                    acc[j] = 0.0;
+                break;
+            case GK_PUSH1:
+#pragma unroll
+                for(int j = 0; j < EPT; ++j)
+                    acc[j] = 1.0;
+                break;
+            case GK_ADD_ACC: {
+                const double w = P[in.p], nm = Nm[in.n];
+#pragma unroll
+                for(int j = 0; j < EPT; ++j)
+                    acc[j] += w * T[j] * nm;
+            } break;
+            case GK_ADD_LAST: {
+                double tw = 0;
+                for(int i = 0; i < in.k; ++i)
+                    tw += P[in.p + i];
+                const double nm = Nm[in.n];
+#pragma unroll
+                for(int j = 0; j < EPT; ++j)
+                    acc[j] += (1 - tw) * T[j] * nm;
+            } break;
+            case GK_ADDEXT_END: {
+                double tw = 0;
+                for(int i = 0; i < in.k; ++i)
+                    tw += P[in.p + i];
+#pragma unroll
+                for(int j = 0; j < EPT; ++j)
+                    T[j] = acc[j] / tw;
+            } break;
+            case GK_PROD_ACC: {
+                const double nm = Nm[in.n];
+#pragma unroll
+                for(int j = 0; j < EPT; ++j) {
+                    double cur = T[j] * nm;
+                    acc[j] *= cur;
+                }
+            } break;
+            default:  // GK_POP
+#pragma unroll
+                for(int j = 0; j < EPT; ++j)
+                    T[j] = acc[j];
+                break;
+            }
+        }
+#pragma unroll
+        for(int j = 0; j < EPT; ++j) {
+            const double v = gfd::metric(metric, T[j], 0.0, 0.0, norm, A.total_events);
+            if(valid[j])
+                gfd::kb_add(s, c, v);
+        }
+    }
+    store_partial(s, c, partials);
+}
+
 template <int EPT, bool INL, int FEAT>
 __global__ __launch_bounds__(GF_BLOCK) void k_eval_metric(GfProgArgs A, GfInline I, int metric, double *out) {
     const double *__restrict__ P  = INL ? I.p : A.params;
@@ -1476,6 +1639,22 @@ int launch_reduce_metric(const GfProgArgs &a, const GfInline *inl, int metric, P
     }
     return launch_finish(a, partials, nblocks, result, s);
 }
+int launch_reduce_flat(const GfProgArgs &a, const GfInline *inl, int metric, int ncols, int ept, Partial *partials,
+                       GfResult *result, int nblocks, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if(ncols < 1 || ncols > GF_FLAT_COLS || metric >= GF_METRIC_BINAVG_CODE)
+        return -1;
+    const dim3 g(nblocks), b(GF_BLOCK);
+    const GfInline none{};
+    switch((ept == 4 ? 2 : 0) + (inl ? 1 : 0)) {
+    case 0: hipLaunchKernelGGL((k_reduce_flat<2, false>), g, b, 0, s, a, none, metric, ncols, partials); break;
+    case 1: hipLaunchKernelGGL((k_reduce_flat<2, true>), g, b, 0, s, a, *inl, metric, ncols, partials); break;
+    case 2: hipLaunchKernelGGL((k_reduce_flat<4, false>), g, b, 0, s, a, none, metric, ncols, partials); break;
+    default: hipLaunchKernelGGL((k_reduce_flat<4, true>), g, b, 0, s, a, *inl, metric, ncols, partials); break;
+    }
+    return launch_finish(a, partials, nblocks, result, s);
+}
+
 int launch_eval_metric(const GfProgArgs &a, const GfInline *inl, int metric, double *out, int nblocks, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     const dim3 g(nblocks), b(GF_BLOCK);

@@ -189,12 +189,36 @@ void Amp3Body::setNormalisationPoints(int64_t n, unsigned long long seed, Dalitz
     if(proposal && !proposal->isBuilt())
         throw GeneralError("Amp3Body::setNormalisationPoints: the proposal has no table (DalitzProposal::build)");
     const double box[4] = {_m12.getLowerLimit(), _m12.getUpperLimit(), _m13.getLowerLimit(), _m13.getUpperLimit()};
-    engine::flush();
-    engine::check(gf_dalitz_set_norm_points(engine::ctx(), (int)functionIdx, proposal ? proposal->handle() : nullptr,
-                                            n, seed, box),
-                  "gf_dalitz_set_norm_points");
-    normPoints         = n;
-    forceRedoIntegrals = true;
+    auto draw = [&]() {
+        engine::flush();
+        engine::check(gf_dalitz_set_norm_points(engine::ctx(), (int)functionIdx,
+                                                proposal ? proposal->handle() : nullptr, n, seed, box),
+                      "gf_dalitz_set_norm_points");
+        normPoints         = n;
+        forceRedoIntegrals = true;
+    };
+    draw();
+    if(!proposal || !proposal->retrainingEnabled())
+        return;
+    // the retraining trigger: the weight variance of these points at the current couplings
+    // against the variance the proposal had when it was (re)built for the model
+    copyParams();
+    const int batches = n >= 16 ? 16 : (int)n;
+    if(batches < 2)
+        return;
+    const fptype relvar = normalisationRelativeVariance(batches);
+    if(proposal->getReferenceVariance() <= 0) {
+        proposal->setReferenceVariance(relvar);
+        return;
+    }
+    if(!proposal->needsRetraining(relvar, proposal->retraining().factor))
+        return;
+    // train again on events of the model at its current parameters, then redraw
+    UnbinnedDataSet events({_m12, _m13, _eventNumber});
+    generateEvents(events, (size_t)proposal->retraining().events, seed ^ 0x5eedf00dULL);
+    proposal->retrain(events, seed + 1);
+    draw();
+    proposal->setReferenceVariance(normalisationRelativeVariance(batches));
 }
 
 void Amp3Body::setNormalisationGrid() {

@@ -58,4 +58,19 @@ void DalitzProposal::setParameters(const std::vector<float> &p) {
     built_ = false;
 }
 
+void DalitzProposal::setRetraining(const Retraining &r) {
+    if(r.events < 1 || r.steps < 1 || r.batch < 1 || !(r.lr > 0) || r.grid < 2 || !(r.alpha >= 0 && r.alpha <= 1)
+       || !(r.factor > 1))
+        throw GeneralError("DalitzProposal::setRetraining: need events, steps, batch >= 1, lr > 0, grid >= 2, "
+                           "alpha in [0, 1] and factor > 1");
+    retrain_    = r;
+    retrain_on_ = true;
+}
+
+void DalitzProposal::retrain(const UnbinnedDataSet &events, unsigned long long seed) {
+    train(events, retrain_.steps, retrain_.batch, retrain_.lr, seed);
+    build(retrain_.grid, retrain_.alpha);
+    retrains_++;
+}
+
 }  // namespace GooFit

@@ -213,12 +213,26 @@ def build_prod3d(start=1.0, nbins=200):
     return Model(p, [x, y, z], dict(mu=mu, sigma=sigma, alpha=alpha, rate=rate), keep=[gx, ey, ez])
 
 
-def build_dalitz(nbins=1414, coup_seed=3, resonances=None, float_couplings=True, coup_scale=1.0):
+def reseed_widths(table, seed, lo=0.75, hi=1.25):
+    """The resonance table with every width scaled by a factor uniform in [lo, hi) drawn
+    from seed, kept inside [0.05, 0.4] GeV (a table width below 0.05 bounds it from below)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for (name, kind, mass, width, spin, pair) in table:
+        f = rng.uniform(lo, hi)
+        out.append((name, kind, mass, float(np.clip(width * f, min(0.05, width), 0.4)) if kind != "NONRES" else width, spin, pair))
+    return out
+
+
+def build_dalitz(nbins=1414, coup_seed=3, resonances=None, float_couplings=True, coup_scale=1.0, width_seed=None):
     """D0 -> pi+ pi- pi0 with the 16 resonances of DALITZ_RESONANCES, flat efficiency,
-    m12 and m13 on [0, 3] with an nbins x nbins normalisation grid."""
+    m12 and m13 on [0, 3] with an nbins x nbins normalisation grid.  width_seed: widths
+    reseeded by reseed_widths."""
     import goofit_amd as G
     G.require_api()
     table = resonances if resonances is not None else DALITZ_RESONANCES
+    if width_seed is not None:
+        table = reseed_widths(table, width_seed)
     m12 = G.Observable("m12", 0.0, 3.0)
     m13 = G.Observable("m13", 0.0, 3.0)
     m12.numbins = nbins

@@ -47,6 +47,30 @@ class DalitzProposal {
     double getReferenceVariance() const { return ref_var_; }
     bool needsRetraining(double relvar, double factor = 4.0) const { return ref_var_ > 0 && relvar > factor * ref_var_; }
 
+    /// Automatic retraining by Amp3Body::setNormalisationPoints: each time points are
+    /// drawn from this proposal, their relative weight variance is measured at the
+    /// current couplings; the first measurement becomes the reference, and a later one
+    /// above factor x the reference retrains the network on `events` events that the
+    /// model generates at its current parameters (train(steps, batch, lr), build(grid,
+    /// alpha)), draws the points again and takes their variance as the new reference.
+    struct Retraining {
+        int64_t events{2000000};
+        int steps{500};
+        int batch{8192};
+        float lr{6e-3f};
+        int grid{512};
+        double alpha{0.1};
+        double factor{4.0};
+    };
+    void setRetraining(const Retraining &r);
+    void disableRetraining() { retrain_on_ = false; }
+    bool retrainingEnabled() const { return retrain_on_; }
+    const Retraining &retraining() const { return retrain_; }
+    /// Retrains done by the trigger so far.
+    int retrainCount() const { return retrains_; }
+    /// Train and build again on `events` (one retraining of the trigger).
+    void retrain(const UnbinnedDataSet &events, unsigned long long seed);
+
     const double *box() const { return box_; }
     gf_mlp *handle() const { return mlp_; }
 
@@ -56,6 +80,9 @@ class DalitzProposal {
     gf_mlp *mlp_{nullptr};
     bool built_{false};
     double ref_var_{0.0};
+    bool retrain_on_{false};
+    Retraining retrain_;
+    int retrains_{0};
 };
 
 }  // namespace GooFit

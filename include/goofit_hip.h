@@ -111,7 +111,12 @@ enum gf_tune {
                                        current events, the coherent-sum kernel takes |sum c_i W_i|^2 as 0
                                        outside the Dalitz region from the cache (which holds 0 there)
                                        instead of reading m12 and m13 (default 1) */
-    GF_TUNE_COUNT              = 6
+    GF_TUNE_FLAT               = 6, /* a tree that is one leaf, or one product or sum of 1-D leaves
+                                       (Gaussian, Exp, ExpGaus) over at most 4 columns, runs a dedicated
+                                       kernel that loads every column of a tile before evaluating it, with
+                                       2 or 4 events per lane (the value), or 1: 4 from 2^20 events on,
+                                       2 below; 0: the general interpreter (default 1) */
+    GF_TUNE_COUNT              = 7
 };
 
 /* Set / read one knob (gf_tune); values out of range give GF_ERR_INVALID. */

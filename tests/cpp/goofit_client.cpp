@@ -121,8 +121,10 @@ int run_dalitz(int argc, char **argv) {
         std::string name, kind;
         double mass, width, re, im;
         unsigned int spin, pair;
-        if(!(ss >> name >> kind >> mass >> width >> spin >> pair >> re >> im))
+        if(line.find_first_not_of(" \t\r") == std::string::npos)
             continue;
+        if(!(ss >> name >> kind >> mass >> width >> spin >> pair >> re >> im))
+            throw GeneralError("malformed resonance line in " + std::string(argv[2]) + ": " + line);
         // the first coupling fixes the overall complex scale; the others float
         Variable *ar = first ? new Variable(name + "_amp_real", re) : new Variable(name + "_amp_real", re, 0.01, 0, 0);
         Variable *ai = first ? new Variable(name + "_amp_imag", im) : new Variable(name + "_amp_imag", im, 0.01, 0, 0);
@@ -142,6 +144,9 @@ int run_dalitz(int argc, char **argv) {
                                     spin, pair);
         dtop0pp.resonances.push_back(r);
     }
+
+    if(dtop0pp.resonances.empty())
+        throw GeneralError(std::string("no resonance in ") + argv[2]);
 
     Variable constantOne{"constantOne", 1};
     Variable constantZero{"constantZero", 0};

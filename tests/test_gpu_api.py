@@ -239,6 +239,47 @@ def test_dalitz_mc_normalisation_and_learned_proposal():
     assert abs((nll_q - nll_g) - len(ev12) * math.log(got_q / ref)) < 1e-6 * abs(nll_g)
 
 
+def test_proposal_retrains_when_the_model_moves():
+    """The retraining trigger of a learned proposal: the first points drawn set the reference
+    weight variance; after the couplings move so that the narrow f0(980) dominates (where the
+    proposal only has its uniform admixture) the variance exceeds 4x the reference, the next
+    setNormalisationPoints retrains on events generated from the moved model, and the redrawn
+    points are unbiased against the grid with a variance back near the trained level."""
+    m = W.build_dalitz(nbins=1000, coup_seed=5, width_seed=6)
+    pdf = m.pdf
+    ev12, ev13 = W.generate_dalitz(m, 200_000, seed=7)
+    data = W.dalitz_dataset(m, ev12, ev13)
+    pdf.setData(data)
+    pdf.setDataSize(len(ev12))
+    pdf.setForceIntegrals()
+    q = G.DalitzProposal(m.observables[0], m.observables[1], seed=3)
+    q.train(data, steps=600, batch=8192, lr=6e-3, seed=4)
+    q.build(256, 0.1)
+    q.setRetraining(events=200_000, steps=600, batch=8192, lr=6e-3, grid=256, alpha=0.1, factor=4.0)
+    n = 1_000_000
+    pdf.setNormalisationPoints(n, 21, q)
+    ref_var = q.getReferenceVariance()
+    assert ref_var > 0 and q.retrainCount() == 0
+    # the same model again: no retraining
+    pdf.setNormalisationPoints(n, 22, q)
+    assert q.retrainCount() == 0
+    # move the model: f0(980) coupling x 25
+    m.params["f0_980_re"].value *= 25.0
+    m.params["f0_980_im"].value *= 25.0
+    pdf.copyParams()
+    pdf.setNormalisationGrid()
+    grid = pdf.normalize()
+    pdf.setNormalisationPoints(n, 23)  # uniform points: the variance the proposal must beat
+    var_u = pdf.normalisationRelativeVariance(16)
+    pdf.setNormalisationPoints(n, 24, q)
+    assert q.retrainCount() == 1, (ref_var, q.getReferenceVariance())
+    new_var = q.getReferenceVariance()
+    got = pdf.normalize()
+    print(f"relative variance per point: trained {ref_var:.3f}, after retraining {new_var:.3f}, uniform {var_u:.3f}")
+    assert new_var < var_u / 4, (new_var, var_u)
+    assert abs(got / grid - 1) < 5 * math.sqrt(new_var / n) + 2e-4, (got, grid, new_var)
+
+
 def _exp_sample(rate, n, hi, seed):
     rng = np.random.default_rng(seed)
     v = rng.exponential(1 / rate, n)

@@ -48,7 +48,9 @@ def test_client_add1d_golden_nll(tmp_path):
 def _model_file(path, table, coups):
     with open(path, "w") as f:
         for (name, kind, mass, width, spin, pair), c in zip(table, coups):
-            f.write(f"{name} {kind} {mass!r} {width!r} {spin} {pair} {c.real!r} {c.imag!r}\n")
+            # plain Python floats: numpy 2 scalars print as "np.float64(...)"
+            f.write(f"{name} {kind} {float(mass)!r} {float(width)!r} {spin} {pair} {float(c.real)!r} "
+                    f"{float(c.imag)!r}\n")
     return str(path)
 
 
