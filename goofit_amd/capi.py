@@ -30,7 +30,7 @@ EXPORTS = [
     "gf_ctx_get_tuning",
     "gf_events_upload", "gf_events_upload_aos", "gf_events_alloc", "gf_events_column",
     "gf_events_set_identity_column", "gf_program_set", "gf_params_update", "gf_norms_update",
-    "gf_decay3_set", "gf_reduce_metric", "gf_reduce_metric_multi", "gf_eval_metric", "gf_eval_metric_device", "gf_reduce_bins",
+    "gf_decay3_set", "gf_reduce_metric", "gf_reduce_metric_noalloc", "gf_reduce_metric_multi", "gf_eval_metric", "gf_eval_metric_device", "gf_reduce_bins",
     "gf_eval_bins", "gf_grid_cells", "gf_dalitz_grid_cells",
     "gf_dalitz_set_data_size", "gf_dalitz_cache", "gf_dalitz_gram", "gf_dalitz_gram_points",
     "gf_dalitz_gram_points_device", "gf_dalitz_get_cache", "gf_dalitz_cache_gram", "gf_tddp_gram",
@@ -101,6 +101,8 @@ def lib():
         L.gf_dalitz_cache_gram.argtypes = [c_p, ctypes.c_int, _D]
         L.gf_tddp_gram.argtypes = [c_p, ctypes.c_int, _D, _D, _D, _D]
         L.gf_reduce_metric_multi.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D]
+        if hasattr(L, "gf_reduce_metric_noalloc"):
+            L.gf_reduce_metric_noalloc.argtypes = [c_p, ctypes.c_int, c_p, c_p]
         L.gf_time_reduce_metric.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D]
         L.gf_time_reduce_sweep.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D, _D]
         L.gf_time_sweep_back_to_back.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D]
@@ -268,6 +270,13 @@ class Context:
         m = METRIC[metric] if isinstance(metric, str) else metric
         _check(lib().gf_reduce_metric(self.h, m, ctypes.byref(s), ctypes.byref(c)))
         return s.value, c.value
+
+    def reduce_metric_noalloc(self, dev_out: int, metric="NLL", stream: int = 0):
+        """Enqueue the reduction without allocating: the 32-byte result block
+        (sum, residual, error word, sequence number) goes to device address dev_out,
+        on `stream` (a hipStream_t handle; 0: the context's stream).  Returns at once."""
+        m = METRIC[metric] if isinstance(metric, str) else metric
+        _check(lib().gf_reduce_metric_noalloc(self.h, m, ctypes.c_void_p(dev_out), ctypes.c_void_p(stream or None)))
 
     def reduce_metric_multi(self, metric, params_list, norms_list):
         """Metric sums for several (params, norms) sets over the same events."""

@@ -18,7 +18,17 @@
 #include "gf_device.h"
 #include "gf_mlp.h"
 
+#include <mutex>
 #include <random>
+
+// Per-workgroup partials of gf_reduce_metric_noalloc: a fixed device array that
+// every GPU holds once, for the life of the process.  In host code `scratch` only
+// names the symbol; its address on a device comes from hipGetSymbolAddress with
+// that device current (noalloc_partials), and only that address is passed to
+// kernels.
+static __device__ __align__(256) gfk::Partial scratch[GF_NOALLOC_PARTIALS];
+static_assert(GF_NOALLOC_PARTIALS >= GF_MAX_PARTIALS, "the no-allocation scratch holds every reduction grid");
+static_assert(sizeof(gf_result) == sizeof(GfResult), "gf_result is the device result block");
 
 namespace {
 
@@ -142,6 +152,7 @@ struct gf_ctx {
     std::vector<hipEvent_t> ev_sweep;  // gf_time_reduce_sweep
 
     uint32_t res_seq = 0;  // sequence number of the last reduction launched
+    uint32_t noalloc_seq = 0;  // of the last gf_reduce_metric_noalloc
 
     // launch-shape knobs (gf_tune)
     int tune[GF_TUNE_COUNT] = {1, 1, 4, 4, 2000, 1, 1};
@@ -162,6 +173,30 @@ bool valid_ctx(gf_ctx *ctx) {
         return false;
     }
     return hipSetDevice(ctx->device) == hipSuccess;
+}
+
+// Device address of `scratch` on each GPU, looked up once per device (by the
+// first gf_ctx_create on it, with that device current).
+std::mutex g_scratch_mu;
+std::map<int, gfk::Partial *> g_scratch_addr;
+
+gf_status noalloc_partials_init(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if(g_scratch_addr.count(device))
+        return GF_OK;
+    void *ptr    = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if(e != hipSuccess || !ptr)
+        return fail(GF_ERR_HIP, "hipGetSymbolAddress of the no-allocation scratch on device %d failed: %s", device,
+                    hipGetErrorString(e));
+    g_scratch_addr[device] = (gfk::Partial *)ptr;
+    return GF_OK;
+}
+
+gfk::Partial *noalloc_partials(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    return it == g_scratch_addr.end() ? nullptr : it->second;
 }
 
 template <class T>
@@ -1167,6 +1202,9 @@ gf_status gf_ctx_create(gf_ctx **out, int device) {
     if(device < 0 || device >= n)
         return fail(GF_ERR_INVALID, "device %d out of range (%d visible)", device, n);
     GF_HIP(hipSetDevice(device));
+    gf_status sst = noalloc_partials_init(device);
+    if(sst)
+        return sst;
     gf_ctx *c = new gf_ctx();
     c->device = device;
     hipDeviceProp_t prop;
@@ -1489,6 +1527,47 @@ gf_status gf_reduce_metric(gf_ctx *ctx, int metric, double *sum, double *comp) {
     if((st = launch_reduce(ctx, metric)))
         return st;
     return read_result(ctx, sum, comp);
+}
+
+gf_status gf_reduce_metric_noalloc(gf_ctx *ctx, int metric, gf_result *dev_out, void *stream) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    if(!dev_out)
+        return fail(GF_ERR_INVALID, "null result block");
+    gf_status st = check_events_for_program(ctx, metric);
+    if(st)
+        return st;
+    GfInline inl;
+    if(!make_inline(ctx, inl))
+        return fail(GF_ERR_STATE,
+                    "no-allocation reduction: %zu parameters / %zu normalisation values do not fit the kernel "
+                    "arguments (%d / %d), or GF_TUNE_INLINE_ARGS is off",
+                    ctx->params.size(), ctx->norms.size(), GF_PIN, GF_NIN);
+    gfk::Partial *partials = noalloc_partials(ctx->device);
+    if(!partials)
+        return fail(GF_ERR_STATE, "no-allocation scratch of device %d not set up", ctx->device);
+    const bool fast = use_fast_dalitz(ctx, metric);
+    const bool flat = !fast && use_flat(ctx, metric);
+    const int nblocks = fast ? fast_blocks(ctx, ctx->n) : event_blocks(ctx, ctx->n);
+    if(nblocks > GF_NOALLOC_PARTIALS)
+        return fail(GF_ERR_STATE, "no-allocation reduction: %d workgroups exceed the scratch of %d partials", nblocks,
+                    GF_NOALLOC_PARTIALS);
+    GfProgArgs a = prog_args(ctx);
+    a.seq        = ++ctx->noalloc_seq;
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    GfResult *res = reinterpret_cast<GfResult *>(dev_out);
+    int rc;
+    if(fast)
+        rc = gfk::launch_dalitz_fast(a, &inl, const_efficiency(ctx), zero_outside(ctx), metric, partials, res,
+                                     nullptr, nblocks, s);
+    else if(flat)
+        rc = gfk::launch_reduce_flat(a, &inl, metric, ctx->max_col + 1, flat_ept(ctx), partials, res,
+                                     nblocks, s);
+    else
+        rc = gfk::launch_reduce_metric(a, &inl, metric, partials, res, nblocks, s);
+    if(rc)
+        return fail(GF_ERR_HIP, "no-allocation reduce launch failed (%d)", rc);
+    return GF_OK;
 }
 
 gf_status gf_reduce_metric_multi(gf_ctx *ctx, int metric, int npts, const double *params, int np,

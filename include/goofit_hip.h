@@ -187,6 +187,41 @@ gf_status gf_decay3_set(gf_ctx *ctx, const double decay[6]);
  * NULL) the residual correction term that was folded in. */
 gf_status gf_reduce_metric(gf_ctx *ctx, int metric, double *sum, double *comp);
 
+/* Result block of gf_reduce_metric_noalloc, written to device memory by the
+ * reduction's last kernel: the compensated total, the residual correction that
+ * was folded in, the device error word of the sweep (bit 0: an Amp3Body event
+ * number outside its wave cache; bit 1: a node kind the kernel was not built
+ * with; 0 = no error) and the call's sequence number on its context. */
+typedef struct gf_result {
+    double sum;
+    double residual;
+    int32_t error;
+    uint32_t seq;
+    int32_t pad[2];
+} gf_result;
+
+/* Largest number of per-workgroup partials the no-allocation reduction holds
+ * (its static scratch is GF_NOALLOC_PARTIALS x 16 bytes per GPU). */
+#define GF_NOALLOC_PARTIALS 4096
+
+/* gf_reduce_metric without any allocation, for callers that cannot allocate
+ * device or pinned memory while they run: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  No counterpart in the
+ * reference.  The same kernels as gf_reduce_metric (identical sum, bit for bit)
+ * are enqueued on `stream` (a hipStream_t; NULL: the context's stream) without
+ * waiting; the result block goes to the caller's device memory *dev_out.  The
+ * per-workgroup partials live in a fixed device array of the library
+ * (GF_NOALLOC_PARTIALS pairs = 64 KiB on each GPU, set up by the first
+ * gf_ctx_create on that GPU and held until the process ends), not in the
+ * context.  Refused with GF_ERR_STATE: parameter or normalisation arrays larger
+ * than the kernel arguments hold (256 and 128 values) or GF_TUNE_INLINE_ARGS
+ * off, and a grid beyond the scratch.  Work the context queued on its own
+ * stream (uploads, wave caches) must be complete, or `stream` must be that
+ * stream.  All calls on one GPU share the scratch array: calls whose launches
+ * overlap on one GPU (two streams, two contexts on the same device) race and
+ * must be ordered by the caller. */
+gf_status gf_reduce_metric_noalloc(gf_ctx *ctx, int metric, gf_result *dev_out, void *stream);
+
 /* The metric sums for npts parameter sets over the same events: params holds
  * npts consecutive copies of the flattened parameter array (np values each, np
  * as in gf_params_update), norms npts copies of the normalisation array;
