@@ -563,6 +563,7 @@ PYBIND11_MODULE(_goofit, m) {
         .def("useGraph", &BatchFitter::useGraph)
         .def("useMultiPoint", &BatchFitter::useMultiPoint)
         .def("multiPoint", &BatchFitter::multiPoint)
+        .def("useDeviceNormalisation", &BatchFitter::useDeviceNormalisation)
         .def("__len__", &BatchFitter::size)
         .def("values", &BatchFitter::values)
         .def("errors", &BatchFitter::errors)
@@ -579,6 +580,7 @@ PYBIND11_MODULE(_goofit, m) {
             d["host_prep_s"] = s.host_prep_s;
             d["gather_s"]   = s.gather_s;
             d["chunk"]      = s.chunk;
+            d["device_norms"] = s.device_norms;
             return d;
         });
 

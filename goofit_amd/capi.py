@@ -42,10 +42,13 @@ EXPORTS = [
     "gf_dalitz_generate", "gf_batch_create", "gf_batch_destroy", "gf_batch_set_toy", "gf_batch_reduce",
     "gf_batch_set_chunk", "gf_batch_get_chunk", "gf_batch_use_graph", "gf_batch_autotune",
     "gf_batch_add_point", "gf_batch_reduce_points", "gf_batch_clear_points", "gf_batch_submit_points",
-    "gf_batch_wait_points",
+    "gf_batch_wait_points", "gf_batch_set_norm_ops",
 ]
 
 _D = ctypes.POINTER(ctypes.c_double)
+# gf_norm_op of include/goofit_hip.h
+NORM_OP_DTYPE = np.dtype([("kind", "<i4"), ("out", "<i4"), ("p", "<i4"), ("n", "<i4"), ("g", "<i4"), ("pad", "<i4"),
+                          ("c0", "<f8"), ("c1", "<f8")])
 _I = ctypes.POINTER(ctypes.c_int32)
 _lib = None
 
@@ -140,6 +143,8 @@ def lib():
         L.gf_batch_clear_points.argtypes = [c_p]
         L.gf_batch_submit_points.argtypes = [c_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]
         L.gf_batch_wait_points.argtypes = [c_p, ctypes.c_uint32, _D]
+        if hasattr(L, "gf_batch_set_norm_ops"):
+            L.gf_batch_set_norm_ops.argtypes = [c_p, c_p, ctypes.c_int, _D, _D, ctypes.c_int]
         L.gf_dalitz_generate.argtypes = [c_p, ctypes.c_int, ctypes.c_int64, ctypes.c_uint64, _D, ctypes.c_double,
                                          ctypes.c_int, c_p, c_p, ctypes.POINTER(ctypes.c_int64), _D]
         L.gf_time_dalitz_cache.argtypes = [c_p, ctypes.c_int, ctypes.c_int, _D]
@@ -523,9 +528,23 @@ class Batch:
         _check(lib().gf_batch_reduce(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), m, _dp(out)))
         return out
 
-    def add_point(self, t, params, norms):
-        p, n = _f64(params), _f64(norms)
-        _check(lib().gf_batch_add_point(self.h, t, _dp(p), _dp(n)))
+    def add_point(self, t, params, norms=None):
+        """norms None: computed in the launch by the installed normalisation ops."""
+        p = _f64(params)
+        n = _f64(norms) if norms is not None else None
+        _check(lib().gf_batch_add_point(self.h, t, _dp(p), _dp(n) if n is not None else None))
+
+    def set_norm_ops(self, ops, template, table=()):
+        """ops: list of dicts (kind "scale" | "exp" | "dalitz", out, p, n, g, c0, c1); [] removes them."""
+        kinds = {"scale": 1, "exp": 2, "dalitz": 3}
+        rec = np.zeros(len(ops), dtype=NORM_OP_DTYPE)
+        for i, op in enumerate(ops):
+            rec[i] = (kinds[op["kind"]], op["out"], op["p"], op.get("n", 0), op.get("g", 0), 0, op.get("c0", 0.0),
+                      op.get("c1", 0.0))
+        tm = _f64(template)
+        tb = _f64(table) if len(table) else np.zeros(1)
+        _check(lib().gf_batch_set_norm_ops(self.h, rec.ctypes.data_as(ctypes.c_void_p) if len(ops) else None,
+                                           len(ops), _dp(tm), _dp(tb), len(table)))
 
     def reduce_points(self, metric="NLL"):
         """Evaluate every added point: [ntoys, BATCH_MAX_POINTS] array, NaN where no point was added."""

@@ -2961,7 +2961,13 @@ struct gf_batch {
         uint32_t ticket = 0;  // stamp value of its launch
     } pts[2];
     int pts_cur = 0;  // the buffer gf_batch_add_point fills
+    // normalisation of the points on the device (gf_batch_set_norm_ops)
+    int norm_nops = 0;
+    double *d_norm_tmpl = nullptr;    // nn
+    GfNormOp *d_norm_ops = nullptr;   // norm_nops
+    double *d_norm_table = nullptr;
 };
+static_assert(sizeof(gf_norm_op) == sizeof(GfNormOp), "gf_norm_op is GfNormOp");
 static_assert(GF_BATCH_PTS == GF_BATCH_MAX_POINTS, "point limits of the C ABI and the kernels");
 
 namespace {
@@ -3146,8 +3152,9 @@ void gf_batch_destroy(gf_batch *b) {
         (void)hipStreamSynchronize(b->ctx->stream);
     }
     batch_drop_graph(b);
-    void *dev[] = {b->d_stage,    b->d_first, b->d_work, b->d_partials, b->d_counter, b->pts[0].d, b->pts[0].d_partials,
-                   b->pts[1].d, b->pts[1].d_partials};
+    void *dev[] = {b->d_stage,    b->d_first,         b->d_work,     b->d_partials,
+                   b->d_counter,  b->pts[0].d,        b->pts[1].d,   b->pts[0].d_partials,
+                   b->pts[1].d_partials, b->d_norm_tmpl, b->d_norm_ops, b->d_norm_table};
     for(void *p : dev)
         if(p)
             (void)hipFree(p);
@@ -3238,7 +3245,7 @@ void batch_points_clear(gf_batch *b, Points &q) {
 gf_status gf_batch_add_point(gf_batch *b, int toy, const double *params, const double *norms) {
     if(!b || toy < 0 || toy >= b->ntoys)
         return fail(GF_ERR_INVALID, "toy index out of range");
-    if(!params || !norms)
+    if(!params || (!norms && b->norm_nops == 0))
         return fail(GF_ERR_INVALID, "null parameter or normalisation array");
     Points &q = b->pts[b->pts_cur];
     if(q.in_flight)
@@ -3256,7 +3263,8 @@ gf_status gf_batch_add_point(gf_batch *b, int toy, const double *params, const d
     const size_t W = (size_t)(b->np + b->nn);
     double *slot   = q.h + b->ntoys + (size_t)q.slots * W;
     memcpy(slot, params, (size_t)b->np * sizeof(double));
-    memcpy(slot + b->np, norms, (size_t)b->nn * sizeof(double));
+    if(b->norm_nops == 0)  // otherwise the launch computes them
+        memcpy(slot + b->np, norms, (size_t)b->nn * sizeof(double));
     tbl[2 * toy + 1]++;
     q.slots++;
     q.last = toy;
@@ -3320,6 +3328,10 @@ gf_status gf_batch_submit_points(gf_batch *b, int metric, uint32_t *ticket) {
         zo = std::isfinite(q.h[b->ntoys + (size_t)k * W + ctx->ins[1].p + ctx->ins[1].a]);
     GF_HIP(hipMemcpyAsync(q.d, q.h, ((size_t)b->ntoys + (size_t)q.slots * W) * sizeof(double), hipMemcpyHostToDevice,
                           ctx->stream));
+    if(b->norm_nops > 0
+       && gfk::launch_batch_norms(q.d + b->ntoys, q.slots, b->np, b->nn, b->d_norm_tmpl, b->d_norm_ops, b->norm_nops,
+                                  b->d_norm_table, ctx->stream))
+        return fail(GF_ERR_HIP, "normalisation launch failed");
     const int groups = (q.max + GF_MULTI_PTS - 1) / GF_MULTI_PTS;
     GfProgArgs a     = prog_args(ctx);
     const int nres   = fast ? ctx->dalitz[0].nres : 0;
@@ -3369,6 +3381,49 @@ gf_status gf_batch_reduce_points(gf_batch *b, int metric, double *out) {
     if(st)
         return st;
     return gf_batch_wait_points(b, ticket, out);
+}
+
+gf_status gf_batch_set_norm_ops(gf_batch *b, const gf_norm_op *ops, int nops, const double *norms_template,
+                                const double *table, int nt) {
+    if(!b || nops < 0 || (nops > 0 && (!ops || !norms_template)) || nt < 0 || (nt > 0 && !table))
+        return fail(GF_ERR_INVALID, "bad normalisation ops");
+    if(!valid_ctx(b->ctx))
+        return GF_ERR_INVALID;
+    for(const Points &q : b->pts)
+        if(q.in_flight || q.slots > 0)
+            return fail(GF_ERR_STATE, "normalisation ops change while points are staged or in flight");
+    for(int o = 0; o < nops; o++) {
+        const gf_norm_op &op = ops[o];
+        bool ok = op.out >= 0 && op.out < b->nn && op.p >= 0 && op.p < b->np;
+        if(op.kind == GF_NORM_DALITZ)
+            ok = ok && op.n >= 1 && op.n <= GF_MAX_RES && op.p + 2 * op.n <= b->np && op.g >= 0
+                 && (int64_t)op.g + 2 * (int64_t)op.n * op.n <= nt;
+        else if(op.kind != GF_NORM_SCALE && op.kind != GF_NORM_EXP)
+            ok = false;
+        if(!ok)
+            return fail(GF_ERR_INVALID, "normalisation op %d (kind %d, out %d, p %d, n %d, g %d) outside the arrays", o,
+                        op.kind, op.out, op.p, op.n, op.g);
+    }
+    GF_HIP(hipStreamSynchronize(b->ctx->stream));
+    void *old[] = {b->d_norm_tmpl, b->d_norm_ops, b->d_norm_table};
+    for(void *p : old)
+        if(p)
+            GF_HIP(hipFree(p));
+    b->d_norm_tmpl  = nullptr;
+    b->d_norm_ops   = nullptr;
+    b->d_norm_table = nullptr;
+    b->norm_nops    = 0;
+    if(nops == 0)
+        return GF_OK;
+    GF_HIP(hipMalloc((void **)&b->d_norm_tmpl, (size_t)b->nn * sizeof(double)));
+    GF_HIP(hipMalloc((void **)&b->d_norm_ops, (size_t)nops * sizeof(GfNormOp)));
+    GF_HIP(hipMalloc((void **)&b->d_norm_table, (size_t)std::max(nt, 1) * sizeof(double)));
+    GF_HIP(hipMemcpy(b->d_norm_tmpl, norms_template, (size_t)b->nn * sizeof(double), hipMemcpyHostToDevice));
+    GF_HIP(hipMemcpy(b->d_norm_ops, ops, (size_t)nops * sizeof(GfNormOp), hipMemcpyHostToDevice));
+    if(nt > 0)
+        GF_HIP(hipMemcpy(b->d_norm_table, table, (size_t)nt * sizeof(double), hipMemcpyHostToDevice));
+    b->norm_nops = nops;
+    return GF_OK;
 }
 
 gf_status gf_batch_set_chunk(gf_batch *b, int chunk) {

@@ -150,6 +150,13 @@ struct GfBatchWork {
     double total;
 };
 
+// A normalisation computed on the device for a batched point (gf_norm_op of the C ABI).
+enum GfNormKind : int32_t { GN_SCALE = 1, GN_EXP = 2, GN_DALITZ = 3 };
+struct GfNormOp {
+    int32_t kind, out, p, n, g, pad;
+    double c0, c1;
+};
+
 // Host-side launchers (gf_kernels.hip).
 namespace gfk {
 struct Partial {
@@ -200,6 +207,10 @@ int launch_batch(const GfProgArgs &a, const GfBatchWork *work, int nblocks, cons
 // result[2 (t GF_BATCH_PTS + p) ..] (host-mapped), then the stamp.
 // fast: the coherent-sum program of launch_dalitz_fast.
 #define GF_BATCH_PTS 64
+// Normalisation arrays of nslots staged points (slot k: np parameters then nn norms at
+// stage + k (np + nn)): the template, then each op's 1 / integral from the slot's parameters.
+int launch_batch_norms(double *stage, int nslots, int np, int nn, const double *tmpl, const GfNormOp *ops, int nops,
+                       const double *table, void *stream);
 int launch_batch_points(const GfProgArgs &a, int nres, const GfBatchWork *work, int nblocks, int groups,
                         const double *stage, int np, int nn, const int *tbl, bool fast, bool const_eff,
                         bool zero_outside, int metric, Partial *partials, const int *first, int ntoys, double *result,

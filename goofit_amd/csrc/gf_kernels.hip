@@ -1258,6 +1258,58 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_dalitz_points(GfProgArgs A, 
     }
 }
 
+// The normalisation arrays of staged batch points, one lane per point: the template,
+// then norms[op.out] = 1 / I for each op with I from the point's parameters in the
+// host's operation order (GooPdf::normalize: ret = 1 * integral, norm = 1 / ret;
+// GaussianPdf.cu:28-35, ExpPdf.cu:121-132, Amp3Body.cu:334-358).
+__global__ __launch_bounds__(256) void k_batch_norms(double *__restrict__ stage, int nslots, int np, int nn,
+                                                     const double *__restrict__ tmpl,
+                                                     const GfNormOp *__restrict__ ops, int nops,
+                                                     const double *__restrict__ table) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if(k >= nslots)
+        return;
+    const double *__restrict__ P = stage + (int64_t)k * (np + nn);
+    double *__restrict__ Nm      = stage + (int64_t)k * (np + nn) + np;
+    for(int i = 0; i < nn; ++i)
+        Nm[i] = tmpl[i];
+    for(int o = 0; o < nops; ++o) {
+        const GfNormOp op = ops[o];
+        double integral;
+        if(op.kind == GN_SCALE) {
+            integral = P[op.p];
+            integral *= op.c0;
+        } else if(op.kind == GN_EXP) {
+            const double a = P[op.p];
+            if(0 == a) {
+                integral = op.c1 - op.c0;
+            } else {
+                integral = exp(a * op.c1) - exp(a * op.c0);
+                integral /= a;
+            }
+        } else {
+            // sum_ij c_i conj(c_j) T_ij, each complex product written out
+            const double *__restrict__ cp = P + op.p;
+            const double *__restrict__ T  = table + op.g;
+            double sum_re = 0;
+            for(int i = 0; i < op.n; ++i) {
+                const double ar = cp[2 * i], ai = cp[2 * i + 1];
+                for(int j = 0; j < op.n; ++j) {
+                    const double br = cp[2 * j], bi = -cp[2 * j + 1];
+                    const double pr = ar * br - ai * bi, pi = ar * bi + ai * br;
+                    const double ir = T[2 * (i * op.n + j)], ii = T[2 * (i * op.n + j) + 1];
+                    sum_re += pr * ir - pi * ii;
+                }
+            }
+            integral = sum_re;
+            integral *= op.c0;
+        }
+        double ret = 1;
+        ret *= integral;
+        Nm[op.out] = 1.0 / ret;
+    }
+}
+
 // One workgroup per toy: for each of its points, the partials of the toy's work
 // entries [first[t], first[t + 1]) in the order of k_finish_toys into
 // result[2 (t GF_BATCH_PTS + p) ..] (host-mapped).
@@ -1807,6 +1859,15 @@ int launch_batch(const GfProgArgs &a, const GfBatchWork *work, int nblocks, cons
     if((rc = check(hipGetLastError())))
         return rc;
     hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, s, counter, stamp);
+    return check(hipGetLastError());
+}
+
+int launch_batch_norms(double *stage, int nslots, int np, int nn, const double *tmpl, const GfNormOp *ops, int nops,
+                       const double *table, void *stream) {
+    if(nslots <= 0)
+        return 0;
+    hipLaunchKernelGGL(k_batch_norms, dim3((nslots + 255) / 256), dim3(256), 0, (hipStream_t)stream, stage, nslots, np,
+                       nn, tmpl, ops, nops, table);
     return check(hipGetLastError());
 }
 

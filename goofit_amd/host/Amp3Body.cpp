@@ -183,6 +183,34 @@ fptype Amp3Body::normalize() {
     return ret;
 }
 
+// normalize()'s last step as a device op: 1 / (Re sum_ij c_i conj(c_j) I_ij * binSizeFactor)
+// over the integrals of the last normalize(); the subtree below keeps normalisation 1.
+bool Amp3Body::batchNormOps(std::vector<gf_norm_op> &ops, std::vector<double> &table) const {
+    if(fitControl && !fitControl->metricIsPdf())
+        return true;
+    const size_t nres = decayInfo.resonances.size();
+    if(nres == 0 || integrals.size() != nres * nres)
+        return false;
+    double binSizeFactor = 1;
+    if(normPoints == 0) {
+        binSizeFactor *= _m12.getBinSize();
+        binSizeFactor *= _m13.getBinSize();
+    }
+    gf_norm_op op{};
+    op.kind = GF_NORM_DALITZ;
+    op.out  = (int32_t)normalIdx + 1;
+    op.p    = (int32_t)parametersIdx + 1;
+    op.n    = (int32_t)nres;
+    op.g    = (int32_t)table.size();
+    op.c0   = binSizeFactor;
+    for(const auto &v : integrals) {
+        table.push_back(v.real());
+        table.push_back(v.imag());
+    }
+    ops.push_back(op);
+    return true;
+}
+
 void Amp3Body::setNormalisationPoints(int64_t n, unsigned long long seed, DalitzProposal *proposal) {
     if(n < 1)
         throw GeneralError("Amp3Body::setNormalisationPoints: need at least one point");

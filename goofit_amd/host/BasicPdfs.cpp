@@ -23,11 +23,30 @@ GaussianPdf::GaussianPdf(std::string n, Observable _x, Variable mean, Variable s
 
 // Over all of R, whatever the observable range (GaussianPdf::integrate,
 // src/PDFs/basic/GaussianPdf.cu:28-35).
-fptype GaussianPdf::integrate(fptype, fptype) const {
+namespace {
+fptype gaussian_norm_factor() {
     static const fptype rootPi = std::sqrt(std::atan2(0.0, -1.0));
-    fptype sigma               = host_parameters.at(parametersIdx + 2);
-    sigma *= root2 * rootPi;
+    return root2 * rootPi;
+}
+}  // namespace
+
+fptype GaussianPdf::integrate(fptype, fptype) const {
+    fptype sigma = host_parameters.at(parametersIdx + 2);
+    sigma *= gaussian_norm_factor();
     return sigma;
+}
+
+// normalize(): 1 / (sigma * sqrt 2 sqrt pi), the same product on the device
+bool GaussianPdf::batchNormOps(std::vector<gf_norm_op> &ops, std::vector<double> &) const {
+    if(fitControl && !fitControl->metricIsPdf())
+        return true;
+    gf_norm_op op{};
+    op.kind = GF_NORM_SCALE;
+    op.out  = (int32_t)normalIdx + 1;
+    op.p    = (int32_t)parametersIdx + 2;
+    op.c0   = gaussian_norm_factor();
+    ops.push_back(op);
+    return true;
 }
 
 // ---------------------------------------------------------------- Exp
@@ -67,6 +86,22 @@ ExpPdf::ExpPdf(std::string n, Observable _x, std::vector<Variable> &weights, Var
 bool ExpPdf::hasAnalyticIntegral() const { return 1 == host_parameters.at(parametersIdx); }
 
 // (e^(alpha hi) - e^(alpha lo)) / alpha, or hi - lo at alpha = 0 (ExpPdf.cu:121-132)
+// normalize() of the one-parameter form: 1 / ((exp(alpha hi) - exp(alpha lo)) / alpha)
+bool ExpPdf::batchNormOps(std::vector<gf_norm_op> &ops, std::vector<double> &table) const {
+    if(fitControl && !fitControl->metricIsPdf())
+        return true;
+    if(!hasAnalyticIntegral())
+        return GooPdf::batchNormOps(ops, table);
+    gf_norm_op op{};
+    op.kind = GF_NORM_EXP;
+    op.out  = (int32_t)normalIdx + 1;
+    op.p    = (int32_t)parametersIdx + 1;
+    op.c0   = observablesList.at(0).getLowerLimit();
+    op.c1   = observablesList.at(0).getUpperLimit();
+    ops.push_back(op);
+    return true;
+}
+
 fptype ExpPdf::integrate(fptype lo, fptype hi) const {
     fptype alpha = host_parameters.at(parametersIdx + 1);
     if(0 == alpha)

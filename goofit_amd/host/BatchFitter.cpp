@@ -12,6 +12,7 @@
 #include <Minuit2/MnMigrad.h>
 
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <memory>
 #include <mutex>
@@ -144,6 +145,30 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
                           "gf_batch_set_toy");
         engine::check(gf_batch_autotune(batch_, GF_METRIC_NLL, &stats_.chunk, nullptr), "gf_batch_autotune");
     }
+    // the points' normalisations inside the launches, when the model has device ops for
+    // every normalisation the floating parameters move (the template: this normalize())
+    std::vector<gf_norm_op> nops;
+    std::vector<double> ntable;
+    const bool dev_norm = multi_ && device_norms_ && pdf_->batchNormOps(nops, ntable);
+    engine::check(gf_batch_set_norm_ops(batch_, dev_norm ? nops.data() : nullptr, dev_norm ? (int)nops.size() : 0,
+                                        host_normalizations.data(), ntable.data(), (int)ntable.size()),
+                  "gf_batch_set_norm_ops");
+    stats_.device_norms = dev_norm;
+    // with device norms a point is its flat parameter array: each Variable's value
+    // (Params::from_minuit_vector: Minuit value minus blinding) at each of its slots
+    struct ParamSlot {
+        int fitter;
+        unsigned int slot;
+        fptype blind;
+    };
+    std::vector<ParamSlot> pmap;
+    for(auto &sl : slots)
+        if(sl.first.getFitterIndex() >= 0)
+            pmap.push_back({sl.first.getFitterIndex(), sl.second, sl.first.getBlind(Variable::Key())});
+    std::vector<fptype> pbuf(host_parameters.begin(), host_parameters.end());
+    const std::vector<unsigned int> yields = pdf_->extendedYieldSlots();
+    // the template norms serve points whose normalisation no floating parameter moves
+    const fptype *point_norms = nops.empty() ? host_normalizations.data() : nullptr;
     Batcher B;
     B.cv_toy.reset(new std::condition_variable[T]);
     B.x.assign(T, nullptr);
@@ -212,6 +237,21 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
     int64_t points    = 0;
     auto prepare = [&](int t) {
         extra[t].assign(B.xn[t], 0.0);
+        if(dev_norm) {
+            for(size_t k = 0; k < B.xn[t]; k++) {
+                const std::vector<double> &x = (*B.x[t])[B.x0[t] + k];
+                for(const ParamSlot &m : pmap)
+                    pbuf[m.slot] = x.at(m.fitter) - m.blind;
+                if(!yields.empty()) {  // AddPdf::nllExtraTermFor with the toy's event count
+                    fptype expEvents = 0;
+                    for(unsigned int y : yields)
+                        expEvents += pbuf[y];
+                    extra[t][k] = expEvents - (fptype)count_[t] * std::log(expEvents);
+                }
+                engine::check(gf_batch_add_point(batch_, t, pbuf.data(), point_norms), "gf_batch_add_point");
+            }
+            return;
+        }
         for(size_t k = 0; k < B.xn[t]; k++) {
             // push the point into the shared Variables, then the host arrays as
             // Params::from_minuit_vector: flag the values that move, so that an

@@ -72,6 +72,28 @@ fptype AddPdf::normalize() {
     return ret;
 }
 
+// The sum keeps normalisation 1 and its components normalise separately, unless they
+// share the common norm 1 / sum (which moves with every weight: no device op).
+bool AddPdf::batchNormOps(std::vector<gf_norm_op> &ops, std::vector<double> &table) const {
+    if(getCommonNorm())
+        return GooPdf::batchNormOps(ops, table);
+    for(PdfBase *c : components) {
+        const auto *g = dynamic_cast<const GooPdf *>(c);
+        if(!g || !g->batchNormOps(ops, table))
+            return false;
+    }
+    return true;
+}
+
+// The weights of an extended sum are the yields (nllExtraTermFor adds them in this order).
+std::vector<unsigned int> AddPdf::extendedYieldSlots() const {
+    std::vector<unsigned int> out;
+    if(extended)
+        for(size_t i = 0; i < components.size(); ++i)
+            out.push_back(parametersIdx + 1 + (unsigned int)i);
+    return out;
+}
+
 // AddPdf::calculateNLL, AddPdf.cu:187-204: extended likelihood adds E - N log E.
 double AddPdf::calculateNLL() {
     double ret = GooPdf::calculateNLL();
@@ -120,6 +142,19 @@ ProdPdf::ProdPdf(std::string n, std::vector<PdfBase *> comps)
                 varOverlaps = false;
     registerFunction("ptr_to_ProdPdfs", GF_FN_PROD);
     initialize();
+}
+
+// Separately normalised components: their ops (the product keeps normalisation 1);
+// overlapping observables normalise the product numerically.
+bool ProdPdf::batchNormOps(std::vector<gf_norm_op> &ops, std::vector<double> &table) const {
+    if(varOverlaps)
+        return GooPdf::batchNormOps(ops, table);
+    for(PdfBase *c : components) {
+        const auto *g = dynamic_cast<const GooPdf *>(c);
+        if(!g || !g->batchNormOps(ops, table))
+            return false;
+    }
+    return true;
 }
 
 // ProdPdf::normalize, ProdPdf.cu:75-99

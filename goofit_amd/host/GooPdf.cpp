@@ -97,6 +97,17 @@ fptype GooPdf::normalize() {
     return ret;
 }
 
+// A normalisation no floating parameter moves stays at its value in the template
+// array; any other numerical normalisation has no device op.
+bool GooPdf::batchNormOps(std::vector<gf_norm_op> &, std::vector<double> &) const {
+    if(fitControl && !fitControl->metricIsPdf())
+        return true;
+    for(const Variable &v : getParameters())
+        if(!v.IsFixed())
+            return false;
+    return true;
+}
+
 fptype GooPdf::integrate(fptype, fptype) const { return 0; }
 bool GooPdf::hasAnalyticIntegral() const { return false; }
 

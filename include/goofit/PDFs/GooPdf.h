@@ -9,6 +9,7 @@
 #include <goofit/FitControl.h>
 #include <goofit/PdfBase.h>
 #include <goofit/UnbinnedDataSet.h>
+#include <goofit_hip.h>
 
 namespace GooFit {
 
@@ -43,6 +44,16 @@ class GooPdf : public PdfBase {
     virtual double nllExtraTerm() const { return 0.0; }
     /// The same term for a data set of nevents events (a toy's share of the rows).
     virtual double nllExtraTermFor(fptype) const { return 0.0; }
+    /// The normalisations of this subtree that floating parameters move, as device ops
+    /// for batched points (gf_batch_set_norm_ops): appends them (integrals they read go
+    /// to table) and returns true, or returns false when one of them has no such form (a
+    /// numerical integral over floating parameters).  Call after normalize(): the ops
+    /// reproduce the normalisations normalize() computes from the current arrays.  No
+    /// counterpart in the reference, which normalises on the host at every call.
+    virtual bool batchNormOps(std::vector<gf_norm_op> &ops, std::vector<double> &table) const;
+    /// Parameter-array slots whose values sum to the expected event count of the
+    /// extended-likelihood term of nllExtraTermFor (empty: no such term).
+    virtual std::vector<unsigned int> extendedYieldSlots() const { return {}; }
     /// The normalisation of this node is positive (calculateNLL aborts otherwise).
     bool normalisationPositive() const;
     std::vector<fptype> evaluateAtPoints(Observable var);

@@ -14,6 +14,7 @@ class ExpPdf : public GooPdf {
     ExpPdf(std::string n, Observable _x, std::vector<Variable> &weights, Variable offset);
     fptype integrate(fptype lo, fptype hi) const override;
     bool hasAnalyticIntegral() const override;
+    bool batchNormOps(std::vector<gf_norm_op> &ops, std::vector<double> &table) const override;
 };
 
 }  // namespace GooFit

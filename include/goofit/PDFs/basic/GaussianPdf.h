@@ -11,6 +11,7 @@ class GaussianPdf : public GooPdf {
     GaussianPdf(std::string n, Observable _x, Variable m, Variable s);
     fptype integrate(fptype lo, fptype hi) const override;
     bool hasAnalyticIntegral() const override { return true; }
+    bool batchNormOps(std::vector<gf_norm_op> &ops, std::vector<double> &table) const override;
 };
 
 }  // namespace GooFit

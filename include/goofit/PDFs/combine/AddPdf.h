@@ -14,6 +14,8 @@ class AddPdf : public CombinePdf {
     double nllExtraTerm() const override;
     double nllExtraTermFor(fptype nevents) const override;
     bool hasAnalyticIntegral() const override { return false; }
+    bool batchNormOps(std::vector<gf_norm_op> &ops, std::vector<double> &table) const override;
+    std::vector<unsigned int> extendedYieldSlots() const override;
 
   private:
     bool extended;

@@ -10,6 +10,7 @@ class ProdPdf : public CombinePdf {
     ProdPdf(std::string n, std::vector<PdfBase *> comps);
     fptype normalize() override;
     bool hasAnalyticIntegral() const override { return false; }
+    bool batchNormOps(std::vector<gf_norm_op> &ops, std::vector<double> &table) const override;
 
   private:
     bool varOverlaps;

@@ -24,6 +24,9 @@ class Amp3Body : public Amp3BodyBase {
              GooPdf *eff = nullptr);
 
     fptype normalize() override;
+    /// The coupling quadratic form over the integrals of the last normalize() as a device
+    /// op (the resonance masses and widths must stay fixed while the op is in use).
+    bool batchNormOps(std::vector<gf_norm_op> &ops, std::vector<double> &table) const override;
     /// Allocate the wave cache for dataSize events starting at row offset
     /// (call after setData).
     void setDataSize(unsigned int dataSize, unsigned int evtSize = 3, unsigned int offset = 0);

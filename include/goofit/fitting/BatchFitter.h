@@ -10,10 +10,12 @@
 // preparing the requests costs more than the GPU work, a round goes out in two
 // stages so that one is evaluated while the other is prepared.  With
 // useMultiPoint(false) every point is its own request, served by a captured
-// hipGraph with one parameter set per toy.  The normalisations
-// are computed on the host per toy (analytic integrals, or an Amp3Body's cached
-// integral matrix), so the model's non-analytic parts must not depend on the
-// floating parameters: fit() refuses a floating resonance mass or width.
+// hipGraph with one parameter set per toy.  The normalisations of the points are
+// computed inside the batched launch when the model allows it
+// (GooPdf::batchNormOps: analytic integrals, an Amp3Body's quadratic form over its
+// cached integral matrix), otherwise on the host per point; either way the
+// model's non-analytic parts must not depend on the floating parameters: fit()
+// refuses a floating resonance mass or width.
 #pragma once
 
 #include <cstdint>
@@ -32,9 +34,10 @@ struct BatchFitStats {
     int64_t fcn_calls  = 0;  // FCN points served
     double seconds     = 0;  // wall time of fit()
     double gpu_wait_s  = 0;  // time spent in the batched reductions
-    double host_prep_s = 0;  // time the batcher spends setting each served toy's parameters and norms
+    double host_prep_s = 0;  // time the batcher spends staging each served toy's parameters (and norms)
     double gather_s    = 0;  // time the batcher waits for the toys' next FCN requests
     int chunk          = 0;  // events per workgroup chosen by the autotuner
+    bool device_norms  = false;  // the points' normalisations were computed in the launches
 };
 
 class BatchFitter {
@@ -65,6 +68,9 @@ class BatchFitter {
     /// Several points per toy and launch (default) or one.
     void useMultiPoint(bool on) { multi_ = on; }
     bool multiPoint() const { return multi_; }
+    /// With several points per launch: normalise the points on the device when the
+    /// model allows it (default), or always on the host.
+    void useDeviceNormalisation(bool on) { device_norms_ = on; }
 
   private:
     GooPdf *pdf_;
@@ -75,6 +81,7 @@ class BatchFitter {
     std::vector<double> fval_;
     BatchFitStats stats_;
     bool multi_{true};
+    bool device_norms_{true};
 };
 
 }  // namespace GooFit

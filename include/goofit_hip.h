@@ -461,6 +461,35 @@ gf_status gf_batch_clear_points(gf_batch *b);
 gf_status gf_batch_submit_points(gf_batch *b, int metric, uint32_t *ticket);
 gf_status gf_batch_wait_points(gf_batch *b, uint32_t ticket, double *out);
 
+/* Normalisation of the points on the device.  The reference normalises on the
+ * host at every FCN call: GooPdf::normalize (src/PDFs/GooPdf.cu:340-396) with the
+ * analytic integrals of GaussianPdf (src/PDFs/basic/GaussianPdf.cu:28-35) and
+ * ExpPdf (src/PDFs/basic/ExpPdf.cu:121-132), and Amp3Body::normalize's quadratic
+ * form of the couplings over the cached integrals (src/PDFs/physics/
+ * Amp3Body.cu:334-358).  With ops installed, a point added with norms == NULL gets
+ * its normalisation array inside the launch of gf_batch_submit_points: the
+ * template array installed here, then norms[op.out] = 1 / I for each op, with I
+ * from that point's own parameters P:
+ *   GF_NORM_SCALE   I = P[p] * c0                       (GaussianPdf: c0 = sqrt 2 sqrt pi)
+ *   GF_NORM_EXP     I = (exp(a c1) - exp(a c0)) / a, a = P[p]; c1 - c0 if a == 0  (ExpPdf on [c0, c1])
+ *   GF_NORM_DALITZ  I = c0 Re sum_ij c_i conj(c_j) T_ij, c_i = P[p + 2i] + i P[p + 2i + 1],
+ *                   T_ij = table[g + 2 (i n + j)] + i table[g + 2 (i n + j) + 1], i, j < n
+ * each in the host's operation order.  nops = 0 removes the ops (points then need
+ * their norms again).  The table (nt doubles) and the template (nn values) are
+ * copied. */
+enum gf_norm_kind { GF_NORM_SCALE = 1, GF_NORM_EXP = 2, GF_NORM_DALITZ = 3 };
+typedef struct gf_norm_op {
+    int32_t kind;  /* gf_norm_kind */
+    int32_t out;   /* index in the normalisation array */
+    int32_t p;     /* index in the parameter array */
+    int32_t n;     /* GF_NORM_DALITZ: resonance count (<= 16) */
+    int32_t g;     /* GF_NORM_DALITZ: first entry in the table */
+    int32_t pad;
+    double c0, c1;
+} gf_norm_op;
+gf_status gf_batch_set_norm_ops(gf_batch *b, const gf_norm_op *ops, int nops, const double *norms_template,
+                                const double *table, int nt);
+
 /* ---- timing ------------------------------------------------------------- */
 
 /* Run gf_reduce_metric `reps` times back to back on the context stream and

@@ -279,6 +279,7 @@ def test_batch_fitter_multi_point_equals_single_point():
     for multi in (True, False):
         bf = G.BatchFitter(m.pdf, first, counts)
         bf.useMultiPoint(multi)
+        bf.useDeviceNormalisation(False)  # host norms on both sides: the same function values
         assert bf.multiPoint() == multi
         bf.fit(threads=3, autotune=False)
         out[multi] = (bf.values(), bf.nfcn(), bf.valid(), bf.stats())
@@ -359,3 +360,105 @@ def test_batch_fitter_refuses_a_floating_resonance_mass():
         mass.fixed = True
     bf.fit(threads=2)
     assert min(bf.nfcn()) > 0
+
+
+def _fit_both_ways(pdf, first, counts, threads=4):
+    """The same batched fits with the points normalised in the launches and on the host."""
+    out = {}
+    for dev in (True, False):
+        bf = G.BatchFitter(pdf, first, counts)
+        bf.useDeviceNormalisation(dev)
+        bf.fit(threads=threads, autotune=False)
+        st = bf.stats()
+        assert st["device_norms"] == dev
+        out[dev] = (np.array(bf.values()), np.array(bf.errors()), list(bf.nfcn()), list(bf.fval()), list(bf.valid()),
+                    st)
+    return out
+
+
+def test_device_normalisation_addpdf_matches_host():
+    """Gaussian and Exp integrals of the AddPdf computed per point inside the launch: the
+    Gaussian's is the host's product (same bits), the Exp's differs at most in the last bits
+    of exp: fits within 1e-5 sigma of the host-normalised ones."""
+    counts = [4000, 9000, 6000, 15_000, 7000, 5000]
+    data = _toys(counts, 95)
+    m = W.build_add1d(start=1.1)
+    d = G.UnbinnedDataSet(m.observables)
+    d.from_columns([np.concatenate(data)])
+    m.pdf.setData(d)
+    first = list(np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(int))
+    out = _fit_both_ways(m.pdf, first, counts, threads=3)
+    vd, ed, _, fd, okd, _ = out[True]
+    vh, eh, _, fh, okh, _ = out[False]
+    assert all(okd) and all(okh)
+    free = eh[0] > 0
+    shift = np.abs(vd - vh)[:, free] / eh[:, free]
+    assert shift.max() < 1e-5, shift.max()
+    np.testing.assert_allclose(fd, fh, rtol=1e-12)
+
+
+def test_device_normalisation_dalitz_is_the_host_quadratic_form():
+    """An Amp3Body's coupling quadratic form over its cached integrals computed per point
+    inside the launch, in the host's operation order: the same function values, hence the
+    same fits and call counts as with the host's normalize() per point."""
+    counts = [5000, 8000, 6000, 7000]
+    n = sum(counts)
+    dz = W.build_dalitz(nbins=200, resonances=W.DALITZ_RESONANCES[:6])
+    m12, m13 = W.generate_dalitz(dz, n, seed=61)
+    d = G.UnbinnedDataSet(dz.observables)
+    d.from_columns([np.asarray(m12), np.asarray(m13), np.arange(n, dtype=np.float64)])
+    dz.pdf.setData(d)
+    dz.pdf.setDataSize(n)
+    truth = {k: v.value for k, v in dz.params.items()}
+    rng = np.random.default_rng(62)
+    for k in dz.params:
+        dz.params[k].value = truth[k] * (1 + 0.05 * rng.standard_normal())
+    first = list(np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(int))
+    out = _fit_both_ways(dz.pdf, first, counts, threads=4)
+    assert np.array_equal(out[True][0], out[False][0])
+    assert out[True][2] == out[False][2]
+    assert out[True][3] == out[False][3]
+    assert all(out[True][4])
+
+
+def test_extended_addpdf_toys_match_sequential_fits():
+    """Extended AddPdf (signal and background yields): each toy's batched fit adds its own
+    E - N log E (AddPdf.cu:187-204) with its own event count, so the yields come out within
+    0.01 sigma of one FitManager fit per toy."""
+    counts = [3000, 7000, 5000, 9000]
+    data = _toys(counts, 120)
+    x = G.Observable("x", -5.0, 5.0)
+    mu = G.Variable("mu", 0.32, 0.01, -2.0, 2.0)
+    sg = G.Variable("sigma", 0.85, 0.01, 0.1, 3.0)
+    al = G.Variable("alpha", -0.38, 0.01, -2.0, -0.01)
+    ns = G.Variable("nsig", 1500.0, 10.0, 0.0, 50000.0)
+    nb = G.Variable("nbkg", 4000.0, 10.0, 0.0, 50000.0)
+    g = G.GaussianPdf("g", x, mu, sg)
+    e = G.ExpPdf("e", x, al)
+    tot = G.AddPdf("ext", [ns, nb], [g, e])
+    params = dict(mu=mu, sigma=sg, alpha=al, nsig=ns, nbkg=nb)
+    start = {k: v.value for k, v in params.items()}
+    d = G.UnbinnedDataSet([x])
+    d.from_columns([np.concatenate(data)])
+    tot.setData(d)
+    first = list(np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(int))
+    bf = G.BatchFitter(tot, first, counts)
+    bf.fit(threads=4, autotune=False)
+    assert bf.stats()["device_norms"]
+    vals, valid = np.array(bf.values()), bf.valid()
+    for t, xs in enumerate(data):
+        for k, v in start.items():
+            params[k].value = v
+        dt = G.UnbinnedDataSet([x])
+        dt.from_columns([xs])
+        tot.setData(dt)
+        fm = G.FitManager(tot)
+        fm.setVerbosity(0)
+        r = fm.fit()
+        assert valid[t] and r.IsValid()
+        sv, se = np.array(r.values()), np.array(r.errors())
+        free = se > 0
+        shift = np.abs(vals[t] - sv)[free] / se[free]
+        assert shift.max() < 0.01, (t, shift)
+        # the yields add up to the toy's size (extended maximum likelihood)
+        assert abs(params["nsig"].value + params["nbkg"].value - counts[t]) < 1e-3 * counts[t]
