@@ -96,36 +96,46 @@ def grid_integral(model, pdf, nb):
     return val
 
 
-JUDGE_GRIDS = (2000, 2828, 4000, 5657, 8000)  # cells per axis; each step halves the cell area
+JUDGE_NODES = (1000, 2000, 4000)  # nodes per axis of the judge's product Gauss-Legendre rule
+MIDPOINT_GRIDS = (1414, 2000, 2828, 4000, 5657, 8000)  # cells per axis of the midpoint grids beside it
 
 
 def norm_error_study(model, pdf, train_events, sync, t_train_gen, target=1e-4, seeds=6):
     """Normalisation-integral error at equal Monte-Carlo points, uniform phase-space MC vs the
-    learned proposal, judged against a converged deterministic integral: the midpoint grid
-    refined until successive grids agree far below the errors measured (the finest of
-    JUDGE_GRIDS); the 1e8-point uniform FP64 Monte-Carlo integral is reported beside it.  Then
-    a MIGRAD fit with each method at the point count that reaches `target` relative error,
-    and one on the midpoint grid.  The learned method is charged its training-event
-    generation and training.  Leaves pdf on its midpoint grid."""
+    learned proposal, judged against a converged deterministic integral: a product Gauss-
+    Legendre rule over the Dalitz region in (m12, t) coordinates whose edges are the
+    kinematic boundary (workloads.square_dalitz_rule), refined until successive rules agree
+    far below the errors measured.  Reported beside it: the midpoint grids of the reference
+    method (their boundary staircase leaves errors of order 1e-5 that do not fall smoothly
+    with the cell size) and the 1e8-point uniform FP64 Monte-Carlo integral.  Then a MIGRAD
+    fit with each method at the point count that reaches `target` relative error, and one on
+    the midpoint grid.  The learned method is charged its training-event generation and
+    training.  Leaves pdf on its midpoint grid."""
     import goofit_amd as G
     from goofit_amd import workloads as W
     out = {}
     truth = {n: v.value for n, v in model.params.items()}
     grid_norm = pdf.normalize()
-    # judge: midpoint grids of growing resolution (the boundary cells make the error O(cell size))
+    # judge: the boundary-fitted product rule at growing node counts
     ladder = []
-    for nb in JUDGE_GRIDS:
+    for nodes in JUDGE_NODES:
         t0 = time.perf_counter()
-        val = grid_integral(model, pdf, nb)
+        a, b, w = W.square_dalitz_rule(nodes)
+        val = pdf.integralAtPoints(a, b, w)
         sync()
-        ladder.append(dict(cells_per_axis=nb, value=val, seconds=round(time.perf_counter() - t0, 3)))
-    for a, b in zip(ladder[:-1], ladder[1:]):
-        b["vs_previous"] = b["value"] / a["value"] - 1
+        ladder.append(dict(nodes_per_axis=nodes, points=int(a.size), value=val,
+                           seconds=round(time.perf_counter() - t0, 3)))
+        del a, b, w
+    for x, y in zip(ladder[:-1], ladder[1:]):
+        y["vs_previous"] = y["value"] / x["value"] - 1
     ref_norm = ladder[-1]["value"]
-    # first-order convergence (cell size falls by sqrt 2 per step): the finest grid's error is
-    # at most the last refinement / (sqrt 2 - 1)
-    out["reference"] = dict(kind="midpoint grid", cells_per_axis=JUDGE_GRIDS[-1], value=ref_norm,
-                            convergence=ladder, est_rel_err=abs(ladder[-1]["vs_previous"]) / (math.sqrt(2) - 1),
+    grids = []
+    for nb in MIDPOINT_GRIDS:
+        grids.append(dict(cells_per_axis=nb, vs_reference=grid_integral(model, pdf, nb) / ref_norm - 1))
+    sync()
+    out["reference"] = dict(kind="product Gauss-Legendre rule over (m12, t), boundary-fitted",
+                            nodes_per_axis=JUDGE_NODES[-1], value=ref_norm, convergence=ladder,
+                            est_rel_err=abs(ladder[-1]["vs_previous"]), midpoint_grids=grids,
                             fit_grid_vs_reference=grid_norm / ref_norm - 1)
     # the 1e8-point uniform FP64 Monte-Carlo integral, against the judge
     t0 = time.perf_counter()
@@ -134,10 +144,11 @@ def norm_error_study(model, pdf, train_events, sync, t_train_gen, target=1e-4, s
     sync()
     out["reference_1e8"] = dict(points=100_000_000, value=val, seconds=round(time.perf_counter() - t0, 3),
                                 rel_stat_err=math.sqrt(pdf.normalisationRelativeVariance(16) / 1e8),
-                                vs_grid_reference=val / ref_norm - 1)
-    log(f"reference integral: {JUDGE_GRIDS[-1]}^2 midpoint grid, last refinement {ladder[-1]['vs_previous']:.1e}; "
-        f"1e8 uniform points / grid - 1 = {out['reference_1e8']['vs_grid_reference']:.2e} "
-        f"(stat. {out['reference_1e8']['rel_stat_err']:.1e}); fit grid / reference - 1 = {grid_norm / ref_norm - 1:.2e}")
+                                vs_judge=val / ref_norm - 1)
+    log(f"reference integral: {JUDGE_NODES[-1]}^2-node boundary-fitted rule, last refinement "
+        f"{ladder[-1]['vs_previous']:.1e}; midpoint grids vs it: "
+        + ", ".join(f"{g['cells_per_axis']}: {g['vs_reference']:.1e}" for g in grids)
+        + f"; 1e8 uniform points: {out['reference_1e8']['vs_judge']:.2e} (stat. {out['reference_1e8']['rel_stat_err']:.1e})")
 
     def errors(n, proposal, base):
         e, v = [], []
@@ -227,6 +238,7 @@ def norm_error_study(model, pdf, train_events, sync, t_train_gen, target=1e-4, s
     pdf.copyParams()
     pdf.setNormalisationGrid()
     out["fits"] = fits
+    out["proposal_retrains"] = q.retrainCount()
     return out
 
 
@@ -282,8 +294,10 @@ def toy_counts(n, seed0):
     return np.array(out, np.int64)
 
 
-def fit_toys(model, data, counts, truth, sync, seq_toys, threads):
-    """Batched MIGRAD fits of every toy, sequential fits of the first seq_toys; pulls."""
+def fit_toys(model, data, counts, truth, sync, seq_toys, threads, seq_seed=4321):
+    """Batched MIGRAD fits of every toy, sequential fits of a random subset of seq_toys toys
+    (for an Amp3Body the data-independent integrals are computed once and reused, as the
+    batch does: only the per-event lineshapes are recomputed per toy); pulls."""
     import goofit_amd as G
     pdf = model.pdf
     names = sorted(model.params)
@@ -311,10 +325,12 @@ def fit_toys(model, data, counts, truth, sync, seq_toys, threads):
     big = counts[ok] >= np.median(counts)
     pw_big = pulls[big].std(0, ddof=1)
     n_tail = int(np.sum(np.any(np.abs(pulls) > 5, axis=1)))
-    # sequential fits of the first toys, one data set at a time
+    # sequential fits of a random subset, one data set at a time
     seq_vals, t_seq = [], 0.0
     cols = [data.column(o) for o in model.observables]
-    for t in range(seq_toys):
+    subset = np.sort(np.random.default_rng(seq_seed).choice(len(counts), size=min(seq_toys, len(counts)),
+                                                            replace=False))
+    for t in subset:
         for k in names:
             model.params[k].value = truth[k]
         d = G.UnbinnedDataSet(model.observables)
@@ -328,13 +344,15 @@ def fit_toys(model, data, counts, truth, sync, seq_toys, threads):
         pdf.setData(d)
         if hasattr(pdf, "setDataSize"):
             pdf.setDataSize(int(counts[t]))
+            if t != subset[0]:
+                pdf.refreshCacheOnly()
         fm = G.FitManager(pdf)
         fm.setVerbosity(0)
         r = fm.fit()
         sync()
         t_seq += time.perf_counter() - t0
         seq_vals.append((np.array(r.values()), np.array(r.errors())))
-    shifts = [np.max(np.abs(vals[t][free] - sv[free]) / se[free]) for t, (sv, se) in enumerate(seq_vals)]
+    shifts = [np.max(np.abs(vals[t][free] - sv[free]) / se[free]) for t, (sv, se) in zip(subset, seq_vals)]
     for k in names:
         model.params[k].value = truth[k]
     pdf.setData(data)
@@ -343,9 +361,13 @@ def fit_toys(model, data, counts, truth, sync, seq_toys, threads):
     n = len(counts)
     return dict(toys=n, events=int(counts.sum()), free_params=int(free.sum()), valid=int(ok.sum()),
                 batched_s=round(t_batch, 3), batched_toys_per_hour=round(n / t_batch * 3600, 1),
-                sequential_toys=seq_toys, sequential_s=round(t_seq, 3),
-                sequential_toys_per_hour=round(seq_toys / t_seq * 3600, 1) if t_seq > 0 else None,
-                speedup=round((n / t_batch) / (seq_toys / t_seq), 3) if t_seq > 0 else None,
+                sequential_toys=len(subset), sequential_s=round(t_seq, 3),
+                sequential_toys_per_hour=round(len(subset) / t_seq * 3600, 1) if t_seq > 0 else None,
+                sequential_events=int(counts[subset].sum()),
+                speedup=round((n / t_batch) / (len(subset) / t_seq), 3) if t_seq > 0 else None,
+                speedup_per_event=round((counts.sum() / t_batch) / (counts[subset].sum() / t_seq), 3)
+                if t_seq > 0 else None,
+                two_stages=bool(st.get("two_stages", False)), device_norms=bool(st.get("device_norms", False)),
                 launches=st["launches"], fcn_calls=st["fcn_calls"], gpu_wait_s=round(st["gpu_wait_s"], 3),
                 host_prep_s=round(st["host_prep_s"], 3), gather_s=round(st["gather_s"], 3),
                 chunk=st["chunk"], multi_point=bool(bf.multiPoint()), pull_mean=[round(float(v), 4) for v in pm],
@@ -372,17 +394,19 @@ def fit_toys(model, data, counts, truth, sync, seq_toys, threads):
 TOY_GRID = 1600
 
 
-def toy_study(ntoys, sync, seq_toys=16, threads=256, add1d_threads=512):
-    """cfg: ntoys/2 toys of the 1-D AddPdf model and ntoys/2 of an 8-resonance cut of the Dalitz model,
-    Poisson-varied sizes, fitted in batches (512 / 256 concurrent Minuit instances: the AddPdf
-    fits are GPU-bound and gain from fuller launches, the Dalitz fits host-bound) and (a few)
-    sequentially."""
+def toy_study(ntoys, sync, seq_toys=128, threads=256, add1d_threads=512, n_add1d=None, n_dalitz=None):
+    """cfg: ntoys/2 toys of the 1-D AddPdf model and ntoys/2 of an 8-resonance cut of the Dalitz model
+    (or n_add1d and n_dalitz), Poisson-varied sizes, fitted in batches (512 / 256 concurrent Minuit
+    instances: the AddPdf fits are GPU-bound and gain from fuller launches) and a random subset of
+    seq_toys of each model sequentially."""
     import goofit_amd as G
     from goofit_amd import workloads as W
     half = ntoys // 2
+    n_add1d = half if n_add1d is None else n_add1d
+    n_dalitz = half if n_dalitz is None else n_dalitz
     out = {}
     # 1-D AddPdf toys
-    counts = toy_counts(half, 10_000)
+    counts = toy_counts(n_add1d, 10_000)
     t0 = time.perf_counter()
     xs = np.concatenate([W.sample_add1d(int(n), seed=50_000 + t) for t, n in enumerate(counts)])
     t_gen = time.perf_counter() - t0
@@ -399,7 +423,7 @@ def toy_study(ntoys, sync, seq_toys=16, threads=256, add1d_threads=512):
                                                     "max_abs_pull_width_minus_1")})
     del d
     # 8-resonance Dalitz toys
-    counts = toy_counts(half, 20_000)
+    counts = toy_counts(n_dalitz, 20_000)
     md = W.build_dalitz(nbins=TOY_GRID, resonances=W.DALITZ_RESONANCES[:8])
     truth = {k: v.value for k, v in md.params.items()}
     t0 = time.perf_counter()
@@ -418,9 +442,12 @@ def toy_study(ntoys, sync, seq_toys=16, threads=256, add1d_threads=512):
                                                         "sequential_toys_per_hour", "max_abs_pull_mean",
                                                         "max_abs_pull_width_minus_1")})
     tot_t = out["add1d"]["batched_s"] + out["dalitz8"]["batched_s"]
-    seq_rate = 2.0 / (1 / out["add1d"]["sequential_toys_per_hour"] + 1 / out["dalitz8"]["sequential_toys_per_hour"])
-    out["all"] = dict(toys=2 * half, batched_s=round(tot_t, 3), batched_toys_per_hour=round(2 * half / tot_t * 3600, 1),
-                      sequential_toys_per_hour=round(seq_rate, 1))
+    # sequential time of the same mix of toys: each model's toys at that model's sequential rate
+    seq_t = sum(out[k]["toys"] / out[k]["sequential_toys_per_hour"] for k in ("add1d", "dalitz8")) * 3600
+    n_all = n_add1d + n_dalitz
+    out["all"] = dict(toys=n_all, batched_s=round(tot_t, 3), batched_toys_per_hour=round(n_all / tot_t * 3600, 1),
+                      sequential_toys_per_hour=round(n_all / seq_t * 3600, 1),
+                      speedup=round(seq_t / tot_t, 3))
     return out
 
 
@@ -543,6 +570,8 @@ def main():
     ap.add_argument("--pull-study", type=int, default=0, metavar="NTOYS",
                     help="also fit NTOYS 8-resonance Dalitz toys of 2e5 events each for their pulls")
     ap.add_argument("--pull-grid", type=int, default=TOY_GRID, help="normalisation grid of the pull study's fits")
+    ap.add_argument("--pulls", nargs=2, type=int, default=None, metavar=("N_ADD1D", "N_DALITZ8"),
+                    help="pull study at resolution: the toy study's two models with these toy counts (e.g. 16384 8192)")
     ap.add_argument("--profile-only", nargs="?", const="dalitz16", default=None,
                     choices=("dalitz16", "prod3d", "add1d"),
                     help="set up one workload (default dalitz16: the 16-resonance Dalitz model, whose set-up "
@@ -728,6 +757,11 @@ def main():
         log(f"toy study: {args.toys} toys")
         toys = toy_study(args.toys, sync)
 
+    if args.pulls:
+        log(f"pull study: {args.pulls[0]} AddPdf toys and {args.pulls[1]} 8-resonance Dalitz toys")
+        pl = toy_study(0, sync, n_add1d=args.pulls[0], n_dalitz=args.pulls[1])
+        toys = dict(toys or {}, pulls=pl)
+
     if args.pull_study > 0:
         log(f"pull study: {args.pull_study} toys of 2e5 events, {args.pull_grid}^2 normalisation grid")
         ps = pull_study(args.pull_study, 200_000, sync, nbins=args.pull_grid)
@@ -753,7 +787,9 @@ def main():
                                    hbm_gbs=round(bpe * nevt / (k_ms * 1e-3) / 1e9, 1))
             # the whole MIGRAD fit of the workload's free parameters from +20 % starting values
             fmdl = builder(start=1.2)
-            fmdl.pdf.setData(d)
+            fd = G.UnbinnedDataSet(fmdl.observables)
+            fd.from_columns(cols)
+            fmdl.pdf.setData(fd)
             fm = G.FitManager(fmdl.pdf)
             fm.setVerbosity(0)
             sync()
@@ -766,7 +802,7 @@ def main():
                 values={k: v.value for k, v in fmdl.params.items()},
                 errors={k: v.error for k, v in fmdl.params.items()})
             log(name, workloads[name])
-            del d
+            del d, fd
         workloads["tddp16_1e6_threegauss"] = tddp_workload(args, sync)
         log("tddp16_1e6_threegauss", workloads["tddp16_1e6_threegauss"])
 

@@ -397,11 +397,23 @@ PYBIND11_MODULE(_goofit, m) {
         .def("getCachedWave", &Amp3Body::getCachedWave)
         .def("sumCachedWave", &Amp3Body::sumCachedWave)
         .def("fit_fractions", &Amp3Body::fit_fractions)
+        .def(
+            "integralAtPoints",
+            [](Amp3Body &p, py::array_t<double, py::array::c_style | py::array::forcecast> m12,
+               py::array_t<double, py::array::c_style | py::array::forcecast> m13,
+               py::array_t<double, py::array::c_style | py::array::forcecast> w) {
+                std::vector<double> a(m12.data(), m12.data() + m12.size());
+                std::vector<double> b(m13.data(), m13.data() + m13.size());
+                std::vector<double> c(w.data(), w.data() + w.size());
+                return p.integralAtPoints(a, b, c);
+            },
+            py::arg("m12"), py::arg("m13"), py::arg("w"))
         .def("getIntegrals", &Amp3Body::getIntegrals)
         .def("getEfficiencyIndex", &Amp3Body::getEfficiencyIndex)
         .def("setNormalisationPoints", &Amp3Body::setNormalisationPoints, py::arg("n"), py::arg("seed"),
              py::arg("proposal") = nullptr, py::keep_alive<1, 4>())
         .def("setNormalisationGrid", &Amp3Body::setNormalisationGrid)
+        .def("refreshCacheOnly", &Amp3Body::refreshCacheOnly)
         .def("getNormalisationPointCount", &Amp3Body::getNormalisationPointCount)
         .def("normalisationRelativeVariance", &Amp3Body::normalisationRelativeVariance, py::arg("batches") = 16)
         .def("generateEvents", &Amp3Body::generateEvents, py::arg("out"), py::arg("n"), py::arg("seed"),
@@ -564,6 +576,7 @@ PYBIND11_MODULE(_goofit, m) {
         .def("useMultiPoint", &BatchFitter::useMultiPoint)
         .def("multiPoint", &BatchFitter::multiPoint)
         .def("useDeviceNormalisation", &BatchFitter::useDeviceNormalisation)
+        .def("setStages", &BatchFitter::setStages)
         .def("__len__", &BatchFitter::size)
         .def("values", &BatchFitter::values)
         .def("errors", &BatchFitter::errors)
@@ -581,6 +594,7 @@ PYBIND11_MODULE(_goofit, m) {
             d["gather_s"]   = s.gather_s;
             d["chunk"]      = s.chunk;
             d["device_norms"] = s.device_norms;
+            d["two_stages"]   = s.two_stages;
             return d;
         });
 

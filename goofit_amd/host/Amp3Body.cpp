@@ -129,18 +129,25 @@ fptype Amp3Body::normalize() {
     recursiveSetNormalization(1.0);
     const size_t nres = decayInfo.resonances.size();
     uint64_t mask     = 0;
+    bool moved        = false;  // a lineshape parameter changed since the last integrals
     for(size_t i = 0; i < nres; ++i) {
-        redoIntegral[i] = forceRedoIntegrals || decayInfo.resonances[i]->parametersChanged();
+        const bool changed = decayInfo.resonances[i]->parametersChanged();
+        moved              = moved || changed;
+        redoIntegral[i]    = forceRedoIntegrals || changed;
         if(redoIntegral[i])
             mask |= (uint64_t)1 << i;
     }
     forceRedoIntegrals = false;
+    const bool keep_integrals = cacheOnly && !moved && haveIntegrals;
+    cacheOnly                 = false;
     if(mask) {
         engine::flush();
         if(cacheReady) {
             engine::check(gf_dalitz_cache(engine::ctx(), (int)functionIdx, mask), "gf_dalitz_cache");
             engine::bump_cache_epoch();
         }
+    }
+    if(mask && !keep_integrals) {
         std::vector<double> out(2 * nres * nres);
         if(normPoints > 0) {
             engine::check(gf_dalitz_gram_norm_points(engine::ctx(), (int)functionIdx, (int)efficiencyFunction, 0,
@@ -154,6 +161,7 @@ fptype Amp3Body::normalize() {
         }
         for(size_t k = 0; k < nres * nres; k++)
             integrals[k] = {out[2 * k], out[2 * k + 1]};
+        haveIntegrals = true;
     }
     // sum_ij c_i conj(c_j) I_ij in the reference's order, each complex product
     // written out as (a + ib)(c + id) = (ac - bd) + i(ad + bc), which is what the
@@ -224,6 +232,7 @@ void Amp3Body::setNormalisationPoints(int64_t n, unsigned long long seed, Dalitz
                       "gf_dalitz_set_norm_points");
         normPoints         = n;
         forceRedoIntegrals = true;
+        cacheOnly          = false;
     };
     draw();
     if(!proposal || !proposal->retrainingEnabled())
@@ -252,6 +261,7 @@ void Amp3Body::setNormalisationPoints(int64_t n, unsigned long long seed, Dalitz
 void Amp3Body::setNormalisationGrid() {
     normPoints         = 0;
     forceRedoIntegrals = true;
+    cacheOnly          = false;
 }
 
 fptype Amp3Body::normalisationRelativeVariance(int batches) {
@@ -320,6 +330,30 @@ int64_t Amp3Body::generateEvents(UnbinnedDataSet &out, size_t n, unsigned long l
         throw GeneralError("Amp3Body::generateEvents: the data set needs m12, m13 and the event number");
     out.appendColumns(cols, n);
     return tried;
+}
+
+fptype Amp3Body::integralAtPoints(const std::vector<fptype> &m12, const std::vector<fptype> &m13,
+                                  const std::vector<fptype> &w) {
+    if(m12.size() != m13.size() || m12.size() != w.size() || m12.empty())
+        throw GeneralError("Amp3Body::integralAtPoints: need equal, non-empty m12, m13 and weight arrays");
+    copyParams();
+    engine::flush();
+    const size_t nres = decayInfo.resonances.size();
+    std::vector<double> out(2 * nres * nres);
+    engine::check(gf_dalitz_gram_points(engine::ctx(), (int)functionIdx, m12.data(), m13.data(), w.data(),
+                                        (int64_t)m12.size(), (int)efficiencyFunction, out.data()),
+                  "gf_dalitz_gram_points");
+    const fptype *cp = host_parameters.data() + parametersIdx + 1;
+    fptype sum_re    = 0;
+    for(size_t i = 0; i < nres; ++i) {
+        const fptype ar = cp[2 * i], ai = cp[2 * i + 1];
+        for(size_t j = 0; j < nres; ++j) {
+            const fptype br = cp[2 * j], bi = -cp[2 * j + 1];
+            const fptype pr = ar * br - ai * bi, pi = ar * bi + ai * br;
+            sum_re += pr * out[2 * (i * nres + j)] - pi * out[2 * (i * nres + j) + 1];
+        }
+    }
+    return sum_re;
 }
 
 const std::vector<std::complex<fptype>> Amp3Body::getCachedWave(size_t i) const {

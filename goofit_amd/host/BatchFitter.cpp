@@ -359,9 +359,12 @@ void BatchFitter::fit(int threads, unsigned int strategy, double tolerance, bool
             // which outweighs the second launch's fixed cost (its grid still spans every
             // toy's rows).  Decided once, from the first twelve rounds, run unsplit (the
             // first ones, single points, would overstate the preparation)
-            if(multi_ && !split_decided && stats_.launches >= 12) {
-                split         = stats_.host_prep_s > stats_.gpu_wait_s;
+            if(multi_ && !split_decided && (stages_ != 0 || stats_.launches >= 12)) {
+                // a second stage costs one more launch per round (tens of microseconds) and
+                // hides the preparation and the Minuit work of half the toys behind the GPU
+                split         = stages_ == 2 || (stages_ == 0 && stats_.host_prep_s > 0.1 * stats_.gpu_wait_s);
                 split_decided = true;
+                stats_.two_stages = split;
             }
             const size_t stage = split ? (size_t)std::max(1, B.running / 2) : (size_t)T + 1;
             const size_t ready = serve.size() + taken.size();

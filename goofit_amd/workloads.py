@@ -162,6 +162,42 @@ def dalitz_area(decay=DALITZ_DECAY, n=4000):
     return float(0.5 * (hi - lo) * np.sum(w * width))
 
 
+def square_dalitz_rule(nodes, decay=DALITZ_DECAY, order=8):
+    """Points and weights of a product quadrature over the Dalitz region in (m12, m13):
+    m12 = lo + (hi - lo) (1 - cos(pi u)) / 2 and, at each m12, m13 = centre + half-width * t,
+    with u in [0, 1] and t in [-1, 1] each a composite Gauss-Legendre rule of nodes // order
+    panels of `order` nodes; weight = wu * wt * dm12/du * half-width.  The kinematic boundary
+    is an edge of the (u, t) rectangle, so unlike the midpoint grid no cell straddles it, and
+    the cosine map absorbs the square-root behaviour of the half-width at both ends of m12;
+    all nodes lie strictly inside the plot."""
+    M, d1, d2, d3 = decay[:4]
+    panels = max(1, nodes // order)
+    x, w = np.polynomial.legendre.leggauss(order)
+
+    def composite(lo, hi):
+        edges = np.linspace(lo, hi, panels + 1)
+        half = 0.5 * np.diff(edges)
+        mid = 0.5 * (edges[1:] + edges[:-1])
+        return (mid[:, None] + half[:, None] * x[None, :]).ravel(), (half[:, None] * w[None, :]).ravel()
+
+    lo, hi = (d1 + d2) ** 2, (M - d3) ** 2
+    u, wu = composite(0.0, 1.0)
+    m12 = lo + (hi - lo) * 0.5 * (1.0 - np.cos(np.pi * u))
+    w12 = wu * (hi - lo) * 0.5 * np.pi * np.sin(np.pi * u)
+    t, wt = composite(-1.0, 1.0)
+    s = np.sqrt(m12)
+    e1 = 0.5 * (m12 - d2 * d2 + d1 * d1) / s
+    e3 = 0.5 * (M * M - m12 - d3 * d3) / s
+    r1 = np.sqrt(np.maximum(e1 * e1 - d1 * d1, 0))
+    r3 = np.sqrt(np.maximum(e3 * e3 - d3 * d3, 0))
+    centre = (e1 + e3) ** 2 - r1 * r1 - r3 * r3
+    half = 2.0 * r1 * r3
+    P12 = np.repeat(m12, t.size)
+    P13 = (centre[:, None] + half[:, None] * t[None, :]).ravel()
+    W = (w12[:, None] * half[:, None] * wt[None, :]).ravel()
+    return P12, P13, W
+
+
 # --------------------------------------------------------- models (C++ API)
 
 

@@ -31,6 +31,14 @@ class Amp3Body : public Amp3BodyBase {
     /// (call after setData).
     void setDataSize(unsigned int dataSize, unsigned int evtSize = 3, unsigned int offset = 0);
     void setForceIntegrals(bool f = true) { forceRedoIntegrals = f; }
+    /// The next normalize() recomputes the wave cache of every resonance from the
+    /// current events but keeps the integrals of the last one, unless a resonance
+    /// parameter moved: for fitting one model to many data sets in turn (call after
+    /// setDataSize).  No counterpart in the reference, which recomputes both.
+    void refreshCacheOnly() {
+        forceRedoIntegrals = true;
+        cacheOnly          = true;
+    }
 
     void populateArrays() override;
 
@@ -44,6 +52,13 @@ class Amp3Body : public Amp3BodyBase {
 
     /// Fit fractions from the data-event cache (cache must be filled).
     std::vector<std::vector<fptype>> fit_fractions();
+
+    /// Integral of |sum_i c_i W_i|^2 eff over the plot by the quadrature (m12[k], m13[k],
+    /// w[k]) at the current couplings: the integrals I_ij as one Gram over the points,
+    /// then the quadratic form of normalize() (no bin-area factor).  For judging a
+    /// normalisation method against a converged rule; no counterpart in the reference.
+    fptype integralAtPoints(const std::vector<fptype> &m12, const std::vector<fptype> &m13,
+                            const std::vector<fptype> &w);
 
     /// Normalisation integrals I_ij of the last normalize() (n x n).
     const std::vector<std::complex<fptype>> &getIntegrals() const { return integrals; }
@@ -86,6 +101,8 @@ class Amp3Body : public Amp3BodyBase {
     unsigned int efficiencyFunction{0};
     bool cacheReady{false};
     int64_t normPoints{0};  // 0: midpoint grid
+    bool cacheOnly{false};     // refreshCacheOnly: the next normalize() keeps the integrals
+    bool haveIntegrals{false};  // the integrals have been computed at least once
 };
 
 }  // namespace GooFit

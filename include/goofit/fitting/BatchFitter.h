@@ -38,6 +38,7 @@ struct BatchFitStats {
     double gather_s    = 0;  // time the batcher waits for the toys' next FCN requests
     int chunk          = 0;  // events per workgroup chosen by the autotuner
     bool device_norms  = false;  // the points' normalisations were computed in the launches
+    bool two_stages    = false;  // rounds went out in two pipelined stages
 };
 
 class BatchFitter {
@@ -71,6 +72,9 @@ class BatchFitter {
     /// With several points per launch: normalise the points on the device when the
     /// model allows it (default), or always on the host.
     void useDeviceNormalisation(bool on) { device_norms_ = on; }
+    /// Rounds of several points per toy in one stage (1), two pipelined stages (2), or
+    /// decided from the first rounds' preparation and GPU times (0, default).
+    void setStages(int n) { stages_ = (n == 1 || n == 2) ? n : 0; }
 
   private:
     GooPdf *pdf_;
@@ -82,6 +86,7 @@ class BatchFitter {
     BatchFitStats stats_;
     bool multi_{true};
     bool device_norms_{true};
+    int stages_{0};
 };
 
 }  // namespace GooFit
