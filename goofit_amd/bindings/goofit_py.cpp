@@ -414,6 +414,16 @@ PYBIND11_MODULE(_goofit, m) {
              py::arg("proposal") = nullptr, py::keep_alive<1, 4>())
         .def("setNormalisationGrid", &Amp3Body::setNormalisationGrid)
         .def("refreshCacheOnly", &Amp3Body::refreshCacheOnly)
+        .def(
+            "setNormalisationRule",
+            [](Amp3Body &p, py::array_t<double, py::array::c_style | py::array::forcecast> m12,
+               py::array_t<double, py::array::c_style | py::array::forcecast> m13,
+               py::array_t<double, py::array::c_style | py::array::forcecast> w) {
+                p.setNormalisationRule(std::vector<double>(m12.data(), m12.data() + m12.size()),
+                                       std::vector<double>(m13.data(), m13.data() + m13.size()),
+                                       std::vector<double>(w.data(), w.data() + w.size()));
+            },
+            py::arg("m12"), py::arg("m13"), py::arg("w"))
         .def("getNormalisationPointCount", &Amp3Body::getNormalisationPointCount)
         .def("normalisationRelativeVariance", &Amp3Body::normalisationRelativeVariance, py::arg("batches") = 16)
         .def("generateEvents", &Amp3Body::generateEvents, py::arg("out"), py::arg("n"), py::arg("seed"),
@@ -483,6 +493,9 @@ PYBIND11_MODULE(_goofit, m) {
              py::arg("seed") = 1, py::arg("batch_cap") = 65536)
         .def("train", &DalitzProposal::train, py::arg("events"), py::arg("steps") = 2000, py::arg("batch") = 8192,
              py::arg("lr") = 3e-3f, py::arg("seed") = 1)
+        .def("trainOnModel", &DalitzProposal::trainOnModel, py::arg("model"), py::arg("nevents"),
+             py::arg("gen_seed"), py::arg("steps") = 2000, py::arg("batch") = 8192, py::arg("lr") = 3e-3f,
+             py::arg("seed") = 1)
         .def("build", &DalitzProposal::build, py::arg("grid") = 512, py::arg("alpha") = 0.1)
         .def("isBuilt", &DalitzProposal::isBuilt)
         .def("evaluate",

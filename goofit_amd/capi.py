@@ -42,7 +42,7 @@ EXPORTS = [
     "gf_dalitz_generate", "gf_batch_create", "gf_batch_destroy", "gf_batch_set_toy", "gf_batch_reduce",
     "gf_batch_set_chunk", "gf_batch_get_chunk", "gf_batch_use_graph", "gf_batch_autotune",
     "gf_batch_add_point", "gf_batch_reduce_points", "gf_batch_clear_points", "gf_batch_submit_points",
-    "gf_batch_wait_points", "gf_batch_set_norm_ops",
+    "gf_batch_wait_points", "gf_batch_set_norm_ops", "gf_dalitz_set_norm_rule", "gf_mlp_train_generated",
 ]
 
 _D = ctypes.POINTER(ctypes.c_double)

@@ -1237,11 +1237,14 @@ gf_status gf_ctx_create(gf_ctx **out, int device) {
     TRY(hipMalloc((void **)&c->d_gram_tiles, 768 * sizeof(double)));
     TRY(hipHostMalloc((void **)&c->h_result, 64, hipHostMallocDefault));
     TRY(hipHostMalloc((void **)&c->h_gram_tiles, 768 * sizeof(double), hipHostMallocDefault));
-    TRY(hipMemset(c->d_counter, 0, 64));
-    TRY(hipMemset(c->d_result, 0, 64));
-    TRY(hipMemset(c->d_err, 0, 64));
-    TRY(hipMemset(c->d_cols, 0, GF_MAX_OBS * sizeof(double *)));
-    TRY(hipMemset(c->d_dz, 0, GF_MAX_DALITZ * sizeof(GfDalitzSlot)));
+    // device memory is written on the context's own stream: the stream is non-blocking,
+    // so work on the null stream (plain hipMemset / hipMemcpy) is not ordered with it
+    TRY(hipMemsetAsync(c->d_counter, 0, 64, c->stream));
+    TRY(hipMemsetAsync(c->d_result, 0, 64, c->stream));
+    TRY(hipMemsetAsync(c->d_err, 0, 64, c->stream));
+    TRY(hipMemsetAsync(c->d_cols, 0, GF_MAX_OBS * sizeof(double *), c->stream));
+    TRY(hipMemsetAsync(c->d_dz, 0, GF_MAX_DALITZ * sizeof(GfDalitzSlot), c->stream));
+    TRY(hipStreamSynchronize(c->stream));
     TRY(hipEventCreate(&c->ev0));
     TRY(hipEventCreate(&c->ev1));
 #undef TRY
@@ -1325,7 +1328,8 @@ static gf_status alloc_columns(gf_ctx *ctx, int ncols, int64_t n) {
     double *ptrs[GF_MAX_OBS];
     for(int k = 0; k < GF_MAX_OBS; k++)
         ptrs[k] = k < ncols ? ctx->col_base + (size_t)k * (size_t)n : ctx->col_base;
-    GF_HIP(hipMemcpy(ctx->d_cols, ptrs, sizeof(ptrs), hipMemcpyHostToDevice));
+    GF_HIP(hipMemcpyAsync(ctx->d_cols, ptrs, sizeof(ptrs), hipMemcpyHostToDevice, ctx->stream));
+    GF_HIP(hipStreamSynchronize(ctx->stream));
     return GF_OK;
 }
 
@@ -1347,7 +1351,9 @@ gf_status gf_events_upload(gf_ctx *ctx, int nobs, int64_t n, const double *const
     if(st)
         return st;
     for(int k = 0; k < nobs && n > 0; k++)
-        GF_HIP(hipMemcpy(ctx->col_base + (size_t)k * n, cols[k], (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+        GF_HIP(hipMemcpyAsync(ctx->col_base + (size_t)k * n, cols[k], (size_t)n * sizeof(double),
+                              hipMemcpyHostToDevice, ctx->stream));
+    GF_HIP(hipStreamSynchronize(ctx->stream));
     return GF_OK;
 }
 
@@ -1459,10 +1465,11 @@ gf_status gf_program_set(gf_ctx *ctx, const int32_t *funcs, int nfunc, const dou
         return st;
     GF_HIP(hipStreamSynchronize(ctx->stream));
     ctx->stage_in_flight = false;
-    GF_HIP(hipMemcpy(ctx->d_params, params, np * sizeof(double), hipMemcpyHostToDevice));
-    GF_HIP(hipMemcpy(ctx->d_consts, consts, nc * sizeof(double), hipMemcpyHostToDevice));
-    GF_HIP(hipMemcpy(ctx->d_norms, norms, nn * sizeof(double), hipMemcpyHostToDevice));
-    GF_HIP(hipMemcpy(ctx->d_obsid, obsid.data(), no * sizeof(int32_t), hipMemcpyHostToDevice));
+    GF_HIP(hipMemcpyAsync(ctx->d_params, params, np * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    GF_HIP(hipMemcpyAsync(ctx->d_consts, consts, nc * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    GF_HIP(hipMemcpyAsync(ctx->d_norms, norms, nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    GF_HIP(hipMemcpyAsync(ctx->d_obsid, obsid.data(), no * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    GF_HIP(hipStreamSynchronize(ctx->stream));
     ctx->params_dirty = ctx->norms_dirty = false;
     if((st = upload_ins(ctx)))
         return st;
@@ -1698,7 +1705,8 @@ gf_status gf_eval_metric(gf_ctx *ctx, int metric, double *out, int64_t n) {
            || hipStreamSynchronize(ctx->stream) != hipSuccess)
             st = fail(GF_ERR_HIP, "copy of the evaluated events failed");
         else if(err) {
-            (void)hipMemset(ctx->d_err, 0, sizeof(int32_t));
+            (void)hipMemsetAsync(ctx->d_err, 0, sizeof(int32_t), ctx->stream);
+            (void)hipStreamSynchronize(ctx->stream);
             st = (err & 2) ? fail(GF_ERR_STATE, "internal: a Dalitz node reached a kernel compiled without it "
                                                 "(error word %d)", err)
                            : fail(GF_ERR_INVALID, "Dalitz event number outside the wave cache (error word %d)", err);
@@ -1912,7 +1920,9 @@ gf_status gf_dalitz_set_data_size(gf_ctx *ctx, int dalitz_func, int64_t n, int64
         wc.wave = nullptr;
         return fail(GF_ERR_NOMEM, "cannot allocate a wave cache of %zu bytes", bytes);
     }
-    GF_HIP(hipMemset(wc.wave, 0, bytes));
+    // zeroed on the context stream, ordered before the kernels that fill it (a memset on
+    // the null stream can still be running when they write)
+    GF_HIP(hipMemsetAsync(wc.wave, 0, bytes, ctx->stream));
     if(d->td) {
         if(hipMalloc((void **)&wc.wave_b, bytes) != hipSuccess) {
             wc.wave_b = nullptr;
@@ -1920,8 +1930,9 @@ gf_status gf_dalitz_set_data_size(gf_ctx *ctx, int dalitz_func, int64_t n, int64
             wc.wave = nullptr;
             return fail(GF_ERR_NOMEM, "cannot allocate a wave cache of %zu bytes", bytes);
         }
-        GF_HIP(hipMemset(wc.wave_b, 0, bytes));
+        GF_HIP(hipMemsetAsync(wc.wave_b, 0, bytes, ctx->stream));
     }
+    GF_HIP(hipStreamSynchronize(ctx->stream));
     wc.n      = n;
     wc.offset = offset;
     wc.filled = 0;
@@ -2020,9 +2031,10 @@ gf_status gf_dalitz_gram_points(gf_ctx *ctx, int dalitz_func, const double *m12,
     if(hipMalloc((void **)&buf, bytes) != hipSuccess)
         return fail(GF_ERR_NOMEM, "cannot allocate %zu bytes of points", bytes);
     gf_status st = GF_OK;
-    if(hipMemcpy(buf, m12, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
-       || hipMemcpy(buf + np, m13, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
-       || (w && hipMemcpy(buf + 2 * np, w, np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
+    if(hipMemcpyAsync(buf, m12, np * sizeof(double), hipMemcpyHostToDevice, ctx->stream) != hipSuccess
+       || hipMemcpyAsync(buf + np, m13, np * sizeof(double), hipMemcpyHostToDevice, ctx->stream) != hipSuccess
+       || (w && hipMemcpyAsync(buf + 2 * np, w, np * sizeof(double), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+       || hipStreamSynchronize(ctx->stream) != hipSuccess)
         st = fail(GF_ERR_HIP, "copy of the points failed");
     if(!st)
         st = gf_dalitz_gram_points_device(ctx, dalitz_func, buf, buf + np, w ? buf + 2 * np : nullptr, np, eff_func,
@@ -2121,8 +2133,9 @@ gf_status gf_tddp_gram(gf_ctx *ctx, int tddp_func, const double grid6[6], double
             std::vector<GfResDesc> st_res(idx.size());
             for(size_t k = 0; k < idx.size(); k++)
                 st_res[k] = all[idx[k]];
-            if(hipMemcpy(dstack, st_res.data(), st_res.size() * sizeof(GfResDesc), hipMemcpyHostToDevice)
-               != hipSuccess) {
+            if(hipMemcpyAsync(dstack, st_res.data(), st_res.size() * sizeof(GfResDesc), hipMemcpyHostToDevice,
+                              ctx->stream) != hipSuccess
+               || hipStreamSynchronize(ctx->stream) != hipSuccess) {
                 st = fail(GF_ERR_HIP, "copy of the Gram stack failed");
                 break;
             }
@@ -2366,11 +2379,11 @@ gf_status mlp_layout_check(gf_ctx *ctx) {
         return fail(GF_ERR_NOMEM, "probe buffer");
     }
     gf_status st = GF_OK;
-    if(hipMemcpy(da, a8.data(), 512 * 2, hipMemcpyHostToDevice) != hipSuccess
-       || hipMemcpy(da + 512, b8.data(), 512 * 2, hipMemcpyHostToDevice) != hipSuccess
+    if(hipMemcpyAsync(da, a8.data(), 512 * 2, hipMemcpyHostToDevice, ctx->stream) != hipSuccess
+       || hipMemcpyAsync(da + 512, b8.data(), 512 * 2, hipMemcpyHostToDevice, ctx->stream) != hipSuccess
        || gfm::launch_bf16_probe(da, da + 512, dd, ctx->stream)
-       || hipStreamSynchronize(ctx->stream) != hipSuccess
-       || hipMemcpy(d.data(), dd, 256 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+       || hipMemcpyAsync(d.data(), dd, 256 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess
+       || hipStreamSynchronize(ctx->stream) != hipSuccess)
         st = fail(GF_ERR_HIP, "BF16 MFMA probe failed");
     (void)hipFree(da);
     (void)hipFree(dd);
@@ -2725,6 +2738,23 @@ gf_status gf_sample_uniform(gf_ctx *ctx, const double box[4], int64_t n, uint64_
     return GF_OK;
 }
 
+gf_status gf_mlp_train_generated(gf_mlp *m, int dalitz_func, int64_t nevents, uint64_t gen_seed, const double box[4],
+                                 int steps, int batch, float lr, uint64_t seed, float *loss_first, float *loss_last) {
+    if(!m || nevents < 1 || !box)
+        return fail(GF_ERR_INVALID, "need a network, a box and at least one event");
+    if(!valid_ctx(m->ctx))
+        return GF_ERR_INVALID;
+    double *buf = nullptr;
+    GF_HIP(hipMalloc((void **)&buf, (size_t)nevents * 2 * sizeof(double)));
+    gf_status st = gf_dalitz_generate(m->ctx, dalitz_func, nevents, gen_seed, box, 0.0, 0, buf, buf + nevents,
+                                      nullptr, nullptr);
+    if(!st)
+        st = gf_mlp_train(m, buf, buf + nevents, nevents, box, steps, batch, lr, seed, loss_first, loss_last);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    (void)hipFree(buf);
+    return st;
+}
+
 gf_status gf_mlp_train_host(gf_mlp *m, const double *m12, const double *m13, int64_t ndata, const double box[4],
                             int steps, int batch, float lr, uint64_t seed, float *loss_first, float *loss_last) {
     if(!m || !m12 || !m13 || ndata < 1)
@@ -2734,8 +2764,9 @@ gf_status gf_mlp_train_host(gf_mlp *m, const double *m12, const double *m13, int
     double *buf = nullptr;
     GF_HIP(hipMalloc((void **)&buf, (size_t)ndata * 2 * sizeof(double)));
     gf_status st = GF_OK;
-    if(hipMemcpy(buf, m12, (size_t)ndata * 8, hipMemcpyHostToDevice) != hipSuccess
-       || hipMemcpy(buf + ndata, m13, (size_t)ndata * 8, hipMemcpyHostToDevice) != hipSuccess)
+    if(hipMemcpyAsync(buf, m12, (size_t)ndata * 8, hipMemcpyHostToDevice, m->ctx->stream) != hipSuccess
+       || hipMemcpyAsync(buf + ndata, m13, (size_t)ndata * 8, hipMemcpyHostToDevice, m->ctx->stream) != hipSuccess
+       || hipStreamSynchronize(m->ctx->stream) != hipSuccess)
         st = fail(GF_ERR_HIP, "copy of the training events failed");
     if(!st)
         st = gf_mlp_train(m, buf, buf + ndata, ndata, box, steps, batch, lr, seed, loss_first, loss_last);
@@ -2776,6 +2807,39 @@ gf_status gf_dalitz_set_norm_points(gf_ctx *ctx, int dalitz_func, gf_mlp *propos
                             : gf_sample_uniform(ctx, box, n, seed, np.buf, np.buf + n, np.buf + 2 * n);
     if(st)
         return st;
+    np.n = n;
+    return GF_OK;
+}
+
+gf_status gf_dalitz_set_norm_rule(gf_ctx *ctx, int dalitz_func, const double *m12, const double *m13,
+                                  const double *w, int64_t n) {
+    if(!valid_ctx(ctx))
+        return GF_ERR_INVALID;
+    DalitzNode *d = find_dalitz(ctx, dalitz_func, nullptr);
+    if(!d)
+        return fail(GF_ERR_INVALID, "function %d is not an Amp3Body node", dalitz_func);
+    if(d->td)
+        return fail(GF_ERR_INVALID, "point normalisation is not available for Amp3Body_TD");
+    if(n < 1 || !m12 || !m13 || !w)
+        return fail(GF_ERR_INVALID, "need at least one point with its weight");
+    auto &np = ctx->norm_points[d->cache_key];
+    if(np.cap < n) {
+        if(np.buf) {
+            GF_HIP(hipStreamSynchronize(ctx->stream));
+            GF_HIP(hipFree(np.buf));
+        }
+        np.buf = nullptr;
+        np.cap = np.n = 0;
+        if(hipMalloc((void **)&np.buf, (size_t)n * 3 * sizeof(double)) != hipSuccess)
+            return fail(GF_ERR_NOMEM, "cannot allocate %lld normalisation points", (long long)n);
+        np.cap = n;
+    }
+    np.n = 0;
+    GF_HIP(hipStreamSynchronize(ctx->stream));
+    GF_HIP(hipMemcpyAsync(np.buf, m12, (size_t)n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    GF_HIP(hipMemcpyAsync(np.buf + n, m13, (size_t)n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    GF_HIP(hipMemcpyAsync(np.buf + 2 * n, w, (size_t)n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    GF_HIP(hipStreamSynchronize(ctx->stream));
     np.n = n;
     return GF_OK;
 }
@@ -3017,8 +3081,10 @@ gf_status batch_layout(gf_batch *b, int chunk) {
     b->d_partials = nullptr;
     GF_HIP(hipMalloc((void **)&b->d_work, w.size() * sizeof(GfBatchWork)));
     GF_HIP(hipMalloc((void **)&b->d_partials, w.size() * sizeof(gfk::Partial)));
-    GF_HIP(hipMemcpy(b->d_work, w.data(), w.size() * sizeof(GfBatchWork), hipMemcpyHostToDevice));
-    GF_HIP(hipMemcpy(b->d_first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice));
+    GF_HIP(hipMemcpyAsync(b->d_work, w.data(), w.size() * sizeof(GfBatchWork), hipMemcpyHostToDevice, b->ctx->stream));
+    GF_HIP(hipMemcpyAsync(b->d_first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice,
+                          b->ctx->stream));
+    GF_HIP(hipStreamSynchronize(b->ctx->stream));
     b->chunk   = chunk;
     b->nblocks = (int)w.size();
     return GF_OK;
@@ -3125,7 +3191,9 @@ gf_status gf_batch_create(gf_ctx *ctx, int ntoys, const int64_t *first, const in
        || hipHostGetDevicePointer((void **)&b->d_res, b->h_res, 0) != hipSuccess
        || hipHostMalloc((void **)&b->h_stamp, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess
        || hipHostGetDevicePointer((void **)&b->d_stamp, b->h_stamp, 0) != hipSuccess
-       || hipMalloc((void **)&b->d_counter, 64) != hipSuccess || hipMemset(b->d_counter, 0, 64) != hipSuccess)
+       || hipMalloc((void **)&b->d_counter, 64) != hipSuccess
+       || hipMemsetAsync(b->d_counter, 0, 64, ctx->stream) != hipSuccess
+       || hipStreamSynchronize(ctx->stream) != hipSuccess)
         st = fail(GF_ERR_NOMEM, "cannot allocate the batch buffers");
     if(!st) {
         *b->h_stamp = 0;
@@ -3418,10 +3486,12 @@ gf_status gf_batch_set_norm_ops(gf_batch *b, const gf_norm_op *ops, int nops, co
     GF_HIP(hipMalloc((void **)&b->d_norm_tmpl, (size_t)b->nn * sizeof(double)));
     GF_HIP(hipMalloc((void **)&b->d_norm_ops, (size_t)nops * sizeof(GfNormOp)));
     GF_HIP(hipMalloc((void **)&b->d_norm_table, (size_t)std::max(nt, 1) * sizeof(double)));
-    GF_HIP(hipMemcpy(b->d_norm_tmpl, norms_template, (size_t)b->nn * sizeof(double), hipMemcpyHostToDevice));
-    GF_HIP(hipMemcpy(b->d_norm_ops, ops, (size_t)nops * sizeof(GfNormOp), hipMemcpyHostToDevice));
+    hipStream_t s = b->ctx->stream;
+    GF_HIP(hipMemcpyAsync(b->d_norm_tmpl, norms_template, (size_t)b->nn * sizeof(double), hipMemcpyHostToDevice, s));
+    GF_HIP(hipMemcpyAsync(b->d_norm_ops, ops, (size_t)nops * sizeof(GfNormOp), hipMemcpyHostToDevice, s));
     if(nt > 0)
-        GF_HIP(hipMemcpy(b->d_norm_table, table, (size_t)nt * sizeof(double), hipMemcpyHostToDevice));
+        GF_HIP(hipMemcpyAsync(b->d_norm_table, table, (size_t)nt * sizeof(double), hipMemcpyHostToDevice, s));
+    GF_HIP(hipStreamSynchronize(s));
     b->norm_nops = nops;
     return GF_OK;
 }

@@ -251,11 +251,22 @@ void Amp3Body::setNormalisationPoints(int64_t n, unsigned long long seed, Dalitz
     if(!proposal->needsRetraining(relvar, proposal->retraining().factor))
         return;
     // train again on events of the model at its current parameters, then redraw
-    UnbinnedDataSet events({_m12, _m13, _eventNumber});
-    generateEvents(events, (size_t)proposal->retraining().events, seed ^ 0x5eedf00dULL);
-    proposal->retrain(events, seed + 1);
+    proposal->retrain(*this, seed);
     draw();
     proposal->setReferenceVariance(normalisationRelativeVariance(batches));
+}
+
+void Amp3Body::setNormalisationRule(const std::vector<fptype> &m12, const std::vector<fptype> &m13,
+                                    const std::vector<fptype> &w) {
+    if(m12.empty() || m12.size() != m13.size() || m12.size() != w.size())
+        throw GeneralError("Amp3Body::setNormalisationRule: need equal, non-empty m12, m13 and weight arrays");
+    engine::flush();
+    engine::check(gf_dalitz_set_norm_rule(engine::ctx(), (int)functionIdx, m12.data(), m13.data(), w.data(),
+                                          (int64_t)m12.size()),
+                  "gf_dalitz_set_norm_rule");
+    normPoints         = (int64_t)m12.size();
+    forceRedoIntegrals = true;
+    cacheOnly          = false;
 }
 
 void Amp3Body::setNormalisationGrid() {

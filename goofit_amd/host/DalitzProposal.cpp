@@ -2,6 +2,7 @@
 // the C ABI).
 #include <goofit/Engine.h>
 #include <goofit/Error.h>
+#include <goofit/PDFs/physics/Amp3Body.h>
 #include <goofit/PDFs/physics/DalitzProposal.h>
 
 namespace GooFit {
@@ -29,6 +30,20 @@ std::pair<float, float> DalitzProposal::train(const UnbinnedDataSet &events, int
     engine::check(gf_mlp_train_host(mlp_, a.data(), b.data(), (int64_t)a.size(), box_, steps, batch, lr, seed, &l0,
                                     &l1),
                   "gf_mlp_train_host");
+    built_ = false;
+    return {l0, l1};
+}
+
+std::pair<float, float> DalitzProposal::trainOnModel(Amp3Body &model, int64_t nevents, unsigned long long gen_seed,
+                                                     int steps, int batch, float lr, unsigned long long seed) {
+    if(nevents < 1)
+        throw GeneralError("DalitzProposal::trainOnModel: no events");
+    model.copyParams();
+    engine::flush();
+    float l0 = 0.f, l1 = 0.f;
+    engine::check(gf_mlp_train_generated(mlp_, (int)model.getFunctionIndex(), nevents, gen_seed, box_, steps, batch,
+                                         lr, seed, &l0, &l1),
+                  "gf_mlp_train_generated");
     built_ = false;
     return {l0, l1};
 }
@@ -67,8 +82,8 @@ void DalitzProposal::setRetraining(const Retraining &r) {
     retrain_on_ = true;
 }
 
-void DalitzProposal::retrain(const UnbinnedDataSet &events, unsigned long long seed) {
-    train(events, retrain_.steps, retrain_.batch, retrain_.lr, seed);
+void DalitzProposal::retrain(Amp3Body &model, unsigned long long seed) {
+    trainOnModel(model, retrain_.events, seed ^ 0x5eedf00dULL, retrain_.steps, retrain_.batch, retrain_.lr, seed + 1);
     build(retrain_.grid, retrain_.alpha);
     retrains_++;
 }

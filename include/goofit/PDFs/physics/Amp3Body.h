@@ -70,6 +70,13 @@ class Amp3Body : public Amp3BodyBase {
     /// importance weights.  The integrals are then sum_p w_p eff B_i conj(B_j) (no bin
     /// area factor).  No counterpart in the reference.
     void setNormalisationPoints(int64_t n, unsigned long long seed, DalitzProposal *proposal = nullptr);
+    /// Normalise by a deterministic quadrature rule over the plot (points m12[k], m13[k]
+    /// with weights w[k], e.g. a product Gauss-Legendre rule fitted to the kinematic
+    /// boundary): the integrals are sum_k w_k eff B_i conj(B_j) (no bin-area factor).
+    /// Unlike the midpoint grid, no cell straddles the boundary.  No counterpart in the
+    /// reference.
+    void setNormalisationRule(const std::vector<fptype> &m12, const std::vector<fptype> &m13,
+                              const std::vector<fptype> &w);
     /// Back to the reference's midpoint grid (the default).
     void setNormalisationGrid();
     int64_t getNormalisationPointCount() const { return normPoints; }

@@ -19,6 +19,8 @@
 
 namespace GooFit {
 
+class Amp3Body;
+
 class DalitzProposal {
   public:
     /// A network with seeded initial weights over the box of m12 x m13 (their limits).
@@ -31,6 +33,11 @@ class DalitzProposal {
     /// loss of the first and of the last step.
     std::pair<float, float> train(const UnbinnedDataSet &events, int steps = 2000, int batch = 8192,
                                   float lr = 3e-3f, unsigned long long seed = 1);
+    /// The same training on nevents events of the model's density at its current
+    /// parameters, generated on the GPU and kept there (no host copy of the events).
+    std::pair<float, float> trainOnModel(Amp3Body &model, int64_t nevents, unsigned long long gen_seed,
+                                         int steps = 2000, int batch = 8192, float lr = 3e-3f,
+                                         unsigned long long seed = 1);
     /// Tabulate the proposal on grid x grid cells with uniform mixture weight alpha.
     void build(int grid = 512, double alpha = 0.1);
     bool isBuilt() const { return built_; }
@@ -68,8 +75,8 @@ class DalitzProposal {
     const Retraining &retraining() const { return retrain_; }
     /// Retrains done by the trigger so far.
     int retrainCount() const { return retrains_; }
-    /// Train and build again on `events` (one retraining of the trigger).
-    void retrain(const UnbinnedDataSet &events, unsigned long long seed);
+    /// Train and build again on events of the model (one retraining of the trigger).
+    void retrain(Amp3Body &model, unsigned long long seed);
 
     const double *box() const { return box_; }
     gf_mlp *handle() const { return mlp_; }

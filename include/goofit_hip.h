@@ -380,6 +380,12 @@ gf_status gf_mlp_use_graph(gf_mlp *m, int on);
 gf_status gf_mlp_train_host(gf_mlp *m, const double *m12, const double *m13, int64_t ndata, const double box[4],
                             int steps, int batch, float lr, uint64_t seed, float *loss_first, float *loss_last);
 
+/* gf_mlp_train on nevents events of Amp3Body dalitz_func's density at its current
+ * parameters, generated on the device (gf_dalitz_generate with gen_seed and the
+ * estimated envelope) and never copied to the host. */
+gf_status gf_mlp_train_generated(gf_mlp *m, int dalitz_func, int64_t nevents, uint64_t gen_seed, const double box[4],
+                                 int steps, int batch, float lr, uint64_t seed, float *loss_first, float *loss_last);
+
 /* Tabulate the proposal on a grid x grid cell-centre table over box:
  * p_cell = (1 - alpha) exp(f) / sum exp(f) + alpha / grid^2 (0 < alpha <= 1). */
 gf_status gf_mlp_build_proposal(gf_mlp *m, const double box[4], int grid, double alpha);
@@ -391,6 +397,14 @@ gf_status gf_mlp_sample(gf_mlp *m, int64_t n, uint64_t seed, double *dev_m12, do
  * tabulated proposal when proposal is not NULL, else uniform over box. */
 gf_status gf_dalitz_set_norm_points(gf_ctx *ctx, int dalitz_func, gf_mlp *proposal, int64_t n, uint64_t seed,
                                     const double box[4]);
+
+/* Keep the n points and weights of a deterministic quadrature rule (host arrays)
+ * as the normalisation points of Amp3Body dalitz_func, in place of drawn ones:
+ * the integrals become sum_k w_k eff B_i conj(B_j) over them.  No reference
+ * counterpart (the reference integrates on the midpoint grid,
+ * src/PDFs/physics/detail/SpecialResonanceIntegrator.cu:52-121). */
+gf_status gf_dalitz_set_norm_rule(gf_ctx *ctx, int dalitz_func, const double *m12, const double *m13,
+                                  const double *w, int64_t n);
 
 /* Integrals I_ij = sum_p w_p eff_p B_i(p) conj(B_j(p)) over the kept points
  * [offset, offset + count) (weights as drawn for all n points). */

@@ -324,10 +324,33 @@ def test_integral_at_points_matches_the_oracle():
     gram = (waves * w) @ waves.conj().T
     ref = O.dalitz_norm_from_integrals(coups, gram, 1.0, 1.0)
     assert math.isclose(got, ref, rel_tol=1e-11), (got, ref)
+    # the same rule as the Amp3Body's normalisation: normalize() returns that integral
+    m.pdf.setNormalisationRule(a, b, w)
+    assert math.isclose(m.pdf.normalize(), got, rel_tol=1e-13)
+    m.pdf.setNormalisationGrid()
     finer = m.pdf.integralAtPoints(*W.square_dalitz_rule(960))
     assert abs(got / finer - 1) < 1e-5, (got, finer)
     a, b, w = W.square_dalitz_rule(1920)
     assert abs(finer / m.pdf.integralAtPoints(a, b, w) - 1) < 1e-8
+
+
+def test_first_nll_after_set_data_size_on_a_large_cache():
+    """setDataSize zeroes a new wave cache (here 16 x 1.2e7 x 16 B = 3 GB); the zeroing is
+    ordered before the lineshape kernel that fills it, so the first NLL equals the NLL after
+    the cache is recomputed (a zeroing still running when the kernel wrote used to erase part
+    of the cache)."""
+    m = W.build_dalitz(nbins=400, resonances=W.DALITZ_RESONANCES[:8])
+    n = 12_000_000
+    a12, a13 = W.dalitz_uniform(n, seed=91)
+    data = W.dalitz_dataset(m, a12, a13)
+    m.pdf.setData(data)
+    values = []
+    for _ in range(2):
+        m.pdf.setDataSize(n)
+        values.append(m.pdf.calculateNLL())
+        m.pdf.setForceIntegrals()
+        values.append(m.pdf.calculateNLL())
+    assert len(set(values)) == 1, values
 
 
 def _exp_sample(rate, n, hi, seed):
