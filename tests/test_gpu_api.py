@@ -335,7 +335,7 @@ def test_integral_at_points_matches_the_oracle():
 
 
 def test_first_nll_after_set_data_size_on_a_large_cache():
-    """setDataSize zeroes a new wave cache (here 16 x 1.2e7 x 16 B = 3 GB); the zeroing is
+    """setDataSize zeroes a new wave cache (here 8 x 1.2e7 x 16 B = 1.5 GB); the zeroing is
     ordered before the lineshape kernel that fills it, so the first NLL equals the NLL after
     the cache is recomputed (a zeroing still running when the kernel wrote used to erase part
     of the cache)."""
