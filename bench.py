@@ -640,15 +640,21 @@ def main():
         log("profile-only run done: 20 NLL reductions, 3 copies of 256 MiB")
         return
 
-    # dominant kernel: the coherent-sum NLL sweep over the events, timed with HIP
+    # dominant kernel: the coherent-sum NLL sweep over the events, which adds its
+    # workgroups' partials itself (the in-kernel finish, on by default), timed with HIP
     # events on the engine's stream: 200 sweeps launched back to back between two
     # events (the kernel's duration plus the dispatch gap between launches), and
     # 200 whole reductions with events around each sweep (which adds the idle gap
     # before each dispatch); measured first, so that the device is at its working
-    # clocks for the timed steps
+    # clocks for the timed steps.  The sweep alone (its partials left to a second
+    # launch) is timed back to back beside it.
     reps = 200
+    fused = G.engine.get_tuning(G.engine.TUNE_FUSED_FINISH) == 1
     ms_b2b = G.engine.time_sweep_back_to_back(1, reps)
     ms_sweep, ms_span = G.engine.time_reduce_sweep(1, reps)
+    G.engine.set_tuning(G.engine.TUNE_FUSED_FINISH, 0)
+    ms_sweep_only = G.engine.time_sweep_back_to_back(1, reps)
+    G.engine.set_tuning(G.engine.TUNE_FUSED_FINISH, 1 if fused else 0)
 
     dt = time_steps(pdf, model.params, args.steps, args.warmup, sync)
     ms_per_step = dt / args.steps * 1e3
@@ -662,15 +668,21 @@ def main():
     bytes_per_event = 16 * 16 + (0 if zero_outside else 16)
     achieved = bytes_per_event * args.events / (ms_kernel * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic() if args.events == 1_000_000 else (None, None)
+    achieved_sweep_only = bytes_per_event * args.events / (ms_sweep_only * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=traffic_src,
-                    kernel="k_dalitz_fast (coherent-sum NLL sweep)", kernel_ms=round(ms_kernel, 5),
+                    kernel="k_dalitz_fast (coherent-sum NLL sweep"
+                           + (", partials added in the kernel)" if fused else ")"),
+                    kernel_ms=round(ms_kernel, 5),
                     kernel_timing="200 sweeps back to back between two HIP events on the engine stream",
                     kernel_ms_within_a_call=round(ms_sweep / reps, 5),
                     reduction_ms=round(ms_span / reps, 5), bytes_per_event=bytes_per_event,
-                    bytes_per_launch=bytes_per_event * args.events)
+                    bytes_per_launch=bytes_per_event * args.events,
+                    kernel_ms_sweep_only=round(ms_sweep_only, 5),
+                    frac_sweep_only=round(achieved_sweep_only / HBM_PEAK_GBS, 4))
     log(f"NLL sweep {ms_kernel * 1e3:.1f} us back to back -> {achieved:.0f} GB/s; {ms_sweep / reps * 1e3:.1f} us "
-        f"within a call; whole reduction {ms_span / reps * 1e3:.1f} us")
+        f"within a call; whole reduction {ms_span / reps * 1e3:.1f} us; the sweep alone "
+        f"{ms_sweep_only * 1e3:.1f} us ({achieved_sweep_only:.0f} GB/s)")
     # practical streaming-read rate of this device: torch's own reduction over 256 MiB,
     # timed the same way (events around each launch on the launching stream)
     buf = torch.ones(32 << 20, dtype=torch.float64, device="cuda")

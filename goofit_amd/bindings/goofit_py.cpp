@@ -774,6 +774,7 @@ PYBIND11_MODULE(_goofit, m) {
     me.attr("TUNE_SPIN_US")             = (int)GF_TUNE_SPIN_US;
     me.attr("TUNE_ZERO_OUTSIDE")        = (int)GF_TUNE_ZERO_OUTSIDE;
     me.attr("TUNE_FLAT")                = (int)GF_TUNE_FLAT;
+    me.attr("TUNE_FUSED_FINISH")        = (int)GF_TUNE_FUSED_FINISH;
     me.def("callnumber", []() { return host_callnumber; });
     me.def("flattened", []() {
         return py::make_tuple(host_function_table, host_parameters, host_constants, host_observables,

@@ -134,6 +134,7 @@ struct gf_ctx {
     GfResult *d_res_mapped   = nullptr;  // its device address
     double *h_result         = nullptr;  // pinned staging for small copies
     int32_t *d_err           = nullptr;  // device error word (bit 0: cache index out of range, bit 1: node kind not compiled in)
+    unsigned long long *d_fuse_slots = nullptr;  // in-kernel finish: 2 x GF_MAX_PARTIALS words
     bool fast_dalitz         = false;    // program is [DALITZ, MULTI_POLY, DALITZ_END, END]
     bool flat                = false;    // program is one level of 1-D leaves (flat_program)
     double *d_gram_partials  = nullptr;
@@ -155,7 +156,7 @@ struct gf_ctx {
     uint32_t noalloc_seq = 0;  // of the last gf_reduce_metric_noalloc
 
     // launch-shape knobs (gf_tune)
-    int tune[GF_TUNE_COUNT] = {1, 1, 4, 4, 2000, 1, 1};
+    int tune[GF_TUNE_COUNT] = {1, 1, 4, 4, 2000, 1, 1, 1};
 
     uint64_t events_gen = 1;  // bumped whenever the event columns may change
 
@@ -812,6 +813,8 @@ GfProgArgs prog_args(gf_ctx *ctx) {
     a.seq          = ctx->res_seq;
     a.mark         = nullptr;
     a.sweep_only   = 0;
+    a.fuse_slots   = nullptr;
+    a.fuse_result  = nullptr;
     return a;
 }
 
@@ -882,6 +885,10 @@ gf_status read_result(gf_ctx *ctx, double *sum, double *comp) {
     r.sum  = h->sum;
     r.comp = h->comp;
     r.err  = h->err;
+    if(r.err & GF_DEVERR_FINISH)
+        return fail(GF_ERR_HIP, "in-kernel finish: a workgroup's partial was not visible to the finishing workgroup "
+                                "(error word %d)",
+                    r.err);
     if(r.err & 2)
         return fail(GF_ERR_STATE, "internal: a Dalitz node reached a kernel compiled without it (error word %d)",
                     r.err);
@@ -988,8 +995,13 @@ gf_status launch_reduce(gf_ctx *ctx, int metric, hipEvent_t mark = nullptr, bool
     GfProgArgs a = prog_args(ctx);
     a.mark       = mark;
     a.sweep_only = sweep_only ? 1 : 0;
+    const bool fast = use_fast_dalitz(ctx, metric);
+    if(ctx->tune[GF_TUNE_FUSED_FINISH] && (fast || use_flat(ctx, metric))) {
+        a.fuse_slots  = ctx->d_fuse_slots;
+        a.fuse_result = ctx->d_res_mapped;
+    }
     int rc;
-    if(use_fast_dalitz(ctx, metric))
+    if(fast)
         rc = gfk::launch_dalitz_fast(a, have_inl ? &inl : nullptr, const_efficiency(ctx), zero_outside(ctx), metric, ctx->d_partials,
                                      ctx->d_res_mapped, nullptr, fast_blocks(ctx, ctx->n),
                                      ctx->stream);
@@ -1160,6 +1172,7 @@ gf_status gf_ctx_set_tuning(gf_ctx *ctx, int key, int value) {
     case GF_TUNE_INLINE_ARGS:
     case GF_TUNE_FAST_DALITZ:
     case GF_TUNE_ZERO_OUTSIDE:
+    case GF_TUNE_FUSED_FINISH:
         if(value != 0 && value != 1)
             return fail(GF_ERR_INVALID, "tuning key %d takes 0 or 1 (got %d)", key, value);
         break;
@@ -1228,6 +1241,7 @@ gf_status gf_ctx_create(gf_ctx **out, int device) {
     TRY(hipMalloc((void **)&c->d_counter, 64));
     TRY(hipMalloc((void **)&c->d_result, 64));
     TRY(hipMalloc((void **)&c->d_err, 64));
+    TRY(hipMalloc((void **)&c->d_fuse_slots, 2 * GF_MAX_PARTIALS * sizeof(unsigned long long)));
     TRY(hipHostMalloc((void **)&c->h_res, sizeof(GfResult), hipHostMallocMapped | hipHostMallocCoherent));
     TRY(hipHostGetDevicePointer((void **)&c->d_res_mapped, c->h_res, 0));
     // Pinned blocks are recycled between contexts: a block freed by an earlier context
@@ -1244,6 +1258,11 @@ gf_status gf_ctx_create(gf_ctx **out, int device) {
     TRY(hipMemsetAsync(c->d_err, 0, 64, c->stream));
     TRY(hipMemsetAsync(c->d_cols, 0, GF_MAX_OBS * sizeof(double *), c->stream));
     TRY(hipMemsetAsync(c->d_dz, 0, GF_MAX_DALITZ * sizeof(GfDalitzSlot), c->stream));
+    if(gfk::launch_fuse_init(c->d_fuse_slots, c->stream)) {
+        st = fail(GF_ERR_HIP, "in-kernel finish slots: init launch failed");
+        gf_ctx_destroy(c);
+        return st;
+    }
     TRY(hipStreamSynchronize(c->stream));
     TRY(hipEventCreate(&c->ev0));
     TRY(hipEventCreate(&c->ev1));
@@ -1279,7 +1298,8 @@ void gf_ctx_destroy(gf_ctx *c) {
         (void)hipHostFree(c->h_multi_out);
     void *dev[] = {c->col_base,  c->d_cols,   c->d_params,   c->d_consts,        c->d_norms,
                    c->d_obsid,   c->d_ins,    c->d_dz,       c->d_res,           c->d_partials,
-                   c->d_counter, c->d_result, c->d_err, c->d_gram_partials, c->d_gram_tiles};
+                   c->d_counter, c->d_result, c->d_err, c->d_gram_partials, c->d_gram_tiles,
+                   c->d_fuse_slots};
     for(void *p : dev)
         if(p)
             (void)hipFree(p);

@@ -17,6 +17,12 @@
 #define GF_MAX_DALITZ 4   // Dalitz nodes per program
 #define GF_BLOCK 256      // threads per block of the event kernels
 #define GF_MAX_PARTIALS 4096
+// An empty slot of the in-kernel finish: a signalling NaN.  Arithmetic never
+// returns one (IEEE operations return quiet NaNs), so no partial equals it.
+#define GF_SLOT_EMPTY 0x7FF4A5A5A5A5A5A5ull
+// Device error word bit: a partial of the in-kernel finish did not become visible
+// within the finishing workgroup's bounded wait.
+#define GF_DEVERR_FINISH 4
 
 // Instruction codes of the compiled program.  Leaf codes compute T; the
 // combining steps act on the accumulator of one nesting level and are encoded
@@ -114,6 +120,12 @@ struct GfProgArgs {
     int32_t has_dz;              // 1: the program holds a Dalitz node (either kind)
     int32_t *err;                // device error word (bit 0: cache index out of range, bit 1: node kind not compiled in)
     uint32_t seq;                // sequence number k_finish stamps on its GfResult
+    // In-kernel finish (lone-Amp3Body and flat reductions; null: k_finish adds the
+    // partials).  fuse_slots: 2 x GF_MAX_PARTIALS words (sum, residual per workgroup),
+    // GF_SLOT_EMPTY when unused and between launches; fuse_result: the block k_finish
+    // would write.
+    unsigned long long *fuse_slots;
+    struct GfResult *fuse_result;
     void *mark;                  // host only: hipEvent_t recorded between a sweep and its k_finish
     int32_t sweep_only;          // host only: launch the sweep without its k_finish (timing)
 };
@@ -194,6 +206,8 @@ int launch_eval_bins(const GfProgArgs &a, int pc0, const double *ranges, const i
                      double *out, int nblocks, void *stream);
 // Cell centres of a k-dimensional grid: out[d * total + bin] (device).
 int launch_grid_cells(const double *ranges, int k, int64_t total, double *out, void *stream);
+// Mark every slot of the in-kernel finish empty.
+int launch_fuse_init(unsigned long long *slots, void *stream);
 // Cell centres of a Dalitz integration grid grid6 (host array), cells 0..ncells-1 (device outputs).
 int launch_dalitz_cells(const double *grid6, int td_grid, int64_t ncells, double *m12, double *m13, void *stream);
 // Toy batch: per-workgroup sweep with per-toy parameters, per-toy finishing

@@ -507,6 +507,102 @@ __global__ __launch_bounds__(GF_BLOCK) void k_finish(const Partial *partials, in
     }
 }
 
+// The in-kernel finish: k_finish's work done inside the sweep by its last
+// workgroup (the highest index), with no second launch.  Every other workgroup
+// stores its (sum, residual) pair into its slot with agent-scope atomic stores
+// (coherent across the XCDs' L2 caches) and exits.  Workgroups are dispatched in
+// index order on each XCD, so the last one starts after every workgroup it waits
+// for has been placed and holds no resource they need.  It reads the slots with
+// agent-scope atomic loads -- a thread's slots t, t+256, ... four at a time, all
+// eight loads in flight together -- until none is GF_SLOT_EMPTY, adds them exactly
+// as k_finish does (same order, same bits; its own pair taken from LDS), empties
+// them for the next launch and writes the result block.  No workgroup runs a
+// release fence: on this device an agent-scope release writes back the XCD's L2,
+// once per workgroup.  The wait is bounded; a slot still empty after it sets
+// GF_DEVERR_FINISH in the result's error word.
+__device__ __forceinline__ void finish_in_kernel(double s, double c, const GfProgArgs &A) {
+    constexpr uint32_t spin_limit = 1u << 18;
+    constexpr int group           = 4;
+    __shared__ double own_s, own_c;
+    block_reduce(s, c);
+    unsigned long long *slots = A.fuse_slots;
+    const int nparts          = (int)gridDim.x;
+    const int mine            = nparts - 1;
+    if((int)blockIdx.x != mine) {
+        if(threadIdx.x == 0) {
+            __hip_atomic_store(slots + 2 * blockIdx.x, (unsigned long long)__double_as_longlong(s),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(slots + 2 * blockIdx.x + 1, (unsigned long long)__double_as_longlong(c),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return;
+    }
+    if(threadIdx.x == 0) {
+        own_s = s;
+        own_c = c;
+    }
+    __syncthreads();
+    double ts = 0.0, tc = 0.0;
+    int late  = 0;
+    for(int i0 = threadIdx.x; i0 < nparts; i0 += group * GF_BLOCK) {
+        unsigned long long vs[group], vc[group];
+#pragma unroll
+        for(int g = 0; g < group; ++g)
+            vs[g] = vc[g] = GF_SLOT_EMPTY;
+        for(uint32_t spin = 0;; ++spin) {
+#pragma unroll
+            for(int g = 0; g < group; ++g) {
+                const int i = i0 + g * GF_BLOCK;
+                if(i < mine && (vs[g] == GF_SLOT_EMPTY || vc[g] == GF_SLOT_EMPTY)) {
+                    vs[g] = __hip_atomic_load(slots + 2 * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    vc[g] = __hip_atomic_load(slots + 2 * i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+            bool done = true;
+#pragma unroll
+            for(int g = 0; g < group; ++g)
+                if(i0 + g * GF_BLOCK < mine && (vs[g] == GF_SLOT_EMPTY || vc[g] == GF_SLOT_EMPTY))
+                    done = false;
+            if(done)
+                break;
+            if(spin == spin_limit) {
+                late = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+#pragma unroll
+        for(int g = 0; g < group; ++g) {
+            const int i = i0 + g * GF_BLOCK;
+            if(i < mine) {
+                gfd::kb_merge(ts, tc, __longlong_as_double((long long)vs[g]), __longlong_as_double((long long)vc[g]));
+                __hip_atomic_store(slots + 2 * i, GF_SLOT_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(slots + 2 * i + 1, GF_SLOT_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else if(i == mine) {
+                gfd::kb_merge(ts, tc, own_s, own_c);
+            }
+        }
+    }
+    late = __syncthreads_or(late);
+    block_reduce(ts, tc);
+    if(threadIdx.x == 0) {
+        GfResult *result = A.fuse_result;
+        const int32_t e  = *A.err;  // written by earlier launches only
+        result->sum      = gfd::kb_total(ts, tc);
+        result->comp     = tc;
+        result->err      = e | (late ? GF_DEVERR_FINISH : 0);
+        if(e)
+            *A.err = 0;
+        __threadfence_system();
+        __hip_atomic_store(&result->seq, A.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fuse_init(unsigned long long *slots) {
+    for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * GF_MAX_PARTIALS; i += gridDim.x * blockDim.x)
+        slots[i] = GF_SLOT_EMPTY;
+}
+
 // ------------------------------------------------------------- event kernels
 
 template <int EPT, bool INL, int FEAT>
@@ -707,7 +803,10 @@ __global__ __launch_bounds__(GF_BLOCK) void k_reduce_flat(GfProgArgs A, GfInline
                 gfd::kb_add(s, c, v);
         }
     }
-    store_partial(s, c, partials);
+    if(A.fuse_slots)
+        finish_in_kernel(s, c, A);
+    else
+        store_partial(s, c, partials);
 }
 
 template <int EPT, bool INL, int FEAT>
@@ -832,8 +931,12 @@ __global__ __launch_bounds__(256) void k_dalitz_fast(GfProgArgs A, GfInline I, i
             gfd::kb_add(s, c, v);
         }
     }
-    if(!WRITE)
-        store_partial(s, c, partials);
+    if(!WRITE) {
+        if(A.fuse_slots)
+            finish_in_kernel(s, c, A);
+        else
+            store_partial(s, c, partials);
+    }
 }
 
 // The coherent-sum metric sum of up to GF_MULTI_PTS parameter sets over the same
@@ -1664,7 +1767,7 @@ static int launch_finish(const GfProgArgs &a, Partial *partials, int nblocks, Gf
         return rc;
     if(a.mark && (rc = check(hipEventRecord((hipEvent_t)a.mark, s))))
         return rc;
-    if(a.sweep_only)
+    if(a.sweep_only || a.fuse_slots)  // a fused sweep has finished itself
         return 0;
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(GF_BLOCK), 0, s, partials, nblocks, result, a.err, a.seq);
     return check(hipGetLastError());
@@ -1676,6 +1779,8 @@ static int feature_level(const GfProgArgs &a) { return a.td ? 2 : (a.has_dz ? 1 
 int launch_reduce_metric(const GfProgArgs &a, const GfInline *inl, int metric, Partial *partials, GfResult *result,
                          int nblocks, void *stream) {
     hipStream_t s = (hipStream_t)stream;
+    if(a.fuse_slots)  // the interpreter has no in-kernel finish
+        return -1;
     // time-dependent nodes: the kernels compiled with the TD ops (two events per lane
     // measured 1.5 % faster than one on the 16-resonance three-Gaussian workload);
     // programs without Dalitz nodes: the kernels compiled without them
@@ -1812,6 +1917,8 @@ int launch_reduce_bins(const GfProgArgs &a, int pc0, const double *ranges, const
                        Partial *partials, GfResult *result, int nblocks, void *stream) {
     const GfGrid G = make_grid(ranges, slots, k);
     hipStream_t s  = (hipStream_t)stream;
+    if(a.fuse_slots)  // no in-kernel finish here
+        return -1;
     hipLaunchKernelGGL(k_reduce_bins<2>, dim3(nblocks), dim3(GF_BLOCK), 0, s, a, pc0, G, total, partials);
     return launch_finish(a, partials, nblocks, result, s);
 }
@@ -1831,6 +1938,11 @@ static unsigned cell_blocks(int64_t total) {
 int launch_grid_cells(const double *ranges, int k, int64_t total, double *out, void *stream) {
     const GfGrid G = make_grid(ranges, nullptr, k);
     hipLaunchKernelGGL(k_grid_cells, dim3(cell_blocks(total)), dim3(256), 0, (hipStream_t)stream, G, total, out);
+    return check(hipGetLastError());
+}
+
+int launch_fuse_init(unsigned long long *slots, void *stream) {
+    hipLaunchKernelGGL(k_fuse_init, dim3(2 * GF_MAX_PARTIALS / 256), dim3(256), 0, (hipStream_t)stream, slots);
     return check(hipGetLastError());
 }
 

@@ -116,7 +116,11 @@ enum gf_tune {
                                        kernel that loads every column of a tile before evaluating it, with
                                        2 or 4 events per lane (the value), or 1: 4 from 2^20 events on,
                                        2 below; 0: the general interpreter (default 1) */
-    GF_TUNE_COUNT              = 7
+    GF_TUNE_FUSED_FINISH       = 7, /* 1: the coherent-sum and flat reductions add their workgroups'
+                                       partials inside the sweep (the last workgroup to finish does it,
+                                       in k_finish's order: the same bits) instead of in a second launch;
+                                       0: a second launch (default 1) */
+    GF_TUNE_COUNT              = 8
 };
 
 /* Set / read one knob (gf_tune); values out of range give GF_ERR_INVALID. */
@@ -513,14 +517,16 @@ gf_status gf_time_reduce_metric(gf_ctx *ctx, int metric, int reps, double *ms_to
 
 /* Same launches (reps <= 4096), with events around each event sweep: ms_sweep is
  * the summed device time of the sweep kernels alone (without the kernel that
- * adds the per-workgroup partials), ms_total (optional) the whole span.
+ * adds the per-workgroup partials, or including that work where the sweep does it
+ * itself, GF_TUNE_FUSED_FINISH), ms_total (optional) the whole span.
  * No reference counterpart. */
 gf_status gf_time_reduce_sweep(gf_ctx *ctx, int metric, int reps, double *ms_sweep, double *ms_total);
 
 /* The event sweep alone, reps <= 4096 launches back to back between two events
- * (no finishing kernel in between and no result read): ms_per_sweep = elapsed /
- * reps, the sweep kernel's duration plus the gap between back-to-back dispatches.
- * The next reduction of the context is unaffected.  No reference counterpart. */
+ * (no finishing kernel in between and no result read; a sweep that adds its
+ * partials itself still does): ms_per_sweep = elapsed / reps, the sweep kernel's
+ * duration plus the gap between back-to-back dispatches.  The next reduction of
+ * the context is unaffected.  No reference counterpart. */
 gf_status gf_time_sweep_back_to_back(gf_ctx *ctx, int metric, int reps, double *ms_per_sweep);
 
 /* Same for gf_dalitz_cache (all resonances) and gf_dalitz_gram. */

@@ -42,20 +42,29 @@ def _sync(ctx, M):
 
 
 def _variants(ctx, ref):
-    """NLL under every (inline, workgroups per CU) setting; the defaults restored after."""
+    """NLL under every (inline, workgroups per CU) setting, each with the partials added
+    inside the sweep and by a second launch (the same bits); the defaults restored after."""
     out = {}
     try:
         for blocks in (1, 4, 16):
             ctx.set_tuning(capi.TUNE_BLOCKS_PER_CU, blocks)
+            ctx.set_tuning(capi.TUNE_FAST_BLOCKS_PER_CU, blocks)
             for inline in (1, 0):
                 ctx.set_tuning(capi.TUNE_INLINE_ARGS, inline)
-                got = ctx.reduce_metric("NLL")[0]
+                got = {}
+                for fused in (1, 0):
+                    ctx.set_tuning(capi.TUNE_FUSED_FINISH, fused)
+                    got[fused] = ctx.reduce_metric("NLL")
+                assert got[1] == got[0], (blocks, inline, got)
+                got = got[1][0]
                 assert math.isclose(got, ref, rel_tol=NLL_RTOL), (blocks, inline, got, ref)
                 out[blocks, inline] = got
             assert out[blocks, 1] == out[blocks, 0], blocks
     finally:
         ctx.set_tuning(capi.TUNE_BLOCKS_PER_CU, 4)
+        ctx.set_tuning(capi.TUNE_FAST_BLOCKS_PER_CU, 4)
         ctx.set_tuning(capi.TUNE_INLINE_ARGS, 1)
+        ctx.set_tuning(capi.TUNE_FUSED_FINISH, 1)
     return out
 
 
@@ -183,9 +192,9 @@ def test_calculate_nll_aborts_on_non_positive_normalisation():
 
 
 def test_back_to_back_sweep_timing_leaves_the_context_as_it_was(ctx):
-    """gf_time_sweep_back_to_back launches sweeps without their finishing kernel: the
-    reductions before and after it give the same bits, with parameters inline and in
-    device arrays."""
+    """gf_time_sweep_back_to_back launches sweeps without their finishing kernel (or sweeps
+    that add their partials themselves): the reductions before and after it give the same
+    bits, with parameters inline and in device arrays."""
     n = 20_000
     top, _ = dalitz(nbins=64)
     aos = dalitz_events(n, 35)
@@ -207,3 +216,75 @@ def test_back_to_back_sweep_timing_leaves_the_context_as_it_was(ctx):
             assert before == after and math.isclose(after, ref, rel_tol=NLL_RTOL)
     finally:
         ctx.set_tuning(capi.TUNE_INLINE_ARGS, 1)
+
+
+def test_in_kernel_finish_between_other_launches(ctx):
+    """The coherent-sum sweep with its partials added inside the kernel leaves its slots empty
+    for the next launch: 60 reductions with moving couplings, interleaved with back-to-back
+    timing sweeps, multi-point sweeps, per-event evaluations and the interpreter, each equal
+    bit for bit to the same reduction finished by a second launch."""
+    n = 30_000
+    top, _ = dalitz(nbins=64)
+    aos = dalitz_events(n, 36)
+    M = O.Model(top, aos, decay=W.DALITZ_DECAY)
+    M.calculate_nll()
+    _install(ctx, top, [aos[:, 0], aos[:, 1], aos[:, 2]], decay=W.DALITZ_DECAY)
+    ctx.set_identity_column(2)
+    ctx.dalitz_set_data_size(top.function_idx, n)
+    M.op.sync_params()
+    ctx.update_params(M.op.p)
+    ctx.dalitz_cache(top.function_idx)
+    _sync(ctx, M)
+    base = np.array(M.op.p, dtype=np.float64)
+    rng = np.random.default_rng(7)
+    try:
+        for it in range(60):
+            p = base.copy()  # the couplings move (the waves do not depend on them)
+            c0 = top.parameters_idx + 1
+            p[c0:c0 + 8] *= 1 + 0.05 * rng.standard_normal(8)
+            ctx.update_params(p)
+            ctx.set_tuning(capi.TUNE_FUSED_FINISH, 1)
+            fused = ctx.reduce_metric("NLL")
+            kind = it % 4
+            if kind == 0:
+                assert ctx.time_sweep_back_to_back("NLL", 3) > 0
+            elif kind == 1:
+                multi = ctx.reduce_metric_multi("NLL", [p, p], [M.op.n, M.op.n])
+                assert multi[0] == fused[0] and multi[1] == fused[0]
+            elif kind == 2:
+                assert ctx.eval_metric("NLL").size == n
+            else:
+                ctx.set_tuning(capi.TUNE_FAST_DALITZ, 0)
+                slow = ctx.reduce_metric("NLL")[0]
+                ctx.set_tuning(capi.TUNE_FAST_DALITZ, 1)
+                assert math.isclose(slow, fused[0], rel_tol=1e-13)
+            ctx.set_tuning(capi.TUNE_FUSED_FINISH, 0)
+            second = ctx.reduce_metric("NLL")
+            assert fused == second, (it, fused, second)
+            ctx.set_tuning(capi.TUNE_FUSED_FINISH, 1)
+            assert ctx.reduce_metric("NLL") == fused, it
+    finally:
+        ctx.set_tuning(capi.TUNE_FUSED_FINISH, 1)
+        ctx.set_tuning(capi.TUNE_FAST_DALITZ, 1)
+
+
+@pytest.mark.parametrize("n", [1, 255, 257, 4097, 1_000_003])
+def test_in_kernel_finish_flat_sizes(ctx, n):
+    """The flat ProdPdf sweep at sizes from one event (a single workgroup finishes itself) to
+    1e6 + 3 (a grid capped by the device): the same bits with the partials added in the
+    kernel or by a second launch, and the oracle's NLL."""
+    top3, _ = prod3d()
+    xs, ys, zs = W.sample_prod3d(n, seed=40 + n % 97)
+    _install(ctx, top3, [xs, ys, zs])
+    M3 = O.Model(top3, np.stack([xs, ys, zs], axis=1))
+    ref = M3.calculate_nll()
+    _sync(ctx, M3)
+    try:
+        got = {}
+        for fused in (1, 0, 1):
+            ctx.set_tuning(capi.TUNE_FUSED_FINISH, fused)
+            got.setdefault(fused, []).append(ctx.reduce_metric("NLL"))
+        assert got[1][0] == got[0][0] == got[1][1], got
+        assert math.isclose(got[1][0][0], ref, rel_tol=NLL_RTOL)
+    finally:
+        ctx.set_tuning(capi.TUNE_FUSED_FINISH, 1)
