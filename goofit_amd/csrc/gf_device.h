@@ -3,12 +3,17 @@
 // Each function states, in the same operation order, what the reference's
 // device function computes (file:line given per function), so that results
 // round the same way on both sides; the library is built with
-// -ffp-contract=off so no multiply-add is fused behind our back.
+// -ffp-contract=off so no multiply-add is fused behind our back.  The log of
+// the likelihood metrics is this library's own (gf_math.h: within one unit in
+// the last place of the correctly rounded value, half the device library's
+// FP64 instructions); exp, erfc, pow, sqrt and the trigonometric functions are
+// the device library's.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include "gf_internal.h"
+#include "gf_math.h"
 
 namespace gfd {
 
@@ -95,14 +100,14 @@ __device__ __forceinline__ double metric(int m, double rawPdf, double evt0, doub
                                          double totalEvents) {
     switch(m) {
     case 0: return rawPdf;
-    case 1: rawPdf *= norm; return rawPdf > 0.0 ? -log(rawPdf) : 0.0;
+    case 1: rawPdf *= norm; return rawPdf > 0.0 ? -gfmath::log_d(rawPdf) : 0.0;
     case 2: return rawPdf * norm;
     case 3: {
         rawPdf *= norm;
         rawPdf *= evt1;
         if(rawPdf > 0) {
             double expEvents = totalEvents * rawPdf;
-            return (expEvents - evt0 * log(expEvents));
+            return (expEvents - evt0 * gfmath::log_d(expEvents));
         }
         return 0;
     }
