@@ -613,7 +613,8 @@ __global__ __launch_bounds__(GF_BLOCK) void k_reduce_metric(GfProgArgs A, GfInli
     const int64_t tile = (int64_t)GF_BLOCK * EPT;
     const double norm  = Nm[1];
     const bool binned  = metric >= GF_METRIC_BINAVG_CODE;
-    double s = 0.0, c = 0.0;
+    const bool nll     = metric == gfd::metric_nll;
+    double s = nll ? 1.0 : 0.0, c = 0.0;  // the NLL as a product: (mantissa, exponent)
     for(int64_t base = (int64_t)blockIdx.x * tile; base < n; base += (int64_t)gridDim.x * tile) {
         ColSrc<EPT> src;
         src.cols = A.cols;
@@ -630,6 +631,11 @@ __global__ __launch_bounds__(GF_BLOCK) void k_reduce_metric(GfProgArgs A, GfInli
         run<EPT, FEAT>(A, P, Nm, 0, src, row, valid, T);
 #pragma unroll
         for(int j = 0; j < EPT; ++j) {
+            if(nll) {
+                if(valid[j])
+                    gfd::nll_add(s, c, T[j], norm);
+                continue;
+            }
             double e0 = 0.0, e1 = 0.0;
             if(binned) {
                 e0 = A.cols[A.nobs - 2][row[j]];
@@ -640,6 +646,8 @@ __global__ __launch_bounds__(GF_BLOCK) void k_reduce_metric(GfProgArgs A, GfInli
                 gfd::kb_add(s, c, v);
         }
     }
+    if(nll)
+        s = gfd::nll_total(s, c), c = 0.0;
     store_partial(s, c, partials);
 }
 
@@ -711,7 +719,8 @@ __global__ __launch_bounds__(GF_BLOCK) void k_reduce_flat(GfProgArgs A, GfInline
     const int64_t n    = A.nevents;
     const int64_t tile = (int64_t)GF_BLOCK * EPT;
     const double norm  = Nm[1];
-    double s = 0.0, c = 0.0;
+    const bool nll     = metric == gfd::metric_nll;
+    double s = nll ? 1.0 : 0.0, c = 0.0;  // the NLL as a product: (mantissa, exponent)
     FlatSrc<EPT> src;
 #pragma unroll
     for(int j = 0; j < EPT; ++j)
@@ -798,11 +807,18 @@ __global__ __launch_bounds__(GF_BLOCK) void k_reduce_flat(GfProgArgs A, GfInline
         }
 #pragma unroll
         for(int j = 0; j < EPT; ++j) {
+            if(nll) {
+                if(valid[j])
+                    gfd::nll_add(s, c, T[j], norm);
+                continue;
+            }
             const double v = gfd::metric(metric, T[j], 0.0, 0.0, norm, A.total_events);
             if(valid[j])
                 gfd::kb_add(s, c, v);
         }
     }
+    if(nll)
+        s = gfd::nll_total(s, c), c = 0.0;
     if(A.fuse_slots)
         finish_in_kernel(s, c, A);
     else
@@ -899,7 +915,8 @@ __global__ __launch_bounds__(256) void k_dalitz_fast(GfProgArgs A, GfInline I, i
     const double norm     = Nm[1];
     const double *__restrict__ c12 = A.cols[dz.col_m12];
     const double *__restrict__ c13 = A.cols[dz.col_m13];
-    double s = 0.0, c = 0.0;
+    const bool nll        = !WRITE && metric == gfd::metric_nll;
+    double s = nll ? 1.0 : 0.0, c = 0.0;  // the NLL as a product: (mantissa, exponent)
     for(int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
         const int64_t e0 = base + threadIdx.x;
         const bool valid = e0 < n;
@@ -923,15 +940,19 @@ __global__ __launch_bounds__(256) void k_dalitz_fast(GfProgArgs A, GfInline I, i
         // a degree-0 polynomial is its constant coefficient (numBoxes == 1)
         const double eff = CONST_EFF ? P[effi.p + effi.a] : multipoly_at(effi, A, gfd::as_const(P), e);
         const double T   = inside ? (re * re + im * im) * eff : 0.0;
-        const double v   = gfd::metric(metric, T, 0.0, 0.0, norm, A.total_events);
         if(WRITE) {
             if(valid)
-                out[e] = v;
+                out[e] = gfd::metric(metric, T, 0.0, 0.0, norm, A.total_events);
+        } else if(nll) {
+            if(valid)
+                gfd::nll_add(s, c, T, norm);
         } else if(valid) {
-            gfd::kb_add(s, c, v);
+            gfd::kb_add(s, c, gfd::metric(metric, T, 0.0, 0.0, norm, A.total_events));
         }
     }
     if(!WRITE) {
+        if(nll)
+            s = gfd::nll_total(s, c), c = 0.0;
         if(A.fuse_slots)
             finish_in_kernel(s, c, A);
         else
@@ -987,7 +1008,7 @@ __device__ __forceinline__ void stage_sets(MultiSets &S, const double *__restric
 // NR >= nres with zero couplings and waves past nres, whose fused terms add
 // exact zeros).  params / norms: npts consecutive copies of the flattened arrays
 // (np and nn doubles each).  partials: [npts][gridDim.x].
-template <bool CONST_EFF, bool ZERO_OUT, int NR>
+template <bool CONST_EFF, bool ZERO_OUT, int NR, bool NLL>
 __global__ __launch_bounds__(256) void k_dalitz_multi(GfProgArgs A, const double *__restrict__ params, int np,
                                                       const double *__restrict__ norms, int nn, int npts, int metric,
                                                       Partial *partials) {
@@ -1001,10 +1022,13 @@ __global__ __launch_bounds__(256) void k_dalitz_multi(GfProgArgs A, const double
     const double *__restrict__ c12 = A.cols[dz.col_m12];
     const double *__restrict__ c13 = A.cols[dz.col_m13];
     stage_sets<CONST_EFF>(S, params, np, norms, nn, npts, dzi.p, effi.p + effi.a, nres);
-    double s[GF_MULTI_PTS], c[GF_MULTI_PTS];
+    constexpr bool nll = NLL;  // the launcher sets it for metric == NLL
+    double s[GF_MULTI_PTS], c[GF_MULTI_PTS];  // per set: (sum, residual), or the NLL's (mantissa, exponent)
 #pragma unroll
-    for(int p = 0; p < GF_MULTI_PTS; ++p)
-        s[p] = c[p] = 0.0;
+    for(int p = 0; p < GF_MULTI_PTS; ++p) {
+        s[p] = nll ? 1.0 : 0.0;
+        c[p] = 0.0;
+    }
     for(int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
         // keep the LDS reads of the sets inside the loop: hoisted, the 16 x NR
         // loop-invariant couplings would not fit in registers
@@ -1032,9 +1056,12 @@ __global__ __launch_bounds__(256) void k_dalitz_multi(GfProgArgs A, const double
                 const double eff =
                     CONST_EFF ? S.eff[p] : multipoly_at(effi, A, gfd::as_const(params + (int64_t)p * np), e);
                 const double T = inside ? (re * re + im * im) * eff : 0.0;
-                const double v = gfd::metric(metric, T, 0.0, 0.0, S.norm[p], A.total_events);
-                if(valid)
-                    gfd::kb_add(s[p], c[p], v);
+                if(nll) {
+                    if(valid)
+                        gfd::nll_add(s[p], c[p], T, S.norm[p]);
+                } else if(valid) {
+                    gfd::kb_add(s[p], c[p], gfd::metric(metric, T, 0.0, 0.0, S.norm[p], A.total_events));
+                }
             }
         }
     }
@@ -1042,6 +1069,8 @@ __global__ __launch_bounds__(256) void k_dalitz_multi(GfProgArgs A, const double
     for(int p = 0; p < GF_MULTI_PTS; ++p) {
         if(p < npts) {
             double ss = s[p], cc = c[p];
+            if(nll)
+                ss = gfd::nll_total(ss, cc), cc = 0.0;
             block_reduce(ss, cc);
             if(threadIdx.x == 0) {
                 partials[(int64_t)p * gridDim.x + blockIdx.x].s = ss;
@@ -1217,7 +1246,10 @@ __device__ __forceinline__ void batch_rows(const GfProgArgs &A, const GfBatchWor
                                            const double *__restrict__ Nm, int metric, double &s, double &c) {
     const double norm  = Nm[1];
     const bool binned  = metric >= GF_METRIC_BINAVG_CODE;
+    const bool nll     = metric == gfd::metric_nll;
     const int64_t tile = (int64_t)GF_BLOCK * EPT;
+    if(nll)  // the NLL as a product: (mantissa, exponent)
+        s = 1.0, c = 0.0;
     for(int64_t base = w.start; base < w.end; base += tile) {
         ColSrc<EPT> src;
         src.cols = A.cols;
@@ -1236,6 +1268,10 @@ __device__ __forceinline__ void batch_rows(const GfProgArgs &A, const GfBatchWor
         for(int j = 0; j < EPT; ++j) {
             if(!valid[j])
                 continue;
+            if(nll) {
+                gfd::nll_add(s, c, T[j], norm);
+                continue;
+            }
             double e0 = 0.0, e1 = 0.0;
             if(binned) {
                 e0 = A.cols[A.nobs - 2][row[j]];
@@ -1244,6 +1280,8 @@ __device__ __forceinline__ void batch_rows(const GfProgArgs &A, const GfBatchWor
             gfd::kb_add(s, c, gfd::metric(metric, T[j], e0, e1, norm, w.total));
         }
     }
+    if(nll)
+        s = gfd::nll_total(s, c), c = 0.0;
 }
 
 // Metric sums of many toys in one launch: workgroup b evaluates the rows of
@@ -1291,7 +1329,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_points(GfProgArgs A, const G
 // Several points per toy, coherent-sum program (as k_dalitz_multi): workgroup
 // (b, g) reads the waves of each row of work[b] once and adds the row's metric
 // for every point p in [16 g, min(16 g + 16, npts)) of its toy.
-template <bool CONST_EFF, bool ZERO_OUT, int NR>
+template <bool CONST_EFF, bool ZERO_OUT, int NR, bool NLL>
 __global__ __launch_bounds__(GF_BLOCK) void k_batch_dalitz_points(GfProgArgs A, const GfBatchWork *__restrict__ work,
                                                                   const double *__restrict__ stage, int np, int nn,
                                                                   const int *__restrict__ tbl, int metric,
@@ -1314,10 +1352,13 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_dalitz_points(GfProgArgs A, 
     const double *__restrict__ c12 = A.cols[dz.col_m12];
     const double *__restrict__ c13 = A.cols[dz.col_m13];
     stage_sets<CONST_EFF>(S, S0, W, S0 + np, W, cnt, dzi.p, effi.p + effi.a, nres);
-    double s[GF_MULTI_PTS], c[GF_MULTI_PTS];
+    constexpr bool nll = NLL;  // the launcher sets it for metric == NLL
+    double s[GF_MULTI_PTS], c[GF_MULTI_PTS];  // per point: (sum, residual), or the NLL's (mantissa, exponent)
 #pragma unroll
-    for(int p = 0; p < GF_MULTI_PTS; ++p)
-        s[p] = c[p] = 0.0;
+    for(int p = 0; p < GF_MULTI_PTS; ++p) {
+        s[p] = nll ? 1.0 : 0.0;
+        c[p] = 0.0;
+    }
     for(int64_t base = w.start; base < w.end; base += GF_BLOCK) {
         asm volatile("" ::: "memory");  // the set reads stay in the loop (as k_dalitz_multi)
         const int64_t e0 = base + threadIdx.x;
@@ -1342,9 +1383,12 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_dalitz_points(GfProgArgs A, 
                 }
                 const double eff = CONST_EFF ? S.eff[p] : multipoly_at(effi, A, gfd::as_const(S0 + p * W), e);
                 const double T   = inside ? (re * re + im * im) * eff : 0.0;
-                const double v   = gfd::metric(metric, T, 0.0, 0.0, S.norm[p], w.total);
-                if(valid)
-                    gfd::kb_add(s[p], c[p], v);
+                if(nll) {
+                    if(valid)
+                        gfd::nll_add(s[p], c[p], T, S.norm[p]);
+                } else if(valid) {
+                    gfd::kb_add(s[p], c[p], gfd::metric(metric, T, 0.0, 0.0, S.norm[p], w.total));
+                }
             }
         }
     }
@@ -1352,6 +1396,8 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_dalitz_points(GfProgArgs A, 
     for(int p = 0; p < GF_MULTI_PTS; ++p) {
         if(p < cnt) {
             double ss = s[p], cc = c[p];
+            if(nll)
+                ss = gfd::nll_total(ss, cc), cc = 0.0;
             block_reduce(ss, cc);
             if(threadIdx.x == 0) {
                 partials[(int64_t)blockIdx.x * GF_BATCH_PTS + p0 + p].s = ss;
@@ -1836,23 +1882,32 @@ static void launch_fast_t(const GfProgArgs &a, const GfInline *inl, int metric, 
 
 // The multi-set kernels for nres resonances: the resonance loop unrolled to the
 // next multiple of 4 (NR), the variant by efficiency and zero-outside use.
-#define GF_MULTI_SWITCH(KERNEL, NRES, CE, ZO, ...)                                                                  \
+#define GF_MULTI_CASES(KERNEL, NLLB, ...)                                                                           \
+    switch(v_ * 4 + nr_ / 4 - 1) {                                                                                   \
+    case 0: hipLaunchKernelGGL((KERNEL<false, false, 4, NLLB>), __VA_ARGS__); break;                                \
+    case 1: hipLaunchKernelGGL((KERNEL<false, false, 8, NLLB>), __VA_ARGS__); break;                                \
+    case 2: hipLaunchKernelGGL((KERNEL<false, false, 12, NLLB>), __VA_ARGS__); break;                               \
+    case 3: hipLaunchKernelGGL((KERNEL<false, false, 16, NLLB>), __VA_ARGS__); break;                               \
+    case 4: hipLaunchKernelGGL((KERNEL<true, false, 4, NLLB>), __VA_ARGS__); break;                                 \
+    case 5: hipLaunchKernelGGL((KERNEL<true, false, 8, NLLB>), __VA_ARGS__); break;                                 \
+    case 6: hipLaunchKernelGGL((KERNEL<true, false, 12, NLLB>), __VA_ARGS__); break;                                \
+    case 7: hipLaunchKernelGGL((KERNEL<true, false, 16, NLLB>), __VA_ARGS__); break;                                \
+    case 8: hipLaunchKernelGGL((KERNEL<true, true, 4, NLLB>), __VA_ARGS__); break;                                  \
+    case 9: hipLaunchKernelGGL((KERNEL<true, true, 8, NLLB>), __VA_ARGS__); break;                                  \
+    case 10: hipLaunchKernelGGL((KERNEL<true, true, 12, NLLB>), __VA_ARGS__); break;                                \
+    default: hipLaunchKernelGGL((KERNEL<true, true, 16, NLLB>), __VA_ARGS__); break;                                \
+    }
+// The variant for nres resonances (loop unrolled to the next multiple of 4), the
+// efficiency / zero-outside use, and whether the metric is the NLL (its own
+// accumulation, compiled apart so that each variant holds only its own path).
+#define GF_MULTI_SWITCH(KERNEL, NRES, CE, ZO, NLL, ...)                                                             \
     do {                                                                                                             \
         const int nr_ = (NRES) <= 4 ? 4 : (NRES) <= 8 ? 8 : (NRES) <= 12 ? 12 : 16;                                  \
         const int v_  = (CE) ? ((ZO) ? 2 : 1) : 0;                                                                   \
-        switch(v_ * 4 + nr_ / 4 - 1) {                                                                               \
-        case 0: hipLaunchKernelGGL((KERNEL<false, false, 4>), __VA_ARGS__); break;                                  \
-        case 1: hipLaunchKernelGGL((KERNEL<false, false, 8>), __VA_ARGS__); break;                                  \
-        case 2: hipLaunchKernelGGL((KERNEL<false, false, 12>), __VA_ARGS__); break;                                 \
-        case 3: hipLaunchKernelGGL((KERNEL<false, false, 16>), __VA_ARGS__); break;                                 \
-        case 4: hipLaunchKernelGGL((KERNEL<true, false, 4>), __VA_ARGS__); break;                                   \
-        case 5: hipLaunchKernelGGL((KERNEL<true, false, 8>), __VA_ARGS__); break;                                   \
-        case 6: hipLaunchKernelGGL((KERNEL<true, false, 12>), __VA_ARGS__); break;                                  \
-        case 7: hipLaunchKernelGGL((KERNEL<true, false, 16>), __VA_ARGS__); break;                                  \
-        case 8: hipLaunchKernelGGL((KERNEL<true, true, 4>), __VA_ARGS__); break;                                    \
-        case 9: hipLaunchKernelGGL((KERNEL<true, true, 8>), __VA_ARGS__); break;                                    \
-        case 10: hipLaunchKernelGGL((KERNEL<true, true, 12>), __VA_ARGS__); break;                                  \
-        default: hipLaunchKernelGGL((KERNEL<true, true, 16>), __VA_ARGS__); break;                                  \
+        if(NLL) {                                                                                                    \
+            GF_MULTI_CASES(KERNEL, true, __VA_ARGS__)                                                                \
+        } else {                                                                                                     \
+            GF_MULTI_CASES(KERNEL, false, __VA_ARGS__)                                                               \
         }                                                                                                            \
     } while(0)
 
@@ -1862,7 +1917,7 @@ int launch_dalitz_multi(const GfProgArgs &a, int nres, const double *params, int
     hipStream_t s = (hipStream_t)stream;
     if(npts < 1 || npts > GF_MULTI_PTS || nres < 1 || nres > GF_MAX_RES)
         return -1;
-    GF_MULTI_SWITCH(k_dalitz_multi, nres, const_eff, zero_outside && const_eff, dim3(nblocks), dim3(256), 0, s, a,
+    GF_MULTI_SWITCH(k_dalitz_multi, nres, const_eff, zero_outside && const_eff, metric == gfd::metric_nll, dim3(nblocks), dim3(256), 0, s, a,
                     params, np, norms, nn, npts, metric, partials);
     int rc = check(hipGetLastError());
     if(rc)
@@ -2001,7 +2056,7 @@ int launch_batch_points(const GfProgArgs &a, int nres, const GfBatchWork *work, 
         hipLaunchKernelGGL((k_batch_points<2, 0>), grid, dim3(GF_BLOCK), 0, s, a, work, stage, np, nn, tbl, metric,
                            partials);
     else
-        GF_MULTI_SWITCH(k_batch_dalitz_points, nres, const_eff, zero_outside && const_eff, grid, dim3(GF_BLOCK), 0, s,
+        GF_MULTI_SWITCH(k_batch_dalitz_points, nres, const_eff, zero_outside && const_eff, metric == gfd::metric_nll, grid, dim3(GF_BLOCK), 0, s,
                         a, work, stage, np, nn, tbl, metric, partials);
     int rc = check(hipGetLastError());
     if(rc)
