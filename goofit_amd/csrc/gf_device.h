@@ -126,32 +126,15 @@ __device__ __forceinline__ double metric(int m, double rawPdf, double evt0, doub
     }
 }
 
-// The NLL metric over a lane's events as one logarithm: sum_e -log(v_e) over the
-// events with v_e = rawPdf_e * norm > 0 (the others add 0, as in metric()) equals
-// -log(prod_e v_e).  The product is kept as a mantissa m and a binary exponent e
-// (an integer, held in a double: exact below 2^53): each factor's own mantissa
-// (frexp, exact) multiplies m (one rounding), the result is renormalised into
-// [0.5, 1) (frexp, exact) and the exponents are added, so the product neither
-// overflows nor underflows, and each event adds one rounding of relative size 2^-53
-// to it -- an absolute error in the sum of logarithms below that of taking each
-// event's logarithm.  A factor v <= 0 or NaN becomes 1.0 = 0.5 * 2^1, whose halving
-// is exact and undone by the renormalisation; v = +inf makes the product +inf (the
-// sum -inf), as the per-event form gives.  A lane starts from (m, e) = (1, 0).  Every
-// kernel that reduces the NLL metric accumulates it this way, in its own event order
-// per lane, in the registers of its (sum, residual) pair, so kernels that assign
-// events to lanes alike give the same bits (the per-event values of
-// evaluate_with_metric still use metric()).
+// The NLL metric over a lane's events as one logarithm (gfmath::nll_add /
+// nll_total): every kernel that reduces the NLL metric accumulates it this way, in
+// its own event order per lane, in the registers of its (sum, residual) pair
+// starting from (1, 0), so kernels that assign events to lanes alike give the same
+// bits (the per-event values of evaluate_with_metric still use metric()).
 __device__ __forceinline__ void nll_add(double &m, double &e, double rawPdf, double norm) {
-    const double v = rawPdf * norm;
-    int ev, em;
-    const double a = frexp(v > 0.0 ? v : 1.0, &ev);
-    m              = frexp(m * a, &em);
-    e += (double)(ev + em);
+    gfmath::nll_add(m, e, rawPdf * norm);
 }
-// the lane's sum of -log(v_e)
-__device__ __forceinline__ double nll_total(double m, double e) {
-    return -fma(e, gfmath::ln2_hi, fma(e, gfmath::ln2_lo, gfmath::log_d(m)));
-}
+__device__ __forceinline__ double nll_total(double m, double e) { return gfmath::nll_total(m, e); }
 constexpr int metric_nll = 1;
 
 // --------------------------------------------------------- Dalitz kinematics

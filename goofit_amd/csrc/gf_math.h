@@ -1,4 +1,5 @@
-// gf_math.h -- FP64 log for the likelihood metrics.
+// gf_math.h -- FP64 log for the likelihood metrics, and the NLL of many events as
+// one logarithm.
 //
 // Every event of every likelihood sweep takes one logarithm per parameter point
 // (-log of the normalised PDF value), and the multi-point sweeps -- up to 16
@@ -71,5 +72,24 @@ GF_MATH_HD inline double log_d(double x) {
     y                 = x == HUGE_VAL ? HUGE_VAL : y;
     return y;
 }
+
+// The sum of -log(v) over many values as one logarithm: sum -log(v_k) over the v_k > 0
+// (the others add 0, as the NLL metric gives them) equals -log(prod v_k).  The product
+// is kept as a mantissa m and a binary exponent e (an integer, held in a double: exact
+// below 2^53): each factor's own mantissa (frexp, exact) multiplies m (one rounding),
+// the result is renormalised into [0.5, 1) (frexp, exact) and the exponents are added,
+// so the product neither overflows nor underflows, and each factor adds one rounding of
+// relative size 2^-53 to it -- an absolute error in the sum of logarithms below that of
+// taking each logarithm apart.  A factor v <= 0 or NaN becomes 1.0 = 0.5 * 2^1, whose
+// halving is exact and undone by the renormalisation; v = +inf makes the product +inf
+// (the sum -inf), as the per-value form gives.  Start from (m, e) = (1, 0).
+GF_MATH_HD inline void nll_add(double &m, double &e, double v) {
+    int ev, em;
+    const double a = frexp(v > 0.0 ? v : 1.0, &ev);
+    m              = frexp(m * a, &em);
+    e += (double)(ev + em);
+}
+// sum of -log(v_k) = -(log m + e ln 2)
+GF_MATH_HD inline double nll_total(double m, double e) { return -fma(e, ln2_hi, fma(e, ln2_lo, log_d(m))); }
 
 }  // namespace gfmath

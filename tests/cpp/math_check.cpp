@@ -73,5 +73,52 @@ int main(int argc, char **argv) {
             pow2_bad++;
     }
     std::printf("log_powers_of_two not_nearest=%lld\n", pow2_bad);
+
+    // the sum of -log(v) as one logarithm of a frexp-normalised product, against the
+    // long double sum of -logl(v) (Kahan-compensated), over values spread across every
+    // binary exponent with subnormals, zeros, negatives and NaNs mixed in (which add 0)
+    for(int trial = 0; trial < 3; trial++) {
+        const long long n = trial == 0 ? N : (trial == 1 ? 4 : 1000);
+        double m = 1.0, e = 0.0;
+        long double ref = 0.0L, comp = 0.0L, scale = 0.0L;
+        for(long long i = 0; i < n; i++) {
+            const double r = u(rng);
+            double v;
+            if(trial == 1)
+                v = std::ldexp(0.5 + 0.5 * u(rng), -10 + (int)(u(rng) * 17.0));  // likelihood-like values
+            else if(r < 0.05)
+                v = 0.0;
+            else if(r < 0.07)
+                v = -u(rng);
+            else if(r < 0.08)
+                v = NAN;
+            else
+                v = std::ldexp(u(rng), 1023 - (int)(u(rng) * 2098.0));  // 2^1023 down to subnormals
+            gfmath::nll_add(m, e, v);
+            if(v > 0.0) {
+                const long double t = -logl((long double)v) - comp;
+                const long double s = ref + t;
+                comp                = (s - ref) - t;
+                ref                 = s;
+                scale += fabsl(logl((long double)v));
+            }
+        }
+        const double got = gfmath::nll_total(m, e);
+        std::printf("nll_product trial=%d n=%lld rel_err=%.3e ulps_of_scale=%.3f\n", trial, n,
+                    (double)(fabsl((long double)got - ref) / fabsl(ref)),
+                    (double)(fabsl((long double)got - ref) / (scale * 0x1p-53L)));
+    }
+    {
+        double m = 1.0, e = 0.0;
+        const double got_empty = gfmath::nll_total(m, e);
+        gfmath::nll_add(m, e, 0.25);
+        gfmath::nll_add(m, e, HUGE_VAL);
+        gfmath::nll_add(m, e, 3.0);
+        std::printf("nll_special empty=%.17g with_inf=%.17g\n", got_empty, gfmath::nll_total(m, e));
+        double m2 = 1.0, e2 = 0.0;
+        gfmath::nll_add(m2, e2, 0.125);
+        gfmath::nll_add(m2, e2, 8.0);
+        std::printf("nll_special exact_pair=%.17g\n", gfmath::nll_total(m2, e2));
+    }
     return 0;
 }

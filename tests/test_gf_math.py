@@ -56,3 +56,21 @@ def test_special_values(report):
 def test_log_of_powers_of_two(report):
     m = re.search(r"log_powers_of_two not_nearest=(\d+)", report)
     assert m and int(m.group(1)) == 0
+
+
+def test_nll_as_one_logarithm(report):
+    """gfmath::nll_add / nll_total, the per-lane NLL of every likelihood kernel: the sum of
+    -log(v) over 2e6 values across every binary exponent (subnormals included, with zeros,
+    negatives and NaNs that add 0), over 4 likelihood-like values and over 1000 values,
+    against the compensated long double sum of -logl(v): within 1e-16 relative, a small
+    fraction of one rounding of the summed magnitudes (a dropped exponent, a wrong ln 2 part
+    or a factor not renormalised is off by many orders)."""
+    rows = re.findall(r"nll_product trial=(\d) n=(\d+) rel_err=(\S+) ulps_of_scale=(\S+)", report)
+    assert len(rows) == 3
+    for trial, n, rel, ulps in rows:
+        assert float(rel) < 1e-16, (trial, n, rel)
+        assert float(ulps) < 1.0, (trial, n, ulps)
+    m = re.search(r"nll_special empty=(\S+) with_inf=(\S+)", report)
+    assert float(m.group(1)) == 0.0 and float(m.group(2)) == -math.inf
+    m = re.search(r"nll_special exact_pair=(\S+)", report)
+    assert float(m.group(1)) == 0.0  # 0.125 * 8 = 1 exactly

@@ -288,3 +288,34 @@ def test_in_kernel_finish_flat_sizes(ctx, n):
         assert math.isclose(got[1][0][0], ref, rel_tol=NLL_RTOL)
     finally:
         ctx.set_tuning(capi.TUNE_FUSED_FINISH, 1)
+
+
+def test_nll_with_subnormal_and_zero_pdf_values(ctx):
+    """The NLL as one logarithm per lane over PDF values that reach the subnormal range and
+    underflow to 0 (a Gaussian of width 0.05 over |x| < 2.2: exp(-0.5 (x/0.05)^2) passes
+    1e-308 near |x| = 1.88 and is 0 beyond 1.93, so those events add 0, as the reference's
+    metric gives them): the flat kernel, the interpreter, and both finishes give the same bits,
+    within 1e-11 of the oracle's per-event -log sum."""
+    n = 200_000
+    top3, _ = prod3d(sigma=0.05)
+    rng = np.random.default_rng(44)
+    xs = rng.uniform(-2.2, 2.2, n)
+    _, ys, zs = W.sample_prod3d(n, seed=45)
+    _install(ctx, top3, [xs, ys, zs])
+    M3 = O.Model(top3, np.stack([xs, ys, zs], axis=1))
+    ref = M3.calculate_nll()
+    _sync(ctx, M3)
+    g = np.exp(-0.5 * (xs / 0.05) ** 2)
+    assert np.any((g > 0) & (g < 2.3e-308)) and np.any(g == 0)
+    got = {}
+    try:
+        for flat in (1, 0):
+            ctx.set_tuning(capi.TUNE_FLAT, flat)
+            for fused in (1, 0):
+                ctx.set_tuning(capi.TUNE_FUSED_FINISH, fused)
+                got[flat, fused] = ctx.reduce_metric("NLL")
+    finally:
+        ctx.set_tuning(capi.TUNE_FLAT, 1)
+        ctx.set_tuning(capi.TUNE_FUSED_FINISH, 1)
+    assert len(set(got.values())) == 1, got
+    assert math.isclose(got[1, 1][0], ref, rel_tol=NLL_RTOL), (got[1, 1], ref)
