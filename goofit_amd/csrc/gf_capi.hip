@@ -156,7 +156,7 @@ struct gf_ctx {
     uint32_t noalloc_seq = 0;  // of the last gf_reduce_metric_noalloc
 
     // launch-shape knobs (gf_tune)
-    int tune[GF_TUNE_COUNT] = {1, 1, 4, 4, 2000, 1, 1, 1};
+    int tune[GF_TUNE_COUNT] = {1, 1, 4, 3, 2000, 1, 1, 1, 1};
 
     uint64_t events_gen = 1;  // bumped whenever the event columns may change
 
@@ -815,6 +815,8 @@ GfProgArgs prog_args(gf_ctx *ctx) {
     a.sweep_only   = 0;
     a.fuse_slots   = nullptr;
     a.fuse_result  = nullptr;
+    a.coh_mx_nres  = 0;
+    a.coh_group    = 0;
     return a;
 }
 
@@ -974,6 +976,17 @@ gf_status efficiency_mode(gf_ctx *ctx, int eff_func, int &pc, double &value) {
     return subprogram(ctx, eff_func, pc);
 }
 
+// The multi-point NLL of a lone Amp3Body on the matrix cores (k_coh_mx): the knob on, the
+// result layout of v_mfma_f64_16x16x4f64 the one the kernel assumes (probed at context
+// creation), and the NLL of a constant efficiency with the waves settling inDalitz.
+// Returns the resonance count for GfProgArgs::coh_mx_nres, or 0.
+int coh_mx_nres(const gf_ctx *ctx, int metric) {
+    if(!ctx->tune[GF_TUNE_COH_MFMA] || ctx->mfma_layout != 2 || metric != GF_METRIC_NLL || ctx->dalitz.empty()
+       || !const_efficiency(ctx) || !zero_outside(ctx))
+        return 0;
+    return ctx->dalitz[0].nres;
+}
+
 int fast_blocks(const gf_ctx *ctx, int64_t n) {
     int64_t nb  = (n + 255) / 256;
     int64_t cap = (int64_t)ctx->cu_count * ctx->tune[GF_TUNE_FAST_BLOCKS_PER_CU];
@@ -996,6 +1009,8 @@ gf_status launch_reduce(gf_ctx *ctx, int metric, hipEvent_t mark = nullptr, bool
     a.mark       = mark;
     a.sweep_only = sweep_only ? 1 : 0;
     const bool fast = use_fast_dalitz(ctx, metric);
+    if(fast)
+        a.coh_group = coh_mx_nres(ctx, metric) > 0;
     if(ctx->tune[GF_TUNE_FUSED_FINISH] && (fast || use_flat(ctx, metric))) {
         a.fuse_slots  = ctx->d_fuse_slots;
         a.fuse_result = ctx->d_res_mapped;
@@ -1173,6 +1188,7 @@ gf_status gf_ctx_set_tuning(gf_ctx *ctx, int key, int value) {
     case GF_TUNE_FAST_DALITZ:
     case GF_TUNE_ZERO_OUTSIDE:
     case GF_TUNE_FUSED_FINISH:
+    case GF_TUNE_COH_MFMA:
         if(value != 0 && value != 1)
             return fail(GF_ERR_INVALID, "tuning key %d takes 0 or 1 (got %d)", key, value);
         break;
@@ -1267,6 +1283,12 @@ gf_status gf_ctx_create(gf_ctx **out, int device) {
     TRY(hipEventCreate(&c->ev0));
     TRY(hipEventCreate(&c->ev1));
 #undef TRY
+    // the result layout of v_mfma_f64_16x16x4f64, probed once here so that the choice of
+    // the likelihood kernels (k_coh_mx needs layout 2) does not depend on call history
+    if((st = ensure_layout(c))) {
+        gf_ctx_destroy(c);
+        return st;
+    }
     *out = c;
     return GF_OK;
 }
@@ -1581,6 +1603,8 @@ gf_status gf_reduce_metric_noalloc(gf_ctx *ctx, int metric, gf_result *dev_out, 
                     GF_NOALLOC_PARTIALS);
     GfProgArgs a = prog_args(ctx);
     a.seq        = ++ctx->noalloc_seq;
+    if(fast)
+        a.coh_group = coh_mx_nres(ctx, metric) > 0;
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     GfResult *res = reinterpret_cast<GfResult *>(dev_out);
     int rc;
@@ -1667,6 +1691,7 @@ gf_status gf_reduce_metric_multi(gf_ctx *ctx, int metric, int npts, const double
     const bool ce = const_efficiency(ctx);
     GfProgArgs a  = prog_args(ctx);
     const int nb  = fast_blocks(ctx, ctx->n);
+    a.coh_mx_nres = coh_mx_nres(ctx, metric);
     for(int k0 = 0; k0 < npts; k0 += GF_MULTI_PTS) {
         const int cnt = std::min(GF_MULTI_PTS, npts - k0);
         int rc = gfk::launch_dalitz_multi(a, ctx->dalitz[0].nres, ctx->d_multi + (size_t)k0 * np, np,

@@ -128,6 +128,10 @@ struct GfProgArgs {
     struct GfResult *fuse_result;
     void *mark;                  // host only: hipEvent_t recorded between a sweep and its k_finish
     int32_t sweep_only;          // host only: launch the sweep without its k_finish (timing)
+    int32_t coh_mx_nres;         // host only: > 0 runs a lone Amp3Body's multi-point NLL (constant efficiency,
+                                 // waves 0 outside) on the matrix cores, k_coh_mx, for this many resonances
+    int32_t coh_group;           // 1: k_dalitz_fast groups the NLL's events as k_coh_mx does (set when the
+                                 // multi-point sums of the context run there)
 };
 
 #define GF_METRIC_BINAVG_CODE 3  // metrics from here on read (bin entry, bin volume/error) columns

@@ -446,6 +446,9 @@ __device__ __forceinline__ void run(const GfProgArgs &A, const double *__restric
     }
 }
 
+// the four result registers of a lane of v_mfma_f64_16x16x4f64
+typedef double gf_double4 __attribute__((ext_vector_type(4)));
+
 // ------------------------------------------------------------- reductions
 
 // Reduce (s, c) over the block; the result is valid in thread 0.
@@ -470,6 +473,17 @@ __device__ __forceinline__ void block_reduce(double &s, double &c) {
             gfd::kb_merge(s, c, ls[w], lc[w]);
     }
     __syncthreads();
+}
+
+// Reduce (s, c) over the 16 lanes of a lane's group (lanes 16 g .. 16 g + 15 of its
+// wave): the result is valid in every lane of the group.
+__device__ __forceinline__ void group_reduce(double &s, double &c) {
+#pragma unroll
+    for(int off = 8; off > 0; off >>= 1) {
+        double s2 = __shfl_xor(s, off, 64);
+        double c2 = __shfl_xor(c, off, 64);
+        gfd::kb_merge(s, c, s2, c2);
+    }
 }
 
 // Reduce the block's (sum, residual) pair and store it as partial blockIdx.x.
@@ -520,11 +534,12 @@ __global__ __launch_bounds__(GF_BLOCK) void k_finish(const Partial *partials, in
 // release fence: on this device an agent-scope release writes back the XCD's L2,
 // once per workgroup.  The wait is bounded; a slot still empty after it sets
 // GF_DEVERR_FINISH in the result's error word.
-__device__ __forceinline__ void finish_in_kernel(double s, double c, const GfProgArgs &A) {
+// finish_reduced: the same for a workgroup whose pair is reduced already (valid in
+// thread 0); every thread of the workgroup calls it.
+__device__ __forceinline__ void finish_reduced(double s, double c, const GfProgArgs &A) {
     constexpr uint32_t spin_limit = 1u << 18;
     constexpr int group           = 4;
     __shared__ double own_s, own_c;
-    block_reduce(s, c);
     unsigned long long *slots = A.fuse_slots;
     const int nparts          = (int)gridDim.x;
     const int mine            = nparts - 1;
@@ -596,6 +611,11 @@ __device__ __forceinline__ void finish_in_kernel(double s, double c, const GfPro
         __threadfence_system();
         __hip_atomic_store(&result->seq, A.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
+}
+
+__device__ __forceinline__ void finish_in_kernel(double s, double c, const GfProgArgs &A) {
+    block_reduce(s, c);
+    finish_reduced(s, c, A);
 }
 
 __global__ __launch_bounds__(256) void k_fuse_init(unsigned long long *slots) {
@@ -916,6 +936,9 @@ __global__ __launch_bounds__(256) void k_dalitz_fast(GfProgArgs A, GfInline I, i
     const double *__restrict__ c12 = A.cols[dz.col_m12];
     const double *__restrict__ c13 = A.cols[dz.col_m13];
     const bool nll        = !WRITE && metric == gfd::metric_nll;
+    // the events of the NLL grouped as k_coh_mx groups a set's (the multi-point sums of
+    // these parameters run there): lanes 0-15 of a wave hold the products, below
+    const bool grp        = nll && CONST_EFF && ZERO_OUT && A.coh_group;
     double s = nll ? 1.0 : 0.0, c = 0.0;  // the NLL as a product: (mantissa, exponent)
     for(int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
         const int64_t e0 = base + threadIdx.x;
@@ -943,6 +966,15 @@ __global__ __launch_bounds__(256) void k_dalitz_fast(GfProgArgs A, GfInline I, i
         if(WRITE) {
             if(valid)
                 out[e] = gfd::metric(metric, T, 0.0, 0.0, norm, A.total_events);
+        } else if(grp) {
+            // k_coh_mx's grouping: lane (0, j) takes events 16 t + j of the wave, in tile order
+            const int64_t w0 = base + (threadIdx.x & ~63);
+#pragma unroll
+            for(int t = 0; t < 4; ++t) {
+                const double v = t == 0 ? T : __shfl(T, 16 * t + (threadIdx.x & 15), 64);
+                if((threadIdx.x & 63) < 16 && w0 + 16 * t + (threadIdx.x & 15) < n)
+                    gfd::nll_add(s, c, v, norm);
+            }
         } else if(nll) {
             if(valid)
                 gfd::nll_add(s, c, T, norm);
@@ -950,7 +982,29 @@ __global__ __launch_bounds__(256) void k_dalitz_fast(GfProgArgs A, GfInline I, i
             gfd::kb_add(s, c, gfd::metric(metric, T, 0.0, 0.0, norm, A.total_events));
         }
     }
-    if(!WRITE) {
+    if(!WRITE && grp) {
+        // as k_coh_mx reduces a set: the 16 lanes of the group, then the four waves in order
+        __shared__ double gls[4], glc[4];
+        double sv = (threadIdx.x & 63) < 16 ? gfd::nll_total(s, c) : 0.0, cv = 0.0;
+        group_reduce(sv, cv);
+        if((threadIdx.x & 63) == 0) {
+            gls[threadIdx.x >> 6] = sv;
+            glc[threadIdx.x >> 6] = cv;
+        }
+        __syncthreads();
+        if(threadIdx.x == 0) {
+            sv = gls[0];
+            cv = glc[0];
+            for(int wv = 1; wv < 4; ++wv)
+                gfd::kb_merge(sv, cv, gls[wv], glc[wv]);
+        }
+        if(A.fuse_slots) {
+            finish_reduced(sv, cv, A);
+        } else if(threadIdx.x == 0) {
+            partials[blockIdx.x].s = sv;
+            partials[blockIdx.x].c = cv;
+        }
+    } else if(!WRITE) {
         if(nll)
             s = gfd::nll_total(s, c), c = 0.0;
         if(A.fuse_slots)
@@ -1077,6 +1131,110 @@ __global__ __launch_bounds__(256) void k_dalitz_multi(GfProgArgs A, const double
                 partials[(int64_t)p * gridDim.x + blockIdx.x].c = cc;
             }
         }
+    }
+}
+
+// ------------------------------------------------------- coherent sums on MFMA
+//
+// The NLL of up to 16 coupling sets over the same events, constant efficiency, waves 0
+// outside the Dalitz region (the k_dalitz_multi<true, true, NR, true> case), with the
+// coherent sums of a wave's 64 events formed on the FP64 matrix cores.  Per tile of 16
+// events, Re_p and Im_p are rows of real matrix products M W over K = 2 NR terms, on
+// v_mfma_f64_16x16x4f64 (NR / 2 steps of K = 4).  The instruction adds its K terms to
+// the accumulator one after the other, each as a fused multiply-add, so with the terms of
+// resonance i ordered as the vector path's gfd::cmac orders them -- Re: -ci wi, then
+// cr wr; Im: ci wr, then cr wi -- every element equals cmac's result, bit for bit.
+// Operand layouts (gfx950; the host uses this kernel only after its probe of the
+// instruction found the result layout assumed here): lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16] of a K-step and holds D[4 r + l / 16][l % 16] in result register
+// r.  The couplings are loop-invariant A operands (16 doubles a lane, read from LDS once);
+// each lane loads both parts of one resonance of one event per K-step straight from the
+// wave cache (lanes 0-31 read resonance 2 kk of 16 consecutive events, 256 contiguous
+// bytes, lanes 32-63 the next resonance).  Lane (g = l / 16, j = l % 16) ends tile t
+// holding |A|^2 eff of sets 4 r + g of event 16 t + j, and multiplies it into those
+// sets' NLL products: per set, lane (g, j) takes events 16 t + j of each of the wave's
+// tiles in order.  The sets are then reduced over the 16 lanes of their group (xor 8, 4,
+// 2, 1) and the four waves in order, partials [p][b].  k_dalitz_fast groups the events
+// of a set the same way whenever this kernel serves the multi-point sums (GfProgArgs::
+// coh_group), so a multi-point sum equals the single sweep of the same parameters.
+template <int NR>
+__global__ __launch_bounds__(256) void k_coh_mx(GfProgArgs A, const double *__restrict__ params, int np,
+                                                const double *__restrict__ norms, int nn, int npts,
+                                                Partial *partials) {
+    static_assert(NR % 4 == 0 && NR <= GF_MAX_RES, "NR: 4, 8, 12 or 16");
+    constexpr int KS = NR / 2;  // K-steps of 4 terms (two resonances)
+    __shared__ MultiSets S;
+    __shared__ double ls[4][GF_MULTI_PTS], lc[4][GF_MULTI_PTS];
+    const GfIns dzi       = A.ins[0];
+    const GfIns effi      = A.ins[1];
+    const GfDalitzSlot dz = gfd::as_const(A.dz)[dzi.x];
+    const int nres        = dzi.a;
+    const int64_t n       = A.nevents;
+    stage_sets<true>(S, params, np, norms, nn, npts, dzi.p, effi.p + effi.a, nres);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, j = lane & 15;
+    // A[j][k] with k = 4 kk + g: resonance k / 2; the even term of a resonance comes first
+    double are[KS], aim[KS];
+#pragma unroll
+    for(int kk = 0; kk < KS; ++kk) {
+        const int k     = 4 * kk + g;
+        const double2 c = S.c[j][k >> 1];
+        are[kk]         = (k & 1) ? c.x : -c.y;  // Re: -ci (with wi), then cr (with wr)
+        aim[kk]         = (k & 1) ? c.x : c.y;   // Im: ci (with wr), then cr (with wi)
+    }
+    double eff[4], nrm[4], m[4], ex[4];
+#pragma unroll
+    for(int r = 0; r < 4; ++r) {
+        eff[r] = S.eff[4 * r + g];
+        nrm[r] = S.norm[4 * r + g];
+        m[r]   = 1.0;
+        ex[r]  = 0.0;
+    }
+    const double2 *__restrict__ wv = dz.wave;
+    for(int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e0 = base + 64 * wave;
+#pragma unroll
+        for(int t = 0; t < 4; ++t) {
+            const int64_t ev  = e0 + 16 * t + j;
+            const int64_t row = ev < n ? ev : n - 1;  // loads stay in bounds; the event is masked below
+            double bre[KS], bim[KS];
+#pragma unroll
+            for(int kk = 0; kk < KS; ++kk) {
+                const int k    = 4 * kk + g, i = k >> 1;
+                const double2 w = i < nres ? wv[(int64_t)i * dz.ncache + row] : make_double2(0.0, 0.0);
+                bre[kk]        = (k & 1) ? w.x : w.y;
+                bim[kk]        = (k & 1) ? w.y : w.x;
+            }
+            gf_double4 re = {0, 0, 0, 0}, im = {0, 0, 0, 0};
+#pragma unroll
+            for(int kk = 0; kk < KS; ++kk) {
+                re = __builtin_amdgcn_mfma_f64_16x16x4f64(are[kk], bre[kk], re, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[kk], bim[kk], im, 0, 0, 0);
+            }
+            if(ev < n) {
+#pragma unroll
+                for(int r = 0; r < 4; ++r)
+                    if(4 * r + g < npts)
+                        gfd::nll_add(m[r], ex[r], (re[r] * re[r] + im[r] * im[r]) * eff[r], nrm[r]);
+            }
+        }
+    }
+#pragma unroll
+    for(int r = 0; r < 4; ++r) {
+        double sv = gfd::nll_total(m[r], ex[r]), cv = 0.0;
+        group_reduce(sv, cv);
+        if(j == 0) {
+            ls[wave][4 * r + g] = sv;
+            lc[wave][4 * r + g] = cv;
+        }
+    }
+    __syncthreads();
+    if((int)threadIdx.x < npts) {
+        const int p = threadIdx.x;
+        double sv   = ls[0][p], cv = lc[0][p];
+        for(int w = 1; w < 4; ++w)
+            gfd::kb_merge(sv, cv, ls[w][p], lc[w][p]);
+        partials[(int64_t)p * gridDim.x + blockIdx.x].s = sv;
+        partials[(int64_t)p * gridDim.x + blockIdx.x].c = cv;
     }
 }
 
@@ -1616,7 +1774,6 @@ __global__ __launch_bounds__(256) void k_compact(const double *__restrict__ c12,
 
 // ------------------------------------------------------------- FP64-MFMA Gram
 
-typedef double gf_double4 __attribute__((ext_vector_type(4)));
 
 #define GG_BLOCK 128  // two waves
 #define GG_LD 65      // padded LDS row (doubles): 16 lanes reading one column hit distinct banks
@@ -1911,12 +2068,31 @@ static void launch_fast_t(const GfProgArgs &a, const GfInline *inl, int metric, 
         }                                                                                                            \
     } while(0)
 
+// k_coh_mx for nres resonances (NR = nres rounded up to 4).
+#define GF_COH_MX(NRES, ...)                                                                                         \
+    do {                                                                                                             \
+        switch((NRES) <= 4 ? 4 : (NRES) <= 8 ? 8 : (NRES) <= 12 ? 12 : 16) {                                         \
+        case 4: hipLaunchKernelGGL((k_coh_mx<4>), __VA_ARGS__); break;                                              \
+        case 8: hipLaunchKernelGGL((k_coh_mx<8>), __VA_ARGS__); break;                                              \
+        case 12: hipLaunchKernelGGL((k_coh_mx<12>), __VA_ARGS__); break;                                            \
+        default: hipLaunchKernelGGL((k_coh_mx<16>), __VA_ARGS__); break;                                            \
+        }                                                                                                            \
+    } while(0)
+
 int launch_dalitz_multi(const GfProgArgs &a, int nres, const double *params, int np, const double *norms, int nn,
                         int npts, bool const_eff, bool zero_outside, int metric, Partial *partials, double *out,
                         int nblocks, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     if(npts < 1 || npts > GF_MULTI_PTS || nres < 1 || nres > GF_MAX_RES)
         return -1;
+    if(a.coh_mx_nres > 0 && const_eff && zero_outside && metric == gfd::metric_nll) {
+        GF_COH_MX(nres, dim3(nblocks), dim3(256), 0, s, a, params, np, norms, nn, npts, partials);
+        int rc = check(hipGetLastError());
+        if(rc)
+            return rc;
+        hipLaunchKernelGGL(k_finish_multi, dim3(npts), dim3(GF_BLOCK), 0, s, partials, nblocks, out);
+        return check(hipGetLastError());
+    }
     GF_MULTI_SWITCH(k_dalitz_multi, nres, const_eff, zero_outside && const_eff, metric == gfd::metric_nll, dim3(nblocks), dim3(256), 0, s, a,
                     params, np, norms, nn, npts, metric, partials);
     int rc = check(hipGetLastError());

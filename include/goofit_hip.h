@@ -104,7 +104,7 @@ enum gf_tune {
                                        arguments (no upload copy); 0: device buffers (default 1) */
     GF_TUNE_FAST_DALITZ        = 1, /* 1: dedicated coherent-sum kernel for a lone Amp3Body (default 1) */
     GF_TUNE_BLOCKS_PER_CU      = 2, /* interpreter reduction grid cap, workgroups per CU (1..64, default 4) */
-    GF_TUNE_FAST_BLOCKS_PER_CU = 3, /* coherent-sum kernel grid cap, workgroups per CU (1..64, default 4) */
+    GF_TUNE_FAST_BLOCKS_PER_CU = 3, /* coherent-sum kernel grid cap, workgroups per CU (1..64, default 3) */
     GF_TUNE_SPIN_US            = 4, /* a reduction's caller polls the result for up to this many
                                        microseconds before blocking on the stream (0..1e6, default 2000) */
     GF_TUNE_ZERO_OUTSIDE       = 5, /* 1: with a constant efficiency and a wave cache filled from the
@@ -120,7 +120,11 @@ enum gf_tune {
                                        partials inside the sweep (the last workgroup to finish does it,
                                        in k_finish's order: the same bits) instead of in a second launch;
                                        0: a second launch (default 1) */
-    GF_TUNE_COUNT              = 8
+    GF_TUNE_COH_MFMA           = 8, /* 1: the multi-point NLL of a lone Amp3Body with a constant
+                                       efficiency (gf_reduce_metric_multi) forms its coherent sums on the
+                                       FP64 matrix cores -- the same bits as on the vector ALUs; 0: on the
+                                       vector ALUs (default 1) */
+    GF_TUNE_COUNT              = 9
 };
 
 /* Set / read one knob (gf_tune); values out of range give GF_ERR_INVALID. */

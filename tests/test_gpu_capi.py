@@ -462,3 +462,53 @@ def test_dalitz_generate(ctx):
     ndf = inside.sum() - 1
     assert ndf > 50
     assert chi2 / ndf < 1.35, (chi2, ndf)
+
+
+@pytest.mark.parametrize("nres,n", [(16, 100_000), (12, 40_001), (8, 70_001), (5, 3_000), (3, 257)])
+def test_multi_point_sums_on_the_matrix_cores_equal_single_sweeps(ctx, nres, n):
+    """gf_reduce_metric_multi forms the coherent sums of its parameter sets on the FP64
+    matrix cores (k_coh_mx) when the NLL of a constant efficiency allows it, and the single
+    sweep then groups each lane's events the same way; v_mfma_f64_16x16x4f64, fed its K
+    terms in the order of the vector path's fused complex multiply-adds, gives the same
+    elements bit for bit.  So with the matrix-core path on, and with it off (vector
+    kernels, their own grouping), every multi-point sum equals the single reduction of the
+    same parameters exactly; the two paths agree with each other to rounding (1e-13) and
+    with the oracle (1e-11).  16 and 5 perturbed coupling sets."""
+    table = W.DALITZ_RESONANCES[:nres]
+    top, _ = dalitz(nbins=64, resonances=table)
+    aos = dalitz_events(n, 40 + nres)
+    M = O.Model(top, aos, decay=W.DALITZ_DECAY)
+    ref = M.calculate_nll()
+    _install(ctx, top, decay=W.DALITZ_DECAY)
+    ctx.upload_columns([aos[:, 0], aos[:, 1], aos[:, 2]])
+    ctx.set_identity_column(2)
+    ctx.dalitz_set_data_size(top.function_idx, n)
+    M.op.sync_params()
+    ctx.update_params(M.op.p)
+    ctx.dalitz_cache(top.function_idx)
+    _sync(ctx, M)
+    p0, n0 = M.op.p.copy(), M.op.n.copy()
+    rng = np.random.default_rng(nres)
+    c0 = top.parameters_idx + 1
+    try:
+        for npts in (16, 5):
+            sets = []
+            for _ in range(npts):
+                p = p0.copy()
+                p[c0:c0 + 2 * nres] *= 1 + 0.1 * rng.standard_normal(2 * nres)
+                sets.append(p)
+            got = {}
+            for mx in (1, 0):
+                ctx.set_tuning(capi.TUNE_COH_MFMA, mx)
+                got[mx] = ctx.reduce_metric_multi("NLL", sets, [n0] * npts)
+                for k in (0, npts // 2, npts - 1):
+                    ctx.update_params(sets[k])
+                    assert ctx.reduce_metric("NLL")[0] == got[mx][k], (mx, k)
+                ctx.update_params(p0)
+            np.testing.assert_allclose(got[1], got[0], rtol=1e-13, atol=0)
+        for mx in (1, 0):
+            ctx.set_tuning(capi.TUNE_COH_MFMA, mx)
+            assert math.isclose(ctx.reduce_metric_multi("NLL", [p0], [n0])[0], ref, rel_tol=1e-11)
+    finally:
+        ctx.set_tuning(capi.TUNE_COH_MFMA, 1)
+        ctx.update_params(p0)

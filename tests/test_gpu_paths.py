@@ -62,7 +62,7 @@ def _variants(ctx, ref):
             assert out[blocks, 1] == out[blocks, 0], blocks
     finally:
         ctx.set_tuning(capi.TUNE_BLOCKS_PER_CU, 4)
-        ctx.set_tuning(capi.TUNE_FAST_BLOCKS_PER_CU, 4)
+        ctx.set_tuning(capi.TUNE_FAST_BLOCKS_PER_CU, 3)
         ctx.set_tuning(capi.TUNE_INLINE_ARGS, 1)
         ctx.set_tuning(capi.TUNE_FUSED_FINISH, 1)
     return out
