@@ -42,6 +42,50 @@ __device__ __forceinline__ void cmac(double &re, double &im, double cr, double c
     im = fma(cr, w.y, fma(ci, w.x, im));
 }
 
+// The coherent sum sum_{i < nres} c_i W_i (device_DalitzPlot's totalAmp,
+// src/PDFs/physics/Amp3Body.cu:88-108) with one fused multiply-add per real term, in
+// groups of four resonances 4m .. 4m + 3: Re adds cr wr of the four, then -ci wi of
+// the four; Im adds ci wr of the four, then cr wi of the four.  This is the order in
+// which the FP64 matrix-core kernel adds its K terms (one 16x16x4 step takes one
+// component of four resonances), so the vector and matrix-core sweeps give the same
+// bits.  cpl(i) returns (cr, ci) of resonance i and is called for i < nres only; w
+// holds NR >= nres waves, zero past nres.  A group that reaches past nres adds its
+// terms past nres too (zero coupling, zero wave), as the matrix cores do: x + 0 is x,
+// so only the sign of a zero sum could differ, which |A|^2 does not see.  Groups that
+// lie wholly past nres are skipped, as the matrix-core kernel never forms them.
+template <int NR, class Cpl>
+__device__ __forceinline__ void coh_group(double &re, double &im, const double2 (&c)[4], const double2 *w) {
+#pragma unroll
+    for(int u = 0; u < 4; ++u) {
+        re = fma(c[u].x, w[u].x, re);
+        im = fma(c[u].y, w[u].x, im);
+    }
+#pragma unroll
+    for(int u = 0; u < 4; ++u) {
+        re = fma(-c[u].y, w[u].y, re);
+        im = fma(c[u].x, w[u].y, im);
+    }
+}
+template <int NR, class Cpl>
+__device__ __forceinline__ void coh_sum(double &re, double &im, const Cpl &cpl, const double2 *w, int nres) {
+    static_assert(NR % 4 == 0, "groups of four resonances");
+#pragma unroll
+    for(int m = 0; m < NR; m += 4) {
+        double2 c[4];  // each coupling read once
+        if(m + 4 <= nres) {
+#pragma unroll
+            for(int u = 0; u < 4; ++u)
+                c[u] = cpl(m + u);
+            coh_group<NR, Cpl>(re, im, c, w + m);
+        } else if(m < nres) {
+#pragma unroll
+            for(int u = 0; u < 4; ++u)
+                c[u] = m + u < nres ? cpl(m + u) : make_double2(0.0, 0.0);
+            coh_group<NR, Cpl>(re, im, c, w + m);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ grids
 
 // Centre of local bin b of n equal bins on [lo, hi]: ((hi - lo) / n) * (b + 0.5) + lo,

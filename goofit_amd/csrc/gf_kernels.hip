@@ -171,8 +171,9 @@ __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs
             im[j] = 0.0;
         }
         // Loads of GF_DZ_CHUNK resonances are issued back to back before any of them is
-        // consumed, so each wave keeps 8 * EPT 16-byte loads in flight; the sum still
-        // runs over i = 0, 1, ... in order, as the reference's loop does.
+        // consumed, so each wave keeps 8 * EPT 16-byte loads in flight; the terms are
+        // added in gfd::coh_sum's order (groups of four resonances), as every other
+        // coherent-sum kernel adds them.
         constexpr int GF_DZ_CHUNK = 8;
         for(int i0 = 0; i0 < in.a; i0 += GF_DZ_CHUNK) {
             double2 w[GF_DZ_CHUNK][EPT];
@@ -185,15 +186,15 @@ __device__ __forceinline__ void step(int kind, const GfIns &in, const GfProgArgs
                     w[u][j] = use ? b[en[j]] : make_double2(0.0, 0.0);
             }
 #pragma unroll
-            for(int u = 0; u < GF_DZ_CHUNK; ++u) {
-                if(i0 + u < in.a) {
-                    const double ar = P[in.p + 2 * (i0 + u)], ai = P[in.p + 2 * (i0 + u) + 1];
+            for(int j = 0; j < EPT; ++j) {
+                double2 wj[GF_DZ_CHUNK];
 #pragma unroll
-                    for(int j = 0; j < EPT; ++j) {
-                        const double2 me = w[u][j];
-                        gfd::cmac(re[j], im[j], ar, ai, me);
-                    }
-                }
+                for(int u = 0; u < GF_DZ_CHUNK; ++u)
+                    wj[u] = w[u][j];
+                gfd::coh_sum<GF_DZ_CHUNK>(
+                    re[j], im[j],
+                    [&](int u) { return make_double2(P[in.p + 2 * (i0 + u)], P[in.p + 2 * (i0 + u) + 1]); }, wj,
+                    in.a - i0);
             }
         }
 #pragma unroll
@@ -951,13 +952,8 @@ __global__ __launch_bounds__(256) void k_dalitz_fast(GfProgArgs A, GfInline I, i
         const double m12 = ZERO_OUT ? 0.0 : c12[e];
         const double m13 = ZERO_OUT ? 0.0 : c13[e];
         double re = 0.0, im = 0.0;
-#pragma unroll
-        for(int i = 0; i < GF_MAX_RES; ++i) {
-            if(i < nres) {
-                const double ar = P[dzi.p + 2 * i], ai = P[dzi.p + 2 * i + 1];
-                gfd::cmac(re, im, ar, ai, w[i]);
-            }
-        }
+        gfd::coh_sum<GF_MAX_RES>(
+            re, im, [&](int i) { return make_double2(P[dzi.p + 2 * i], P[dzi.p + 2 * i + 1]); }, w, nres);
         const bool inside =
             ZERO_OUT || gfd::in_dalitz(m12, m13, D.motherMass, D.daug1Mass, D.daug2Mass, D.daug3Mass);
         // a degree-0 polynomial is its constant coefficient (numBoxes == 1)
@@ -1102,11 +1098,7 @@ __global__ __launch_bounds__(256) void k_dalitz_multi(GfProgArgs A, const double
         for(int p = 0; p < GF_MULTI_PTS; ++p) {
             if(p < npts) {
                 double re = 0.0, im = 0.0;
-#pragma unroll
-                for(int i = 0; i < NR; ++i) {
-                    const double2 cp = S.c[p][i];
-                    gfd::cmac(re, im, cp.x, cp.y, w[i]);
-                }
+                gfd::coh_sum<NR>(re, im, [&](int i) { return S.c[p][i]; }, w, nres);
                 const double eff =
                     CONST_EFF ? S.eff[p] : multipoly_at(effi, A, gfd::as_const(params + (int64_t)p * np), e);
                 const double T = inside ? (re * re + im * im) * eff : 0.0;
@@ -1141,28 +1133,31 @@ __global__ __launch_bounds__(256) void k_dalitz_multi(GfProgArgs A, const double
 // coherent sums of a wave's 64 events formed on the FP64 matrix cores.  Per tile of 16
 // events, Re_p and Im_p are rows of real matrix products M W over K = 2 NR terms, on
 // v_mfma_f64_16x16x4f64 (NR / 2 steps of K = 4).  The instruction adds its K terms to
-// the accumulator one after the other, each as a fused multiply-add, so with the terms of
-// resonance i ordered as the vector path's gfd::cmac orders them -- Re: -ci wi, then
-// cr wr; Im: ci wr, then cr wi -- every element equals cmac's result, bit for bit.
+// the accumulator one after the other, each as a fused multiply-add.  K-step 2 m takes
+// the real parts of the waves of resonances 4 m .. 4 m + 3 (Re: cr wr; Im: ci wr),
+// K-step 2 m + 1 their imaginary parts (Re: -ci wi; Im: cr wi): the order of
+// gfd::coh_sum, so every element equals the vector sweeps' sum bit for bit.
 // Operand layouts (gfx950; the host uses this kernel only after its probe of the
 // instruction found the result layout assumed here): lane l supplies A[l % 16][l / 16]
 // and B[l / 16][l % 16] of a K-step and holds D[4 r + l / 16][l % 16] in result register
-// r.  The couplings are loop-invariant A operands (16 doubles a lane, read from LDS once);
-// each lane loads both parts of one resonance of one event per K-step straight from the
-// wave cache (lanes 0-31 read resonance 2 kk of 16 consecutive events, 256 contiguous
-// bytes, lanes 32-63 the next resonance).  Lane (g = l / 16, j = l % 16) ends tile t
-// holding |A|^2 eff of sets 4 r + g of event 16 t + j, and multiplies it into those
-// sets' NLL products: per set, lane (g, j) takes events 16 t + j of each of the wave's
-// tiles in order.  The sets are then reduced over the 16 lanes of their group (xor 8, 4,
-// 2, 1) and the four waves in order, partials [p][b].  k_dalitz_fast groups the events
-// of a set the same way whenever this kernel serves the multi-point sums (GfProgArgs::
-// coh_group), so a multi-point sum equals the single sweep of the same parameters.
+// r.  The couplings are loop-invariant A operands (16 doubles a lane, read from LDS once).
+// Lane (g = l / 16, j = l % 16) loads the wave of resonance 4 m + g of event 16 t + j
+// (one 16-byte load per group of four resonances: a 256-byte run of 16 events per
+// resonance) and gives its real part to K-step 2 m and its imaginary part to 2 m + 1,
+// for both products.  Lane (g, j) ends tile t holding |A|^2 eff of sets 4 r + g of event
+// 16 t + j, and multiplies it into those sets' NLL products: per set, lane (g, j) takes
+// events 16 t + j of each of the wave's tiles in order.  The sets are then reduced over
+// the 16 lanes of their group (xor 8, 4, 2, 1) and the four waves in order, partials
+// [p][b].  k_dalitz_fast groups the events of a set the same way whenever this kernel
+// serves the multi-point sums (GfProgArgs::coh_group), so a multi-point sum equals the
+// single sweep of the same parameters.  The register budget is that of three waves per
+// SIMD, the occupancy the grid of three workgroups per CU asks for.
 template <int NR>
-__global__ __launch_bounds__(256) void k_coh_mx(GfProgArgs A, const double *__restrict__ params, int np,
-                                                const double *__restrict__ norms, int nn, int npts,
-                                                Partial *partials) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_coh_mx(
+    GfProgArgs A, const double *__restrict__ params, int np, const double *__restrict__ norms, int nn, int npts,
+    Partial *partials) {
     static_assert(NR % 4 == 0 && NR <= GF_MAX_RES, "NR: 4, 8, 12 or 16");
-    constexpr int KS = NR / 2;  // K-steps of 4 terms (two resonances)
+    constexpr int NG = NR / 4;  // groups of four resonances: K-steps 2 m (real parts), 2 m + 1 (imaginary)
     __shared__ MultiSets S;
     __shared__ double ls[4][GF_MULTI_PTS], lc[4][GF_MULTI_PTS];
     const GfIns dzi       = A.ins[0];
@@ -1172,55 +1167,97 @@ __global__ __launch_bounds__(256) void k_coh_mx(GfProgArgs A, const double *__re
     const int64_t n       = A.nevents;
     stage_sets<true>(S, params, np, norms, nn, npts, dzi.p, effi.p + effi.a, nres);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, j = lane & 15;
-    // A[j][k] with k = 4 kk + g: resonance k / 2; the even term of a resonance comes first
-    double are[KS], aim[KS];
+    // A[j][k]: set j's factor of K term k = 4 kk + g (resonance 4 (kk / 2) + g)
+    double are[2 * NG], aim[2 * NG];
 #pragma unroll
-    for(int kk = 0; kk < KS; ++kk) {
-        const int k     = 4 * kk + g;
-        const double2 c = S.c[j][k >> 1];
-        are[kk]         = (k & 1) ? c.x : -c.y;  // Re: -ci (with wi), then cr (with wr)
-        aim[kk]         = (k & 1) ? c.x : c.y;   // Im: ci (with wr), then cr (with wi)
+    for(int m = 0; m < NG; ++m) {
+        const double2 c = S.c[j][4 * m + g];
+        are[2 * m]      = c.x;   // Re: cr wr
+        aim[2 * m]      = c.y;   // Im: ci wr
+        are[2 * m + 1]  = -c.y;  // Re: -ci wi
+        aim[2 * m + 1]  = c.x;   // Im: cr wi
     }
-    double eff[4], nrm[4], m[4], ex[4];
+    double m_[4], ex[4];
 #pragma unroll
     for(int r = 0; r < 4; ++r) {
-        eff[r] = S.eff[4 * r + g];
-        nrm[r] = S.norm[4 * r + g];
-        m[r]   = 1.0;
-        ex[r]  = 0.0;
+        m_[r] = 1.0;
+        ex[r] = 0.0;
     }
+    // The wave's tiles in order: tile q = 4 p + t covers events first + p stride + 64 wave +
+    // 16 t + (0..15) (pass p of the grid-stride loop, quarter t of the wave's 64 events).
+    // Tile q + 2's waves are requested before tile q's products start, so two tiles of
+    // loads stay in flight behind the matrix cores; a tile past the end reads row n - 1
+    // (one line per resonance) and is never used.
     const double2 *__restrict__ wv = dz.wave;
-    for(int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t e0 = base + 64 * wave;
+    const int64_t stride           = (int64_t)gridDim.x * blockDim.x;
+    const int64_t first            = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t ntile            = first < n ? 4 * ((n - 1 - first) / stride + 1) : 0;
+    auto tile_event = [&](int64_t q) { return first + (q >> 2) * stride + 64 * wave + 16 * (q & 3) + j; };
+    // Only the last group can reach past nres (NR - 4 < nres <= NR): there the load reads
+    // resonance nres - 1 and the value is replaced by zero, without a branch around it.
+    auto load_tile = [&](int64_t q, double2(&w)[NG]) {
+        const int64_t ev  = tile_event(q);
+        const int64_t row = ev < n ? ev : n - 1;  // loads stay in bounds; the event is masked below
 #pragma unroll
-        for(int t = 0; t < 4; ++t) {
-            const int64_t ev  = e0 + 16 * t + j;
-            const int64_t row = ev < n ? ev : n - 1;  // loads stay in bounds; the event is masked below
-            double bre[KS], bim[KS];
+        for(int m = 0; m < NG; ++m) {
+            const int i  = 4 * m + g;
+            const int ir = (m == NG - 1 && i >= nres) ? nres - 1 : i;
+            w[m]         = wv[(int64_t)ir * dz.ncache + row];
+        }
+    };
+    auto run_tile = [&](int64_t q, const double2(&w)[NG]) {
+        gf_double4 re = {0, 0, 0, 0}, im = {0, 0, 0, 0};
 #pragma unroll
-            for(int kk = 0; kk < KS; ++kk) {
-                const int k    = 4 * kk + g, i = k >> 1;
-                const double2 w = i < nres ? wv[(int64_t)i * dz.ncache + row] : make_double2(0.0, 0.0);
-                bre[kk]        = (k & 1) ? w.x : w.y;
-                bim[kk]        = (k & 1) ? w.y : w.x;
-            }
-            gf_double4 re = {0, 0, 0, 0}, im = {0, 0, 0, 0};
+        for(int m = 0; m < NG; ++m) {
+            const bool pad   = m == NG - 1 && 4 * m + g >= nres;
+            const double2 wk = pad ? make_double2(0.0, 0.0) : w[m];
+            re               = __builtin_amdgcn_mfma_f64_16x16x4f64(are[2 * m], wk.x, re, 0, 0, 0);
+            im               = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[2 * m], wk.x, im, 0, 0, 0);
+            re               = __builtin_amdgcn_mfma_f64_16x16x4f64(are[2 * m + 1], wk.y, re, 0, 0, 0);
+            im               = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[2 * m + 1], wk.y, im, 0, 0, 0);
+        }
+        // The sets' efficiencies and normalisations are read from LDS per tile (held in
+        // registers they would cost the occupancy of three waves per SIMD).  A lane past the
+        // last event or set multiplies its product by exactly 1 (mantissa 1/2, exponent
+        // +1, renormalised back: the pair is unchanged), which keeps the loop free of
+        // branches.
+        asm volatile("" ::: "memory");
+        const bool live = tile_event(q) < n;
+        double ef[4], nm[4];
 #pragma unroll
-            for(int kk = 0; kk < KS; ++kk) {
-                re = __builtin_amdgcn_mfma_f64_16x16x4f64(are[kk], bre[kk], re, 0, 0, 0);
-                im = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[kk], bim[kk], im, 0, 0, 0);
-            }
-            if(ev < n) {
+        for(int r = 0; r < 4; ++r) {
+            ef[r] = S.eff[4 * r + g];
+            nm[r] = S.norm[4 * r + g];
+        }
 #pragma unroll
-                for(int r = 0; r < 4; ++r)
-                    if(4 * r + g < npts)
-                        gfd::nll_add(m[r], ex[r], (re[r] * re[r] + im[r] * im[r]) * eff[r], nrm[r]);
-            }
+        for(int r = 0; r < 4; ++r) {
+            const bool use = live && 4 * r + g < npts;
+            const double v = (re[r] * re[r] + im[r] * im[r]) * ef[r];
+            gfd::nll_add(m_[r], ex[r], use ? v : 1.0, use ? nm[r] : 1.0);
+        }
+    };
+    if(ntile > 0) {
+        double2 wa[NG], wb[NG], wc[NG];
+        load_tile(0, wa);
+        load_tile(1, wb);
+        for(int64_t q = 0;; q += 3) {
+            load_tile(q + 2, wc);
+            run_tile(q, wa);
+            if(q + 1 >= ntile)
+                break;
+            load_tile(q + 3, wa);
+            run_tile(q + 1, wb);
+            if(q + 2 >= ntile)
+                break;
+            load_tile(q + 4, wb);
+            run_tile(q + 2, wc);
+            if(q + 3 >= ntile)
+                break;
         }
     }
 #pragma unroll
     for(int r = 0; r < 4; ++r) {
-        double sv = gfd::nll_total(m[r], ex[r]), cv = 0.0;
+        double sv = gfd::nll_total(m_[r], ex[r]), cv = 0.0;
         group_reduce(sv, cv);
         if(j == 0) {
             ls[wave][4 * r + g] = sv;
@@ -1533,6 +1570,8 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_dalitz_points(GfProgArgs A, 
 #pragma unroll
         for(int p = 0; p < GF_MULTI_PTS; ++p) {
             if(p < cnt) {
+                // a toy's points are compared with one another, never with the single
+                // sweep, so the sum keeps the per-resonance order (fewer live registers)
                 double re = 0.0, im = 0.0;
 #pragma unroll
                 for(int i = 0; i < NR; ++i) {
