@@ -776,6 +776,7 @@ PYBIND11_MODULE(_goofit, m) {
     me.attr("TUNE_FLAT")                = (int)GF_TUNE_FLAT;
     me.attr("TUNE_FUSED_FINISH")        = (int)GF_TUNE_FUSED_FINISH;
     me.attr("TUNE_COH_MFMA")            = (int)GF_TUNE_COH_MFMA;
+    me.attr("TUNE_BATCH_MX_PTS")        = (int)GF_TUNE_BATCH_MX_PTS;
     me.def("callnumber", []() { return host_callnumber; });
     me.def("flattened", []() {
         return py::make_tuple(host_function_table, host_parameters, host_constants, host_observables,

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libgoofit_hip.so")
 # include/goofit_hip.h
 GF_OK, GF_ERR_INVALID, GF_ERR_HIP, GF_ERR_NOMEM, GF_ERR_STATE, GF_ERR_NODEVICE = range(6)
 (TUNE_INLINE_ARGS, TUNE_FAST_DALITZ, TUNE_BLOCKS_PER_CU, TUNE_FAST_BLOCKS_PER_CU, TUNE_SPIN_US,
- TUNE_ZERO_OUTSIDE, TUNE_FLAT, TUNE_FUSED_FINISH, TUNE_COH_MFMA) = range(9)
+ TUNE_ZERO_OUTSIDE, TUNE_FLAT, TUNE_FUSED_FINISH, TUNE_COH_MFMA, TUNE_BATCH_MX_PTS) = range(10)
 FN = dict(NONE=0, GAUSSIAN=1, EXP=2, EXP_OFFSET=3, EXP_POLY=4, EXP_POLY_OFFSET=5, EXPGAUS=6,
           POLYNOMIAL=7, OFFSET_POLYNOMIAL=8, MULTI_POLYNOMIAL=9, ADD=10, ADD_EXT=11, PROD=12,
           DALITZ=13, RBW=14, RBW_SYM=15, GS=16, NONRES=17)

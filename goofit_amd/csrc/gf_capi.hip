@@ -156,7 +156,7 @@ struct gf_ctx {
     uint32_t noalloc_seq = 0;  // of the last gf_reduce_metric_noalloc
 
     // launch-shape knobs (gf_tune)
-    int tune[GF_TUNE_COUNT] = {1, 1, 4, 3, 2000, 1, 1, 1, 1};
+    int tune[GF_TUNE_COUNT] = {1, 1, 4, 3, 2000, 1, 1, 1, 1, 0};
 
     uint64_t events_gen = 1;  // bumped whenever the event columns may change
 
@@ -1196,6 +1196,11 @@ gf_status gf_ctx_set_tuning(gf_ctx *ctx, int key, int value) {
     case GF_TUNE_FAST_BLOCKS_PER_CU:
         if(value < 1 || value > 64)
             return fail(GF_ERR_INVALID, "tuning key %d takes 1..64 workgroups per CU (got %d)", key, value);
+        break;
+    case GF_TUNE_BATCH_MX_PTS:
+        if(value != 0 && (value < 2 || value > GF_MULTI_PTS + 1))
+            return fail(GF_ERR_INVALID, "tuning key %d takes 0 or 2..%d points (got %d)", key, GF_MULTI_PTS + 1,
+                        value);
         break;
     case GF_TUNE_SPIN_US:
         if(value < 0 || value > 1000000)
@@ -3448,9 +3453,18 @@ gf_status gf_batch_submit_points(gf_batch *b, int metric, uint32_t *ticket) {
     const int groups = (q.max + GF_MULTI_PTS - 1) / GF_MULTI_PTS;
     GfProgArgs a     = prog_args(ctx);
     const int nres   = fast ? ctx->dalitz[0].nres : 0;
+    // groups of at least mx_min points take the coherent sums on the matrix cores (0: none).
+    // Automatic: measured on 1024-2048 Dalitz toys, groups of 4 or more gain at 12 and 16
+    // resonances (toy batch 9 % and 13 % faster) and lose at 8, where the vector kernel
+    // does half the work per point and keeps four waves per SIMD.
+    int mx_min = 0;
+    if(fast && ce && zo && metric == GF_METRIC_NLL && ctx->mfma_layout == 2) {
+        const int knob = ctx->tune[GF_TUNE_BATCH_MX_PTS];
+        mx_min         = knob ? knob : (nres > 8 ? 4 : 0);
+    }
     if(gfk::launch_batch_points(a, nres, b->d_work, b->nblocks, groups, q.d + b->ntoys, b->np, b->nn,
-                                (const int *)q.d, fast, ce, zo, metric, q.d_partials, b->d_first, b->ntoys, q.d_res,
-                                b->d_counter, b->d_stamp, ctx->stream))
+                                (const int *)q.d, fast, ce, zo, metric, mx_min, q.d_partials, b->d_first, b->ntoys,
+                                q.d_res, b->d_counter, b->d_stamp, ctx->stream))
         return fail(GF_ERR_HIP, "batch launch failed");
     q.ticket    = ++b->expect;
     q.in_flight = true;

@@ -231,8 +231,8 @@ int launch_batch_norms(double *stage, int nslots, int np, int nn, const double *
                        const double *table, void *stream);
 int launch_batch_points(const GfProgArgs &a, int nres, const GfBatchWork *work, int nblocks, int groups,
                         const double *stage, int np, int nn, const int *tbl, bool fast, bool const_eff,
-                        bool zero_outside, int metric, Partial *partials, const int *first, int ntoys, double *result,
-                        uint32_t *counter, uint32_t *stamp, void *stream);
+                        bool zero_outside, int metric, int mx_min, Partial *partials, const int *first, int ntoys,
+                        double *result, uint32_t *counter, uint32_t *stamp, void *stream);
 // Accept-reject generation: candidates (uniform in box) with accept flags, and
 // the ordered compaction of the accepted ones.
 // eff_pc >= 0: efficiency subprogram; -1: efficiency 1; -2: the constant eff_const.

@@ -1152,20 +1152,18 @@ __global__ __launch_bounds__(256) void k_dalitz_multi(GfProgArgs A, const double
 // serves the multi-point sums (GfProgArgs::coh_group), so a multi-point sum equals the
 // single sweep of the same parameters.  The register budget is that of three waves per
 // SIMD, the occupancy the grid of three workgroups per CU asks for.
+//
+// coh_mx_rows runs the tiles of this workgroup's events (rows first + k stride +
+// 0..255 below end, wave w taking rows 64 w .. 64 w + 63 of each) for the sets staged in
+// S and leaves set p's (sum, residual) of the workgroup in ls[p], lc[p] (every thread
+// returns after the workgroup barrier).  All 256 threads call it.
 template <int NR>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_coh_mx(
-    GfProgArgs A, const double *__restrict__ params, int np, const double *__restrict__ norms, int nn, int npts,
-    Partial *partials) {
+__device__ __forceinline__ void coh_mx_rows(const MultiSets &S, const GfDalitzSlot &dz, int nres, int64_t first,
+                                            int64_t stride, int64_t end, int npts, double (&ls)[GF_MULTI_PTS],
+                                            double (&lc)[GF_MULTI_PTS]) {
     static_assert(NR % 4 == 0 && NR <= GF_MAX_RES, "NR: 4, 8, 12 or 16");
     constexpr int NG = NR / 4;  // groups of four resonances: K-steps 2 m (real parts), 2 m + 1 (imaginary)
-    __shared__ MultiSets S;
-    __shared__ double ls[4][GF_MULTI_PTS], lc[4][GF_MULTI_PTS];
-    const GfIns dzi       = A.ins[0];
-    const GfIns effi      = A.ins[1];
-    const GfDalitzSlot dz = gfd::as_const(A.dz)[dzi.x];
-    const int nres        = dzi.a;
-    const int64_t n       = A.nevents;
-    stage_sets<true>(S, params, np, norms, nn, npts, dzi.p, effi.p + effi.a, nres);
+    __shared__ double wls[4][GF_MULTI_PTS], wlc[4][GF_MULTI_PTS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, j = lane & 15;
     // A[j][k]: set j's factor of K term k = 4 kk + g (resonance 4 (kk / 2) + g)
     double are[2 * NG], aim[2 * NG];
@@ -1183,21 +1181,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         m_[r] = 1.0;
         ex[r] = 0.0;
     }
-    // The wave's tiles in order: tile q = 4 p + t covers events first + p stride + 64 wave +
-    // 16 t + (0..15) (pass p of the grid-stride loop, quarter t of the wave's 64 events).
+    // The wave's tiles in order: tile q = 4 p + t covers rows first + p stride + 64 wave +
+    // 16 t + (0..15) (pass p of the strided loop, quarter t of the wave's 64 rows).
     // Tile q + 2's waves are requested before tile q's products start, so two tiles of
-    // loads stay in flight behind the matrix cores; a tile past the end reads row n - 1
+    // loads stay in flight behind the matrix cores; a tile past the end reads row end - 1
     // (one line per resonance) and is never used.
     const double2 *__restrict__ wv = dz.wave;
-    const int64_t stride           = (int64_t)gridDim.x * blockDim.x;
-    const int64_t first            = (int64_t)blockIdx.x * blockDim.x;
-    const int64_t ntile            = first < n ? 4 * ((n - 1 - first) / stride + 1) : 0;
+    const int64_t ntile            = first < end ? 4 * ((end - 1 - first) / stride + 1) : 0;
     auto tile_event = [&](int64_t q) { return first + (q >> 2) * stride + 64 * wave + 16 * (q & 3) + j; };
     // Only the last group can reach past nres (NR - 4 < nres <= NR): there the load reads
     // resonance nres - 1 and the value is replaced by zero, without a branch around it.
     auto load_tile = [&](int64_t q, double2(&w)[NG]) {
         const int64_t ev  = tile_event(q);
-        const int64_t row = ev < n ? ev : n - 1;  // loads stay in bounds; the event is masked below
+        const int64_t row = ev < end ? ev : end - 1;  // loads stay in bounds; the row is masked below
 #pragma unroll
         for(int m = 0; m < NG; ++m) {
             const int i  = 4 * m + g;
@@ -1218,11 +1214,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
         // The sets' efficiencies and normalisations are read from LDS per tile (held in
         // registers they would cost the occupancy of three waves per SIMD).  A lane past the
-        // last event or set multiplies its product by exactly 1 (mantissa 1/2, exponent
-        // +1, renormalised back: the pair is unchanged), which keeps the loop free of
-        // branches.
+        // last row or set multiplies its product by exactly 1 (mantissa 1/2, exponent +1,
+        // renormalised back: the pair is unchanged), which keeps the loop free of branches.
         asm volatile("" ::: "memory");
-        const bool live = tile_event(q) < n;
+        const bool live = tile_event(q) < end;
         double ef[4], nm[4];
 #pragma unroll
         for(int r = 0; r < 4; ++r) {
@@ -1255,23 +1250,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 break;
         }
     }
+    // set p: the 16 lanes of its group (xor 8, 4, 2, 1), then the four waves in order
 #pragma unroll
     for(int r = 0; r < 4; ++r) {
         double sv = gfd::nll_total(m_[r], ex[r]), cv = 0.0;
         group_reduce(sv, cv);
         if(j == 0) {
-            ls[wave][4 * r + g] = sv;
-            lc[wave][4 * r + g] = cv;
+            wls[wave][4 * r + g] = sv;
+            wlc[wave][4 * r + g] = cv;
         }
     }
     __syncthreads();
     if((int)threadIdx.x < npts) {
         const int p = threadIdx.x;
-        double sv   = ls[0][p], cv = lc[0][p];
+        double sv   = wls[0][p], cv = wlc[0][p];
         for(int w = 1; w < 4; ++w)
-            gfd::kb_merge(sv, cv, ls[w][p], lc[w][p]);
-        partials[(int64_t)p * gridDim.x + blockIdx.x].s = sv;
-        partials[(int64_t)p * gridDim.x + blockIdx.x].c = cv;
+            gfd::kb_merge(sv, cv, wls[w][p], wlc[w][p]);
+        ls[p] = sv;
+        lc[p] = cv;
+    }
+    __syncthreads();
+}
+
+template <int NR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_coh_mx(
+    GfProgArgs A, const double *__restrict__ params, int np, const double *__restrict__ norms, int nn, int npts,
+    Partial *partials) {
+    __shared__ MultiSets S;
+    __shared__ double ls[GF_MULTI_PTS], lc[GF_MULTI_PTS];
+    const GfIns dzi       = A.ins[0];
+    const GfIns effi      = A.ins[1];
+    const GfDalitzSlot dz = gfd::as_const(A.dz)[dzi.x];
+    const int nres        = dzi.a;
+    stage_sets<true>(S, params, np, norms, nn, npts, dzi.p, effi.p + effi.a, nres);
+    coh_mx_rows<NR>(S, dz, nres, (int64_t)blockIdx.x * blockDim.x, (int64_t)gridDim.x * blockDim.x, A.nevents,
+                    npts, ls, lc);
+    if((int)threadIdx.x < npts) {
+        const int p                                    = threadIdx.x;
+        partials[(int64_t)p * gridDim.x + blockIdx.x].s = ls[p];
+        partials[(int64_t)p * gridDim.x + blockIdx.x].c = lc[p];
     }
 }
 
@@ -1523,19 +1540,20 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_points(GfProgArgs A, const G
 
 // Several points per toy, coherent-sum program (as k_dalitz_multi): workgroup
 // (b, g) reads the waves of each row of work[b] once and adds the row's metric
-// for every point p in [16 g, min(16 g + 16, npts)) of its toy.
+// for every point p in [16 g, min(16 g + 16, npts)) of its toy.  A group of
+// mx_min points or more is left to k_batch_coh_mx (launched over the same grid).
 template <bool CONST_EFF, bool ZERO_OUT, int NR, bool NLL>
 __global__ __launch_bounds__(GF_BLOCK) void k_batch_dalitz_points(GfProgArgs A, const GfBatchWork *__restrict__ work,
                                                                   const double *__restrict__ stage, int np, int nn,
                                                                   const int *__restrict__ tbl, int metric,
-                                                                  Partial *partials) {
+                                                                  int mx_min, Partial *partials) {
     __shared__ MultiSets S;
     const GfBatchWork w = work[blockIdx.x];
     const int slot0     = tbl[2 * w.toy];
     const int npts      = tbl[2 * w.toy + 1];
     const int p0        = (int)blockIdx.y * GF_MULTI_PTS;
     const int cnt       = min(npts - p0, GF_MULTI_PTS);
-    if(cnt <= 0)
+    if(cnt <= 0 || cnt >= mx_min)
         return;  // uniform over the workgroup
     const GfIns dzi       = A.ins[0];
     const GfIns effi      = A.ins[1];
@@ -1601,6 +1619,41 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_dalitz_points(GfProgArgs A, 
                 partials[(int64_t)blockIdx.x * GF_BATCH_PTS + p0 + p].c = cc;
             }
         }
+    }
+}
+
+// The groups of k_batch_dalitz_points with mx_min points or more, for the NLL of a
+// lone Amp3Body with constant efficiency and waves 0 outside the Dalitz region: the
+// coherent sums on the FP64 matrix cores (coh_mx_rows over the rows of the
+// workgroup's chunk, as k_coh_mx runs them over the whole sample).  A smaller group
+// (a line-search point, say) exits here and is taken by the vector kernel.  A
+// point's value depends on the size of the group it came in, deterministically; the
+// points of a toy meet only in its minimiser, which sees differences of an ulp.
+template <int NR>
+__global__ __launch_bounds__(GF_BLOCK) void k_batch_coh_mx(GfProgArgs A, const GfBatchWork *__restrict__ work,
+                                                           const double *__restrict__ stage, int np, int nn,
+                                                           const int *__restrict__ tbl, int mx_min,
+                                                           Partial *partials) {
+    const GfBatchWork w = work[blockIdx.x];
+    const int npts      = tbl[2 * w.toy + 1];
+    const int p0        = (int)blockIdx.y * GF_MULTI_PTS;
+    const int cnt       = min(npts - p0, GF_MULTI_PTS);
+    if(cnt < mx_min)
+        return;  // uniform over the workgroup
+    __shared__ MultiSets S;
+    __shared__ double ls[GF_MULTI_PTS], lc[GF_MULTI_PTS];
+    const GfIns dzi                = A.ins[0];
+    const GfIns effi               = A.ins[1];
+    const GfDalitzSlot dz          = gfd::as_const(A.dz)[dzi.x];
+    const int nres                 = dzi.a;
+    const int64_t W                = np + nn;
+    const double *__restrict__ S0  = stage + (int64_t)(tbl[2 * w.toy] + p0) * W;
+    stage_sets<true>(S, S0, W, S0 + np, W, cnt, dzi.p, effi.p + effi.a, nres);
+    coh_mx_rows<NR>(S, dz, nres, w.start, GF_BLOCK, w.end, cnt, ls, lc);
+    if((int)threadIdx.x < cnt) {
+        const int p                                             = threadIdx.x;
+        partials[(int64_t)blockIdx.x * GF_BATCH_PTS + p0 + p].s = ls[p];
+        partials[(int64_t)blockIdx.x * GF_BATCH_PTS + p0 + p].c = lc[p];
     }
 }
 
@@ -2107,14 +2160,14 @@ static void launch_fast_t(const GfProgArgs &a, const GfInline *inl, int metric, 
         }                                                                                                            \
     } while(0)
 
-// k_coh_mx for nres resonances (NR = nres rounded up to 4).
-#define GF_COH_MX(NRES, ...)                                                                                         \
+// A matrix-core kernel (k_coh_mx, k_batch_coh_mx) for nres resonances (NR = nres rounded up to 4).
+#define GF_COH_MX(KERNEL, NRES, ...)                                                                                 \
     do {                                                                                                             \
         switch((NRES) <= 4 ? 4 : (NRES) <= 8 ? 8 : (NRES) <= 12 ? 12 : 16) {                                         \
-        case 4: hipLaunchKernelGGL((k_coh_mx<4>), __VA_ARGS__); break;                                              \
-        case 8: hipLaunchKernelGGL((k_coh_mx<8>), __VA_ARGS__); break;                                              \
-        case 12: hipLaunchKernelGGL((k_coh_mx<12>), __VA_ARGS__); break;                                            \
-        default: hipLaunchKernelGGL((k_coh_mx<16>), __VA_ARGS__); break;                                            \
+        case 4: hipLaunchKernelGGL((KERNEL<4>), __VA_ARGS__); break;                                                \
+        case 8: hipLaunchKernelGGL((KERNEL<8>), __VA_ARGS__); break;                                                \
+        case 12: hipLaunchKernelGGL((KERNEL<12>), __VA_ARGS__); break;                                              \
+        default: hipLaunchKernelGGL((KERNEL<16>), __VA_ARGS__); break;                                              \
         }                                                                                                            \
     } while(0)
 
@@ -2125,7 +2178,7 @@ int launch_dalitz_multi(const GfProgArgs &a, int nres, const double *params, int
     if(npts < 1 || npts > GF_MULTI_PTS || nres < 1 || nres > GF_MAX_RES)
         return -1;
     if(a.coh_mx_nres > 0 && const_eff && zero_outside && metric == gfd::metric_nll) {
-        GF_COH_MX(nres, dim3(nblocks), dim3(256), 0, s, a, params, np, norms, nn, npts, partials);
+        GF_COH_MX(k_coh_mx, nres, dim3(nblocks), dim3(256), 0, s, a, params, np, norms, nn, npts, partials);
         int rc = check(hipGetLastError());
         if(rc)
             return rc;
@@ -2255,8 +2308,8 @@ int launch_batch_norms(double *stage, int nslots, int np, int nn, const double *
 
 int launch_batch_points(const GfProgArgs &a, int nres, const GfBatchWork *work, int nblocks, int groups,
                         const double *stage, int np, int nn, const int *tbl, bool fast, bool const_eff,
-                        bool zero_outside, int metric, Partial *partials, const int *first, int ntoys, double *result,
-                        uint32_t *counter, uint32_t *stamp, void *stream) {
+                        bool zero_outside, int metric, int mx_min, Partial *partials, const int *first, int ntoys,
+                        double *result, uint32_t *counter, uint32_t *stamp, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     if(groups < 1 || groups * GF_MULTI_PTS > GF_BATCH_PTS || (fast && (nres < 1 || nres > GF_MAX_RES)))
         return -1;
@@ -2270,9 +2323,19 @@ int launch_batch_points(const GfProgArgs &a, int nres, const GfBatchWork *work, 
     else if(!fast)
         hipLaunchKernelGGL((k_batch_points<2, 0>), grid, dim3(GF_BLOCK), 0, s, a, work, stage, np, nn, tbl, metric,
                            partials);
-    else
+    else {
+        // groups of mx_min points or more on the matrix cores, the others on the vector ALUs
+        const bool mx = mx_min > 0 && const_eff && zero_outside && metric == gfd::metric_nll;
+        if(mx) {
+            GF_COH_MX(k_batch_coh_mx, nres, grid, dim3(GF_BLOCK), 0, s, a, work, stage, np, nn, tbl, mx_min,
+                      partials);
+            int rc = check(hipGetLastError());
+            if(rc)
+                return rc;
+        }
         GF_MULTI_SWITCH(k_batch_dalitz_points, nres, const_eff, zero_outside && const_eff, metric == gfd::metric_nll, grid, dim3(GF_BLOCK), 0, s,
-                        a, work, stage, np, nn, tbl, metric, partials);
+                        a, work, stage, np, nn, tbl, metric, mx ? mx_min : GF_MULTI_PTS + 1, partials);
+    }
     int rc = check(hipGetLastError());
     if(rc)
         return rc;

@@ -124,7 +124,12 @@ enum gf_tune {
                                        efficiency (gf_reduce_metric_multi) forms its coherent sums on the
                                        FP64 matrix cores -- the same bits as on the vector ALUs; 0: on the
                                        vector ALUs (default 1) */
-    GF_TUNE_COUNT              = 9
+    GF_TUNE_BATCH_MX_PTS       = 9, /* toy batches of a lone Amp3Body's NLL with a constant efficiency:
+                                       a toy's group of at least this many points (of 16) forms its
+                                       coherent sums on the FP64 matrix cores, a smaller one on the vector
+                                       ALUs (2..17, 17: never); 0: groups of 4 or more when the model has
+                                       more than 8 resonances, else never (default 0) */
+    GF_TUNE_COUNT              = 10
 };
 
 /* Set / read one knob (gf_tune); values out of range give GF_ERR_INVALID. */

@@ -222,7 +222,8 @@ def test_batch_points_dalitz(ctx, const_eff):
     """The coherent-sum form of gf_batch_reduce_points (waves read once per 16 points):
     a toy over all rows reproduces the oracle's NLL at its first point; every point agrees
     with the interpreter's gf_batch_reduce within 1e-12 relative; with a constant
-    efficiency the kernel that skips m12/m13 gives the same bits as the one that reads them."""
+    efficiency the vector kernel that skips m12/m13 gives the same bits as the one that
+    reads them, and the matrix-core groups agree with it within 1e-13."""
     n = 40_000
     eff = masses = None
     if not const_eff:
@@ -255,12 +256,33 @@ def test_batch_points_dalitz(ctx, const_eff):
                 if active[t]:
                     assert math.isclose(got[t, k], single[t], rel_tol=1e-12), (t, k, got[t, k], single[t])
         if const_eff:
-            ctx.set_tuning(capi.TUNE_ZERO_OUTSIDE, 0)
+            # with 16 resonances groups of 4 or more points sum on the matrix cores by default
+            # (TUNE_BATCH_MX_PTS 0): against the vector kernel for every group (17: never),
+            # and with every group of two or more points there (2)
+            assert ctx.get_tuning(capi.TUNE_BATCH_MX_PTS) == 0
+            ctx.set_tuning(capi.TUNE_BATCH_MX_PTS, 17)
             try:
+                valu = request()
+                ctx.set_tuning(capi.TUNE_ZERO_OUTSIDE, 0)
                 plain = request()
             finally:
                 ctx.set_tuning(capi.TUNE_ZERO_OUTSIDE, 1)
-            np.testing.assert_array_equal(plain, got)
+                ctx.set_tuning(capi.TUNE_BATCH_MX_PTS, 0)
+            np.testing.assert_array_equal(plain, valu)
+            np.testing.assert_allclose(got, valu, rtol=1e-13)
+            assert not np.array_equal(got, valu, equal_nan=True)  # the matrix-core groups ran
+            assert got[1, 0] == valu[1, 0]  # a lone point stays on the vector ALUs
+            ctx.set_tuning(capi.TUNE_BATCH_MX_PTS, 2)
+            try:
+                mx2 = request()
+            finally:
+                ctx.set_tuning(capi.TUNE_BATCH_MX_PTS, 0)
+            np.testing.assert_allclose(mx2, valu, rtol=1e-13)
+            assert mx2[1, 0] == valu[1, 0]
+            np.testing.assert_array_equal(mx2[2, :16], got[2, :16])  # full groups alike
+            np.testing.assert_array_equal(mx2[0, 16:20], got[0, 16:20])  # a group of 4 in both
+            with pytest.raises(capi.GooFitError):
+                ctx.set_tuning(capi.TUNE_BATCH_MX_PTS, 1)
     finally:
         b.close()
 
