@@ -918,7 +918,8 @@ __device__ __forceinline__ double multipoly_at(const GfIns &in, const GfProgArgs
 
 // |sum_i c_i W_i[e]|^2 * eff for a Dalitz node at the root of the tree.  All
 // wave loads of an event are issued before any arithmetic (16 x 16 bytes per lane
-// in flight), the rest is device_DalitzPlot's arithmetic in its order.
+// in flight); the rest is device_DalitzPlot's arithmetic, the coherent sum in
+// gfd::coh_sum's grouped order (the matrix-core kernel's).
 // ZERO_OUT (constant efficiency, cache filled from these events): the waves of an event
 // outside the Dalitz region are all 0, so |A|^2 eff is the reference's 0 there without
 // reading m12 and m13 (256 instead of 272 bytes per event).
