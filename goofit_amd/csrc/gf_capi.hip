@@ -1652,7 +1652,8 @@ gf_status gf_reduce_metric_multi(gf_ctx *ctx, int metric, int npts, const double
         gf_norms_update(ctx, n0.data(), nn);
         return st;
     }
-    // coherent-sum path: GF_MULTI_PTS sets per sweep, the waves read once per sweep
+    // coherent-sum path: GF_MULTI_PTS sets per sweep (GF_MX_SETS on the matrix cores), the
+    // waves read once per sweep
     const size_t vals = (size_t)npts * (size_t)(np + nn);
     if(vals > ctx->cap_multi) {
         GF_HIP(hipStreamSynchronize(ctx->stream));
@@ -1676,7 +1677,7 @@ gf_status gf_reduce_metric_multi(gf_ctx *ctx, int metric, int npts, const double
         ctx->cap_multi = c;
     }
     if(!ctx->d_multi_partials
-       && hipMalloc((void **)&ctx->d_multi_partials, (size_t)GF_MULTI_PTS * GF_MAX_PARTIALS * sizeof(gfk::Partial))
+       && hipMalloc((void **)&ctx->d_multi_partials, (size_t)GF_MX_SETS * GF_MAX_PARTIALS * sizeof(gfk::Partial))
               != hipSuccess) {
         ctx->d_multi_partials = nullptr;
         return fail(GF_ERR_NOMEM, "cannot allocate the multi-set partials");
@@ -1697,8 +1698,9 @@ gf_status gf_reduce_metric_multi(gf_ctx *ctx, int metric, int npts, const double
     GfProgArgs a  = prog_args(ctx);
     const int nb  = fast_blocks(ctx, ctx->n);
     a.coh_mx_nres = coh_mx_nres(ctx, metric);
-    for(int k0 = 0; k0 < npts; k0 += GF_MULTI_PTS) {
-        const int cnt = std::min(GF_MULTI_PTS, npts - k0);
+    const int chunk = (a.coh_mx_nres > 0 && ce && zo) ? GF_MX_SETS : GF_MULTI_PTS;
+    for(int k0 = 0; k0 < npts; k0 += chunk) {
+        const int cnt = std::min(chunk, npts - k0);
         int rc = gfk::launch_dalitz_multi(a, ctx->dalitz[0].nres, ctx->d_multi + (size_t)k0 * np, np,
                                           ctx->d_multi + (size_t)npts * np + (size_t)k0 * nn, nn, cnt, ce, zo, metric,
                                           ctx->d_multi_partials, ctx->d_multi_out + 2 * (size_t)k0, nb, ctx->stream);

@@ -196,6 +196,7 @@ int launch_reduce_flat(const GfProgArgs &a, const GfInline *inl, int metric, int
 // norms: npts x nn, device) over the same events in one sweep; out[2p], out[2p+1] =
 // sum and residual of set p (device).  partials: npts x nblocks.
 #define GF_MULTI_PTS 16
+#define GF_MX_SETS 32  // sets per sweep of the matrix-core NLL path (two groups of GF_MULTI_PTS)
 int launch_dalitz_multi(const GfProgArgs &a, int nres, const double *params, int np, const double *norms, int nn,
                         int npts, bool const_eff, bool zero_outside, int metric, Partial *partials, double *out,
                         int nblocks, void *stream);

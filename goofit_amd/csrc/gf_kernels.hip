@@ -1155,38 +1155,45 @@ __global__ __launch_bounds__(256) void k_dalitz_multi(GfProgArgs A, const double
 // SIMD, the occupancy the grid of three workgroups per CU asks for.
 //
 // coh_mx_rows runs the tiles of this workgroup's events (rows first + k stride +
-// 0..255 below end, wave w taking rows 64 w .. 64 w + 63 of each) for the sets staged in
-// S and leaves set p's (sum, residual) of the workgroup in ls[p], lc[p] (every thread
-// returns after the workgroup barrier).  All 256 threads call it.
-template <int NR>
-__device__ __forceinline__ void coh_mx_rows(const MultiSets &S, const GfDalitzSlot &dz, int nres, int64_t first,
-                                            int64_t stride, int64_t end, int npts, double (&ls)[GF_MULTI_PTS],
-                                            double (&lc)[GF_MULTI_PTS]) {
+// 0..255 below end, wave w taking rows 64 w .. 64 w + 63 of each) for the NSG x 16 sets
+// staged in S[0 .. NSG - 1] (npts of them in all) and leaves set p's (sum, residual) of
+// the workgroup in ls[p], lc[p] (every thread returns after the workgroup barrier).
+// With two set groups the wave loads of a tile serve both (the HBM bytes of a sweep are
+// shared by 32 sets); the first group's couplings stay in registers, the second's are
+// read from LDS per tile, so the kernel keeps three waves per SIMD.  Every set is
+// reduced over the same lanes and events in the same order whichever group it is in.
+// All 256 threads call it.
+template <int NR, int NSG>
+__device__ __forceinline__ void coh_mx_rows(const MultiSets *S, const GfDalitzSlot &dz, int nres, int64_t first,
+                                            int64_t stride, int64_t end, int npts, double *ls, double *lc) {
     static_assert(NR % 4 == 0 && NR <= GF_MAX_RES, "NR: 4, 8, 12 or 16");
+    static_assert(NSG == 1 || NSG == 2, "one or two groups of 16 sets");
     constexpr int NG = NR / 4;  // groups of four resonances: K-steps 2 m (real parts), 2 m + 1 (imaginary)
-    __shared__ double wls[4][GF_MULTI_PTS], wlc[4][GF_MULTI_PTS];
+    __shared__ double wls[4][GF_MX_SETS], wlc[4][GF_MX_SETS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, j = lane & 15;
-    // A[j][k]: set j's factor of K term k = 4 kk + g (resonance 4 (kk / 2) + g)
+    // A[j][k]: set j's factor of K term k = 4 kk + g (resonance 4 (kk / 2) + g): K-step 2 m
+    // takes (cr | ci) against wr, 2 m + 1 (-ci | cr) against wi
+    auto a_ops = [&](const double2 c, double &ar0, double &ai0, double &ar1, double &ai1) {
+        ar0 = c.x;   // Re: cr wr
+        ai0 = c.y;   // Im: ci wr
+        ar1 = -c.y;  // Re: -ci wi
+        ai1 = c.x;   // Im: cr wi
+    };
     double are[2 * NG], aim[2 * NG];
 #pragma unroll
-    for(int m = 0; m < NG; ++m) {
-        const double2 c = S.c[j][4 * m + g];
-        are[2 * m]      = c.x;   // Re: cr wr
-        aim[2 * m]      = c.y;   // Im: ci wr
-        are[2 * m + 1]  = -c.y;  // Re: -ci wi
-        aim[2 * m + 1]  = c.x;   // Im: cr wi
-    }
-    double m_[4], ex[4];
+    for(int m = 0; m < NG; ++m)
+        a_ops(S[0].c[j][4 * m + g], are[2 * m], aim[2 * m], are[2 * m + 1], aim[2 * m + 1]);
+    double m_[4 * NSG], ex[4 * NSG];
 #pragma unroll
-    for(int r = 0; r < 4; ++r) {
+    for(int r = 0; r < 4 * NSG; ++r) {
         m_[r] = 1.0;
         ex[r] = 0.0;
     }
     // The wave's tiles in order: tile q = 4 p + t covers rows first + p stride + 64 wave +
     // 16 t + (0..15) (pass p of the strided loop, quarter t of the wave's 64 rows).
-    // Tile q + 2's waves are requested before tile q's products start, so two tiles of
-    // loads stay in flight behind the matrix cores; a tile past the end reads row end - 1
-    // (one line per resonance) and is never used.
+    // Tile q + 2's waves (q + 1's with two set groups) are requested before tile q's
+    // products start, so loads stay in flight behind the matrix cores; a tile past the end
+    // reads row end - 1 (one line per resonance) and is never used.
     const double2 *__restrict__ wv = dz.wave;
     const int64_t ntile            = first < end ? 4 * ((end - 1 - first) / stride + 1) : 0;
     auto tile_event = [&](int64_t q) { return first + (q >> 2) * stride + 64 * wave + 16 * (q & 3) + j; };
@@ -1203,36 +1210,46 @@ __device__ __forceinline__ void coh_mx_rows(const MultiSets &S, const GfDalitzSl
         }
     };
     auto run_tile = [&](int64_t q, const double2(&w)[NG]) {
-        gf_double4 re = {0, 0, 0, 0}, im = {0, 0, 0, 0};
-#pragma unroll
-        for(int m = 0; m < NG; ++m) {
-            const bool pad   = m == NG - 1 && 4 * m + g >= nres;
-            const double2 wk = pad ? make_double2(0.0, 0.0) : w[m];
-            re               = __builtin_amdgcn_mfma_f64_16x16x4f64(are[2 * m], wk.x, re, 0, 0, 0);
-            im               = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[2 * m], wk.x, im, 0, 0, 0);
-            re               = __builtin_amdgcn_mfma_f64_16x16x4f64(are[2 * m + 1], wk.y, re, 0, 0, 0);
-            im               = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[2 * m + 1], wk.y, im, 0, 0, 0);
-        }
-        // The sets' efficiencies and normalisations are read from LDS per tile (held in
-        // registers they would cost the occupancy of three waves per SIMD).  A lane past the
-        // last row or set multiplies its product by exactly 1 (mantissa 1/2, exponent +1,
-        // renormalised back: the pair is unchanged), which keeps the loop free of branches.
-        asm volatile("" ::: "memory");
         const bool live = tile_event(q) < end;
-        double ef[4], nm[4];
 #pragma unroll
-        for(int r = 0; r < 4; ++r) {
-            ef[r] = S.eff[4 * r + g];
-            nm[r] = S.norm[4 * r + g];
-        }
+        for(int sg = 0; sg < NSG; ++sg) {
+            gf_double4 re = {0, 0, 0, 0}, im = {0, 0, 0, 0};
 #pragma unroll
-        for(int r = 0; r < 4; ++r) {
-            const bool use = live && 4 * r + g < npts;
-            const double v = (re[r] * re[r] + im[r] * im[r]) * ef[r];
-            gfd::nll_add(m_[r], ex[r], use ? v : 1.0, use ? nm[r] : 1.0);
+            for(int m = 0; m < NG; ++m) {
+                const bool pad   = m == NG - 1 && 4 * m + g >= nres;
+                const double2 wk = pad ? make_double2(0.0, 0.0) : w[m];
+                double ar0, ai0, ar1, ai1;
+                if(sg == 0) {
+                    ar0 = are[2 * m], ai0 = aim[2 * m], ar1 = are[2 * m + 1], ai1 = aim[2 * m + 1];
+                } else {
+                    a_ops(S[sg].c[j][4 * m + g], ar0, ai0, ar1, ai1);
+                }
+                re = __builtin_amdgcn_mfma_f64_16x16x4f64(ar0, wk.x, re, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f64_16x16x4f64(ai0, wk.x, im, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f64_16x16x4f64(ar1, wk.y, re, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f64_16x16x4f64(ai1, wk.y, im, 0, 0, 0);
+            }
+            // The sets' efficiencies and normalisations are read from LDS per tile (held in
+            // registers they would cost the occupancy of three waves per SIMD).  A lane past
+            // the last row or set multiplies its product by exactly 1 (mantissa 1/2, exponent
+            // +1, renormalised back: the pair is unchanged), which keeps the loop free of
+            // branches.
+            asm volatile("" ::: "memory");
+            double ef[4], nm[4];
+#pragma unroll
+            for(int r = 0; r < 4; ++r) {
+                ef[r] = S[sg].eff[4 * r + g];
+                nm[r] = S[sg].norm[4 * r + g];
+            }
+#pragma unroll
+            for(int r = 0; r < 4; ++r) {
+                const bool use = live && GF_MULTI_PTS * sg + 4 * r + g < npts;
+                const double v = (re[r] * re[r] + im[r] * im[r]) * ef[r];
+                gfd::nll_add(m_[4 * sg + r], ex[4 * sg + r], use ? v : 1.0, use ? nm[r] : 1.0);
+            }
         }
     };
-    if(ntile > 0) {
+    if(ntile > 0 && NSG == 1) {
         double2 wa[NG], wb[NG], wc[NG];
         load_tile(0, wa);
         load_tile(1, wb);
@@ -1250,15 +1267,31 @@ __device__ __forceinline__ void coh_mx_rows(const MultiSets &S, const GfDalitzSl
             if(q + 3 >= ntile)
                 break;
         }
+    } else if(ntile > 0) {
+        // two groups: a tile's products take twice as long, so one tile of loads ahead
+        // covers the same time (and the registers of a third buffer are not there)
+        double2 wa[NG], wb[NG];
+        load_tile(0, wa);
+        for(int64_t q = 0;; q += 2) {
+            load_tile(q + 1, wb);
+            run_tile(q, wa);
+            if(q + 1 >= ntile)
+                break;
+            load_tile(q + 2, wa);
+            run_tile(q + 1, wb);
+            if(q + 2 >= ntile)
+                break;
+        }
     }
     // set p: the 16 lanes of its group (xor 8, 4, 2, 1), then the four waves in order
 #pragma unroll
-    for(int r = 0; r < 4; ++r) {
+    for(int r = 0; r < 4 * NSG; ++r) {
         double sv = gfd::nll_total(m_[r], ex[r]), cv = 0.0;
         group_reduce(sv, cv);
         if(j == 0) {
-            wls[wave][4 * r + g] = sv;
-            wlc[wave][4 * r + g] = cv;
+            const int p    = GF_MULTI_PTS * (r >> 2) + 4 * (r & 3) + g;
+            wls[wave][p]   = sv;
+            wlc[wave][p]   = cv;
         }
     }
     __syncthreads();
@@ -1273,19 +1306,24 @@ __device__ __forceinline__ void coh_mx_rows(const MultiSets &S, const GfDalitzSl
     __syncthreads();
 }
 
-template <int NR>
+// Up to NSG x 16 sets per sweep (partials [p][b]); NSG = 2 shares each wave load
+// between two groups of 16 sets.
+template <int NR, int NSG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_coh_mx(
     GfProgArgs A, const double *__restrict__ params, int np, const double *__restrict__ norms, int nn, int npts,
     Partial *partials) {
-    __shared__ MultiSets S;
-    __shared__ double ls[GF_MULTI_PTS], lc[GF_MULTI_PTS];
+    __shared__ MultiSets S[NSG];
+    __shared__ double ls[GF_MX_SETS], lc[GF_MX_SETS];
     const GfIns dzi       = A.ins[0];
     const GfIns effi      = A.ins[1];
     const GfDalitzSlot dz = gfd::as_const(A.dz)[dzi.x];
     const int nres        = dzi.a;
-    stage_sets<true>(S, params, np, norms, nn, npts, dzi.p, effi.p + effi.a, nres);
-    coh_mx_rows<NR>(S, dz, nres, (int64_t)blockIdx.x * blockDim.x, (int64_t)gridDim.x * blockDim.x, A.nevents,
-                    npts, ls, lc);
+#pragma unroll
+    for(int sg = 0; sg < NSG; ++sg)
+        stage_sets<true>(S[sg], params + (int64_t)GF_MULTI_PTS * sg * np, np, norms + (int64_t)GF_MULTI_PTS * sg * nn,
+                         nn, min(max(npts - GF_MULTI_PTS * sg, 0), GF_MULTI_PTS), dzi.p, effi.p + effi.a, nres);
+    coh_mx_rows<NR, NSG>(S, dz, nres, (int64_t)blockIdx.x * blockDim.x, (int64_t)gridDim.x * blockDim.x, A.nevents,
+                         npts, ls, lc);
     if((int)threadIdx.x < npts) {
         const int p                                    = threadIdx.x;
         partials[(int64_t)p * gridDim.x + blockIdx.x].s = ls[p];
@@ -1641,7 +1679,7 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_coh_mx(GfProgArgs A, const G
     const int cnt       = min(npts - p0, GF_MULTI_PTS);
     if(cnt < mx_min)
         return;  // uniform over the workgroup
-    __shared__ MultiSets S;
+    __shared__ MultiSets S[1];
     __shared__ double ls[GF_MULTI_PTS], lc[GF_MULTI_PTS];
     const GfIns dzi                = A.ins[0];
     const GfIns effi               = A.ins[1];
@@ -1649,8 +1687,8 @@ __global__ __launch_bounds__(GF_BLOCK) void k_batch_coh_mx(GfProgArgs A, const G
     const int nres                 = dzi.a;
     const int64_t W                = np + nn;
     const double *__restrict__ S0  = stage + (int64_t)(tbl[2 * w.toy] + p0) * W;
-    stage_sets<true>(S, S0, W, S0 + np, W, cnt, dzi.p, effi.p + effi.a, nres);
-    coh_mx_rows<NR>(S, dz, nres, w.start, GF_BLOCK, w.end, cnt, ls, lc);
+    stage_sets<true>(S[0], S0, W, S0 + np, W, cnt, dzi.p, effi.p + effi.a, nres);
+    coh_mx_rows<NR, 1>(S, dz, nres, w.start, GF_BLOCK, w.end, cnt, ls, lc);
     if((int)threadIdx.x < cnt) {
         const int p                                             = threadIdx.x;
         partials[(int64_t)blockIdx.x * GF_BATCH_PTS + p0 + p].s = ls[p];
@@ -2161,7 +2199,7 @@ static void launch_fast_t(const GfProgArgs &a, const GfInline *inl, int metric, 
         }                                                                                                            \
     } while(0)
 
-// A matrix-core kernel (k_coh_mx, k_batch_coh_mx) for nres resonances (NR = nres rounded up to 4).
+// k_batch_coh_mx for nres resonances (NR = nres rounded up to 4).
 #define GF_COH_MX(KERNEL, NRES, ...)                                                                                 \
     do {                                                                                                             \
         switch((NRES) <= 4 ? 4 : (NRES) <= 8 ? 8 : (NRES) <= 12 ? 12 : 16) {                                         \
@@ -2172,14 +2210,35 @@ static void launch_fast_t(const GfProgArgs &a, const GfInline *inl, int metric, 
         }                                                                                                            \
     } while(0)
 
+// k_coh_mx<NR, NSG> for nres resonances (NR = nres rounded up to 4).
+template <int NSG>
+static void launch_coh_mx(int nres, int nblocks, hipStream_t s, const GfProgArgs &a, const double *params, int np,
+                          const double *norms, int nn, int npts, Partial *partials) {
+    const dim3 grid(nblocks), block(256);
+    switch(nres <= 4 ? 4 : nres <= 8 ? 8 : nres <= 12 ? 12 : 16) {
+    case 4: hipLaunchKernelGGL((k_coh_mx<4, NSG>), grid, block, 0, s, a, params, np, norms, nn, npts, partials); break;
+    case 8: hipLaunchKernelGGL((k_coh_mx<8, NSG>), grid, block, 0, s, a, params, np, norms, nn, npts, partials); break;
+    case 12:
+        hipLaunchKernelGGL((k_coh_mx<12, NSG>), grid, block, 0, s, a, params, np, norms, nn, npts, partials);
+        break;
+    default:
+        hipLaunchKernelGGL((k_coh_mx<16, NSG>), grid, block, 0, s, a, params, np, norms, nn, npts, partials);
+        break;
+    }
+}
+
 int launch_dalitz_multi(const GfProgArgs &a, int nres, const double *params, int np, const double *norms, int nn,
                         int npts, bool const_eff, bool zero_outside, int metric, Partial *partials, double *out,
                         int nblocks, void *stream) {
     hipStream_t s = (hipStream_t)stream;
-    if(npts < 1 || npts > GF_MULTI_PTS || nres < 1 || nres > GF_MAX_RES)
+    const bool mx = a.coh_mx_nres > 0 && const_eff && zero_outside && metric == gfd::metric_nll;
+    if(npts < 1 || npts > (mx ? GF_MX_SETS : GF_MULTI_PTS) || nres < 1 || nres > GF_MAX_RES)
         return -1;
-    if(a.coh_mx_nres > 0 && const_eff && zero_outside && metric == gfd::metric_nll) {
-        GF_COH_MX(k_coh_mx, nres, dim3(nblocks), dim3(256), 0, s, a, params, np, norms, nn, npts, partials);
+    if(mx) {
+        if(npts > GF_MULTI_PTS)
+            launch_coh_mx<2>(nres, nblocks, s, a, params, np, norms, nn, npts, partials);
+        else
+            launch_coh_mx<1>(nres, nblocks, s, a, params, np, norms, nn, npts, partials);
         int rc = check(hipGetLastError());
         if(rc)
             return rc;

@@ -473,7 +473,9 @@ def test_multi_point_sums_on_the_matrix_cores_equal_single_sweeps(ctx, nres, n):
     elements bit for bit.  So with the matrix-core path on, and with it off (vector
     kernels, their own grouping), every multi-point sum equals the single reduction of the
     same parameters exactly; the two paths agree with each other to rounding (1e-13) and
-    with the oracle (1e-11).  16 and 5 perturbed coupling sets."""
+    with the oracle (1e-11).  5, 16, 23, 32 and 45 perturbed coupling sets: the matrix-core
+    sweep takes up to 32 sets (two groups of 16 sharing each wave load), so these cover one
+    group, two full and partial groups, and a request spread over two sweeps."""
     table = W.DALITZ_RESONANCES[:nres]
     top, _ = dalitz(nbins=64, resonances=table)
     aos = dalitz_events(n, 40 + nres)
@@ -491,7 +493,7 @@ def test_multi_point_sums_on_the_matrix_cores_equal_single_sweeps(ctx, nres, n):
     rng = np.random.default_rng(nres)
     c0 = top.parameters_idx + 1
     try:
-        for npts in (16, 5):
+        for npts in (16, 5, 32, 23, 45):
             sets = []
             for _ in range(npts):
                 p = p0.copy()
@@ -501,9 +503,9 @@ def test_multi_point_sums_on_the_matrix_cores_equal_single_sweeps(ctx, nres, n):
             for mx in (1, 0):
                 ctx.set_tuning(capi.TUNE_COH_MFMA, mx)
                 got[mx] = ctx.reduce_metric_multi("NLL", sets, [n0] * npts)
-                for k in (0, npts // 2, npts - 1):
+                for k in sorted({0, npts // 2, min(17, npts - 1), npts - 1}):
                     ctx.update_params(sets[k])
-                    assert ctx.reduce_metric("NLL")[0] == got[mx][k], (mx, k)
+                    assert ctx.reduce_metric("NLL")[0] == got[mx][k], (mx, npts, k)
                 ctx.update_params(p0)
             np.testing.assert_allclose(got[1], got[0], rtol=1e-13, atol=0)
         for mx in (1, 0):
