@@ -1025,7 +1025,9 @@ __global__ __launch_bounds__(256) void k_dalitz_fast(GfProgArgs A, GfInline I, i
 // loop then reads them with broadcast LDS loads instead of one scalar load and
 // wait per coupling.
 struct MultiSets {
-    double2 c[GF_MULTI_PTS][GF_MAX_RES];
+    // one row per set, padded by one entry: 16 lanes reading resonance i of sets 0..15
+    // (the matrix-core kernels' A operands) then hit 16 different bank groups
+    double2 c[GF_MULTI_PTS][GF_MAX_RES + 1];
     double eff[GF_MULTI_PTS], norm[GF_MULTI_PTS];
 };
 static_assert(GF_MULTI_PTS * GF_MAX_RES == 256, "one staging thread per (set, resonance)");
