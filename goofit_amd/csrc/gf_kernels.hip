@@ -1185,11 +1185,12 @@ __device__ __forceinline__ void coh_mx_rows(const MultiSets *S, const GfDalitzSl
 #pragma unroll
     for(int m = 0; m < NG; ++m)
         a_ops(S[0].c[j][4 * m + g], are[2 * m], aim[2 * m], are[2 * m + 1], aim[2 * m + 1]);
-    double m_[4 * NSG], ex[4 * NSG];
+    double m_[4 * NSG];  // per set: the NLL product's mantissa and (integer) exponent
+    int ex[4 * NSG];
 #pragma unroll
     for(int r = 0; r < 4 * NSG; ++r) {
         m_[r] = 1.0;
-        ex[r] = 0.0;
+        ex[r] = 0;
     }
     // The wave's tiles in order: tile q = 4 p + t covers rows first + p stride + 64 wave +
     // 16 t + (0..15) (pass p of the strided loop, quarter t of the wave's 64 rows).
@@ -1212,7 +1213,8 @@ __device__ __forceinline__ void coh_mx_rows(const MultiSets *S, const GfDalitzSl
         }
     };
     auto run_tile = [&](int64_t q, const double2(&w)[NG]) {
-        const bool live = tile_event(q) < end;
+        const bool live     = tile_event(q) < end;
+        const bool all_live = tile_event(q) - j + 15 < end;  // uniform over the wave
 #pragma unroll
         for(int sg = 0; sg < NSG; ++sg) {
             gf_double4 re = {0, 0, 0, 0}, im = {0, 0, 0, 0};
@@ -1232,10 +1234,11 @@ __device__ __forceinline__ void coh_mx_rows(const MultiSets *S, const GfDalitzSl
                 im = __builtin_amdgcn_mfma_f64_16x16x4f64(ai1, wk.y, im, 0, 0, 0);
             }
             // The sets' efficiencies and normalisations are read from LDS per tile (held in
-            // registers they would cost the occupancy of three waves per SIMD).  A lane past
-            // the last row or set multiplies its product by exactly 1 (mantissa 1/2, exponent
-            // +1, renormalised back: the pair is unchanged), which keeps the loop free of
-            // branches.
+            // registers they would cost the occupancy of three waves per SIMD).  A set past
+            // npts has efficiency and normalisation 0 (stage_sets), so its factor is 0, which
+            // the product takes as exactly 1 (mantissa 1/2, exponent +1, renormalised back:
+            // the pair is unchanged); a lane past the last row is given the factor 1 outright,
+            // in the one tile per wave that reaches past it.
             asm volatile("" ::: "memory");
             double ef[4], nm[4];
 #pragma unroll
@@ -1243,11 +1246,16 @@ __device__ __forceinline__ void coh_mx_rows(const MultiSets *S, const GfDalitzSl
                 ef[r] = S[sg].eff[4 * r + g];
                 nm[r] = S[sg].norm[4 * r + g];
             }
+            if(all_live) {
 #pragma unroll
-            for(int r = 0; r < 4; ++r) {
-                const bool use = live && GF_MULTI_PTS * sg + 4 * r + g < npts;
-                const double v = (re[r] * re[r] + im[r] * im[r]) * ef[r];
-                gfd::nll_add(m_[4 * sg + r], ex[4 * sg + r], use ? v : 1.0, use ? nm[r] : 1.0);
+                for(int r = 0; r < 4; ++r)
+                    gfmath::nll_add(m_[4 * sg + r], ex[4 * sg + r], ((re[r] * re[r] + im[r] * im[r]) * ef[r]) * nm[r]);
+            } else {
+#pragma unroll
+                for(int r = 0; r < 4; ++r) {
+                    const double v = ((re[r] * re[r] + im[r] * im[r]) * ef[r]) * nm[r];
+                    gfmath::nll_add(m_[4 * sg + r], ex[4 * sg + r], live ? v : 1.0);
+                }
             }
         }
     };
@@ -1288,7 +1296,7 @@ __device__ __forceinline__ void coh_mx_rows(const MultiSets *S, const GfDalitzSl
     // set p: the 16 lanes of its group (xor 8, 4, 2, 1), then the four waves in order
 #pragma unroll
     for(int r = 0; r < 4 * NSG; ++r) {
-        double sv = gfd::nll_total(m_[r], ex[r]), cv = 0.0;
+        double sv = gfd::nll_total(m_[r], (double)ex[r]), cv = 0.0;
         group_reduce(sv, cv);
         if(j == 0) {
             const int p    = GF_MULTI_PTS * (r >> 2) + 4 * (r & 3) + g;

@@ -89,6 +89,16 @@ GF_MATH_HD inline void nll_add(double &m, double &e, double v) {
     m              = frexp(m * a, &em);
     e += (double)(ev + em);
 }
+// The same product with the exponent in an integer register (no conversion per factor):
+// m is the double form's m bit for bit and e its exponent's value, for any sequence of
+// factors whose exponent sum stays within int range (+-2^31: over 2^20 factors at the
+// extreme exponents).
+GF_MATH_HD inline void nll_add(double &m, int &e, double v) {
+    int ev, em;
+    const double a = frexp(v > 0.0 ? v : 1.0, &ev);
+    m              = frexp(m * a, &em);
+    e += ev + em;
+}
 // sum of -log(v_k) = -(log m + e ln 2)
 GF_MATH_HD inline double nll_total(double m, double e) { return -fma(e, ln2_hi, fma(e, ln2_lo, log_d(m))); }
 

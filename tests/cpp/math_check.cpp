@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <random>
 
 #include "gf_math.h"
@@ -119,6 +120,23 @@ int main(int argc, char **argv) {
         gfmath::nll_add(m2, e2, 0.125);
         gfmath::nll_add(m2, e2, 8.0);
         std::printf("nll_special exact_pair=%.17g\n", gfmath::nll_total(m2, e2));
+    }
+    {
+        // the integer-exponent form against the double form, factor by factor
+        std::mt19937_64 rng(77);
+        std::uniform_real_distribution<double> u(0.0, 1.0);
+        double m = 1.0, e = 0.0, mi = 1.0;
+        int ei = 0;
+        long long bad = 0, n = 0;
+        for(; n < 1000000; n++) {
+            const double r = u(rng);
+            const double v = r < 0.02 ? 0.0 : r < 0.03 ? NAN : std::ldexp(u(rng), 1023 - (int)(u(rng) * 2098.0));
+            gfmath::nll_add(m, e, v);
+            gfmath::nll_add(mi, ei, v);
+            if(std::memcmp(&m, &mi, sizeof m) != 0 || e != (double)ei)
+                bad++;
+        }
+        std::printf("nll_int_exponent n=%lld mismatches=%lld\n", n, bad);
     }
     return 0;
 }

@@ -74,3 +74,11 @@ def test_nll_as_one_logarithm(report):
     assert float(m.group(1)) == 0.0 and float(m.group(2)) == -math.inf
     m = re.search(r"nll_special exact_pair=(\S+)", report)
     assert float(m.group(1)) == 0.0  # 0.125 * 8 = 1 exactly
+
+
+def test_nll_integer_exponent_form(report):
+    """The overload that keeps the exponent in an int (the matrix-core kernels') tracks
+    the double form exactly: the same mantissa bits and exponent value after each of 1e6
+    factors across every binary exponent, zeros and NaNs included."""
+    m = re.search(r"nll_int_exponent n=(\d+) mismatches=(\d+)", report)
+    assert m and int(m.group(1)) == 1_000_000 and int(m.group(2)) == 0
