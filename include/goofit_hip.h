@@ -239,9 +239,10 @@ gf_status gf_reduce_metric_noalloc(gf_ctx *ctx, int metric, gf_result *dev_out, 
  * npts consecutive copies of the flattened parameter array (np values each, np
  * as in gf_params_update), norms npts copies of the normalisation array;
  * out[k] = the sum gf_reduce_metric returns with set k installed.  For a lone
- * Amp3Body (the coherent-sum path) one sweep serves 16 sets, reading each
- * event's waves once, with each set's result bit-identical to its own
- * gf_reduce_metric; otherwise the sets are reduced one after another.  The
+ * Amp3Body (the coherent-sum path) one sweep serves 16 sets (32 when the NLL's
+ * coherent sums run on the matrix cores, GF_TUNE_COH_MFMA), reading each event's
+ * waves once, with each set's result bit-identical to its own gf_reduce_metric;
+ * otherwise the sets are reduced one after another.  The
  * context's own arrays are left as they were.  No reference counterpart: it
  * serves the independent points of a numerical derivative
  * (Minuit2::FCNBase::EvaluateMany of include/Minuit2/Minuit2.h) in one request. */
