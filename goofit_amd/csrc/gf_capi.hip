@@ -161,6 +161,7 @@ struct gf_ctx {
     uint64_t events_gen = 1;  // bumped whenever the event columns may change
 
     // gf_reduce_metric_multi: parameter/norm sets, partials and results
+    // d_multi_out: the device address of the host-mapped h_multi_out (the multi-point results)
     double *d_multi = nullptr, *h_multi = nullptr, *d_multi_out = nullptr, *h_multi_out = nullptr;
     size_t cap_multi = 0;
     gfk::Partial *d_multi_partials = nullptr;
@@ -1315,8 +1316,6 @@ void gf_ctx_destroy(gf_ctx *c) {
             (void)hipFree(kv.second.buf);
     if(c->d_multi)
         (void)hipFree(c->d_multi);
-    if(c->d_multi_out)
-        (void)hipFree(c->d_multi_out);
     if(c->d_multi_partials)
         (void)hipFree(c->d_multi_partials);
     if(c->h_multi)
@@ -1661,8 +1660,6 @@ gf_status gf_reduce_metric_multi(gf_ctx *ctx, int metric, int npts, const double
             GF_HIP(hipFree(ctx->d_multi));
         if(ctx->h_multi)
             GF_HIP(hipHostFree(ctx->h_multi));
-        if(ctx->d_multi_out)
-            GF_HIP(hipFree(ctx->d_multi_out));
         if(ctx->h_multi_out)
             GF_HIP(hipHostFree(ctx->h_multi_out));
         ctx->d_multi = ctx->h_multi = ctx->d_multi_out = ctx->h_multi_out = nullptr;
@@ -1671,8 +1668,9 @@ gf_status gf_reduce_metric_multi(gf_ctx *ctx, int metric, int npts, const double
         const size_t o = 2 * (c / 2 + 64);  // result pairs: at least 2 per set
         if(hipMalloc((void **)&ctx->d_multi, c * sizeof(double)) != hipSuccess
            || hipHostMalloc((void **)&ctx->h_multi, c * sizeof(double), hipHostMallocDefault) != hipSuccess
-           || hipMalloc((void **)&ctx->d_multi_out, o * sizeof(double)) != hipSuccess
-           || hipHostMalloc((void **)&ctx->h_multi_out, o * sizeof(double), hipHostMallocDefault) != hipSuccess)
+           || hipHostMalloc((void **)&ctx->h_multi_out, o * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent)
+                  != hipSuccess
+           || hipHostGetDevicePointer((void **)&ctx->d_multi_out, ctx->h_multi_out, 0) != hipSuccess)
             return fail(GF_ERR_NOMEM, "cannot allocate %zu parameter values for %d sets", c, npts);
         ctx->cap_multi = c;
     }
@@ -1699,6 +1697,12 @@ gf_status gf_reduce_metric_multi(gf_ctx *ctx, int metric, int npts, const double
     const int nb  = fast_blocks(ctx, ctx->n);
     a.coh_mx_nres = coh_mx_nres(ctx, metric);
     const int chunk = (a.coh_mx_nres > 0 && ce && zo) ? GF_MX_SETS : GF_MULTI_PTS;
+    // the result slots' empty marker (no launch of this context that writes them is in
+    // flight: every earlier request was read back before it returned)
+    for(int k = 0; k < npts; k++) {
+        const unsigned long long empty = GF_SLOT_EMPTY;
+        memcpy(ctx->h_multi_out + 2 * (size_t)k, &empty, sizeof empty);
+    }
     for(int k0 = 0; k0 < npts; k0 += chunk) {
         const int cnt = std::min(chunk, npts - k0);
         int rc = gfk::launch_dalitz_multi(a, ctx->dalitz[0].nres, ctx->d_multi + (size_t)k0 * np, np,
@@ -1707,9 +1711,36 @@ gf_status gf_reduce_metric_multi(gf_ctx *ctx, int metric, int npts, const double
         if(rc)
             return fail(GF_ERR_HIP, "multi-set launch failed (%d)", rc);
     }
-    GF_HIP(hipMemcpyAsync(ctx->h_multi_out, ctx->d_multi_out, 2 * (size_t)npts * sizeof(double),
-                          hipMemcpyDeviceToHost, ctx->stream));
-    GF_HIP(hipStreamSynchronize(ctx->stream));
+    // k_finish_multi stores each sum over the empty marker left below, after its residual
+    // and a system-scope fence; the host polls the sums as read_result polls its block
+    // (up to GF_TUNE_SPIN_US), then blocks on the stream, which also reports a failed kernel.
+    volatile unsigned long long *res = (volatile unsigned long long *)ctx->h_multi_out;
+    auto ready = [&]() {
+        for(int k = 0; k < npts; k++)
+            if(__atomic_load_n((const unsigned long long *)&res[2 * k], __ATOMIC_ACQUIRE) == GF_SLOT_EMPTY)
+                return false;
+        return true;
+    };
+    bool seen = false;
+    if(ctx->tune[GF_TUNE_SPIN_US] > 0) {
+        const auto t0    = std::chrono::steady_clock::now();
+        const auto limit = std::chrono::microseconds(ctx->tune[GF_TUNE_SPIN_US]);
+        for(unsigned it = 0;; ++it) {
+            if(ready()) {
+                seen = true;
+                break;
+            }
+            if((it & 255u) == 255u && std::chrono::steady_clock::now() - t0 > limit)
+                break;
+            __builtin_ia32_pause();
+        }
+    }
+    if(!seen) {
+        GF_HIP(hipStreamSynchronize(ctx->stream));
+        if(!ready())
+            return fail(GF_ERR_HIP, "multi-set reduction finished without writing its results");
+    }
+    ctx->stage_in_flight = false;  // every command before the last k_finish_multi is complete
     for(int k = 0; k < npts; k++)
         out[k] = ctx->h_multi_out[2 * k];
     return GF_OK;

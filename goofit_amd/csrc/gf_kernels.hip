@@ -1342,7 +1342,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 }
 
 // k_finish for several sums: workgroup p adds the nparts partials of set p in the
-// order k_finish uses and writes {sum, residual} to out[2p].
+// order k_finish uses and writes {sum, residual} to out[2p] (host-mapped memory that
+// the host polls): the residual, a system-scope fence, then the sum over the empty
+// marker the host left there.
 __global__ __launch_bounds__(GF_BLOCK) void k_finish_multi(const Partial *partials, int nparts, double *out) {
     const Partial *q = partials + (int64_t)blockIdx.x * nparts;
     double ts = 0.0, tc = 0.0;
@@ -1350,8 +1352,11 @@ __global__ __launch_bounds__(GF_BLOCK) void k_finish_multi(const Partial *partia
         gfd::kb_merge(ts, tc, q[i].s, q[i].c);
     block_reduce(ts, tc);
     if(threadIdx.x == 0) {
-        out[2 * blockIdx.x]     = gfd::kb_total(ts, tc);
         out[2 * blockIdx.x + 1] = tc;
+        __threadfence_system();
+        __hip_atomic_store((unsigned long long *)(out + 2 * blockIdx.x),
+                           (unsigned long long)__double_as_longlong(gfd::kb_total(ts, tc)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
