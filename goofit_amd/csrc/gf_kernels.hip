@@ -1772,6 +1772,33 @@ __global__ __launch_bounds__(GF_BLOCK) void k_finish_toy_points(const Partial *_
                                                                 double *__restrict__ result) {
     const int t    = blockIdx.x;
     const int npts = tbl[2 * t + 1];
+    const int f0 = first[t], f1 = first[t + 1];
+    if(f1 - f0 <= 64) {  // uniform over the workgroup
+        // Every partial of the toy lies in wave 0 of block_reduce's layout (thread i holds
+        // entry i), and waves 1-3 hold exact (+0, +0) pairs.  So one wave per point gives
+        // block_reduce's bits -- the same butterfly, then three zero pairs merged in
+        // order -- and the four waves take the points in turn, without a barrier.
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for(int p = wave; p < npts; p += GF_BLOCK / 64) {
+            double s = 0.0, c = 0.0;
+            if(f0 + lane < f1)
+                gfd::kb_merge(s, c, partials[(int64_t)(f0 + lane) * GF_BATCH_PTS + p].s,
+                              partials[(int64_t)(f0 + lane) * GF_BATCH_PTS + p].c);
+#pragma unroll
+            for(int off = 32; off > 0; off >>= 1) {
+                double s2 = __shfl_xor(s, off, 64);
+                double c2 = __shfl_xor(c, off, 64);
+                gfd::kb_merge(s, c, s2, c2);
+            }
+            if(lane == 0) {
+                for(int w = 1; w < GF_BLOCK / 64; ++w)
+                    gfd::kb_merge(s, c, 0.0, 0.0);
+                result[2 * ((int64_t)t * GF_BATCH_PTS + p)]     = gfd::kb_total(s, c);
+                result[2 * ((int64_t)t * GF_BATCH_PTS + p) + 1] = c;
+            }
+        }
+        return;
+    }
     for(int p = 0; p < npts; ++p) {
         double s = 0.0, c = 0.0;
         for(int i = first[t] + threadIdx.x; i < first[t + 1]; i += GF_BLOCK)
